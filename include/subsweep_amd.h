@@ -1,0 +1,164 @@
+/*
+ * subsweep_amd — C ABI of the MI355X sweep radiative-transfer + hydrogen
+ * chemistry library (libsubsweep_amd.so).
+ *
+ * This is the boundary a Rust `SweepPlugin` replacement would bind through
+ * `extern "C"`: the reference's plugin registers `init_sweep_system` (startup
+ * stage InitSweep) and `run_sweep_system` (stage Sweep), src/sweep/mod.rs:122-169,
+ * and keeps all hot-path state inside one non-Send `Sweep<HydrogenOnly>`
+ * resource, src/sweep/mod.rs:139,171-191. Each entry point below replaces one
+ * piece of that resource's life cycle; the reference location is cited on each.
+ *
+ * Conventions
+ *  - SI units throughout, like the reference's unit system (src/units/mod.rs:9-110).
+ *  - Cell arrays have N entries, dense index = ParticleId.index
+ *    (src/sweep/active_list.rs:16-46 requires ids 0..N-1).
+ *  - Per-(cell, direction) arrays are direction-major: element [d * N + c].
+ *  - Caller-owned host buffers are copied into handle-owned device buffers.
+ *  - Every call returns an int status: SSW_OK (0) or a negative SSW_ERR_*;
+ *    ssw_last_error() gives the message of the last failure on this thread.
+ *  - One handle is used from one thread at a time. Distinct handles run on
+ *    distinct HIP streams and may be driven concurrently.
+ */
+#ifndef SUBSWEEP_AMD_H
+#define SUBSWEEP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSW_OK 0
+#define SSW_ERR_INVALID (-1)    /* bad argument or call order                  */
+#define SSW_ERR_OOM (-2)        /* device allocation failed                    */
+#define SSW_ERR_BUDGET (-3)     /* request exceeds the configured HBM budget   */
+#define SSW_ERR_CYCLE (-4)      /* dependency cycle in a direction's face graph */
+#define SSW_ERR_HIP (-5)        /* HIP runtime error                           */
+#define SSW_ERR_GRID (-6)       /* face lists not mirror-symmetric             */
+#define SSW_ERR_NO_DEVICE (-7)  /* no HIP device visible                       */
+
+/* Face connection kinds, the single-rank subset of ParticleType
+ * (src/sweep/grid/cell.rs:16-23): Local(id), LocalPeriodic(id), Boundary. */
+#define SSW_FACE_LOCAL 0
+#define SSW_FACE_PERIODIC 1
+#define SSW_FACE_BOUNDARY 2
+
+/* Transport precision of the sweep (the chemistry is always FP64). */
+#define SSW_TRANSPORT_FP64 64
+#define SSW_TRANSPORT_FP32 32
+
+/* SweepParameters (src/sweep/parameters.rs:8-46) plus the cosmology scale
+ * factor the reference hands to HydrogenOnly (src/sweep/mod.rs:685-690). */
+typedef struct ssw_sweep_parameters {
+    /* DirectionsSpecification::Num: 1, 16, 21, 32, 64 or 84 (HEALPix bins,
+     * src/sweep/direction/mod.rs:58-75) ...                                  */
+    int32_t num_directions;
+    /* ... or DirectionsSpecification::Explicit: num_directions x 3 doubles,
+     * normalised by the library (src/sweep/direction/mod.rs:97-109). NULL for Num. */
+    const double* explicit_directions;
+    int32_t num_timestep_levels;           /* >= 1, < 32                         */
+    int32_t periodic;                      /* 0: periodic faces act as boundary
+                                              (src/voronoi/constructor/mod.rs:125-135) */
+    double max_timestep;                   /* s                                   */
+    double significant_rate_threshold;     /* s^-1, default 0                     */
+    double timestep_safety_factor;         /* default 0.1                         */
+    double chemistry_timestep_safety_factor; /* default 0.1                       */
+    int32_t prevent_cooling;               /* default 1                           */
+    int32_t transport_precision;           /* SSW_TRANSPORT_FP64 / _FP32          */
+    double scale_factor;                   /* cosmological a (1 if non-cosmological) */
+} ssw_sweep_parameters;
+
+typedef struct ssw_sweep ssw_sweep;
+
+/* Fills *p with the reference defaults (src/sweep/parameters.rs:64-78):
+ * threshold 0, safety factors 0.1, prevent_cooling true, FP64 transport,
+ * scale factor 1. num_directions/levels/max_timestep are left for the caller. */
+void ssw_default_parameters(ssw_sweep_parameters* p);
+
+/* Creates the sweep state; replaces Sweep::new (src/sweep/mod.rs:194-232) as
+ * called from init_sweep_system (src/sweep/mod.rs:634-692). */
+int ssw_create(const ssw_sweep_parameters* params, ssw_sweep** out);
+
+/* Uploads the grid: the `Cell` components (src/sweep/grid/cell.rs:92-97)
+ * flattened to CSR. face_offsets has N+1 entries; for face f of cell c,
+ * face_neighbour[f] is the dense neighbour index (ignored for boundary faces),
+ * face_kind[f] an SSW_FACE_* value, face_area[f] the area (m^2) and
+ * face_normal[3f..3f+2] the unit normal pointing from c to the neighbour.
+ * Local faces must come in mirror pairs (c->n with normal v, n->c with -v),
+ * as the reference's Voronoi grid produces them (src/voronoi/cell.rs:63-71).
+ * Builds the per-direction dependency graph once and checks it for cycles. */
+int ssw_upload_grid(ssw_sweep* s, int64_t num_cells, const int64_t* face_offsets,
+                    const int32_t* face_neighbour, const uint8_t* face_kind,
+                    const double* face_area, const double* face_normal,
+                    const double* cell_volume, const double* cell_size);
+
+/* Uploads the per-cell state the reference copies from ECS components into
+ * each Site (src/sweep/mod.rs:656-671): density (kg m^-3), ionised hydrogen
+ * fraction, temperature (K) and source rate (photons s^-1, src/source_systems.rs:48-80).
+ * Resets fluxes, timestep levels and the timestep state like Sweep::new. */
+int ssw_upload_state(ssw_sweep* s, const double* density, const double* ionized_fraction,
+                     const double* temperature, const double* source);
+
+/* One call of Sweep::run_sweeps (src/sweep/mod.rs:258-272): every sweep of the
+ * timestep-level cadence, chemistry after each, then the level update. Writes
+ * the elapsed simulation time (s). The reference skips the very first
+ * run_sweep_system call (src/sweep/mod.rs:708-714); that is the caller's job. */
+int ssw_step(ssw_sweep* s, double* time_elapsed);
+
+/* Copies results back, like the tail of run_sweep_system (src/sweep/mod.rs:715-738).
+ * Any pointer may be NULL. photon_rate = sum over directions of the incoming
+ * rate (no source, no periodic term, src/sweep/mod.rs:727-730). */
+int ssw_download(ssw_sweep* s, double* ionized_fraction, double* temperature,
+                 double* species_timestep, double* photon_rate, uint8_t* level,
+                 double* change_timescale);
+
+/* Parity hook: per-(direction, cell) upwind dependency counts of
+ * Sweep::init_counts (src/sweep/mod.rs:346-386) for a sweep at `sweep_level`
+ * under the current timestep levels; inactive cells get 0xFFFFFFFF. */
+int ssw_download_counts(ssw_sweep* s, int32_t sweep_level, uint32_t* counts);
+
+/* Parity hook: Kahn depth of every (direction, cell) in the dependency graph
+ * restricted to the cells active at `sweep_level` (the level-set order in which
+ * the device solves them); inactive cells get 0xFFFFFFFF. */
+int ssw_download_levelsets(ssw_sweep* s, int32_t sweep_level, uint32_t* depth);
+
+/* Parity hook: incoming, outgoing and periodic-source rates per
+ * (direction, cell) as Site holds them (src/sweep/site.rs:12-23). Any may be NULL. */
+int ssw_download_fluxes(ssw_sweep* s, double* incoming, double* outgoing, double* periodic);
+
+typedef struct ssw_info {
+    int64_t num_cells;
+    int64_t num_faces;
+    int32_t num_directions;
+    int32_t max_depth;             /* Kahn depth bound of the full graph          */
+    int64_t updates_last_step;     /* cell-direction updates of the last step     */
+    int64_t sweeps_last_step;
+    double downwind_local;         /* mean downwind faces per cell-dir by kind    */
+    double downwind_periodic;
+    double downwind_boundary;
+    int64_t device_bytes;          /* device memory held by this handle           */
+    int64_t chemistry_failures;    /* cells whose chemistry hit the depth limit   */
+} ssw_info;
+
+int ssw_get_info(ssw_sweep* s, ssw_info* info);
+
+/* Kernel timing of the last `ssw_step` when timing is enabled: average duration
+ * in ms of each launch of the transport kernel, total ms of the transport
+ * kernels, and total ms of the chemistry kernels, from HIP events recorded on
+ * the handle's stream. */
+int ssw_set_timing(ssw_sweep* s, int32_t enabled);
+int ssw_get_timing(ssw_sweep* s, double* transport_avg_ms, double* transport_total_ms,
+                   int64_t* transport_launches, double* chemistry_total_ms);
+
+/* Releases the handle and its device memory. */
+void ssw_destroy(ssw_sweep* s);
+
+/* Message of the last error on the calling thread ("" if none). */
+const char* ssw_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SUBSWEEP_AMD_H */

@@ -1,0 +1,151 @@
+"""ctypes bindings of libsubsweep_amd.so (the C ABI in include/subsweep_amd.h)
+and libssw_grid.so. Loading fails loudly when a library has not been built:
+there is no CPU fallback for the sweep."""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SWEEP_LIB_PATH = os.path.join(HERE, "libsubsweep_amd.so")
+GRID_LIB_PATH = os.path.join(HERE, "libssw_grid.so")
+
+SSW_OK = 0
+SSW_ERR_INVALID = -1
+SSW_ERR_OOM = -2
+SSW_ERR_BUDGET = -3
+SSW_ERR_CYCLE = -4
+SSW_ERR_HIP = -5
+SSW_ERR_GRID = -6
+SSW_ERR_NO_DEVICE = -7
+
+FACE_LOCAL = 0
+FACE_PERIODIC = 1
+FACE_BOUNDARY = 2
+
+TRANSPORT_FP64 = 64
+TRANSPORT_FP32 = 32
+
+
+class SweepParametersC(C.Structure):
+    _fields_ = [
+        ("num_directions", C.c_int32),
+        ("explicit_directions", C.POINTER(C.c_double)),
+        ("num_timestep_levels", C.c_int32),
+        ("periodic", C.c_int32),
+        ("max_timestep", C.c_double),
+        ("significant_rate_threshold", C.c_double),
+        ("timestep_safety_factor", C.c_double),
+        ("chemistry_timestep_safety_factor", C.c_double),
+        ("prevent_cooling", C.c_int32),
+        ("transport_precision", C.c_int32),
+        ("scale_factor", C.c_double),
+    ]
+
+
+class InfoC(C.Structure):
+    _fields_ = [
+        ("num_cells", C.c_int64),
+        ("num_faces", C.c_int64),
+        ("num_directions", C.c_int32),
+        ("max_depth", C.c_int32),
+        ("updates_last_step", C.c_int64),
+        ("sweeps_last_step", C.c_int64),
+        ("downwind_local", C.c_double),
+        ("downwind_periodic", C.c_double),
+        ("downwind_boundary", C.c_double),
+        ("device_bytes", C.c_int64),
+        ("chemistry_failures", C.c_int64),
+    ]
+
+
+class SweepError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"subsweep_amd error {code}: {message}")
+        self.code = code
+
+
+P = C.c_void_p
+PD = C.POINTER(C.c_double)
+PI64 = C.POINTER(C.c_int64)
+PI32 = C.POINTER(C.c_int32)
+PU8 = C.POINTER(C.c_uint8)
+PU32 = C.POINTER(C.c_uint32)
+
+# Every symbol the header declares, with its signature.
+SWEEP_SIGNATURES = {
+    "ssw_default_parameters": (None, [C.POINTER(SweepParametersC)]),
+    "ssw_create": (C.c_int, [C.POINTER(SweepParametersC), C.POINTER(P)]),
+    "ssw_upload_grid": (C.c_int, [P, C.c_int64, PI64, PI32, PU8, PD, PD, PD, PD]),
+    "ssw_upload_state": (C.c_int, [P, PD, PD, PD, PD]),
+    "ssw_step": (C.c_int, [P, PD]),
+    "ssw_download": (C.c_int, [P, PD, PD, PD, PD, PU8, PD]),
+    "ssw_download_counts": (C.c_int, [P, C.c_int32, PU32]),
+    "ssw_download_levelsets": (C.c_int, [P, C.c_int32, PU32]),
+    "ssw_download_fluxes": (C.c_int, [P, PD, PD, PD]),
+    "ssw_get_info": (C.c_int, [P, C.POINTER(InfoC)]),
+    "ssw_set_timing": (C.c_int, [P, C.c_int32]),
+    "ssw_get_timing": (C.c_int, [P, PD, PD, PI64, PD]),
+    "ssw_destroy": (None, [P]),
+    "ssw_last_error": (C.c_char_p, []),
+}
+
+GRID_SIGNATURES = {
+    "ssw_points_jittered_lattice": (None, [C.c_int64, C.c_double, C.c_double, C.c_uint64, PD]),
+    "ssw_points_uniform": (None, [C.c_int64, C.c_double, C.c_uint64, PD]),
+    "ssw_lognormal_density": (None, [C.c_int64, C.c_double, C.c_double, C.c_uint64, PD]),
+    "ssw_assign_sources": (C.c_int, [C.c_int64, PD, C.c_int64, PD, PD, PD, PI64]),
+    "ssw_grid_cartesian": (None, [C.c_int64, C.c_double, C.c_int, PI64, PI32, PU8, PD, PD, PD, PD]),
+    "ssw_voronoi_build": (P, [C.c_int64, PD, PD, C.c_int, PD, C.c_int]),
+    "ssw_voronoi_num_faces": (C.c_int64, [P]),
+    "ssw_voronoi_dropped_faces": (C.c_int64, [P]),
+    "ssw_voronoi_open_cells": (C.c_int64, [P]),
+    "ssw_voronoi_get": (None, [P, PI64, PI32, PU8, PD, PD, PD, PD]),
+    "ssw_voronoi_free": (None, [P]),
+    "ssw_grid_last_error": (C.c_char_p, []),
+}
+
+_sweep_lib = None
+_grid_lib = None
+
+
+def _load(path: str, sigs: dict):
+    if not os.path.exists(path):
+        raise ImportError(
+            f"{os.path.basename(path)} is not built; run `python -m subsweep_amd.build` "
+            f"(or __graft_entry__.build()) first"
+        )
+    lib = C.CDLL(path)
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def sweep_lib():
+    global _sweep_lib
+    if _sweep_lib is None:
+        _sweep_lib = _load(SWEEP_LIB_PATH, SWEEP_SIGNATURES)
+    return _sweep_lib
+
+
+def grid_lib():
+    global _grid_lib
+    if _grid_lib is None:
+        _grid_lib = _load(GRID_LIB_PATH, GRID_SIGNATURES)
+    return _grid_lib
+
+
+def check(code: int) -> None:
+    if code != SSW_OK:
+        msg = sweep_lib().ssw_last_error().decode(errors="replace")
+        raise SweepError(code, msg)
+
+
+def ptr(arr, ctype):
+    """Pointer to a contiguous numpy array (or NULL for None)."""
+    if arr is None:
+        return None
+    return arr.ctypes.data_as(C.POINTER(ctype))

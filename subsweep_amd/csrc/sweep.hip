@@ -1,0 +1,1210 @@
+// libsubsweep_amd: MI355X (gfx950) sweep radiative transfer + hydrogen chemistry.
+//
+// Device layout (one handle = one resident problem):
+//   faces      CSR over cells: off[N+1] (u32), geo[F] = {nx, ny, nz, area} (f64x4),
+//              nk[F] = neighbour | kind << 30 (u32)
+//   cell state rho, x, T, source/D, attenuation, timestep, previous rate,
+//              change timescale (f64[N]); timestep level (u8[N])
+//   fluxes     direction-major [d * N + c]:
+//              X = outgoing rate / downwind effective area (FP64 or scaled FP32)
+//              I = incoming rate at solve time (FP64 or scaled FP32)
+//              Psnap[d * Nper + slot] periodic source snapshot of the Nper cells
+//              that have a periodic face
+//   schedule   per sweep level l (the active set {level >= l}) and direction d:
+//              queue[l][d][0..N) of cells in Kahn order, level-set offsets
+//              lvloff[l][d][0..Kmax+1], three rotating append counters.
+//
+// The transport is the reference's upwind sweep (src/sweep/mod.rs:412-513)
+// restated in pull form. A cell that is solved distributes the change of its
+// outgoing rate over its downwind faces in proportion to area * (n . d)
+// (:453-461), so the incoming rate of a cell equals, up to rounding,
+//   I[c,d] = sum_{upwind local faces f} area_f (-n_f . d) * O[nb,d] / A[nb,d]
+// with A[nb,d] = sum of area (n . d) over nb's downwind faces. Storing
+// X = O / A lets every task gather its upwind neighbours' X instead of
+// scattering with atomics; the result depends only on the face order, not on
+// the order in which tasks run. Tasks are run level set by level set: level k
+// holds the (cell, direction) pairs whose active upwind neighbours are all in
+// levels < k (Kahn layering of init_counts' counts, src/sweep/mod.rs:346-386).
+// The level sets of each active set are built on the device during the first
+// sweep that needs them (per-direction queues fed by ballot-aggregated atomic
+// appends) and replayed by later sweeps until a timestep-level change alters
+// the active set.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/subsweep_amd.h"
+#include "chemistry.h"
+#include "direction_bins.h"
+#include "level_math.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kKindLocal = 0;
+constexpr uint32_t kKindPeriodic = 1;
+constexpr uint32_t kNbrMask = 0x3FFFFFFFu;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int kLanes = 16;             // lanes per task in the transport kernel
+constexpr uint32_t kCellBits = 25;      // task record: cell index bits
+constexpr uint32_t kCellMask = (1u << kCellBits) - 1u;
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                           \
+    do {                                                                                        \
+        hipError_t _e = (expr);                                                                 \
+        if (_e != hipSuccess)                                                                   \
+            return fail(_e == hipErrorOutOfMemory ? SSW_ERR_OOM : SSW_ERR_HIP,                  \
+                        std::string(#expr) + ": " + hipGetErrorString(_e));                     \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// device helpers
+
+__device__ __forceinline__ double fdot(const double4& g, double dx, double dy, double dz) {
+    // glam DVec3::dot: (x*x') + (y*y') + (z*z'), no contraction
+    return (g.x * dx + g.y * dy) + g.z * dz;
+}
+
+// Appends `value` for every lane with `pred` to dst[*counter ...], one atomic per wave.
+__device__ __forceinline__ void wave_append(bool pred, uint32_t value, uint32_t* counter, uint32_t* dst) {
+    unsigned long long m = __ballot(pred);
+    if (m == 0ull) return;
+    unsigned long long act = __ballot(1);
+    int leader = __ffsll((long long)act) - 1;
+    int lane = __lane_id();
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (pred) dst[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+
+struct Geo {
+    int64_t N;
+    int D;
+    int L;
+    int Kcap;                // offsets per (slot, direction) = Kcap + 2
+    const uint32_t* off;
+    const double4* geo;
+    const uint32_t* nk;
+    const double* dirs;      // D x 3
+    const uint8_t* level;
+    const uint32_t* pslot;   // N: periodic slot or kNone
+    const uint32_t* pcells;  // Nper
+    int64_t Nper;
+    uint2* queue;            // L x D x N task records
+    uint32_t* lvloff;        // L x D x (Kcap + 2)
+    uint32_t* ctr;           // L x 3 x D
+    uint32_t* cnt;           // D x N
+    uint32_t* need;          // L
+};
+
+template <typename R>
+struct Flux {
+    R* X;                    // D x N
+    R* I;                    // D x N
+    R* P;                    // D x Nper
+    const double* src_over_d;
+    const double* att;
+    double thr;              // threshold in the storage scale
+    double inv_lref;         // storage scale of rates
+    double inv_aref;         // storage scale of areas
+};
+
+// ---------------------------------------------------------------------------
+// grid checks and statistics (once per grid)
+
+// Every local face c->nb needs a mirror face nb->c with exactly the negated
+// normal, which makes the upwind/downwind classification of the pair agree.
+__global__ void k_check_mirror(Geo g, unsigned long long* bad) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    for (uint32_t f = g.off[c]; f < g.off[c + 1]; f++) {
+        uint32_t nk = g.nk[f];
+        if ((nk >> 30) != kKindLocal) continue;
+        uint32_t nb = nk & kNbrMask;
+        if ((int64_t)nb >= g.N || (int64_t)nb == c) {
+            atomicAdd(bad, 1ull);
+            continue;
+        }
+        double4 a = g.geo[f];
+        bool found = false;
+        for (uint32_t h = g.off[nb]; h < g.off[nb + 1]; h++) {
+            uint32_t hk = g.nk[h];
+            if ((hk >> 30) != kKindLocal || (int64_t)(hk & kNbrMask) != c) continue;
+            double4 b = g.geo[h];
+            if (b.x == -a.x && b.y == -a.y && b.z == -a.z) found = true;
+        }
+        if (!found) atomicAdd(bad, 1ull);
+    }
+}
+
+// Downwind faces per (cell, direction) by kind, for the byte model.
+__global__ void k_downwind_stats(Geo g, unsigned long long* out3) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int d = blockIdx.y;
+    unsigned long long n[3] = {0, 0, 0};
+    if (c < g.N) {
+        double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+        for (uint32_t f = g.off[c]; f < g.off[c + 1]; f++) {
+            if (fdot(g.geo[f], dx, dy, dz) > 0.0) n[g.nk[f] >> 30]++;
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        unsigned long long v = n[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+        if (__lane_id() == 0 && v) atomicAdd(&out3[k], v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// per-sweep kernels
+
+// Resets the append counters of slot `lvl` when its schedule is rebuilt and
+// takes the periodic-source snapshot of the active periodic cells.
+template <typename R>
+__global__ void k_begin(Geo g, Flux<R> fl, int lvl) {
+    int d = blockIdx.y;
+    const bool build = g.need[lvl] != 0;
+    if (build && blockIdx.x == 0 && threadIdx.x == 0) {
+        uint32_t* ctr = g.ctr + (size_t)lvl * 3 * g.D;
+        ctr[0 * g.D + d] = 0;
+        ctr[1 * g.D + d] = 0;
+        g.lvloff[((size_t)lvl * g.D + d) * (g.Kcap + 2)] = 0;
+    }
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (fl.P == nullptr || s >= g.Nper) return;
+    uint32_t c = g.pcells[s];
+    if (g.level[c] < lvl) return;
+    double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+    R acc = 0;
+    for (uint32_t f = g.off[c]; f < g.off[c + 1]; f++) {
+        uint32_t nk = g.nk[f];
+        if ((nk >> 30) != kKindPeriodic) continue;
+        double4 a = g.geo[f];
+        double dd = fdot(a, dx, dy, dz);
+        if (dd < 0.0) acc += (R)(a.w * fl.inv_aref * -dd) * fl.X[(size_t)d * g.N + (nk & kNbrMask)];
+    }
+    fl.P[(size_t)d * g.Nper + s] = acc;
+}
+
+// A queued task: the cell and its face range, so that solving it needs no
+// lookup of the face offsets. x = cell | face_count << 25, y = first face.
+__device__ __forceinline__ uint2 task_record(const Geo& g, uint32_t c) {
+    uint32_t f0 = g.off[c], f1 = g.off[c + 1];
+    return make_uint2(c | ((f1 - f0) << kCellBits), f0);
+}
+
+// Appends a task record for every lane with `pred`, one atomic per wave.
+__device__ __forceinline__ void wave_append_rec(bool pred, uint2 rec, uint32_t* counter, uint2* dst) {
+    unsigned long long m = __ballot(pred);
+    if (m == 0ull) return;
+    unsigned long long act = __ballot(1);
+    int leader = __ffsll((long long)act) - 1;
+    int lane = __lane_id();
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (pred) dst[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = rec;
+}
+
+// init_counts (src/sweep/mod.rs:346-386) + get_initial_tasks (:388-398) for
+// one direction per blockIdx.y: counts active local upwind neighbours and
+// queues the cells with count 0 as level set 0.
+__global__ void k_count(Geo g, int lvl, int only_counts) {
+    const bool build = only_counts || g.need[lvl] != 0;
+    if (!build) return;
+    int d = blockIdx.y;
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = c < g.N && g.level[c] >= lvl;
+    uint32_t n = 0;
+    if (active) {
+        double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+        for (uint32_t f = g.off[c]; f < g.off[c + 1]; f++) {
+            uint32_t nk = g.nk[f];
+            if ((nk >> 30) != kKindLocal) continue;
+            if (!(fdot(g.geo[f], dx, dy, dz) < 0.0)) continue;
+            if (g.level[nk & kNbrMask] >= lvl) n++;
+        }
+    }
+    if (c < g.N) g.cnt[(size_t)d * g.N + c] = active ? n : kNone;
+    if (only_counts) return;
+    uint2* q = g.queue + ((size_t)lvl * g.D + d) * g.N;
+    uint2 rec = active ? task_record(g, (uint32_t)c) : make_uint2(0, 0);
+    wave_append_rec(active && n == 0, rec, g.ctr + (size_t)lvl * 3 * g.D + d, q);
+}
+
+// Solves level set k of every direction (blockIdx.y = direction). Each task
+// (one cell in one direction) is handled by kLanes lanes, one face per lane,
+// and the per-face sums are combined by lane shuffles. While the schedule of
+// this active set is being built, the kernel also decrements the counters of
+// active local downwind neighbours and queues those that reach zero as level
+// set k+1 (handle_local_neighbour, src/sweep/mod.rs:487-503).
+template <typename R, bool kTransport>
+__global__ void __launch_bounds__(kBlock) k_sweep_level(Geo g, Flux<R> fl, int lvl, int k) {
+    const int d = blockIdx.y;
+    const bool build = g.need[lvl] != 0;
+    uint32_t* lo = g.lvloff + ((size_t)lvl * g.D + d) * (g.Kcap + 2);
+    uint32_t* ctr = g.ctr + (size_t)lvl * 3 * g.D;
+    const uint32_t beg = lo[k];
+    uint32_t end;
+    if (build) {
+        end = beg + ctr[(k % 3) * g.D + d];
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            lo[k + 1] = end;
+            ctr[((k + 2) % 3) * g.D + d] = 0;
+        }
+    } else {
+        end = lo[k + 1];
+    }
+    if (beg >= end) return;
+    uint2* q = g.queue + ((size_t)lvl * g.D + d) * g.N;
+    uint32_t* next_ctr = ctr + ((k + 1) % 3) * g.D + d;
+    const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+    const size_t dN = (size_t)d * g.N;
+    constexpr uint32_t kTasksPerBlock = kBlock / kLanes;
+    const uint32_t sub = threadIdx.x % kLanes;
+    const uint32_t stride = gridDim.x * kTasksPerBlock;
+    for (uint32_t i0 = beg + blockIdx.x * kTasksPerBlock; i0 < end; i0 += stride) {
+        const uint32_t i = i0 + threadIdx.x / kLanes;
+        const bool valid = i < end;
+        const uint2 rec = valid ? q[i] : make_uint2(0, 0);
+        const uint32_t c = rec.x & kCellMask;
+        const uint32_t nf = rec.x >> kCellBits;
+        double A = 0.0;
+        R inc = 0;
+        for (uint32_t j = sub; j < nf; j += kLanes) {
+            const uint32_t f = rec.y + j;
+            const double4 a = g.geo[f];
+            const uint32_t nk = g.nk[f];
+            const uint32_t nb = nk & kNbrMask;
+            const double dd = fdot(a, dx, dy, dz);
+            bool ready = false;
+            if (dd > 0.0) {
+                A += a.w * fl.inv_aref * dd;
+                if (build && (nk >> 30) == kKindLocal && g.level[nb] >= lvl)
+                    ready = atomicSub(&g.cnt[dN + nb], 1u) == 1u;
+            } else if (kTransport && dd < 0.0 && (nk >> 30) == kKindLocal) {
+                inc += (R)(a.w * fl.inv_aref * -dd) * fl.X[dN + nb];
+            }
+            if (build) wave_append_rec(ready, ready ? task_record(g, nb) : make_uint2(0, 0), next_ctr, q + end);
+        }
+        if (kTransport) {
+#pragma unroll
+            for (int m = kLanes / 2; m > 0; m >>= 1) {
+                A += __shfl_xor(A, m, kLanes);
+                inc += __shfl_xor(inc, m, kLanes);
+            }
+            if (valid && sub == 0) {
+                if (inc < (R)0) inc = 0;  // Photons::make_positive
+                const uint32_t ps = g.pslot[c];
+                R per = (ps != kNone && fl.P) ? fl.P[(size_t)d * g.Nper + ps] : (R)0;
+                R in = (inc + (R)fl.src_over_d[c]) + per;  // Site::get_rate
+                R out = (in < (R)fl.thr) ? (R)0 : in * (R)fl.att[c];
+                fl.X[dN + c] = (A > 0.0) ? out / (R)A : (R)0;
+                fl.I[dN + c] = inc;
+            }
+        }
+    }
+}
+
+struct ChemArgs {
+    double* rho;
+    double* x;
+    double* T;
+    double* src;             // source per direction bin (SI)
+    double* att;
+    double* species_ts;
+    double* prev_rate;
+    double* change_ts;
+    const double* vol;
+    const double* size;
+    double max_ts;
+    double thr;
+    double safety;
+    double scale_factor;
+    int prevent_cooling;
+    double lref;             // storage scale of I (rate = I * lref)
+    double aref;
+    unsigned long long* failures;
+    unsigned long long* active;
+};
+
+// update_chemistry (src/sweep/mod.rs:549-574) for the active cells, then the
+// attenuation of the updated cell for the next sweep.
+template <typename R>
+__global__ void k_chemistry(Geo g, Flux<R> fl, ChemArgs ca, int lvl) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c == 0) g.need[lvl] = 0;  // the schedule of this active set is complete
+    if (c >= g.N) return;
+    int lev = g.level[c];
+    {
+        unsigned long long m = __ballot(lev >= lvl);
+        if (__lane_id() == __ffsll((long long)__ballot(1)) - 1 && m) atomicAdd(ca.active, (unsigned long long)__popcll(m));
+    }
+    if (lev < lvl) return;
+    const double dt = ca.max_ts * ssw::powi(0.5, lev);
+    const bool has_p = g.pslot[c] != kNone && fl.P;
+    double rate = 0.0;
+    for (int d = 0; d < g.D; d++) {
+        double inc = (double)fl.I[(size_t)d * g.N + c] * ca.lref;
+        double per = 0.0;
+        if (has_p) {  // current periodic source (the sweep has finished)
+            double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+            R acc = 0;
+            for (uint32_t f = g.off[c]; f < g.off[c + 1]; f++) {
+                uint32_t nk = g.nk[f];
+                if ((nk >> 30) != kKindPeriodic) continue;
+                double4 a = g.geo[f];
+                double dd = fdot(a, dx, dy, dz);
+                if (dd < 0.0) acc += (R)(a.w * fl.inv_aref * -dd) * fl.X[(size_t)d * g.N + (nk & kNbrMask)];
+            }
+            per = (double)acc * ca.lref;
+        }
+        rate += (inc + ca.src[c]) + per;
+    }
+    double rel;
+    if (fabs(rate) < fabs(ca.thr)) {
+        rel = 0.0;
+    } else {  // PhotonRate::relative_change_to(..).abs() (src/chemistry/mod.rs:73-77)
+        rel = fabs(fmin(fabs(fabs(rate - ca.prev_rate[c]) / rate), ssw::kInvEps));
+    }
+    ca.prev_rate[c] = rate;
+    const double rate_ts = dt / rel;
+    ssw::Cell cell;
+    cell.x = ca.x[c];
+    cell.T = ca.T[c];
+    cell.rho = ca.rho[c];
+    cell.vol = ca.vol[c];
+    cell.len = ca.size[c];
+    cell.rate = rate;
+    cell.a = ca.scale_factor;
+    ssw::Floor fl0{ca.prevent_cooling != 0, cell.T, cell.x};
+    int proc = 0;
+    bool failed = false;
+    double used = ssw::perform_timestep(cell, fl0, dt, ca.safety, proc, failed);
+    if (failed) atomicAdd(ca.failures, 1ull);
+    ca.x[c] = cell.x;
+    ca.T[c] = cell.T;
+    ca.species_ts[c] = used;
+    ca.change_ts[c] = (rate_ts < used) ? rate_ts : used;  // Timescale::min
+    ca.att[c] = ssw::non_absorbed_fraction(cell.rho, cell.x, cell.len);
+}
+
+// update_timestep_levels (src/sweep/mod.rs:576-589) for all cells. Marks the
+// schedules whose active set changed.
+__global__ void k_levels(Geo g, const double* change_ts, uint8_t* level, double max_ts, double safety,
+                         int lowest) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    double desired = safety * change_ts[c];
+    int lev = ssw::level_from_ratio(max_ts / desired, g.L);
+    if (lev < lowest) lev = lowest;
+    int old = level[c];
+    level[c] = (uint8_t)lev;
+    for (int l = 0; l < g.L; l++)
+        if ((old >= l) != (lev >= l)) g.need[l] = 1;
+}
+
+// Pull evaluation of the current incoming / outgoing / periodic rates of every
+// (cell, direction) in SI units (parity hooks and the photon-rate output).
+template <typename R>
+__global__ void k_fluxes(Geo g, Flux<R> fl, double lref, double* I_out, double* O_out, double* P_out,
+                         double* photon_rate) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    double total = 0.0;
+    for (int d = 0; d < g.D; d++) {
+        double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+        double A = 0.0;
+        R inc = 0, per = 0;
+        for (uint32_t f = g.off[c]; f < g.off[c + 1]; f++) {
+            double4 a = g.geo[f];
+            uint32_t nk = g.nk[f];
+            double dd = fdot(a, dx, dy, dz);
+            if (dd > 0.0) {
+                A += a.w * dd;
+            } else if (dd < 0.0 && (nk >> 30) != 2u) {
+                R v = (R)(a.w * fl.inv_aref * -dd) * fl.X[(size_t)d * g.N + (nk & kNbrMask)];
+                if ((nk >> 30) == kKindLocal) inc += v;
+                else per += v;
+            }
+        }
+        double inc_si = (double)inc * lref;
+        total += inc_si;
+        size_t o = (size_t)d * g.N + c;
+        if (I_out) I_out[o] = inc_si;
+        if (P_out) P_out[o] = (double)per * lref;
+        if (O_out) O_out[o] = (double)fl.X[o] * (A * fl.inv_aref) * lref;
+    }
+    if (photon_rate) photon_rate[c] = total;
+}
+
+// Kahn depth of every queued task of slot `lvl`.
+__global__ void k_scatter_depth(Geo g, int lvl, int kmax, uint32_t* depth) {
+    int d = blockIdx.y;
+    const uint32_t* lo = g.lvloff + ((size_t)lvl * g.D + d) * (g.Kcap + 2);
+    const uint2* q = g.queue + ((size_t)lvl * g.D + d) * g.N;
+    for (int k = 0; k < kmax; k++) {
+        for (uint32_t i = lo[k] + blockIdx.x * blockDim.x + threadIdx.x; i < lo[k + 1];
+             i += gridDim.x * blockDim.x)
+            depth[(size_t)d * g.N + (q[i].x & kCellMask)] = (uint32_t)k;
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_attenuation(const double* rho, const double* x, const double* size, double* att, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) att[i] = ssw::non_absorbed_fraction(rho[i], x[i], size[i]);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+// TimestepState (src/sweep/timestep_state.rs:4-80)
+struct TimestepState {
+    double max_ts = 0;
+    int L = 1;
+    int lowest = 0;
+    bool first_done = false;
+    void init(double m, int n) {
+        max_ts = m;
+        L = n;
+        lowest = n - 1;
+        first_done = false;
+    }
+    std::vector<int> sweep_order() const {
+        int num = L - lowest;
+        std::vector<int> out;
+        for (uint32_t i = 0; i < (1u << (num - 1)); i++) {
+            int fb = i ? __builtin_ctz(i) : num - 1;
+            out.push_back(lowest + (num - 1 - fb));
+        }
+        return out;
+    }
+    void advance() {
+        if (first_done && lowest > 0) lowest -= 1;
+        if (!first_done) first_done = true;
+    }
+    double current_max_timestep() const { return max_ts * ssw::powi(0.5, lowest); }
+};
+
+unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+struct ssw_sweep {
+    ssw_sweep_parameters p{};
+    int D = 0;
+    std::vector<double> dirs;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool fp32 = false;
+    // grid
+    int64_t N = 0, F = 0, Nper = 0;
+    bool have_grid = false, have_state = false;
+    uint32_t *d_off = nullptr, *d_nk = nullptr, *d_pslot = nullptr, *d_pcells = nullptr;
+    double4* d_geo = nullptr;
+    double *d_dirs = nullptr, *d_vol = nullptr, *d_size = nullptr;
+    double aref = 1.0;
+    int kmax = 0;
+    int64_t max_width = 0;
+    double dw[3] = {0, 0, 0};
+    // schedule
+    uint2* d_queue = nullptr;
+    uint32_t *d_lvloff = nullptr, *d_ctr = nullptr, *d_cnt = nullptr, *d_need = nullptr;
+    int kcap = 0;
+    // state
+    double *d_rho = nullptr, *d_x = nullptr, *d_T = nullptr, *d_src = nullptr, *d_att = nullptr;
+    double *d_sts = nullptr, *d_prev = nullptr, *d_change = nullptr, *d_src_t = nullptr;
+    unsigned long long* d_active = nullptr;
+    uint8_t* d_level = nullptr;
+    void *d_X = nullptr, *d_I = nullptr, *d_P = nullptr;
+    unsigned long long* d_fail = nullptr;
+    double lref = 1.0;
+    TimestepState ts;
+    int64_t bytes = 0;
+    int64_t updates_last = 0, sweeps_last = 0;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    double t_transport_total = 0, t_chem_total = 0;
+    int64_t n_transport = 0;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(ssw_sweep* s, T** p, size_t count) {
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(SSW_ERR_OOM, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
+                                     " bytes failed: " + hipGetErrorString(e));
+    }
+    s->bytes += (int64_t)(count * sizeof(T));
+    return SSW_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+#define TRY(expr)              \
+    do {                       \
+        int _r = (expr);       \
+        if (_r != SSW_OK) return _r; \
+    } while (0)
+
+Geo make_geo(ssw_sweep* s) {
+    Geo g;
+    g.N = s->N;
+    g.D = s->D;
+    g.L = s->p.num_timestep_levels;
+    g.Kcap = s->kcap;
+    g.off = s->d_off;
+    g.geo = s->d_geo;
+    g.nk = s->d_nk;
+    g.dirs = s->d_dirs;
+    g.level = s->d_level;
+    g.pslot = s->d_pslot;
+    g.pcells = s->d_pcells;
+    g.Nper = s->Nper;
+    g.queue = s->d_queue;
+    g.lvloff = s->d_lvloff;
+    g.ctr = s->d_ctr;
+    g.cnt = s->d_cnt;
+    g.need = s->d_need;
+    return g;
+}
+
+template <typename R>
+Flux<R> make_flux(ssw_sweep* s) {
+    Flux<R> f;
+    f.X = (R*)s->d_X;
+    f.I = (R*)s->d_I;
+    f.P = (R*)s->d_P;
+    f.src_over_d = s->fp32 ? s->d_src_t : s->d_src;
+    f.att = s->d_att;
+    f.inv_lref = 1.0 / s->lref;
+    f.inv_aref = 1.0 / s->aref;
+    f.thr = s->p.significant_rate_threshold * f.inv_lref;
+    return f;
+}
+
+int set_need_all(ssw_sweep* s, uint32_t v) {
+    k_fill_u32<<<1, kBlock, 0, s->stream>>>(s->d_need, v, s->p.num_timestep_levels);
+    HIP_TRY(hipGetLastError());
+    return SSW_OK;
+}
+
+unsigned sweep_grid_x(ssw_sweep* s) {
+    // enough blocks for the widest level set of one direction in the full graph
+    // (plus headroom for restricted graphs); a grid-stride loop covers the rest
+    int64_t w = s->max_width + s->max_width / 2 + 1;
+    int64_t b = (w + (kBlock / kLanes) - 1) / (kBlock / kLanes);
+    return (unsigned)std::min<int64_t>(std::max<int64_t>(b, 1), 4096);
+}
+
+// Builds the full-graph schedule once (all cells active) to find the depth
+// bound and detect cycles. Grows the level-offset arrays as needed.
+int build_full_schedule(ssw_sweep* s) {
+    const int L = s->p.num_timestep_levels;
+    s->max_width = 0;
+    int kcap = 256;
+    s->kcap = kcap;
+    TRY(dalloc(s, &s->d_lvloff, (size_t)L * s->D * (kcap + 2)));
+    TRY(set_need_all(s, 1u));
+    Geo g = make_geo(s);
+    Flux<double> fl{};
+    dim3 gb(grid_for(s->N), s->D);
+    k_begin<double><<<dim3(1, s->D), kBlock, 0, s->stream>>>(g, fl, 0);
+    k_count<<<gb, kBlock, 0, s->stream>>>(g, 0, 0);
+    HIP_TRY(hipGetLastError());
+    std::vector<uint32_t> hctr(s->D);
+    int64_t total = 0;
+    int k = 0;
+    unsigned gx = grid_for(s->N);
+    for (;; k++) {
+        if (k + 2 > kcap) {  // grow the offsets of slot 0
+            int ncap = kcap * 2;
+            uint32_t* nl = nullptr;
+            TRY(dalloc(s, &nl, (size_t)L * s->D * (ncap + 2)));
+            std::vector<uint32_t> host((size_t)s->D * (kcap + 2));
+            HIP_TRY(hipMemcpyAsync(host.data(), s->d_lvloff, host.size() * 4, hipMemcpyDeviceToHost, s->stream));
+            HIP_TRY(hipStreamSynchronize(s->stream));
+            std::vector<uint32_t> grown((size_t)s->D * (ncap + 2), 0);
+            for (int d = 0; d < s->D; d++)
+                std::memcpy(&grown[(size_t)d * (ncap + 2)], &host[(size_t)d * (kcap + 2)], (kcap + 2) * 4);
+            HIP_TRY(hipMemcpyAsync(nl, grown.data(), grown.size() * 4, hipMemcpyHostToDevice, s->stream));
+            HIP_TRY(hipStreamSynchronize(s->stream));
+            dfree(s->d_lvloff);
+            s->d_lvloff = nl;
+            kcap = ncap;
+            s->kcap = kcap;
+            g = make_geo(s);
+        }
+        // size of level k
+        HIP_TRY(hipMemcpyAsync(hctr.data(), s->d_ctr + (size_t)(k % 3) * s->D, s->D * 4, hipMemcpyDeviceToHost,
+                               s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        int64_t lv = 0, wk = 0;
+        for (int d = 0; d < s->D; d++) {
+            lv += hctr[d];
+            wk = std::max<int64_t>(wk, hctr[d]);
+        }
+        s->max_width = std::max(s->max_width, wk);
+        gx = (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, (wk + kBlock / kLanes - 1) / (kBlock / kLanes)));
+        if (lv == 0) break;
+        total += lv;
+        if (k > s->N + 1) return fail(SSW_ERR_CYCLE, "dependency graph deeper than the number of cells");
+        k_sweep_level<double, false><<<dim3(gx, s->D), kBlock, 0, s->stream>>>(g, fl, 0, k);
+        HIP_TRY(hipGetLastError());
+    }
+    if (total != s->N * (int64_t)s->D)
+        return fail(SSW_ERR_CYCLE, "dependency cycle: " + std::to_string(s->N * (int64_t)s->D - total) +
+                                       " cell-direction pairs never become ready");
+    s->kmax = std::max(1, k);
+    // final offsets with the exact bound
+    s->kcap = s->kmax;
+    TRY(dalloc(s, &s->d_lvloff, (size_t)L * s->D * (s->kcap + 2)));
+    return SSW_OK;
+}
+
+template <typename R>
+int run_step(ssw_sweep* s, double* elapsed) {
+    Geo g = make_geo(s);
+    Flux<R> fl = make_flux<R>(s);
+    ChemArgs ca;
+    ca.rho = s->d_rho;
+    ca.x = s->d_x;
+    ca.T = s->d_T;
+    ca.src = s->d_src;
+    ca.att = s->d_att;
+    ca.species_ts = s->d_sts;
+    ca.prev_rate = s->d_prev;
+    ca.change_ts = s->d_change;
+    ca.vol = s->d_vol;
+    ca.size = s->d_size;
+    ca.max_ts = s->p.max_timestep;
+    ca.thr = s->p.significant_rate_threshold;
+    ca.safety = s->p.chemistry_timestep_safety_factor;
+    ca.scale_factor = s->p.scale_factor;
+    ca.prevent_cooling = s->p.prevent_cooling;
+    ca.lref = s->lref;
+    ca.aref = s->aref;
+    ca.failures = s->d_fail;
+    ca.active = s->d_active;
+    Flux<R> flt = fl;
+    std::vector<int> order = s->ts.sweep_order();
+    k_fill_u32<<<1, 2, 0, s->stream>>>((uint32_t*)s->d_active, 0u, 2);
+    const unsigned gx = sweep_grid_x(s);
+    const dim3 gper(grid_for(std::max<int64_t>(s->Nper, 1)), s->D);
+    const dim3 gcount(grid_for(s->N), s->D);
+    const unsigned gcells = grid_for(s->N);
+    s->sweeps_last = 0;
+    s->t_transport_total = s->t_chem_total = 0;
+    s->n_transport = 0;
+    size_t evn = 0;
+    auto mark = [&](void) -> hipEvent_t {
+        if (evn >= s->ev.size()) {
+            hipEvent_t e;
+            (void)hipEventCreate(&e);
+            s->ev.push_back(e);
+        }
+        hipEvent_t e = s->ev[evn++];
+        (void)hipEventRecord(e, s->stream);
+        return e;
+    };
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> tr, ch;
+    for (int lvl : order) {
+        k_begin<R><<<gper, kBlock, 0, s->stream>>>(g, flt, lvl);
+        k_count<<<gcount, kBlock, 0, s->stream>>>(g, lvl, 0);
+        for (int k = 0; k < s->kmax; k++) {
+            hipEvent_t a = s->timing ? mark() : nullptr;
+            k_sweep_level<R, true><<<dim3(gx, s->D), kBlock, 0, s->stream>>>(g, flt, lvl, k);
+            if (s->timing) tr.push_back({a, mark()});
+        }
+        hipEvent_t a = s->timing ? mark() : nullptr;
+        k_chemistry<R><<<gcells, kBlock, 0, s->stream>>>(g, fl, ca, lvl);
+        if (s->timing) ch.push_back({a, mark()});
+        s->sweeps_last++;
+    }
+    HIP_TRY(hipGetLastError());
+    double el = s->ts.current_max_timestep();
+    s->ts.advance();
+    k_levels<<<gcells, kBlock, 0, s->stream>>>(g, s->d_change, s->d_level, s->p.max_timestep,
+                                               s->p.timestep_safety_factor, s->ts.lowest);
+    HIP_TRY(hipGetLastError());
+    if (s->timing) {
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        for (auto& pr : tr) {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, pr.first, pr.second);
+            s->t_transport_total += ms;
+            s->n_transport++;
+        }
+        for (auto& pr : ch) {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, pr.first, pr.second);
+            s->t_chem_total += ms;
+        }
+    }
+    if (elapsed) *elapsed = el;
+    return SSW_OK;
+}
+
+void free_state(ssw_sweep* s) {
+    dfree(s->d_rho);
+    dfree(s->d_x);
+    dfree(s->d_T);
+    dfree(s->d_src);
+    dfree(s->d_att);
+    dfree(s->d_sts);
+    dfree(s->d_prev);
+    dfree(s->d_change);
+    dfree(s->d_src_t);
+    dfree(s->d_X);
+    dfree(s->d_I);
+    dfree(s->d_P);
+}
+
+void free_grid(ssw_sweep* s) {
+    dfree(s->d_off);
+    dfree(s->d_geo);
+    dfree(s->d_nk);
+    dfree(s->d_vol);
+    dfree(s->d_size);
+    dfree(s->d_pslot);
+    dfree(s->d_pcells);
+    dfree(s->d_queue);
+    dfree(s->d_lvloff);
+    dfree(s->d_ctr);
+    dfree(s->d_cnt);
+    dfree(s->d_level);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ssw_last_error(void) { return g_err.c_str(); }
+
+void ssw_default_parameters(ssw_sweep_parameters* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->num_directions = 84;
+    p->explicit_directions = nullptr;
+    p->num_timestep_levels = 1;
+    p->periodic = 0;
+    p->max_timestep = 1e6 * 3.15576e7;
+    p->significant_rate_threshold = 0.0;
+    p->timestep_safety_factor = 0.01 * 10.0;            // percent(10.0)
+    p->chemistry_timestep_safety_factor = 0.01 * 10.0;
+    p->prevent_cooling = 1;
+    p->transport_precision = SSW_TRANSPORT_FP64;
+    p->scale_factor = 1.0;
+}
+
+int ssw_create(const ssw_sweep_parameters* params, ssw_sweep** out) {
+    if (!params || !out) return fail(SSW_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const ssw_sweep_parameters& p = *params;
+    if (p.num_timestep_levels < 1 || p.num_timestep_levels >= 32)
+        return fail(SSW_ERR_INVALID, "num_timestep_levels must be in [1, 31]");
+    if (p.transport_precision != SSW_TRANSPORT_FP64 && p.transport_precision != SSW_TRANSPORT_FP32)
+        return fail(SSW_ERR_INVALID, "transport_precision must be 32 or 64");
+    if (!(p.max_timestep > 0.0)) return fail(SSW_ERR_INVALID, "max_timestep must be positive");
+    std::vector<double> dirs;
+    if (p.explicit_directions) {
+        if (p.num_directions < 1 || p.num_directions > 127)
+            return fail(SSW_ERR_INVALID, "explicit direction count must be in [1, 127]");
+        for (int d = 0; d < p.num_directions; d++) {
+            const double* v = p.explicit_directions + 3 * d;
+            double len = std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+            if (!(len > 0.0)) return fail(SSW_ERR_INVALID, "zero explicit direction");
+            double r = 1.0 / len;  // glam normalize: v * (1/|v|)
+            dirs.push_back(v[0] * r);
+            dirs.push_back(v[1] * r);
+            dirs.push_back(v[2] * r);
+        }
+    } else {
+        const double(*bins)[3] = nullptr;
+        static const double one[1][3] = {{1.0, 0.0, 0.0}};
+        switch (p.num_directions) {  // Directions::from_num (src/sweep/direction/mod.rs:58-75)
+            case 1: bins = one; break;
+            case 16: bins = SSW_DIRECTION_BINS_16; break;
+            case 21: bins = SSW_DIRECTION_BINS_21; break;
+            case 32: bins = SSW_DIRECTION_BINS_32; break;
+            case 64: bins = SSW_DIRECTION_BINS_64; break;
+            case 84: bins = SSW_DIRECTION_BINS_84; break;
+            default: return fail(SSW_ERR_INVALID, "num_directions must be 1, 16, 21, 32, 64 or 84");
+        }
+        for (int d = 0; d < p.num_directions; d++)
+            for (int k = 0; k < 3; k++) dirs.push_back(bins[d][k]);
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(SSW_ERR_NO_DEVICE, "no HIP device");
+    ssw_sweep* s = new ssw_sweep();
+    s->p = p;
+    s->p.explicit_directions = nullptr;
+    s->D = p.num_directions;
+    s->dirs = dirs;
+    s->fp32 = p.transport_precision == SSW_TRANSPORT_FP32;
+    (void)hipGetDevice(&s->device);
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        return fail(SSW_ERR_HIP, "hipStreamCreate failed");
+    }
+    int r = dalloc(s, &s->d_dirs, dirs.size());
+    if (r == SSW_OK) r = dalloc(s, &s->d_need, (size_t)p.num_timestep_levels);
+    if (r == SSW_OK) r = dalloc(s, &s->d_fail, 1);
+    if (r == SSW_OK) r = dalloc(s, &s->d_active, 1);
+    if (r != SSW_OK) {
+        ssw_destroy(s);
+        return r;
+    }
+    if (hipMemcpy(s->d_dirs, dirs.data(), dirs.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+        ssw_destroy(s);
+        return fail(SSW_ERR_HIP, "copy of directions failed");
+    }
+    s->ts.init(p.max_timestep, p.num_timestep_levels);
+    *out = s;
+    return SSW_OK;
+}
+
+void ssw_destroy(ssw_sweep* s) {
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    free_state(s);
+    free_grid(s);
+    dfree(s->d_dirs);
+    dfree(s->d_need);
+    dfree(s->d_fail);
+    dfree(s->d_active);
+    for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* nbr, const uint8_t* kind,
+                    const double* area, const double* normal, const double* volume, const double* size) {
+    if (!s || !off || !nbr || !kind || !area || !normal || !volume || !size)
+        return fail(SSW_ERR_INVALID, "null argument");
+    if (n <= 0 || n > (int64_t)kCellMask) return fail(SSW_ERR_INVALID, "cell count must be in [1, 2^25 - 1]");
+    if ((int64_t)s->D * n >= (int64_t)0xFFFFFFFFll) return fail(SSW_ERR_INVALID, "cells x directions exceeds 2^32");
+    if (off[0] != 0) return fail(SSW_ERR_INVALID, "face_offsets[0] must be 0");
+    for (int64_t c = 0; c < n; c++) {
+        if (off[c + 1] < off[c]) return fail(SSW_ERR_INVALID, "face_offsets must be non-decreasing");
+        if (off[c + 1] - off[c] > 127) return fail(SSW_ERR_INVALID, "a cell has more than 127 faces");
+    }
+    const int64_t F = off[n];
+    if (F >= (int64_t)0xFFFFFFFFll) return fail(SSW_ERR_INVALID, "too many faces");
+    free_state(s);
+    free_grid(s);
+    s->have_grid = s->have_state = false;
+    s->N = n;
+    s->F = F;
+    // pack faces; a non-periodic sweep treats periodic faces as boundaries
+    std::vector<double4> geo(F);
+    std::vector<uint32_t> nk(F);
+    std::vector<uint32_t> off32(n + 1);
+    std::vector<uint32_t> pslot(n, kNone), pcells;
+    double asum = 0.0;
+    int64_t nlocal = 0;
+    for (int64_t c = 0; c <= n; c++) off32[c] = (uint32_t)off[c];
+    for (int64_t c = 0; c < n; c++) {
+        bool has_p = false;
+        for (int64_t f = off[c]; f < off[c + 1]; f++) {
+            uint32_t k = kind[f];
+            if (k > 2) return fail(SSW_ERR_INVALID, "face_kind must be 0, 1 or 2");
+            if (k == 1 && !s->p.periodic) k = 2;
+            int64_t nb = (k == 2) ? 0 : nbr[f];
+            if (k != 2 && (nb < 0 || nb >= n)) return fail(SSW_ERR_INVALID, "face_neighbour out of range");
+            geo[f] = make_double4(normal[3 * f], normal[3 * f + 1], normal[3 * f + 2], area[f]);
+            nk[f] = (uint32_t)nb | (k << 30);
+            if (k == 1) has_p = true;
+            asum += area[f];
+            nlocal++;
+        }
+        if (has_p) {
+            pslot[c] = (uint32_t)pcells.size();
+            pcells.push_back((uint32_t)c);
+        }
+    }
+    // Areas are carried in units of a power of two near the mean face area, so
+    // that X = O / A keeps the range of O (SI areas reach 1e42 m^2); scaling
+    // by a power of two is exact.
+    s->aref = (nlocal > 0 && asum > 0.0) ? std::ldexp(1.0, (int)std::lround(std::log2(asum / (double)nlocal))) : 1.0;
+    s->Nper = (int64_t)pcells.size();
+    const int L = s->p.num_timestep_levels;
+    TRY(dalloc(s, &s->d_off, n + 1));
+    TRY(dalloc(s, &s->d_geo, F));
+    TRY(dalloc(s, &s->d_nk, F));
+    TRY(dalloc(s, &s->d_vol, n));
+    TRY(dalloc(s, &s->d_size, n));
+    TRY(dalloc(s, &s->d_pslot, n));
+    TRY(dalloc(s, &s->d_pcells, std::max<int64_t>(1, s->Nper)));
+    TRY(dalloc(s, &s->d_level, n));
+    TRY(dalloc(s, &s->d_queue, (size_t)L * s->D * n));
+    TRY(dalloc(s, &s->d_ctr, (size_t)L * 3 * s->D));
+    TRY(dalloc(s, &s->d_cnt, (size_t)s->D * n));
+    HIP_TRY(hipMemcpyAsync(s->d_off, off32.data(), (n + 1) * 4, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_geo, geo.data(), F * sizeof(double4), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_nk, nk.data(), F * 4, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_vol, volume, n * 8, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_size, size, n * 8, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_pslot, pslot.data(), n * 4, hipMemcpyHostToDevice, s->stream));
+    if (s->Nper)
+        HIP_TRY(hipMemcpyAsync(s->d_pcells, pcells.data(), s->Nper * 4, hipMemcpyHostToDevice, s->stream));
+    std::vector<uint8_t> lev(n, (uint8_t)(L - 1));
+    HIP_TRY(hipMemcpyAsync(s->d_level, lev.data(), n, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_ctr, 0, (size_t)L * 3 * s->D * 4, s->stream));
+    // mirror check
+    unsigned long long* d_bad = nullptr;
+    TRY(dalloc(s, &d_bad, 4));
+    HIP_TRY(hipMemsetAsync(d_bad, 0, 4 * 8, s->stream));
+    Geo g = make_geo(s);
+    k_check_mirror<<<grid_for(n), kBlock, 0, s->stream>>>(g, d_bad);
+    k_downwind_stats<<<dim3(grid_for(n), s->D), kBlock, 0, s->stream>>>(g, d_bad + 1);
+    HIP_TRY(hipGetLastError());
+    unsigned long long hb[4] = {0, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(hb, d_bad, sizeof(hb), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    dfree(d_bad);
+    s->bytes -= 32;
+    if (hb[0] != 0)
+        return fail(SSW_ERR_GRID, std::to_string(hb[0]) +
+                                      " local faces have no mirror face with the exactly negated normal");
+    for (int k = 0; k < 3; k++) s->dw[k] = (double)hb[k + 1] / ((double)n * s->D);
+    TRY(build_full_schedule(s));
+    s->have_grid = true;
+    return SSW_OK;
+}
+
+int ssw_upload_state(ssw_sweep* s, const double* density, const double* xhii, const double* temp,
+                     const double* source) {
+    if (!s || !density || !xhii || !temp || !source) return fail(SSW_ERR_INVALID, "null argument");
+    if (!s->have_grid) return fail(SSW_ERR_INVALID, "ssw_upload_grid must be called first");
+    free_state(s);
+    const int64_t n = s->N;
+    const int L = s->p.num_timestep_levels;
+    std::vector<double> sod(n);
+    double lmax = 0.0;
+    for (int64_t c = 0; c < n; c++) {
+        sod[c] = source[c] / (double)s->D;  // Site::source_per_direction_bin
+        lmax = std::max(lmax, std::fabs(sod[c]));
+    }
+    // FP32 transport carries rates in units of a power of two near the largest
+    // source per direction bin (photon rates reach 1e52 s^-1, above FP32 range).
+    s->lref = (s->fp32 && lmax > 0.0) ? std::ldexp(1.0, (int)std::lround(std::log2(lmax))) : 1.0;
+    const size_t es = s->fp32 ? 4 : 8;
+    TRY(dalloc(s, &s->d_rho, n));
+    TRY(dalloc(s, &s->d_x, n));
+    TRY(dalloc(s, &s->d_T, n));
+    TRY(dalloc(s, &s->d_src, n));
+    TRY(dalloc(s, &s->d_att, n));
+    TRY(dalloc(s, &s->d_sts, n));
+    TRY(dalloc(s, &s->d_prev, n));
+    TRY(dalloc(s, &s->d_change, n));
+    if (s->fp32) TRY(dalloc(s, &s->d_src_t, n));
+    TRY(dalloc(s, (uint8_t**)&s->d_X, (size_t)s->D * n * es));
+    TRY(dalloc(s, (uint8_t**)&s->d_I, (size_t)s->D * n * es));
+    if (s->Nper) TRY(dalloc(s, (uint8_t**)&s->d_P, (size_t)s->D * s->Nper * es));
+    HIP_TRY(hipMemcpyAsync(s->d_rho, density, n * 8, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_x, xhii, n * 8, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_T, temp, n * 8, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_src, sod.data(), n * 8, hipMemcpyHostToDevice, s->stream));
+    std::vector<double> sod_scaled;
+    if (s->fp32) {  // the FP32 transport reads source/D in its storage scale
+        sod_scaled.resize(n);
+        for (int64_t c = 0; c < n; c++) sod_scaled[c] = sod[c] / s->lref;
+        HIP_TRY(hipMemcpyAsync(s->d_src_t, sod_scaled.data(), n * 8, hipMemcpyHostToDevice, s->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_sts, 0, n * 8, s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_prev, 0, n * 8, s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_change, 0, n * 8, s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_X, 0, (size_t)s->D * n * es, s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_I, 0, (size_t)s->D * n * es, s->stream));
+    if (s->Nper) HIP_TRY(hipMemsetAsync(s->d_P, 0, (size_t)s->D * s->Nper * es, s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_fail, 0, 8, s->stream));
+    std::vector<uint8_t> lev(n, (uint8_t)(L - 1));  // src/sweep/mod.rs:206
+    HIP_TRY(hipMemcpyAsync(s->d_level, lev.data(), n, hipMemcpyHostToDevice, s->stream));
+    k_attenuation<<<grid_for(n), kBlock, 0, s->stream>>>(s->d_rho, s->d_x, s->d_size, s->d_att, n);
+    HIP_TRY(hipGetLastError());
+    TRY(set_need_all(s, 1u));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->ts.init(s->p.max_timestep, L);
+    s->have_state = true;
+    return SSW_OK;
+}
+
+int ssw_step(ssw_sweep* s, double* elapsed) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    if (!s->have_state) return fail(SSW_ERR_INVALID, "ssw_upload_state must be called first");
+    int r = s->fp32 ? run_step<float>(s, elapsed) : run_step<double>(s, elapsed);
+    if (r != SSW_OK) return r;
+    return SSW_OK;
+}
+
+int ssw_download(ssw_sweep* s, double* x, double* T, double* sts, double* photon, uint8_t* level, double* change) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    if (!s->have_state) return fail(SSW_ERR_INVALID, "ssw_upload_state must be called first");
+    const int64_t n = s->N;
+    if (x) HIP_TRY(hipMemcpyAsync(x, s->d_x, n * 8, hipMemcpyDeviceToHost, s->stream));
+    if (T) HIP_TRY(hipMemcpyAsync(T, s->d_T, n * 8, hipMemcpyDeviceToHost, s->stream));
+    if (sts) HIP_TRY(hipMemcpyAsync(sts, s->d_sts, n * 8, hipMemcpyDeviceToHost, s->stream));
+    if (change) HIP_TRY(hipMemcpyAsync(change, s->d_change, n * 8, hipMemcpyDeviceToHost, s->stream));
+    if (level) HIP_TRY(hipMemcpyAsync(level, s->d_level, n, hipMemcpyDeviceToHost, s->stream));
+    if (photon) {
+        double* d_pr = nullptr;
+        TRY(dalloc(s, &d_pr, n));
+        Geo g = make_geo(s);
+        if (s->fp32)
+            k_fluxes<float><<<grid_for(n), kBlock, 0, s->stream>>>(g, make_flux<float>(s), s->lref, nullptr, nullptr,
+                                                                  nullptr, d_pr);
+        else
+            k_fluxes<double><<<grid_for(n), kBlock, 0, s->stream>>>(g, make_flux<double>(s), s->lref, nullptr,
+                                                                   nullptr, nullptr, d_pr);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(photon, d_pr, n * 8, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        dfree(d_pr);
+        s->bytes -= n * 8;
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SSW_OK;
+}
+
+int ssw_download_counts(ssw_sweep* s, int32_t lvl, uint32_t* counts) {
+    if (!s || !counts) return fail(SSW_ERR_INVALID, "null argument");
+    if (!s->have_grid) return fail(SSW_ERR_INVALID, "ssw_upload_grid must be called first");
+    if (lvl < 0 || lvl >= s->p.num_timestep_levels) return fail(SSW_ERR_INVALID, "sweep_level out of range");
+    Geo g = make_geo(s);
+    k_count<<<dim3(grid_for(s->N), s->D), kBlock, 0, s->stream>>>(g, lvl, 1);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(counts, s->d_cnt, (size_t)s->D * s->N * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SSW_OK;
+}
+
+int ssw_download_levelsets(ssw_sweep* s, int32_t lvl, uint32_t* depth) {
+    if (!s || !depth) return fail(SSW_ERR_INVALID, "null argument");
+    if (!s->have_grid) return fail(SSW_ERR_INVALID, "ssw_upload_grid must be called first");
+    if (lvl < 0 || lvl >= s->p.num_timestep_levels) return fail(SSW_ERR_INVALID, "sweep_level out of range");
+    Geo g = make_geo(s);
+    uint32_t need = 0;
+    HIP_TRY(hipMemcpyAsync(&need, s->d_need + lvl, 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (need) {  // build the schedule of this active set without transport
+        Flux<double> fl{};
+        k_begin<double><<<dim3(1, s->D), kBlock, 0, s->stream>>>(g, fl, lvl);
+        k_count<<<dim3(grid_for(s->N), s->D), kBlock, 0, s->stream>>>(g, lvl, 0);
+        for (int k = 0; k < s->kmax; k++)
+            k_sweep_level<double, false><<<dim3(sweep_grid_x(s), s->D), kBlock, 0, s->stream>>>(g, fl, lvl, k);
+        k_fill_u32<<<1, 1, 0, s->stream>>>(s->d_need + lvl, 0u, 1);
+        HIP_TRY(hipGetLastError());
+    }
+    uint32_t* d_depth = nullptr;
+    TRY(dalloc(s, &d_depth, (size_t)s->D * s->N));
+    k_fill_u32<<<grid_for((int64_t)s->D * s->N), kBlock, 0, s->stream>>>(d_depth, kNone, (int64_t)s->D * s->N);
+    k_scatter_depth<<<dim3(64, s->D), kBlock, 0, s->stream>>>(g, lvl, s->kmax, d_depth);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(depth, d_depth, (size_t)s->D * s->N * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    dfree(d_depth);
+    s->bytes -= (int64_t)s->D * s->N * 4;
+    return SSW_OK;
+}
+
+int ssw_download_fluxes(ssw_sweep* s, double* incoming, double* outgoing, double* periodic) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    if (!s->have_state) return fail(SSW_ERR_INVALID, "ssw_upload_state must be called first");
+    const size_t m = (size_t)s->D * s->N;
+    double *dI = nullptr, *dO = nullptr, *dP = nullptr;
+    TRY(dalloc(s, &dI, m));
+    TRY(dalloc(s, &dO, m));
+    TRY(dalloc(s, &dP, m));
+    Geo g = make_geo(s);
+    if (s->fp32)
+        k_fluxes<float><<<grid_for(s->N), kBlock, 0, s->stream>>>(g, make_flux<float>(s), s->lref, dI, dO, dP, nullptr);
+    else
+        k_fluxes<double><<<grid_for(s->N), kBlock, 0, s->stream>>>(g, make_flux<double>(s), s->lref, dI, dO, dP,
+                                                                  nullptr);
+    HIP_TRY(hipGetLastError());
+    if (incoming) HIP_TRY(hipMemcpyAsync(incoming, dI, m * 8, hipMemcpyDeviceToHost, s->stream));
+    if (outgoing) HIP_TRY(hipMemcpyAsync(outgoing, dO, m * 8, hipMemcpyDeviceToHost, s->stream));
+    if (periodic) HIP_TRY(hipMemcpyAsync(periodic, dP, m * 8, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    dfree(dI);
+    dfree(dO);
+    dfree(dP);
+    s->bytes -= (int64_t)(3 * m * 8);
+    return SSW_OK;
+}
+
+int ssw_get_info(ssw_sweep* s, ssw_info* info) {
+    if (!s || !info) return fail(SSW_ERR_INVALID, "null argument");
+    std::memset(info, 0, sizeof(*info));
+    info->num_cells = s->N;
+    info->num_faces = s->F;
+    info->num_directions = s->D;
+    info->max_depth = s->kmax;
+    info->sweeps_last_step = s->sweeps_last;
+    info->downwind_local = s->dw[0];
+    info->downwind_periodic = s->dw[1];
+    info->downwind_boundary = s->dw[2];
+    info->device_bytes = s->bytes;
+    if (s->have_state) {
+        unsigned long long f = 0;
+        HIP_TRY(hipMemcpyAsync(&f, s->d_fail, 8, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        info->chemistry_failures = (int64_t)f;
+        unsigned long long a = 0;
+        HIP_TRY(hipMemcpyAsync(&a, s->d_active, 8, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        info->updates_last_step = (int64_t)a * s->D;
+    }
+    return SSW_OK;
+}
+
+int ssw_set_timing(ssw_sweep* s, int32_t enabled) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    s->timing = enabled != 0;
+    return SSW_OK;
+}
+
+int ssw_get_timing(ssw_sweep* s, double* avg, double* total, int64_t* launches, double* chem) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    if (avg) *avg = s->n_transport ? s->t_transport_total / (double)s->n_transport : 0.0;
+    if (total) *total = s->t_transport_total;
+    if (launches) *launches = s->n_transport;
+    if (chem) *chem = s->t_chem_total;
+    return SSW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,200 @@
+"""Host-side mirror of the reference's sweep plugin interface over the C ABI.
+
+Reference names and meaning are kept:
+
+* `SweepParameters` - the `sweep` parameter section (src/sweep/parameters.rs:8-46),
+  defaults as in :64-78, plus the cosmological scale factor handed to
+  HydrogenOnly (src/sweep/mod.rs:685-690) and the transport precision.
+* `Sweep` - the device-resident `Sweep<HydrogenOnly>` resource
+  (src/sweep/mod.rs:171-191): built from the grid and the initial per-cell
+  components (init_sweep_system, :634-692); `run_sweeps()` is one
+  `Sweep::run_sweeps` call (:258-272) and returns the elapsed time.
+* `SweepPlugin` / `run_sweep_system` - the ECS step wrapper: it skips its very
+  first call (IsFirstTime, :708-714), advances the simulation time and copies
+  ionised fraction, temperature, timestep, photon rate and ionisation time
+  back (:715-738).
+
+Errors raise `SweepError` with the library's message (the reference panics).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+from .grid import Grid
+
+
+@dataclass
+class SweepParameters:
+    directions: int | list = 84
+    num_timestep_levels: int = 1
+    periodic: bool = False
+    max_timestep: float = 1e6 * 3.15576e7
+    rotate_directions: bool = False
+    significant_rate_threshold: float = 0.0
+    timestep_safety_factor: float = 0.01 * 10.0
+    chemistry_timestep_safety_factor: float = 0.01 * 10.0
+    check_deadlock: bool = False
+    prevent_cooling: bool = True
+    num_tasks_to_solve_before_send_receive: int = 10000
+    # not in the reference's section:
+    scale_factor: float = 1.0
+    transport_precision: int = 64
+
+    def to_c(self):
+        p = N.SweepParametersC()
+        keep = None
+        if isinstance(self.directions, (int, np.integer)):
+            p.num_directions = int(self.directions)
+            p.explicit_directions = None
+        else:
+            keep = np.ascontiguousarray(np.asarray(self.directions, dtype=np.float64).reshape(-1, 3))
+            p.num_directions = keep.shape[0]
+            p.explicit_directions = keep.ctypes.data_as(C.POINTER(C.c_double))
+        if self.rotate_directions:
+            raise N.SweepError(N.SSW_ERR_INVALID, "rotate_directions is not supported")
+        p.num_timestep_levels = int(self.num_timestep_levels)
+        p.periodic = 1 if self.periodic else 0
+        p.max_timestep = float(self.max_timestep)
+        p.significant_rate_threshold = float(self.significant_rate_threshold)
+        p.timestep_safety_factor = float(self.timestep_safety_factor)
+        p.chemistry_timestep_safety_factor = float(self.chemistry_timestep_safety_factor)
+        p.prevent_cooling = 1 if self.prevent_cooling else 0
+        p.transport_precision = int(self.transport_precision)
+        p.scale_factor = float(self.scale_factor)
+        return p, keep
+
+
+@dataclass
+class SweepResults:
+    ionized_hydrogen_fraction: np.ndarray
+    temperature: np.ndarray
+    timestep: np.ndarray
+    photon_rate: np.ndarray
+    level: np.ndarray
+    change_timescale: np.ndarray
+
+
+class Sweep:
+    """Device-resident sweep state of one problem (one HIP stream)."""
+
+    def __init__(self, params: SweepParameters, grid: Grid, density, ionized_fraction, temperature, source):
+        lib = N.sweep_lib()
+        self._lib = lib
+        self.params = params
+        pc, keep = params.to_c()
+        h = C.c_void_p()
+        N.check(lib.ssw_create(C.byref(pc), C.byref(h)))
+        del keep
+        self._h = h
+        self.num_cells = grid.num_cells
+        try:
+            off, nbr, kind, area, normal, vol, size = grid.arrays()
+            N.check(lib.ssw_upload_grid(h, grid.num_cells, N.ptr(off, C.c_int64), N.ptr(nbr, C.c_int32),
+                                        N.ptr(kind, C.c_uint8), N.ptr(area, C.c_double), N.ptr(normal, C.c_double),
+                                        N.ptr(vol, C.c_double), N.ptr(size, C.c_double)))
+            self.upload_state(density, ionized_fraction, temperature, source)
+        except Exception:
+            self.close()
+            raise
+        self.num_directions = self.info().num_directions
+
+    def upload_state(self, density, ionized_fraction, temperature, source):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (density, ionized_fraction, temperature, source)]
+        for a in arrs:
+            if a.shape != (self.num_cells,):
+                raise N.SweepError(N.SSW_ERR_INVALID, "state arrays must have one entry per cell")
+        N.check(self._lib.ssw_upload_state(self._h, *[N.ptr(a, C.c_double) for a in arrs]))
+
+    def run_sweeps(self) -> float:
+        el = C.c_double()
+        N.check(self._lib.ssw_step(self._h, C.byref(el)))
+        return el.value
+
+    def results(self, photon_rate: bool = True) -> SweepResults:
+        n = self.num_cells
+        x, T, ts, pr, ch = (np.empty(n) for _ in range(5))
+        lev = np.empty(n, np.uint8)
+        N.check(self._lib.ssw_download(self._h, N.ptr(x, C.c_double), N.ptr(T, C.c_double), N.ptr(ts, C.c_double),
+                                       N.ptr(pr, C.c_double) if photon_rate else None, N.ptr(lev, C.c_uint8),
+                                       N.ptr(ch, C.c_double)))
+        return SweepResults(x, T, ts, pr if photon_rate else None, lev, ch)
+
+    def counts(self, sweep_level: int) -> np.ndarray:
+        out = np.empty(self.num_directions * self.num_cells, np.uint32)
+        N.check(self._lib.ssw_download_counts(self._h, sweep_level, N.ptr(out, C.c_uint32)))
+        return out.reshape(self.num_directions, self.num_cells)
+
+    def levelsets(self, sweep_level: int) -> np.ndarray:
+        out = np.empty(self.num_directions * self.num_cells, np.uint32)
+        N.check(self._lib.ssw_download_levelsets(self._h, sweep_level, N.ptr(out, C.c_uint32)))
+        return out.reshape(self.num_directions, self.num_cells)
+
+    def fluxes(self):
+        m = self.num_directions * self.num_cells
+        i, o, p = np.empty(m), np.empty(m), np.empty(m)
+        N.check(self._lib.ssw_download_fluxes(self._h, N.ptr(i, C.c_double), N.ptr(o, C.c_double),
+                                              N.ptr(p, C.c_double)))
+        shape = (self.num_directions, self.num_cells)
+        return i.reshape(shape), o.reshape(shape), p.reshape(shape)
+
+    def info(self) -> N.InfoC:
+        inf = N.InfoC()
+        N.check(self._lib.ssw_get_info(self._h, C.byref(inf)))
+        return inf
+
+    def set_timing(self, enabled: bool) -> None:
+        N.check(self._lib.ssw_set_timing(self._h, 1 if enabled else 0))
+
+    def timing(self):
+        avg, tot, chem = C.c_double(), C.c_double(), C.c_double()
+        n = C.c_int64()
+        N.check(self._lib.ssw_get_timing(self._h, C.byref(avg), C.byref(tot), C.byref(n), C.byref(chem)))
+        return {"transport_avg_ms": avg.value, "transport_total_ms": tot.value, "transport_launches": n.value,
+                "chemistry_total_ms": chem.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.ssw_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+@dataclass
+class SweepPlugin:
+    """The plugin's step system over a `Sweep`: run_sweep_system
+    (src/sweep/mod.rs:694-739). The first call only marks IsFirstTime."""
+
+    sweep: Sweep
+    time: float = 0.0
+    is_first_time: bool = True
+    ionization_time: np.ndarray | None = None
+    last: SweepResults | None = field(default=None, repr=False)
+
+    def run_sweep_system(self) -> SweepResults | None:
+        if self.is_first_time:
+            self.is_first_time = False
+            return None
+        self.time += self.sweep.run_sweeps()
+        res = self.sweep.results()
+        if self.ionization_time is None:
+            self.ionization_time = np.zeros(res.ionized_hydrogen_fraction.shape)
+        newly = (res.ionized_hydrogen_fraction > 0.5) & (self.ionization_time == 0.0)
+        self.ionization_time[newly] = self.time
+        self.last = res
+        return res

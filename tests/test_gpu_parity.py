@@ -1,0 +1,144 @@
+"""GPU parity: the HIP sweep (through the C ABI) against the oracle on the
+same grids and inputs.
+
+Bars:
+  * dependency counts, level-set (Kahn) depths and timestep levels: bit-exact;
+  * FP64 transport: per-cell ionised fraction, temperature, photon rate and
+    per-(cell, direction) fluxes within 1e-10 relative (summation order only);
+  * FP32 transport + FP64 chemistry: ionised fraction within 1e-5 relative,
+    the error of every field reported.
+"""
+
+import numpy as np
+import pytest
+
+from subsweep_amd import Sweep, SweepError
+from subsweep_amd import workloads as W
+from oracle.oracle import OracleSweep
+
+pytestmark = pytest.mark.gpu
+
+NONE = np.uint32(0xFFFFFFFF)
+
+
+def _pair(pb):
+    g = Sweep(pb.params, pb.grid, *pb.state())
+    o = OracleSweep(pb.params, pb.grid, *pb.state())
+    return g, o
+
+
+def _rel_ok(a, b, rtol, atol_frac=0.0, name=""):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    scale = np.maximum(np.abs(a), np.abs(b))
+    atol = atol_frac * (np.abs(b).max() if b.size else 0.0)
+    err = np.abs(a - b)
+    bad = err > rtol * scale + atol
+    if bad.any():
+        i = np.argmax(np.where(bad, err / np.maximum(scale, 1e-300), 0))
+        raise AssertionError(f"{name}: {bad.sum()} of {a.size} outside rtol {rtol}; worst at {i}: {a.flat[i]!r} vs {b.flat[i]!r}")
+
+
+def _compare_structure(g, o, levels):
+    for lvl in range(levels):
+        cg, co = g.counts(lvl), o.counts(lvl)
+        assert np.array_equal(cg, co), f"counts differ at sweep level {lvl}"
+        dg, do = g.levelsets(lvl), o.levelsets(lvl)
+        assert np.array_equal(dg, do), f"level sets differ at sweep level {lvl}"
+
+
+# The timescales are ratios dt * safety / |change / value| and dt / |(R - R_prev) / R|:
+# the subtraction amplifies the summation-order differences of R by R / |R - R_prev|,
+# so they are compared at 1e-6 (their conditioning), while the integer levels
+# derived from them must still agree exactly.
+TIMESCALE_RTOL = 1e-6
+
+
+def _compare_fields(g, o, rtol, flux_atol_frac=1e-12):
+    rg, ro = g.results(), o.results()
+    assert np.array_equal(rg.level, ro.level), "timestep levels differ"
+    _rel_ok(rg.ionized_hydrogen_fraction, ro.ionized_hydrogen_fraction, rtol, name="x_HII")
+    _rel_ok(rg.temperature, ro.temperature, rtol, name="T")
+    _rel_ok(rg.timestep, ro.timestep, max(rtol, TIMESCALE_RTOL), name="species timestep")
+    _rel_ok(rg.change_timescale, ro.change_timescale, max(rtol, TIMESCALE_RTOL), name="change timescale")
+    _rel_ok(rg.photon_rate, ro.photon_rate, rtol, flux_atol_frac, name="photon rate")
+    ig, og, pg = g.fluxes()
+    io, oo, po = o.fluxes()
+    _rel_ok(ig, io, rtol, flux_atol_frac, name="incoming")
+    _rel_ok(og, oo, rtol, flux_atol_frac, name="outgoing")
+    _rel_ok(pg, po, rtol, flux_atol_frac, name="periodic")
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("dirs", [16, 84])
+def test_cartesian_fp64(periodic, dirs, hip_available):
+    pb = W.cartesian_problem(8, dirs, levels=1, periodic=periodic, source_cells=(0, 200, 511), steps=3)
+    g, o = _pair(pb)
+    _compare_structure(g, o, 1)
+    for _ in range(pb.steps):
+        eg, eo = g.run_sweeps(), o.run_sweeps()
+        assert eg == eo
+        _compare_fields(g, o, 1e-10)
+
+
+@pytest.mark.parametrize("levels", [1, 4])
+def test_voronoi_fp64_levels(levels, hip_available):
+    pb = W.jittered_problem(12, 1.0, "8", 16, levels=levels, periodic=True, steps=6)
+    pb = pb.with_params(max_timestep=1e-3 * 3.15576e13, timestep_safety_factor=0.1)
+    g, o = _pair(pb)
+    _compare_structure(g, o, levels)
+    for s in range(pb.steps):
+        eg, eo = g.run_sweeps(), o.run_sweeps()
+        assert eg == eo
+        _compare_fields(g, o, 1e-10)
+        _compare_structure(g, o, levels)
+
+
+def test_voronoi_84_nonperiodic(hip_available):
+    pb = W.jittered_problem(10, 1.5, "4", 84, levels=2, periodic=False, threshold=1e-5, steps=3)
+    g, o = _pair(pb)
+    _compare_structure(g, o, 2)
+    for _ in range(pb.steps):
+        assert g.run_sweeps() == o.run_sweeps()
+        _compare_fields(g, o, 1e-10)
+
+
+def test_fp32_transport_error(hip_available):
+    pb = W.jittered_problem(12, 1.0, "8", 16, levels=1, periodic=True, steps=4)
+    g32 = Sweep(pb.params.__class__(**{**pb.params.__dict__, "transport_precision": 32}), pb.grid, *pb.state())
+    o = OracleSweep(pb.params, pb.grid, *pb.state())
+    for _ in range(pb.steps):
+        g32.run_sweeps()
+        o.run_sweeps()
+    r32, ro = g32.results(), o.results()
+    _rel_ok(r32.ionized_hydrogen_fraction, ro.ionized_hydrogen_fraction, 1e-5, name="x_HII fp32")
+    _rel_ok(r32.temperature, ro.temperature, 1e-5, name="T fp32")
+
+
+def test_cycle_is_reported(hip_available):
+    # three cells whose local faces form a cycle for direction (1, 0, 0):
+    # normals are mirror pairs but inconsistent with any ordering.
+    from subsweep_amd.grid import Grid
+    s = np.sqrt(0.5)
+    # cell 0 -> 1 -> 2 -> 0 all point along +x for their forward faces
+    nrm = {(0, 1): (1.0, 0.0, 0.0), (1, 2): (s, s, 0.0), (2, 0): (s, -s, 0.0)}
+    faces = {0: [], 1: [], 2: []}
+    for (a, b), n in nrm.items():
+        faces[a].append((b, n))
+        faces[b].append((a, tuple(-v for v in n)))
+    off = [0]
+    nbr, kind, area, normal = [], [], [], []
+    for c in range(3):
+        for b, n in faces[c]:
+            nbr.append(b)
+            kind.append(0)
+            area.append(1.0)
+            normal.append(n)
+        off.append(len(nbr))
+    grid = Grid(np.array(off, np.int64), np.array(nbr, np.int32), np.array(kind, np.uint8), np.array(area),
+                np.array(normal), np.ones(3), np.ones(3))
+    from subsweep_amd import SweepParameters
+    with pytest.raises(SweepError) as ei:
+        Sweep(SweepParameters(directions=[[1.0, 0.0, 0.0]]), grid, np.ones(3), np.full(3, 1e-10),
+              np.full(3, 1e4), np.zeros(3))
+    assert ei.value.code == -4
