@@ -143,6 +143,15 @@ typedef struct ssw_info {
 
 int ssw_get_info(ssw_sweep* s, ssw_info* info);
 
+/* Transport kernel selection (no counterpart in the reference, which has one
+ * CPU code path). 0: one launch per level set; 1: one persistent launch per
+ * sweep ordered with agent-scope fences; 2 (default): one persistent launch
+ * per sweep with each direction's work kept on one XCD. Sweeps whose schedule
+ * is (re)built use mode 0. A persistent sweep that exceeds its wait bound is
+ * re-run with mode 0; ssw_transport_fallbacks counts those re-runs. */
+int ssw_set_transport_mode(ssw_sweep* s, int32_t mode);
+int64_t ssw_transport_fallbacks(ssw_sweep* s);
+
 /* Kernel timing of the last `ssw_step` when timing is enabled: average duration
  * in ms of each launch of the transport kernel, total ms of the transport
  * kernels, and total ms of the chemistry kernels, from HIP events recorded on
@@ -150,6 +159,19 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info);
 int ssw_set_timing(ssw_sweep* s, int32_t enabled);
 int ssw_get_timing(ssw_sweep* s, double* transport_avg_ms, double* transport_total_ms,
                    int64_t* transport_launches, double* chemistry_total_ms);
+
+/* Host-side helpers, the same code the device kernels run.
+ * ssw_timestep_level: TimestepLevel::from_max_timestep_and_desired_timestep
+ * (src/sweep/timestep_level.rs:27-36), ceil(log2(max/desired)) saturated to
+ * [0, num_levels-1], evaluated exactly from the binary exponent.
+ * ssw_chemistry_timestep: one Solver::perform_timestep of a single cell
+ * (src/chemistry/hydrogen_only/mod.rs:394-441); state = {x_HII, T} in/out,
+ * returns the timescale, *process = 0 temperature / 1 ionisation fraction,
+ * *failed = 1 when the sub-stepping exceeded its depth (timescale dt/10). */
+int32_t ssw_timestep_level(int32_t num_levels, double max_timestep, double desired_timestep);
+double ssw_chemistry_timestep(double* state, double density, double volume, double length, double rate,
+                              double scale_factor, int32_t prevent_cooling, double timestep, double safety,
+                              int32_t* process, int32_t* failed);
 
 /* Releases the handle and its device memory. */
 void ssw_destroy(ssw_sweep* s);

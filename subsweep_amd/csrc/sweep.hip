@@ -36,6 +36,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -321,6 +322,229 @@ __global__ void __launch_bounds__(kBlock) k_sweep_level(Geo g, Flux<R> fl, int l
     }
 }
 
+// ---------------------------------------------------------------------------
+// persistent dataflow transport (replays a built schedule in one launch)
+//
+// The tasks of one direction are queued in Kahn order, so "every task of
+// levels < k is done" is equivalent to "done[d] >= lvloff[d][k]" for a counter
+// done[d] of completed tasks. Work is cut into chunks of kChunk tasks of one
+// (level, direction); each XCD slot owns a set of directions and a list of its
+// chunks in (level, direction) order, handed out by a per-slot ticket counter.
+// A chunk only waits for chunks with smaller tickets of its own list, which are
+// held by workgroups that are already running, so the kernel needs no
+// residency guarantee. Static task data (task record, faces, source,
+// attenuation, periodic snapshot) is loaded before the wait; only the gather of
+// the upwind X values lies on the dependency chain.
+//
+// Memory ordering. kAgent: release = __threadfence() (L2 writeback) before the
+// counter increment, acquire = agent fence (L2 invalidate) after the wait,
+// which orders the accesses for any placement of workgroups. Otherwise all
+// chunks of a direction run on workgroups of one XCD (the slot of the XCC id
+// the workgroup reads from HW_REG_XCC_ID), whose L2 is shared: the producer
+// waits for its stores to reach L2 (s_waitcnt vmcnt(0)) before the increment,
+// and the consumer reads X with agent-scope loads, which bypass the CU's L1.
+//
+// Every wait is bounded; a timed-out wait sets `abort` and all workgroups
+// leave. The host then re-runs the sweep with the level-synchronous kernels,
+// which is safe because solving a task twice with final inputs gives the same
+// result (the pull form is idempotent).
+
+constexpr int kChunk = kBlock / kLanes;    // tasks per chunk (one per 16 lanes)
+constexpr int kLineWords = 32;             // one counter per 128-byte line
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+struct Persist {
+    const uint4* chunks;       // {d, k, start, count}; slot j's list starts at j * cap
+    const uint32_t* nchunks;   // [slot] list lengths
+    uint32_t cap;              // list capacity per slot
+    uint32_t* ticket;          // [slot * kLineWords]
+    uint32_t* done;            // [d * kLineWords]
+    uint32_t* abort;
+    const int* xcc_slot;       // [16] XCC id -> slot or -1
+};
+
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename R>
+__device__ __forceinline__ R ld_agent_r(const R* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int xcc_id() {
+    // HW_REG_XCC_ID (hwreg 20), bits [15:0]; the XCC index is in the low bits
+    return (int)(__builtin_amdgcn_s_getreg((20) | (0 << 6) | (15 << 11)) & 15u);
+}
+
+template <typename R, bool kAgent>
+__global__ void __launch_bounds__(kBlock) k_sweep_persistent(Geo g, Flux<R> fl, Persist p, int lvl) {
+    __shared__ uint4 s_chunk;
+    __shared__ int s_stop;
+    const int tid = threadIdx.x;
+    const int slot = p.xcc_slot[xcc_id()];
+    if (slot < 0) return;
+    const uint32_t cbeg = (uint32_t)slot * p.cap, nch = p.nchunks[slot];
+    const uint32_t sub = tid % kLanes;
+    const uint32_t tslot = tid / kLanes;
+    for (;;) {
+        if (tid == 0) {
+            uint32_t t = atomicAdd(&p.ticket[slot * kLineWords], 1u);
+            s_chunk = (t < nch && ld_agent(p.abort) == 0u) ? p.chunks[cbeg + t] : make_uint4(kNone, 0, 0, 0);
+            s_stop = 0;
+        }
+        __syncthreads();
+        const uint4 ch = s_chunk;
+        if (ch.x == kNone) break;
+        const uint32_t d = ch.x, k = ch.y, cnt = ch.w;
+        const bool valid = tslot < cnt;
+        const uint2* q = g.queue + ((size_t)lvl * g.D + d) * g.N;
+        const size_t dN = (size_t)d * g.N;
+        const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+        // ---- static part: faces of this lane (first two per lane), task data
+        const uint2 rec = valid ? q[ch.z + tslot] : make_uint2(0, 0);
+        const uint32_t c = rec.x & kCellMask;
+        const uint32_t nf = rec.x >> kCellBits;
+        double A = 0.0;
+        uint32_t nb0 = 0, nb1 = 0;
+        R w0 = 0, w1 = 0;
+        {
+            const uint32_t j = sub;
+            if (j < nf) {
+                const double4 a = g.geo[rec.y + j];
+                const uint32_t nk = g.nk[rec.y + j];
+                const double dd = fdot(a, dx, dy, dz);
+                if (dd > 0.0) A += a.w * fl.inv_aref * dd;
+                else if (dd < 0.0 && (nk >> 30) == kKindLocal) {
+                    w0 = (R)(a.w * fl.inv_aref * -dd);
+                    nb0 = nk & kNbrMask;
+                }
+            }
+        }
+        {
+            const uint32_t j = sub + kLanes;
+            if (j < nf) {
+                const double4 a = g.geo[rec.y + j];
+                const uint32_t nk = g.nk[rec.y + j];
+                const double dd = fdot(a, dx, dy, dz);
+                if (dd > 0.0) A += a.w * fl.inv_aref * dd;
+                else if (dd < 0.0 && (nk >> 30) == kKindLocal) {
+                    w1 = (R)(a.w * fl.inv_aref * -dd);
+                    nb1 = nk & kNbrMask;
+                }
+            }
+        }
+        R src = 0, per = 0;
+        double att = 0.0;
+        if (valid && sub == 0) {
+            src = (R)fl.src_over_d[c];
+            att = fl.att[c];
+            const uint32_t ps = g.pslot[c];
+            per = (ps != kNone && fl.P) ? fl.P[(size_t)d * g.Nper + ps] : (R)0;
+        }
+        // ---- wait until every task of lower levels of direction d is done
+        if (tid == 0) {
+            const uint32_t target = g.lvloff[((size_t)lvl * g.D + d) * (g.Kcap + 2) + k];
+            uint32_t spins = 0;
+            while (ld_agent(&p.done[d * kLineWords]) < target) {
+                if (++spins > kSpinLimit || ld_agent(p.abort) != 0u) {
+                    atomicOr(p.abort, 1u);
+                    s_stop = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __syncthreads();
+        if (s_stop) break;
+        if (kAgent) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        // ---- dependent part: gather upwind X
+        R inc = 0;
+        if (w0 != (R)0) inc += w0 * (kAgent ? fl.X[dN + nb0] : ld_agent_r(&fl.X[dN + nb0]));
+        if (w1 != (R)0) inc += w1 * (kAgent ? fl.X[dN + nb1] : ld_agent_r(&fl.X[dN + nb1]));
+        for (uint32_t j = sub + 2 * kLanes; j < nf; j += kLanes) {  // cells with more than 32 faces
+            const double4 a = g.geo[rec.y + j];
+            const uint32_t nk = g.nk[rec.y + j];
+            const double dd = fdot(a, dx, dy, dz);
+            if (dd > 0.0) A += a.w * fl.inv_aref * dd;
+            else if (dd < 0.0 && (nk >> 30) == kKindLocal) {
+                const uint32_t nb = nk & kNbrMask;
+                inc += (R)(a.w * fl.inv_aref * -dd) * (kAgent ? fl.X[dN + nb] : ld_agent_r(&fl.X[dN + nb]));
+            }
+        }
+#pragma unroll
+        for (int m = kLanes / 2; m > 0; m >>= 1) {
+            A += __shfl_xor(A, m, kLanes);
+            inc += __shfl_xor(inc, m, kLanes);
+        }
+        if (valid && sub == 0) {
+            if (inc < (R)0) inc = 0;  // Photons::make_positive
+            R in = (inc + src) + per;  // Site::get_rate
+            R out = (in < (R)fl.thr) ? (R)0 : in * (R)att;
+            fl.X[dN + c] = (A > 0.0) ? out / (R)A : (R)0;
+            fl.I[dN + c] = inc;
+        }
+        // ---- publish
+        if (kAgent) __threadfence();
+        else __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): stores have reached L2
+        __syncthreads();
+        if (tid == 0) atomicAdd(&p.done[d * kLineWords], cnt);
+    }
+}
+
+// Chunk lists of slot `lvl`'s schedule: one block per XCD slot walks the levels
+// in order and emits the chunks of its directions.
+__global__ void k_build_chunks(Geo g, int lvl, int kmax, const int* dir_slot, uint4* chunks, uint32_t* chunk_off,
+                               uint32_t cap_per_slot) {
+    const int slot = blockIdx.x;
+    __shared__ uint32_t s_pieces[128];
+    __shared__ uint32_t s_base;
+    const int d = threadIdx.x;
+    if (threadIdx.x == 0) s_base = 0;
+    __syncthreads();
+    uint4* out = chunks + (size_t)slot * cap_per_slot;
+    const bool mine = d < g.D && dir_slot[d] == slot;
+    const uint32_t* lo = g.lvloff + ((size_t)lvl * g.D + (mine ? d : 0)) * (g.Kcap + 2);
+    for (int k = 0; k < kmax; k++) {
+        uint32_t beg = 0, w = 0;
+        if (mine) {
+            beg = lo[k];
+            w = lo[k + 1] - beg;
+        }
+        uint32_t pieces = (w + kChunk - 1) / kChunk;
+        if (threadIdx.x < 128) s_pieces[threadIdx.x] = pieces;
+        __syncthreads();
+        uint32_t before = 0;
+        for (int e = 0; e < d && e < 128; e++) before += s_pieces[e];
+        uint32_t total = 0;
+        if (threadIdx.x == 0)
+            for (int e = 0; e < 128; e++) total += s_pieces[e];
+        const uint32_t base = s_base;
+        for (uint32_t pc = 0; pc < pieces; pc++) {
+            uint32_t idx = base + before + pc;
+            if (idx < cap_per_slot) {
+                uint32_t st = beg + pc * kChunk;
+                out[idx] = make_uint4((uint32_t)d, (uint32_t)k, st, min((uint32_t)kChunk, beg + w - st));
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_base = base + total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) chunk_off[slot] = min(s_base, cap_per_slot);
+}
+
+// Flags the sweep for re-execution unless every direction completed all its tasks.
+__global__ void k_verify_done(Geo g, Persist p, int lvl, int kmax) {
+    int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= g.D) return;
+    uint32_t total = g.lvloff[((size_t)lvl * g.D + d) * (g.Kcap + 2) + kmax];
+    if (p.done[d * kLineWords] != total) atomicOr(p.abort, 1u);
+}
+
+__global__ void k_probe_xcc(uint32_t* mask) {
+    if (threadIdx.x == 0) atomicOr(mask, 1u << xcc_id());
+}
+
 struct ChemArgs {
     double* rho;
     double* x;
@@ -542,6 +766,19 @@ struct ssw_sweep {
     TimestepState ts;
     int64_t bytes = 0;
     int64_t updates_last = 0, sweeps_last = 0;
+    // persistent transport
+    int mode = 2;                      // 0 level-synchronous, 1 persistent (agent), 2 persistent (XCD-local)
+    int nslots = 1;
+    int* d_xcc_slot = nullptr;         // 16
+    int* d_dir_slot = nullptr;         // D
+    uint4* d_chunks = nullptr;         // L x nslots x cap
+    uint32_t* d_nchunks = nullptr;     // L x nslots
+    uint32_t* d_pctl = nullptr;        // tickets (nslots x 32), done (D x 32), abort (1)
+    uint32_t chunk_cap = 0;
+    int pgrid = 0;
+    int64_t pctl_words = 0;
+    std::vector<int> chunks_ok;        // per level slot: chunk lists match the schedule
+    int64_t fallbacks = 0;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -628,6 +865,68 @@ unsigned sweep_grid_x(ssw_sweep* s) {
     int64_t w = s->max_width + s->max_width / 2 + 1;
     int64_t b = (w + (kBlock / kLanes) - 1) / (kBlock / kLanes);
     return (unsigned)std::min<int64_t>(std::max<int64_t>(b, 1), 4096);
+}
+
+Persist make_persist(ssw_sweep* s, int lvl) {
+    Persist p;
+    p.chunks = s->d_chunks + (size_t)lvl * s->nslots * s->chunk_cap;
+    p.nchunks = s->d_nchunks + (size_t)lvl * s->nslots;
+    p.cap = s->chunk_cap;
+    p.ticket = s->d_pctl;
+    p.done = s->d_pctl + (size_t)s->nslots * kLineWords;
+    p.abort = s->d_pctl + (size_t)(s->nslots + s->D) * kLineWords;
+    p.xcc_slot = s->d_xcc_slot;
+    return p;
+}
+
+// Finds the XCC ids the device dispatches to and gives each one a slot;
+// directions are dealt to the slots round-robin.
+int setup_xcc_slots(ssw_sweep* s) {
+    uint32_t* d_mask = nullptr;
+    TRY(dalloc(s, &d_mask, 1));
+    k_fill_u32<<<1, 1, 0, s->stream>>>(d_mask, 0u, 1);
+    k_probe_xcc<<<4096, 64, 0, s->stream>>>(d_mask);
+    HIP_TRY(hipGetLastError());
+    uint32_t mask = 0;
+    HIP_TRY(hipMemcpyAsync(&mask, d_mask, 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    dfree(d_mask);
+    s->bytes -= 4;
+    int slot_of[16];
+    int n = 0;
+    for (int x = 0; x < 16; x++) slot_of[x] = (mask >> x) & 1u ? n++ : -1;
+    if (n == 0) {  // no id seen: everything in slot 0
+        for (int x = 0; x < 16; x++) slot_of[x] = 0;
+        n = 1;
+    }
+    s->nslots = n;
+    std::vector<int> dir_slot(s->D);
+    for (int d = 0; d < s->D; d++) dir_slot[d] = d % n;
+    TRY(dalloc(s, &s->d_xcc_slot, 16));
+    TRY(dalloc(s, &s->d_dir_slot, s->D));
+    HIP_TRY(hipMemcpy(s->d_xcc_slot, slot_of, sizeof(slot_of), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->d_dir_slot, dir_slot.data(), s->D * sizeof(int), hipMemcpyHostToDevice));
+    s->pctl_words = (int64_t)(n + s->D + 1) * kLineWords;
+    TRY(dalloc(s, &s->d_pctl, (size_t)s->pctl_words));
+    int dev = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persistent<double, false>, kBlock, 0);
+    s->pgrid = std::max(n, std::max(1, cus) * std::max(1, per_cu));
+    return SSW_OK;
+}
+
+// Chunk-list storage for every level slot, sized from the depth bound.
+int alloc_chunks(ssw_sweep* s) {
+    const int L = s->p.num_timestep_levels;
+    const int64_t dirs_per_slot = (s->D + s->nslots - 1) / s->nslots;
+    const int64_t cap = (s->N * dirs_per_slot + kChunk - 1) / kChunk + (int64_t)s->kmax * dirs_per_slot + 16;
+    if (cap >= (int64_t)0xFFFFFFFFll) return fail(SSW_ERR_INVALID, "chunk list too large");
+    s->chunk_cap = (uint32_t)cap;
+    TRY(dalloc(s, &s->d_chunks, (size_t)L * s->nslots * s->chunk_cap));
+    TRY(dalloc(s, &s->d_nchunks, (size_t)L * s->nslots));
+    s->chunks_ok.assign(L, 0);
+    return SSW_OK;
 }
 
 // Builds the full-graph schedule once (all cells active) to find the depth
@@ -741,14 +1040,51 @@ int run_step(ssw_sweep* s, double* elapsed) {
         return e;
     };
     std::vector<std::pair<hipEvent_t, hipEvent_t>> tr, ch;
+    const int L = s->p.num_timestep_levels;
+    // Which schedules get (re)built in this step: those are swept with the
+    // level-synchronous kernels, the others replayed by the persistent kernel.
+    std::vector<uint32_t> need(L, 1u);
+    if (s->mode != 0) {
+        HIP_TRY(hipMemcpyAsync(need.data(), s->d_need, L * 4, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        for (int l = 0; l < L; l++)
+            if (need[l]) s->chunks_ok[l] = 0;
+    }
     for (int lvl : order) {
         k_begin<R><<<gper, kBlock, 0, s->stream>>>(g, flt, lvl);
         k_count<<<gcount, kBlock, 0, s->stream>>>(g, lvl, 0);
-        for (int k = 0; k < s->kmax; k++) {
+        bool done = false;
+        if (s->mode != 0 && !need[lvl] && s->chunks_ok[lvl]) {
+            Persist pp = make_persist(s, lvl);
+            k_fill_u32<<<grid_for(s->pctl_words), kBlock, 0, s->stream>>>(s->d_pctl, 0u, s->pctl_words);
             hipEvent_t a = s->timing ? mark() : nullptr;
-            k_sweep_level<R, true><<<dim3(gx, s->D), kBlock, 0, s->stream>>>(g, flt, lvl, k);
+            if (s->mode == 1)
+                k_sweep_persistent<R, true><<<s->pgrid, kBlock, 0, s->stream>>>(g, flt, pp, lvl);
+            else
+                k_sweep_persistent<R, false><<<s->pgrid, kBlock, 0, s->stream>>>(g, flt, pp, lvl);
             if (s->timing) tr.push_back({a, mark()});
+            k_verify_done<<<1, 128, 0, s->stream>>>(g, pp, lvl, s->kmax);
+            HIP_TRY(hipGetLastError());
+            uint32_t aborted = 0;
+            HIP_TRY(hipMemcpyAsync(&aborted, pp.abort, 4, hipMemcpyDeviceToHost, s->stream));
+            HIP_TRY(hipStreamSynchronize(s->stream));
+            if (aborted) s->fallbacks++;  // re-run this sweep with the level-synchronous kernels
+            else done = true;
         }
+        if (!done) {
+            for (int k = 0; k < s->kmax; k++) {
+                hipEvent_t a = s->timing ? mark() : nullptr;
+                k_sweep_level<R, true><<<dim3(gx, s->D), kBlock, 0, s->stream>>>(g, flt, lvl, k);
+                if (s->timing) tr.push_back({a, mark()});
+            }
+        }
+        if (s->mode != 0 && !s->chunks_ok[lvl]) {
+            k_build_chunks<<<s->nslots, 128, 0, s->stream>>>(g, lvl, s->kmax, s->d_dir_slot,
+                                                             s->d_chunks + (size_t)lvl * s->nslots * s->chunk_cap,
+                                                             s->d_nchunks + (size_t)lvl * s->nslots, s->chunk_cap);
+            s->chunks_ok[lvl] = 1;
+        }
+        need[lvl] = 0;
         hipEvent_t a = s->timing ? mark() : nullptr;
         k_chemistry<R><<<gcells, kBlock, 0, s->stream>>>(g, fl, ca, lvl);
         if (s->timing) ch.push_back({a, mark()});
@@ -806,6 +1142,8 @@ void free_grid(ssw_sweep* s) {
     dfree(s->d_ctr);
     dfree(s->d_cnt);
     dfree(s->d_level);
+    dfree(s->d_chunks);
+    dfree(s->d_nchunks);
 }
 
 }  // namespace
@@ -892,6 +1230,12 @@ int ssw_create(const ssw_sweep_parameters* params, ssw_sweep** out) {
         return fail(SSW_ERR_HIP, "copy of directions failed");
     }
     s->ts.init(p.max_timestep, p.num_timestep_levels);
+    if (const char* m = getenv("SSW_TRANSPORT_MODE")) s->mode = atoi(m);
+    r = setup_xcc_slots(s);
+    if (r != SSW_OK) {
+        ssw_destroy(s);
+        return r;
+    }
     *out = s;
     return SSW_OK;
 }
@@ -905,6 +1249,9 @@ void ssw_destroy(ssw_sweep* s) {
     dfree(s->d_need);
     dfree(s->d_fail);
     dfree(s->d_active);
+    dfree(s->d_xcc_slot);
+    dfree(s->d_dir_slot);
+    dfree(s->d_pctl);
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -1001,6 +1348,7 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
                                       " local faces have no mirror face with the exactly negated normal");
     for (int k = 0; k < 3; k++) s->dw[k] = (double)hb[k + 1] / ((double)n * s->D);
     TRY(build_full_schedule(s));
+    TRY(alloc_chunks(s));
     s->have_grid = true;
     return SSW_OK;
 }
@@ -1058,6 +1406,7 @@ int ssw_upload_state(ssw_sweep* s, const double* density, const double* xhii, co
     HIP_TRY(hipGetLastError());
     TRY(set_need_all(s, 1u));
     HIP_TRY(hipStreamSynchronize(s->stream));
+    s->chunks_ok.assign(L, 0);
     s->ts.init(s->p.max_timestep, L);
     s->have_state = true;
     return SSW_OK;
@@ -1128,6 +1477,7 @@ int ssw_download_levelsets(ssw_sweep* s, int32_t lvl, uint32_t* depth) {
             k_sweep_level<double, false><<<dim3(sweep_grid_x(s), s->D), kBlock, 0, s->stream>>>(g, fl, lvl, k);
         k_fill_u32<<<1, 1, 0, s->stream>>>(s->d_need + lvl, 0u, 1);
         HIP_TRY(hipGetLastError());
+        if ((int)s->chunks_ok.size() > lvl) s->chunks_ok[lvl] = 0;
     }
     uint32_t* d_depth = nullptr;
     TRY(dalloc(s, &d_depth, (size_t)s->D * s->N));
@@ -1191,6 +1541,42 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info) {
     }
     return SSW_OK;
 }
+
+int32_t ssw_timestep_level(int32_t num_levels, double max_timestep, double desired_timestep) {
+    if (num_levels < 1) return 0;
+    return ssw::level_from_ratio(max_timestep / desired_timestep, num_levels);
+}
+
+double ssw_chemistry_timestep(double* state, double density, double volume, double length, double rate,
+                              double scale_factor, int32_t prevent_cooling, double timestep, double safety,
+                              int32_t* process, int32_t* failed) {
+    ssw::Cell c;
+    c.x = state[0];
+    c.T = state[1];
+    c.rho = density;
+    c.vol = volume;
+    c.len = length;
+    c.rate = rate;
+    c.a = scale_factor;
+    ssw::Floor f{prevent_cooling != 0, c.T, c.x};
+    int proc = 0;
+    bool fl = false;
+    double ts = ssw::perform_timestep(c, f, timestep, safety, proc, fl);
+    state[0] = c.x;
+    state[1] = c.T;
+    if (process) *process = proc;
+    if (failed) *failed = fl ? 1 : 0;
+    return ts;
+}
+
+int ssw_set_transport_mode(ssw_sweep* s, int32_t mode) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    if (mode < 0 || mode > 2) return fail(SSW_ERR_INVALID, "mode must be 0, 1 or 2");
+    s->mode = mode;
+    return SSW_OK;
+}
+
+int64_t ssw_transport_fallbacks(ssw_sweep* s) { return s ? s->fallbacks : -1; }
 
 int ssw_set_timing(ssw_sweep* s, int32_t enabled) {
     if (!s) return fail(SSW_ERR_INVALID, "null handle");

@@ -147,6 +147,14 @@ class Sweep:
         N.check(self._lib.ssw_get_info(self._h, C.byref(inf)))
         return inf
 
+    def set_transport_mode(self, mode: int) -> None:
+        """0: one launch per level set; 1: persistent, agent-scope ordering;
+        2: persistent, each direction on one XCD (default)."""
+        N.check(self._lib.ssw_set_transport_mode(self._h, int(mode)))
+
+    def transport_fallbacks(self) -> int:
+        return int(self._lib.ssw_transport_fallbacks(self._h))
+
     def set_timing(self, enabled: bool) -> None:
         N.check(self._lib.ssw_set_timing(self._h, 1 if enabled else 0))
 

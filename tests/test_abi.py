@@ -1,0 +1,54 @@
+"""The C ABI (CPU, no device work): libsubsweep_amd.so loads, exports every
+function include/subsweep_amd.h declares, fills the reference defaults, and
+rejects invalid parameters before touching a device."""
+
+import ctypes as C
+import os
+import re
+
+from subsweep_amd import _native as N
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "subsweep_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ssw_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_functions()
+    assert len(names) >= 18
+    lib = C.CDLL(N.SWEEP_LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) <= set(N.SWEEP_SIGNATURES), set(names) - set(N.SWEEP_SIGNATURES)
+
+
+def test_default_parameters_are_the_reference_defaults():
+    p = N.SweepParametersC()
+    N.sweep_lib().ssw_default_parameters(C.byref(p))
+    # src/sweep/parameters.rs:64-78
+    assert p.significant_rate_threshold == 0.0
+    assert p.timestep_safety_factor == 0.01 * 10.0
+    assert p.chemistry_timestep_safety_factor == 0.01 * 10.0
+    assert p.prevent_cooling == 1
+    assert p.transport_precision == 64
+
+
+def test_invalid_parameters_are_rejected():
+    lib = N.sweep_lib()
+    p = N.SweepParametersC()
+    lib.ssw_default_parameters(C.byref(p))
+    h = C.c_void_p()
+    p.num_timestep_levels = 0
+    assert lib.ssw_create(C.byref(p), C.byref(h)) == N.SSW_ERR_INVALID
+    p.num_timestep_levels = 1
+    p.transport_precision = 16
+    assert lib.ssw_create(C.byref(p), C.byref(h)) == N.SSW_ERR_INVALID
+    p.transport_precision = 64
+    p.num_directions = 17
+    assert lib.ssw_create(C.byref(p), C.byref(h)) == N.SSW_ERR_INVALID
+    assert b"num_directions" in lib.ssw_last_error()
+    assert not h.value
