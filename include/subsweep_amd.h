@@ -113,6 +113,9 @@ int ssw_download(ssw_sweep* s, double* ionized_fraction, double* temperature,
                  double* species_timestep, double* photon_rate, uint8_t* level,
                  double* change_timescale);
 
+/* Waits until all work queued on the handle's stream has finished. */
+int ssw_synchronize(ssw_sweep* s);
+
 /* Parity hook: per-(direction, cell) upwind dependency counts of
  * Sweep::init_counts (src/sweep/mod.rs:346-386) for a sweep at `sweep_level`
  * under the current timestep levels; inactive cells get 0xFFFFFFFF. */
@@ -151,6 +154,13 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info);
  * re-run with mode 0; ssw_transport_fallbacks counts those re-runs. */
 int ssw_set_transport_mode(ssw_sweep* s, int32_t mode);
 int64_t ssw_transport_fallbacks(ssw_sweep* s);
+/* Why the last fallback happened: {recorded, direction, level, awaited task
+ * count, observed task count, XCD slot, ticket, chunks in the slot}. */
+int ssw_transport_last_abort(ssw_sweep* s, uint32_t* out8);
+/* Layout of the persistent work lists of one sweep level (36 ints): number
+ * of XCD slots, workgroups per launch, list capacity, lists valid, the XCC id ->
+ * slot map (16) and the chunks per slot (16, -1 past the last slot). */
+int ssw_transport_layout(ssw_sweep* s, int32_t sweep_level, int32_t* out36);
 
 /* Kernel timing of the last `ssw_step` when timing is enabled: average duration
  * in ms of each launch of the transport kernel, total ms of the transport

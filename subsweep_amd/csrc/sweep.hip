@@ -322,6 +322,84 @@ __global__ void __launch_bounds__(kBlock) k_sweep_level(Geo g, Flux<R> fl, int l
     }
 }
 
+// Per level set k of slot `lvl`: prefix over directions of the level widths,
+// pre[k][d] (d = 0..D), and the level's total task count tot[k] = pre[k][D].
+__global__ void k_level_prefix(Geo g, int lvl, uint32_t* pre, uint32_t* tot) {
+    const int k = blockIdx.x;
+    __shared__ uint32_t w[128];
+    const int d = threadIdx.x;
+    uint32_t v = 0;
+    if (d < g.D) {
+        const uint32_t* lo = g.lvloff + ((size_t)lvl * g.D + d) * (g.Kcap + 2);
+        v = lo[k + 1] - lo[k];
+    }
+    w[d] = v;
+    __syncthreads();
+    if (d == 0) {
+        uint32_t acc = 0;
+        uint32_t* out = pre + (size_t)k * (g.D + 1);
+        for (int e = 0; e < g.D; e++) {
+            out[e] = acc;
+            acc += w[e];
+        }
+        out[g.D] = acc;
+        tot[k] = acc;
+    }
+}
+
+// Replays level set k of a built schedule with exactly one task group per
+// task: flat task i belongs to the direction d with pre[k][d] <= i < pre[k][d+1].
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_sweep_flat(Geo g, Flux<R> fl, const uint32_t* pre_k, int lvl, int k) {
+    __shared__ uint32_t s_pre[129];
+    for (int e = threadIdx.x; e <= g.D; e += blockDim.x) s_pre[e] = pre_k[e];
+    __syncthreads();
+    const uint32_t total = s_pre[g.D];
+    const uint32_t i = blockIdx.x * (kBlock / kLanes) + threadIdx.x / kLanes;
+    const uint32_t sub = threadIdx.x % kLanes;
+    const bool valid = i < total;
+    int d = 0;
+    if (valid) {  // last d with s_pre[d] <= i
+        int lo_d = 0, hi_d = g.D - 1;
+        while (lo_d < hi_d) {
+            int mid = (lo_d + hi_d + 1) >> 1;
+            if (s_pre[mid] <= i) lo_d = mid;
+            else hi_d = mid - 1;
+        }
+        d = lo_d;
+    }
+    const uint32_t* lo = g.lvloff + ((size_t)lvl * g.D + d) * (g.Kcap + 2);
+    const uint2* q = g.queue + ((size_t)lvl * g.D + d) * g.N;
+    const uint2 rec = valid ? q[lo[k] + (i - s_pre[d])] : make_uint2(0, 0);
+    const uint32_t c = rec.x & kCellMask;
+    const uint32_t nf = rec.x >> kCellBits;
+    const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+    const size_t dN = (size_t)d * g.N;
+    double A = 0.0;
+    R inc = 0;
+    for (uint32_t j = sub; j < nf; j += kLanes) {
+        const double4 a = g.geo[rec.y + j];
+        const uint32_t nk = g.nk[rec.y + j];
+        const double dd = fdot(a, dx, dy, dz);
+        if (dd > 0.0) A += a.w * fl.inv_aref * dd;
+        else if (dd < 0.0 && (nk >> 30) == kKindLocal) inc += (R)(a.w * fl.inv_aref * -dd) * fl.X[dN + (nk & kNbrMask)];
+    }
+#pragma unroll
+    for (int m = kLanes / 2; m > 0; m >>= 1) {
+        A += __shfl_xor(A, m, kLanes);
+        inc += __shfl_xor(inc, m, kLanes);
+    }
+    if (valid && sub == 0) {
+        if (inc < (R)0) inc = 0;  // Photons::make_positive
+        const uint32_t ps = g.pslot[c];
+        R per = (ps != kNone && fl.P) ? fl.P[(size_t)d * g.Nper + ps] : (R)0;
+        R in = (inc + (R)fl.src_over_d[c]) + per;  // Site::get_rate
+        R out = (in < (R)fl.thr) ? (R)0 : in * (R)fl.att[c];
+        fl.X[dN + c] = (A > 0.0) ? out / (R)A : (R)0;
+        fl.I[dN + c] = inc;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // persistent dataflow transport (replays a built schedule in one launch)
 //
@@ -351,7 +429,6 @@ __global__ void __launch_bounds__(kBlock) k_sweep_level(Geo g, Flux<R> fl, int l
 
 constexpr int kChunk = kBlock / kLanes;    // tasks per chunk (one per 16 lanes)
 constexpr int kLineWords = 32;             // one counter per 128-byte line
-constexpr uint32_t kSpinLimit = 1u << 22;
 
 struct Persist {
     const uint4* chunks;       // {d, k, start, count}; slot j's list starts at j * cap
@@ -360,6 +437,8 @@ struct Persist {
     uint32_t* ticket;          // [slot * kLineWords]
     uint32_t* done;            // [d * kLineWords]
     uint32_t* abort;
+    uint32_t* diag;            // first abort: {flag, d, k, target, observed, slot, ticket, nch}
+    uint64_t wait_ticks;       // wall-clock bound of one wait
     const int* xcc_slot;       // [16] XCC id -> slot or -1
 };
 
@@ -387,8 +466,9 @@ __global__ void __launch_bounds__(kBlock) k_sweep_persistent(Geo g, Flux<R> fl, 
     const uint32_t sub = tid % kLanes;
     const uint32_t tslot = tid / kLanes;
     for (;;) {
+        uint32_t t = 0;
         if (tid == 0) {
-            uint32_t t = atomicAdd(&p.ticket[slot * kLineWords], 1u);
+            t = atomicAdd(&p.ticket[slot * kLineWords], 1u);
             s_chunk = (t < nch && ld_agent(p.abort) == 0u) ? p.chunks[cbeg + t] : make_uint4(kNone, 0, 0, 0);
             s_stop = 0;
         }
@@ -441,17 +521,43 @@ __global__ void __launch_bounds__(kBlock) k_sweep_persistent(Geo g, Flux<R> fl, 
             const uint32_t ps = g.pslot[c];
             per = (ps != kNone && fl.P) ? fl.P[(size_t)d * g.Nper + ps] : (R)0;
         }
-        // ---- wait until every task of lower levels of direction d is done
+        // ---- wait until every task of lower levels of direction d is done.
+        // Polls are cheap relaxed agent-scope loads; every 8th poll reads the
+        // counter with an atomic read-modify-write (the coherent copy), every
+        // 32nd also checks the abort word and the wall-clock budget.
         if (tid == 0) {
             const uint32_t target = g.lvloff[((size_t)lvl * g.D + d) * (g.Kcap + 2) + k];
-            uint32_t spins = 0;
-            while (ld_agent(&p.done[d * kLineWords]) < target) {
-                if (++spins > kSpinLimit || ld_agent(p.abort) != 0u) {
-                    atomicOr(p.abort, 1u);
-                    s_stop = 1;
-                    break;
+            uint32_t* ctr = &p.done[d * kLineWords];
+            uint32_t seen = ld_agent(ctr);
+            uint32_t polls = 0;
+            uint64_t t0 = 0;
+            int nap = 1;
+            while (seen < target) {
+                polls++;
+                if ((polls & 31u) == 0u) {
+                    const uint64_t now = wall_clock64();
+                    if (t0 == 0) t0 = now;
+                    if (ld_agent(p.abort) != 0u || now - t0 > p.wait_ticks) {
+                        if (atomicOr(p.abort, 1u) == 0u) {  // first to give up: record why
+                            p.diag[1] = d;
+                            p.diag[2] = k;
+                            p.diag[3] = target;
+                            p.diag[4] = seen;
+                            p.diag[5] = (uint32_t)slot;
+                            p.diag[6] = t;
+                            p.diag[7] = nch;
+                            __threadfence();
+                            p.diag[0] = 1u;
+                        }
+                        s_stop = 1;
+                        break;
+                    }
                 }
-                __builtin_amdgcn_s_sleep(1);
+                if (nap < 16) nap <<= 1;
+                for (int z = 0; z < nap; z++) __builtin_amdgcn_s_sleep(2);
+                seen = ((polls & 7u) == 0u)
+                           ? __hip_atomic_fetch_add(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                           : ld_agent(ctr);
             }
         }
         __syncthreads();
@@ -777,8 +883,13 @@ struct ssw_sweep {
     uint32_t chunk_cap = 0;
     int pgrid = 0;
     int64_t pctl_words = 0;
-    std::vector<int> chunks_ok;        // per level slot: chunk lists match the schedule
+    uint64_t wait_ticks = 2000000;     // 20 ms at the usual 100 MHz wall clock
+    std::vector<int> chunks_ok;        // per level slot: replay data (chunk lists, level prefixes) valid
+    uint32_t* d_pre = nullptr;         // L x kmax x (D+1) level prefixes over directions
+    uint32_t* d_tot = nullptr;         // L x kmax level totals
+    std::vector<std::vector<uint32_t>> level_tot;  // host copy of d_tot per slot
     int64_t fallbacks = 0;
+    uint32_t last_abort[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -875,6 +986,8 @@ Persist make_persist(ssw_sweep* s, int lvl) {
     p.ticket = s->d_pctl;
     p.done = s->d_pctl + (size_t)s->nslots * kLineWords;
     p.abort = s->d_pctl + (size_t)(s->nslots + s->D) * kLineWords;
+    p.diag = p.abort + 8;
+    p.wait_ticks = s->wait_ticks;
     p.xcc_slot = s->d_xcc_slot;
     return p;
 }
@@ -912,7 +1025,10 @@ int setup_xcc_slots(ssw_sweep* s) {
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persistent<double, false>, kBlock, 0);
-    s->pgrid = std::max(n, std::max(1, cus) * std::max(1, per_cu));
+    s->pgrid = std::max(n, std::max(1, cus) * std::min(4, std::max(1, per_cu)));
+    int rate_khz = 0;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && rate_khz > 0)
+        s->wait_ticks = (uint64_t)rate_khz * 20;  // 20 ms
     return SSW_OK;
 }
 
@@ -925,7 +1041,27 @@ int alloc_chunks(ssw_sweep* s) {
     s->chunk_cap = (uint32_t)cap;
     TRY(dalloc(s, &s->d_chunks, (size_t)L * s->nslots * s->chunk_cap));
     TRY(dalloc(s, &s->d_nchunks, (size_t)L * s->nslots));
+    TRY(dalloc(s, &s->d_pre, (size_t)L * s->kmax * (s->D + 1)));
+    TRY(dalloc(s, &s->d_tot, (size_t)L * s->kmax));
     s->chunks_ok.assign(L, 0);
+    s->level_tot.assign(L, std::vector<uint32_t>());
+    return SSW_OK;
+}
+
+// After a schedule is built: the persistent kernel's chunk lists and the level
+// prefixes of the exact-grid replay; the level totals are copied to the host.
+int build_replay_data(ssw_sweep* s, const Geo& g, int lvl) {
+    k_build_chunks<<<s->nslots, 128, 0, s->stream>>>(g, lvl, s->kmax, s->d_dir_slot,
+                                                     s->d_chunks + (size_t)lvl * s->nslots * s->chunk_cap,
+                                                     s->d_nchunks + (size_t)lvl * s->nslots, s->chunk_cap);
+    k_level_prefix<<<s->kmax, 128, 0, s->stream>>>(g, lvl, s->d_pre + (size_t)lvl * s->kmax * (s->D + 1),
+                                                   s->d_tot + (size_t)lvl * s->kmax);
+    HIP_TRY(hipGetLastError());
+    s->level_tot[lvl].resize(s->kmax);
+    HIP_TRY(hipMemcpyAsync(s->level_tot[lvl].data(), s->d_tot + (size_t)lvl * s->kmax, s->kmax * 4,
+                           hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->chunks_ok[lvl] = 1;
     return SSW_OK;
 }
 
@@ -1044,12 +1180,10 @@ int run_step(ssw_sweep* s, double* elapsed) {
     // Which schedules get (re)built in this step: those are swept with the
     // level-synchronous kernels, the others replayed by the persistent kernel.
     std::vector<uint32_t> need(L, 1u);
-    if (s->mode != 0) {
-        HIP_TRY(hipMemcpyAsync(need.data(), s->d_need, L * 4, hipMemcpyDeviceToHost, s->stream));
-        HIP_TRY(hipStreamSynchronize(s->stream));
-        for (int l = 0; l < L; l++)
-            if (need[l]) s->chunks_ok[l] = 0;
-    }
+    HIP_TRY(hipMemcpyAsync(need.data(), s->d_need, L * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int l = 0; l < L; l++)
+        if (need[l]) s->chunks_ok[l] = 0;
     for (int lvl : order) {
         k_begin<R><<<gper, kBlock, 0, s->stream>>>(g, flt, lvl);
         k_count<<<gcount, kBlock, 0, s->stream>>>(g, lvl, 0);
@@ -1068,22 +1202,32 @@ int run_step(ssw_sweep* s, double* elapsed) {
             uint32_t aborted = 0;
             HIP_TRY(hipMemcpyAsync(&aborted, pp.abort, 4, hipMemcpyDeviceToHost, s->stream));
             HIP_TRY(hipStreamSynchronize(s->stream));
-            if (aborted) s->fallbacks++;  // re-run this sweep with the level-synchronous kernels
-            else done = true;
+            if (aborted) {  // re-run this sweep with the level-synchronous kernels
+                s->fallbacks++;
+                HIP_TRY(hipMemcpy(s->last_abort, pp.diag, sizeof(s->last_abort), hipMemcpyDeviceToHost));
+            } else {
+                done = true;
+            }
         }
-        if (!done) {
+        if (!done && !need[lvl] && s->chunks_ok[lvl]) {  // replay level by level, exact grids
+            const std::vector<uint32_t>& tot = s->level_tot[lvl];
+            for (int k = 0; k < s->kmax; k++) {
+                if (tot[k] == 0) continue;
+                hipEvent_t a = s->timing ? mark() : nullptr;
+                k_sweep_flat<R><<<(tot[k] + kChunk - 1) / kChunk, kBlock, 0, s->stream>>>(
+                    g, flt, s->d_pre + ((size_t)lvl * s->kmax + k) * (s->D + 1), lvl, k);
+                if (s->timing) tr.push_back({a, mark()});
+            }
+            done = true;
+        }
+        if (!done) {  // build (or first replay): one launch per level for every direction
             for (int k = 0; k < s->kmax; k++) {
                 hipEvent_t a = s->timing ? mark() : nullptr;
                 k_sweep_level<R, true><<<dim3(gx, s->D), kBlock, 0, s->stream>>>(g, flt, lvl, k);
                 if (s->timing) tr.push_back({a, mark()});
             }
         }
-        if (s->mode != 0 && !s->chunks_ok[lvl]) {
-            k_build_chunks<<<s->nslots, 128, 0, s->stream>>>(g, lvl, s->kmax, s->d_dir_slot,
-                                                             s->d_chunks + (size_t)lvl * s->nslots * s->chunk_cap,
-                                                             s->d_nchunks + (size_t)lvl * s->nslots, s->chunk_cap);
-            s->chunks_ok[lvl] = 1;
-        }
+        if (!s->chunks_ok[lvl]) TRY(build_replay_data(s, g, lvl));
         need[lvl] = 0;
         hipEvent_t a = s->timing ? mark() : nullptr;
         k_chemistry<R><<<gcells, kBlock, 0, s->stream>>>(g, fl, ca, lvl);
@@ -1144,6 +1288,8 @@ void free_grid(ssw_sweep* s) {
     dfree(s->d_level);
     dfree(s->d_chunks);
     dfree(s->d_nchunks);
+    dfree(s->d_pre);
+    dfree(s->d_tot);
 }
 
 }  // namespace
@@ -1569,6 +1715,12 @@ double ssw_chemistry_timestep(double* state, double density, double volume, doub
     return ts;
 }
 
+int ssw_synchronize(ssw_sweep* s) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SSW_OK;
+}
+
 int ssw_set_transport_mode(ssw_sweep* s, int32_t mode) {
     if (!s) return fail(SSW_ERR_INVALID, "null handle");
     if (mode < 0 || mode > 2) return fail(SSW_ERR_INVALID, "mode must be 0, 1 or 2");
@@ -1577,6 +1729,31 @@ int ssw_set_transport_mode(ssw_sweep* s, int32_t mode) {
 }
 
 int64_t ssw_transport_fallbacks(ssw_sweep* s) { return s ? s->fallbacks : -1; }
+
+int ssw_transport_layout(ssw_sweep* s, int32_t sweep_level, int32_t* out) {
+    if (!s || !out) return fail(SSW_ERR_INVALID, "null argument");
+    if (sweep_level < 0 || sweep_level >= s->p.num_timestep_levels)
+        return fail(SSW_ERR_INVALID, "sweep_level out of range");
+    out[0] = s->nslots;
+    out[1] = s->pgrid;
+    out[2] = (int32_t)s->chunk_cap;
+    out[3] = s->chunks_ok.empty() ? 0 : s->chunks_ok[sweep_level];
+    int slots[16];
+    HIP_TRY(hipMemcpy(slots, s->d_xcc_slot, sizeof(slots), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 16; i++) out[4 + i] = slots[i];
+    std::vector<uint32_t> nch(s->nslots, 0);
+    if (s->d_nchunks)
+        HIP_TRY(hipMemcpy(nch.data(), s->d_nchunks + (size_t)sweep_level * s->nslots, s->nslots * 4,
+                          hipMemcpyDeviceToHost));
+    for (int i = 0; i < 16; i++) out[20 + i] = i < s->nslots ? (int32_t)nch[i] : -1;
+    return SSW_OK;
+}
+
+int ssw_transport_last_abort(ssw_sweep* s, uint32_t* out8) {
+    if (!s || !out8) return fail(SSW_ERR_INVALID, "null argument");
+    std::memcpy(out8, s->last_abort, sizeof(s->last_abort));
+    return SSW_OK;
+}
 
 int ssw_set_timing(ssw_sweep* s, int32_t enabled) {
     if (!s) return fail(SSW_ERR_INVALID, "null handle");
