@@ -15,8 +15,9 @@ per-update byte model: 8+8 B incoming/periodic reads, 16 B outgoing
 read/write, 16 B per-cell absorbed-rate accumulation, and per downwind face
 36/28/12 B for local/periodic/boundary faces in FP64, 24/16/8 B and 4+4+8 B
 in FP32) times the updates of one launch of the transport kernel, divided by
-its average duration from HIP events recorded on the library's stream over
-the timed steps. cpu_baseline times the oracle (the single-threaded CPU
+its average duration from HIP events recorded on the library's stream around
+every launch, in extra steps run right after the timed steps (the timed steps
+themselves carry no per-launch events). cpu_baseline times the oracle (the single-threaded CPU
 restatement of the reference's sweep) on one step of the same workload.
 """
 
@@ -46,41 +47,48 @@ def bytes_per_update(info, precision: int) -> float:
     return base + info.downwind_local * bl + info.downwind_periodic * bp + info.downwind_boundary * bb
 
 
-def run_gpu(pb, precision: int, steps: int, warmup: int):
+def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3):
     from subsweep_amd import Sweep
     pp = pb.with_params(transport_precision=precision)
     s = Sweep(pp.params, pp.grid, *pp.state())
     for _ in range(warmup):
         s.run_sweeps()
     s.synchronize()
-    s.set_timing(True)
-    kernel_ms, launches, updates = 0.0, 0, 0
+    # timed region: whole steps, no per-launch events
+    updates = 0
     t0 = time.perf_counter()
     for _ in range(steps):
+        s.run_sweeps()
+        updates += s.info().updates_last_step
+    s.synchronize()
+    t1 = time.perf_counter()
+    # kernel timing: the same steps again with HIP events around every launch
+    # of the transport kernel on the library's stream
+    s.set_timing(True)
+    kernel_ms, launches, prof_updates = 0.0, 0, 0
+    for _ in range(profile_steps):
         s.run_sweeps()
         tm = s.timing()
         kernel_ms += tm["transport_total_ms"]
         launches += tm["transport_launches"]
-        updates += s.info().updates_last_step
-    s.synchronize()
-    t1 = time.perf_counter()
+        prof_updates += s.info().updates_last_step
+    s.set_timing(False)
     info = s.info()
-    fallbacks = s.transport_fallbacks()
     s.close()
     elapsed = t1 - t0
     bpu = bytes_per_update(info, precision)
     avg_launch_ms = kernel_ms / max(1, launches)
-    upd_per_launch = updates / max(1, launches)
+    upd_per_launch = prof_updates / max(1, launches)
     achieved = upd_per_launch * bpu / (avg_launch_ms * 1e-3) / 1e9
     return {
         "updates_per_s": updates / elapsed,
         "ms_per_step": elapsed / steps * 1e3,
         "updates_per_step": updates / steps,
         "avg_launch_ms": avg_launch_ms,
-        "launches": launches,
+        "launches_per_step": launches / max(1, profile_steps),
+        "kernel_ms_per_step": kernel_ms / max(1, profile_steps),
         "bytes_per_update": bpu,
         "achieved_gbs": achieved,
-        "fallbacks": fallbacks,
         "info": info,
     }
 
@@ -103,7 +111,21 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB mebibytes (counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--precision", type=int, default=0, choices=(0, 32, 64),
+                    help="run only this transport precision (0: both)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build keeps the whole domain on a single MI355X; "
               "--gpus above 1 is not supported", file=sys.stderr)
@@ -114,12 +136,16 @@ def main():
     pb = W.precision_64()
     log(f"grid: {pb.grid.num_cells} cells, {pb.grid.num_faces} faces, built in {time.perf_counter() - t:.1f} s")
 
-    fp32 = run_gpu(pb, 32, args.steps, args.warmup)
-    log(f"fp32 transport: {fp32['updates_per_s'] / 1e9:.3f} G updates/s, {fp32['ms_per_step']:.2f} ms/step, "
-        f"kernel {fp32['avg_launch_ms']:.3f} ms/launch, {fp32['achieved_gbs']:.0f} GB/s")
-    fp64 = run_gpu(pb, 64, args.steps, args.warmup)
-    log(f"fp64 transport: {fp64['updates_per_s'] / 1e9:.3f} G updates/s, {fp64['ms_per_step']:.2f} ms/step, "
-        f"kernel {fp64['avg_launch_ms']:.3f} ms/launch, {fp64['achieved_gbs']:.0f} GB/s")
+    if args.precision in (0, 32):
+        fp32 = run_gpu(pb, 32, args.steps, args.warmup)
+        log(f"fp32 transport: {fp32['updates_per_s'] / 1e9:.3f} G updates/s, {fp32['ms_per_step']:.2f} ms/step, "
+            f"kernel {fp32['avg_launch_ms']:.3f} ms/launch, {fp32['achieved_gbs']:.0f} GB/s")
+    if args.precision in (0, 64):
+        fp64 = run_gpu(pb, 64, args.steps, args.warmup)
+        log(f"fp64 transport: {fp64['updates_per_s'] / 1e9:.3f} G updates/s, {fp64['ms_per_step']:.2f} ms/step, "
+            f"kernel {fp64['avg_launch_ms']:.3f} ms/launch, {fp64['achieved_gbs']:.0f} GB/s")
+    if args.precision != 0:  # profiling runs: no JSON line
+        return
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "transport_traffic.json")
@@ -169,14 +195,15 @@ def main():
             "unit": "GB/s",
             "frac": fp32["achieved_gbs"] / PEAK_HBM_GBS,
             "traffic": traffic,
-            "kernel": "k_sweep_persistent (one launch per sweep)",
+            "kernel": "k_sweep_rec (one launch per Kahn level set, all directions, graph replay)",
             "avg_launch_ms": fp32["avg_launch_ms"],
+            "launches_per_step": fp32["launches_per_step"],
+            "kernel_ms_per_step": fp32["kernel_ms_per_step"],
             "bytes_per_update": fp32["bytes_per_update"],
             "fp64_achieved": fp64["achieved_gbs"],
             "fp64_bytes_per_update": fp64["bytes_per_update"],
         },
         "cpu_baseline": cpu,
-        "transport_fallbacks": fp32["fallbacks"] + fp64["fallbacks"],
     }
     print(json.dumps(line), flush=True)
 

@@ -147,20 +147,11 @@ typedef struct ssw_info {
 int ssw_get_info(ssw_sweep* s, ssw_info* info);
 
 /* Transport kernel selection (no counterpart in the reference, which has one
- * CPU code path). 0: one launch per level set; 1: one persistent launch per
- * sweep ordered with agent-scope fences; 2 (default): one persistent launch
- * per sweep with each direction's work kept on one XCD. Sweeps whose schedule
- * is (re)built use mode 0. A persistent sweep that exceeds its wait bound is
- * re-run with mode 0; ssw_transport_fallbacks counts those re-runs. */
+ * CPU code path). 0 (the only mode): one launch per Kahn level set, replayed
+ * as a captured graph. Any other value returns SSW_ERR_INVALID: the
+ * persistent one-launch-per-sweep kernel that modes 1 and 2 selected did not
+ * terminate on MI355X and has been removed. */
 int ssw_set_transport_mode(ssw_sweep* s, int32_t mode);
-int64_t ssw_transport_fallbacks(ssw_sweep* s);
-/* Why the last fallback happened: {recorded, direction, level, awaited task
- * count, observed task count, XCD slot, ticket, chunks in the slot}. */
-int ssw_transport_last_abort(ssw_sweep* s, uint32_t* out8);
-/* Layout of the persistent work lists of one sweep level (36 ints): number
- * of XCD slots, workgroups per launch, list capacity, lists valid, the XCC id ->
- * slot map (16) and the chunks per slot (16, -1 past the last slot). */
-int ssw_transport_layout(ssw_sweep* s, int32_t sweep_level, int32_t* out36);
 
 /* Kernel timing of the last `ssw_step` when timing is enabled: average duration
  * in ms of each launch of the transport kernel, total ms of the transport

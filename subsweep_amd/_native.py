@@ -87,9 +87,6 @@ SWEEP_SIGNATURES = {
     "ssw_get_info": (C.c_int, [P, C.POINTER(InfoC)]),
     "ssw_synchronize": (C.c_int, [P]),
     "ssw_set_transport_mode": (C.c_int, [P, C.c_int32]),
-    "ssw_transport_fallbacks": (C.c_int64, [P]),
-    "ssw_transport_last_abort": (C.c_int, [P, PU32]),
-    "ssw_transport_layout": (C.c_int, [P, C.c_int32, PI32]),
     "ssw_set_timing": (C.c_int, [P, C.c_int32]),
     "ssw_get_timing": (C.c_int, [P, PD, PD, PI64, PD]),
     "ssw_timestep_level": (C.c_int32, [C.c_int32, C.c_double, C.c_double]),

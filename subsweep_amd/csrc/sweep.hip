@@ -2,17 +2,18 @@
 //
 // Device layout (one handle = one resident problem):
 //   faces      CSR over cells: off[N+1] (u32), geo[F] = {nx, ny, nz, area} (f64x4),
-//              nk[F] = neighbour | kind << 30 (u32)
+//              nk[F] = neighbour | kind << 30 (u32); for the Nper cells with a
+//              periodic face also a compact list of those faces
 //   cell state rho, x, T, source/D, attenuation, timestep, previous rate,
 //              change timescale (f64[N]); timestep level (u8[N])
 //   fluxes     direction-major [d * N + c]:
 //              X = outgoing rate / downwind effective area (FP64 or scaled FP32)
 //              I = incoming rate at solve time (FP64 or scaled FP32)
-//              Psnap[d * Nper + slot] periodic source snapshot of the Nper cells
-//              that have a periodic face
-//   schedule   per sweep level l (the active set {level >= l}) and direction d:
-//              queue[l][d][0..N) of cells in Kahn order, level-set offsets
-//              lvloff[l][d][0..Kmax+1], three rotating append counters.
+//              P[d * Nper + slot] periodic-source snapshot of the periodic cells
+//   schedule   per sweep level l (the active set {level >= l}): the (cell,
+//              direction) tasks in Kahn order, level-major, as records
+//              {cell | face_count << 25, first_face, direction, 0}; lstart[l][k]
+//              is where level set k begins.
 //
 // The transport is the reference's upwind sweep (src/sweep/mod.rs:412-513)
 // restated in pull form. A cell that is solved distributes the change of its
@@ -25,10 +26,10 @@
 // the order in which tasks run. Tasks are run level set by level set: level k
 // holds the (cell, direction) pairs whose active upwind neighbours are all in
 // levels < k (Kahn layering of init_counts' counts, src/sweep/mod.rs:346-386).
-// The level sets of each active set are built on the device during the first
-// sweep that needs them (per-direction queues fed by ballot-aggregated atomic
-// appends) and replayed by later sweeps until a timestep-level change alters
-// the active set.
+// The level sets of an active set are built on the device by the first sweep
+// that needs them (counter decrements, ballot-aggregated appends) and replayed
+// by later sweeps, as one captured HIP graph of exact-size launches, until a
+// timestep-level change alters the active set.
 
 #include <hip/hip_runtime.h>
 
@@ -36,8 +37,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
-#include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -51,10 +52,12 @@ namespace {
 constexpr int kBlock = 256;
 constexpr uint32_t kKindLocal = 0;
 constexpr uint32_t kKindPeriodic = 1;
+constexpr uint32_t kKindBoundary = 2;
 constexpr uint32_t kNbrMask = 0x3FFFFFFFu;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr int kLanes = 16;             // lanes per task in the transport kernel
-constexpr uint32_t kCellBits = 25;      // task record: cell index bits
+constexpr int kLanes = 16;                 // lanes per task in the transport kernels
+constexpr int kTasksPerBlock = kBlock / kLanes;
+constexpr uint32_t kCellBits = 25;         // task record: cell index bits
 constexpr uint32_t kCellMask = (1u << kCellBits) - 1u;
 
 thread_local std::string g_err;
@@ -72,6 +75,12 @@ int fail(int code, const std::string& msg) {
                         std::string(#expr) + ": " + hipGetErrorString(_e));                     \
     } while (0)
 
+#define TRY(expr)                    \
+    do {                             \
+        int _r = (expr);             \
+        if (_r != SSW_OK) return _r; \
+    } while (0)
+
 // ---------------------------------------------------------------------------
 // device helpers
 
@@ -80,24 +89,11 @@ __device__ __forceinline__ double fdot(const double4& g, double dx, double dy, d
     return (g.x * dx + g.y * dy) + g.z * dz;
 }
 
-// Appends `value` for every lane with `pred` to dst[*counter ...], one atomic per wave.
-__device__ __forceinline__ void wave_append(bool pred, uint32_t value, uint32_t* counter, uint32_t* dst) {
-    unsigned long long m = __ballot(pred);
-    if (m == 0ull) return;
-    unsigned long long act = __ballot(1);
-    int leader = __ffsll((long long)act) - 1;
-    int lane = __lane_id();
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    if (pred) dst[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = value;
-}
-
 struct Geo {
     int64_t N;
     int D;
     int L;
-    int Kcap;                // offsets per (slot, direction) = Kcap + 2
+    int Kcap;                // level starts per slot = Kcap + 2
     const uint32_t* off;
     const double4* geo;
     const uint32_t* nk;
@@ -105,12 +101,14 @@ struct Geo {
     const uint8_t* level;
     const uint32_t* pslot;   // N: periodic slot or kNone
     const uint32_t* pcells;  // Nper
+    const uint32_t* pfoff;   // Nper + 1: periodic faces of each periodic cell
+    const uint32_t* pfidx;   // face indices
     int64_t Nper;
-    uint2* queue;            // L x D x N task records
-    uint32_t* lvloff;        // L x D x (Kcap + 2)
-    uint32_t* ctr;           // L x 3 x D
-    uint32_t* cnt;           // D x N
-    uint32_t* need;          // L
+    uint4* flat;             // L x (D * N) task records, level-major
+    uint32_t* lstart;        // L x (Kcap + 2)
+    uint32_t* ctr;           // L x 3 rotating append counters
+    uint32_t* cnt;           // D x N missing-upwind counters
+    uint32_t* need;          // L: schedule must be (re)built
 };
 
 template <typename R>
@@ -124,6 +122,24 @@ struct Flux {
     double inv_lref;         // storage scale of rates
     double inv_aref;         // storage scale of areas
 };
+
+__device__ __forceinline__ uint4 task_record(const Geo& g, uint32_t c, uint32_t d) {
+    const uint32_t f0 = g.off[c], f1 = g.off[c + 1];
+    return make_uint4(c | ((f1 - f0) << kCellBits), f0, d, 0u);
+}
+
+// Appends `rec` for every lane with `pred` at dst[*counter ...], one atomic per wave.
+__device__ __forceinline__ void wave_append(bool pred, uint4 rec, uint32_t* counter, uint4* dst) {
+    unsigned long long m = __ballot(pred);
+    if (m == 0ull) return;
+    unsigned long long act = __ballot(1);
+    int leader = __ffsll((long long)act) - 1;
+    int lane = __lane_id();
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (pred) dst[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = rec;
+}
 
 // ---------------------------------------------------------------------------
 // grid checks and statistics (once per grid)
@@ -175,16 +191,14 @@ __global__ void k_downwind_stats(Geo g, unsigned long long* out3) {
 // per-sweep kernels
 
 // Resets the append counters of slot `lvl` when its schedule is rebuilt and
-// takes the periodic-source snapshot of the active periodic cells.
+// takes the periodic-source snapshot of the active periodic cells (the
+// transport reads the periodic source as it was when the sweep started).
 template <typename R>
 __global__ void k_begin(Geo g, Flux<R> fl, int lvl) {
     int d = blockIdx.y;
-    const bool build = g.need[lvl] != 0;
-    if (build && blockIdx.x == 0 && threadIdx.x == 0) {
-        uint32_t* ctr = g.ctr + (size_t)lvl * 3 * g.D;
-        ctr[0 * g.D + d] = 0;
-        ctr[1 * g.D + d] = 0;
-        g.lvloff[((size_t)lvl * g.D + d) * (g.Kcap + 2)] = 0;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 3 && g.need[lvl] != 0) {
+        g.ctr[lvl * 3 + threadIdx.x] = 0;
+        if (threadIdx.x == 0) g.lstart[(size_t)lvl * (g.Kcap + 2)] = 0;
     }
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (fl.P == nullptr || s >= g.Nper) return;
@@ -192,9 +206,9 @@ __global__ void k_begin(Geo g, Flux<R> fl, int lvl) {
     if (g.level[c] < lvl) return;
     double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
     R acc = 0;
-    for (uint32_t f = g.off[c]; f < g.off[c + 1]; f++) {
+    for (uint32_t e = g.pfoff[s]; e < g.pfoff[s + 1]; e++) {
+        const uint32_t f = g.pfidx[e];
         uint32_t nk = g.nk[f];
-        if ((nk >> 30) != kKindPeriodic) continue;
         double4 a = g.geo[f];
         double dd = fdot(a, dx, dy, dz);
         if (dd < 0.0) acc += (R)(a.w * fl.inv_aref * -dd) * fl.X[(size_t)d * g.N + (nk & kNbrMask)];
@@ -202,29 +216,9 @@ __global__ void k_begin(Geo g, Flux<R> fl, int lvl) {
     fl.P[(size_t)d * g.Nper + s] = acc;
 }
 
-// A queued task: the cell and its face range, so that solving it needs no
-// lookup of the face offsets. x = cell | face_count << 25, y = first face.
-__device__ __forceinline__ uint2 task_record(const Geo& g, uint32_t c) {
-    uint32_t f0 = g.off[c], f1 = g.off[c + 1];
-    return make_uint2(c | ((f1 - f0) << kCellBits), f0);
-}
-
-// Appends a task record for every lane with `pred`, one atomic per wave.
-__device__ __forceinline__ void wave_append_rec(bool pred, uint2 rec, uint32_t* counter, uint2* dst) {
-    unsigned long long m = __ballot(pred);
-    if (m == 0ull) return;
-    unsigned long long act = __ballot(1);
-    int leader = __ffsll((long long)act) - 1;
-    int lane = __lane_id();
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    if (pred) dst[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = rec;
-}
-
-// init_counts (src/sweep/mod.rs:346-386) + get_initial_tasks (:388-398) for
-// one direction per blockIdx.y: counts active local upwind neighbours and
-// queues the cells with count 0 as level set 0.
+// init_counts (src/sweep/mod.rs:346-386) + get_initial_tasks (:388-398),
+// blockIdx.y = direction: counts active local upwind neighbours and queues
+// the tasks with count 0 as level set 0.
 __global__ void k_count(Geo g, int lvl, int only_counts) {
     const bool build = only_counts || g.need[lvl] != 0;
     if (!build) return;
@@ -243,52 +237,50 @@ __global__ void k_count(Geo g, int lvl, int only_counts) {
     }
     if (c < g.N) g.cnt[(size_t)d * g.N + c] = active ? n : kNone;
     if (only_counts) return;
-    uint2* q = g.queue + ((size_t)lvl * g.D + d) * g.N;
-    uint2 rec = active ? task_record(g, (uint32_t)c) : make_uint2(0, 0);
-    wave_append_rec(active && n == 0, rec, g.ctr + (size_t)lvl * 3 * g.D + d, q);
+    uint4* flat = g.flat + (size_t)lvl * g.D * g.N;
+    uint4 rec = active ? task_record(g, (uint32_t)c, (uint32_t)d) : make_uint4(0, 0, 0, 0);
+    wave_append(active && n == 0, rec, &g.ctr[lvl * 3 + 0], flat);
 }
 
-// Solves level set k of every direction (blockIdx.y = direction). Each task
-// (one cell in one direction) is handled by kLanes lanes, one face per lane,
-// and the per-face sums are combined by lane shuffles. While the schedule of
-// this active set is being built, the kernel also decrements the counters of
-// active local downwind neighbours and queues those that reach zero as level
-// set k+1 (handle_local_neighbour, src/sweep/mod.rs:487-503).
+// Solves level set k while the schedule of slot `lvl` is being built (or
+// replays it when it was built without a graph). kLanes lanes per task, one
+// face each, per-face sums combined by lane shuffles. While building, also
+// decrements the counters of active local downwind neighbours and queues those
+// that reach zero as level set k+1 (handle_local_neighbour, src/sweep/mod.rs:487-503).
 template <typename R, bool kTransport>
-__global__ void __launch_bounds__(kBlock) k_sweep_level(Geo g, Flux<R> fl, int lvl, int k) {
-    const int d = blockIdx.y;
+__global__ void __launch_bounds__(kBlock) k_sweep_build(Geo g, Flux<R> fl, int lvl, int k) {
     const bool build = g.need[lvl] != 0;
-    uint32_t* lo = g.lvloff + ((size_t)lvl * g.D + d) * (g.Kcap + 2);
-    uint32_t* ctr = g.ctr + (size_t)lvl * 3 * g.D;
-    const uint32_t beg = lo[k];
+    uint32_t* ls = g.lstart + (size_t)lvl * (g.Kcap + 2);
+    uint32_t* ctr = g.ctr + lvl * 3;
+    const uint32_t beg = ls[k];
     uint32_t end;
     if (build) {
-        end = beg + ctr[(k % 3) * g.D + d];
+        end = beg + ctr[k % 3];
         if (blockIdx.x == 0 && threadIdx.x == 0) {
-            lo[k + 1] = end;
-            ctr[((k + 2) % 3) * g.D + d] = 0;
+            ls[k + 1] = end;
+            ctr[(k + 2) % 3] = 0;
         }
     } else {
-        end = lo[k + 1];
+        end = ls[k + 1];
     }
     if (beg >= end) return;
-    uint2* q = g.queue + ((size_t)lvl * g.D + d) * g.N;
-    uint32_t* next_ctr = ctr + ((k + 1) % 3) * g.D + d;
-    const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
-    const size_t dN = (size_t)d * g.N;
-    constexpr uint32_t kTasksPerBlock = kBlock / kLanes;
+    uint4* flat = g.flat + (size_t)lvl * g.D * g.N;
+    uint32_t* next_ctr = ctr + (k + 1) % 3;
     const uint32_t sub = threadIdx.x % kLanes;
     const uint32_t stride = gridDim.x * kTasksPerBlock;
     for (uint32_t i0 = beg + blockIdx.x * kTasksPerBlock; i0 < end; i0 += stride) {
         const uint32_t i = i0 + threadIdx.x / kLanes;
         const bool valid = i < end;
-        const uint2 rec = valid ? q[i] : make_uint2(0, 0);
-        const uint32_t c = rec.x & kCellMask;
-        const uint32_t nf = rec.x >> kCellBits;
+        const uint4 r = valid ? flat[i] : make_uint4(0, 0, 0, 0);
+        const uint32_t c = r.x & kCellMask;
+        const uint32_t nf = r.x >> kCellBits;
+        const uint32_t d = r.z;
+        const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+        const size_t dN = (size_t)d * g.N;
         double A = 0.0;
         R inc = 0;
         for (uint32_t j = sub; j < nf; j += kLanes) {
-            const uint32_t f = rec.y + j;
+            const uint32_t f = r.y + j;
             const double4 a = g.geo[f];
             const uint32_t nk = g.nk[f];
             const uint32_t nb = nk & kNbrMask;
@@ -301,7 +293,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep_level(Geo g, Flux<R> fl, int l
             } else if (kTransport && dd < 0.0 && (nk >> 30) == kKindLocal) {
                 inc += (R)(a.w * fl.inv_aref * -dd) * fl.X[dN + nb];
             }
-            if (build) wave_append_rec(ready, ready ? task_record(g, nb) : make_uint2(0, 0), next_ctr, q + end);
+            if (build) wave_append(ready, ready ? task_record(g, nb, d) : make_uint4(0, 0, 0, 0), next_ctr, flat + end);
         }
         if (kTransport) {
 #pragma unroll
@@ -322,64 +314,25 @@ __global__ void __launch_bounds__(kBlock) k_sweep_level(Geo g, Flux<R> fl, int l
     }
 }
 
-// Per level set k of slot `lvl`: prefix over directions of the level widths,
-// pre[k][d] (d = 0..D), and the level's total task count tot[k] = pre[k][D].
-__global__ void k_level_prefix(Geo g, int lvl, uint32_t* pre, uint32_t* tot) {
-    const int k = blockIdx.x;
-    __shared__ uint32_t w[128];
-    const int d = threadIdx.x;
-    uint32_t v = 0;
-    if (d < g.D) {
-        const uint32_t* lo = g.lvloff + ((size_t)lvl * g.D + d) * (g.Kcap + 2);
-        v = lo[k + 1] - lo[k];
-    }
-    w[d] = v;
-    __syncthreads();
-    if (d == 0) {
-        uint32_t acc = 0;
-        uint32_t* out = pre + (size_t)k * (g.D + 1);
-        for (int e = 0; e < g.D; e++) {
-            out[e] = acc;
-            acc += w[e];
-        }
-        out[g.D] = acc;
-        tot[k] = acc;
-    }
-}
-
-// Replays level set k of a built schedule with exactly one task group per
-// task: flat task i belongs to the direction d with pre[k][d] <= i < pre[k][d+1].
+// Replays one level set of a built schedule: `count` tasks of the level-major
+// array, kLanes lanes per task. Three dependent loads per task: the record,
+// the faces, the upwind X values.
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_sweep_flat(Geo g, Flux<R> fl, const uint32_t* pre_k, int lvl, int k) {
-    __shared__ uint32_t s_pre[129];
-    for (int e = threadIdx.x; e <= g.D; e += blockDim.x) s_pre[e] = pre_k[e];
-    __syncthreads();
-    const uint32_t total = s_pre[g.D];
-    const uint32_t i = blockIdx.x * (kBlock / kLanes) + threadIdx.x / kLanes;
+__global__ void __launch_bounds__(kBlock) k_sweep_rec(Geo g, Flux<R> fl, const uint4* recs, uint32_t count) {
+    const uint32_t i = blockIdx.x * kTasksPerBlock + threadIdx.x / kLanes;
     const uint32_t sub = threadIdx.x % kLanes;
-    const bool valid = i < total;
-    int d = 0;
-    if (valid) {  // last d with s_pre[d] <= i
-        int lo_d = 0, hi_d = g.D - 1;
-        while (lo_d < hi_d) {
-            int mid = (lo_d + hi_d + 1) >> 1;
-            if (s_pre[mid] <= i) lo_d = mid;
-            else hi_d = mid - 1;
-        }
-        d = lo_d;
-    }
-    const uint32_t* lo = g.lvloff + ((size_t)lvl * g.D + d) * (g.Kcap + 2);
-    const uint2* q = g.queue + ((size_t)lvl * g.D + d) * g.N;
-    const uint2 rec = valid ? q[lo[k] + (i - s_pre[d])] : make_uint2(0, 0);
-    const uint32_t c = rec.x & kCellMask;
-    const uint32_t nf = rec.x >> kCellBits;
+    const bool valid = i < count;
+    const uint4 r = valid ? recs[i] : make_uint4(0, 0, 0, 0);
+    const uint32_t c = r.x & kCellMask;
+    const uint32_t nf = r.x >> kCellBits;
+    const uint32_t d = r.z;
     const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
     const size_t dN = (size_t)d * g.N;
     double A = 0.0;
     R inc = 0;
     for (uint32_t j = sub; j < nf; j += kLanes) {
-        const double4 a = g.geo[rec.y + j];
-        const uint32_t nk = g.nk[rec.y + j];
+        const double4 a = g.geo[r.y + j];
+        const uint32_t nk = g.nk[r.y + j];
         const double dd = fdot(a, dx, dy, dz);
         if (dd > 0.0) A += a.w * fl.inv_aref * dd;
         else if (dd < 0.0 && (nk >> 30) == kKindLocal) inc += (R)(a.w * fl.inv_aref * -dd) * fl.X[dN + (nk & kNbrMask)];
@@ -400,257 +353,6 @@ __global__ void __launch_bounds__(kBlock) k_sweep_flat(Geo g, Flux<R> fl, const 
     }
 }
 
-// ---------------------------------------------------------------------------
-// persistent dataflow transport (replays a built schedule in one launch)
-//
-// The tasks of one direction are queued in Kahn order, so "every task of
-// levels < k is done" is equivalent to "done[d] >= lvloff[d][k]" for a counter
-// done[d] of completed tasks. Work is cut into chunks of kChunk tasks of one
-// (level, direction); each XCD slot owns a set of directions and a list of its
-// chunks in (level, direction) order, handed out by a per-slot ticket counter.
-// A chunk only waits for chunks with smaller tickets of its own list, which are
-// held by workgroups that are already running, so the kernel needs no
-// residency guarantee. Static task data (task record, faces, source,
-// attenuation, periodic snapshot) is loaded before the wait; only the gather of
-// the upwind X values lies on the dependency chain.
-//
-// Memory ordering. kAgent: release = __threadfence() (L2 writeback) before the
-// counter increment, acquire = agent fence (L2 invalidate) after the wait,
-// which orders the accesses for any placement of workgroups. Otherwise all
-// chunks of a direction run on workgroups of one XCD (the slot of the XCC id
-// the workgroup reads from HW_REG_XCC_ID), whose L2 is shared: the producer
-// waits for its stores to reach L2 (s_waitcnt vmcnt(0)) before the increment,
-// and the consumer reads X with agent-scope loads, which bypass the CU's L1.
-//
-// Every wait is bounded; a timed-out wait sets `abort` and all workgroups
-// leave. The host then re-runs the sweep with the level-synchronous kernels,
-// which is safe because solving a task twice with final inputs gives the same
-// result (the pull form is idempotent).
-
-constexpr int kChunk = kBlock / kLanes;    // tasks per chunk (one per 16 lanes)
-constexpr int kLineWords = 32;             // one counter per 128-byte line
-
-struct Persist {
-    const uint4* chunks;       // {d, k, start, count}; slot j's list starts at j * cap
-    const uint32_t* nchunks;   // [slot] list lengths
-    uint32_t cap;              // list capacity per slot
-    uint32_t* ticket;          // [slot * kLineWords]
-    uint32_t* done;            // [d * kLineWords]
-    uint32_t* abort;
-    uint32_t* diag;            // first abort: {flag, d, k, target, observed, slot, ticket, nch}
-    uint64_t wait_ticks;       // wall-clock bound of one wait
-    const int* xcc_slot;       // [16] XCC id -> slot or -1
-};
-
-__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename R>
-__device__ __forceinline__ R ld_agent_r(const R* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ int xcc_id() {
-    // HW_REG_XCC_ID (hwreg 20), bits [15:0]; the XCC index is in the low bits
-    return (int)(__builtin_amdgcn_s_getreg((20) | (0 << 6) | (15 << 11)) & 15u);
-}
-
-template <typename R, bool kAgent>
-__global__ void __launch_bounds__(kBlock) k_sweep_persistent(Geo g, Flux<R> fl, Persist p, int lvl) {
-    __shared__ uint4 s_chunk;
-    __shared__ int s_stop;
-    const int tid = threadIdx.x;
-    const int slot = p.xcc_slot[xcc_id()];
-    if (slot < 0) return;
-    const uint32_t cbeg = (uint32_t)slot * p.cap, nch = p.nchunks[slot];
-    const uint32_t sub = tid % kLanes;
-    const uint32_t tslot = tid / kLanes;
-    for (;;) {
-        uint32_t t = 0;
-        if (tid == 0) {
-            t = atomicAdd(&p.ticket[slot * kLineWords], 1u);
-            s_chunk = (t < nch && ld_agent(p.abort) == 0u) ? p.chunks[cbeg + t] : make_uint4(kNone, 0, 0, 0);
-            s_stop = 0;
-        }
-        __syncthreads();
-        const uint4 ch = s_chunk;
-        if (ch.x == kNone) break;
-        const uint32_t d = ch.x, k = ch.y, cnt = ch.w;
-        const bool valid = tslot < cnt;
-        const uint2* q = g.queue + ((size_t)lvl * g.D + d) * g.N;
-        const size_t dN = (size_t)d * g.N;
-        const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
-        // ---- static part: faces of this lane (first two per lane), task data
-        const uint2 rec = valid ? q[ch.z + tslot] : make_uint2(0, 0);
-        const uint32_t c = rec.x & kCellMask;
-        const uint32_t nf = rec.x >> kCellBits;
-        double A = 0.0;
-        uint32_t nb0 = 0, nb1 = 0;
-        R w0 = 0, w1 = 0;
-        {
-            const uint32_t j = sub;
-            if (j < nf) {
-                const double4 a = g.geo[rec.y + j];
-                const uint32_t nk = g.nk[rec.y + j];
-                const double dd = fdot(a, dx, dy, dz);
-                if (dd > 0.0) A += a.w * fl.inv_aref * dd;
-                else if (dd < 0.0 && (nk >> 30) == kKindLocal) {
-                    w0 = (R)(a.w * fl.inv_aref * -dd);
-                    nb0 = nk & kNbrMask;
-                }
-            }
-        }
-        {
-            const uint32_t j = sub + kLanes;
-            if (j < nf) {
-                const double4 a = g.geo[rec.y + j];
-                const uint32_t nk = g.nk[rec.y + j];
-                const double dd = fdot(a, dx, dy, dz);
-                if (dd > 0.0) A += a.w * fl.inv_aref * dd;
-                else if (dd < 0.0 && (nk >> 30) == kKindLocal) {
-                    w1 = (R)(a.w * fl.inv_aref * -dd);
-                    nb1 = nk & kNbrMask;
-                }
-            }
-        }
-        R src = 0, per = 0;
-        double att = 0.0;
-        if (valid && sub == 0) {
-            src = (R)fl.src_over_d[c];
-            att = fl.att[c];
-            const uint32_t ps = g.pslot[c];
-            per = (ps != kNone && fl.P) ? fl.P[(size_t)d * g.Nper + ps] : (R)0;
-        }
-        // ---- wait until every task of lower levels of direction d is done.
-        // Polls are cheap relaxed agent-scope loads; every 8th poll reads the
-        // counter with an atomic read-modify-write (the coherent copy), every
-        // 32nd also checks the abort word and the wall-clock budget.
-        if (tid == 0) {
-            const uint32_t target = g.lvloff[((size_t)lvl * g.D + d) * (g.Kcap + 2) + k];
-            uint32_t* ctr = &p.done[d * kLineWords];
-            uint32_t seen = ld_agent(ctr);
-            uint32_t polls = 0;
-            uint64_t t0 = 0;
-            int nap = 1;
-            while (seen < target) {
-                polls++;
-                if ((polls & 31u) == 0u) {
-                    const uint64_t now = wall_clock64();
-                    if (t0 == 0) t0 = now;
-                    if (ld_agent(p.abort) != 0u || now - t0 > p.wait_ticks) {
-                        if (atomicOr(p.abort, 1u) == 0u) {  // first to give up: record why
-                            p.diag[1] = d;
-                            p.diag[2] = k;
-                            p.diag[3] = target;
-                            p.diag[4] = seen;
-                            p.diag[5] = (uint32_t)slot;
-                            p.diag[6] = t;
-                            p.diag[7] = nch;
-                            __threadfence();
-                            p.diag[0] = 1u;
-                        }
-                        s_stop = 1;
-                        break;
-                    }
-                }
-                if (nap < 16) nap <<= 1;
-                for (int z = 0; z < nap; z++) __builtin_amdgcn_s_sleep(2);
-                seen = ((polls & 7u) == 0u)
-                           ? __hip_atomic_fetch_add(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                           : ld_agent(ctr);
-            }
-        }
-        __syncthreads();
-        if (s_stop) break;
-        if (kAgent) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        // ---- dependent part: gather upwind X
-        R inc = 0;
-        if (w0 != (R)0) inc += w0 * (kAgent ? fl.X[dN + nb0] : ld_agent_r(&fl.X[dN + nb0]));
-        if (w1 != (R)0) inc += w1 * (kAgent ? fl.X[dN + nb1] : ld_agent_r(&fl.X[dN + nb1]));
-        for (uint32_t j = sub + 2 * kLanes; j < nf; j += kLanes) {  // cells with more than 32 faces
-            const double4 a = g.geo[rec.y + j];
-            const uint32_t nk = g.nk[rec.y + j];
-            const double dd = fdot(a, dx, dy, dz);
-            if (dd > 0.0) A += a.w * fl.inv_aref * dd;
-            else if (dd < 0.0 && (nk >> 30) == kKindLocal) {
-                const uint32_t nb = nk & kNbrMask;
-                inc += (R)(a.w * fl.inv_aref * -dd) * (kAgent ? fl.X[dN + nb] : ld_agent_r(&fl.X[dN + nb]));
-            }
-        }
-#pragma unroll
-        for (int m = kLanes / 2; m > 0; m >>= 1) {
-            A += __shfl_xor(A, m, kLanes);
-            inc += __shfl_xor(inc, m, kLanes);
-        }
-        if (valid && sub == 0) {
-            if (inc < (R)0) inc = 0;  // Photons::make_positive
-            R in = (inc + src) + per;  // Site::get_rate
-            R out = (in < (R)fl.thr) ? (R)0 : in * (R)att;
-            fl.X[dN + c] = (A > 0.0) ? out / (R)A : (R)0;
-            fl.I[dN + c] = inc;
-        }
-        // ---- publish
-        if (kAgent) __threadfence();
-        else __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): stores have reached L2
-        __syncthreads();
-        if (tid == 0) atomicAdd(&p.done[d * kLineWords], cnt);
-    }
-}
-
-// Chunk lists of slot `lvl`'s schedule: one block per XCD slot walks the levels
-// in order and emits the chunks of its directions.
-__global__ void k_build_chunks(Geo g, int lvl, int kmax, const int* dir_slot, uint4* chunks, uint32_t* chunk_off,
-                               uint32_t cap_per_slot) {
-    const int slot = blockIdx.x;
-    __shared__ uint32_t s_pieces[128];
-    __shared__ uint32_t s_base;
-    const int d = threadIdx.x;
-    if (threadIdx.x == 0) s_base = 0;
-    __syncthreads();
-    uint4* out = chunks + (size_t)slot * cap_per_slot;
-    const bool mine = d < g.D && dir_slot[d] == slot;
-    const uint32_t* lo = g.lvloff + ((size_t)lvl * g.D + (mine ? d : 0)) * (g.Kcap + 2);
-    for (int k = 0; k < kmax; k++) {
-        uint32_t beg = 0, w = 0;
-        if (mine) {
-            beg = lo[k];
-            w = lo[k + 1] - beg;
-        }
-        uint32_t pieces = (w + kChunk - 1) / kChunk;
-        if (threadIdx.x < 128) s_pieces[threadIdx.x] = pieces;
-        __syncthreads();
-        uint32_t before = 0;
-        for (int e = 0; e < d && e < 128; e++) before += s_pieces[e];
-        uint32_t total = 0;
-        if (threadIdx.x == 0)
-            for (int e = 0; e < 128; e++) total += s_pieces[e];
-        const uint32_t base = s_base;
-        for (uint32_t pc = 0; pc < pieces; pc++) {
-            uint32_t idx = base + before + pc;
-            if (idx < cap_per_slot) {
-                uint32_t st = beg + pc * kChunk;
-                out[idx] = make_uint4((uint32_t)d, (uint32_t)k, st, min((uint32_t)kChunk, beg + w - st));
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) s_base = base + total;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) chunk_off[slot] = min(s_base, cap_per_slot);
-}
-
-// Flags the sweep for re-execution unless every direction completed all its tasks.
-__global__ void k_verify_done(Geo g, Persist p, int lvl, int kmax) {
-    int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= g.D) return;
-    uint32_t total = g.lvloff[((size_t)lvl * g.D + d) * (g.Kcap + 2) + kmax];
-    if (p.done[d * kLineWords] != total) atomicOr(p.abort, 1u);
-}
-
-__global__ void k_probe_xcc(uint32_t* mask) {
-    if (threadIdx.x == 0) atomicOr(mask, 1u << xcc_id());
-}
-
 struct ChemArgs {
     double* rho;
     double* x;
@@ -668,7 +370,6 @@ struct ChemArgs {
     double scale_factor;
     int prevent_cooling;
     double lref;             // storage scale of I (rate = I * lref)
-    double aref;
     unsigned long long* failures;
     unsigned long long* active;
 };
@@ -683,11 +384,13 @@ __global__ void k_chemistry(Geo g, Flux<R> fl, ChemArgs ca, int lvl) {
     int lev = g.level[c];
     {
         unsigned long long m = __ballot(lev >= lvl);
-        if (__lane_id() == __ffsll((long long)__ballot(1)) - 1 && m) atomicAdd(ca.active, (unsigned long long)__popcll(m));
+        if (__lane_id() == __ffsll((long long)__ballot(1)) - 1 && m)
+            atomicAdd(ca.active, (unsigned long long)__popcll(m));
     }
     if (lev < lvl) return;
     const double dt = ca.max_ts * ssw::powi(0.5, lev);
-    const bool has_p = g.pslot[c] != kNone && fl.P;
+    const uint32_t ps = g.pslot[c];
+    const bool has_p = ps != kNone && fl.P;
     double rate = 0.0;
     for (int d = 0; d < g.D; d++) {
         double inc = (double)fl.I[(size_t)d * g.N + c] * ca.lref;
@@ -695,16 +398,16 @@ __global__ void k_chemistry(Geo g, Flux<R> fl, ChemArgs ca, int lvl) {
         if (has_p) {  // current periodic source (the sweep has finished)
             double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
             R acc = 0;
-            for (uint32_t f = g.off[c]; f < g.off[c + 1]; f++) {
+            for (uint32_t e = g.pfoff[ps]; e < g.pfoff[ps + 1]; e++) {
+                const uint32_t f = g.pfidx[e];
                 uint32_t nk = g.nk[f];
-                if ((nk >> 30) != kKindPeriodic) continue;
                 double4 a = g.geo[f];
                 double dd = fdot(a, dx, dy, dz);
                 if (dd < 0.0) acc += (R)(a.w * fl.inv_aref * -dd) * fl.X[(size_t)d * g.N + (nk & kNbrMask)];
             }
             per = (double)acc * ca.lref;
         }
-        rate += (inc + ca.src[c]) + per;
+        rate += (inc + ca.src[c]) + per;  // sum over d of Site::get_rate, in direction order
     }
     double rel;
     if (fabs(rate) < fabs(ca.thr)) {
@@ -767,7 +470,7 @@ __global__ void k_fluxes(Geo g, Flux<R> fl, double lref, double* I_out, double* 
             double dd = fdot(a, dx, dy, dz);
             if (dd > 0.0) {
                 A += a.w * dd;
-            } else if (dd < 0.0 && (nk >> 30) != 2u) {
+            } else if (dd < 0.0 && (nk >> 30) != kKindBoundary) {
                 R v = (R)(a.w * fl.inv_aref * -dd) * fl.X[(size_t)d * g.N + (nk & kNbrMask)];
                 if ((nk >> 30) == kKindLocal) inc += v;
                 else per += v;
@@ -783,15 +486,21 @@ __global__ void k_fluxes(Geo g, Flux<R> fl, double lref, double* I_out, double* 
     if (photon_rate) photon_rate[c] = total;
 }
 
-// Kahn depth of every queued task of slot `lvl`.
+// Kahn depth of every task of slot `lvl`: entry i of the level-major array is
+// in level set k with lstart[k] <= i < lstart[k+1].
 __global__ void k_scatter_depth(Geo g, int lvl, int kmax, uint32_t* depth) {
-    int d = blockIdx.y;
-    const uint32_t* lo = g.lvloff + ((size_t)lvl * g.D + d) * (g.Kcap + 2);
-    const uint2* q = g.queue + ((size_t)lvl * g.D + d) * g.N;
-    for (int k = 0; k < kmax; k++) {
-        for (uint32_t i = lo[k] + blockIdx.x * blockDim.x + threadIdx.x; i < lo[k + 1];
-             i += gridDim.x * blockDim.x)
-            depth[(size_t)d * g.N + (q[i].x & kCellMask)] = (uint32_t)k;
+    const uint32_t* ls = g.lstart + (size_t)lvl * (g.Kcap + 2);
+    const uint4* flat = g.flat + (size_t)lvl * g.D * g.N;
+    const uint32_t total = ls[kmax];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        int a = 0, b = kmax - 1;  // last k with ls[k] <= i
+        while (a < b) {
+            int m = (a + b + 1) >> 1;
+            if (ls[m] <= i) a = m;
+            else b = m - 1;
+        }
+        const uint4 r = flat[i];
+        depth[(size_t)r.z * g.N + (r.x & kCellMask)] = (uint32_t)a;
     }
 }
 
@@ -820,7 +529,7 @@ struct TimestepState {
         lowest = n - 1;
         first_done = false;
     }
-    std::vector<int> sweep_order() const {
+    std::vector<int> sweep_order() const {  // :22-27, 70-75
         int num = L - lowest;
         std::vector<int> out;
         for (uint32_t i = 0; i < (1u << (num - 1)); i++) {
@@ -829,7 +538,7 @@ struct TimestepState {
         }
         return out;
     }
-    void advance() {
+    void advance() {  // :37-48
         if (first_done && lowest > 0) lowest -= 1;
         if (!first_done) first_done = true;
     }
@@ -851,6 +560,7 @@ struct ssw_sweep {
     int64_t N = 0, F = 0, Nper = 0;
     bool have_grid = false, have_state = false;
     uint32_t *d_off = nullptr, *d_nk = nullptr, *d_pslot = nullptr, *d_pcells = nullptr;
+    uint32_t *d_pfoff = nullptr, *d_pfidx = nullptr;
     double4* d_geo = nullptr;
     double *d_dirs = nullptr, *d_vol = nullptr, *d_size = nullptr;
     double aref = 1.0;
@@ -858,9 +568,12 @@ struct ssw_sweep {
     int64_t max_width = 0;
     double dw[3] = {0, 0, 0};
     // schedule
-    uint2* d_queue = nullptr;
-    uint32_t *d_lvloff = nullptr, *d_ctr = nullptr, *d_cnt = nullptr, *d_need = nullptr;
+    uint4* d_flat = nullptr;
+    uint32_t *d_lstart = nullptr, *d_ctr = nullptr, *d_cnt = nullptr, *d_need = nullptr;
     int kcap = 0;
+    std::vector<int> replay_ok;                     // per slot: host copy of the level starts and graph valid
+    std::vector<std::vector<uint32_t>> level_start; // per slot: lstart[0..kmax]
+    std::vector<hipGraphExec_t> graphs;             // per slot: replay of its level sets
     // state
     double *d_rho = nullptr, *d_x = nullptr, *d_T = nullptr, *d_src = nullptr, *d_att = nullptr;
     double *d_sts = nullptr, *d_prev = nullptr, *d_change = nullptr, *d_src_t = nullptr;
@@ -871,25 +584,7 @@ struct ssw_sweep {
     double lref = 1.0;
     TimestepState ts;
     int64_t bytes = 0;
-    int64_t updates_last = 0, sweeps_last = 0;
-    // persistent transport
-    int mode = 2;                      // 0 level-synchronous, 1 persistent (agent), 2 persistent (XCD-local)
-    int nslots = 1;
-    int* d_xcc_slot = nullptr;         // 16
-    int* d_dir_slot = nullptr;         // D
-    uint4* d_chunks = nullptr;         // L x nslots x cap
-    uint32_t* d_nchunks = nullptr;     // L x nslots
-    uint32_t* d_pctl = nullptr;        // tickets (nslots x 32), done (D x 32), abort (1)
-    uint32_t chunk_cap = 0;
-    int pgrid = 0;
-    int64_t pctl_words = 0;
-    uint64_t wait_ticks = 2000000;     // 20 ms at the usual 100 MHz wall clock
-    std::vector<int> chunks_ok;        // per level slot: replay data (chunk lists, level prefixes) valid
-    uint32_t* d_pre = nullptr;         // L x kmax x (D+1) level prefixes over directions
-    uint32_t* d_tot = nullptr;         // L x kmax level totals
-    std::vector<std::vector<uint32_t>> level_tot;  // host copy of d_tot per slot
-    int64_t fallbacks = 0;
-    uint32_t last_abort[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t sweeps_last = 0;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -922,12 +617,6 @@ void dfree(T*& p) {
     p = nullptr;
 }
 
-#define TRY(expr)              \
-    do {                       \
-        int _r = (expr);       \
-        if (_r != SSW_OK) return _r; \
-    } while (0)
-
 Geo make_geo(ssw_sweep* s) {
     Geo g;
     g.N = s->N;
@@ -941,9 +630,11 @@ Geo make_geo(ssw_sweep* s) {
     g.level = s->d_level;
     g.pslot = s->d_pslot;
     g.pcells = s->d_pcells;
+    g.pfoff = s->d_pfoff;
+    g.pfidx = s->d_pfidx;
     g.Nper = s->Nper;
-    g.queue = s->d_queue;
-    g.lvloff = s->d_lvloff;
+    g.flat = s->d_flat;
+    g.lstart = s->d_lstart;
     g.ctr = s->d_ctr;
     g.cnt = s->d_cnt;
     g.need = s->d_need;
@@ -970,163 +661,131 @@ int set_need_all(ssw_sweep* s, uint32_t v) {
     return SSW_OK;
 }
 
-unsigned sweep_grid_x(ssw_sweep* s) {
-    // enough blocks for the widest level set of one direction in the full graph
-    // (plus headroom for restricted graphs); a grid-stride loop covers the rest
+void invalidate_replays(ssw_sweep* s) {
+    for (hipGraphExec_t& ge : s->graphs)
+        if (ge) {
+            (void)hipGraphExecDestroy(ge);
+            ge = nullptr;
+        }
+    std::fill(s->replay_ok.begin(), s->replay_ok.end(), 0);
+}
+
+unsigned build_grid_x(ssw_sweep* s) {
+    // enough blocks for the widest level set of the full graph (all directions),
+    // with headroom for restricted graphs; a grid-stride loop covers the rest
     int64_t w = s->max_width + s->max_width / 2 + 1;
-    int64_t b = (w + (kBlock / kLanes) - 1) / (kBlock / kLanes);
-    return (unsigned)std::min<int64_t>(std::max<int64_t>(b, 1), 4096);
+    int64_t b = (w + kTasksPerBlock - 1) / kTasksPerBlock;
+    return (unsigned)std::min<int64_t>(std::max<int64_t>(b, 1), 8192);
 }
 
-Persist make_persist(ssw_sweep* s, int lvl) {
-    Persist p;
-    p.chunks = s->d_chunks + (size_t)lvl * s->nslots * s->chunk_cap;
-    p.nchunks = s->d_nchunks + (size_t)lvl * s->nslots;
-    p.cap = s->chunk_cap;
-    p.ticket = s->d_pctl;
-    p.done = s->d_pctl + (size_t)s->nslots * kLineWords;
-    p.abort = s->d_pctl + (size_t)(s->nslots + s->D) * kLineWords;
-    p.diag = p.abort + 8;
-    p.wait_ticks = s->wait_ticks;
-    p.xcc_slot = s->d_xcc_slot;
-    return p;
-}
-
-// Finds the XCC ids the device dispatches to and gives each one a slot;
-// directions are dealt to the slots round-robin.
-int setup_xcc_slots(ssw_sweep* s) {
-    uint32_t* d_mask = nullptr;
-    TRY(dalloc(s, &d_mask, 1));
-    k_fill_u32<<<1, 1, 0, s->stream>>>(d_mask, 0u, 1);
-    k_probe_xcc<<<4096, 64, 0, s->stream>>>(d_mask);
-    HIP_TRY(hipGetLastError());
-    uint32_t mask = 0;
-    HIP_TRY(hipMemcpyAsync(&mask, d_mask, 4, hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    dfree(d_mask);
-    s->bytes -= 4;
-    int slot_of[16];
-    int n = 0;
-    for (int x = 0; x < 16; x++) slot_of[x] = (mask >> x) & 1u ? n++ : -1;
-    if (n == 0) {  // no id seen: everything in slot 0
-        for (int x = 0; x < 16; x++) slot_of[x] = 0;
-        n = 1;
-    }
-    s->nslots = n;
-    std::vector<int> dir_slot(s->D);
-    for (int d = 0; d < s->D; d++) dir_slot[d] = d % n;
-    TRY(dalloc(s, &s->d_xcc_slot, 16));
-    TRY(dalloc(s, &s->d_dir_slot, s->D));
-    HIP_TRY(hipMemcpy(s->d_xcc_slot, slot_of, sizeof(slot_of), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(s->d_dir_slot, dir_slot.data(), s->D * sizeof(int), hipMemcpyHostToDevice));
-    s->pctl_words = (int64_t)(n + s->D + 1) * kLineWords;
-    TRY(dalloc(s, &s->d_pctl, (size_t)s->pctl_words));
-    int dev = 0, cus = 0, per_cu = 0;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persistent<double, false>, kBlock, 0);
-    s->pgrid = std::max(n, std::max(1, cus) * std::min(4, std::max(1, per_cu)));
-    int rate_khz = 0;
-    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && rate_khz > 0)
-        s->wait_ticks = (uint64_t)rate_khz * 20;  // 20 ms
-    return SSW_OK;
-}
-
-// Chunk-list storage for every level slot, sized from the depth bound.
-int alloc_chunks(ssw_sweep* s) {
-    const int L = s->p.num_timestep_levels;
-    const int64_t dirs_per_slot = (s->D + s->nslots - 1) / s->nslots;
-    const int64_t cap = (s->N * dirs_per_slot + kChunk - 1) / kChunk + (int64_t)s->kmax * dirs_per_slot + 16;
-    if (cap >= (int64_t)0xFFFFFFFFll) return fail(SSW_ERR_INVALID, "chunk list too large");
-    s->chunk_cap = (uint32_t)cap;
-    TRY(dalloc(s, &s->d_chunks, (size_t)L * s->nslots * s->chunk_cap));
-    TRY(dalloc(s, &s->d_nchunks, (size_t)L * s->nslots));
-    TRY(dalloc(s, &s->d_pre, (size_t)L * s->kmax * (s->D + 1)));
-    TRY(dalloc(s, &s->d_tot, (size_t)L * s->kmax));
-    s->chunks_ok.assign(L, 0);
-    s->level_tot.assign(L, std::vector<uint32_t>());
-    return SSW_OK;
-}
-
-// After a schedule is built: the persistent kernel's chunk lists and the level
-// prefixes of the exact-grid replay; the level totals are copied to the host.
-int build_replay_data(ssw_sweep* s, const Geo& g, int lvl) {
-    k_build_chunks<<<s->nslots, 128, 0, s->stream>>>(g, lvl, s->kmax, s->d_dir_slot,
-                                                     s->d_chunks + (size_t)lvl * s->nslots * s->chunk_cap,
-                                                     s->d_nchunks + (size_t)lvl * s->nslots, s->chunk_cap);
-    k_level_prefix<<<s->kmax, 128, 0, s->stream>>>(g, lvl, s->d_pre + (size_t)lvl * s->kmax * (s->D + 1),
-                                                   s->d_tot + (size_t)lvl * s->kmax);
-    HIP_TRY(hipGetLastError());
-    s->level_tot[lvl].resize(s->kmax);
-    HIP_TRY(hipMemcpyAsync(s->level_tot[lvl].data(), s->d_tot + (size_t)lvl * s->kmax, s->kmax * 4,
-                           hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    s->chunks_ok[lvl] = 1;
-    return SSW_OK;
-}
-
-// Builds the full-graph schedule once (all cells active) to find the depth
-// bound and detect cycles. Grows the level-offset arrays as needed.
+// Builds the schedule of the full graph (all cells active) once per grid: the
+// depth bound kmax and the widest level set; a graph that never empties its
+// queue has a cycle. Grows the level-start array of slot 0 as needed.
 int build_full_schedule(ssw_sweep* s) {
     const int L = s->p.num_timestep_levels;
     s->max_width = 0;
     int kcap = 256;
     s->kcap = kcap;
-    TRY(dalloc(s, &s->d_lvloff, (size_t)L * s->D * (kcap + 2)));
+    TRY(dalloc(s, &s->d_lstart, (size_t)L * (kcap + 2)));
     TRY(set_need_all(s, 1u));
     Geo g = make_geo(s);
     Flux<double> fl{};
-    dim3 gb(grid_for(s->N), s->D);
-    k_begin<double><<<dim3(1, s->D), kBlock, 0, s->stream>>>(g, fl, 0);
-    k_count<<<gb, kBlock, 0, s->stream>>>(g, 0, 0);
+    k_begin<double><<<dim3(1, 1), kBlock, 0, s->stream>>>(g, fl, 0);
+    k_count<<<dim3(grid_for(s->N), s->D), kBlock, 0, s->stream>>>(g, 0, 0);
     HIP_TRY(hipGetLastError());
-    std::vector<uint32_t> hctr(s->D);
     int64_t total = 0;
     int k = 0;
-    unsigned gx = grid_for(s->N);
     for (;; k++) {
-        if (k + 2 > kcap) {  // grow the offsets of slot 0
+        if (k + 2 > kcap) {  // grow the level starts of slot 0
             int ncap = kcap * 2;
             uint32_t* nl = nullptr;
-            TRY(dalloc(s, &nl, (size_t)L * s->D * (ncap + 2)));
-            std::vector<uint32_t> host((size_t)s->D * (kcap + 2));
-            HIP_TRY(hipMemcpyAsync(host.data(), s->d_lvloff, host.size() * 4, hipMemcpyDeviceToHost, s->stream));
+            TRY(dalloc(s, &nl, (size_t)L * (ncap + 2)));
+            HIP_TRY(hipMemcpyAsync(nl, s->d_lstart, (kcap + 2) * 4, hipMemcpyDeviceToDevice, s->stream));
             HIP_TRY(hipStreamSynchronize(s->stream));
-            std::vector<uint32_t> grown((size_t)s->D * (ncap + 2), 0);
-            for (int d = 0; d < s->D; d++)
-                std::memcpy(&grown[(size_t)d * (ncap + 2)], &host[(size_t)d * (kcap + 2)], (kcap + 2) * 4);
-            HIP_TRY(hipMemcpyAsync(nl, grown.data(), grown.size() * 4, hipMemcpyHostToDevice, s->stream));
-            HIP_TRY(hipStreamSynchronize(s->stream));
-            dfree(s->d_lvloff);
-            s->d_lvloff = nl;
+            dfree(s->d_lstart);
+            s->d_lstart = nl;
             kcap = ncap;
             s->kcap = kcap;
             g = make_geo(s);
         }
-        // size of level k
-        HIP_TRY(hipMemcpyAsync(hctr.data(), s->d_ctr + (size_t)(k % 3) * s->D, s->D * 4, hipMemcpyDeviceToHost,
-                               s->stream));
+        uint32_t width = 0;
+        HIP_TRY(hipMemcpyAsync(&width, s->d_ctr + (k % 3), 4, hipMemcpyDeviceToHost, s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
-        int64_t lv = 0, wk = 0;
-        for (int d = 0; d < s->D; d++) {
-            lv += hctr[d];
-            wk = std::max<int64_t>(wk, hctr[d]);
-        }
-        s->max_width = std::max(s->max_width, wk);
-        gx = (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, (wk + kBlock / kLanes - 1) / (kBlock / kLanes)));
-        if (lv == 0) break;
-        total += lv;
+        if (width == 0) break;
+        total += width;
+        s->max_width = std::max<int64_t>(s->max_width, width);
         if (k > s->N + 1) return fail(SSW_ERR_CYCLE, "dependency graph deeper than the number of cells");
-        k_sweep_level<double, false><<<dim3(gx, s->D), kBlock, 0, s->stream>>>(g, fl, 0, k);
+        unsigned gx = (unsigned)std::min<int64_t>(8192, (width + kTasksPerBlock - 1) / kTasksPerBlock);
+        k_sweep_build<double, false><<<gx, kBlock, 0, s->stream>>>(g, fl, 0, k);
         HIP_TRY(hipGetLastError());
     }
     if (total != s->N * (int64_t)s->D)
         return fail(SSW_ERR_CYCLE, "dependency cycle: " + std::to_string(s->N * (int64_t)s->D - total) +
                                        " cell-direction pairs never become ready");
     s->kmax = std::max(1, k);
-    // final offsets with the exact bound
     s->kcap = s->kmax;
-    TRY(dalloc(s, &s->d_lvloff, (size_t)L * s->D * (s->kcap + 2)));
+    TRY(dalloc(s, &s->d_lstart, (size_t)L * (s->kcap + 2)));
+    s->replay_ok.assign(L, 0);
+    s->level_start.assign(L, std::vector<uint32_t>());
+    for (hipGraphExec_t ge : s->graphs)
+        if (ge) (void)hipGraphExecDestroy(ge);
+    s->graphs.assign(L, nullptr);
+    return SSW_OK;
+}
+
+// Launches the replay of slot `lvl`'s level sets on the handle's stream, one
+// exact-size launch per non-empty level set; with `ev` each launch is
+// bracketed by events. Returns the number of launches.
+template <typename R>
+int launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev,
+                    const std::function<hipEvent_t()>* mark) {
+    Flux<R> fl = make_flux<R>(s);
+    const std::vector<uint32_t>& st = s->level_start[lvl];
+    const uint4* flat = s->d_flat + (size_t)lvl * s->D * s->N;
+    int n = 0;
+    for (int k = 0; k < s->kmax; k++) {
+        const uint32_t count = st[k + 1] - st[k];
+        if (count == 0) continue;
+        hipEvent_t a = ev ? (*mark)() : nullptr;
+        k_sweep_rec<R><<<(count + kTasksPerBlock - 1) / kTasksPerBlock, kBlock, 0, s->stream>>>(g, fl, flat + st[k],
+                                                                                               count);
+        if (ev) ev->push_back({a, (*mark)()});
+        n++;
+    }
+    return n;
+}
+
+int launch_replay(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev,
+                  const std::function<hipEvent_t()>* mark = nullptr) {
+    return s->fp32 ? launch_replay_t<float>(s, g, lvl, ev, mark) : launch_replay_t<double>(s, g, lvl, ev, mark);
+}
+
+// After a schedule is built: copies its level starts to the host and captures
+// the replay of its level sets as one graph.
+int build_replay_data(ssw_sweep* s, const Geo& g, int lvl) {
+    std::vector<uint32_t>& st = s->level_start[lvl];
+    st.resize(s->kmax + 1);
+    HIP_TRY(hipMemcpyAsync(st.data(), s->d_lstart + (size_t)lvl * (s->kcap + 2), (s->kmax + 1) * 4,
+                           hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->graphs[lvl]) {
+        (void)hipGraphExecDestroy(s->graphs[lvl]);
+        s->graphs[lvl] = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    launch_replay(s, g, lvl, nullptr);
+    hipError_t ce = hipStreamEndCapture(s->stream, &graph);
+    if (ce != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return fail(SSW_ERR_HIP, std::string("graph capture failed: ") + hipGetErrorString(ce));
+    }
+    hipGraphExec_t ge = nullptr;
+    hipError_t ie = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) return fail(SSW_ERR_HIP, std::string("graph instantiation failed: ") + hipGetErrorString(ie));
+    s->graphs[lvl] = ge;
+    s->replay_ok[lvl] = 1;
     return SSW_OK;
 }
 
@@ -1151,13 +810,12 @@ int run_step(ssw_sweep* s, double* elapsed) {
     ca.scale_factor = s->p.scale_factor;
     ca.prevent_cooling = s->p.prevent_cooling;
     ca.lref = s->lref;
-    ca.aref = s->aref;
     ca.failures = s->d_fail;
     ca.active = s->d_active;
-    Flux<R> flt = fl;
-    std::vector<int> order = s->ts.sweep_order();
+    const std::vector<int> order = s->ts.sweep_order();
+    const int L = s->p.num_timestep_levels;
     k_fill_u32<<<1, 2, 0, s->stream>>>((uint32_t*)s->d_active, 0u, 2);
-    const unsigned gx = sweep_grid_x(s);
+    const unsigned gx = build_grid_x(s);
     const dim3 gper(grid_for(std::max<int64_t>(s->Nper, 1)), s->D);
     const dim3 gcount(grid_for(s->N), s->D);
     const unsigned gcells = grid_for(s->N);
@@ -1165,7 +823,7 @@ int run_step(ssw_sweep* s, double* elapsed) {
     s->t_transport_total = s->t_chem_total = 0;
     s->n_transport = 0;
     size_t evn = 0;
-    auto mark = [&](void) -> hipEvent_t {
+    std::function<hipEvent_t()> mark = [&](void) -> hipEvent_t {
         if (evn >= s->ev.size()) {
             hipEvent_t e;
             (void)hipEventCreate(&e);
@@ -1176,59 +834,35 @@ int run_step(ssw_sweep* s, double* elapsed) {
         return e;
     };
     std::vector<std::pair<hipEvent_t, hipEvent_t>> tr, ch;
-    const int L = s->p.num_timestep_levels;
-    // Which schedules get (re)built in this step: those are swept with the
-    // level-synchronous kernels, the others replayed by the persistent kernel.
+    // Which schedules are (re)built in this step (device flags set by the level
+    // update of the previous step).
     std::vector<uint32_t> need(L, 1u);
     HIP_TRY(hipMemcpyAsync(need.data(), s->d_need, L * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     for (int l = 0; l < L; l++)
-        if (need[l]) s->chunks_ok[l] = 0;
+        if (need[l] && s->replay_ok[l]) {
+            s->replay_ok[l] = 0;
+            if (s->graphs[l]) {
+                (void)hipGraphExecDestroy(s->graphs[l]);
+                s->graphs[l] = nullptr;
+            }
+        }
     for (int lvl : order) {
-        k_begin<R><<<gper, kBlock, 0, s->stream>>>(g, flt, lvl);
-        k_count<<<gcount, kBlock, 0, s->stream>>>(g, lvl, 0);
-        bool done = false;
-        if (s->mode != 0 && !need[lvl] && s->chunks_ok[lvl]) {
-            Persist pp = make_persist(s, lvl);
-            k_fill_u32<<<grid_for(s->pctl_words), kBlock, 0, s->stream>>>(s->d_pctl, 0u, s->pctl_words);
-            hipEvent_t a = s->timing ? mark() : nullptr;
-            if (s->mode == 1)
-                k_sweep_persistent<R, true><<<s->pgrid, kBlock, 0, s->stream>>>(g, flt, pp, lvl);
-            else
-                k_sweep_persistent<R, false><<<s->pgrid, kBlock, 0, s->stream>>>(g, flt, pp, lvl);
-            if (s->timing) tr.push_back({a, mark()});
-            k_verify_done<<<1, 128, 0, s->stream>>>(g, pp, lvl, s->kmax);
+        k_begin<R><<<gper, kBlock, 0, s->stream>>>(g, fl, lvl);
+        if (need[lvl]) k_count<<<gcount, kBlock, 0, s->stream>>>(g, lvl, 0);
+        if (!need[lvl] && s->replay_ok[lvl]) {
+            if (s->timing) launch_replay(s, g, lvl, &tr, &mark);
+            else HIP_TRY(hipGraphLaunch(s->graphs[lvl], s->stream));
+        } else {  // build (or replay a schedule built without transport)
+            for (int k = 0; k < s->kmax; k++) {
+                hipEvent_t a = s->timing ? mark() : nullptr;
+                k_sweep_build<R, true><<<gx, kBlock, 0, s->stream>>>(g, fl, lvl, k);
+                if (s->timing) tr.push_back({a, mark()});
+            }
             HIP_TRY(hipGetLastError());
-            uint32_t aborted = 0;
-            HIP_TRY(hipMemcpyAsync(&aborted, pp.abort, 4, hipMemcpyDeviceToHost, s->stream));
-            HIP_TRY(hipStreamSynchronize(s->stream));
-            if (aborted) {  // re-run this sweep with the level-synchronous kernels
-                s->fallbacks++;
-                HIP_TRY(hipMemcpy(s->last_abort, pp.diag, sizeof(s->last_abort), hipMemcpyDeviceToHost));
-            } else {
-                done = true;
-            }
+            TRY(build_replay_data(s, g, lvl));
+            need[lvl] = 0;
         }
-        if (!done && !need[lvl] && s->chunks_ok[lvl]) {  // replay level by level, exact grids
-            const std::vector<uint32_t>& tot = s->level_tot[lvl];
-            for (int k = 0; k < s->kmax; k++) {
-                if (tot[k] == 0) continue;
-                hipEvent_t a = s->timing ? mark() : nullptr;
-                k_sweep_flat<R><<<(tot[k] + kChunk - 1) / kChunk, kBlock, 0, s->stream>>>(
-                    g, flt, s->d_pre + ((size_t)lvl * s->kmax + k) * (s->D + 1), lvl, k);
-                if (s->timing) tr.push_back({a, mark()});
-            }
-            done = true;
-        }
-        if (!done) {  // build (or first replay): one launch per level for every direction
-            for (int k = 0; k < s->kmax; k++) {
-                hipEvent_t a = s->timing ? mark() : nullptr;
-                k_sweep_level<R, true><<<dim3(gx, s->D), kBlock, 0, s->stream>>>(g, flt, lvl, k);
-                if (s->timing) tr.push_back({a, mark()});
-            }
-        }
-        if (!s->chunks_ok[lvl]) TRY(build_replay_data(s, g, lvl));
-        need[lvl] = 0;
         hipEvent_t a = s->timing ? mark() : nullptr;
         k_chemistry<R><<<gcells, kBlock, 0, s->stream>>>(g, fl, ca, lvl);
         if (s->timing) ch.push_back({a, mark()});
@@ -1274,6 +908,7 @@ void free_state(ssw_sweep* s) {
 }
 
 void free_grid(ssw_sweep* s) {
+    invalidate_replays(s);
     dfree(s->d_off);
     dfree(s->d_geo);
     dfree(s->d_nk);
@@ -1281,15 +916,13 @@ void free_grid(ssw_sweep* s) {
     dfree(s->d_size);
     dfree(s->d_pslot);
     dfree(s->d_pcells);
-    dfree(s->d_queue);
-    dfree(s->d_lvloff);
+    dfree(s->d_pfoff);
+    dfree(s->d_pfidx);
+    dfree(s->d_flat);
+    dfree(s->d_lstart);
     dfree(s->d_ctr);
     dfree(s->d_cnt);
     dfree(s->d_level);
-    dfree(s->d_chunks);
-    dfree(s->d_nchunks);
-    dfree(s->d_pre);
-    dfree(s->d_tot);
 }
 
 }  // namespace
@@ -1306,7 +939,7 @@ void ssw_default_parameters(ssw_sweep_parameters* p) {
     p->periodic = 0;
     p->max_timestep = 1e6 * 3.15576e7;
     p->significant_rate_threshold = 0.0;
-    p->timestep_safety_factor = 0.01 * 10.0;            // percent(10.0)
+    p->timestep_safety_factor = 0.01 * 10.0;  // percent(10.0)
     p->chemistry_timestep_safety_factor = 0.01 * 10.0;
     p->prevent_cooling = 1;
     p->transport_precision = SSW_TRANSPORT_FP64;
@@ -1376,12 +1009,6 @@ int ssw_create(const ssw_sweep_parameters* params, ssw_sweep** out) {
         return fail(SSW_ERR_HIP, "copy of directions failed");
     }
     s->ts.init(p.max_timestep, p.num_timestep_levels);
-    if (const char* m = getenv("SSW_TRANSPORT_MODE")) s->mode = atoi(m);
-    r = setup_xcc_slots(s);
-    if (r != SSW_OK) {
-        ssw_destroy(s);
-        return r;
-    }
     *out = s;
     return SSW_OK;
 }
@@ -1395,9 +1022,6 @@ void ssw_destroy(ssw_sweep* s) {
     dfree(s->d_need);
     dfree(s->d_fail);
     dfree(s->d_active);
-    dfree(s->d_xcc_slot);
-    dfree(s->d_dir_slot);
-    dfree(s->d_pctl);
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -1425,33 +1049,36 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     std::vector<double4> geo(F);
     std::vector<uint32_t> nk(F);
     std::vector<uint32_t> off32(n + 1);
-    std::vector<uint32_t> pslot(n, kNone), pcells;
+    std::vector<uint32_t> pslot(n, kNone), pcells, pfoff(1, 0), pfidx;
     double asum = 0.0;
-    int64_t nlocal = 0;
+    int64_t nfaces = 0;
     for (int64_t c = 0; c <= n; c++) off32[c] = (uint32_t)off[c];
     for (int64_t c = 0; c < n; c++) {
         bool has_p = false;
         for (int64_t f = off[c]; f < off[c + 1]; f++) {
             uint32_t k = kind[f];
             if (k > 2) return fail(SSW_ERR_INVALID, "face_kind must be 0, 1 or 2");
-            if (k == 1 && !s->p.periodic) k = 2;
-            int64_t nb = (k == 2) ? 0 : nbr[f];
-            if (k != 2 && (nb < 0 || nb >= n)) return fail(SSW_ERR_INVALID, "face_neighbour out of range");
+            if (k == kKindPeriodic && !s->p.periodic) k = kKindBoundary;
+            int64_t nb = (k == kKindBoundary) ? 0 : nbr[f];
+            if (k != kKindBoundary && (nb < 0 || nb >= n)) return fail(SSW_ERR_INVALID, "face_neighbour out of range");
             geo[f] = make_double4(normal[3 * f], normal[3 * f + 1], normal[3 * f + 2], area[f]);
             nk[f] = (uint32_t)nb | (k << 30);
-            if (k == 1) has_p = true;
+            if (k == kKindPeriodic) has_p = true;
             asum += area[f];
-            nlocal++;
+            nfaces++;
         }
         if (has_p) {
             pslot[c] = (uint32_t)pcells.size();
             pcells.push_back((uint32_t)c);
+            for (int64_t f = off[c]; f < off[c + 1]; f++)
+                if ((nk[f] >> 30) == kKindPeriodic) pfidx.push_back((uint32_t)f);
+            pfoff.push_back((uint32_t)pfidx.size());
         }
     }
     // Areas are carried in units of a power of two near the mean face area, so
     // that X = O / A keeps the range of O (SI areas reach 1e42 m^2); scaling
     // by a power of two is exact.
-    s->aref = (nlocal > 0 && asum > 0.0) ? std::ldexp(1.0, (int)std::lround(std::log2(asum / (double)nlocal))) : 1.0;
+    s->aref = (nfaces > 0 && asum > 0.0) ? std::ldexp(1.0, (int)std::lround(std::log2(asum / (double)nfaces))) : 1.0;
     s->Nper = (int64_t)pcells.size();
     const int L = s->p.num_timestep_levels;
     TRY(dalloc(s, &s->d_off, n + 1));
@@ -1461,9 +1088,11 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     TRY(dalloc(s, &s->d_size, n));
     TRY(dalloc(s, &s->d_pslot, n));
     TRY(dalloc(s, &s->d_pcells, std::max<int64_t>(1, s->Nper)));
+    TRY(dalloc(s, &s->d_pfoff, pfoff.size()));
+    TRY(dalloc(s, &s->d_pfidx, std::max<size_t>(1, pfidx.size())));
     TRY(dalloc(s, &s->d_level, n));
-    TRY(dalloc(s, &s->d_queue, (size_t)L * s->D * n));
-    TRY(dalloc(s, &s->d_ctr, (size_t)L * 3 * s->D));
+    TRY(dalloc(s, &s->d_flat, (size_t)L * s->D * n));
+    TRY(dalloc(s, &s->d_ctr, (size_t)L * 3));
     TRY(dalloc(s, &s->d_cnt, (size_t)s->D * n));
     HIP_TRY(hipMemcpyAsync(s->d_off, off32.data(), (n + 1) * 4, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_geo, geo.data(), F * sizeof(double4), hipMemcpyHostToDevice, s->stream));
@@ -1471,12 +1100,15 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     HIP_TRY(hipMemcpyAsync(s->d_vol, volume, n * 8, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_size, size, n * 8, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_pslot, pslot.data(), n * 4, hipMemcpyHostToDevice, s->stream));
-    if (s->Nper)
+    if (s->Nper) {
         HIP_TRY(hipMemcpyAsync(s->d_pcells, pcells.data(), s->Nper * 4, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->d_pfidx, pfidx.data(), pfidx.size() * 4, hipMemcpyHostToDevice, s->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(s->d_pfoff, pfoff.data(), pfoff.size() * 4, hipMemcpyHostToDevice, s->stream));
     std::vector<uint8_t> lev(n, (uint8_t)(L - 1));
     HIP_TRY(hipMemcpyAsync(s->d_level, lev.data(), n, hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipMemsetAsync(s->d_ctr, 0, (size_t)L * 3 * s->D * 4, s->stream));
-    // mirror check
+    HIP_TRY(hipMemsetAsync(s->d_ctr, 0, (size_t)L * 3 * 4, s->stream));
+    // mirror check and downwind statistics
     unsigned long long* d_bad = nullptr;
     TRY(dalloc(s, &d_bad, 4));
     HIP_TRY(hipMemsetAsync(d_bad, 0, 4 * 8, s->stream));
@@ -1494,7 +1126,6 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
                                       " local faces have no mirror face with the exactly negated normal");
     for (int k = 0; k < 3; k++) s->dw[k] = (double)hb[k + 1] / ((double)n * s->D);
     TRY(build_full_schedule(s));
-    TRY(alloc_chunks(s));
     s->have_grid = true;
     return SSW_OK;
 }
@@ -1503,6 +1134,7 @@ int ssw_upload_state(ssw_sweep* s, const double* density, const double* xhii, co
                      const double* source) {
     if (!s || !density || !xhii || !temp || !source) return fail(SSW_ERR_INVALID, "null argument");
     if (!s->have_grid) return fail(SSW_ERR_INVALID, "ssw_upload_grid must be called first");
+    invalidate_replays(s);
     free_state(s);
     const int64_t n = s->N;
     const int L = s->p.num_timestep_levels;
@@ -1546,13 +1178,13 @@ int ssw_upload_state(ssw_sweep* s, const double* density, const double* xhii, co
     HIP_TRY(hipMemsetAsync(s->d_I, 0, (size_t)s->D * n * es, s->stream));
     if (s->Nper) HIP_TRY(hipMemsetAsync(s->d_P, 0, (size_t)s->D * s->Nper * es, s->stream));
     HIP_TRY(hipMemsetAsync(s->d_fail, 0, 8, s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_active, 0, 8, s->stream));
     std::vector<uint8_t> lev(n, (uint8_t)(L - 1));  // src/sweep/mod.rs:206
     HIP_TRY(hipMemcpyAsync(s->d_level, lev.data(), n, hipMemcpyHostToDevice, s->stream));
     k_attenuation<<<grid_for(n), kBlock, 0, s->stream>>>(s->d_rho, s->d_x, s->d_size, s->d_att, n);
     HIP_TRY(hipGetLastError());
     TRY(set_need_all(s, 1u));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    s->chunks_ok.assign(L, 0);
     s->ts.init(s->p.max_timestep, L);
     s->have_state = true;
     return SSW_OK;
@@ -1561,8 +1193,12 @@ int ssw_upload_state(ssw_sweep* s, const double* density, const double* xhii, co
 int ssw_step(ssw_sweep* s, double* elapsed) {
     if (!s) return fail(SSW_ERR_INVALID, "null handle");
     if (!s->have_state) return fail(SSW_ERR_INVALID, "ssw_upload_state must be called first");
-    int r = s->fp32 ? run_step<float>(s, elapsed) : run_step<double>(s, elapsed);
-    if (r != SSW_OK) return r;
+    return s->fp32 ? run_step<float>(s, elapsed) : run_step<double>(s, elapsed);
+}
+
+int ssw_synchronize(ssw_sweep* s) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    HIP_TRY(hipStreamSynchronize(s->stream));
     return SSW_OK;
 }
 
@@ -1617,18 +1253,25 @@ int ssw_download_levelsets(ssw_sweep* s, int32_t lvl, uint32_t* depth) {
     HIP_TRY(hipStreamSynchronize(s->stream));
     if (need) {  // build the schedule of this active set without transport
         Flux<double> fl{};
-        k_begin<double><<<dim3(1, s->D), kBlock, 0, s->stream>>>(g, fl, lvl);
+        k_begin<double><<<dim3(1, 1), kBlock, 0, s->stream>>>(g, fl, lvl);
         k_count<<<dim3(grid_for(s->N), s->D), kBlock, 0, s->stream>>>(g, lvl, 0);
         for (int k = 0; k < s->kmax; k++)
-            k_sweep_level<double, false><<<dim3(sweep_grid_x(s), s->D), kBlock, 0, s->stream>>>(g, fl, lvl, k);
+            k_sweep_build<double, false><<<build_grid_x(s), kBlock, 0, s->stream>>>(g, fl, lvl, k);
         k_fill_u32<<<1, 1, 0, s->stream>>>(s->d_need + lvl, 0u, 1);
         HIP_TRY(hipGetLastError());
-        if ((int)s->chunks_ok.size() > lvl) s->chunks_ok[lvl] = 0;
+        if ((int)s->replay_ok.size() > lvl) {
+            s->replay_ok[lvl] = 0;
+            if (s->graphs[lvl]) {
+                (void)hipGraphExecDestroy(s->graphs[lvl]);
+                s->graphs[lvl] = nullptr;
+            }
+        }
     }
     uint32_t* d_depth = nullptr;
     TRY(dalloc(s, &d_depth, (size_t)s->D * s->N));
     k_fill_u32<<<grid_for((int64_t)s->D * s->N), kBlock, 0, s->stream>>>(d_depth, kNone, (int64_t)s->D * s->N);
-    k_scatter_depth<<<dim3(64, s->D), kBlock, 0, s->stream>>>(g, lvl, s->kmax, d_depth);
+    k_scatter_depth<<<grid_for(std::min<int64_t>((int64_t)s->D * s->N, 1 << 22)), kBlock, 0, s->stream>>>(
+        g, lvl, s->kmax, d_depth);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(depth, d_depth, (size_t)s->D * s->N * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -1676,13 +1319,11 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info) {
     info->downwind_boundary = s->dw[2];
     info->device_bytes = s->bytes;
     if (s->have_state) {
-        unsigned long long f = 0;
+        unsigned long long f = 0, a = 0;
         HIP_TRY(hipMemcpyAsync(&f, s->d_fail, 8, hipMemcpyDeviceToHost, s->stream));
-        HIP_TRY(hipStreamSynchronize(s->stream));
-        info->chemistry_failures = (int64_t)f;
-        unsigned long long a = 0;
         HIP_TRY(hipMemcpyAsync(&a, s->d_active, 8, hipMemcpyDeviceToHost, s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
+        info->chemistry_failures = (int64_t)f;
         info->updates_last_step = (int64_t)a * s->D;
     }
     return SSW_OK;
@@ -1715,43 +1356,11 @@ double ssw_chemistry_timestep(double* state, double density, double volume, doub
     return ts;
 }
 
-int ssw_synchronize(ssw_sweep* s) {
-    if (!s) return fail(SSW_ERR_INVALID, "null handle");
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    return SSW_OK;
-}
-
 int ssw_set_transport_mode(ssw_sweep* s, int32_t mode) {
     if (!s) return fail(SSW_ERR_INVALID, "null handle");
-    if (mode < 0 || mode > 2) return fail(SSW_ERR_INVALID, "mode must be 0, 1 or 2");
-    s->mode = mode;
-    return SSW_OK;
-}
-
-int64_t ssw_transport_fallbacks(ssw_sweep* s) { return s ? s->fallbacks : -1; }
-
-int ssw_transport_layout(ssw_sweep* s, int32_t sweep_level, int32_t* out) {
-    if (!s || !out) return fail(SSW_ERR_INVALID, "null argument");
-    if (sweep_level < 0 || sweep_level >= s->p.num_timestep_levels)
-        return fail(SSW_ERR_INVALID, "sweep_level out of range");
-    out[0] = s->nslots;
-    out[1] = s->pgrid;
-    out[2] = (int32_t)s->chunk_cap;
-    out[3] = s->chunks_ok.empty() ? 0 : s->chunks_ok[sweep_level];
-    int slots[16];
-    HIP_TRY(hipMemcpy(slots, s->d_xcc_slot, sizeof(slots), hipMemcpyDeviceToHost));
-    for (int i = 0; i < 16; i++) out[4 + i] = slots[i];
-    std::vector<uint32_t> nch(s->nslots, 0);
-    if (s->d_nchunks)
-        HIP_TRY(hipMemcpy(nch.data(), s->d_nchunks + (size_t)sweep_level * s->nslots, s->nslots * 4,
-                          hipMemcpyDeviceToHost));
-    for (int i = 0; i < 16; i++) out[20 + i] = i < s->nslots ? (int32_t)nch[i] : -1;
-    return SSW_OK;
-}
-
-int ssw_transport_last_abort(ssw_sweep* s, uint32_t* out8) {
-    if (!s || !out8) return fail(SSW_ERR_INVALID, "null argument");
-    std::memcpy(out8, s->last_abort, sizeof(s->last_abort));
+    if (mode != 0)
+        return fail(SSW_ERR_INVALID, "transport mode " + std::to_string(mode) +
+                                         " is not available: only the level-synchronous transport (0) is built");
     return SSW_OK;
 }
 

@@ -151,24 +151,10 @@ class Sweep:
         return inf
 
     def set_transport_mode(self, mode: int) -> None:
-        """0: one launch per level set; 1: persistent, agent-scope ordering;
-        2: persistent, each direction on one XCD (default)."""
+        """0 (the only mode): one launch per Kahn level set, replayed as a
+        captured graph. Other values raise: the persistent kernel they
+        selected has been removed."""
         N.check(self._lib.ssw_set_transport_mode(self._h, int(mode)))
-
-    def transport_fallbacks(self) -> int:
-        return int(self._lib.ssw_transport_fallbacks(self._h))
-
-    def transport_layout(self, sweep_level: int = 0) -> dict:
-        out = np.zeros(36, np.int32)
-        N.check(self._lib.ssw_transport_layout(self._h, sweep_level, N.ptr(out, C.c_int32)))
-        return {"slots": int(out[0]), "workgroups": int(out[1]), "capacity": int(out[2]),
-                "valid": int(out[3]), "xcc_slot": out[4:20].tolist(), "chunks": out[20:36].tolist()}
-
-    def transport_last_abort(self) -> dict:
-        out = np.zeros(8, np.uint32)
-        N.check(self._lib.ssw_transport_last_abort(self._h, N.ptr(out, C.c_uint32)))
-        keys = ["recorded", "direction", "level", "awaited", "observed", "slot", "ticket", "chunks"]
-        return dict(zip(keys, out.tolist()))
 
     def set_timing(self, enabled: bool) -> None:
         N.check(self._lib.ssw_set_timing(self._h, 1 if enabled else 0))
