@@ -231,49 +231,125 @@ struct Grid {
 
 thread_local std::string g_err;
 
-struct BucketGrid {
-    int nb[3];
-    double L[3];
-    double bs[3];
-    std::vector<int64_t> start;  // nb^3 + 1
-    std::vector<int64_t> items;
-    void build(const double* pts, int64_t n, const double* box, double target_per_bucket) {
-        double vol = box[0] * box[1] * box[2];
-        double bsz = std::cbrt(vol * target_per_bucket / (double)n);
-        for (int k = 0; k < 3; k++) {
-            L[k] = box[k];
-            nb[k] = std::max(1, (int)std::floor(box[k] / bsz));
-            if (nb[k] > 2048) nb[k] = 2048;
-            bs[k] = box[k] / nb[k];
-        }
-        int64_t nt = (int64_t)nb[0] * nb[1] * nb[2];
-        std::vector<int64_t> cnt(nt + 1, 0);
-        std::vector<int64_t> b(n);
-        for (int64_t i = 0; i < n; i++) {
-            b[i] = bucket_of(pts + 3 * i);
-            cnt[b[i] + 1]++;
-        }
-        for (int64_t i = 0; i < nt; i++) cnt[i + 1] += cnt[i];
-        start = cnt;
-        items.assign(n, 0);
-        std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
-        for (int64_t i = 0; i < n; i++) items[fill[b[i]]++] = i;
-    }
-    int coord(double v, int k) const {
-        int c = (int)std::floor(v / bs[k]);
-        if (c < 0) c = 0;
-        if (c >= nb[k]) c = nb[k] - 1;
-        return c;
-    }
-    int64_t bucket_of(const double* p) const {
-        return ((int64_t)coord(p[2], 2) * nb[1] + coord(p[1], 1)) * nb[0] + coord(p[0], 0);
-    }
-};
-
 struct Cand {
     double d2;
     int64_t j;
     int32_t s[3];
+};
+
+// k-d tree over generator positions (and their periodic images) for the
+// k-nearest candidate queries of the cell construction.
+struct KdTree {
+    struct Item {
+        V3 p;
+        int64_t j;
+        int32_t s[3];
+    };
+    struct Node {
+        double lo[3], hi[3];
+        int64_t b, e;
+        int32_t l, r;  // children, -1 for a leaf
+    };
+    std::vector<Item> items;
+    std::vector<Node> nodes;
+
+    static double axis(const V3& p, int k) { return k == 0 ? p.x : (k == 1 ? p.y : p.z); }
+
+    int build_rec(int64_t b, int64_t e) {
+        Node nd;
+        for (int k = 0; k < 3; k++) {
+            nd.lo[k] = HUGE_VAL;
+            nd.hi[k] = -HUGE_VAL;
+        }
+        for (int64_t i = b; i < e; i++)
+            for (int k = 0; k < 3; k++) {
+                nd.lo[k] = std::min(nd.lo[k], axis(items[i].p, k));
+                nd.hi[k] = std::max(nd.hi[k], axis(items[i].p, k));
+            }
+        nd.b = b;
+        nd.e = e;
+        nd.l = nd.r = -1;
+        const int id = (int)nodes.size();
+        nodes.push_back(nd);
+        if (e - b > 8) {
+            int dim = 0;
+            for (int k = 1; k < 3; k++)
+                if (nd.hi[k] - nd.lo[k] > nd.hi[dim] - nd.lo[dim]) dim = k;
+            const int64_t mid = (b + e) / 2;
+            std::nth_element(items.begin() + b, items.begin() + mid, items.begin() + e,
+                             [dim](const Item& x, const Item& y) { return axis(x.p, dim) < axis(y.p, dim); });
+            const int l = build_rec(b, mid);
+            const int r = build_rec(mid, e);
+            nodes[id].l = l;
+            nodes[id].r = r;
+        }
+        return id;
+    }
+
+    void build() {
+        nodes.clear();
+        nodes.reserve((size_t)(items.size() / 4 + 16));
+        if (!items.empty()) build_rec(0, (int64_t)items.size());
+    }
+
+    static double box_d2(const Node& n, const V3& q) {
+        double d2 = 0.0;
+        for (int k = 0; k < 3; k++) {
+            const double v = axis(q, k);
+            const double d = v < n.lo[k] ? n.lo[k] - v : (v > n.hi[k] ? v - n.hi[k] : 0.0);
+            d2 += d * d;
+        }
+        return d2;
+    }
+
+    // The k nearest items to q (excluding item `self` with zero shift), as
+    // candidates relative to `pa`, sorted by (d2, index, shift).
+    void knn(const V3& q, const V3& pa, int64_t self, size_t k, std::vector<Cand>& out,
+             std::vector<std::pair<int, double>>& stack) const {
+        out.clear();
+        if (nodes.empty()) return;
+        auto worse = [](const Cand& a, const Cand& b) { return a.d2 < b.d2; };  // max-heap on d2
+        stack.clear();
+        stack.push_back({0, box_d2(nodes[0], q)});
+        while (!stack.empty()) {
+            auto [ni, nd2] = stack.back();
+            stack.pop_back();
+            if (out.size() == k && nd2 > out.front().d2) continue;
+            const Node& n = nodes[ni];
+            if (n.l < 0) {
+                for (int64_t i = n.b; i < n.e; i++) {
+                    const Item& it = items[i];
+                    if (it.j == self && it.s[0] == 0 && it.s[1] == 0 && it.s[2] == 0) continue;
+                    const V3 d = it.p - pa;
+                    const double d2 = dot(d, d);
+                    if (out.size() < k) {
+                        out.push_back({d2, it.j, {it.s[0], it.s[1], it.s[2]}});
+                        std::push_heap(out.begin(), out.end(), worse);
+                    } else if (d2 < out.front().d2) {
+                        std::pop_heap(out.begin(), out.end(), worse);
+                        out.back() = {d2, it.j, {it.s[0], it.s[1], it.s[2]}};
+                        std::push_heap(out.begin(), out.end(), worse);
+                    }
+                }
+            } else {
+                const double dl = box_d2(nodes[n.l], q), dr = box_d2(nodes[n.r], q);
+                if (dl <= dr) {
+                    stack.push_back({n.r, dr});
+                    stack.push_back({n.l, dl});
+                } else {
+                    stack.push_back({n.l, dl});
+                    stack.push_back({n.r, dr});
+                }
+            }
+        }
+        std::sort(out.begin(), out.end(), [](const Cand& a, const Cand& b) {
+            if (a.d2 != b.d2) return a.d2 < b.d2;
+            if (a.j != b.j) return a.j < b.j;
+            for (int k = 0; k < 3; k++)
+                if (a.s[k] != b.s[k]) return a.s[k] < b.s[k];
+            return false;
+        });
+    }
 };
 
 }  // namespace
@@ -404,7 +480,6 @@ void* ssw_voronoi_build(int64_t n, const double* pts, const double* box, int mod
                         int num_threads) {
     Grid* g = new Grid();
     g->N = n;
-    BucketGrid bg;
     double boxv[3] = {box[0], box[1], box[2]};
     if (mode == 1) {
         // shift points into a box covering [lo, hi]
@@ -415,158 +490,173 @@ void* ssw_voronoi_build(int64_t n, const double* pts, const double* box, int mod
     std::vector<double> local(3 * n);
     for (int64_t i = 0; i < n; i++)
         for (int k = 0; k < 3; k++) local[3 * i + k] = (mode == 1) ? pts[3 * i + k] - lo_hi[k] : pts[3 * i + k];
-    bg.build(local.data(), n, boxv, 4.0);
-    double mean_sp = std::cbrt(boxv[0] * boxv[1] * boxv[2] / (double)n);
+    const double mean_sp = std::cbrt(boxv[0] * boxv[1] * boxv[2] / (double)n);
+    // Candidate generators: the points and, in the periodic box, their images
+    // within `margin` of the box. Cells whose security sphere leaves the imaged
+    // region are built again with a tree of wider margin.
+    auto make_tree = [&](double margin, KdTree& kd) {
+        kd.items.clear();
+        kd.items.reserve((size_t)n * (mode == 0 ? 2 : 1));
+        for (int64_t j = 0; j < n; j++) {
+            const V3 q{local[3 * j], local[3 * j + 1], local[3 * j + 2]};
+            for (int sz = -1; sz <= 1; sz++)
+                for (int sy = -1; sy <= 1; sy++)
+                    for (int sx = -1; sx <= 1; sx++) {
+                        if (mode == 1 && (sx || sy || sz)) continue;
+                        const V3 im{q.x + sx * boxv[0], q.y + sy * boxv[1], q.z + sz * boxv[2]};
+                        if (im.x < -margin || im.x > boxv[0] + margin || im.y < -margin ||
+                            im.y > boxv[1] + margin || im.z < -margin || im.z > boxv[2] + margin)
+                            continue;
+                        kd.items.push_back({im, j, {sx, sy, sz}});
+                    }
+        }
+        kd.build();
+    };
     std::vector<std::vector<BuiltFace>> cells(n);
     std::vector<double> vols(n, 0.0);
     std::atomic<int64_t> open_cells{0};
 #ifdef _OPENMP
     if (num_threads > 0) omp_set_num_threads(num_threads);
 #endif
+    const double bmax = std::max(boxv[0], std::max(boxv[1], boxv[2]));
+    double margin = (mode == 0) ? std::min(0.5, std::max(0.1, 8.0 * mean_sp / bmax)) * bmax : 0.0;
+    std::vector<int64_t> todo(n);
+    for (int64_t i = 0; i < n; i++) todo[i] = i;
+    std::vector<double> need(n, 0.0);  // margin a cell needs beyond the current one
+    for (;;) {
+        KdTree kd;
+        make_tree(margin, kd);
 #pragma omp parallel
-    {
-        Poly poly;
-        std::vector<Cand> cands;
+        {
+            Poly poly;
+            std::vector<Cand> cands;
+            std::vector<std::pair<int, double>> stack;
 #pragma omp for schedule(dynamic, 256)
-        for (int64_t i = 0; i < n; i++) {
-            // work relative to the generator for accuracy
-            V3 pa{local[3 * i], local[3 * i + 1], local[3 * i + 2]};
-            V3 p{0.0, 0.0, 0.0};
-            double tol = 1e-12 * mean_sp;
-            if (mode == 0) {
-                double hb = std::max(boxv[0], std::max(boxv[1], boxv[2]));
-                poly.init_box(V3{-hb, -hb, -hb}, V3{hb, hb, hb});
-            } else {
-                poly.init_box(V3{0, 0, 0} - pa, V3{boxv[0], boxv[1], boxv[2]} - pa);
-            }
-            int bc[3] = {bg.coord(pa.x, 0), bg.coord(pa.y, 1), bg.coord(pa.z, 2)};
-            int maxr = std::max(bg.nb[0], std::max(bg.nb[1], bg.nb[2])) + 1;
-            int done_r = -1;
-            double applied2 = -1.0;  // candidates with d2 <= applied2 were clipped already
-            for (int r = 1; r <= maxr; r++) {
-                cands.clear();
-                for (int oz = -r; oz <= r; oz++)
-                    for (int oy = -r; oy <= r; oy++)
-                        for (int ox = -r; ox <= r; ox++) {
-                            int o[3] = {ox, oy, oz};
-                            int bi[3];
-                            int32_t sh[3];
-                            bool skip = false;
-                            for (int k = 0; k < 3; k++) {
-                                int v = bc[k] + o[k];
-                                int m = bg.nb[k];
-                                int s = (int)std::floor((double)v / m);
-                                bi[k] = v - s * m;
-                                sh[k] = s;
-                                if (mode == 1 && s != 0) skip = true;
-                            }
-                            if (skip) continue;
-                            int64_t b = ((int64_t)bi[2] * bg.nb[1] + bi[1]) * bg.nb[0] + bi[0];
-                            for (int64_t t = bg.start[b]; t < bg.start[b + 1]; t++) {
-                                int64_t j = bg.items[t];
-                                if (j == i && sh[0] == 0 && sh[1] == 0 && sh[2] == 0) continue;
-                                V3 q{local[3 * j] + sh[0] * boxv[0], local[3 * j + 1] + sh[1] * boxv[1],
-                                     local[3 * j + 2] + sh[2] * boxv[2]};
-                                V3 d = q - pa;
-                                double d2 = dot(d, d);
-                                if (d2 <= applied2) continue;
-                                cands.push_back({d2, j, {sh[0], sh[1], sh[2]}});
-                            }
-                        }
-                std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) {
-                    if (a.d2 != b.d2) return a.d2 < b.d2;
-                    if (a.j != b.j) return a.j < b.j;
+            for (size_t ti = 0; ti < todo.size(); ti++) {
+                const int64_t i = todo[ti];
+                // work relative to the generator for accuracy
+                V3 pa{local[3 * i], local[3 * i + 1], local[3 * i + 2]};
+                V3 p{0.0, 0.0, 0.0};
+                double tol = 1e-12 * mean_sp;
+                if (mode == 0) poly.init_box(V3{-bmax, -bmax, -bmax}, V3{bmax, bmax, bmax});
+                else poly.init_box(V3{0, 0, 0} - pa, V3{boxv[0], boxv[1], boxv[2]} - pa);
+                // Clip with the k nearest candidates in order of distance; the
+                // cell is complete once no candidate beyond the k-th can cut it
+                // (distance > 2 x the farthest vertex). A larger k repeats the
+                // smaller query's candidates first (same total order); those are
+                // skipped, so every plane is applied once.
+                int done_r = -1;
+                const size_t total = kd.items.size() - 1;
+                bool have_last = false;
+                Cand last{0.0, 0, {0, 0, 0}};
+                auto after_last = [&](const Cand& c) {
+                    if (!have_last) return true;
+                    if (c.d2 != last.d2) return c.d2 > last.d2;
+                    if (c.j != last.j) return c.j > last.j;
                     for (int k = 0; k < 3; k++)
-                        if (a.s[k] != b.s[k]) return a.s[k] < b.s[k];
+                        if (c.s[k] != last.s[k]) return c.s[k] > last.s[k];
                     return false;
-                });
-                // points within radius r*min(bs) are complete
-                double complete = r * std::min(bg.bs[0], std::min(bg.bs[1], bg.bs[2]));
-                double complete2 = complete * complete;
-                double rmax2 = poly.max_dist2(p);
-                bool stop = false;
-                for (const Cand& c : cands) {
-                    if (c.d2 > complete2) break;
-                    if (c.d2 > 4.0 * rmax2 * (1.0 + 1e-9)) {
-                        stop = true;
+                };
+                for (size_t k = std::min<size_t>(64, total);; k = std::min(total, k * 4)) {
+                    kd.knn(pa, pa, i, k, cands, stack);
+                    double rmax2 = poly.max_dist2(p);
+                    bool stop = false;
+                    for (const Cand& c : cands) {
+                        if (!after_last(c)) continue;
+                        last = c;
+                        have_last = true;
+                        if (c.d2 > 4.0 * rmax2 * (1.0 + 1e-9)) {
+                            stop = true;
+                            break;
+                        }
+                        V3 q{local[3 * c.j] + c.s[0] * boxv[0] - pa.x, local[3 * c.j + 1] + c.s[1] * boxv[1] - pa.y,
+                             local[3 * c.j + 2] + c.s[2] * boxv[2] - pa.z};
+                        V3 m = q * 0.5;
+                        if (poly.clip(m, q, c.j, c.s, tol)) rmax2 = poly.max_dist2(p);
+                    }
+                    const bool all = cands.size() >= total;
+                    if (stop || all || 4.0 * rmax2 * (1.0 + 1e-9) < cands.back().d2) {
+                        done_r = 1;
                         break;
                     }
-                    V3 q{local[3 * c.j] + c.s[0] * boxv[0] - pa.x, local[3 * c.j + 1] + c.s[1] * boxv[1] - pa.y,
-                         local[3 * c.j + 2] + c.s[2] * boxv[2] - pa.z};
-                    V3 m = q * 0.5;
-                    if (poly.clip(m, q, c.j, c.s, tol)) rmax2 = poly.max_dist2(p);
                 }
-                applied2 = complete2;
-                if (stop || 4.0 * rmax2 <= complete2) {
-                    done_r = r;
-                    break;
-                }
-            }
-            if (done_r < 0 && mode == 0) open_cells++;
-            // collect faces
-            std::vector<BuiltFace>& out = cells[i];
-            double vol = 0.0;
-            for (const Face& f : poly.faces) {
-                double a = polygon_area(f.pts);
-                BuiltFace bf;
-                bf.nb = f.label;
-                bf.shift[0] = f.shift[0];
-                bf.shift[1] = f.shift[1];
-                bf.shift[2] = f.shift[2];
-                bf.area = a;
-                if (f.label >= 0) {
-                    V3 q{local[3 * f.label] + f.shift[0] * boxv[0], local[3 * f.label + 1] + f.shift[1] * boxv[1],
-                         local[3 * f.label + 2] + f.shift[2] * boxv[2]};
-                    bf.normal = normalize(q - pa);
-                } else {
-                    bf.normal = normalize(f.plane_n);
-                }
-                // pyramid_volume (src/voronoi/cell.rs:153-157)
-                bf.height = std::fabs(dot(bf.normal, p - f.pts[0]));
-                vol += 1.0 / 3.0 * a * bf.height;
-                bool merged = false;
-                for (BuiltFace& e : out) {
-                    if (e.nb == bf.nb && bf.nb >= 0 && e.shift[0] == bf.shift[0] && e.shift[1] == bf.shift[1] &&
-                        e.shift[2] == bf.shift[2]) {
-                        if (bf.area > e.area) e = bf;
-                        merged = true;
+                need[i] = 0.0;
+                if (mode == 0 && done_r > 0) {
+                    // the security sphere must lie inside the imaged region
+                    const double r = 2.0 * std::sqrt(poly.max_dist2(p)) * (1.0 + 1e-9);
+                    double over = 0.0;
+                    over = std::max(over, r - pa.x);
+                    over = std::max(over, r - pa.y);
+                    over = std::max(over, r - pa.z);
+                    over = std::max(over, pa.x + r - boxv[0]);
+                    over = std::max(over, pa.y + r - boxv[1]);
+                    over = std::max(over, pa.z + r - boxv[2]);
+                    if (over > margin) {
+                        if (margin < bmax) {
+                            need[i] = over;
+                            continue;  // built again with a wider margin
+                        }
+                        done_r = -1;
                     }
                 }
-                if (!merged) out.push_back(bf);
+                if (done_r < 0 && mode == 0) open_cells++;
+                // collect faces
+                std::vector<BuiltFace>& out = cells[i];
+                double vol = 0.0;
+                for (const Face& f : poly.faces) {
+                    double a = polygon_area(f.pts);
+                    BuiltFace bf;
+                    bf.nb = f.label;
+                    bf.shift[0] = f.shift[0];
+                    bf.shift[1] = f.shift[1];
+                    bf.shift[2] = f.shift[2];
+                    bf.area = a;
+                    if (f.label >= 0) {
+                        V3 q{local[3 * f.label] + f.shift[0] * boxv[0], local[3 * f.label + 1] + f.shift[1] * boxv[1],
+                             local[3 * f.label + 2] + f.shift[2] * boxv[2]};
+                        bf.normal = normalize(q - pa);
+                    } else {
+                        bf.normal = normalize(f.plane_n);
+                    }
+                    // pyramid_volume (src/voronoi/cell.rs:153-157)
+                    bf.height = std::fabs(dot(bf.normal, p - f.pts[0]));
+                    vol += 1.0 / 3.0 * a * bf.height;
+                    bool merged = false;
+                    for (BuiltFace& e : out) {
+                        if (e.nb == bf.nb && bf.nb >= 0 && e.shift[0] == bf.shift[0] && e.shift[1] == bf.shift[1] &&
+                            e.shift[2] == bf.shift[2]) {
+                            if (bf.area > e.area) e = bf;
+                            merged = true;
+                        }
+                    }
+                    if (!merged) out.push_back(bf);
+                }
+                vols[i] = vol;
             }
-            vols[i] = vol;
         }
+        std::vector<int64_t> again;
+        double widest = 0.0;
+        for (int64_t i : todo)
+            if (need[i] > 0.0) {
+                again.push_back(i);
+                widest = std::max(widest, need[i]);
+            }
+        if (again.empty()) break;
+        todo.swap(again);
+        margin = std::min(bmax, widest * 1.05);
     }
     g->open_cells = open_cells.load();
     // Mirror consistency: keep neighbour faces whose twin exists, share the area.
-    auto key = [](int64_t a, int64_t b, const int32_t* s) {
-        uint64_t h = (uint64_t)a * 0x9E3779B97F4A7C15ull ^ ((uint64_t)b + 0x632BE59BD9B4E019ull) * 0xC2B2AE3D27D4EB4Full;
-        h ^= ((uint64_t)(s[0] + 8) << 0) | ((uint64_t)(s[1] + 8) << 8) | ((uint64_t)(s[2] + 8) << 16);
-        return h;
-    };
-    // index of faces by (cell, nb, shift)
-    std::unordered_map<uint64_t, std::vector<std::pair<int64_t, int>>> idx;
-    idx.reserve((size_t)n * 16);
-    for (int64_t i = 0; i < n; i++)
-        for (int f = 0; f < (int)cells[i].size(); f++) {
-            const BuiltFace& bf = cells[i][f];
-            if (bf.nb < 0) continue;
-            idx[key(i, bf.nb, bf.shift)].push_back({i, f});
-        }
     auto find_twin = [&](int64_t i, const BuiltFace& bf) -> const BuiltFace* {
-        int32_t ms[3] = {-bf.shift[0], -bf.shift[1], -bf.shift[2]};
-        auto it = idx.find(key(bf.nb, i, ms));
-        if (it == idx.end()) return nullptr;
-        for (auto& pr : it->second) {
-            const BuiltFace& t = cells[pr.first][pr.second];
-            if (pr.first == bf.nb && t.nb == i && t.shift[0] == ms[0] && t.shift[1] == ms[1] && t.shift[2] == ms[2])
-                return &t;
-        }
+        const int32_t ms[3] = {-bf.shift[0], -bf.shift[1], -bf.shift[2]};
+        for (const BuiltFace& t : cells[bf.nb])
+            if (t.nb == i && t.shift[0] == ms[0] && t.shift[1] == ms[1] && t.shift[2] == ms[2]) return &t;
         return nullptr;
     };
     g->off.assign(n + 1, 0);
     std::vector<std::vector<BuiltFace>> kept(n);
     int64_t dropped = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : dropped)
     for (int64_t i = 0; i < n; i++) {
         for (const BuiltFace& bf : cells[i]) {
             if (bf.nb < 0) {
@@ -591,8 +681,8 @@ void* ssw_voronoi_build(int64_t n, const double* pts, const double* box, int mod
                 if (a.shift[k] != b.shift[k]) return a.shift[k] < b.shift[k];
             return false;
         });
-        g->off[i + 1] = g->off[i] + (int64_t)kept[i].size();
     }
+    for (int64_t i = 0; i < n; i++) g->off[i + 1] = g->off[i] + (int64_t)kept[i].size();
     g->dropped_faces = dropped;
     int64_t F = g->off[n];
     g->nbr.resize(F);

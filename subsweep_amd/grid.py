@@ -136,6 +136,48 @@ def lognormal_density(n: int, rho_mean: float, sigma_ln: float, seed: int) -> np
     return out
 
 
+def gaussian_random_field(mesh: int, slope: float, smoothing: float, seed: int) -> np.ndarray:
+    """Seeded Gaussian random field on a periodic mesh^3 grid with power
+    spectrum P(k) ~ k^slope, Gaussian-smoothed with radius `smoothing` mesh
+    cells, normalised to zero mean and unit variance (numpy PCG64 stream)."""
+    rng = np.random.default_rng(seed)
+    white = rng.standard_normal((mesh, mesh, mesh))
+    fk = np.fft.rfftn(white)
+    kx = np.fft.fftfreq(mesh)[:, None, None]
+    ky = np.fft.fftfreq(mesh)[None, :, None]
+    kz = np.fft.rfftfreq(mesh)[None, None, :]
+    k2 = kx * kx + ky * ky + kz * kz  # cycles per mesh cell
+    with np.errstate(divide="ignore"):
+        amp = np.where(k2 > 0, k2 ** (slope / 4.0), 0.0)  # sqrt(P) = k^(slope/2)
+    amp = amp * np.exp(-0.5 * (2.0 * np.pi) ** 2 * k2 * smoothing ** 2)
+    g = np.fft.irfftn(fk * amp, s=(mesh, mesh, mesh))
+    g -= g.mean()
+    g /= g.std()
+    return g
+
+
+def clustered_points(n: int, box: float, mesh: int = 128, sigma_ln: float = 2.0, slope: float = -2.0,
+                     smoothing: float = 2.0, seed: int = 1340):
+    """n points sampled with probability proportional to the lognormal field
+    delta = exp(sigma g - sigma^2 / 2) of a Gaussian random field g (above) on a
+    mesh^3 grid over the periodic box: mesh cells are drawn from the CDF of
+    delta, positions uniform within the drawn cell. Returns (points [n, 3],
+    delta of each point's mesh cell)."""
+    g = gaussian_random_field(mesh, slope, smoothing, seed)
+    delta = np.exp(sigma_ln * g - 0.5 * sigma_ln * sigma_ln).reshape(-1)
+    cdf = np.cumsum(delta)
+    cdf /= cdf[-1]
+    rng = np.random.default_rng(seed + 1)
+    cell = np.minimum(np.searchsorted(cdf, rng.random(n), side="right"), delta.size - 1)
+    ix = cell // (mesh * mesh)
+    iy = (cell // mesh) % mesh
+    iz = cell % mesh
+    h = box / mesh
+    pts = (np.stack([ix, iy, iz], axis=1) + rng.random((n, 3))) * h
+    pts = np.minimum(pts, np.nextafter(box, 0.0))
+    return pts, delta[cell]
+
+
 def assign_sources(points: np.ndarray, positions: np.ndarray, rates: np.ndarray):
     """Adds each source's rate to its nearest cell (Euclidean, non-periodic),
     as set_source_terms_system does (src/source_systems.rs:48-80). Returns

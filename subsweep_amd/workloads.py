@@ -9,9 +9,18 @@
                      84 directions, 1 level, threshold 1e-5 s^-1.
   single_source_32   32^3 = 32768 cells, sigma_ln 1.0, one 1e50 photons/s
                      source at the box centre, 16 directions, 1 level.
+  clustered_problem  points sampled from a lognormal field (Gaussian random
+                     field, P(k) ~ k^-2, smoothed on 2 mesh cells, 128^3
+                     mesh, sigma_ln 2), cell density 1e-27 g/cm^3 times the
+                     field at the generator, sources on the densest cells
+                     with luminosities from p(L) ~ L^-2 on [1e48, 1e52]
+                     photons/s times an escape fraction of 0.01.
+  clustered_2m       2097152 clustered cells, 1024 sources, 84 directions,
+                     4 timestep levels, FP32 transport.
   cartesian_problem  the reference's cartesian test grid with uniform gas.
 
-Seeds: lattice 1338, density 1339, sources 1337.
+Seeds: lattice 1338, density 1339, sources 1337, clustered field 1340/1341,
+clustered sources 1342.
 """
 
 from __future__ import annotations
@@ -21,7 +30,8 @@ from dataclasses import dataclass, replace
 import numpy as np
 
 from . import units as U
-from .grid import Grid, assign_sources, cartesian_grid, jittered_lattice, lognormal_density, voronoi_grid
+from .grid import (Grid, assign_sources, cartesian_grid, clustered_points, jittered_lattice, lognormal_density,
+                   voronoi_grid)
 from .sweep import SweepParameters
 
 BOX = 4000.0 * U.KILOPARSEC
@@ -79,6 +89,37 @@ def precision_64(**kw) -> Problem:
     kw.setdefault("threshold", 1e-5)
     kw.setdefault("steps", 20)
     return jittered_problem(64, 1.5, "64", 84, name="precision_64", **kw)
+
+
+def power_law_luminosities(k: int, lo: float, hi: float, seed: int) -> np.ndarray:
+    """k draws from p(L) ~ L^-2 on [lo, hi] by inverting the CDF
+    F(L) = (1/lo - 1/L) / (1/lo - 1/hi)."""
+    u = np.random.default_rng(seed).random(k)
+    return 1.0 / (1.0 / lo - u * (1.0 / lo - 1.0 / hi))
+
+
+def clustered_problem(n: int, num_directions: int = 84, levels: int = 4, sources: int = 1024,
+                      sigma_ln: float = 2.0, mesh: int = 128, escape_fraction: float = 0.01,
+                      threshold: float = 0.0, periodic: bool = True, steps: int = 10, precision: int = 32,
+                      timestep_safety_factor: float = 0.1, name: str | None = None, threads: int = 0) -> Problem:
+    pts, delta = clustered_points(n, BOX, mesh=mesh, sigma_ln=sigma_ln, seed=1340)
+    grid = voronoi_grid(pts, BOX, threads=threads)
+    N = grid.num_cells
+    rho = RHO_MEAN * delta
+    # the densest cells (ties by index) each get one source
+    order = np.lexsort((np.arange(N), -rho))
+    src = np.zeros(N)
+    src[order[:sources]] = power_law_luminosities(sources, 1e48, 1e52, 1342) * escape_fraction
+    params = SweepParameters(
+        directions=num_directions, num_timestep_levels=levels, periodic=periodic,
+        max_timestep=1.0 * U.MEGAYEARS, significant_rate_threshold=threshold,
+        timestep_safety_factor=timestep_safety_factor, transport_precision=precision,
+    )
+    return Problem(name or f"clustered{n}", grid, params, rho, np.full(N, 1e-10), np.full(N, 1e4), src, steps)
+
+
+def clustered_2m(**kw) -> Problem:
+    return clustered_problem(2097152, name="clustered_2m", **kw)
 
 
 def cartesian_problem(n: int, num_directions: int, levels: int = 1, periodic: bool = False,
