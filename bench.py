@@ -33,6 +33,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+KERNELS = {3: "k_sweep_rec (geometric replay: one launch per Kahn level set, all directions, 16 lanes per task)",
+           4: "k_sweep_comp (compiled replay: one launch per Kahn level set, all directions, one lane per task)"}
 
 
 def log(*a):
@@ -47,10 +49,10 @@ def bytes_per_update(info, precision: int) -> float:
     return base + info.downwind_local * bl + info.downwind_periodic * bp + info.downwind_boundary * bb
 
 
-def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3):
+def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3, mode: int = 0):
     from subsweep_amd import Sweep
     pp = pb.with_params(transport_precision=precision)
-    s = Sweep(pp.params, pp.grid, *pp.state())
+    s = Sweep(pp.params, pp.grid, *pp.state(), transport_mode=mode)
     for _ in range(warmup):
         s.run_sweeps()
     s.synchronize()
@@ -74,6 +76,7 @@ def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3)
         prof_updates += s.info().updates_last_step
     s.set_timing(False)
     info = s.info()
+    res = s.results()
     s.close()
     elapsed = t1 - t0
     bpu = bytes_per_update(info, precision)
@@ -89,7 +92,58 @@ def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3)
         "kernel_ms_per_step": kernel_ms / max(1, profile_steps),
         "bytes_per_update": bpu,
         "achieved_gbs": achieved,
+        "kernel": KERNELS.get(info.transport_mode, "?"),
+        "build_ms_per_step": None,
         "info": info,
+        "results": res,
+    }
+
+
+def field_errors(a, b):
+    """Relative error of each output field of `a` against `b` (max and 99th
+    percentile over cells). Photon rates are compared where b's rate exceeds
+    1e-6 of its maximum (relative errors of rates 1e6 below the peak, deep in
+    shadow, are rounding noise in both precisions)."""
+    import numpy as np
+    out = {}
+    for name, u, v in (("x_HII", a.ionized_hydrogen_fraction, b.ionized_hydrogen_fraction),
+                       ("temperature", a.temperature, b.temperature),
+                       ("photon_rate", a.photon_rate, b.photon_rate)):
+        keep = np.ones(v.shape, bool) if name != "photon_rate" else np.abs(v) > 1e-6 * np.abs(v).max()
+        rel = np.abs(u[keep] - v[keep]) / np.maximum(np.abs(v[keep]), 1e-300)
+        out[name] = {"max": float(rel.max()) if rel.size else 0.0,
+                     "p99": float(np.percentile(rel, 99)) if rel.size else 0.0}
+    out["timestep_levels_equal"] = float(np.mean(a.level == b.level))
+    return out
+
+
+def run_clustered(steps: int = 4, warmup: int = 2):
+    """2097152-cell clustered grid, 84 directions, 4 timestep levels, 1024
+    sources: FP32 transport vs FP64 transport (FP64 chemistry in both). The
+    warmup steps are the two first steps, which sweep only the top level."""
+    from subsweep_amd import workloads as W
+    t = time.perf_counter()
+    pb = W.clustered_2m()
+    log(f"clustered grid: {pb.grid.num_cells} cells, {pb.grid.num_faces} faces, built in "
+        f"{time.perf_counter() - t:.1f} s")
+    r32 = run_gpu(pb, 32, steps, warmup, profile_steps=1)
+    log(f"clustered fp32: {r32['updates_per_s'] / 1e9:.3f} G updates/s, {r32['ms_per_step']:.1f} ms/step")
+    r64 = run_gpu(pb, 64, steps, warmup, profile_steps=1)
+    log(f"clustered fp64: {r64['updates_per_s'] / 1e9:.3f} G updates/s, {r64['ms_per_step']:.1f} ms/step")
+    return {
+        "workload": "2097152-cell clustered lognormal-field Voronoi grid (sigma_ln 2), 84 directions, "
+                    "4 timestep levels, 1024 sources, escape fraction 0.01",
+        "steps": steps,
+        "warmup": warmup,
+        "dag_depth": int(r32["info"].max_depth),
+        "fp32_updates_per_s": r32["updates_per_s"],
+        "fp64_updates_per_s": r64["updates_per_s"],
+        "fp32_over_fp64": r32["updates_per_s"] / r64["updates_per_s"],
+        "fp32_ms_per_step": r32["ms_per_step"],
+        "fp64_ms_per_step": r64["ms_per_step"],
+        "fp32_transport_mode": int(r32["info"].transport_mode),
+        "fp64_transport_mode": int(r64["info"].transport_mode),
+        "fp32_error_vs_fp64": field_errors(r32["results"], r64["results"]),
     }
 
 
@@ -105,6 +159,110 @@ def run_cpu_baseline(pb, budget_s: float = 30.0):
             "sample": f"1 step ({upd} cell-direction updates) of the same workload, FP64, single thread"}
 
 
+def serving_pool(num_grids: int, seed: int):
+    """Grids of the serving workload: sizes log-uniform in [8192, 524288] cells,
+    alternately jittered-lattice Voronoi (n^3 cells, n = round(cbrt(size)))
+    and clustered lognormal-field Voronoi."""
+    import numpy as np
+    from subsweep_amd import grid as G
+    from subsweep_amd import workloads as W
+    rng = np.random.default_rng(seed)
+    pool = []
+    for i in range(num_grids):
+        size = int(np.exp(rng.uniform(np.log(8192), np.log(524288))))
+        if i % 2 == 0:
+            n = max(2, int(round(size ** (1.0 / 3.0))))
+            pts = G.jittered_lattice(n, W.BOX, 0.3, seed + 10 + i)
+            delta = None
+        else:
+            pts, delta = G.clustered_points(size, W.BOX, mesh=64, sigma_ln=2.0, seed=seed + 10 + i)
+        pool.append((G.voronoi_grid(pts, W.BOX), pts, delta))
+    return pool
+
+
+def serving_requests(pool, count: int, seed: int, steps: int):
+    """Request mix: a pool grid, directions from {16, 32, 64, 84}, timestep
+    levels from {1, 2, 4}, 1-64 sources of 1e50 photons/s at uniform positions,
+    lognormal gas (sigma_ln 1) on jittered grids and the field's density on
+    clustered grids, FP32 transport, `steps` steps."""
+    import numpy as np
+    from subsweep_amd import SweepParameters
+    from subsweep_amd import grid as G
+    from subsweep_amd import units as U
+    from subsweep_amd import workloads as W
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        grid, pts, delta = pool[int(rng.integers(len(pool)))]
+        N = grid.num_cells
+        dirs = int(rng.choice([16, 32, 64, 84]))
+        levels = int(rng.choice([1, 2, 4]))
+        ns = int(rng.integers(1, 65))
+        src, _ = G.assign_sources(pts, rng.uniform(0.0, W.BOX, size=(ns, 3)), np.full(ns, 1e50))
+        rho = W.RHO_MEAN * delta if delta is not None else G.lognormal_density(N, W.RHO_MEAN, 1.0, seed + 1000 + k)
+        params = SweepParameters(directions=dirs, num_timestep_levels=levels, periodic=True,
+                                 max_timestep=1.0 * U.MEGAYEARS, transport_precision=32)
+        out.append(W.Problem(f"request{k}", grid, params, rho, np.full(N, 1e-10), np.full(N, 1e4), src, steps))
+    return out
+
+
+def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, steps: int = 10):
+    import threading
+    import numpy as np
+    from subsweep_amd.serve import Server
+    t = time.perf_counter()
+    pool = serving_pool(8, seed)
+    reqs = serving_requests(pool, count, seed + 1, steps)
+    log(f"serving: {len(pool)} grids ({', '.join(str(g.num_cells) for g, _, _ in pool)} cells), "
+        f"{count} requests built in {time.perf_counter() - t:.1f} s")
+    budget = int(budget_gb * (1 << 30))
+    # capacity: a saturated batch of 8 requests on the same server shape
+    with Server(slots=slots, hbm_budget_bytes=budget) as sv:
+        t0 = time.perf_counter()
+        cal = [sv.wait(x) for x in [sv.submit(r, steps) for r in reqs[:8]]]
+        capacity = len(cal) / (time.perf_counter() - t0)
+    rate = 0.8 * capacity  # Poisson arrivals at 80 % of the measured capacity
+    gaps = np.random.default_rng(seed + 2).exponential(1.0 / rate, size=count)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    with Server(slots=slots, hbm_budget_bytes=budget) as sv:
+        tickets = [None] * count
+        start = time.perf_counter()
+
+        def feeder():
+            for i, r in enumerate(reqs):
+                delay = arrivals[i] - (time.perf_counter() - start)
+                if delay > 0:
+                    time.sleep(delay)
+                tickets[i] = sv.submit(r, steps)
+
+        th = threading.Thread(target=feeder)
+        th.start()
+        th.join()
+        res = [sv.wait(x) for x in tickets]
+        stats = sv.stats()
+    bad = [r for r in res if r.status != 0]
+    if bad:
+        raise RuntimeError(f"{len(bad)} serving requests failed: {bad[0].error}")
+    lat = np.array([r.latency_s for r in res])
+    t_first = min(r.submit_s for r in res)
+    t_last = max(r.end_s for r in res)
+    updates = sum(r.updates for r in res)
+    return {
+        "requests": count,
+        "slots": slots,
+        "hbm_budget_gb": budget_gb,
+        "steps_per_request": steps,
+        "arrival_rate_per_s": rate,
+        "arrival_rule": "Poisson at 0.8 x the completion rate of a saturated 8-request batch",
+        "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
+        "mean_service_ms": float(np.mean([r.end_s - r.start_s for r in res]) * 1e3),
+        "updates_per_s": updates / (t_last - t_first),
+        "peak_running": stats["peak_running"],
+        "peak_reserved_gb": stats["peak_reserved_bytes"] / (1 << 30),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +273,13 @@ def main():
                     help="only run a plain device copy of MIB mebibytes (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--precision", type=int, default=0, choices=(0, 32, 64),
                     help="run only this transport precision (0: both)")
+    ap.add_argument("--serving-requests", type=int, default=256,
+                    help="requests of the serving-loop measurement (0: skip it)")
+    ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--no-clustered", action="store_true",
+                    help="skip the 2097152-cell clustered FP32-vs-FP64 measurement")
+    ap.add_argument("--mode", type=int, default=0, choices=(0, 3, 4),
+                    help="transport replay form: 0 automatic, 3 geometric, 4 compiled")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -137,11 +302,11 @@ def main():
     log(f"grid: {pb.grid.num_cells} cells, {pb.grid.num_faces} faces, built in {time.perf_counter() - t:.1f} s")
 
     if args.precision in (0, 32):
-        fp32 = run_gpu(pb, 32, args.steps, args.warmup)
+        fp32 = run_gpu(pb, 32, args.steps, args.warmup, mode=args.mode)
         log(f"fp32 transport: {fp32['updates_per_s'] / 1e9:.3f} G updates/s, {fp32['ms_per_step']:.2f} ms/step, "
             f"kernel {fp32['avg_launch_ms']:.3f} ms/launch, {fp32['achieved_gbs']:.0f} GB/s")
     if args.precision in (0, 64):
-        fp64 = run_gpu(pb, 64, args.steps, args.warmup)
+        fp64 = run_gpu(pb, 64, args.steps, args.warmup, mode=args.mode)
         log(f"fp64 transport: {fp64['updates_per_s'] / 1e9:.3f} G updates/s, {fp64['ms_per_step']:.2f} ms/step, "
             f"kernel {fp64['avg_launch_ms']:.3f} ms/launch, {fp64['achieved_gbs']:.0f} GB/s")
     if args.precision != 0:  # profiling runs: no JSON line
@@ -154,6 +319,16 @@ def main():
             traffic = json.load(open(tpath)).get("fp32_bytes_per_launch")
         except Exception:
             traffic = None
+
+    clustered = None
+    if not args.no_clustered:
+        clustered = run_clustered()
+
+    serving = None
+    if args.serving_requests > 0:
+        serving = run_serving(args.serving_requests, args.serving_slots, 200.0)
+        log(f"serving: p50 {serving['p50_latency_ms']:.1f} ms, p99 {serving['p99_latency_ms']:.1f} ms, "
+            f"{serving['updates_per_s'] / 1e9:.3f} G updates/s, peak {serving['peak_running']} running")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -195,7 +370,7 @@ def main():
             "unit": "GB/s",
             "frac": fp32["achieved_gbs"] / PEAK_HBM_GBS,
             "traffic": traffic,
-            "kernel": "k_sweep_rec (one launch per Kahn level set, all directions, graph replay)",
+            "kernel": fp32["kernel"],
             "avg_launch_ms": fp32["avg_launch_ms"],
             "launches_per_step": fp32["launches_per_step"],
             "kernel_ms_per_step": fp32["kernel_ms_per_step"],
@@ -204,6 +379,9 @@ def main():
             "fp64_bytes_per_update": fp64["bytes_per_update"],
         },
         "cpu_baseline": cpu,
+        "fp32_error_vs_fp64": field_errors(fp32["results"], fp64["results"]),
+        "clustered_2m": clustered,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 

@@ -142,24 +142,47 @@ typedef struct ssw_info {
     double downwind_boundary;
     int64_t device_bytes;          /* device memory held by this handle           */
     int64_t chemistry_failures;    /* cells whose chemistry hit the depth limit   */
+    int32_t transport_mode;        /* SSW_TRANSPORT_MODE_GEOMETRIC / _COMPILED    */
+    int32_t schedule_builds_last_step; /* sweeps of the last step that rebuilt
+                                          their level sets                      */
+    int64_t overflow_entries;      /* compiled upwind entries beyond the planes   */
+    int64_t graph_capture_failures; /* replays launched one by one because their
+                                       graph could not be captured            */
 } ssw_info;
 
 int ssw_get_info(ssw_sweep* s, ssw_info* info);
 
-/* Transport kernel selection (no counterpart in the reference, which has one
- * CPU code path). 0 (the only mode): one launch per Kahn level set, replayed
- * as a captured graph. Any other value returns SSW_ERR_INVALID: the
- * persistent one-launch-per-sweep kernel that modes 1 and 2 selected did not
- * terminate on MI355X and has been removed. */
+/* Replay form of the transport (no counterpart in the reference, which has one
+ * CPU code path); set before ssw_upload_grid. Both forms solve the same level
+ * sets with the same arithmetic and give bitwise the same results.
+ *  SSW_TRANSPORT_MODE_GEOMETRIC  task records (16 B per cell-direction and
+ *      timestep level); every replay re-reads the face geometry.
+ *  SSW_TRANSPORT_MODE_COMPILED   per task its upwind (neighbour, weight) list
+ *      in level-position order (about 16 + 4 + 12 x 8 B per cell-direction and
+ *      timestep level in FP32, twice the entry size in FP64), one lane per task.
+ *  SSW_TRANSPORT_MODE_AUTO       compiled when its arrays fit in half of the
+ *      free device memory at ssw_upload_grid, else geometric (default).
+ * Values 1 and 2 selected a persistent one-launch-per-sweep kernel that did
+ * not terminate on MI355X; it has been removed and they return SSW_ERR_INVALID. */
+#define SSW_TRANSPORT_MODE_AUTO 0
+#define SSW_TRANSPORT_MODE_GEOMETRIC 3
+#define SSW_TRANSPORT_MODE_COMPILED 4
 int ssw_set_transport_mode(ssw_sweep* s, int32_t mode);
 
-/* Kernel timing of the last `ssw_step` when timing is enabled: average duration
- * in ms of each launch of the transport kernel, total ms of the transport
- * kernels, and total ms of the chemistry kernels, from HIP events recorded on
- * the handle's stream. */
+/* Raw transport state in its storage precision and scale (FP32 or FP64 per
+ * the transport precision, direction-major): X = outgoing rate / downwind
+ * effective area and I = incoming rate. For bitwise comparisons. */
+int ssw_download_transport(ssw_sweep* s, void* X, void* I);
+
+/* Kernel timing of the last `ssw_step` when timing is enabled, from HIP events
+ * recorded on the handle's stream: average duration in ms of each replay
+ * launch of the transport kernel (one per level set), their total, the total
+ * of the chemistry kernels, and the total of the schedule builds (counting
+ * plus the level-by-level build-and-solve launches of sweeps whose active set
+ * changed). */
 int ssw_set_timing(ssw_sweep* s, int32_t enabled);
 int ssw_get_timing(ssw_sweep* s, double* transport_avg_ms, double* transport_total_ms,
-                   int64_t* transport_launches, double* chemistry_total_ms);
+                   int64_t* transport_launches, double* chemistry_total_ms, double* build_total_ms);
 
 /* Host-side helpers, the same code the device kernels run.
  * ssw_timestep_level: TimestepLevel::from_max_timestep_and_desired_timestep
@@ -173,6 +196,73 @@ int32_t ssw_timestep_level(int32_t num_levels, double max_timestep, double desir
 double ssw_chemistry_timestep(double* state, double density, double volume, double length, double rate,
                               double scale_factor, int32_t prevent_cooling, double timestep, double safety,
                               int32_t* process, int32_t* failed);
+
+/* Device memory a handle would hold for a problem of num_cells cells,
+ * num_faces directed faces and num_periodic_cells cells with a periodic face,
+ * in the given transport mode (GEOMETRIC or COMPILED): an upper estimate used
+ * for admission against an HBM budget. No counterpart in the reference. */
+int64_t ssw_estimate_device_bytes(const ssw_sweep_parameters* params, int64_t num_cells, int64_t num_faces,
+                                  int64_t num_periodic_cells, int32_t transport_mode);
+
+/* Request loop (no counterpart in the reference, which runs one simulation
+ * per process): a server with `slots` worker threads runs independent solves
+ * concurrently, each in its own handle and HIP stream, so their kernels
+ * overlap on the device. Requests are admitted in submission order when a
+ * worker is free and their device-memory estimate fits in what is left of
+ * the HBM budget; a request above the whole budget fails with
+ * SSW_ERR_BUDGET. Each request runs create, upload grid and state, `steps`
+ * steps and a download into its output buffers (any may be NULL), then frees
+ * its device memory. Input and output buffers must stay valid until
+ * ssw_server_wait returns for the request. Results do not depend on what else
+ * runs: every kernel's arithmetic is independent of scheduling. */
+typedef struct ssw_server ssw_server;
+
+typedef struct ssw_request {
+    ssw_sweep_parameters params;
+    int32_t transport_mode;        /* SSW_TRANSPORT_MODE_*; AUTO: compiled if it
+                                      fits budget / slots, else geometric      */
+    int32_t steps;                 /* ssw_step calls                          */
+    int64_t num_cells;             /* grid, as for ssw_upload_grid            */
+    const int64_t* face_offsets;
+    const int32_t* face_neighbour;
+    const uint8_t* face_kind;
+    const double* face_area;
+    const double* face_normal;
+    const double* cell_volume;
+    const double* cell_size;
+    const double* density;         /* state, as for ssw_upload_state          */
+    const double* ionized_fraction;
+    const double* temperature;
+    const double* source;
+    double* out_ionized_fraction;  /* results, as for ssw_download            */
+    double* out_temperature;
+    double* out_timestep;
+    double* out_photon_rate;
+    uint8_t* out_level;
+    double* out_change_timescale;
+} ssw_request;
+
+typedef struct ssw_request_result {
+    int32_t status;                /* SSW_OK or the failing call's SSW_ERR_*  */
+    int32_t transport_mode;        /* mode the request ran in                 */
+    double submit_s;               /* seconds since server creation           */
+    double start_s;                /* admitted                                */
+    double end_s;                  /* results downloaded, memory released     */
+    int64_t updates;               /* cell-direction updates of all steps     */
+    int64_t device_bytes;          /* device memory its handle held           */
+    int64_t estimated_bytes;       /* the estimate it was admitted with       */
+    char error[256];
+} ssw_request_result;
+
+int ssw_server_create(int32_t slots, int64_t hbm_budget_bytes, ssw_server** out);
+/* Queues a copy of *request (not of its arrays); *ticket identifies it. */
+int ssw_server_submit(ssw_server* server, const ssw_request* request, int64_t* ticket);
+/* Blocks until the request has finished; each ticket can be waited on once. */
+int ssw_server_wait(ssw_server* server, int64_t ticket, ssw_request_result* result);
+/* Highest reserved budget and highest number of concurrently running requests so far. */
+int ssw_server_stats(ssw_server* server, int64_t* peak_reserved_bytes, int32_t* peak_running);
+/* Runs every queued request to completion, then stops the workers. */
+void ssw_server_destroy(ssw_server* server);
 
 /* Releases the handle and its device memory. */
 void ssw_destroy(ssw_sweep* s);

@@ -44,6 +44,11 @@ class SweepParametersC(C.Structure):
     ]
 
 
+TRANSPORT_MODE_AUTO = 0
+TRANSPORT_MODE_GEOMETRIC = 3
+TRANSPORT_MODE_COMPILED = 4
+
+
 class InfoC(C.Structure):
     _fields_ = [
         ("num_cells", C.c_int64),
@@ -57,6 +62,10 @@ class InfoC(C.Structure):
         ("downwind_boundary", C.c_double),
         ("device_bytes", C.c_int64),
         ("chemistry_failures", C.c_int64),
+        ("transport_mode", C.c_int32),
+        ("schedule_builds_last_step", C.c_int32),
+        ("overflow_entries", C.c_int64),
+        ("graph_capture_failures", C.c_int64),
     ]
 
 
@@ -64,6 +73,46 @@ class SweepError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"subsweep_amd error {code}: {message}")
         self.code = code
+
+
+class RequestC(C.Structure):
+    _fields_ = [
+        ("params", SweepParametersC),
+        ("transport_mode", C.c_int32),
+        ("steps", C.c_int32),
+        ("num_cells", C.c_int64),
+        ("face_offsets", C.c_void_p),
+        ("face_neighbour", C.c_void_p),
+        ("face_kind", C.c_void_p),
+        ("face_area", C.c_void_p),
+        ("face_normal", C.c_void_p),
+        ("cell_volume", C.c_void_p),
+        ("cell_size", C.c_void_p),
+        ("density", C.c_void_p),
+        ("ionized_fraction", C.c_void_p),
+        ("temperature", C.c_void_p),
+        ("source", C.c_void_p),
+        ("out_ionized_fraction", C.c_void_p),
+        ("out_temperature", C.c_void_p),
+        ("out_timestep", C.c_void_p),
+        ("out_photon_rate", C.c_void_p),
+        ("out_level", C.c_void_p),
+        ("out_change_timescale", C.c_void_p),
+    ]
+
+
+class RequestResultC(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32),
+        ("transport_mode", C.c_int32),
+        ("submit_s", C.c_double),
+        ("start_s", C.c_double),
+        ("end_s", C.c_double),
+        ("updates", C.c_int64),
+        ("device_bytes", C.c_int64),
+        ("estimated_bytes", C.c_int64),
+        ("error", C.c_char * 256),
+    ]
 
 
 P = C.c_void_p
@@ -88,12 +137,20 @@ SWEEP_SIGNATURES = {
     "ssw_synchronize": (C.c_int, [P]),
     "ssw_set_transport_mode": (C.c_int, [P, C.c_int32]),
     "ssw_set_timing": (C.c_int, [P, C.c_int32]),
-    "ssw_get_timing": (C.c_int, [P, PD, PD, PI64, PD]),
+    "ssw_get_timing": (C.c_int, [P, PD, PD, PI64, PD, PD]),
+    "ssw_download_transport": (C.c_int, [P, P, P]),
     "ssw_timestep_level": (C.c_int32, [C.c_int32, C.c_double, C.c_double]),
     "ssw_chemistry_timestep": (C.c_double, [PD, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                             C.c_int32, C.c_double, C.c_double, PI32, PI32]),
     "ssw_destroy": (None, [P]),
     "ssw_last_error": (C.c_char_p, []),
+    "ssw_estimate_device_bytes": (C.c_int64, [C.POINTER(SweepParametersC), C.c_int64, C.c_int64, C.c_int64,
+                                              C.c_int32]),
+    "ssw_server_create": (C.c_int, [C.c_int32, C.c_int64, C.POINTER(P)]),
+    "ssw_server_submit": (C.c_int, [P, C.POINTER(RequestC), PI64]),
+    "ssw_server_wait": (C.c_int, [P, C.c_int64, C.POINTER(RequestResultC)]),
+    "ssw_server_stats": (C.c_int, [P, PI64, PI32]),
+    "ssw_server_destroy": (None, [P]),
 }
 
 GRID_SIGNATURES = {

@@ -42,8 +42,18 @@ def _run(cmd: list[str]) -> None:
 
 def build_sweep() -> str:
     src = os.path.join(CSRC, "sweep.hip")
+    server = os.path.join(CSRC, "server.cpp")
+    sweep_obj = os.path.join(CSRC, "sweep.o")
+    server_obj = os.path.join(CSRC, "server.o")
     tmp = SWEEP_LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, src, "-o", tmp])
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    _run([HIPCC, *compile_flags, "-c", src, "-o", sweep_obj])
+    # the request loop is host-only C++
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-pthread", "-I", os.path.join(ROOT, "include"), "-c", server,
+          "-o", server_obj])
+    _run([HIPCC, "-shared", "-fPIC", sweep_obj, server_obj, "-lpthread", "-o", tmp])
+    for o in (sweep_obj, server_obj):
+        os.remove(o)
     os.replace(tmp, SWEEP_LIB)
     return SWEEP_LIB
 

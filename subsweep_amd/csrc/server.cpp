@@ -1,0 +1,237 @@
+// Request loop over libsubsweep_amd handles (host C++, no device code).
+//
+// A server owns `slots` worker threads. Each admitted request gets its own
+// handle, and with it its own HIP stream and device memory, so requests that
+// run at the same time overlap on the device. Requests are admitted in arrival
+// order, each when a worker is free and the device-memory estimate of the
+// request fits in what is left of the server's HBM budget; a request whose
+// estimate exceeds the whole budget fails with SSW_ERR_BUDGET. A request runs
+// create -> upload grid -> upload state -> `steps` x step -> download ->
+// destroy, so its grid and flux state stay resident on the device for the
+// whole solve, and its memory returns to the budget when it completes.
+//
+// The reference has no counterpart (it runs one simulation per process); the
+// serving loop is how this library is driven with many independent solves.
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/subsweep_amd.h"
+
+namespace {
+
+struct Req {
+    ssw_request r;
+    int64_t ticket = 0;
+    int64_t estimate = 0;
+    int32_t mode = SSW_TRANSPORT_MODE_GEOMETRIC;
+    double t_submit = 0, t_start = 0, t_end = 0;
+    int32_t status = SSW_OK;
+    std::string error;
+    int64_t updates = 0;
+    int64_t device_bytes = 0;
+    bool done = false;
+};
+
+double now_s(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+struct ssw_server {
+    int slots = 1;
+    int64_t budget = 0;
+    std::chrono::steady_clock::time_point t0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Req*> queue;
+    std::unordered_map<int64_t, std::unique_ptr<Req>> reqs;
+    int64_t next_ticket = 1;
+    int64_t reserved = 0;
+    int64_t peak_reserved = 0;
+    int running = 0;
+    int peak_running = 0;
+    bool stop = false;
+    std::vector<std::thread> workers;
+};
+
+namespace {
+
+void run_request(Req* q) {
+    ssw_sweep* s = nullptr;
+    const ssw_request& r = q->r;
+    auto fail_with = [&](int code) {
+        q->status = code;
+        q->error = ssw_last_error();
+    };
+    int rc = ssw_create(&r.params, &s);
+    if (rc != SSW_OK) {
+        fail_with(rc);
+        return;
+    }
+    rc = ssw_set_transport_mode(s, q->mode);
+    if (rc == SSW_OK)
+        rc = ssw_upload_grid(s, r.num_cells, r.face_offsets, r.face_neighbour, r.face_kind, r.face_area, r.face_normal,
+                             r.cell_volume, r.cell_size);
+    if (rc == SSW_OK) rc = ssw_upload_state(s, r.density, r.ionized_fraction, r.temperature, r.source);
+    ssw_info info;
+    for (int32_t k = 0; rc == SSW_OK && k < r.steps; k++) {
+        double el = 0.0;
+        rc = ssw_step(s, &el);
+        if (rc == SSW_OK) rc = ssw_get_info(s, &info);
+        if (rc == SSW_OK) q->updates += info.updates_last_step;
+    }
+    if (rc == SSW_OK)
+        rc = ssw_download(s, r.out_ionized_fraction, r.out_temperature, r.out_timestep, r.out_photon_rate, r.out_level,
+                          r.out_change_timescale);
+    if (rc == SSW_OK) rc = ssw_get_info(s, &info);
+    if (rc == SSW_OK) q->device_bytes = info.device_bytes;
+    if (rc != SSW_OK) fail_with(rc);
+    ssw_destroy(s);
+}
+
+void worker(ssw_server* sv) {
+    for (;;) {
+        Req* q = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(sv->mu);
+            for (;;) {
+                if (sv->stop && sv->queue.empty()) return;
+                if (!sv->queue.empty()) {
+                    Req* head = sv->queue.front();
+                    if (head->estimate > sv->budget) {  // can never fit
+                        sv->queue.pop_front();
+                        head->status = SSW_ERR_BUDGET;
+                        head->error = "request needs an estimated " + std::to_string(head->estimate) +
+                                      " bytes of device memory, above the budget of " + std::to_string(sv->budget);
+                        head->t_start = head->t_end = now_s(sv->t0);
+                        head->done = true;
+                        sv->cv_done.notify_all();
+                        continue;
+                    }
+                    if (sv->reserved + head->estimate <= sv->budget) {
+                        sv->queue.pop_front();
+                        q = head;
+                        sv->reserved += q->estimate;
+                        sv->running++;
+                        sv->peak_reserved = std::max(sv->peak_reserved, sv->reserved);
+                        sv->peak_running = std::max(sv->peak_running, sv->running);
+                        q->t_start = now_s(sv->t0);
+                        break;
+                    }
+                }
+                sv->cv_work.wait(lk);
+            }
+        }
+        run_request(q);
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            q->t_end = now_s(sv->t0);
+            q->done = true;
+            sv->reserved -= q->estimate;
+            sv->running--;
+        }
+        sv->cv_done.notify_all();
+        sv->cv_work.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssw_server_create(int32_t slots, int64_t hbm_budget_bytes, ssw_server** out) {
+    if (!out || slots < 1 || slots > 64 || hbm_budget_bytes <= 0) return SSW_ERR_INVALID;
+    ssw_server* sv = new ssw_server();
+    sv->slots = slots;
+    sv->budget = hbm_budget_bytes;
+    sv->t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < slots; i++) sv->workers.emplace_back(worker, sv);
+    *out = sv;
+    return SSW_OK;
+}
+
+int ssw_server_submit(ssw_server* sv, const ssw_request* r, int64_t* ticket) {
+    if (!sv || !r || !ticket || r->num_cells <= 0 || r->steps < 0) return SSW_ERR_INVALID;
+    auto q = std::make_unique<Req>();
+    q->r = *r;
+    const int64_t F = r->face_offsets[r->num_cells];
+    int64_t nper = 0;  // cells with a periodic face (periodic sweeps only)
+    if (r->params.periodic)
+        for (int64_t c = 0; c < r->num_cells; c++)
+            for (int64_t f = r->face_offsets[c]; f < r->face_offsets[c + 1]; f++)
+                if (r->face_kind[f] == SSW_FACE_PERIODIC) {
+                    nper++;
+                    break;
+                }
+    q->mode = r->transport_mode;
+    if (q->mode == SSW_TRANSPORT_MODE_AUTO) {
+        // compiled when it fits the budget shared by the slots, else geometric
+        const int64_t comp = ssw_estimate_device_bytes(&r->params, r->num_cells, F, nper, SSW_TRANSPORT_MODE_COMPILED);
+        q->mode = comp <= sv->budget / sv->slots ? SSW_TRANSPORT_MODE_COMPILED : SSW_TRANSPORT_MODE_GEOMETRIC;
+    }
+    q->estimate = ssw_estimate_device_bytes(&r->params, r->num_cells, F, nper, q->mode);
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (sv->stop) return SSW_ERR_INVALID;
+        q->ticket = sv->next_ticket++;
+        q->t_submit = now_s(sv->t0);
+        *ticket = q->ticket;
+        sv->queue.push_back(q.get());
+        sv->reqs[q->ticket] = std::move(q);
+    }
+    sv->cv_work.notify_all();
+    return SSW_OK;
+}
+
+int ssw_server_wait(ssw_server* sv, int64_t ticket, ssw_request_result* res) {
+    if (!sv || !res) return SSW_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end()) return SSW_ERR_INVALID;
+    Req* q = it->second.get();
+    sv->cv_done.wait(lk, [&] { return q->done; });
+    std::memset(res, 0, sizeof(*res));
+    res->status = q->status;
+    res->transport_mode = q->mode;
+    res->submit_s = q->t_submit;
+    res->start_s = q->t_start;
+    res->end_s = q->t_end;
+    res->updates = q->updates;
+    res->device_bytes = q->device_bytes;
+    res->estimated_bytes = q->estimate;
+    std::strncpy(res->error, q->error.c_str(), sizeof(res->error) - 1);
+    sv->reqs.erase(it);
+    return SSW_OK;
+}
+
+int ssw_server_stats(ssw_server* sv, int64_t* peak_reserved_bytes, int32_t* peak_running) {
+    if (!sv) return SSW_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (peak_reserved_bytes) *peak_reserved_bytes = sv->peak_reserved;
+    if (peak_running) *peak_running = sv->peak_running;
+    return SSW_OK;
+}
+
+void ssw_server_destroy(ssw_server* sv) {
+    if (!sv) return;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv_work.notify_all();
+    for (std::thread& t : sv->workers) t.join();
+    delete sv;
+}
+
+}  // extern "C"

@@ -11,9 +11,19 @@
 //              I = incoming rate at solve time (FP64 or scaled FP32)
 //              P[d * Nper + slot] periodic-source snapshot of the periodic cells
 //   schedule   per sweep level l (the active set {level >= l}): the (cell,
-//              direction) tasks in Kahn order, level-major, as records
-//              {cell | face_count << 25, first_face, direction, 0}; lstart[l][k]
-//              is where level set k begins.
+//              direction) tasks in Kahn order, level-major; lstart[l][k] is where
+//              level set k begins. Two replay forms:
+//              geometric  task records {cell | face_count << 25, first_face,
+//                         direction, periodic slot} (16 B per task and sweep level)
+//              compiled   per task a header {cell | direction << 25, periodic
+//                         slot, overflow start, upwind count}, the effective area
+//                         A, and its upwind local faces as (neighbour, weight)
+//                         pairs in kPlanes planes: entry q of the task at
+//                         position i of a level set starting at b with width W
+//                         is planes[kPlanes * b + q * W + (i - b)] (coalesced
+//                         across tasks, addressable without reading the header);
+//                         entries q >= kPlanes (rare) sit in a shared overflow
+//                         list, indexed per (direction, cell).
 //
 // The transport is the reference's upwind sweep (src/sweep/mod.rs:412-513)
 // restated in pull form. A cell that is solved distributes the change of its
@@ -22,14 +32,16 @@
 //   I[c,d] = sum_{upwind local faces f} area_f (-n_f . d) * O[nb,d] / A[nb,d]
 // with A[nb,d] = sum of area (n . d) over nb's downwind faces. Storing
 // X = O / A lets every task gather its upwind neighbours' X instead of
-// scattering with atomics; the result depends only on the face order, not on
-// the order in which tasks run. Tasks are run level set by level set: level k
-// holds the (cell, direction) pairs whose active upwind neighbours are all in
-// levels < k (Kahn layering of init_counts' counts, src/sweep/mod.rs:346-386).
-// The level sets of an active set are built on the device by the first sweep
-// that needs them (counter decrements, ballot-aggregated appends) and replayed
-// by later sweeps, as one captured HIP graph of exact-size launches, until a
-// timestep-level change alters the active set.
+// scattering with atomics. The upwind sum is evaluated in face order, one term
+// after the other, in every kernel, so the build, the geometric replay and the
+// compiled replay give bitwise the same result, whatever order tasks run in.
+// Tasks are run level set by level set: level k holds the (cell, direction)
+// pairs whose active upwind neighbours are all in levels < k (Kahn layering of
+// init_counts' counts, src/sweep/mod.rs:346-386). The level sets of an active
+// set are built on the device by the first sweep that needs them (counter
+// decrements, block-aggregated appends) and replayed by later sweeps, as one
+// captured HIP graph of exact-size launches, until a timestep-level change
+// alters the active set.
 
 #include <hip/hip_runtime.h>
 
@@ -40,6 +52,7 @@
 #include <cstring>
 #include <functional>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/subsweep_amd.h"
@@ -55,10 +68,14 @@ constexpr uint32_t kKindPeriodic = 1;
 constexpr uint32_t kKindBoundary = 2;
 constexpr uint32_t kNbrMask = 0x3FFFFFFFu;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr int kLanes = 16;                 // lanes per task in the transport kernels
+constexpr int kLanes = 16;                 // lanes per task in the build / geometric kernels
 constexpr int kTasksPerBlock = kBlock / kLanes;
 constexpr uint32_t kCellBits = 25;         // task record: cell index bits
 constexpr uint32_t kCellMask = (1u << kCellBits) - 1u;
+constexpr int kMaxFaces = 127;             // faces per cell
+constexpr int kQueueCap = kTasksPerBlock * kMaxFaces;
+constexpr int kPlanes = 12;                // upwind entries per task held in planes
+constexpr int kRecBlock = 64;              // compiled replay: one lane per task
 
 thread_local std::string g_err;
 
@@ -89,6 +106,21 @@ __device__ __forceinline__ double fdot(const double4& g, double dx, double dy, d
     return (g.x * dx + g.y * dy) + g.z * dz;
 }
 
+// Upwind entry of the compiled schedule.
+template <typename R>
+struct Entry;
+template <>
+struct __align__(8) Entry<float> {
+    uint32_t nb;
+    float w;
+};
+template <>
+struct __align__(16) Entry<double> {
+    uint32_t nb;
+    uint32_t pad;
+    double w;
+};
+
 struct Geo {
     int64_t N;
     int D;
@@ -104,7 +136,8 @@ struct Geo {
     const uint32_t* pfoff;   // Nper + 1: periodic faces of each periodic cell
     const uint32_t* pfidx;   // face indices
     int64_t Nper;
-    uint4* flat;             // L x (D * N) task records, level-major
+    uint4* flat;             // task records: per slot (geometric) or one slot of scratch (compiled)
+    size_t flat_slot_stride; // D * N (geometric) or 0 (compiled)
     uint32_t* lstart;        // L x (Kcap + 2)
     uint32_t* ctr;           // L x 3 rotating append counters
     uint32_t* cnt;           // D x N missing-upwind counters
@@ -123,12 +156,22 @@ struct Flux {
     double inv_aref;         // storage scale of areas
 };
 
-__device__ __forceinline__ uint4 task_record(const Geo& g, uint32_t c, uint32_t d) {
-    const uint32_t f0 = g.off[c], f1 = g.off[c + 1];
-    return make_uint4(c | ((f1 - f0) << kCellBits), f0, d, 0u);
+// Compiled schedule (null pointers in geometric mode).
+template <typename R>
+struct Comp {
+    uint4* hdr;              // L x D*N
+    R* A;                    // L x D*N
+    Entry<R>* planes;        // L x kPlanes*D*N
+    const uint32_t* ovf_start;  // D x N: first overflow entry of a task with > kPlanes upwind entries
+    const Entry<R>* ovf;
+};
+
+__device__ __forceinline__ uint4 task_record(uint32_t c, uint32_t f0, uint32_t f1, uint32_t d, uint32_t ps) {
+    return make_uint4(c | ((f1 - f0) << kCellBits), f0, d, ps);
 }
 
-// Appends `rec` for every lane with `pred` at dst[*counter ...], one atomic per wave.
+// Appends `rec` for every lane with `pred` at dst[*counter ...], one atomic per
+// wave; lanes outside the current control flow take no part.
 __device__ __forceinline__ void wave_append(bool pred, uint4 rec, uint32_t* counter, uint4* dst) {
     unsigned long long m = __ballot(pred);
     if (m == 0ull) return;
@@ -139,6 +182,105 @@ __device__ __forceinline__ void wave_append(bool pred, uint4 rec, uint32_t* coun
     if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
     base = __shfl(base, leader);
     if (pred) dst[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = rec;
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+    return v;
+}
+
+// Outgoing rate of a solved task (Site::get_rate + HydrogenOnly::get_outgoing_rate,
+// src/sweep/site.rs:49-56, src/chemistry/hydrogen_only/mod.rs:72-88), stored as X = O / A.
+template <typename R>
+__device__ __forceinline__ void finish_task(const Flux<R>& fl, size_t o, uint32_t c, R inc, R per, R A) {
+    if (inc < (R)0) inc = 0;  // Photons::make_positive
+    R in = (inc + (R)fl.src_over_d[c]) + per;
+    R out = (in < (R)fl.thr) ? (R)0 : in * (R)fl.att[c];
+    fl.X[o] = (A > (R)0) ? out / A : (R)0;
+    fl.I[o] = inc;
+}
+
+// The face loop of one task, kLanes lanes per task, shared by the build and
+// the geometric replay so that both evaluate identical arithmetic. Lane `sub`
+// handles faces sub, sub + kLanes, ...; the loop runs to the longest face list
+// of the wave so that every lane takes part in the shuffles. The upwind sum is
+// accumulated in face order on every lane of the group; A is a fixed-shape
+// tree over the lanes' partial sums.
+//   kTransport  gather the upwind X and sum
+//   kBuild      decrement the downwind neighbours' counters and queue the
+//               tasks that become ready into the block queue `sq`
+//   kCompile    write the task's upwind entries into the planes
+template <typename R, bool kTransport, bool kBuild, bool kCompile>
+__device__ __forceinline__ void task_faces(const Geo& g, const Flux<R>& fl, bool valid, uint4 r, uint32_t sub,
+                                           uint32_t nfmax, double& A_out, R& inc_out, uint32_t& nup_out,
+                                           uint4* sq, uint32_t* sq_n, Entry<R>* planes, uint32_t W,
+                                           uint32_t pos) {
+    const uint32_t c = r.x & kCellMask;
+    const uint32_t nf = valid ? (r.x >> kCellBits) : 0u;
+    const uint32_t d = r.z;
+    const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+    const size_t dN = (size_t)d * g.N;
+    const int lane = __lane_id();
+    const int gbase = lane & ~(kLanes - 1);
+    double A = 0.0;
+    R inc = 0;
+    uint32_t nup = 0;
+    for (uint32_t base = 0; base < nfmax; base += kLanes) {
+        const uint32_t j = base + sub;
+        const bool fa = j < nf;
+        const uint32_t f = r.y + j;
+        const double4 a = fa ? g.geo[f] : make_double4(0.0, 0.0, 0.0, 0.0);
+        const uint32_t nk = fa ? g.nk[f] : (kKindBoundary << 30);
+        const uint32_t nb = nk & kNbrMask;
+        const bool local = (nk >> 30) == kKindLocal;
+        const double dd = fdot(a, dx, dy, dz);
+        const bool up = fa && local && dd < 0.0;
+        const bool down = fa && dd > 0.0;
+        R w = 0, v = 0;
+        if (up) w = (R)(a.w * fl.inv_aref * -dd);
+        if (kTransport && up) v = w * fl.X[dN + nb];
+        bool ready = false;
+        uint4 rec = make_uint4(0, 0, 0, 0);
+        if (kBuild && down && local) {
+            // neighbour record loads are issued beside the decrement
+            const uint32_t f0n = g.off[nb], f1n = g.off[nb + 1], psn = g.pslot[nb];
+            ready = atomicSub(&g.cnt[dN + nb], 1u) == 1u;
+            rec = task_record(nb, f0n, f1n, d, psn);
+        }
+        if (down) A += a.w * fl.inv_aref * dd;
+        if (kTransport) {
+#pragma unroll
+            for (int l = 0; l < kLanes; l++) inc += __shfl(v, gbase + l);
+        }
+        if (kCompile) {
+            const unsigned long long m = __ballot(up);
+            const uint32_t seg = (uint32_t)(m >> gbase) & 0xFFFFu;
+            const uint32_t q = nup + (uint32_t)__popc(seg & ((1u << sub) - 1u));
+            if (up && q < (uint32_t)kPlanes) {
+                Entry<R> e;
+                e.nb = nb;
+                if constexpr (sizeof(R) == 8) e.pad = 0;
+                e.w = w;
+                planes[(size_t)q * W + pos] = e;
+            }
+            nup += (uint32_t)__popc(seg);
+        }
+        if (kBuild) {
+            const unsigned long long m = __ballot(ready);
+            if (m) {
+                uint32_t b = 0;
+                if (lane == 0) b = atomicAdd(sq_n, (uint32_t)__popcll(m));
+                b = __shfl(b, 0);
+                if (ready) sq[b + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = rec;
+            }
+        }
+    }
+#pragma unroll
+    for (int m = kLanes / 2; m > 0; m >>= 1) A += __shfl_xor(A, m, kLanes);
+    A_out = A;
+    inc_out = inc;
+    nup_out = nup;
 }
 
 // ---------------------------------------------------------------------------
@@ -169,21 +311,71 @@ __global__ void k_check_mirror(Geo g, unsigned long long* bad) {
     }
 }
 
-// Downwind faces per (cell, direction) by kind, for the byte model.
-__global__ void k_downwind_stats(Geo g, unsigned long long* out3) {
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int d = blockIdx.y;
-    unsigned long long n[3] = {0, 0, 0};
-    if (c < g.N) {
-        double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+// Thread t handles (cell t / D, direction t % D): the lanes of a wave share one
+// or two cells, so the face loads are broadcasts.
+__device__ __forceinline__ bool cell_dir(const Geo& g, uint32_t t, uint32_t& c, uint32_t& d) {
+    c = t / (uint32_t)g.D;
+    d = t - c * (uint32_t)g.D;
+    return (int64_t)c < g.N;
+}
+
+// Per (cell, direction): downwind faces by kind (the byte model) and the
+// number of upwind local faces beyond kPlanes (overflow entries).
+__global__ void k_face_stats(Geo g, unsigned long long* out4, uint32_t* ovf_count) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t c, d;
+    unsigned long long n[4] = {0, 0, 0, 0};
+    if (cell_dir(g, t, c, d)) {
+        const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+        uint32_t nup = 0;
         for (uint32_t f = g.off[c]; f < g.off[c + 1]; f++) {
-            if (fdot(g.geo[f], dx, dy, dz) > 0.0) n[g.nk[f] >> 30]++;
+            const uint32_t kind = g.nk[f] >> 30;
+            const double dd = fdot(g.geo[f], dx, dy, dz);
+            if (dd > 0.0) n[kind]++;
+            else if (dd < 0.0 && kind == kKindLocal) nup++;
         }
+        const uint32_t extra = nup > (uint32_t)kPlanes ? nup - kPlanes : 0u;
+        if (ovf_count) ovf_count[(size_t)d * g.N + c] = extra;
+        n[3] = extra;
     }
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < 4; k++) {
         unsigned long long v = n[k];
         for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
-        if (__lane_id() == 0 && v) atomicAdd(&out3[k], v);
+        if (__lane_id() == 0 && v) atomicAdd(&out4[k], v);
+    }
+}
+
+// Overflow entries (upwind entries q >= kPlanes) of every (cell, direction), in
+// face order, at ovf_start[d*N + c] (assigned here by an atomic cursor).
+template <typename R>
+__global__ void k_overflow(Geo g, double inv_aref, const uint32_t* ovf_count, uint32_t* ovf_start,
+                           Entry<R>* ovf, uint32_t* cursor) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t c, d;
+    if (!cell_dir(g, t, c, d)) return;
+    const size_t o = (size_t)d * g.N + c;
+    const uint32_t extra = ovf_count[o];
+    if (extra == 0) {
+        ovf_start[o] = 0;
+        return;
+    }
+    const uint32_t s = atomicAdd(cursor, extra);
+    ovf_start[o] = s;
+    const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+    uint32_t q = 0;
+    for (uint32_t f = g.off[c]; f < g.off[c + 1]; f++) {
+        const uint32_t nk = g.nk[f];
+        const double4 a = g.geo[f];
+        const double dd = fdot(a, dx, dy, dz);
+        if (!((nk >> 30) == kKindLocal && dd < 0.0)) continue;
+        if (q >= (uint32_t)kPlanes) {
+            Entry<R> e;
+            e.nb = nk & kNbrMask;
+            if constexpr (sizeof(R) == 8) e.pad = 0;
+            e.w = (R)(a.w * inv_aref * -dd);
+            ovf[s + q - kPlanes] = e;
+        }
+        q++;
     }
 }
 
@@ -193,164 +385,173 @@ __global__ void k_downwind_stats(Geo g, unsigned long long* out3) {
 // Resets the append counters of slot `lvl` when its schedule is rebuilt and
 // takes the periodic-source snapshot of the active periodic cells (the
 // transport reads the periodic source as it was when the sweep started).
+// Thread t handles (periodic cell t / D, direction t % D).
 template <typename R>
 __global__ void k_begin(Geo g, Flux<R> fl, int lvl) {
-    int d = blockIdx.y;
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 3 && g.need[lvl] != 0) {
+    if (blockIdx.x == 0 && threadIdx.x < 3 && g.need[lvl] != 0) {
         g.ctr[lvl * 3 + threadIdx.x] = 0;
         if (threadIdx.x == 0) g.lstart[(size_t)lvl * (g.Kcap + 2)] = 0;
     }
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (fl.P == nullptr || s >= g.Nper) return;
-    uint32_t c = g.pcells[s];
+    if (fl.P == nullptr) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = t / (uint32_t)g.D;
+    const uint32_t d = t - s * (uint32_t)g.D;
+    if ((int64_t)s >= g.Nper) return;
+    const uint32_t c = g.pcells[s];
     if (g.level[c] < lvl) return;
-    double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+    const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
     R acc = 0;
     for (uint32_t e = g.pfoff[s]; e < g.pfoff[s + 1]; e++) {
         const uint32_t f = g.pfidx[e];
-        uint32_t nk = g.nk[f];
-        double4 a = g.geo[f];
-        double dd = fdot(a, dx, dy, dz);
+        const uint32_t nk = g.nk[f];
+        const double4 a = g.geo[f];
+        const double dd = fdot(a, dx, dy, dz);
         if (dd < 0.0) acc += (R)(a.w * fl.inv_aref * -dd) * fl.X[(size_t)d * g.N + (nk & kNbrMask)];
     }
     fl.P[(size_t)d * g.Nper + s] = acc;
 }
 
-// init_counts (src/sweep/mod.rs:346-386) + get_initial_tasks (:388-398),
-// blockIdx.y = direction: counts active local upwind neighbours and queues
-// the tasks with count 0 as level set 0.
+// init_counts (src/sweep/mod.rs:346-386) + get_initial_tasks (:388-398):
+// counts the active local upwind neighbours of every active (cell, direction)
+// and queues the tasks with count 0 as level set 0. Thread t handles
+// (cell t / D, direction t % D).
 __global__ void k_count(Geo g, int lvl, int only_counts) {
     const bool build = only_counts || g.need[lvl] != 0;
     if (!build) return;
-    int d = blockIdx.y;
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool active = c < g.N && g.level[c] >= lvl;
-    uint32_t n = 0;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t c, d;
+    const bool in = cell_dir(g, t, c, d);
+    const bool active = in && g.level[c] >= lvl;
+    uint32_t n = 0, f0 = 0, f1 = 0;
     if (active) {
-        double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
-        for (uint32_t f = g.off[c]; f < g.off[c + 1]; f++) {
-            uint32_t nk = g.nk[f];
+        const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+        f0 = g.off[c];
+        f1 = g.off[c + 1];
+        for (uint32_t f = f0; f < f1; f++) {
+            const uint32_t nk = g.nk[f];
             if ((nk >> 30) != kKindLocal) continue;
             if (!(fdot(g.geo[f], dx, dy, dz) < 0.0)) continue;
             if (g.level[nk & kNbrMask] >= lvl) n++;
         }
     }
-    if (c < g.N) g.cnt[(size_t)d * g.N + c] = active ? n : kNone;
+    if (in) g.cnt[(size_t)d * g.N + c] = active ? n : kNone;
     if (only_counts) return;
-    uint4* flat = g.flat + (size_t)lvl * g.D * g.N;
-    uint4 rec = active ? task_record(g, (uint32_t)c, (uint32_t)d) : make_uint4(0, 0, 0, 0);
-    wave_append(active && n == 0, rec, &g.ctr[lvl * 3 + 0], flat);
+    uint4* flat = g.flat + (size_t)lvl * g.flat_slot_stride;
+    const bool ready = active && n == 0;
+    wave_append(ready, ready ? task_record(c, f0, f1, d, g.pslot[c]) : make_uint4(0, 0, 0, 0), &g.ctr[lvl * 3 + 0],
+                flat);
 }
 
-// Solves level set k while the schedule of slot `lvl` is being built (or
-// replays it when it was built without a graph). kLanes lanes per task, one
-// face each, per-face sums combined by lane shuffles. While building, also
-// decrements the counters of active local downwind neighbours and queues those
-// that reach zero as level set k+1 (handle_local_neighbour, src/sweep/mod.rs:487-503).
-template <typename R, bool kTransport>
-__global__ void __launch_bounds__(kBlock) k_sweep_build(Geo g, Flux<R> fl, int lvl, int k) {
-    const bool build = g.need[lvl] != 0;
+// Builds level set k+1 of slot `lvl` while solving level set k (kTransport) or
+// only building (schedules for the parity hook and the depth bound). kLanes
+// lanes per task; ready tasks are gathered per block in LDS and appended with
+// one atomic per block. With kCompile also writes the compiled schedule.
+template <typename R, bool kTransport, bool kCompile>
+__global__ void __launch_bounds__(kBlock) k_sweep_build(Geo g, Flux<R> fl, Comp<R> cp, int lvl, int k) {
+    __shared__ uint4 sq[kQueueCap];
+    __shared__ uint32_t sq_n, sq_base, sq_cnt;
     uint32_t* ls = g.lstart + (size_t)lvl * (g.Kcap + 2);
     uint32_t* ctr = g.ctr + lvl * 3;
     const uint32_t beg = ls[k];
-    uint32_t end;
-    if (build) {
-        end = beg + ctr[k % 3];
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            ls[k + 1] = end;
-            ctr[(k + 2) % 3] = 0;
-        }
-    } else {
-        end = ls[k + 1];
+    const uint32_t end = beg + ctr[k % 3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ls[k + 1] = end;
+        ctr[(k + 2) % 3] = 0;
     }
     if (beg >= end) return;
-    uint4* flat = g.flat + (size_t)lvl * g.D * g.N;
+    const uint32_t W = end - beg;
+    uint4* flat = g.flat + (size_t)lvl * g.flat_slot_stride;
     uint32_t* next_ctr = ctr + (k + 1) % 3;
+    const size_t DN = (size_t)g.D * g.N;
+    Entry<R>* planes = kCompile ? cp.planes + (size_t)lvl * kPlanes * DN + (size_t)kPlanes * beg : nullptr;
     const uint32_t sub = threadIdx.x % kLanes;
     const uint32_t stride = gridDim.x * kTasksPerBlock;
+    if (threadIdx.x == 0) sq_n = 0;
+    __syncthreads();
     for (uint32_t i0 = beg + blockIdx.x * kTasksPerBlock; i0 < end; i0 += stride) {
         const uint32_t i = i0 + threadIdx.x / kLanes;
         const bool valid = i < end;
         const uint4 r = valid ? flat[i] : make_uint4(0, 0, 0, 0);
-        const uint32_t c = r.x & kCellMask;
-        const uint32_t nf = r.x >> kCellBits;
-        const uint32_t d = r.z;
-        const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
-        const size_t dN = (size_t)d * g.N;
-        double A = 0.0;
-        R inc = 0;
-        for (uint32_t j = sub; j < nf; j += kLanes) {
-            const uint32_t f = r.y + j;
-            const double4 a = g.geo[f];
-            const uint32_t nk = g.nk[f];
-            const uint32_t nb = nk & kNbrMask;
-            const double dd = fdot(a, dx, dy, dz);
-            bool ready = false;
-            if (dd > 0.0) {
-                A += a.w * fl.inv_aref * dd;
-                if (build && (nk >> 30) == kKindLocal && g.level[nb] >= lvl)
-                    ready = atomicSub(&g.cnt[dN + nb], 1u) == 1u;
-            } else if (kTransport && dd < 0.0 && (nk >> 30) == kKindLocal) {
-                inc += (R)(a.w * fl.inv_aref * -dd) * fl.X[dN + nb];
-            }
-            if (build) wave_append(ready, ready ? task_record(g, nb, d) : make_uint4(0, 0, 0, 0), next_ctr, flat + end);
-        }
-        if (kTransport) {
-#pragma unroll
-            for (int m = kLanes / 2; m > 0; m >>= 1) {
-                A += __shfl_xor(A, m, kLanes);
-                inc += __shfl_xor(inc, m, kLanes);
-            }
-            if (valid && sub == 0) {
-                if (inc < (R)0) inc = 0;  // Photons::make_positive
-                const uint32_t ps = g.pslot[c];
+        const uint32_t nfmax = wave_max(valid ? (r.x >> kCellBits) : 0u);
+        double A;
+        R inc;
+        uint32_t nup;
+        task_faces<R, kTransport, true, kCompile>(g, fl, valid, r, sub, nfmax, A, inc, nup, sq, &sq_n, planes, W,
+                                                  i - beg);
+        if (valid && sub == 0) {
+            const uint32_t c = r.x & kCellMask, d = r.z, ps = r.w;
+            const size_t o = (size_t)d * g.N + c;
+            if (kTransport) {
                 R per = (ps != kNone && fl.P) ? fl.P[(size_t)d * g.Nper + ps] : (R)0;
-                R in = (inc + (R)fl.src_over_d[c]) + per;  // Site::get_rate
-                R out = (in < (R)fl.thr) ? (R)0 : in * (R)fl.att[c];
-                fl.X[dN + c] = (A > 0.0) ? out / (R)A : (R)0;
-                fl.I[dN + c] = inc;
+                finish_task<R>(fl, o, c, inc, per, (R)A);
+            }
+            if (kCompile) {
+                const size_t so = (size_t)lvl * DN + i;
+                cp.hdr[so] = make_uint4(c | (d << kCellBits), ps, nup > (uint32_t)kPlanes ? cp.ovf_start[o] : 0u, nup);
+                cp.A[so] = (R)A;
             }
         }
+        // flush the block queue into level set k+1
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t n = sq_n;
+            sq_cnt = n;
+            sq_base = n ? atomicAdd(next_ctr, n) : 0u;
+            sq_n = 0;
+        }
+        __syncthreads();
+        const uint32_t n = sq_cnt, b = end + sq_base;
+        for (uint32_t t = threadIdx.x; t < n; t += kBlock) flat[b + t] = sq[t];
+        __syncthreads();
     }
 }
 
-// Replays one level set of a built schedule: `count` tasks of the level-major
-// array, kLanes lanes per task. Three dependent loads per task: the record,
-// the faces, the upwind X values.
+// Geometric replay of one level set: `count` task records, kLanes lanes per task.
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_sweep_rec(Geo g, Flux<R> fl, const uint4* recs, uint32_t count) {
     const uint32_t i = blockIdx.x * kTasksPerBlock + threadIdx.x / kLanes;
     const uint32_t sub = threadIdx.x % kLanes;
     const bool valid = i < count;
     const uint4 r = valid ? recs[i] : make_uint4(0, 0, 0, 0);
-    const uint32_t c = r.x & kCellMask;
-    const uint32_t nf = r.x >> kCellBits;
-    const uint32_t d = r.z;
-    const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
-    const size_t dN = (size_t)d * g.N;
-    double A = 0.0;
-    R inc = 0;
-    for (uint32_t j = sub; j < nf; j += kLanes) {
-        const double4 a = g.geo[r.y + j];
-        const uint32_t nk = g.nk[r.y + j];
-        const double dd = fdot(a, dx, dy, dz);
-        if (dd > 0.0) A += a.w * fl.inv_aref * dd;
-        else if (dd < 0.0 && (nk >> 30) == kKindLocal) inc += (R)(a.w * fl.inv_aref * -dd) * fl.X[dN + (nk & kNbrMask)];
-    }
+    const uint32_t ps = r.w;
+    const uint32_t c = r.x & kCellMask, d = r.z;
+    const R per = (valid && ps != kNone && fl.P) ? fl.P[(size_t)d * g.Nper + ps] : (R)0;
+    const uint32_t nfmax = wave_max(valid ? (r.x >> kCellBits) : 0u);
+    double A;
+    R inc;
+    uint32_t nup;
+    task_faces<R, true, false, false>(g, fl, valid, r, sub, nfmax, A, inc, nup, nullptr, nullptr, nullptr, 0, 0);
+    if (valid && sub == 0) finish_task<R>(fl, (size_t)d * g.N + c, c, inc, per, (R)A);
+}
+
+// Compiled replay of one level set of width W: one lane per task. The header,
+// A and the plane entries of a task are independent loads; the upwind X
+// gathers follow. The sum runs over the upwind entries in face order.
+template <typename R>
+__global__ void __launch_bounds__(kRecBlock) k_sweep_comp(Geo g, Flux<R> fl, const uint4* hdr, const R* Av,
+                                                          const Entry<R>* planes, const Entry<R>* ovf, uint32_t W) {
+    const uint32_t i = blockIdx.x * kRecBlock + threadIdx.x;
+    if (i >= W) return;
+    const uint4 h = hdr[i];
+    const R A = Av[i];
+    Entry<R> e[kPlanes];
 #pragma unroll
-    for (int m = kLanes / 2; m > 0; m >>= 1) {
-        A += __shfl_xor(A, m, kLanes);
-        inc += __shfl_xor(inc, m, kLanes);
+    for (int q = 0; q < kPlanes; q++) e[q] = planes[(size_t)q * W + i];
+    const uint32_t c = h.x & kCellMask, d = h.x >> kCellBits, ps = h.y, nup = h.w;
+    const size_t dN = (size_t)d * g.N;
+    R x[kPlanes];
+#pragma unroll
+    for (int q = 0; q < kPlanes; q++) x[q] = ((uint32_t)q < nup) ? fl.X[dN + e[q].nb] : (R)0;
+    const R per = (ps != kNone && fl.P) ? fl.P[(size_t)d * g.Nper + ps] : (R)0;
+    R inc = 0;
+#pragma unroll
+    for (int q = 0; q < kPlanes; q++)
+        if ((uint32_t)q < nup) inc += e[q].w * x[q];
+    for (uint32_t q = kPlanes; q < nup; q++) {
+        const Entry<R> o = ovf[h.z + q - kPlanes];
+        inc += o.w * fl.X[dN + o.nb];
     }
-    if (valid && sub == 0) {
-        if (inc < (R)0) inc = 0;  // Photons::make_positive
-        const uint32_t ps = g.pslot[c];
-        R per = (ps != kNone && fl.P) ? fl.P[(size_t)d * g.Nper + ps] : (R)0;
-        R in = (inc + (R)fl.src_over_d[c]) + per;  // Site::get_rate
-        R out = (in < (R)fl.thr) ? (R)0 : in * (R)fl.att[c];
-        fl.X[dN + c] = (A > 0.0) ? out / (R)A : (R)0;
-        fl.I[dN + c] = inc;
-    }
+    finish_task<R>(fl, dN + c, c, inc, per, A);
 }
 
 struct ChemArgs {
@@ -486,11 +687,11 @@ __global__ void k_fluxes(Geo g, Flux<R> fl, double lref, double* I_out, double* 
     if (photon_rate) photon_rate[c] = total;
 }
 
-// Kahn depth of every task of slot `lvl`: entry i of the level-major array is
-// in level set k with lstart[k] <= i < lstart[k+1].
-__global__ void k_scatter_depth(Geo g, int lvl, int kmax, uint32_t* depth) {
-    const uint32_t* ls = g.lstart + (size_t)lvl * (g.Kcap + 2);
-    const uint4* flat = g.flat + (size_t)lvl * g.D * g.N;
+// Kahn depth of every task of a built schedule: entry i of the level-major
+// array is in level set k with lstart[k] <= i < lstart[k+1]. `compiled`
+// selects the header layout of the entries.
+__global__ void k_scatter_depth(Geo g, const uint4* tasks, const uint32_t* ls, int kmax, int compiled,
+                                uint32_t* depth) {
     const uint32_t total = ls[kmax];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
         int a = 0, b = kmax - 1;  // last k with ls[k] <= i
@@ -499,8 +700,10 @@ __global__ void k_scatter_depth(Geo g, int lvl, int kmax, uint32_t* depth) {
             if (ls[m] <= i) a = m;
             else b = m - 1;
         }
-        const uint4 r = flat[i];
-        depth[(size_t)r.z * g.N + (r.x & kCellMask)] = (uint32_t)a;
+        const uint4 r = tasks[i];
+        const uint32_t c = r.x & kCellMask;
+        const uint32_t d = compiled ? (r.x >> kCellBits) : r.z;
+        depth[(size_t)d * g.N + c] = (uint32_t)a;
     }
 }
 
@@ -556,6 +759,8 @@ struct ssw_sweep {
     int device = 0;
     hipStream_t stream = nullptr;
     bool fp32 = false;
+    int mode_request = SSW_TRANSPORT_MODE_AUTO;
+    bool compiled = false;
     // grid
     int64_t N = 0, F = 0, Nper = 0;
     bool have_grid = false, have_state = false;
@@ -565,14 +770,21 @@ struct ssw_sweep {
     double *d_dirs = nullptr, *d_vol = nullptr, *d_size = nullptr;
     double aref = 1.0;
     int kmax = 0;
-    int64_t max_width = 0;
     double dw[3] = {0, 0, 0};
+    int64_t ovf_total = 0;
+    std::vector<uint32_t> full_start;               // level starts of the full graph
     // schedule
     uint4* d_flat = nullptr;
     uint32_t *d_lstart = nullptr, *d_ctr = nullptr, *d_cnt = nullptr, *d_need = nullptr;
+    uint4* d_hdr = nullptr;
+    void* d_A = nullptr;
+    void* d_planes = nullptr;
+    uint32_t* d_ovf_start = nullptr;
+    void* d_ovf = nullptr;
+    int64_t compiled_bytes = 0;
     int kcap = 0;
     std::vector<int> replay_ok;                     // per slot: host copy of the level starts and graph valid
-    std::vector<std::vector<uint32_t>> level_start; // per slot: lstart[0..kmax]
+    std::vector<std::vector<uint32_t>> level_start; // per slot: lstart[0..kmax] of its last build
     std::vector<hipGraphExec_t> graphs;             // per slot: replay of its level sets
     // state
     double *d_rho = nullptr, *d_x = nullptr, *d_T = nullptr, *d_src = nullptr, *d_att = nullptr;
@@ -584,38 +796,50 @@ struct ssw_sweep {
     double lref = 1.0;
     TimestepState ts;
     int64_t bytes = 0;
+    std::unordered_map<void*, size_t> alloc;
     int64_t sweeps_last = 0;
+    int64_t builds_last = 0;
+    int64_t capture_failures = 0;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
-    double t_transport_total = 0, t_chem_total = 0;
+    double t_transport_total = 0, t_chem_total = 0, t_build_total = 0;
     int64_t n_transport = 0;
 };
 
 namespace {
 
 template <typename T>
-int dalloc(ssw_sweep* s, T** p, size_t count) {
-    if (*p) {
-        (void)hipFree(*p);
-        *p = nullptr;
+void dfree(ssw_sweep* s, T*& p) {
+    if (p) {
+        auto it = s->alloc.find((void*)p);
+        if (it != s->alloc.end()) {
+            s->bytes -= (int64_t)it->second;
+            s->alloc.erase(it);
+        }
+        (void)hipFree((void*)p);
     }
+    p = nullptr;
+}
+
+template <typename T>
+int dalloc(ssw_sweep* s, T** p, size_t count) {
+    dfree(s, *p);
     if (count == 0) count = 1;
     hipError_t e = hipMalloc((void**)p, count * sizeof(T));
     if (e != hipSuccess) {
         *p = nullptr;
+        (void)hipGetLastError();
         return fail(SSW_ERR_OOM, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
                                      " bytes failed: " + hipGetErrorString(e));
     }
     s->bytes += (int64_t)(count * sizeof(T));
+    s->alloc[(void*)*p] = count * sizeof(T);
     return SSW_OK;
 }
 
-template <typename T>
-void dfree(T*& p) {
-    if (p) (void)hipFree((void*)p);
-    p = nullptr;
-}
+size_t real_size(const ssw_sweep* s) { return s->fp32 ? 4 : 8; }
+size_t entry_size(const ssw_sweep* s) { return s->fp32 ? sizeof(Entry<float>) : sizeof(Entry<double>); }
 
 Geo make_geo(ssw_sweep* s) {
     Geo g;
@@ -634,6 +858,7 @@ Geo make_geo(ssw_sweep* s) {
     g.pfidx = s->d_pfidx;
     g.Nper = s->Nper;
     g.flat = s->d_flat;
+    g.flat_slot_stride = s->compiled ? 0 : (size_t)s->D * s->N;
     g.lstart = s->d_lstart;
     g.ctr = s->d_ctr;
     g.cnt = s->d_cnt;
@@ -655,6 +880,17 @@ Flux<R> make_flux(ssw_sweep* s) {
     return f;
 }
 
+template <typename R>
+Comp<R> make_comp(ssw_sweep* s) {
+    Comp<R> c;
+    c.hdr = s->d_hdr;
+    c.A = (R*)s->d_A;
+    c.planes = (Entry<R>*)s->d_planes;
+    c.ovf_start = s->d_ovf_start;
+    c.ovf = (const Entry<R>*)s->d_ovf;
+    return c;
+}
+
 int set_need_all(ssw_sweep* s, uint32_t v) {
     k_fill_u32<<<1, kBlock, 0, s->stream>>>(s->d_need, v, s->p.num_timestep_levels);
     HIP_TRY(hipGetLastError());
@@ -670,55 +906,88 @@ void invalidate_replays(ssw_sweep* s) {
     std::fill(s->replay_ok.begin(), s->replay_ok.end(), 0);
 }
 
-unsigned build_grid_x(ssw_sweep* s) {
-    // enough blocks for the widest level set of the full graph (all directions),
-    // with headroom for restricted graphs; a grid-stride loop covers the rest
-    int64_t w = s->max_width + s->max_width / 2 + 1;
-    int64_t b = (w + kTasksPerBlock - 1) / kTasksPerBlock;
-    return (unsigned)std::min<int64_t>(std::max<int64_t>(b, 1), 8192);
+// Blocks for building level set k of slot `lvl`: from the widths of the slot's
+// last build, else of the full graph, with headroom; the kernel's grid-stride
+// loop covers wider level sets.
+unsigned build_blocks(ssw_sweep* s, int lvl, int k) {
+    const std::vector<uint32_t>& st = s->level_start[lvl].size() > (size_t)k + 1 ? s->level_start[lvl] : s->full_start;
+    int64_t w = 1024;
+    if (st.size() > (size_t)k + 1) w = (int64_t)(st[k + 1] - st[k]);
+    w = w + w / 4 + 64;
+    return (unsigned)std::min<int64_t>(16384, (w + kTasksPerBlock - 1) / kTasksPerBlock);
 }
 
-// Builds the schedule of the full graph (all cells active) once per grid: the
-// depth bound kmax and the widest level set; a graph that never empties its
-// queue has a cycle. Grows the level-start array of slot 0 as needed.
+template <typename R>
+void launch_build(ssw_sweep* s, const Geo& g, int lvl, int k, bool transport, unsigned blocks) {
+    Flux<R> fl = make_flux<R>(s);
+    Comp<R> cp = make_comp<R>(s);
+    if (transport) {
+        if (s->compiled) k_sweep_build<R, true, true><<<blocks, kBlock, 0, s->stream>>>(g, fl, cp, lvl, k);
+        else k_sweep_build<R, true, false><<<blocks, kBlock, 0, s->stream>>>(g, fl, cp, lvl, k);
+    } else {
+        if (s->compiled) k_sweep_build<R, false, true><<<blocks, kBlock, 0, s->stream>>>(g, fl, cp, lvl, k);
+        else k_sweep_build<R, false, false><<<blocks, kBlock, 0, s->stream>>>(g, fl, cp, lvl, k);
+    }
+}
+
+void launch_build_any(ssw_sweep* s, const Geo& g, int lvl, int k, bool transport, unsigned blocks) {
+    if (s->fp32) launch_build<float>(s, g, lvl, k, transport, blocks);
+    else launch_build<double>(s, g, lvl, k, transport, blocks);
+}
+
+// Builds the schedule of the full graph (all cells active) once per grid, with
+// no compiled output: the depth bound kmax and the level widths; a graph that
+// never empties its queue has a cycle. Grows the level starts as needed.
 int build_full_schedule(ssw_sweep* s) {
     const int L = s->p.num_timestep_levels;
-    s->max_width = 0;
     int kcap = 256;
     s->kcap = kcap;
     TRY(dalloc(s, &s->d_lstart, (size_t)L * (kcap + 2)));
     TRY(set_need_all(s, 1u));
     Geo g = make_geo(s);
+    const bool was_compiled = s->compiled;
+    s->compiled = false;  // records only
+    g.flat_slot_stride = 0;
     Flux<double> fl{};
-    k_begin<double><<<dim3(1, 1), kBlock, 0, s->stream>>>(g, fl, 0);
-    k_count<<<dim3(grid_for(s->N), s->D), kBlock, 0, s->stream>>>(g, 0, 0);
+    Comp<double> cp{};
+    k_begin<double><<<1, kBlock, 0, s->stream>>>(g, fl, 0);
+    k_count<<<grid_for((int64_t)s->N * s->D), kBlock, 0, s->stream>>>(g, 0, 0);
     HIP_TRY(hipGetLastError());
+    s->full_start.assign(1, 0u);
     int64_t total = 0;
     int k = 0;
+    int rc = SSW_OK;
     for (;; k++) {
         if (k + 2 > kcap) {  // grow the level starts of slot 0
             int ncap = kcap * 2;
             uint32_t* nl = nullptr;
-            TRY(dalloc(s, &nl, (size_t)L * (ncap + 2)));
+            rc = dalloc(s, &nl, (size_t)L * (ncap + 2));
+            if (rc != SSW_OK) break;
             HIP_TRY(hipMemcpyAsync(nl, s->d_lstart, (kcap + 2) * 4, hipMemcpyDeviceToDevice, s->stream));
             HIP_TRY(hipStreamSynchronize(s->stream));
-            dfree(s->d_lstart);
+            dfree(s, s->d_lstart);
             s->d_lstart = nl;
             kcap = ncap;
             s->kcap = kcap;
-            g = make_geo(s);
+            g.lstart = nl;
+            g.Kcap = kcap;
         }
         uint32_t width = 0;
         HIP_TRY(hipMemcpyAsync(&width, s->d_ctr + (k % 3), 4, hipMemcpyDeviceToHost, s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
         if (width == 0) break;
         total += width;
-        s->max_width = std::max<int64_t>(s->max_width, width);
-        if (k > s->N + 1) return fail(SSW_ERR_CYCLE, "dependency graph deeper than the number of cells");
-        unsigned gx = (unsigned)std::min<int64_t>(8192, (width + kTasksPerBlock - 1) / kTasksPerBlock);
-        k_sweep_build<double, false><<<gx, kBlock, 0, s->stream>>>(g, fl, 0, k);
+        s->full_start.push_back((uint32_t)total);
+        if (k > s->N + 1) {
+            rc = fail(SSW_ERR_CYCLE, "dependency graph deeper than the number of cells");
+            break;
+        }
+        unsigned blocks = (unsigned)std::min<int64_t>(16384, (width + kTasksPerBlock - 1) / kTasksPerBlock);
+        k_sweep_build<double, false, false><<<blocks, kBlock, 0, s->stream>>>(g, fl, cp, 0, k);
         HIP_TRY(hipGetLastError());
     }
+    s->compiled = was_compiled;
+    if (rc != SSW_OK) return rc;
     if (total != s->N * (int64_t)s->D)
         return fail(SSW_ERR_CYCLE, "dependency cycle: " + std::to_string(s->N * (int64_t)s->D - total) +
                                        " cell-direction pairs never become ready");
@@ -735,29 +1004,36 @@ int build_full_schedule(ssw_sweep* s) {
 
 // Launches the replay of slot `lvl`'s level sets on the handle's stream, one
 // exact-size launch per non-empty level set; with `ev` each launch is
-// bracketed by events. Returns the number of launches.
+// bracketed by events.
 template <typename R>
-int launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev,
-                    const std::function<hipEvent_t()>* mark) {
+void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev,
+                     const std::function<hipEvent_t()>* mark) {
     Flux<R> fl = make_flux<R>(s);
+    Comp<R> cp = make_comp<R>(s);
     const std::vector<uint32_t>& st = s->level_start[lvl];
-    const uint4* flat = s->d_flat + (size_t)lvl * s->D * s->N;
-    int n = 0;
+    const size_t DN = (size_t)s->D * s->N;
     for (int k = 0; k < s->kmax; k++) {
-        const uint32_t count = st[k + 1] - st[k];
+        const uint32_t beg = st[k], count = st[k + 1] - st[k];
         if (count == 0) continue;
         hipEvent_t a = ev ? (*mark)() : nullptr;
-        k_sweep_rec<R><<<(count + kTasksPerBlock - 1) / kTasksPerBlock, kBlock, 0, s->stream>>>(g, fl, flat + st[k],
-                                                                                               count);
+        if (s->compiled) {
+            const size_t so = (size_t)lvl * DN + beg;
+            k_sweep_comp<R><<<(count + kRecBlock - 1) / kRecBlock, kRecBlock, 0, s->stream>>>(
+                g, fl, cp.hdr + so, cp.A + so, cp.planes + (size_t)lvl * kPlanes * DN + (size_t)kPlanes * beg, cp.ovf,
+                count);
+        } else {
+            const uint4* recs = s->d_flat + (size_t)lvl * DN + beg;
+            k_sweep_rec<R><<<(count + kTasksPerBlock - 1) / kTasksPerBlock, kBlock, 0, s->stream>>>(g, fl, recs,
+                                                                                                   count);
+        }
         if (ev) ev->push_back({a, (*mark)()});
-        n++;
     }
-    return n;
 }
 
-int launch_replay(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev,
-                  const std::function<hipEvent_t()>* mark = nullptr) {
-    return s->fp32 ? launch_replay_t<float>(s, g, lvl, ev, mark) : launch_replay_t<double>(s, g, lvl, ev, mark);
+void launch_replay(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev,
+                   const std::function<hipEvent_t()>* mark = nullptr) {
+    if (s->fp32) launch_replay_t<float>(s, g, lvl, ev, mark);
+    else launch_replay_t<double>(s, g, lvl, ev, mark);
 }
 
 // After a schedule is built: copies its level starts to the host and captures
@@ -772,20 +1048,32 @@ int build_replay_data(ssw_sweep* s, const Geo& g, int lvl) {
         (void)hipGraphExecDestroy(s->graphs[lvl]);
         s->graphs[lvl] = nullptr;
     }
+    s->replay_ok[lvl] = 1;
+    // One graph per schedule; if the capture fails the level sets are
+    // launched one by one instead (counted in ssw_info).
     hipGraph_t graph = nullptr;
-    HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        (void)hipGetLastError();
+        s->capture_failures++;
+        return SSW_OK;
+    }
     launch_replay(s, g, lvl, nullptr);
     hipError_t ce = hipStreamEndCapture(s->stream, &graph);
-    if (ce != hipSuccess) {
+    if (ce != hipSuccess || !graph) {
         if (graph) (void)hipGraphDestroy(graph);
-        return fail(SSW_ERR_HIP, std::string("graph capture failed: ") + hipGetErrorString(ce));
+        (void)hipGetLastError();
+        s->capture_failures++;
+        return SSW_OK;
     }
     hipGraphExec_t ge = nullptr;
     hipError_t ie = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
-    if (ie != hipSuccess) return fail(SSW_ERR_HIP, std::string("graph instantiation failed: ") + hipGetErrorString(ie));
+    if (ie != hipSuccess) {
+        (void)hipGetLastError();
+        s->capture_failures++;
+        return SSW_OK;
+    }
     s->graphs[lvl] = ge;
-    s->replay_ok[lvl] = 1;
     return SSW_OK;
 }
 
@@ -815,12 +1103,12 @@ int run_step(ssw_sweep* s, double* elapsed) {
     const std::vector<int> order = s->ts.sweep_order();
     const int L = s->p.num_timestep_levels;
     k_fill_u32<<<1, 2, 0, s->stream>>>((uint32_t*)s->d_active, 0u, 2);
-    const unsigned gx = build_grid_x(s);
-    const dim3 gper(grid_for(std::max<int64_t>(s->Nper, 1)), s->D);
-    const dim3 gcount(grid_for(s->N), s->D);
+    const unsigned gper = grid_for(std::max<int64_t>(s->Nper, 1) * s->D);
+    const unsigned gcount = grid_for((int64_t)s->N * s->D);
     const unsigned gcells = grid_for(s->N);
     s->sweeps_last = 0;
-    s->t_transport_total = s->t_chem_total = 0;
+    s->builds_last = 0;
+    s->t_transport_total = s->t_chem_total = s->t_build_total = 0;
     s->n_transport = 0;
     size_t evn = 0;
     std::function<hipEvent_t()> mark = [&](void) -> hipEvent_t {
@@ -833,7 +1121,7 @@ int run_step(ssw_sweep* s, double* elapsed) {
         (void)hipEventRecord(e, s->stream);
         return e;
     };
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> tr, ch;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> tr, ch, bu;
     // Which schedules are (re)built in this step (device flags set by the level
     // update of the previous step).
     std::vector<uint32_t> need(L, 1u);
@@ -849,19 +1137,20 @@ int run_step(ssw_sweep* s, double* elapsed) {
         }
     for (int lvl : order) {
         k_begin<R><<<gper, kBlock, 0, s->stream>>>(g, fl, lvl);
-        if (need[lvl]) k_count<<<gcount, kBlock, 0, s->stream>>>(g, lvl, 0);
-        if (!need[lvl] && s->replay_ok[lvl]) {
+        if (!need[lvl] && !s->replay_ok[lvl]) TRY(build_replay_data(s, g, lvl));  // built by a parity hook
+        if (!need[lvl]) {
             if (s->timing) launch_replay(s, g, lvl, &tr, &mark);
-            else HIP_TRY(hipGraphLaunch(s->graphs[lvl], s->stream));
-        } else {  // build (or replay a schedule built without transport)
-            for (int k = 0; k < s->kmax; k++) {
-                hipEvent_t a = s->timing ? mark() : nullptr;
-                k_sweep_build<R, true><<<gx, kBlock, 0, s->stream>>>(g, fl, lvl, k);
-                if (s->timing) tr.push_back({a, mark()});
-            }
+            else if (s->graphs[lvl]) HIP_TRY(hipGraphLaunch(s->graphs[lvl], s->stream));
+            else launch_replay(s, g, lvl, nullptr);
+        } else {  // build while solving
+            hipEvent_t b0 = s->timing ? mark() : nullptr;
+            k_count<<<gcount, kBlock, 0, s->stream>>>(g, lvl, 0);
+            for (int k = 0; k < s->kmax; k++) launch_build<R>(s, g, lvl, k, true, build_blocks(s, lvl, k));
+            if (s->timing) bu.push_back({b0, mark()});
             HIP_TRY(hipGetLastError());
             TRY(build_replay_data(s, g, lvl));
             need[lvl] = 0;
+            s->builds_last++;
         }
         hipEvent_t a = s->timing ? mark() : nullptr;
         k_chemistry<R><<<gcells, kBlock, 0, s->stream>>>(g, fl, ca, lvl);
@@ -887,42 +1176,81 @@ int run_step(ssw_sweep* s, double* elapsed) {
             (void)hipEventElapsedTime(&ms, pr.first, pr.second);
             s->t_chem_total += ms;
         }
+        for (auto& pr : bu) {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, pr.first, pr.second);
+            s->t_build_total += ms;
+        }
     }
     if (elapsed) *elapsed = el;
     return SSW_OK;
 }
 
 void free_state(ssw_sweep* s) {
-    dfree(s->d_rho);
-    dfree(s->d_x);
-    dfree(s->d_T);
-    dfree(s->d_src);
-    dfree(s->d_att);
-    dfree(s->d_sts);
-    dfree(s->d_prev);
-    dfree(s->d_change);
-    dfree(s->d_src_t);
-    dfree(s->d_X);
-    dfree(s->d_I);
-    dfree(s->d_P);
+    dfree(s, s->d_rho);
+    dfree(s, s->d_x);
+    dfree(s, s->d_T);
+    dfree(s, s->d_src);
+    dfree(s, s->d_att);
+    dfree(s, s->d_sts);
+    dfree(s, s->d_prev);
+    dfree(s, s->d_change);
+    dfree(s, s->d_src_t);
+    dfree(s, s->d_X);
+    dfree(s, s->d_I);
+    dfree(s, s->d_P);
 }
 
 void free_grid(ssw_sweep* s) {
     invalidate_replays(s);
-    dfree(s->d_off);
-    dfree(s->d_geo);
-    dfree(s->d_nk);
-    dfree(s->d_vol);
-    dfree(s->d_size);
-    dfree(s->d_pslot);
-    dfree(s->d_pcells);
-    dfree(s->d_pfoff);
-    dfree(s->d_pfidx);
-    dfree(s->d_flat);
-    dfree(s->d_lstart);
-    dfree(s->d_ctr);
-    dfree(s->d_cnt);
-    dfree(s->d_level);
+    dfree(s, s->d_off);
+    dfree(s, s->d_geo);
+    dfree(s, s->d_nk);
+    dfree(s, s->d_vol);
+    dfree(s, s->d_size);
+    dfree(s, s->d_pslot);
+    dfree(s, s->d_pcells);
+    dfree(s, s->d_pfoff);
+    dfree(s, s->d_pfidx);
+    dfree(s, s->d_flat);
+    dfree(s, s->d_lstart);
+    dfree(s, s->d_ctr);
+    dfree(s, s->d_cnt);
+    dfree(s, s->d_level);
+    dfree(s, s->d_hdr);
+    dfree(s, s->d_A);
+    dfree(s, s->d_planes);
+    dfree(s, s->d_ovf_start);
+    dfree(s, s->d_ovf);
+}
+
+// Allocates the compiled schedule and the overflow entries (once per grid).
+int setup_compiled(ssw_sweep* s, uint32_t* d_ovf_count) {
+    const size_t DN = (size_t)s->D * s->N;
+    const int L = s->p.num_timestep_levels;
+    TRY(dalloc(s, &s->d_hdr, (size_t)L * DN));
+    TRY(dalloc(s, (uint8_t**)&s->d_A, (size_t)L * DN * real_size(s)));
+    TRY(dalloc(s, (uint8_t**)&s->d_planes, (size_t)L * kPlanes * DN * entry_size(s)));
+    TRY(dalloc(s, &s->d_ovf_start, DN));
+    TRY(dalloc(s, (uint8_t**)&s->d_ovf, (size_t)std::max<int64_t>(1, s->ovf_total) * entry_size(s)));
+    uint32_t* d_cursor = nullptr;
+    TRY(dalloc(s, &d_cursor, 1));
+    HIP_TRY(hipMemsetAsync(d_cursor, 0, 4, s->stream));
+    Geo g = make_geo(s);
+    const double inv_aref = 1.0 / s->aref;
+    const unsigned gt = grid_for((int64_t)DN);
+    if (s->fp32)
+        k_overflow<float><<<gt, kBlock, 0, s->stream>>>(g, inv_aref, d_ovf_count, s->d_ovf_start,
+                                                        (Entry<float>*)s->d_ovf, d_cursor);
+    else
+        k_overflow<double><<<gt, kBlock, 0, s->stream>>>(g, inv_aref, d_ovf_count, s->d_ovf_start,
+                                                         (Entry<double>*)s->d_ovf, d_cursor);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    dfree(s, d_cursor);
+    s->compiled_bytes = (int64_t)((size_t)L * DN * (16 + real_size(s) + kPlanes * entry_size(s)) + DN * 4 +
+                                  (size_t)std::max<int64_t>(1, s->ovf_total) * entry_size(s));
+    return SSW_OK;
 }
 
 }  // namespace
@@ -1004,9 +1332,10 @@ int ssw_create(const ssw_sweep_parameters* params, ssw_sweep** out) {
         ssw_destroy(s);
         return r;
     }
-    if (hipMemcpy(s->d_dirs, dirs.data(), dirs.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpy(s->d_dirs, dirs.data(), dirs.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(s->d_fail, 0, 8) != hipSuccess || hipMemset(s->d_active, 0, 8) != hipSuccess) {
         ssw_destroy(s);
-        return fail(SSW_ERR_HIP, "copy of directions failed");
+        return fail(SSW_ERR_HIP, "initialisation of the handle's device memory failed");
     }
     s->ts.init(p.max_timestep, p.num_timestep_levels);
     *out = s;
@@ -1018,13 +1347,23 @@ void ssw_destroy(ssw_sweep* s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     free_state(s);
     free_grid(s);
-    dfree(s->d_dirs);
-    dfree(s->d_need);
-    dfree(s->d_fail);
-    dfree(s->d_active);
+    dfree(s, s->d_dirs);
+    dfree(s, s->d_need);
+    dfree(s, s->d_fail);
+    dfree(s, s->d_active);
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
+}
+
+int ssw_set_transport_mode(ssw_sweep* s, int32_t mode) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    if (mode != SSW_TRANSPORT_MODE_AUTO && mode != SSW_TRANSPORT_MODE_GEOMETRIC && mode != SSW_TRANSPORT_MODE_COMPILED)
+        return fail(SSW_ERR_INVALID, "transport mode " + std::to_string(mode) +
+                                         " is not available (0 automatic, 3 geometric, 4 compiled)");
+    if (s->have_grid) return fail(SSW_ERR_INVALID, "the transport mode must be set before ssw_upload_grid");
+    s->mode_request = mode;
+    return SSW_OK;
 }
 
 int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* nbr, const uint8_t* kind,
@@ -1036,7 +1375,7 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     if (off[0] != 0) return fail(SSW_ERR_INVALID, "face_offsets[0] must be 0");
     for (int64_t c = 0; c < n; c++) {
         if (off[c + 1] < off[c]) return fail(SSW_ERR_INVALID, "face_offsets must be non-decreasing");
-        if (off[c + 1] - off[c] > 127) return fail(SSW_ERR_INVALID, "a cell has more than 127 faces");
+        if (off[c + 1] - off[c] > kMaxFaces) return fail(SSW_ERR_INVALID, "a cell has more than 127 faces");
     }
     const int64_t F = off[n];
     if (F >= (int64_t)0xFFFFFFFFll) return fail(SSW_ERR_INVALID, "too many faces");
@@ -1081,6 +1420,7 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     s->aref = (nfaces > 0 && asum > 0.0) ? std::ldexp(1.0, (int)std::lround(std::log2(asum / (double)nfaces))) : 1.0;
     s->Nper = (int64_t)pcells.size();
     const int L = s->p.num_timestep_levels;
+    const size_t DN = (size_t)s->D * n;
     TRY(dalloc(s, &s->d_off, n + 1));
     TRY(dalloc(s, &s->d_geo, F));
     TRY(dalloc(s, &s->d_nk, F));
@@ -1091,9 +1431,8 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     TRY(dalloc(s, &s->d_pfoff, pfoff.size()));
     TRY(dalloc(s, &s->d_pfidx, std::max<size_t>(1, pfidx.size())));
     TRY(dalloc(s, &s->d_level, n));
-    TRY(dalloc(s, &s->d_flat, (size_t)L * s->D * n));
     TRY(dalloc(s, &s->d_ctr, (size_t)L * 3));
-    TRY(dalloc(s, &s->d_cnt, (size_t)s->D * n));
+    TRY(dalloc(s, &s->d_cnt, DN));
     HIP_TRY(hipMemcpyAsync(s->d_off, off32.data(), (n + 1) * 4, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_geo, geo.data(), F * sizeof(double4), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_nk, nk.data(), F * 4, hipMemcpyHostToDevice, s->stream));
@@ -1108,23 +1447,38 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     std::vector<uint8_t> lev(n, (uint8_t)(L - 1));
     HIP_TRY(hipMemcpyAsync(s->d_level, lev.data(), n, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemsetAsync(s->d_ctr, 0, (size_t)L * 3 * 4, s->stream));
-    // mirror check and downwind statistics
-    unsigned long long* d_bad = nullptr;
-    TRY(dalloc(s, &d_bad, 4));
-    HIP_TRY(hipMemsetAsync(d_bad, 0, 4 * 8, s->stream));
+    // mirror check, downwind statistics and overflow counts
+    unsigned long long* d_stats = nullptr;
+    TRY(dalloc(s, &d_stats, 5));
+    HIP_TRY(hipMemsetAsync(d_stats, 0, 5 * 8, s->stream));
     Geo g = make_geo(s);
-    k_check_mirror<<<grid_for(n), kBlock, 0, s->stream>>>(g, d_bad);
-    k_downwind_stats<<<dim3(grid_for(n), s->D), kBlock, 0, s->stream>>>(g, d_bad + 1);
+    k_check_mirror<<<grid_for(n), kBlock, 0, s->stream>>>(g, d_stats);
+    k_face_stats<<<grid_for((int64_t)DN), kBlock, 0, s->stream>>>(g, d_stats + 1, s->d_cnt);
     HIP_TRY(hipGetLastError());
-    unsigned long long hb[4] = {0, 0, 0, 0};
-    HIP_TRY(hipMemcpyAsync(hb, d_bad, sizeof(hb), hipMemcpyDeviceToHost, s->stream));
+    unsigned long long hb[5] = {0, 0, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(hb, d_stats, sizeof(hb), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    dfree(d_bad);
-    s->bytes -= 32;
+    dfree(s, d_stats);
     if (hb[0] != 0)
         return fail(SSW_ERR_GRID, std::to_string(hb[0]) +
                                       " local faces have no mirror face with the exactly negated normal");
     for (int k = 0; k < 3; k++) s->dw[k] = (double)hb[k + 1] / ((double)n * s->D);
+    s->ovf_total = (int64_t)hb[4];
+    // transport mode: compiled when its arrays fit (automatic: within half of
+    // the free device memory), else geometric
+    const size_t comp_bytes = (size_t)L * DN * (16 + real_size(s) + kPlanes * entry_size(s)) + DN * (16 + 4) +
+                              (size_t)s->ovf_total * entry_size(s);
+    const size_t geom_bytes = (size_t)L * DN * 16;
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    if (s->mode_request == SSW_TRANSPORT_MODE_COMPILED) s->compiled = true;
+    else if (s->mode_request == SSW_TRANSPORT_MODE_GEOMETRIC) s->compiled = false;
+    else s->compiled = comp_bytes <= free_b / 2;
+    if (!s->compiled && geom_bytes > free_b)
+        return fail(SSW_ERR_OOM, "task records need " + std::to_string(geom_bytes) + " bytes, " +
+                                     std::to_string(free_b) + " free");
+    TRY(dalloc(s, &s->d_flat, s->compiled ? DN : (size_t)L * DN));
+    if (s->compiled) TRY(setup_compiled(s, s->d_cnt));
     TRY(build_full_schedule(s));
     s->have_grid = true;
     return SSW_OK;
@@ -1147,7 +1501,7 @@ int ssw_upload_state(ssw_sweep* s, const double* density, const double* xhii, co
     // FP32 transport carries rates in units of a power of two near the largest
     // source per direction bin (photon rates reach 1e52 s^-1, above FP32 range).
     s->lref = (s->fp32 && lmax > 0.0) ? std::ldexp(1.0, (int)std::lround(std::log2(lmax))) : 1.0;
-    const size_t es = s->fp32 ? 4 : 8;
+    const size_t es = real_size(s);
     TRY(dalloc(s, &s->d_rho, n));
     TRY(dalloc(s, &s->d_x, n));
     TRY(dalloc(s, &s->d_T, n));
@@ -1224,8 +1578,7 @@ int ssw_download(ssw_sweep* s, double* x, double* T, double* sts, double* photon
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(photon, d_pr, n * 8, hipMemcpyDeviceToHost, s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
-        dfree(d_pr);
-        s->bytes -= n * 8;
+        dfree(s, d_pr);
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
     return SSW_OK;
@@ -1236,7 +1589,7 @@ int ssw_download_counts(ssw_sweep* s, int32_t lvl, uint32_t* counts) {
     if (!s->have_grid) return fail(SSW_ERR_INVALID, "ssw_upload_grid must be called first");
     if (lvl < 0 || lvl >= s->p.num_timestep_levels) return fail(SSW_ERR_INVALID, "sweep_level out of range");
     Geo g = make_geo(s);
-    k_count<<<dim3(grid_for(s->N), s->D), kBlock, 0, s->stream>>>(g, lvl, 1);
+    k_count<<<grid_for((int64_t)s->N * s->D), kBlock, 0, s->stream>>>(g, lvl, 1);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(counts, s->d_cnt, (size_t)s->D * s->N * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -1252,11 +1605,15 @@ int ssw_download_levelsets(ssw_sweep* s, int32_t lvl, uint32_t* depth) {
     HIP_TRY(hipMemcpyAsync(&need, s->d_need + lvl, 4, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     if (need) {  // build the schedule of this active set without transport
-        Flux<double> fl{};
-        k_begin<double><<<dim3(1, 1), kBlock, 0, s->stream>>>(g, fl, lvl);
-        k_count<<<dim3(grid_for(s->N), s->D), kBlock, 0, s->stream>>>(g, lvl, 0);
-        for (int k = 0; k < s->kmax; k++)
-            k_sweep_build<double, false><<<build_grid_x(s), kBlock, 0, s->stream>>>(g, fl, lvl, k);
+        if (s->fp32) {
+            Flux<float> fl{};
+            k_begin<float><<<1, kBlock, 0, s->stream>>>(g, fl, lvl);
+        } else {
+            Flux<double> fl{};
+            k_begin<double><<<1, kBlock, 0, s->stream>>>(g, fl, lvl);
+        }
+        k_count<<<grid_for((int64_t)s->N * s->D), kBlock, 0, s->stream>>>(g, lvl, 0);
+        for (int k = 0; k < s->kmax; k++) launch_build_any(s, g, lvl, k, false, build_blocks(s, lvl, k));
         k_fill_u32<<<1, 1, 0, s->stream>>>(s->d_need + lvl, 0u, 1);
         HIP_TRY(hipGetLastError());
         if ((int)s->replay_ok.size() > lvl) {
@@ -1268,15 +1625,16 @@ int ssw_download_levelsets(ssw_sweep* s, int32_t lvl, uint32_t* depth) {
         }
     }
     uint32_t* d_depth = nullptr;
-    TRY(dalloc(s, &d_depth, (size_t)s->D * s->N));
-    k_fill_u32<<<grid_for((int64_t)s->D * s->N), kBlock, 0, s->stream>>>(d_depth, kNone, (int64_t)s->D * s->N);
-    k_scatter_depth<<<grid_for(std::min<int64_t>((int64_t)s->D * s->N, 1 << 22)), kBlock, 0, s->stream>>>(
-        g, lvl, s->kmax, d_depth);
+    const int64_t DN = (int64_t)s->D * s->N;
+    TRY(dalloc(s, &d_depth, (size_t)DN));
+    k_fill_u32<<<grid_for(DN), kBlock, 0, s->stream>>>(d_depth, kNone, DN);
+    const uint4* tasks = s->compiled ? s->d_hdr + (size_t)lvl * DN : s->d_flat + (size_t)lvl * DN;
+    k_scatter_depth<<<grid_for(std::min<int64_t>(DN, 1 << 22)), kBlock, 0, s->stream>>>(
+        g, tasks, s->d_lstart + (size_t)lvl * (s->kcap + 2), s->kmax, s->compiled ? 1 : 0, d_depth);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(depth, d_depth, (size_t)s->D * s->N * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(depth, d_depth, (size_t)DN * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    dfree(d_depth);
-    s->bytes -= (int64_t)s->D * s->N * 4;
+    dfree(s, d_depth);
     return SSW_OK;
 }
 
@@ -1299,10 +1657,19 @@ int ssw_download_fluxes(ssw_sweep* s, double* incoming, double* outgoing, double
     if (outgoing) HIP_TRY(hipMemcpyAsync(outgoing, dO, m * 8, hipMemcpyDeviceToHost, s->stream));
     if (periodic) HIP_TRY(hipMemcpyAsync(periodic, dP, m * 8, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    dfree(dI);
-    dfree(dO);
-    dfree(dP);
-    s->bytes -= (int64_t)(3 * m * 8);
+    dfree(s, dI);
+    dfree(s, dO);
+    dfree(s, dP);
+    return SSW_OK;
+}
+
+int ssw_download_transport(ssw_sweep* s, void* X, void* I) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    if (!s->have_state) return fail(SSW_ERR_INVALID, "ssw_upload_state must be called first");
+    const size_t m = (size_t)s->D * s->N * real_size(s);
+    if (X) HIP_TRY(hipMemcpyAsync(X, s->d_X, m, hipMemcpyDeviceToHost, s->stream));
+    if (I) HIP_TRY(hipMemcpyAsync(I, s->d_I, m, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
     return SSW_OK;
 }
 
@@ -1318,6 +1685,11 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info) {
     info->downwind_periodic = s->dw[1];
     info->downwind_boundary = s->dw[2];
     info->device_bytes = s->bytes;
+    info->transport_mode = s->have_grid ? (s->compiled ? SSW_TRANSPORT_MODE_COMPILED : SSW_TRANSPORT_MODE_GEOMETRIC)
+                                        : s->mode_request;
+    info->schedule_builds_last_step = s->builds_last;
+    info->overflow_entries = s->ovf_total;
+    info->graph_capture_failures = s->capture_failures;
     if (s->have_state) {
         unsigned long long f = 0, a = 0;
         HIP_TRY(hipMemcpyAsync(&f, s->d_fail, 8, hipMemcpyDeviceToHost, s->stream));
@@ -1356,12 +1728,21 @@ double ssw_chemistry_timestep(double* state, double density, double volume, doub
     return ts;
 }
 
-int ssw_set_transport_mode(ssw_sweep* s, int32_t mode) {
-    if (!s) return fail(SSW_ERR_INVALID, "null handle");
-    if (mode != 0)
-        return fail(SSW_ERR_INVALID, "transport mode " + std::to_string(mode) +
-                                         " is not available: only the level-synchronous transport (0) is built");
-    return SSW_OK;
+int64_t ssw_estimate_device_bytes(const ssw_sweep_parameters* p, int64_t n, int64_t F, int64_t num_periodic_cells,
+                                  int32_t mode) {
+    if (!p || n <= 0) return 0;
+    const int64_t D = p->num_directions, L = p->num_timestep_levels;
+    const int64_t rs = p->transport_precision == SSW_TRANSPORT_FP32 ? 4 : 8;
+    const int64_t es = rs == 4 ? 8 : 16;
+    const int64_t DN = D * n;
+    int64_t b = 0;
+    b += (n + 1) * 4 + F * 32 + F * 4 + n * 16 + n * 4 + num_periodic_cells * 8 + F * 4 + n + L * 12 + DN * 4;
+    b += L * (n + 2) * 4;                          // level starts (depth < number of cells)
+    b += (mode == SSW_TRANSPORT_MODE_COMPILED) ? DN * 16 : L * DN * 16;
+    if (mode == SSW_TRANSPORT_MODE_COMPILED) b += L * DN * (16 + rs + kPlanes * es) + DN * 4 + DN * es / 16;
+    b += n * 8 * 9 + 2 * DN * rs + D * num_periodic_cells * rs;
+    b += n * 8;                                    // photon-rate download
+    return b + b / 20 + (1 << 20);
 }
 
 int ssw_set_timing(ssw_sweep* s, int32_t enabled) {
@@ -1370,12 +1751,13 @@ int ssw_set_timing(ssw_sweep* s, int32_t enabled) {
     return SSW_OK;
 }
 
-int ssw_get_timing(ssw_sweep* s, double* avg, double* total, int64_t* launches, double* chem) {
+int ssw_get_timing(ssw_sweep* s, double* avg, double* total, int64_t* launches, double* chem, double* build) {
     if (!s) return fail(SSW_ERR_INVALID, "null handle");
     if (avg) *avg = s->n_transport ? s->t_transport_total / (double)s->n_transport : 0.0;
     if (total) *total = s->t_transport_total;
     if (launches) *launches = s->n_transport;
     if (chem) *chem = s->t_chem_total;
+    if (build) *build = s->t_build_total;
     return SSW_OK;
 }
 

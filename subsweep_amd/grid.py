@@ -150,7 +150,7 @@ def gaussian_random_field(mesh: int, slope: float, smoothing: float, seed: int) 
     with np.errstate(divide="ignore"):
         amp = np.where(k2 > 0, k2 ** (slope / 4.0), 0.0)  # sqrt(P) = k^(slope/2)
     amp = amp * np.exp(-0.5 * (2.0 * np.pi) ** 2 * k2 * smoothing ** 2)
-    g = np.fft.irfftn(fk * amp, s=(mesh, mesh, mesh))
+    g = np.fft.irfftn(fk * amp, s=(mesh, mesh, mesh), axes=(0, 1, 2))
     g -= g.mean()
     g /= g.std()
     return g

@@ -82,7 +82,8 @@ class SweepResults:
 class Sweep:
     """Device-resident sweep state of one problem (one HIP stream)."""
 
-    def __init__(self, params: SweepParameters, grid: Grid, density, ionized_fraction, temperature, source):
+    def __init__(self, params: SweepParameters, grid: Grid, density, ionized_fraction, temperature, source,
+                 transport_mode: int = N.TRANSPORT_MODE_AUTO):
         lib = N.sweep_lib()
         self._lib = lib
         self.params = params
@@ -93,6 +94,7 @@ class Sweep:
         self._h = h
         self.num_cells = grid.num_cells
         try:
+            N.check(lib.ssw_set_transport_mode(h, int(transport_mode)))
             off, nbr, kind, area, normal, vol, size = grid.arrays()
             N.check(lib.ssw_upload_grid(h, grid.num_cells, N.ptr(off, C.c_int64), N.ptr(nbr, C.c_int32),
                                         N.ptr(kind, C.c_uint8), N.ptr(area, C.c_double), N.ptr(normal, C.c_double),
@@ -151,20 +153,32 @@ class Sweep:
         return inf
 
     def set_transport_mode(self, mode: int) -> None:
-        """0 (the only mode): one launch per Kahn level set, replayed as a
-        captured graph. Other values raise: the persistent kernel they
-        selected has been removed."""
+        """Replay form (TRANSPORT_MODE_AUTO / _GEOMETRIC / _COMPILED). Only
+        valid before the grid is uploaded, so pass `transport_mode` to the
+        constructor; 1 and 2 (a removed persistent kernel) raise."""
         N.check(self._lib.ssw_set_transport_mode(self._h, int(mode)))
+
+    def transport_state(self):
+        """Raw X (outgoing rate / downwind area) and I in their storage
+        precision and scale, shape (directions, cells)."""
+        dt = np.float32 if self.params.transport_precision == 32 else np.float64
+        m = self.num_directions * self.num_cells
+        x, i = np.empty(m, dt), np.empty(m, dt)
+        N.check(self._lib.ssw_download_transport(self._h, x.ctypes.data_as(C.c_void_p),
+                                                 i.ctypes.data_as(C.c_void_p)))
+        shape = (self.num_directions, self.num_cells)
+        return x.reshape(shape), i.reshape(shape)
 
     def set_timing(self, enabled: bool) -> None:
         N.check(self._lib.ssw_set_timing(self._h, 1 if enabled else 0))
 
     def timing(self):
-        avg, tot, chem = C.c_double(), C.c_double(), C.c_double()
+        avg, tot, chem, build = C.c_double(), C.c_double(), C.c_double(), C.c_double()
         n = C.c_int64()
-        N.check(self._lib.ssw_get_timing(self._h, C.byref(avg), C.byref(tot), C.byref(n), C.byref(chem)))
+        N.check(self._lib.ssw_get_timing(self._h, C.byref(avg), C.byref(tot), C.byref(n), C.byref(chem),
+                                         C.byref(build)))
         return {"transport_avg_ms": avg.value, "transport_total_ms": tot.value, "transport_launches": n.value,
-                "chemistry_total_ms": chem.value}
+                "chemistry_total_ms": chem.value, "build_total_ms": build.value}
 
     def close(self) -> None:
         if getattr(self, "_h", None):
