@@ -49,10 +49,10 @@ def bytes_per_update(info, precision: int) -> float:
     return base + info.downwind_local * bl + info.downwind_periodic * bp + info.downwind_boundary * bb
 
 
-def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3, mode: int = 0):
+def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3, mode: int = 0, groups: int = 0):
     from subsweep_amd import Sweep
     pp = pb.with_params(transport_precision=precision)
-    s = Sweep(pp.params, pp.grid, *pp.state(), transport_mode=mode)
+    s = Sweep(pp.params, pp.grid, *pp.state(), transport_mode=mode, direction_groups=groups)
     for _ in range(warmup):
         s.run_sweeps()
     s.synchronize()
@@ -93,7 +93,8 @@ def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3,
         "bytes_per_update": bpu,
         "achieved_gbs": achieved,
         "kernel": KERNELS.get(info.transport_mode, "?"),
-        "build_ms_per_step": None,
+        "direction_groups": int(info.direction_groups),
+        "transport_mode": int(info.transport_mode),
         "info": info,
         "results": res,
     }
@@ -117,7 +118,20 @@ def field_errors(a, b):
     return out
 
 
-def run_clustered(steps: int = 4, warmup: int = 2):
+def tuned(pb, precision: int, warmup: int, enabled: bool, mode: int, groups: int):
+    """(mode, groups) for a run: the autotuner's choice, or the command line's."""
+    if not enabled:
+        return mode, groups, None
+    from subsweep_amd.autotune import autotune
+    t = time.perf_counter()
+    r = autotune(pb.with_params(transport_precision=precision), warmup=warmup)
+    log(f"autotune fp{precision}: groups {r.direction_groups}, mode {r.transport_mode}, {r.ms_per_step:.2f} ms/step "
+        f"({time.perf_counter() - t:.1f} s; " +
+        ", ".join(f"g{g}/m{m} {ms:.2f}" for g, m, ms in r.table) + ")")
+    return r.transport_mode, r.direction_groups, [list(x) for x in r.table]
+
+
+def run_clustered(steps: int = 4, warmup: int = 2, autotune: bool = True):
     """2097152-cell clustered grid, 84 directions, 4 timestep levels, 1024
     sources: FP32 transport vs FP64 transport (FP64 chemistry in both). The
     warmup steps are the two first steps, which sweep only the top level."""
@@ -126,9 +140,11 @@ def run_clustered(steps: int = 4, warmup: int = 2):
     pb = W.clustered_2m()
     log(f"clustered grid: {pb.grid.num_cells} cells, {pb.grid.num_faces} faces, built in "
         f"{time.perf_counter() - t:.1f} s")
-    r32 = run_gpu(pb, 32, steps, warmup, profile_steps=1)
+    m32, g32, t32 = tuned(pb, 32, warmup, autotune, 0, 0)
+    r32 = run_gpu(pb, 32, steps, warmup, profile_steps=1, mode=m32, groups=g32)
     log(f"clustered fp32: {r32['updates_per_s'] / 1e9:.3f} G updates/s, {r32['ms_per_step']:.1f} ms/step")
-    r64 = run_gpu(pb, 64, steps, warmup, profile_steps=1)
+    m64, g64, t64 = tuned(pb, 64, warmup, autotune, 0, 0)
+    r64 = run_gpu(pb, 64, steps, warmup, profile_steps=1, mode=m64, groups=g64)
     log(f"clustered fp64: {r64['updates_per_s'] / 1e9:.3f} G updates/s, {r64['ms_per_step']:.1f} ms/step")
     return {
         "workload": "2097152-cell clustered lognormal-field Voronoi grid (sigma_ln 2), 84 directions, "
@@ -141,8 +157,12 @@ def run_clustered(steps: int = 4, warmup: int = 2):
         "fp32_over_fp64": r32["updates_per_s"] / r64["updates_per_s"],
         "fp32_ms_per_step": r32["ms_per_step"],
         "fp64_ms_per_step": r64["ms_per_step"],
-        "fp32_transport_mode": int(r32["info"].transport_mode),
-        "fp64_transport_mode": int(r64["info"].transport_mode),
+        "fp32_transport_mode": r32["transport_mode"],
+        "fp64_transport_mode": r64["transport_mode"],
+        "fp32_direction_groups": r32["direction_groups"],
+        "fp64_direction_groups": r64["direction_groups"],
+        "autotune_fp32_ms_per_step": t32,
+        "autotune_fp64_ms_per_step": t64,
         "fp32_error_vs_fp64": field_errors(r32["results"], r64["results"]),
     }
 
@@ -206,7 +226,7 @@ def serving_requests(pool, count: int, seed: int, steps: int):
     return out
 
 
-def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, steps: int = 10):
+def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, steps: int = 10, groups: int = 1):
     import threading
     import numpy as np
     from subsweep_amd.serve import Server
@@ -219,8 +239,9 @@ def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, step
     # capacity: a saturated batch of 8 requests on the same server shape
     with Server(slots=slots, hbm_budget_bytes=budget) as sv:
         t0 = time.perf_counter()
-        cal = [sv.wait(x) for x in [sv.submit(r, steps) for r in reqs[:8]]]
+        cal = [sv.wait(x) for x in [sv.submit(r, steps, direction_groups=groups) for r in reqs[:8]]]
         capacity = len(cal) / (time.perf_counter() - t0)
+    log(f"serving: calibration batch of {len(cal)} requests at {capacity:.2f} requests/s")
     rate = 0.8 * capacity  # Poisson arrivals at 80 % of the measured capacity
     gaps = np.random.default_rng(seed + 2).exponential(1.0 / rate, size=count)
     arrivals = np.cumsum(gaps) - gaps[0]
@@ -233,12 +254,18 @@ def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, step
                 delay = arrivals[i] - (time.perf_counter() - start)
                 if delay > 0:
                     time.sleep(delay)
-                tickets[i] = sv.submit(r, steps)
+                tickets[i] = sv.submit(r, steps, direction_groups=groups)
 
         th = threading.Thread(target=feeder)
         th.start()
+        res = []
+        for i in range(count):
+            while tickets[i] is None:
+                time.sleep(0.01)
+            res.append(sv.wait(tickets[i]))
+            if (i + 1) % 32 == 0:
+                log(f"serving: {i + 1}/{count} done at {time.perf_counter() - start:.1f} s")
         th.join()
-        res = [sv.wait(x) for x in tickets]
         stats = sv.stats()
     bad = [r for r in res if r.status != 0]
     if bad:
@@ -252,6 +279,7 @@ def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, step
         "slots": slots,
         "hbm_budget_gb": budget_gb,
         "steps_per_request": steps,
+        "direction_groups_per_request": groups,
         "arrival_rate_per_s": rate,
         "arrival_rule": "Poisson at 0.8 x the completion rate of a saturated 8-request batch",
         "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
@@ -280,6 +308,9 @@ def main():
                     help="skip the 2097152-cell clustered FP32-vs-FP64 measurement")
     ap.add_argument("--mode", type=int, default=0, choices=(0, 3, 4),
                     help="transport replay form: 0 automatic, 3 geometric, 4 compiled")
+    ap.add_argument("--groups", type=int, default=0, help="direction groups (0: library default)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use --mode/--groups instead of the autotuner's choice")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -301,12 +332,16 @@ def main():
     pb = W.precision_64()
     log(f"grid: {pb.grid.num_cells} cells, {pb.grid.num_faces} faces, built in {time.perf_counter() - t:.1f} s")
 
+    tune = not args.no_autotune and args.precision == 0
+    tune32 = tune64 = None
     if args.precision in (0, 32):
-        fp32 = run_gpu(pb, 32, args.steps, args.warmup, mode=args.mode)
+        m, g, tune32 = tuned(pb, 32, args.warmup, tune, args.mode, args.groups)
+        fp32 = run_gpu(pb, 32, args.steps, args.warmup, mode=m, groups=g)
         log(f"fp32 transport: {fp32['updates_per_s'] / 1e9:.3f} G updates/s, {fp32['ms_per_step']:.2f} ms/step, "
             f"kernel {fp32['avg_launch_ms']:.3f} ms/launch, {fp32['achieved_gbs']:.0f} GB/s")
     if args.precision in (0, 64):
-        fp64 = run_gpu(pb, 64, args.steps, args.warmup, mode=args.mode)
+        m, g, tune64 = tuned(pb, 64, args.warmup, tune, args.mode, args.groups)
+        fp64 = run_gpu(pb, 64, args.steps, args.warmup, mode=m, groups=g)
         log(f"fp64 transport: {fp64['updates_per_s'] / 1e9:.3f} G updates/s, {fp64['ms_per_step']:.2f} ms/step, "
             f"kernel {fp64['avg_launch_ms']:.3f} ms/launch, {fp64['achieved_gbs']:.0f} GB/s")
     if args.precision != 0:  # profiling runs: no JSON line
@@ -322,7 +357,7 @@ def main():
 
     clustered = None
     if not args.no_clustered:
-        clustered = run_clustered()
+        clustered = run_clustered(autotune=not args.no_autotune)
 
     serving = None
     if args.serving_requests > 0:
@@ -378,6 +413,10 @@ def main():
             "fp64_achieved": fp64["achieved_gbs"],
             "fp64_bytes_per_update": fp64["bytes_per_update"],
         },
+        "launch_shape": {"fp32": {"direction_groups": fp32["direction_groups"],
+                                  "transport_mode": fp32["transport_mode"], "autotune_ms_per_step": tune32},
+                         "fp64": {"direction_groups": fp64["direction_groups"],
+                                  "transport_mode": fp64["transport_mode"], "autotune_ms_per_step": tune64}},
         "cpu_baseline": cpu,
         "fp32_error_vs_fp64": field_errors(fp32["results"], fp64["results"]),
         "clustered_2m": clustered,

@@ -148,6 +148,8 @@ typedef struct ssw_info {
     int64_t overflow_entries;      /* compiled upwind entries beyond the planes   */
     int64_t graph_capture_failures; /* replays launched one by one because their
                                        graph could not be captured            */
+    int32_t direction_groups;      /* groups whose level sets run side by side */
+    int32_t reserved0;
 } ssw_info;
 
 int ssw_get_info(ssw_sweep* s, ssw_info* info);
@@ -168,6 +170,14 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info);
 #define SSW_TRANSPORT_MODE_GEOMETRIC 3
 #define SSW_TRANSPORT_MODE_COMPILED 4
 int ssw_set_transport_mode(ssw_sweep* s, int32_t mode);
+
+/* Directions are split into `groups` contiguous blocks (1..16; 0: the
+ * library default of 4, at most the direction count); each block's level
+ * sets run on a HIP stream of its own, so the small, latency-bound launches
+ * of different blocks overlap on the device. Dependencies never cross
+ * directions, so results do not depend on the grouping (bitwise). Set before
+ * ssw_upload_grid. No counterpart in the reference. */
+int ssw_set_direction_groups(ssw_sweep* s, int32_t groups);
 
 /* Raw transport state in its storage precision and scale (FP32 or FP64 per
  * the transport precision, direction-major): X = outgoing rate / downwind
@@ -222,6 +232,8 @@ typedef struct ssw_request {
     int32_t transport_mode;        /* SSW_TRANSPORT_MODE_*; AUTO: compiled if it
                                       fits budget / slots, else geometric      */
     int32_t steps;                 /* ssw_step calls                          */
+    int32_t direction_groups;      /* as ssw_set_direction_groups (0: default) */
+    int32_t reserved0;
     int64_t num_cells;             /* grid, as for ssw_upload_grid            */
     const int64_t* face_offsets;
     const int32_t* face_neighbour;

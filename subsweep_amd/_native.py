@@ -66,6 +66,8 @@ class InfoC(C.Structure):
         ("schedule_builds_last_step", C.c_int32),
         ("overflow_entries", C.c_int64),
         ("graph_capture_failures", C.c_int64),
+        ("direction_groups", C.c_int32),
+        ("reserved0", C.c_int32),
     ]
 
 
@@ -80,6 +82,8 @@ class RequestC(C.Structure):
         ("params", SweepParametersC),
         ("transport_mode", C.c_int32),
         ("steps", C.c_int32),
+        ("direction_groups", C.c_int32),
+        ("reserved0", C.c_int32),
         ("num_cells", C.c_int64),
         ("face_offsets", C.c_void_p),
         ("face_neighbour", C.c_void_p),
@@ -137,6 +141,7 @@ SWEEP_SIGNATURES = {
     "ssw_synchronize": (C.c_int, [P]),
     "ssw_set_transport_mode": (C.c_int, [P, C.c_int32]),
     "ssw_set_timing": (C.c_int, [P, C.c_int32]),
+    "ssw_set_direction_groups": (C.c_int, [P, C.c_int32]),
     "ssw_get_timing": (C.c_int, [P, PD, PD, PI64, PD, PD]),
     "ssw_download_transport": (C.c_int, [P, P, P]),
     "ssw_timestep_level": (C.c_int32, [C.c_int32, C.c_double, C.c_double]),

@@ -47,6 +47,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
+#include <mutex>
+#include <thread>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -76,6 +79,7 @@ constexpr int kMaxFaces = 127;             // faces per cell
 constexpr int kQueueCap = kTasksPerBlock * kMaxFaces;
 constexpr int kPlanes = 12;                // upwind entries per task held in planes
 constexpr int kRecBlock = 64;              // compiled replay: one lane per task
+constexpr int kDefaultGroups = 4;          // direction groups run side by side
 
 thread_local std::string g_err;
 
@@ -125,7 +129,8 @@ struct Geo {
     int64_t N;
     int D;
     int L;
-    int Kcap;                // level starts per slot = Kcap + 2
+    int G;                   // direction groups (schedules run side by side)
+    int Kcap;                // level starts per (slot, group) = Kcap + 2
     const uint32_t* off;
     const double4* geo;
     const uint32_t* nk;
@@ -136,10 +141,12 @@ struct Geo {
     const uint32_t* pfoff;   // Nper + 1: periodic faces of each periodic cell
     const uint32_t* pfidx;   // face indices
     int64_t Nper;
+    const uint8_t* dgroup;   // D: direction group of each direction
+    const uint32_t* gbase;   // G: first task position of a group within a slot (N x its first direction)
     uint4* flat;             // task records: per slot (geometric) or one slot of scratch (compiled)
     size_t flat_slot_stride; // D * N (geometric) or 0 (compiled)
-    uint32_t* lstart;        // L x (Kcap + 2)
-    uint32_t* ctr;           // L x 3 rotating append counters
+    uint32_t* lstart;        // L x G x (Kcap + 2), positions relative to the group's base
+    uint32_t* ctr;           // L x G x 3 rotating append counters
     uint32_t* cnt;           // D x N missing-upwind counters
     uint32_t* need;          // L: schedule must be (re)built
 };
@@ -388,9 +395,9 @@ __global__ void k_overflow(Geo g, double inv_aref, const uint32_t* ovf_count, ui
 // Thread t handles (periodic cell t / D, direction t % D).
 template <typename R>
 __global__ void k_begin(Geo g, Flux<R> fl, int lvl) {
-    if (blockIdx.x == 0 && threadIdx.x < 3 && g.need[lvl] != 0) {
-        g.ctr[lvl * 3 + threadIdx.x] = 0;
-        if (threadIdx.x == 0) g.lstart[(size_t)lvl * (g.Kcap + 2)] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < 3 * g.G && g.need[lvl] != 0) {
+        g.ctr[lvl * 3 * g.G + threadIdx.x] = 0;
+        if (threadIdx.x < g.G) g.lstart[((size_t)lvl * g.G + threadIdx.x) * (g.Kcap + 2)] = 0;
     }
     if (fl.P == nullptr) return;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -438,8 +445,10 @@ __global__ void k_count(Geo g, int lvl, int only_counts) {
     if (only_counts) return;
     uint4* flat = g.flat + (size_t)lvl * g.flat_slot_stride;
     const bool ready = active && n == 0;
-    wave_append(ready, ready ? task_record(c, f0, f1, d, g.pslot[c]) : make_uint4(0, 0, 0, 0), &g.ctr[lvl * 3 + 0],
-                flat);
+    const uint32_t grp = in ? g.dgroup[d] : 0u;
+    const uint4 rec = ready ? task_record(c, f0, f1, d, g.pslot[c]) : make_uint4(0, 0, 0, 0);
+    for (int q = 0; q < g.G; q++)  // one aggregated append per group present in the wave
+        wave_append(ready && grp == (uint32_t)q, rec, &g.ctr[((size_t)lvl * g.G + q) * 3 + 0], flat + g.gbase[q]);
 }
 
 // Builds level set k+1 of slot `lvl` while solving level set k (kTransport) or
@@ -447,11 +456,12 @@ __global__ void k_count(Geo g, int lvl, int only_counts) {
 // lanes per task; ready tasks are gathered per block in LDS and appended with
 // one atomic per block. With kCompile also writes the compiled schedule.
 template <typename R, bool kTransport, bool kCompile>
-__global__ void __launch_bounds__(kBlock) k_sweep_build(Geo g, Flux<R> fl, Comp<R> cp, int lvl, int k) {
+__global__ void __launch_bounds__(kBlock) k_sweep_build(Geo g, Flux<R> fl, Comp<R> cp, int lvl, int k, int grp) {
     __shared__ uint4 sq[kQueueCap];
     __shared__ uint32_t sq_n, sq_base, sq_cnt;
-    uint32_t* ls = g.lstart + (size_t)lvl * (g.Kcap + 2);
-    uint32_t* ctr = g.ctr + lvl * 3;
+    const size_t unit = (size_t)lvl * g.G + grp;
+    uint32_t* ls = g.lstart + unit * (g.Kcap + 2);
+    uint32_t* ctr = g.ctr + unit * 3;
     const uint32_t beg = ls[k];
     const uint32_t end = beg + ctr[k % 3];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -460,10 +470,11 @@ __global__ void __launch_bounds__(kBlock) k_sweep_build(Geo g, Flux<R> fl, Comp<
     }
     if (beg >= end) return;
     const uint32_t W = end - beg;
-    uint4* flat = g.flat + (size_t)lvl * g.flat_slot_stride;
+    const uint32_t gb = g.gbase[grp];
+    uint4* flat = g.flat + (size_t)lvl * g.flat_slot_stride + gb;
     uint32_t* next_ctr = ctr + (k + 1) % 3;
     const size_t DN = (size_t)g.D * g.N;
-    Entry<R>* planes = kCompile ? cp.planes + (size_t)lvl * kPlanes * DN + (size_t)kPlanes * beg : nullptr;
+    Entry<R>* planes = kCompile ? cp.planes + (size_t)lvl * kPlanes * DN + (size_t)kPlanes * (gb + beg) : nullptr;
     const uint32_t sub = threadIdx.x % kLanes;
     const uint32_t stride = gridDim.x * kTasksPerBlock;
     if (threadIdx.x == 0) sq_n = 0;
@@ -486,7 +497,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep_build(Geo g, Flux<R> fl, Comp<
                 finish_task<R>(fl, o, c, inc, per, (R)A);
             }
             if (kCompile) {
-                const size_t so = (size_t)lvl * DN + i;
+                const size_t so = (size_t)lvl * DN + gb + i;
                 cp.hdr[so] = make_uint4(c | (d << kCellBits), ps, nup > (uint32_t)kPlanes ? cp.ovf_start[o] : 0u, nup);
                 cp.A[so] = (R)A;
             }
@@ -750,6 +761,65 @@ struct TimestepState {
 
 unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kBlock - 1) / kBlock); }
 
+// Host threads that submit the launches of direction groups 1..G-1 while the
+// calling thread submits group 0 (a kernel launch costs microseconds of host
+// time, so one thread cannot keep several streams fed).
+struct SubmitPool {
+    std::vector<std::thread> threads;
+    std::mutex mu;
+    std::condition_variable cv, cv_done;
+    std::function<void(int)> job;
+    uint64_t generation = 0;
+    int pending = 0;
+    bool quit = false;
+
+    void start(int workers) {
+        for (int i = 0; i < workers; i++)
+            threads.emplace_back([this, i] {
+                uint64_t seen = 0;
+                for (;;) {
+                    std::function<void(int)> j;
+                    {
+                        std::unique_lock<std::mutex> lk(mu);
+                        cv.wait(lk, [&] { return quit || generation != seen; });
+                        if (quit) return;
+                        seen = generation;
+                        j = job;
+                    }
+                    j(i + 1);
+                    {
+                        std::lock_guard<std::mutex> lk(mu);
+                        pending--;
+                    }
+                    cv_done.notify_one();
+                }
+            });
+    }
+    // Runs f(0) here and f(1..workers) on the pool; returns when all are done.
+    void run(const std::function<void(int)>& f) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            job = f;
+            pending = (int)threads.size();
+            generation++;
+        }
+        cv.notify_all();
+        f(0);
+        std::unique_lock<std::mutex> lk(mu);
+        cv_done.wait(lk, [&] { return pending == 0; });
+    }
+    void stop() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            quit = true;
+        }
+        cv.notify_all();
+        for (std::thread& th : threads) th.join();
+        threads.clear();
+        quit = false;
+    }
+};
+
 }  // namespace
 
 struct ssw_sweep {
@@ -769,10 +839,21 @@ struct ssw_sweep {
     double4* d_geo = nullptr;
     double *d_dirs = nullptr, *d_vol = nullptr, *d_size = nullptr;
     double aref = 1.0;
-    int kmax = 0;
+    int kmax = 0;                                   // deepest group
     double dw[3] = {0, 0, 0};
     int64_t ovf_total = 0;
-    std::vector<uint32_t> full_start;               // level starts of the full graph
+    // direction groups: each group's level sets run on its own stream
+    int G = 1;
+    int G_request = 0;                              // 0: library default
+    std::vector<uint32_t> gbase;                    // first task position of each group in a slot
+    std::vector<int> kmax_g;                        // depth bound of each group
+    uint8_t* d_dgroup = nullptr;
+    uint32_t* d_gbase = nullptr;
+    std::vector<hipStream_t> gstreams;
+    std::vector<hipEvent_t> gjoin;
+    hipEvent_t fork = nullptr;
+    SubmitPool pool;                                // G - 1 submitting threads
+    std::vector<std::vector<uint32_t>> full_start; // per group: level starts of the full graph
     // schedule
     uint4* d_flat = nullptr;
     uint32_t *d_lstart = nullptr, *d_ctr = nullptr, *d_cnt = nullptr, *d_need = nullptr;
@@ -783,9 +864,10 @@ struct ssw_sweep {
     void* d_ovf = nullptr;
     int64_t compiled_bytes = 0;
     int kcap = 0;
-    std::vector<int> replay_ok;                     // per slot: host copy of the level starts and graph valid
-    std::vector<std::vector<uint32_t>> level_start; // per slot: lstart[0..kmax] of its last build
-    std::vector<hipGraphExec_t> graphs;             // per slot: replay of its level sets
+    std::vector<int> replay_ok;                     // per slot: host copy of the level starts valid
+    std::vector<int> replays;                       // per slot: replays since its last build
+    std::vector<std::vector<uint32_t>> level_start; // per (slot, group): level starts of its last build
+    std::vector<hipGraphExec_t> graphs;             // per (slot, group): replay of its level sets
     // state
     double *d_rho = nullptr, *d_x = nullptr, *d_T = nullptr, *d_src = nullptr, *d_att = nullptr;
     double *d_sts = nullptr, *d_prev = nullptr, *d_change = nullptr, *d_src_t = nullptr;
@@ -846,7 +928,10 @@ Geo make_geo(ssw_sweep* s) {
     g.N = s->N;
     g.D = s->D;
     g.L = s->p.num_timestep_levels;
+    g.G = s->G;
     g.Kcap = s->kcap;
+    g.dgroup = s->d_dgroup;
+    g.gbase = s->d_gbase;
     g.off = s->d_off;
     g.geo = s->d_geo;
     g.nk = s->d_nk;
@@ -906,11 +991,14 @@ void invalidate_replays(ssw_sweep* s) {
     std::fill(s->replay_ok.begin(), s->replay_ok.end(), 0);
 }
 
-// Blocks for building level set k of slot `lvl`: from the widths of the slot's
+hipStream_t gstream(ssw_sweep* s, int grp) { return s->G > 1 ? s->gstreams[grp] : s->stream; }
+
+// Blocks for building level set k of (slot, group): from the widths of its
 // last build, else of the full graph, with headroom; the kernel's grid-stride
 // loop covers wider level sets.
-unsigned build_blocks(ssw_sweep* s, int lvl, int k) {
-    const std::vector<uint32_t>& st = s->level_start[lvl].size() > (size_t)k + 1 ? s->level_start[lvl] : s->full_start;
+unsigned build_blocks(ssw_sweep* s, int lvl, int grp, int k) {
+    const std::vector<uint32_t>& last = s->level_start[(size_t)lvl * s->G + grp];
+    const std::vector<uint32_t>& st = last.size() > (size_t)k + 1 ? last : s->full_start[grp];
     int64_t w = 1024;
     if (st.size() > (size_t)k + 1) w = (int64_t)(st[k + 1] - st[k]);
     w = w + w / 4 + 64;
@@ -918,163 +1006,209 @@ unsigned build_blocks(ssw_sweep* s, int lvl, int k) {
 }
 
 template <typename R>
-void launch_build(ssw_sweep* s, const Geo& g, int lvl, int k, bool transport, unsigned blocks) {
+void launch_build(ssw_sweep* s, const Geo& g, int lvl, int grp, int k, bool transport, unsigned blocks,
+                  hipStream_t st) {
     Flux<R> fl = make_flux<R>(s);
     Comp<R> cp = make_comp<R>(s);
     if (transport) {
-        if (s->compiled) k_sweep_build<R, true, true><<<blocks, kBlock, 0, s->stream>>>(g, fl, cp, lvl, k);
-        else k_sweep_build<R, true, false><<<blocks, kBlock, 0, s->stream>>>(g, fl, cp, lvl, k);
+        if (s->compiled) k_sweep_build<R, true, true><<<blocks, kBlock, 0, st>>>(g, fl, cp, lvl, k, grp);
+        else k_sweep_build<R, true, false><<<blocks, kBlock, 0, st>>>(g, fl, cp, lvl, k, grp);
     } else {
-        if (s->compiled) k_sweep_build<R, false, true><<<blocks, kBlock, 0, s->stream>>>(g, fl, cp, lvl, k);
-        else k_sweep_build<R, false, false><<<blocks, kBlock, 0, s->stream>>>(g, fl, cp, lvl, k);
+        if (s->compiled) k_sweep_build<R, false, true><<<blocks, kBlock, 0, st>>>(g, fl, cp, lvl, k, grp);
+        else k_sweep_build<R, false, false><<<blocks, kBlock, 0, st>>>(g, fl, cp, lvl, k, grp);
     }
 }
 
-void launch_build_any(ssw_sweep* s, const Geo& g, int lvl, int k, bool transport, unsigned blocks) {
-    if (s->fp32) launch_build<float>(s, g, lvl, k, transport, blocks);
-    else launch_build<double>(s, g, lvl, k, transport, blocks);
+void launch_build_any(ssw_sweep* s, const Geo& g, int lvl, int grp, int k, bool transport, unsigned blocks,
+                      hipStream_t st) {
+    if (s->fp32) launch_build<float>(s, g, lvl, grp, k, transport, blocks, st);
+    else launch_build<double>(s, g, lvl, grp, k, transport, blocks, st);
 }
 
 // Builds the schedule of the full graph (all cells active) once per grid, with
-// no compiled output: the depth bound kmax and the level widths; a graph that
-// never empties its queue has a cycle. Grows the level starts as needed.
+// no compiled output: the depth bound of every group and its level widths; a
+// graph that never empties its queue has a cycle. The level starts are sized
+// by a guess and the build restarts with more room if a group is deeper.
 int build_full_schedule(ssw_sweep* s) {
-    const int L = s->p.num_timestep_levels;
-    int kcap = 256;
-    s->kcap = kcap;
-    TRY(dalloc(s, &s->d_lstart, (size_t)L * (kcap + 2)));
-    TRY(set_need_all(s, 1u));
-    Geo g = make_geo(s);
+    const int L = s->p.num_timestep_levels, G = s->G;
     const bool was_compiled = s->compiled;
-    s->compiled = false;  // records only
-    g.flat_slot_stride = 0;
-    Flux<double> fl{};
-    Comp<double> cp{};
-    k_begin<double><<<1, kBlock, 0, s->stream>>>(g, fl, 0);
-    k_count<<<grid_for((int64_t)s->N * s->D), kBlock, 0, s->stream>>>(g, 0, 0);
-    HIP_TRY(hipGetLastError());
-    s->full_start.assign(1, 0u);
-    int64_t total = 0;
-    int k = 0;
-    int rc = SSW_OK;
-    for (;; k++) {
-        if (k + 2 > kcap) {  // grow the level starts of slot 0
-            int ncap = kcap * 2;
-            uint32_t* nl = nullptr;
-            rc = dalloc(s, &nl, (size_t)L * (ncap + 2));
-            if (rc != SSW_OK) break;
-            HIP_TRY(hipMemcpyAsync(nl, s->d_lstart, (kcap + 2) * 4, hipMemcpyDeviceToDevice, s->stream));
-            HIP_TRY(hipStreamSynchronize(s->stream));
-            dfree(s, s->d_lstart);
-            s->d_lstart = nl;
-            kcap = ncap;
-            s->kcap = kcap;
-            g.lstart = nl;
-            g.Kcap = kcap;
-        }
-        uint32_t width = 0;
-        HIP_TRY(hipMemcpyAsync(&width, s->d_ctr + (k % 3), 4, hipMemcpyDeviceToHost, s->stream));
-        HIP_TRY(hipStreamSynchronize(s->stream));
-        if (width == 0) break;
-        total += width;
-        s->full_start.push_back((uint32_t)total);
-        if (k > s->N + 1) {
-            rc = fail(SSW_ERR_CYCLE, "dependency graph deeper than the number of cells");
-            break;
-        }
-        unsigned blocks = (unsigned)std::min<int64_t>(16384, (width + kTasksPerBlock - 1) / kTasksPerBlock);
-        k_sweep_build<double, false, false><<<blocks, kBlock, 0, s->stream>>>(g, fl, cp, 0, k);
+    for (int kcap = 1024;; kcap *= 4) {
+        s->kcap = kcap;
+        TRY(dalloc(s, &s->d_lstart, (size_t)L * G * (kcap + 2)));
+        TRY(set_need_all(s, 1u));
+        s->compiled = false;  // records only
+        Geo g = make_geo(s);
+        g.flat_slot_stride = 0;
+        Flux<double> fl{};
+        Comp<double> cp{};
+        k_begin<double><<<1, kBlock, 0, s->stream>>>(g, fl, 0);
+        k_count<<<grid_for((int64_t)s->N * s->D), kBlock, 0, s->stream>>>(g, 0, 0);
         HIP_TRY(hipGetLastError());
+        s->full_start.assign(G, std::vector<uint32_t>(1, 0u));
+        std::vector<int64_t> total(G, 0);
+        std::vector<int> depth(G, -1);
+        std::vector<uint32_t> c3(3 * G);
+        bool grow = false;
+        int rc = SSW_OK;
+        for (int k = 0;; k++) {
+            HIP_TRY(hipMemcpyAsync(c3.data(), s->d_ctr, 3 * G * 4, hipMemcpyDeviceToHost, s->stream));
+            HIP_TRY(hipStreamSynchronize(s->stream));
+            bool any = false;
+            for (int q = 0; q < G; q++) {
+                if (depth[q] >= 0) continue;
+                const uint32_t width = c3[3 * q + k % 3];
+                if (width == 0) {
+                    depth[q] = k;
+                    continue;
+                }
+                any = true;
+                total[q] += width;
+                s->full_start[q].push_back((uint32_t)total[q]);
+            }
+            if (!any) break;
+            if (k + 2 > kcap) {
+                grow = true;
+                break;
+            }
+            if (k > s->N + 1) {
+                rc = fail(SSW_ERR_CYCLE, "dependency graph deeper than the number of cells");
+                break;
+            }
+            for (int q = 0; q < G; q++) {
+                if (depth[q] >= 0) continue;
+                const int64_t width = (int64_t)(s->full_start[q][k + 1] - s->full_start[q][k]);
+                const unsigned blocks = (unsigned)std::min<int64_t>(16384, (width + kTasksPerBlock - 1) / kTasksPerBlock);
+                k_sweep_build<double, false, false><<<blocks, kBlock, 0, s->stream>>>(g, fl, cp, 0, k, q);
+            }
+            HIP_TRY(hipGetLastError());
+        }
+        s->compiled = was_compiled;
+        if (rc != SSW_OK) return rc;
+        if (grow) continue;
+        int64_t all = 0;
+        for (int q = 0; q < G; q++) all += total[q];
+        if (all != s->N * (int64_t)s->D)
+            return fail(SSW_ERR_CYCLE, "dependency cycle: " + std::to_string(s->N * (int64_t)s->D - all) +
+                                           " cell-direction pairs never become ready");
+        s->kmax_g.assign(G, 0);
+        s->kmax = 1;
+        for (int q = 0; q < G; q++) {
+            s->kmax_g[q] = std::max(1, depth[q]);
+            s->kmax = std::max(s->kmax, s->kmax_g[q]);
+        }
+        break;
     }
-    s->compiled = was_compiled;
-    if (rc != SSW_OK) return rc;
-    if (total != s->N * (int64_t)s->D)
-        return fail(SSW_ERR_CYCLE, "dependency cycle: " + std::to_string(s->N * (int64_t)s->D - total) +
-                                       " cell-direction pairs never become ready");
-    s->kmax = std::max(1, k);
     s->kcap = s->kmax;
-    TRY(dalloc(s, &s->d_lstart, (size_t)L * (s->kcap + 2)));
+    TRY(dalloc(s, &s->d_lstart, (size_t)L * G * (s->kcap + 2)));
     s->replay_ok.assign(L, 0);
-    s->level_start.assign(L, std::vector<uint32_t>());
+    s->replays.assign(L, 0);
+    s->level_start.assign((size_t)L * G, std::vector<uint32_t>());
     for (hipGraphExec_t ge : s->graphs)
         if (ge) (void)hipGraphExecDestroy(ge);
-    s->graphs.assign(L, nullptr);
+    s->graphs.assign((size_t)L * G, nullptr);
     return SSW_OK;
 }
 
-// Launches the replay of slot `lvl`'s level sets on the handle's stream, one
+// Launches the replay of (slot, group)'s level sets on stream `st`, one
 // exact-size launch per non-empty level set; with `ev` each launch is
 // bracketed by events.
 template <typename R>
-void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev,
-                     const std::function<hipEvent_t()>* mark) {
+void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, int grp, hipStream_t st,
+                     std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev,
+                     const std::function<hipEvent_t(hipStream_t)>* mark) {
     Flux<R> fl = make_flux<R>(s);
     Comp<R> cp = make_comp<R>(s);
-    const std::vector<uint32_t>& st = s->level_start[lvl];
+    const std::vector<uint32_t>& ls = s->level_start[(size_t)lvl * s->G + grp];
     const size_t DN = (size_t)s->D * s->N;
-    for (int k = 0; k < s->kmax; k++) {
-        const uint32_t beg = st[k], count = st[k + 1] - st[k];
+    const uint32_t gb = s->gbase[grp];
+    for (int k = 0; k < s->kmax_g[grp]; k++) {
+        const uint32_t beg = gb + ls[k], count = ls[k + 1] - ls[k];
         if (count == 0) continue;
-        hipEvent_t a = ev ? (*mark)() : nullptr;
+        hipEvent_t a = ev ? (*mark)(st) : nullptr;
         if (s->compiled) {
             const size_t so = (size_t)lvl * DN + beg;
-            k_sweep_comp<R><<<(count + kRecBlock - 1) / kRecBlock, kRecBlock, 0, s->stream>>>(
+            k_sweep_comp<R><<<(count + kRecBlock - 1) / kRecBlock, kRecBlock, 0, st>>>(
                 g, fl, cp.hdr + so, cp.A + so, cp.planes + (size_t)lvl * kPlanes * DN + (size_t)kPlanes * beg, cp.ovf,
                 count);
         } else {
             const uint4* recs = s->d_flat + (size_t)lvl * DN + beg;
-            k_sweep_rec<R><<<(count + kTasksPerBlock - 1) / kTasksPerBlock, kBlock, 0, s->stream>>>(g, fl, recs,
-                                                                                                   count);
+            k_sweep_rec<R><<<(count + kTasksPerBlock - 1) / kTasksPerBlock, kBlock, 0, st>>>(g, fl, recs, count);
         }
-        if (ev) ev->push_back({a, (*mark)()});
+        if (ev) ev->push_back({a, (*mark)(st)});
     }
 }
 
-void launch_replay(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev,
-                   const std::function<hipEvent_t()>* mark = nullptr) {
-    if (s->fp32) launch_replay_t<float>(s, g, lvl, ev, mark);
-    else launch_replay_t<double>(s, g, lvl, ev, mark);
+void launch_replay(ssw_sweep* s, const Geo& g, int lvl, int grp, hipStream_t st,
+                   std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev,
+                   const std::function<hipEvent_t(hipStream_t)>* mark = nullptr) {
+    if (s->fp32) launch_replay_t<float>(s, g, lvl, grp, st, ev, mark);
+    else launch_replay_t<double>(s, g, lvl, grp, st, ev, mark);
 }
 
-// After a schedule is built: copies its level starts to the host and captures
-// the replay of its level sets as one graph.
+// After a schedule is built: copies its level starts to the host. Its replay
+// graphs are captured on its second replay (a schedule that is rebuilt after
+// one use never pays for a capture).
 int build_replay_data(ssw_sweep* s, const Geo& g, int lvl) {
-    std::vector<uint32_t>& st = s->level_start[lvl];
-    st.resize(s->kmax + 1);
-    HIP_TRY(hipMemcpyAsync(st.data(), s->d_lstart + (size_t)lvl * (s->kcap + 2), (s->kmax + 1) * 4,
+    (void)g;
+    const int G = s->G;
+    std::vector<uint32_t> all((size_t)G * (s->kcap + 2));
+    HIP_TRY(hipMemcpyAsync(all.data(), s->d_lstart + (size_t)lvl * G * (s->kcap + 2), all.size() * 4,
                            hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    if (s->graphs[lvl]) {
-        (void)hipGraphExecDestroy(s->graphs[lvl]);
-        s->graphs[lvl] = nullptr;
+    for (int q = 0; q < G; q++) {
+        std::vector<uint32_t>& ls = s->level_start[(size_t)lvl * G + q];
+        ls.assign(all.begin() + (size_t)q * (s->kcap + 2), all.begin() + (size_t)q * (s->kcap + 2) + s->kmax_g[q] + 1);
+        hipGraphExec_t& slot = s->graphs[(size_t)lvl * G + q];
+        if (slot) {
+            (void)hipGraphExecDestroy(slot);
+            slot = nullptr;
+        }
     }
     s->replay_ok[lvl] = 1;
-    // One graph per schedule; if the capture fails the level sets are
-    // launched one by one instead (counted in ssw_info).
+    s->replays[lvl] = 0;
+    return SSW_OK;
+}
+
+// Captures the replay of (slot, group)'s level sets as one graph on the
+// group's stream (nothing is executed). A failed capture leaves no graph and
+// the level sets are launched one by one (counted in ssw_info).
+void capture_group(ssw_sweep* s, const Geo& g, int lvl, int q, int64_t* failures) {
+    const hipStream_t st = gstream(s, q);
     hipGraph_t graph = nullptr;
-    if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) {
         (void)hipGetLastError();
-        s->capture_failures++;
-        return SSW_OK;
+        (*failures)++;
+        return;
     }
-    launch_replay(s, g, lvl, nullptr);
-    hipError_t ce = hipStreamEndCapture(s->stream, &graph);
+    launch_replay(s, g, lvl, q, st, nullptr);
+    hipError_t ce = hipStreamEndCapture(st, &graph);
     if (ce != hipSuccess || !graph) {
         if (graph) (void)hipGraphDestroy(graph);
         (void)hipGetLastError();
-        s->capture_failures++;
-        return SSW_OK;
+        (*failures)++;
+        return;
     }
     hipGraphExec_t ge = nullptr;
     hipError_t ie = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (ie != hipSuccess) {
         (void)hipGetLastError();
-        s->capture_failures++;
-        return SSW_OK;
+        (*failures)++;
+        return;
     }
-    s->graphs[lvl] = ge;
-    return SSW_OK;
+    s->graphs[(size_t)lvl * s->G + q] = ge;
+}
+
+void drop_replay(ssw_sweep* s, int lvl) {
+    s->replay_ok[lvl] = 0;
+    s->replays[lvl] = 0;
+    for (int q = 0; q < s->G; q++) {
+        hipGraphExec_t& slot = s->graphs[(size_t)lvl * s->G + q];
+        if (slot) {
+            (void)hipGraphExecDestroy(slot);
+            slot = nullptr;
+        }
+    }
 }
 
 template <typename R>
@@ -1111,14 +1245,14 @@ int run_step(ssw_sweep* s, double* elapsed) {
     s->t_transport_total = s->t_chem_total = s->t_build_total = 0;
     s->n_transport = 0;
     size_t evn = 0;
-    std::function<hipEvent_t()> mark = [&](void) -> hipEvent_t {
+    std::function<hipEvent_t(hipStream_t)> mark = [&](hipStream_t st) -> hipEvent_t {
         if (evn >= s->ev.size()) {
             hipEvent_t e;
             (void)hipEventCreate(&e);
             s->ev.push_back(e);
         }
         hipEvent_t e = s->ev[evn++];
-        (void)hipEventRecord(e, s->stream);
+        (void)hipEventRecord(e, st);
         return e;
     };
     std::vector<std::pair<hipEvent_t, hipEvent_t>> tr, ch, bu;
@@ -1128,33 +1262,62 @@ int run_step(ssw_sweep* s, double* elapsed) {
     HIP_TRY(hipMemcpyAsync(need.data(), s->d_need, L * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     for (int l = 0; l < L; l++)
-        if (need[l] && s->replay_ok[l]) {
-            s->replay_ok[l] = 0;
-            if (s->graphs[l]) {
-                (void)hipGraphExecDestroy(s->graphs[l]);
-                s->graphs[l] = nullptr;
-            }
-        }
+        if (need[l] && s->replay_ok[l]) drop_replay(s, l);
+    const int G = s->G;
     for (int lvl : order) {
         k_begin<R><<<gper, kBlock, 0, s->stream>>>(g, fl, lvl);
         if (!need[lvl] && !s->replay_ok[lvl]) TRY(build_replay_data(s, g, lvl));  // built by a parity hook
-        if (!need[lvl]) {
-            if (s->timing) launch_replay(s, g, lvl, &tr, &mark);
-            else if (s->graphs[lvl]) HIP_TRY(hipGraphLaunch(s->graphs[lvl], s->stream));
-            else launch_replay(s, g, lvl, nullptr);
-        } else {  // build while solving
-            hipEvent_t b0 = s->timing ? mark() : nullptr;
-            k_count<<<gcount, kBlock, 0, s->stream>>>(g, lvl, 0);
-            for (int k = 0; k < s->kmax; k++) launch_build<R>(s, g, lvl, k, true, build_blocks(s, lvl, k));
-            if (s->timing) bu.push_back({b0, mark()});
-            HIP_TRY(hipGetLastError());
+        const bool rebuild = need[lvl] != 0;
+        hipEvent_t b0 = (s->timing && rebuild) ? mark(s->stream) : nullptr;
+        if (rebuild) k_count<<<gcount, kBlock, 0, s->stream>>>(g, lvl, 0);
+        if (G > 1) HIP_TRY(hipEventRecord(s->fork, s->stream));
+        const bool capture = !rebuild && !s->timing && s->replays[lvl] >= 1;
+        std::vector<int64_t> cap_fail(G, 0);
+        std::vector<hipError_t> herr(G, hipSuccess);
+        // group q's launches on its stream: replay or build-while-solving
+        auto submit = [&](int q) {
+            const hipStream_t st = gstream(s, q);
+            hipError_t e = hipSuccess;
+            if (G > 1) e = hipStreamWaitEvent(st, s->fork, 0);
+            if (!rebuild) {
+                hipGraphExec_t& ge = s->graphs[(size_t)lvl * G + q];
+                if (s->timing) {
+                    launch_replay(s, g, lvl, q, st, &tr, &mark);
+                } else {
+                    if (!ge && capture) capture_group(s, g, lvl, q, &cap_fail[q]);
+                    if (ge) e = e != hipSuccess ? e : hipGraphLaunch(ge, st);
+                    else launch_replay(s, g, lvl, q, st, nullptr);
+                }
+            } else {
+                for (int k = 0; k < s->kmax_g[q]; k++)
+                    launch_build<R>(s, g, lvl, q, k, true, build_blocks(s, lvl, q, k), st);
+            }
+            if (e == hipSuccess) e = hipGetLastError();  // launch errors are per host thread
+            if (G > 1 && e == hipSuccess) e = hipEventRecord(s->gjoin[q], st);
+            herr[q] = e;
+        };
+        if (G > 1 && !s->timing && !s->pool.threads.empty()) {
+            s->pool.run(submit);
+        } else {
+            for (int q = 0; q < G; q++) submit(q);
+        }
+        for (int q = 0; q < G; q++) {
+            s->capture_failures += cap_fail[q];
+            HIP_TRY(herr[q]);
+            if (G > 1) HIP_TRY(hipStreamWaitEvent(s->stream, s->gjoin[q], 0));
+        }
+        HIP_TRY(hipGetLastError());
+        if (rebuild) {
+            if (s->timing) bu.push_back({b0, mark(s->stream)});
             TRY(build_replay_data(s, g, lvl));
             need[lvl] = 0;
             s->builds_last++;
+        } else {
+            s->replays[lvl]++;
         }
-        hipEvent_t a = s->timing ? mark() : nullptr;
+        hipEvent_t a = s->timing ? mark(s->stream) : nullptr;
         k_chemistry<R><<<gcells, kBlock, 0, s->stream>>>(g, fl, ca, lvl);
-        if (s->timing) ch.push_back({a, mark()});
+        if (s->timing) ch.push_back({a, mark(s->stream)});
         s->sweeps_last++;
     }
     HIP_TRY(hipGetLastError());
@@ -1222,6 +1385,8 @@ void free_grid(ssw_sweep* s) {
     dfree(s, s->d_planes);
     dfree(s, s->d_ovf_start);
     dfree(s, s->d_ovf);
+    dfree(s, s->d_dgroup);
+    dfree(s, s->d_gbase);
 }
 
 // Allocates the compiled schedule and the overflow entries (once per grid).
@@ -1345,6 +1510,7 @@ int ssw_create(const ssw_sweep_parameters* params, ssw_sweep** out) {
 void ssw_destroy(ssw_sweep* s) {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    s->pool.stop();
     free_state(s);
     free_grid(s);
     dfree(s, s->d_dirs);
@@ -1352,6 +1518,9 @@ void ssw_destroy(ssw_sweep* s) {
     dfree(s, s->d_fail);
     dfree(s, s->d_active);
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : s->gjoin) (void)hipEventDestroy(e);
+    if (s->fork) (void)hipEventDestroy(s->fork);
+    for (hipStream_t st : s->gstreams) (void)hipStreamDestroy(st);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
@@ -1431,7 +1600,34 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     TRY(dalloc(s, &s->d_pfoff, pfoff.size()));
     TRY(dalloc(s, &s->d_pfidx, std::max<size_t>(1, pfidx.size())));
     TRY(dalloc(s, &s->d_level, n));
-    TRY(dalloc(s, &s->d_ctr, (size_t)L * 3));
+    // direction groups: contiguous blocks of directions; each group's level
+    // sets run on a stream of their own
+    s->G = s->G_request > 0 ? std::min(s->G_request, s->D) : std::min(kDefaultGroups, s->D);
+    std::vector<uint8_t> dgroup(s->D);
+    s->gbase.assign(s->G, 0u);
+    for (int q = 0; q < s->G; q++) {
+        const int d0 = q * s->D / s->G, d1 = (q + 1) * s->D / s->G;
+        s->gbase[q] = (uint32_t)((int64_t)d0 * n);
+        for (int d = d0; d < d1; d++) dgroup[d] = (uint8_t)q;
+    }
+    while ((int)s->gstreams.size() < s->G) {
+        hipStream_t st = nullptr;
+        HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        s->gstreams.push_back(st);
+        hipEvent_t e = nullptr;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        s->gjoin.push_back(e);
+    }
+    if (!s->fork) HIP_TRY(hipEventCreateWithFlags(&s->fork, hipEventDisableTiming));
+    if ((int)s->pool.threads.size() != s->G - 1) {
+        s->pool.stop();
+        if (s->G > 1) s->pool.start(s->G - 1);
+    }
+    TRY(dalloc(s, &s->d_dgroup, (size_t)s->D));
+    TRY(dalloc(s, &s->d_gbase, (size_t)s->G));
+    HIP_TRY(hipMemcpyAsync(s->d_dgroup, dgroup.data(), s->D, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_gbase, s->gbase.data(), s->G * 4, hipMemcpyHostToDevice, s->stream));
+    TRY(dalloc(s, &s->d_ctr, (size_t)L * s->G * 3));
     TRY(dalloc(s, &s->d_cnt, DN));
     HIP_TRY(hipMemcpyAsync(s->d_off, off32.data(), (n + 1) * 4, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_geo, geo.data(), F * sizeof(double4), hipMemcpyHostToDevice, s->stream));
@@ -1446,7 +1642,7 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     HIP_TRY(hipMemcpyAsync(s->d_pfoff, pfoff.data(), pfoff.size() * 4, hipMemcpyHostToDevice, s->stream));
     std::vector<uint8_t> lev(n, (uint8_t)(L - 1));
     HIP_TRY(hipMemcpyAsync(s->d_level, lev.data(), n, hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipMemsetAsync(s->d_ctr, 0, (size_t)L * 3 * 4, s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_ctr, 0, (size_t)L * s->G * 3 * 4, s->stream));
     // mirror check, downwind statistics and overflow counts
     unsigned long long* d_stats = nullptr;
     TRY(dalloc(s, &d_stats, 5));
@@ -1613,24 +1809,24 @@ int ssw_download_levelsets(ssw_sweep* s, int32_t lvl, uint32_t* depth) {
             k_begin<double><<<1, kBlock, 0, s->stream>>>(g, fl, lvl);
         }
         k_count<<<grid_for((int64_t)s->N * s->D), kBlock, 0, s->stream>>>(g, lvl, 0);
-        for (int k = 0; k < s->kmax; k++) launch_build_any(s, g, lvl, k, false, build_blocks(s, lvl, k));
+        for (int q = 0; q < s->G; q++)
+            for (int k = 0; k < s->kmax_g[q]; k++)
+                launch_build_any(s, g, lvl, q, k, false, build_blocks(s, lvl, q, k), s->stream);
         k_fill_u32<<<1, 1, 0, s->stream>>>(s->d_need + lvl, 0u, 1);
         HIP_TRY(hipGetLastError());
-        if ((int)s->replay_ok.size() > lvl) {
-            s->replay_ok[lvl] = 0;
-            if (s->graphs[lvl]) {
-                (void)hipGraphExecDestroy(s->graphs[lvl]);
-                s->graphs[lvl] = nullptr;
-            }
-        }
+        if ((int)s->replay_ok.size() > lvl) drop_replay(s, lvl);
     }
     uint32_t* d_depth = nullptr;
     const int64_t DN = (int64_t)s->D * s->N;
     TRY(dalloc(s, &d_depth, (size_t)DN));
     k_fill_u32<<<grid_for(DN), kBlock, 0, s->stream>>>(d_depth, kNone, DN);
-    const uint4* tasks = s->compiled ? s->d_hdr + (size_t)lvl * DN : s->d_flat + (size_t)lvl * DN;
-    k_scatter_depth<<<grid_for(std::min<int64_t>(DN, 1 << 22)), kBlock, 0, s->stream>>>(
-        g, tasks, s->d_lstart + (size_t)lvl * (s->kcap + 2), s->kmax, s->compiled ? 1 : 0, d_depth);
+    for (int q = 0; q < s->G; q++) {
+        const size_t base = (size_t)lvl * DN + s->gbase[q];
+        const uint4* tasks = s->compiled ? s->d_hdr + base : s->d_flat + base;
+        k_scatter_depth<<<grid_for(std::min<int64_t>(DN, 1 << 22)), kBlock, 0, s->stream>>>(
+            g, tasks, s->d_lstart + ((size_t)lvl * s->G + q) * (s->kcap + 2), s->kmax_g[q], s->compiled ? 1 : 0,
+            d_depth);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(depth, d_depth, (size_t)DN * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -1690,6 +1886,7 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info) {
     info->schedule_builds_last_step = s->builds_last;
     info->overflow_entries = s->ovf_total;
     info->graph_capture_failures = s->capture_failures;
+    info->direction_groups = s->have_grid ? s->G : s->G_request;
     if (s->have_state) {
         unsigned long long f = 0, a = 0;
         HIP_TRY(hipMemcpyAsync(&f, s->d_fail, 8, hipMemcpyDeviceToHost, s->stream));
@@ -1743,6 +1940,14 @@ int64_t ssw_estimate_device_bytes(const ssw_sweep_parameters* p, int64_t n, int6
     b += n * 8 * 9 + 2 * DN * rs + D * num_periodic_cells * rs;
     b += n * 8;                                    // photon-rate download
     return b + b / 20 + (1 << 20);
+}
+
+int ssw_set_direction_groups(ssw_sweep* s, int32_t groups) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    if (groups < 0 || groups > 16) return fail(SSW_ERR_INVALID, "direction groups must be in [0, 16]");
+    if (s->have_grid) return fail(SSW_ERR_INVALID, "direction groups must be set before ssw_upload_grid");
+    s->G_request = groups;
+    return SSW_OK;
 }
 
 int ssw_set_timing(ssw_sweep* s, int32_t enabled) {
