@@ -172,7 +172,7 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info);
 int ssw_set_transport_mode(ssw_sweep* s, int32_t mode);
 
 /* Directions are split into `groups` contiguous blocks (1..16; 0: the
- * library default of 4, at most the direction count); each block's level
+ * library default of 1, at most the direction count); each block's level
  * sets run on a HIP stream of its own, so the small, latency-bound launches
  * of different blocks overlap on the device. Dependencies never cross
  * directions, so results do not depend on the grouping (bitwise). Set before

@@ -36,7 +36,7 @@ class TuneResult:
         return {"direction_groups": self.direction_groups, "transport_mode": self.transport_mode}
 
 
-def autotune(problem: Problem, groups=(1, 2, 4, 8), modes=(N.TRANSPORT_MODE_COMPILED, N.TRANSPORT_MODE_GEOMETRIC),
+def autotune(problem: Problem, groups=(1, 2, 4), modes=(N.TRANSPORT_MODE_COMPILED, N.TRANSPORT_MODE_GEOMETRIC),
              steps: int = 2, warmup: int = 1) -> TuneResult:
     """Times `steps` steps after `warmup` steps of every (groups, mode)
     candidate; candidates that do not fit in device memory are skipped."""

@@ -46,6 +46,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <condition_variable>
 #include <mutex>
@@ -79,9 +80,13 @@ constexpr int kMaxFaces = 127;             // faces per cell
 constexpr int kQueueCap = kTasksPerBlock * kMaxFaces;
 constexpr int kPlanes = 12;                // upwind entries per task held in planes
 constexpr int kRecBlock = 64;              // compiled replay: one lane per task
-constexpr int kDefaultGroups = 4;          // direction groups run side by side
+constexpr int kDefaultGroups = 1;          // direction groups run side by side
 
 thread_local std::string g_err;
+// Handles alive in the process. Replay graphs are captured only while a single
+// handle exists: a stream capture is invalidated by the device-synchronising
+// calls (allocation, free, blocking copies) that other handles' threads make.
+std::atomic<int> g_live_handles{0};
 
 int fail(int code, const std::string& msg) {
     g_err = msg;
@@ -197,13 +202,34 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     return v;
 }
 
+// Per-cell inputs of a task's outgoing rate, loaded as soon as the task's cell
+// is known so that they arrive with the upwind gathers.
+template <typename R>
+struct CellIn {
+    double src;
+    double att;
+    R per;
+};
+
+template <typename R>
+__device__ __forceinline__ CellIn<R> cell_inputs(const Geo& g, const Flux<R>& fl, uint32_t c, uint32_t d,
+                                                 uint32_t ps) {
+    CellIn<R> ci;
+    ci.src = fl.src_over_d[c];
+    ci.att = fl.att[c];
+    // the periodic load uses slot 0 for cells without one (always a valid index)
+    const R p = fl.P ? fl.P[(size_t)d * g.Nper + (ps != kNone ? ps : 0u)] : (R)0;
+    ci.per = (ps != kNone) ? p : (R)0;
+    return ci;
+}
+
 // Outgoing rate of a solved task (Site::get_rate + HydrogenOnly::get_outgoing_rate,
 // src/sweep/site.rs:49-56, src/chemistry/hydrogen_only/mod.rs:72-88), stored as X = O / A.
 template <typename R>
-__device__ __forceinline__ void finish_task(const Flux<R>& fl, size_t o, uint32_t c, R inc, R per, R A) {
+__device__ __forceinline__ void finish_task(const Flux<R>& fl, size_t o, R inc, const CellIn<R>& ci, R A) {
     if (inc < (R)0) inc = 0;  // Photons::make_positive
-    R in = (inc + (R)fl.src_over_d[c]) + per;
-    R out = (in < (R)fl.thr) ? (R)0 : in * (R)fl.att[c];
+    R in = (inc + (R)ci.src) + ci.per;
+    R out = (in < (R)fl.thr) ? (R)0 : in * (R)ci.att;
     fl.X[o] = (A > (R)0) ? out / A : (R)0;
     fl.I[o] = inc;
 }
@@ -244,15 +270,17 @@ __device__ __forceinline__ void task_faces(const Geo& g, const Flux<R>& fl, bool
         const double dd = fdot(a, dx, dy, dz);
         const bool up = fa && local && dd < 0.0;
         const bool down = fa && dd > 0.0;
-        R w = 0, v = 0;
-        if (up) w = (R)(a.w * fl.inv_aref * -dd);
-        if (kTransport && up) v = w * fl.X[dN + nb];
+        // nb is a valid cell index on every lane (0 for lanes without a face
+        // and for boundary faces), so the gathers and the neighbour loads are
+        // unconditional and issue together; the products are selected after.
+        const R w = up ? (R)(a.w * fl.inv_aref * -dd) : (R)0;
+        const R xv = kTransport ? fl.X[dN + nb] : (R)0;
+        const R v = up ? w * xv : (R)0;
         bool ready = false;
         uint4 rec = make_uint4(0, 0, 0, 0);
-        if (kBuild && down && local) {
-            // neighbour record loads are issued beside the decrement
+        if (kBuild) {
             const uint32_t f0n = g.off[nb], f1n = g.off[nb + 1], psn = g.pslot[nb];
-            ready = atomicSub(&g.cnt[dN + nb], 1u) == 1u;
+            if (down && local) ready = atomicSub(&g.cnt[dN + nb], 1u) == 1u;
             rec = task_record(nb, f0n, f1n, d, psn);
         }
         if (down) A += a.w * fl.inv_aref * dd;
@@ -484,18 +512,17 @@ __global__ void __launch_bounds__(kBlock) k_sweep_build(Geo g, Flux<R> fl, Comp<
         const bool valid = i < end;
         const uint4 r = valid ? flat[i] : make_uint4(0, 0, 0, 0);
         const uint32_t nfmax = wave_max(valid ? (r.x >> kCellBits) : 0u);
+        const uint32_t c = r.x & kCellMask, d = r.z, ps = r.w;
+        CellIn<R> ci{0.0, 0.0, (R)0};
+        if (kTransport) ci = cell_inputs<R>(g, fl, c, d, ps);
         double A;
         R inc;
         uint32_t nup;
         task_faces<R, kTransport, true, kCompile>(g, fl, valid, r, sub, nfmax, A, inc, nup, sq, &sq_n, planes, W,
                                                   i - beg);
         if (valid && sub == 0) {
-            const uint32_t c = r.x & kCellMask, d = r.z, ps = r.w;
             const size_t o = (size_t)d * g.N + c;
-            if (kTransport) {
-                R per = (ps != kNone && fl.P) ? fl.P[(size_t)d * g.Nper + ps] : (R)0;
-                finish_task<R>(fl, o, c, inc, per, (R)A);
-            }
+            if (kTransport) finish_task<R>(fl, o, inc, ci, (R)A);
             if (kCompile) {
                 const size_t so = (size_t)lvl * DN + gb + i;
                 cp.hdr[so] = make_uint4(c | (d << kCellBits), ps, nup > (uint32_t)kPlanes ? cp.ovf_start[o] : 0u, nup);
@@ -526,13 +553,13 @@ __global__ void __launch_bounds__(kBlock) k_sweep_rec(Geo g, Flux<R> fl, const u
     const uint4 r = valid ? recs[i] : make_uint4(0, 0, 0, 0);
     const uint32_t ps = r.w;
     const uint32_t c = r.x & kCellMask, d = r.z;
-    const R per = (valid && ps != kNone && fl.P) ? fl.P[(size_t)d * g.Nper + ps] : (R)0;
+    const CellIn<R> ci = cell_inputs<R>(g, fl, c, d, ps);
     const uint32_t nfmax = wave_max(valid ? (r.x >> kCellBits) : 0u);
     double A;
     R inc;
     uint32_t nup;
     task_faces<R, true, false, false>(g, fl, valid, r, sub, nfmax, A, inc, nup, nullptr, nullptr, nullptr, 0, 0);
-    if (valid && sub == 0) finish_task<R>(fl, (size_t)d * g.N + c, c, inc, per, (R)A);
+    if (valid && sub == 0) finish_task<R>(fl, (size_t)d * g.N + c, inc, ci, (R)A);
 }
 
 // Compiled replay of one level set of width W: one lane per task. The header,
@@ -543,26 +570,28 @@ __global__ void __launch_bounds__(kRecBlock) k_sweep_comp(Geo g, Flux<R> fl, con
                                                           const Entry<R>* planes, const Entry<R>* ovf, uint32_t W) {
     const uint32_t i = blockIdx.x * kRecBlock + threadIdx.x;
     if (i >= W) return;
-    const uint4 h = hdr[i];
-    const R A = Av[i];
+    // first memory step: the plane entries, the header and A (independent)
     Entry<R> e[kPlanes];
 #pragma unroll
     for (int q = 0; q < kPlanes; q++) e[q] = planes[(size_t)q * W + i];
+    const uint4 h = hdr[i];
+    const R A = Av[i];
     const uint32_t c = h.x & kCellMask, d = h.x >> kCellBits, ps = h.y, nup = h.w;
     const size_t dN = (size_t)d * g.N;
+    // second memory step: the upwind X (entries past the upwind count gather
+    // the cell's own X, a valid index, and are not summed), the cell inputs
+    const CellIn<R> ci = cell_inputs<R>(g, fl, c, d, ps);
     R x[kPlanes];
 #pragma unroll
-    for (int q = 0; q < kPlanes; q++) x[q] = ((uint32_t)q < nup) ? fl.X[dN + e[q].nb] : (R)0;
-    const R per = (ps != kNone && fl.P) ? fl.P[(size_t)d * g.Nper + ps] : (R)0;
+    for (int q = 0; q < kPlanes; q++) x[q] = fl.X[dN + ((uint32_t)q < nup ? e[q].nb : c)];
     R inc = 0;
 #pragma unroll
-    for (int q = 0; q < kPlanes; q++)
-        if ((uint32_t)q < nup) inc += e[q].w * x[q];
+    for (int q = 0; q < kPlanes; q++) inc += ((uint32_t)q < nup) ? e[q].w * x[q] : (R)0;
     for (uint32_t q = kPlanes; q < nup; q++) {
         const Entry<R> o = ovf[h.z + q - kPlanes];
         inc += o.w * fl.X[dN + o.nb];
     }
-    finish_task<R>(fl, dN + c, c, inc, per, A);
+    finish_task<R>(fl, dN + c, inc, ci, A);
 }
 
 struct ChemArgs {
@@ -1271,7 +1300,7 @@ int run_step(ssw_sweep* s, double* elapsed) {
         hipEvent_t b0 = (s->timing && rebuild) ? mark(s->stream) : nullptr;
         if (rebuild) k_count<<<gcount, kBlock, 0, s->stream>>>(g, lvl, 0);
         if (G > 1) HIP_TRY(hipEventRecord(s->fork, s->stream));
-        const bool capture = !rebuild && !s->timing && s->replays[lvl] >= 1;
+        const bool capture = !rebuild && !s->timing && s->replays[lvl] >= 1 && g_live_handles.load() == 1;
         std::vector<int64_t> cap_fail(G, 0);
         std::vector<hipError_t> herr(G, hipSuccess);
         // group q's launches on its stream: replay or build-while-solving
@@ -1479,6 +1508,7 @@ int ssw_create(const ssw_sweep_parameters* params, ssw_sweep** out) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(SSW_ERR_NO_DEVICE, "no HIP device");
     ssw_sweep* s = new ssw_sweep();
+    g_live_handles++;
     s->p = p;
     s->p.explicit_directions = nullptr;
     s->D = p.num_directions;
@@ -1487,6 +1517,7 @@ int ssw_create(const ssw_sweep_parameters* params, ssw_sweep** out) {
     (void)hipGetDevice(&s->device);
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
         delete s;
+        g_live_handles--;
         return fail(SSW_ERR_HIP, "hipStreamCreate failed");
     }
     int r = dalloc(s, &s->d_dirs, dirs.size());
@@ -1523,6 +1554,7 @@ void ssw_destroy(ssw_sweep* s) {
     for (hipStream_t st : s->gstreams) (void)hipStreamDestroy(st);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
+    g_live_handles--;
 }
 
 int ssw_set_transport_mode(ssw_sweep* s, int32_t mode) {

@@ -165,3 +165,33 @@ def test_cycle_is_reported(hip_available):
         Sweep(SweepParameters(directions=[[1.0, 0.0, 0.0]]), grid, np.ones(3), np.full(3, 1e-10),
               np.full(3, 1e4), np.zeros(3))
     assert ei.value.code == -4
+
+
+def test_single_source_32_full_size(hip_available):
+    """The 32^3 = 32768-cell, 16-direction, single-source workload at its full
+    size and step count (10 steps), FP64 transport against the oracle."""
+    pb = W.single_source_32(periodic=True)
+    g, o = _pair(pb)
+    _compare_structure(g, o, 1)
+    for _ in range(pb.steps):
+        assert g.run_sweeps() == o.run_sweeps()
+    _compare_fields(g, o, 1e-10)
+
+
+def test_precision_64_fp32_against_oracle(hip_available):
+    """The 262144-cell, 84-direction, 64-source workload (2 steps): FP64
+    transport within 1e-10 of the oracle, FP32 transport's ionised fraction
+    within 1e-5."""
+    pb = W.precision_64()
+    g64 = Sweep(pb.params, pb.grid, *pb.state())
+    g32 = Sweep(pb.with_params(transport_precision=32).params, pb.grid, *pb.state())
+    o = OracleSweep(pb.params, pb.grid, *pb.state())
+    for _ in range(2):
+        e = o.run_sweeps()
+        assert g64.run_sweeps() == e
+        assert g32.run_sweeps() == e
+    _compare_fields(g64, o, 1e-10)
+    r32, ro = g32.results(), o.results()
+    _rel_ok(r32.ionized_hydrogen_fraction, ro.ionized_hydrogen_fraction, 1e-5, name="x_HII fp32")
+    _rel_ok(r32.temperature, ro.temperature, 1e-5, name="T fp32")
+    assert np.array_equal(r32.level, ro.level)
