@@ -118,6 +118,32 @@ def field_errors(a, b):
     return out
 
 
+def run_largest(steps: int = 10):
+    """16777216-cell clustered grid (sigma_ln 2.5, 256^3 field mesh), 8192
+    sources, 84 directions, 4 timestep levels, periodic, threshold 1e-5 s^-1,
+    FP32 transport with FP64 chemistry, `steps` steps, timed whole."""
+    from subsweep_amd import workloads as W
+    t = time.perf_counter()
+    pb = W.clustered_problem(16777216, levels=4, sources=8192, sigma_ln=2.5, mesh=256, threshold=1e-5,
+                             steps=steps, precision=32, name="clustered_16m")
+    log(f"largest grid: {pb.grid.num_cells} cells, {pb.grid.num_faces} faces, built in "
+        f"{time.perf_counter() - t:.1f} s")
+    r = run_gpu(pb, 32, steps, 0, profile_steps=1)
+    log(f"largest fp32: {r['updates_per_s'] / 1e9:.3f} G updates/s, {r['ms_per_step']:.1f} ms/step, "
+        f"mode {r['transport_mode']}, {r['info'].device_bytes / 2**30:.1f} GiB on the device")
+    return {
+        "workload": "16777216-cell clustered lognormal-field Voronoi grid (sigma_ln 2.5), 8192 sources, "
+                    "84 directions, 4 timestep levels, periodic, threshold 1e-5 s^-1, FP32 transport",
+        "steps": steps,
+        "updates_per_s": r["updates_per_s"],
+        "ms_per_step": r["ms_per_step"],
+        "updates_per_step": r["updates_per_step"],
+        "transport_mode": r["transport_mode"],
+        "device_gib": r["info"].device_bytes / 2**30,
+        "dag_depth": int(r["info"].max_depth),
+    }
+
+
 def tuned(pb, precision: int, warmup: int, enabled: bool, mode: int, groups: int):
     """(mode, groups) for a run: the autotuner's choice, or the command line's."""
     if not enabled:
@@ -306,6 +332,8 @@ def main():
     ap.add_argument("--serving-slots", type=int, default=4)
     ap.add_argument("--no-clustered", action="store_true",
                     help="skip the 2097152-cell clustered FP32-vs-FP64 measurement")
+    ap.add_argument("--largest", action="store_true",
+                    help="only run the 16777216-cell largest-domain measurement and print its JSON")
     ap.add_argument("--mode", type=int, default=0, choices=(0, 3, 4),
                     help="transport replay form: 0 automatic, 3 geometric, 4 compiled")
     ap.add_argument("--groups", type=int, default=0, help="direction groups (0: library default)")
@@ -326,6 +354,10 @@ def main():
         print("bench.py measures one GPU: this build keeps the whole domain on a single MI355X; "
               "--gpus above 1 is not supported", file=sys.stderr)
         sys.exit(2)
+
+    if args.largest:
+        print(json.dumps(run_largest()), flush=True)
+        return
 
     from subsweep_amd import workloads as W
     t = time.perf_counter()
