@@ -49,10 +49,11 @@ def bytes_per_update(info, precision: int) -> float:
     return base + info.downwind_local * bl + info.downwind_periodic * bp + info.downwind_boundary * bb
 
 
-def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3, mode: int = 0, groups: int = 0):
+def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3, mode: int = 0, block: int = 0,
+            graphs: bool = True):
     from subsweep_amd import Sweep
     pp = pb.with_params(transport_precision=precision)
-    s = Sweep(pp.params, pp.grid, *pp.state(), transport_mode=mode, direction_groups=groups)
+    s = Sweep(pp.params, pp.grid, *pp.state(), transport_mode=mode, replay_block=block, graph_replay=graphs)
     for _ in range(warmup):
         s.run_sweeps()
     s.synchronize()
@@ -93,7 +94,8 @@ def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3,
         "bytes_per_update": bpu,
         "achieved_gbs": achieved,
         "kernel": KERNELS.get(info.transport_mode, "?"),
-        "direction_groups": int(info.direction_groups),
+        "replay_block": int(info.replay_block),
+        "graph_replay": bool(info.graph_replay),
         "transport_mode": int(info.transport_mode),
         "info": info,
         "results": res,
@@ -144,17 +146,17 @@ def run_largest(steps: int = 10):
     }
 
 
-def tuned(pb, precision: int, warmup: int, enabled: bool, mode: int, groups: int):
-    """(mode, groups) for a run: the autotuner's choice, or the command line's."""
+def tuned(pb, precision: int, warmup: int, enabled: bool, mode: int, block: int):
+    """(mode, block, graphs) for a run: the autotuner's choice, or the command line's."""
     if not enabled:
-        return mode, groups, None
+        return mode, block, True, None
     from subsweep_amd.autotune import autotune
     t = time.perf_counter()
     r = autotune(pb.with_params(transport_precision=precision), warmup=warmup)
-    log(f"autotune fp{precision}: groups {r.direction_groups}, mode {r.transport_mode}, {r.ms_per_step:.2f} ms/step "
-        f"({time.perf_counter() - t:.1f} s; " +
-        ", ".join(f"g{g}/m{m} {ms:.2f}" for g, m, ms in r.table) + ")")
-    return r.transport_mode, r.direction_groups, [list(x) for x in r.table]
+    log(f"autotune fp{precision}: mode {r.transport_mode}, block {r.replay_block}, graphs {r.graph_replay}, "
+        f"{r.ms_per_step:.2f} ms/step ({time.perf_counter() - t:.1f} s; " +
+        ", ".join(f"m{m}/b{b}/g{int(gr)} {ms:.2f}" for m, b, gr, ms in r.table) + ")")
+    return r.transport_mode, r.replay_block, r.graph_replay, [list(x) for x in r.table]
 
 
 def run_clustered(steps: int = 4, warmup: int = 2, autotune: bool = True):
@@ -166,11 +168,11 @@ def run_clustered(steps: int = 4, warmup: int = 2, autotune: bool = True):
     pb = W.clustered_2m()
     log(f"clustered grid: {pb.grid.num_cells} cells, {pb.grid.num_faces} faces, built in "
         f"{time.perf_counter() - t:.1f} s")
-    m32, g32, t32 = tuned(pb, 32, warmup, autotune, 0, 0)
-    r32 = run_gpu(pb, 32, steps, warmup, profile_steps=1, mode=m32, groups=g32)
+    m32, b32, gr32, t32 = tuned(pb, 32, warmup, autotune, 0, 0)
+    r32 = run_gpu(pb, 32, steps, warmup, profile_steps=1, mode=m32, block=b32, graphs=gr32)
     log(f"clustered fp32: {r32['updates_per_s'] / 1e9:.3f} G updates/s, {r32['ms_per_step']:.1f} ms/step")
-    m64, g64, t64 = tuned(pb, 64, warmup, autotune, 0, 0)
-    r64 = run_gpu(pb, 64, steps, warmup, profile_steps=1, mode=m64, groups=g64)
+    m64, b64, gr64, t64 = tuned(pb, 64, warmup, autotune, 0, 0)
+    r64 = run_gpu(pb, 64, steps, warmup, profile_steps=1, mode=m64, block=b64, graphs=gr64)
     log(f"clustered fp64: {r64['updates_per_s'] / 1e9:.3f} G updates/s, {r64['ms_per_step']:.1f} ms/step")
     return {
         "workload": "2097152-cell clustered lognormal-field Voronoi grid (sigma_ln 2), 84 directions, "
@@ -185,8 +187,8 @@ def run_clustered(steps: int = 4, warmup: int = 2, autotune: bool = True):
         "fp64_ms_per_step": r64["ms_per_step"],
         "fp32_transport_mode": r32["transport_mode"],
         "fp64_transport_mode": r64["transport_mode"],
-        "fp32_direction_groups": r32["direction_groups"],
-        "fp64_direction_groups": r64["direction_groups"],
+        "fp32_replay_block": r32["replay_block"],
+        "fp64_replay_block": r64["replay_block"],
         "autotune_fp32_ms_per_step": t32,
         "autotune_fp64_ms_per_step": t64,
         "fp32_error_vs_fp64": field_errors(r32["results"], r64["results"]),
@@ -252,7 +254,7 @@ def serving_requests(pool, count: int, seed: int, steps: int):
     return out
 
 
-def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, steps: int = 10, groups: int = 1):
+def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, steps: int = 10):
     import threading
     import numpy as np
     from subsweep_amd.serve import Server
@@ -265,7 +267,7 @@ def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, step
     # capacity: a saturated batch of 8 requests on the same server shape
     with Server(slots=slots, hbm_budget_bytes=budget) as sv:
         t0 = time.perf_counter()
-        cal = [sv.wait(x) for x in [sv.submit(r, steps, direction_groups=groups) for r in reqs[:8]]]
+        cal = [sv.wait(x) for x in [sv.submit(r, steps) for r in reqs[:8]]]
         capacity = len(cal) / (time.perf_counter() - t0)
     log(f"serving: calibration batch of {len(cal)} requests at {capacity:.2f} requests/s")
     rate = 0.8 * capacity  # Poisson arrivals at 80 % of the measured capacity
@@ -280,7 +282,7 @@ def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, step
                 delay = arrivals[i] - (time.perf_counter() - start)
                 if delay > 0:
                     time.sleep(delay)
-                tickets[i] = sv.submit(r, steps, direction_groups=groups)
+                tickets[i] = sv.submit(r, steps)
 
         th = threading.Thread(target=feeder)
         th.start()
@@ -305,7 +307,6 @@ def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, step
         "slots": slots,
         "hbm_budget_gb": budget_gb,
         "steps_per_request": steps,
-        "direction_groups_per_request": groups,
         "arrival_rate_per_s": rate,
         "arrival_rule": "Poisson at 0.8 x the completion rate of a saturated 8-request batch",
         "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
@@ -336,9 +337,9 @@ def main():
                     help="only run the 16777216-cell largest-domain measurement and print its JSON")
     ap.add_argument("--mode", type=int, default=0, choices=(0, 3, 4),
                     help="transport replay form: 0 automatic, 3 geometric, 4 compiled")
-    ap.add_argument("--groups", type=int, default=0, help="direction groups (0: library default)")
+    ap.add_argument("--block", type=int, default=0, help="compiled replay block size (0: library default)")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="use --mode/--groups instead of the autotuner's choice")
+                    help="use --mode/--block instead of the autotuner's choice")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -367,13 +368,13 @@ def main():
     tune = not args.no_autotune and args.precision == 0
     tune32 = tune64 = None
     if args.precision in (0, 32):
-        m, g, tune32 = tuned(pb, 32, args.warmup, tune, args.mode, args.groups)
-        fp32 = run_gpu(pb, 32, args.steps, args.warmup, mode=m, groups=g)
+        m, b, gr, tune32 = tuned(pb, 32, args.warmup, tune, args.mode, args.block)
+        fp32 = run_gpu(pb, 32, args.steps, args.warmup, mode=m, block=b, graphs=gr)
         log(f"fp32 transport: {fp32['updates_per_s'] / 1e9:.3f} G updates/s, {fp32['ms_per_step']:.2f} ms/step, "
             f"kernel {fp32['avg_launch_ms']:.3f} ms/launch, {fp32['achieved_gbs']:.0f} GB/s")
     if args.precision in (0, 64):
-        m, g, tune64 = tuned(pb, 64, args.warmup, tune, args.mode, args.groups)
-        fp64 = run_gpu(pb, 64, args.steps, args.warmup, mode=m, groups=g)
+        m, b, gr, tune64 = tuned(pb, 64, args.warmup, tune, args.mode, args.block)
+        fp64 = run_gpu(pb, 64, args.steps, args.warmup, mode=m, block=b, graphs=gr)
         log(f"fp64 transport: {fp64['updates_per_s'] / 1e9:.3f} G updates/s, {fp64['ms_per_step']:.2f} ms/step, "
             f"kernel {fp64['avg_launch_ms']:.3f} ms/launch, {fp64['achieved_gbs']:.0f} GB/s")
     if args.precision != 0:  # profiling runs: no JSON line
@@ -445,9 +446,9 @@ def main():
             "fp64_achieved": fp64["achieved_gbs"],
             "fp64_bytes_per_update": fp64["bytes_per_update"],
         },
-        "launch_shape": {"fp32": {"direction_groups": fp32["direction_groups"],
+        "launch_shape": {"fp32": {"replay_block": fp32["replay_block"], "graph_replay": fp32["graph_replay"],
                                   "transport_mode": fp32["transport_mode"], "autotune_ms_per_step": tune32},
-                         "fp64": {"direction_groups": fp64["direction_groups"],
+                         "fp64": {"replay_block": fp64["replay_block"], "graph_replay": fp64["graph_replay"],
                                   "transport_mode": fp64["transport_mode"], "autotune_ms_per_step": tune64}},
         "cpu_baseline": cpu,
         "fp32_error_vs_fp64": field_errors(fp32["results"], fp64["results"]),

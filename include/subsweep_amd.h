@@ -143,13 +143,13 @@ typedef struct ssw_info {
     int64_t device_bytes;          /* device memory held by this handle           */
     int64_t chemistry_failures;    /* cells whose chemistry hit the depth limit   */
     int32_t transport_mode;        /* SSW_TRANSPORT_MODE_GEOMETRIC / _COMPILED    */
-    int32_t schedule_builds_last_step; /* sweeps of the last step that rebuilt
-                                          their level sets                      */
+    int32_t list_builds_last_step; /* sweeps of the last step whose active-set
+                                      list was rebuilt (compaction)            */
     int64_t overflow_entries;      /* compiled upwind entries beyond the planes   */
     int64_t graph_capture_failures; /* replays launched one by one because their
                                        graph could not be captured            */
-    int32_t direction_groups;      /* groups whose level sets run side by side */
-    int32_t reserved0;
+    int32_t replay_block;          /* compiled replay block size               */
+    int32_t graph_replay;          /* 1 if replays are captured as graphs      */
 } ssw_info;
 
 int ssw_get_info(ssw_sweep* s, ssw_info* info);
@@ -157,11 +157,11 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info);
 /* Replay form of the transport (no counterpart in the reference, which has one
  * CPU code path); set before ssw_upload_grid. Both forms solve the same level
  * sets with the same arithmetic and give bitwise the same results.
- *  SSW_TRANSPORT_MODE_GEOMETRIC  task records (16 B per cell-direction and
- *      timestep level); every replay re-reads the face geometry.
+ *  SSW_TRANSPORT_MODE_GEOMETRIC  task records (16 B per cell-direction);
+ *      every replay re-reads the face geometry, 16 lanes per task.
  *  SSW_TRANSPORT_MODE_COMPILED   per task its upwind (neighbour, weight) list
- *      in level-position order (about 16 + 4 + 12 x 8 B per cell-direction and
- *      timestep level in FP32, twice the entry size in FP64), one lane per task.
+ *      in level-position order (about 16 + 4 + 12 x 8 B per cell-direction in
+ *      FP32, twice the entry size in FP64), one lane per task.
  *  SSW_TRANSPORT_MODE_AUTO       compiled when its arrays fit in half of the
  *      free device memory at ssw_upload_grid, else geometric (default).
  * Values 1 and 2 selected a persistent one-launch-per-sweep kernel that did
@@ -171,13 +171,17 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info);
 #define SSW_TRANSPORT_MODE_COMPILED 4
 int ssw_set_transport_mode(ssw_sweep* s, int32_t mode);
 
-/* Directions are split into `groups` contiguous blocks (1..16; 0: the
- * library default of 1, at most the direction count); each block's level
- * sets run on a HIP stream of its own, so the small, latency-bound launches
- * of different blocks overlap on the device. Dependencies never cross
- * directions, so results do not depend on the grouping (bitwise). Set before
- * ssw_upload_grid. No counterpart in the reference. */
-int ssw_set_direction_groups(ssw_sweep* s, int32_t groups);
+/* Block size of the compiled replay kernel (64, 128 or 256 threads, one task
+ * per thread; 0: the default of 64). A launch-shape knob for the autotuner:
+ * results do not depend on it. No counterpart in the reference. */
+int ssw_set_replay_block(ssw_sweep* s, int32_t block);
+
+/* Replays of an unchanged active set are captured as one HIP graph from their
+ * second use on (enabled = 1, default) or launched one level set at a time
+ * (0). Capture needs the process to hold no other handle at that moment
+ * (other threads' allocations invalidate a stream capture), so the request
+ * loop turns it off. No counterpart in the reference. */
+int ssw_set_graph_replay(ssw_sweep* s, int32_t enabled);
 
 /* Raw transport state in its storage precision and scale (FP32 or FP64 per
  * the transport precision, direction-major): X = outgoing rate / downwind
@@ -232,7 +236,7 @@ typedef struct ssw_request {
     int32_t transport_mode;        /* SSW_TRANSPORT_MODE_*; AUTO: compiled if it
                                       fits budget / slots, else geometric      */
     int32_t steps;                 /* ssw_step calls                          */
-    int32_t direction_groups;      /* as ssw_set_direction_groups (0: default) */
+    int32_t replay_block;          /* as ssw_set_replay_block (0: default)     */
     int32_t reserved0;
     int64_t num_cells;             /* grid, as for ssw_upload_grid            */
     const int64_t* face_offsets;

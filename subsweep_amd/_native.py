@@ -63,11 +63,11 @@ class InfoC(C.Structure):
         ("device_bytes", C.c_int64),
         ("chemistry_failures", C.c_int64),
         ("transport_mode", C.c_int32),
-        ("schedule_builds_last_step", C.c_int32),
+        ("list_builds_last_step", C.c_int32),
         ("overflow_entries", C.c_int64),
         ("graph_capture_failures", C.c_int64),
-        ("direction_groups", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("replay_block", C.c_int32),
+        ("graph_replay", C.c_int32),
     ]
 
 
@@ -82,7 +82,7 @@ class RequestC(C.Structure):
         ("params", SweepParametersC),
         ("transport_mode", C.c_int32),
         ("steps", C.c_int32),
-        ("direction_groups", C.c_int32),
+        ("replay_block", C.c_int32),
         ("reserved0", C.c_int32),
         ("num_cells", C.c_int64),
         ("face_offsets", C.c_void_p),
@@ -141,7 +141,8 @@ SWEEP_SIGNATURES = {
     "ssw_synchronize": (C.c_int, [P]),
     "ssw_set_transport_mode": (C.c_int, [P, C.c_int32]),
     "ssw_set_timing": (C.c_int, [P, C.c_int32]),
-    "ssw_set_direction_groups": (C.c_int, [P, C.c_int32]),
+    "ssw_set_replay_block": (C.c_int, [P, C.c_int32]),
+    "ssw_set_graph_replay": (C.c_int, [P, C.c_int32]),
     "ssw_get_timing": (C.c_int, [P, PD, PD, PI64, PD, PD]),
     "ssw_download_transport": (C.c_int, [P, P, P]),
     "ssw_timestep_level": (C.c_int32, [C.c_int32, C.c_double, C.c_double]),

@@ -1,13 +1,14 @@
 """Launch-shape autotuner for one problem.
 
-The transport's throughput on a given grid depends on how its level sets are
-laid out and launched:
+The transport's speed on a given grid depends on how its level sets are
+replayed:
 
-* direction groups - how many blocks of directions run their level sets side
-  by side on separate streams (more groups overlap more of each level's
-  latency, fewer keep each launch wide);
 * transport mode - compiled replay (one lane per task, upwind entries stored
-  per task) or geometric replay (16 lanes per task over the face lists).
+  per task) or geometric replay (16 lanes per task over the face lists);
+* replay block - the thread-block size of the compiled replay (64, 128 or
+  256 tasks per block);
+* graph replay - level-set launches captured as one HIP graph, or launched
+  one at a time.
 
 `autotune` times a few steps of each candidate on the problem itself and
 returns the fastest. Every candidate computes bitwise the same results, so
@@ -16,7 +17,6 @@ the choice affects speed only.
 
 from __future__ import annotations
 
-import itertools
 import time
 from dataclasses import dataclass
 
@@ -27,27 +27,33 @@ from .workloads import Problem
 
 @dataclass
 class TuneResult:
-    direction_groups: int
     transport_mode: int
+    replay_block: int
+    graph_replay: bool
     ms_per_step: float
-    table: list  # (groups, mode, ms per step) of every candidate
+    table: list  # (mode, block, graphs, ms per step) of every candidate
 
     def kwargs(self) -> dict:
-        return {"direction_groups": self.direction_groups, "transport_mode": self.transport_mode}
+        return {"transport_mode": self.transport_mode, "replay_block": self.replay_block,
+                "graph_replay": self.graph_replay}
 
 
-def autotune(problem: Problem, groups=(1, 2, 4), modes=(N.TRANSPORT_MODE_COMPILED, N.TRANSPORT_MODE_GEOMETRIC),
-             steps: int = 2, warmup: int = 1) -> TuneResult:
-    """Times `steps` steps after `warmup` steps of every (groups, mode)
-    candidate; candidates that do not fit in device memory are skipped."""
+def candidates(blocks=(64, 128, 256)):
+    out = [(N.TRANSPORT_MODE_COMPILED, b, True) for b in blocks]
+    out.append((N.TRANSPORT_MODE_COMPILED, blocks[0], False))
+    out.append((N.TRANSPORT_MODE_GEOMETRIC, 0, True))  # one launch shape (16 lanes per task)
+    return out
+
+
+def autotune(problem: Problem, cands=None, steps: int = 2, warmup: int = 2) -> TuneResult:
+    """Times `steps` steps after `warmup` steps of every (mode, block, graphs)
+    candidate on a fresh handle each; candidates that do not fit in device
+    memory are skipped."""
     table = []
-    d = problem.params.directions
-    ndir = d if isinstance(d, int) else len(d)
-    for g, m in itertools.product(groups, modes):
-        if g > ndir:
-            continue
+    for m, b, gr in (cands or candidates()):
         try:
-            s = Sweep(problem.params, problem.grid, *problem.state(), transport_mode=m, direction_groups=g)
+            s = Sweep(problem.params, problem.grid, *problem.state(), transport_mode=m, replay_block=b,
+                      graph_replay=gr)
         except N.SweepError as e:
             if e.code == N.SSW_ERR_OOM:
                 continue
@@ -60,10 +66,10 @@ def autotune(problem: Problem, groups=(1, 2, 4), modes=(N.TRANSPORT_MODE_COMPILE
             for _ in range(steps):
                 s.run_sweeps()
             s.synchronize()
-            table.append((g, m, (time.perf_counter() - t0) / steps * 1e3))
+            table.append((m, b, gr, (time.perf_counter() - t0) / steps * 1e3))
         finally:
             s.close()
     if not table:
         raise RuntimeError("no autotuning candidate fits on the device")
-    g, m, ms = min(table, key=lambda r: r[2])
-    return TuneResult(g, m, ms, table)
+    m, b, gr, ms = min(table, key=lambda r: r[3])
+    return TuneResult(m, b, gr, ms, table)

@@ -80,7 +80,8 @@ void run_request(Req* q) {
         return;
     }
     rc = ssw_set_transport_mode(s, q->mode);
-    if (rc == SSW_OK) rc = ssw_set_direction_groups(s, r.direction_groups);
+    if (rc == SSW_OK) rc = ssw_set_replay_block(s, r.replay_block);
+    if (rc == SSW_OK) rc = ssw_set_graph_replay(s, 0);  // handles of other requests allocate concurrently
     if (rc == SSW_OK)
         rc = ssw_upload_grid(s, r.num_cells, r.face_offsets, r.face_neighbour, r.face_kind, r.face_area, r.face_normal,
                              r.cell_volume, r.cell_size);

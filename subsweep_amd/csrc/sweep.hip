@@ -10,20 +10,26 @@
 //              X = outgoing rate / downwind effective area (FP64 or scaled FP32)
 //              I = incoming rate at solve time (FP64 or scaled FP32)
 //              P[d * Nper + slot] periodic-source snapshot of the periodic cells
-//   schedule   per sweep level l (the active set {level >= l}): the (cell,
-//              direction) tasks in Kahn order, level-major; lstart[l][k] is where
-//              level set k begins. Two replay forms:
+//   schedule   the (cell, direction) tasks of the full dependency graph in Kahn
+//              order, level-major, within a level sorted by (direction, cell);
+//              full_start[k] is where level set k begins. Two replay forms:
 //              geometric  task records {cell | face_count << 25, first_face,
-//                         direction, periodic slot} (16 B per task and sweep level)
+//                         direction, periodic slot} (16 B per task)
 //              compiled   per task a header {cell | direction << 25, periodic
 //                         slot, overflow start, upwind count}, the effective area
 //                         A, and its upwind local faces as (neighbour, weight)
 //                         pairs in kPlanes planes: entry q of the task at
-//                         position i of a level set starting at b with width W
-//                         is planes[kPlanes * b + q * W + (i - b)] (coalesced
+//                         position p of a level set starting at b with width W
+//                         is planes[kPlanes * b + q * W + (p - b)] (coalesced
 //                         across tasks, addressable without reading the header);
-//                         entries q >= kPlanes (rare) sit in a shared overflow
-//                         list, indexed per (direction, cell).
+//                         entries q >= kPlanes (rare) sit in an overflow list,
+//                         indexed per (direction, cell).
+//              A sweep at timestep level l solves the cells with level >= l.
+//              Restricted to those cells, the full graph's level sets are still
+//              a valid order (an active task's active upwind neighbours are in
+//              earlier level sets), so such a sweep replays the full schedule
+//              through a per-level list of the active positions (idx[l]),
+//              made by a stream compaction when the active set changes.
 //
 // The transport is the reference's upwind sweep (src/sweep/mod.rs:412-513)
 // restated in pull form. A cell that is solved distributes the change of its
@@ -33,24 +39,19 @@
 // with A[nb,d] = sum of area (n . d) over nb's downwind faces. Storing
 // X = O / A lets every task gather its upwind neighbours' X instead of
 // scattering with atomics. The upwind sum is evaluated in face order, one term
-// after the other, in every kernel, so the build, the geometric replay and the
-// compiled replay give bitwise the same result, whatever order tasks run in.
-// Tasks are run level set by level set: level k holds the (cell, direction)
-// pairs whose active upwind neighbours are all in levels < k (Kahn layering of
-// init_counts' counts, src/sweep/mod.rs:346-386). The level sets of an active
-// set are built on the device by the first sweep that needs them (counter
-// decrements, block-aggregated appends) and replayed by later sweeps, as one
-// captured HIP graph of exact-size launches, until a timestep-level change
-// alters the active set.
+// after the other, and A by a fixed 16-way tree, in every kernel, so the
+// geometric and the compiled replay give bitwise the same result, whatever
+// order tasks run in. The level sets (Kahn layering of init_counts' counts,
+// src/sweep/mod.rs:346-386) are built on the device once per grid by counter
+// decrements with block-aggregated appends; the level-set parity hook builds
+// the layering of a restricted active set the same way.
 
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <atomic>
 #include <cmath>
-#include <condition_variable>
-#include <mutex>
-#include <thread>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -79,8 +80,7 @@ constexpr uint32_t kCellMask = (1u << kCellBits) - 1u;
 constexpr int kMaxFaces = 127;             // faces per cell
 constexpr int kQueueCap = kTasksPerBlock * kMaxFaces;
 constexpr int kPlanes = 12;                // upwind entries per task held in planes
-constexpr int kRecBlock = 64;              // compiled replay: one lane per task
-constexpr int kDefaultGroups = 1;          // direction groups run side by side
+constexpr int kChunk = 4096;               // compaction chunk (tasks, within one level set)
 
 thread_local std::string g_err;
 // Handles alive in the process. Replay graphs are captured only while a single
@@ -134,8 +134,6 @@ struct Geo {
     int64_t N;
     int D;
     int L;
-    int G;                   // direction groups (schedules run side by side)
-    int Kcap;                // level starts per (slot, group) = Kcap + 2
     const uint32_t* off;
     const double4* geo;
     const uint32_t* nk;
@@ -146,14 +144,11 @@ struct Geo {
     const uint32_t* pfoff;   // Nper + 1: periodic faces of each periodic cell
     const uint32_t* pfidx;   // face indices
     int64_t Nper;
-    const uint8_t* dgroup;   // D: direction group of each direction
-    const uint32_t* gbase;   // G: first task position of a group within a slot (N x its first direction)
-    uint4* flat;             // task records: per slot (geometric) or one slot of scratch (compiled)
-    size_t flat_slot_stride; // D * N (geometric) or 0 (compiled)
-    uint32_t* lstart;        // L x G x (Kcap + 2), positions relative to the group's base
-    uint32_t* ctr;           // L x G x 3 rotating append counters
+    uint4* flat;             // Kahn layering: task records in level order (D*N)
+    uint32_t* lstart;        // Kahn layering: level starts
+    uint32_t* ctr;           // Kahn layering: 3 rotating append counters
     uint32_t* cnt;           // D x N missing-upwind counters
-    uint32_t* need;          // L: schedule must be (re)built
+    uint32_t* need;          // L: the active set of a sweep level changed
 };
 
 template <typename R>
@@ -166,16 +161,6 @@ struct Flux {
     double thr;              // threshold in the storage scale
     double inv_lref;         // storage scale of rates
     double inv_aref;         // storage scale of areas
-};
-
-// Compiled schedule (null pointers in geometric mode).
-template <typename R>
-struct Comp {
-    uint4* hdr;              // L x D*N
-    R* A;                    // L x D*N
-    Entry<R>* planes;        // L x kPlanes*D*N
-    const uint32_t* ovf_start;  // D x N: first overflow entry of a task with > kPlanes upwind entries
-    const Entry<R>* ovf;
 };
 
 __device__ __forceinline__ uint4 task_record(uint32_t c, uint32_t f0, uint32_t f1, uint32_t d, uint32_t ps) {
@@ -234,22 +219,18 @@ __device__ __forceinline__ void finish_task(const Flux<R>& fl, size_t o, R inc, 
     fl.I[o] = inc;
 }
 
-// The face loop of one task, kLanes lanes per task, shared by the build and
-// the geometric replay so that both evaluate identical arithmetic. Lane `sub`
-// handles faces sub, sub + kLanes, ...; the loop runs to the longest face list
-// of the wave so that every lane takes part in the shuffles. The upwind sum is
-// accumulated in face order on every lane of the group; A is a fixed-shape
-// tree over the lanes' partial sums.
+// The face loop of one task, kLanes lanes per task (one face each), shared by
+// the Kahn build and the geometric replay. Lane `sub` handles faces sub,
+// sub + kLanes, ...; the loop runs to the longest face list of the wave so
+// that every lane takes part in the shuffles. The upwind sum is accumulated
+// in face order on every lane of the group; A is the xor-butterfly over the
+// lanes' partial sums.
 //   kTransport  gather the upwind X and sum
 //   kBuild      decrement the downwind neighbours' counters and queue the
 //               tasks that become ready into the block queue `sq`
-//   kCompile    write the task's upwind entries into the planes
-template <typename R, bool kTransport, bool kBuild, bool kCompile>
+template <typename R, bool kTransport, bool kBuild>
 __device__ __forceinline__ void task_faces(const Geo& g, const Flux<R>& fl, bool valid, uint4 r, uint32_t sub,
-                                           uint32_t nfmax, double& A_out, R& inc_out, uint32_t& nup_out,
-                                           uint4* sq, uint32_t* sq_n, Entry<R>* planes, uint32_t W,
-                                           uint32_t pos) {
-    const uint32_t c = r.x & kCellMask;
+                                           uint32_t nfmax, double& A_out, R& inc_out, uint4* sq, uint32_t* sq_n) {
     const uint32_t nf = valid ? (r.x >> kCellBits) : 0u;
     const uint32_t d = r.z;
     const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
@@ -258,7 +239,6 @@ __device__ __forceinline__ void task_faces(const Geo& g, const Flux<R>& fl, bool
     const int gbase = lane & ~(kLanes - 1);
     double A = 0.0;
     R inc = 0;
-    uint32_t nup = 0;
     for (uint32_t base = 0; base < nfmax; base += kLanes) {
         const uint32_t j = base + sub;
         const bool fa = j < nf;
@@ -288,19 +268,6 @@ __device__ __forceinline__ void task_faces(const Geo& g, const Flux<R>& fl, bool
 #pragma unroll
             for (int l = 0; l < kLanes; l++) inc += __shfl(v, gbase + l);
         }
-        if (kCompile) {
-            const unsigned long long m = __ballot(up);
-            const uint32_t seg = (uint32_t)(m >> gbase) & 0xFFFFu;
-            const uint32_t q = nup + (uint32_t)__popc(seg & ((1u << sub) - 1u));
-            if (up && q < (uint32_t)kPlanes) {
-                Entry<R> e;
-                e.nb = nb;
-                if constexpr (sizeof(R) == 8) e.pad = 0;
-                e.w = w;
-                planes[(size_t)q * W + pos] = e;
-            }
-            nup += (uint32_t)__popc(seg);
-        }
         if (kBuild) {
             const unsigned long long m = __ballot(ready);
             if (m) {
@@ -315,7 +282,6 @@ __device__ __forceinline__ void task_faces(const Geo& g, const Flux<R>& fl, bool
     for (int m = kLanes / 2; m > 0; m >>= 1) A += __shfl_xor(A, m, kLanes);
     A_out = A;
     inc_out = inc;
-    nup_out = nup;
 }
 
 // ---------------------------------------------------------------------------
@@ -415,44 +381,20 @@ __global__ void k_overflow(Geo g, double inv_aref, const uint32_t* ovf_count, ui
 }
 
 // ---------------------------------------------------------------------------
-// per-sweep kernels
+// Kahn layering (once per grid for the full graph; on demand for a restricted
+// active set, for the parity hook)
 
-// Resets the append counters of slot `lvl` when its schedule is rebuilt and
-// takes the periodic-source snapshot of the active periodic cells (the
-// transport reads the periodic source as it was when the sweep started).
-// Thread t handles (periodic cell t / D, direction t % D).
-template <typename R>
-__global__ void k_begin(Geo g, Flux<R> fl, int lvl) {
-    if (blockIdx.x == 0 && threadIdx.x < 3 * g.G && g.need[lvl] != 0) {
-        g.ctr[lvl * 3 * g.G + threadIdx.x] = 0;
-        if (threadIdx.x < g.G) g.lstart[((size_t)lvl * g.G + threadIdx.x) * (g.Kcap + 2)] = 0;
-    }
-    if (fl.P == nullptr) return;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t s = t / (uint32_t)g.D;
-    const uint32_t d = t - s * (uint32_t)g.D;
-    if ((int64_t)s >= g.Nper) return;
-    const uint32_t c = g.pcells[s];
-    if (g.level[c] < lvl) return;
-    const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
-    R acc = 0;
-    for (uint32_t e = g.pfoff[s]; e < g.pfoff[s + 1]; e++) {
-        const uint32_t f = g.pfidx[e];
-        const uint32_t nk = g.nk[f];
-        const double4 a = g.geo[f];
-        const double dd = fdot(a, dx, dy, dz);
-        if (dd < 0.0) acc += (R)(a.w * fl.inv_aref * -dd) * fl.X[(size_t)d * g.N + (nk & kNbrMask)];
-    }
-    fl.P[(size_t)d * g.Nper + s] = acc;
+// Resets the append counters and the first level start.
+__global__ void k_kahn_reset(Geo g) {
+    if (threadIdx.x < 3) g.ctr[threadIdx.x] = 0;
+    if (threadIdx.x == 0) g.lstart[0] = 0;
 }
 
-// init_counts (src/sweep/mod.rs:346-386) + get_initial_tasks (:388-398):
-// counts the active local upwind neighbours of every active (cell, direction)
-// and queues the tasks with count 0 as level set 0. Thread t handles
-// (cell t / D, direction t % D).
+// init_counts (src/sweep/mod.rs:346-386) + get_initial_tasks (:388-398) for the
+// active set {level >= lvl}: counts the active local upwind neighbours of every
+// active (cell, direction) and queues the tasks with count 0 as level set 0
+// (unless only_counts). Thread t handles (cell t / D, direction t % D).
 __global__ void k_count(Geo g, int lvl, int only_counts) {
-    const bool build = only_counts || g.need[lvl] != 0;
-    if (!build) return;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t c, d;
     const bool in = cell_dir(g, t, c, d);
@@ -471,25 +413,20 @@ __global__ void k_count(Geo g, int lvl, int only_counts) {
     }
     if (in) g.cnt[(size_t)d * g.N + c] = active ? n : kNone;
     if (only_counts) return;
-    uint4* flat = g.flat + (size_t)lvl * g.flat_slot_stride;
     const bool ready = active && n == 0;
-    const uint32_t grp = in ? g.dgroup[d] : 0u;
-    const uint4 rec = ready ? task_record(c, f0, f1, d, g.pslot[c]) : make_uint4(0, 0, 0, 0);
-    for (int q = 0; q < g.G; q++)  // one aggregated append per group present in the wave
-        wave_append(ready && grp == (uint32_t)q, rec, &g.ctr[((size_t)lvl * g.G + q) * 3 + 0], flat + g.gbase[q]);
+    wave_append(ready, ready ? task_record(c, f0, f1, d, g.pslot[c]) : make_uint4(0, 0, 0, 0), &g.ctr[0], g.flat);
 }
 
-// Builds level set k+1 of slot `lvl` while solving level set k (kTransport) or
-// only building (schedules for the parity hook and the depth bound). kLanes
-// lanes per task; ready tasks are gathered per block in LDS and appended with
-// one atomic per block. With kCompile also writes the compiled schedule.
-template <typename R, bool kTransport, bool kCompile>
-__global__ void __launch_bounds__(kBlock) k_sweep_build(Geo g, Flux<R> fl, Comp<R> cp, int lvl, int k, int grp) {
+// Level set k+1 of the layering from level set k: kLanes lanes per task
+// decrement the counters of the local downwind neighbours (an inactive
+// neighbour's counter starts at 0xFFFFFFFF and never reaches 0); the tasks that
+// become ready are gathered per block in LDS and appended with one atomic per
+// block (handle_local_neighbour, src/sweep/mod.rs:487-503).
+__global__ void __launch_bounds__(kBlock) k_kahn(Geo g, int k) {
     __shared__ uint4 sq[kQueueCap];
     __shared__ uint32_t sq_n, sq_base, sq_cnt;
-    const size_t unit = (size_t)lvl * g.G + grp;
-    uint32_t* ls = g.lstart + unit * (g.Kcap + 2);
-    uint32_t* ctr = g.ctr + unit * 3;
+    uint32_t* ls = g.lstart;
+    uint32_t* ctr = g.ctr;
     const uint32_t beg = ls[k];
     const uint32_t end = beg + ctr[k % 3];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -497,38 +434,20 @@ __global__ void __launch_bounds__(kBlock) k_sweep_build(Geo g, Flux<R> fl, Comp<
         ctr[(k + 2) % 3] = 0;
     }
     if (beg >= end) return;
-    const uint32_t W = end - beg;
-    const uint32_t gb = g.gbase[grp];
-    uint4* flat = g.flat + (size_t)lvl * g.flat_slot_stride + gb;
     uint32_t* next_ctr = ctr + (k + 1) % 3;
-    const size_t DN = (size_t)g.D * g.N;
-    Entry<R>* planes = kCompile ? cp.planes + (size_t)lvl * kPlanes * DN + (size_t)kPlanes * (gb + beg) : nullptr;
     const uint32_t sub = threadIdx.x % kLanes;
     const uint32_t stride = gridDim.x * kTasksPerBlock;
+    Flux<float> fl{};
     if (threadIdx.x == 0) sq_n = 0;
     __syncthreads();
     for (uint32_t i0 = beg + blockIdx.x * kTasksPerBlock; i0 < end; i0 += stride) {
         const uint32_t i = i0 + threadIdx.x / kLanes;
         const bool valid = i < end;
-        const uint4 r = valid ? flat[i] : make_uint4(0, 0, 0, 0);
+        const uint4 r = valid ? g.flat[i] : make_uint4(0, 0, 0, 0);
         const uint32_t nfmax = wave_max(valid ? (r.x >> kCellBits) : 0u);
-        const uint32_t c = r.x & kCellMask, d = r.z, ps = r.w;
-        CellIn<R> ci{0.0, 0.0, (R)0};
-        if (kTransport) ci = cell_inputs<R>(g, fl, c, d, ps);
         double A;
-        R inc;
-        uint32_t nup;
-        task_faces<R, kTransport, true, kCompile>(g, fl, valid, r, sub, nfmax, A, inc, nup, sq, &sq_n, planes, W,
-                                                  i - beg);
-        if (valid && sub == 0) {
-            const size_t o = (size_t)d * g.N + c;
-            if (kTransport) finish_task<R>(fl, o, inc, ci, (R)A);
-            if (kCompile) {
-                const size_t so = (size_t)lvl * DN + gb + i;
-                cp.hdr[so] = make_uint4(c | (d << kCellBits), ps, nup > (uint32_t)kPlanes ? cp.ovf_start[o] : 0u, nup);
-                cp.A[so] = (R)A;
-            }
-        }
+        float inc;
+        task_faces<float, false, true>(g, fl, valid, r, sub, nfmax, A, inc, sq, &sq_n);
         // flush the block queue into level set k+1
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -539,43 +458,194 @@ __global__ void __launch_bounds__(kBlock) k_sweep_build(Geo g, Flux<R> fl, Comp<
         }
         __syncthreads();
         const uint32_t n = sq_cnt, b = end + sq_base;
-        for (uint32_t t = threadIdx.x; t < n; t += kBlock) flat[b + t] = sq[t];
+        for (uint32_t t = threadIdx.x; t < n; t += kBlock) g.flat[b + t] = sq[t];
         __syncthreads();
     }
 }
 
-// Geometric replay of one level set: `count` task records, kLanes lanes per task.
+// Kahn depth of every task of a layering: entry i of the level-major array is
+// in level set k with lstart[k] <= i < lstart[k+1].
+__global__ void k_scatter_depth(Geo g, int kmax, uint32_t* depth) {
+    const uint32_t total = g.lstart[kmax];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        int a = 0, b = kmax - 1;  // last k with lstart[k] <= i
+        while (a < b) {
+            int m = (a + b + 1) >> 1;
+            if (g.lstart[m] <= i) a = m;
+            else b = m - 1;
+        }
+        const uint4 r = g.flat[i];
+        depth[(size_t)r.z * g.N + (r.x & kCellMask)] = (uint32_t)a;
+    }
+}
+
+__global__ void k_iota(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+// The full schedule from the tasks sorted by (depth, direction, cell): task t =
+// d*N + c at position p of level set k (start b, width W). Compiled: header,
+// A and plane entries (one thread per task, faces in order; A as the same
+// 16-lane butterfly the geometric kernels form). Geometric: the task record.
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_sweep_rec(Geo g, Flux<R> fl, const uint4* recs, uint32_t count) {
+__global__ void k_schedule(Geo g, double inv_aref, const uint32_t* sorted, const uint32_t* fstart, int kmax,
+                           int compiled, const uint32_t* ovf_start, uint4* hdr, R* Av, Entry<R>* planes,
+                           uint4* recs) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int64_t)p >= (int64_t)g.D * g.N) return;
+    int a = 0, bb = kmax - 1;  // level set of position p
+    while (a < bb) {
+        int m = (a + bb + 1) >> 1;
+        if (fstart[m] <= p) a = m;
+        else bb = m - 1;
+    }
+    const uint32_t b = fstart[a], W = fstart[a + 1] - b;
+    const uint32_t t = sorted[p];
+    const uint32_t d = t / (uint32_t)g.N, c = t - d * (uint32_t)g.N;
+    const uint32_t f0 = g.off[c], f1 = g.off[c + 1], ps = g.pslot[c];
+    if (!compiled) {
+        recs[p] = task_record(c, f0, f1, d, ps);
+        return;
+    }
+    const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+    double part[kLanes];
+#pragma unroll
+    for (int l = 0; l < kLanes; l++) part[l] = 0.0;
+    uint32_t q = 0;
+    for (uint32_t f = f0; f < f1; f++) {
+        const double4 fa = g.geo[f];
+        const uint32_t nk = g.nk[f];
+        const double dd = fdot(fa, dx, dy, dz);
+        if (dd > 0.0) {
+            const uint32_t l = (f - f0) % kLanes;  // the lane that holds face f in the 16-lane kernels
+            const double term = fa.w * inv_aref * dd;
+#pragma unroll
+            for (int u = 0; u < kLanes; u++)
+                if ((uint32_t)u == l) part[u] += term;
+        } else if (dd < 0.0 && (nk >> 30) == kKindLocal) {
+            if (q < (uint32_t)kPlanes) {
+                Entry<R> e;
+                e.nb = nk & kNbrMask;
+                if constexpr (sizeof(R) == 8) e.pad = 0;
+                e.w = (R)(fa.w * inv_aref * -dd);
+                planes[(size_t)kPlanes * b + (size_t)q * W + (p - b)] = e;
+            }
+            q++;
+        }
+    }
+    // the xor butterfly of the 16-lane kernels, as seen from lane 0
+#pragma unroll
+    for (int m = kLanes / 2; m > 0; m >>= 1) {
+        double nxt[kLanes];
+#pragma unroll
+        for (int l = 0; l < kLanes; l++) nxt[l] = part[l] + part[l ^ m];
+#pragma unroll
+        for (int l = 0; l < kLanes; l++) part[l] = nxt[l];
+    }
+    hdr[p] = make_uint4(c | (d << kCellBits), ps, q > (uint32_t)kPlanes ? ovf_start[(size_t)d * g.N + c] : 0u, q);
+    Av[p] = (R)part[0];
+}
+
+// ---------------------------------------------------------------------------
+// compaction of the full schedule to an active set (per sweep level);
+// chunks[j] = {level set, first position, end position, 0}, never across level sets
+
+__device__ __forceinline__ uint32_t task_cell(const uint4* tasks, uint32_t p) { return tasks[p].x & kCellMask; }
+
+__global__ void __launch_bounds__(kBlock) k_compact_count(Geo g, const uint4* tasks, const uint4* chunks,
+                                                          uint32_t nchunks, int lvl, uint32_t* counts) {
+    const uint32_t j = blockIdx.x;
+    if (j >= nchunks) return;
+    const uint4 ch = chunks[j];
+    __shared__ uint32_t total;
+    if (threadIdx.x == 0) total = 0;
+    __syncthreads();
+    uint32_t n = 0;
+    for (uint32_t p = ch.y + threadIdx.x; p < ch.z; p += kBlock) n += g.level[task_cell(tasks, p)] >= lvl ? 1u : 0u;
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_down(n, o);
+    if (__lane_id() == 0 && n) atomicAdd(&total, n);
+    __syncthreads();
+    if (threadIdx.x == 0) counts[j] = total;
+}
+
+// Writes the active positions of each chunk, in order, relative to the start
+// of their level set: idx[offset[j] + rank] = p - fstart[level].
+__global__ void __launch_bounds__(kBlock) k_compact_write(Geo g, const uint4* tasks, const uint4* chunks,
+                                                          uint32_t nchunks, const uint32_t* fstart, int lvl,
+                                                          const uint32_t* offset, uint32_t* idx) {
+    const uint32_t j = blockIdx.x;
+    if (j >= nchunks) return;
+    const uint4 ch = chunks[j];
+    const uint32_t b = fstart[ch.x];
+    __shared__ uint32_t wave_tot[kBlock / 64];
+    uint32_t base = offset[j];
+    const int lane = __lane_id(), wv = threadIdx.x / 64;
+    for (uint32_t p0 = ch.y; p0 < ch.z; p0 += kBlock) {
+        const uint32_t p = p0 + threadIdx.x;
+        const bool act = p < ch.z && g.level[task_cell(tasks, p)] >= lvl;
+        const unsigned long long m = __ballot(act);
+        if (lane == 0) wave_tot[wv] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = 0, all = 0;
+        for (int w = 0; w < kBlock / 64; w++) {
+            if (w < wv) before += wave_tot[w];
+            all += wave_tot[w];
+        }
+        if (act) idx[base + before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = p - b;
+        base += all;
+        __syncthreads();
+    }
+}
+
+// Compacted level starts: cstart[k] = offset of the level set's first chunk.
+__global__ void k_compact_starts(const uint32_t* first_chunk, const uint32_t* offset, int kmax, uint32_t total,
+                                 uint32_t* cstart) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < kmax) cstart[k] = offset[first_chunk[k]];
+    if (k == kmax) cstart[k] = total;
+}
+
+// ---------------------------------------------------------------------------
+// replay kernels
+
+// Geometric replay of one level set: `count` tasks, kLanes lanes per task; with
+// an index list, task i is record recs[idx[i]].
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_sweep_rec(Geo g, Flux<R> fl, const uint4* recs, const uint32_t* idx,
+                                                      uint32_t count) {
     const uint32_t i = blockIdx.x * kTasksPerBlock + threadIdx.x / kLanes;
     const uint32_t sub = threadIdx.x % kLanes;
     const bool valid = i < count;
-    const uint4 r = valid ? recs[i] : make_uint4(0, 0, 0, 0);
+    const uint32_t p = valid ? (idx ? idx[i] : i) : 0u;
+    const uint4 r = valid ? recs[p] : make_uint4(0, 0, 0, 0);
     const uint32_t ps = r.w;
     const uint32_t c = r.x & kCellMask, d = r.z;
     const CellIn<R> ci = cell_inputs<R>(g, fl, c, d, ps);
     const uint32_t nfmax = wave_max(valid ? (r.x >> kCellBits) : 0u);
     double A;
     R inc;
-    uint32_t nup;
-    task_faces<R, true, false, false>(g, fl, valid, r, sub, nfmax, A, inc, nup, nullptr, nullptr, nullptr, 0, 0);
+    task_faces<R, true, false>(g, fl, valid, r, sub, nfmax, A, inc, nullptr, nullptr);
     if (valid && sub == 0) finish_task<R>(fl, (size_t)d * g.N + c, inc, ci, (R)A);
 }
 
 // Compiled replay of one level set of width W: one lane per task. The header,
 // A and the plane entries of a task are independent loads; the upwind X
-// gathers follow. The sum runs over the upwind entries in face order.
-template <typename R>
+// gathers and the cell inputs follow. With an index list, lane i solves the
+// task at position idx[i] of the level set.
+template <typename R, int kRecBlock>
 __global__ void __launch_bounds__(kRecBlock) k_sweep_comp(Geo g, Flux<R> fl, const uint4* hdr, const R* Av,
-                                                          const Entry<R>* planes, const Entry<R>* ovf, uint32_t W) {
+                                                          const Entry<R>* planes, const Entry<R>* ovf, uint32_t W,
+                                                          const uint32_t* idx, uint32_t count) {
     const uint32_t i = blockIdx.x * kRecBlock + threadIdx.x;
-    if (i >= W) return;
+    if (i >= count) return;
+    const uint32_t p = idx ? idx[i] : i;
     // first memory step: the plane entries, the header and A (independent)
     Entry<R> e[kPlanes];
 #pragma unroll
-    for (int q = 0; q < kPlanes; q++) e[q] = planes[(size_t)q * W + i];
-    const uint4 h = hdr[i];
-    const R A = Av[i];
+    for (int q = 0; q < kPlanes; q++) e[q] = planes[(size_t)q * W + p];
+    const uint4 h = hdr[p];
+    const R A = Av[p];
     const uint32_t c = h.x & kCellMask, d = h.x >> kCellBits, ps = h.y, nup = h.w;
     const size_t dN = (size_t)d * g.N;
     // second memory step: the upwind X (entries past the upwind count gather
@@ -592,6 +662,33 @@ __global__ void __launch_bounds__(kRecBlock) k_sweep_comp(Geo g, Flux<R> fl, con
         inc += o.w * fl.X[dN + o.nb];
     }
     finish_task<R>(fl, dN + c, inc, ci, A);
+}
+
+// ---------------------------------------------------------------------------
+// per-sweep kernels
+
+// Periodic-source snapshot of the active periodic cells (the transport reads
+// the periodic source as it was when the sweep started). Thread t handles
+// (periodic cell t / D, direction t % D).
+template <typename R>
+__global__ void k_begin(Geo g, Flux<R> fl, int lvl) {
+    if (fl.P == nullptr) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = t / (uint32_t)g.D;
+    const uint32_t d = t - s * (uint32_t)g.D;
+    if ((int64_t)s >= g.Nper) return;
+    const uint32_t c = g.pcells[s];
+    if (g.level[c] < lvl) return;
+    const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+    R acc = 0;
+    for (uint32_t e = g.pfoff[s]; e < g.pfoff[s + 1]; e++) {
+        const uint32_t f = g.pfidx[e];
+        const uint32_t nk = g.nk[f];
+        const double4 a = g.geo[f];
+        const double dd = fdot(a, dx, dy, dz);
+        if (dd < 0.0) acc += (R)(a.w * fl.inv_aref * -dd) * fl.X[(size_t)d * g.N + (nk & kNbrMask)];
+    }
+    fl.P[(size_t)d * g.Nper + s] = acc;
 }
 
 struct ChemArgs {
@@ -620,7 +717,7 @@ struct ChemArgs {
 template <typename R>
 __global__ void k_chemistry(Geo g, Flux<R> fl, ChemArgs ca, int lvl) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c == 0) g.need[lvl] = 0;  // the schedule of this active set is complete
+    if (c == 0) g.need[lvl] = 0;  // the active-set list of this sweep level is current
     if (c >= g.N) return;
     int lev = g.level[c];
     {
@@ -679,7 +776,7 @@ __global__ void k_chemistry(Geo g, Flux<R> fl, ChemArgs ca, int lvl) {
 }
 
 // update_timestep_levels (src/sweep/mod.rs:576-589) for all cells. Marks the
-// schedules whose active set changed.
+// sweep levels whose active set changed.
 __global__ void k_levels(Geo g, const double* change_ts, uint8_t* level, double max_ts, double safety,
                          int lowest) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -727,26 +824,6 @@ __global__ void k_fluxes(Geo g, Flux<R> fl, double lref, double* I_out, double* 
     if (photon_rate) photon_rate[c] = total;
 }
 
-// Kahn depth of every task of a built schedule: entry i of the level-major
-// array is in level set k with lstart[k] <= i < lstart[k+1]. `compiled`
-// selects the header layout of the entries.
-__global__ void k_scatter_depth(Geo g, const uint4* tasks, const uint32_t* ls, int kmax, int compiled,
-                                uint32_t* depth) {
-    const uint32_t total = ls[kmax];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
-        int a = 0, b = kmax - 1;  // last k with ls[k] <= i
-        while (a < b) {
-            int m = (a + b + 1) >> 1;
-            if (ls[m] <= i) a = m;
-            else b = m - 1;
-        }
-        const uint4 r = tasks[i];
-        const uint32_t c = r.x & kCellMask;
-        const uint32_t d = compiled ? (r.x >> kCellBits) : r.z;
-        depth[(size_t)d * g.N + c] = (uint32_t)a;
-    }
-}
-
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -790,65 +867,6 @@ struct TimestepState {
 
 unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kBlock - 1) / kBlock); }
 
-// Host threads that submit the launches of direction groups 1..G-1 while the
-// calling thread submits group 0 (a kernel launch costs microseconds of host
-// time, so one thread cannot keep several streams fed).
-struct SubmitPool {
-    std::vector<std::thread> threads;
-    std::mutex mu;
-    std::condition_variable cv, cv_done;
-    std::function<void(int)> job;
-    uint64_t generation = 0;
-    int pending = 0;
-    bool quit = false;
-
-    void start(int workers) {
-        for (int i = 0; i < workers; i++)
-            threads.emplace_back([this, i] {
-                uint64_t seen = 0;
-                for (;;) {
-                    std::function<void(int)> j;
-                    {
-                        std::unique_lock<std::mutex> lk(mu);
-                        cv.wait(lk, [&] { return quit || generation != seen; });
-                        if (quit) return;
-                        seen = generation;
-                        j = job;
-                    }
-                    j(i + 1);
-                    {
-                        std::lock_guard<std::mutex> lk(mu);
-                        pending--;
-                    }
-                    cv_done.notify_one();
-                }
-            });
-    }
-    // Runs f(0) here and f(1..workers) on the pool; returns when all are done.
-    void run(const std::function<void(int)>& f) {
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            job = f;
-            pending = (int)threads.size();
-            generation++;
-        }
-        cv.notify_all();
-        f(0);
-        std::unique_lock<std::mutex> lk(mu);
-        cv_done.wait(lk, [&] { return pending == 0; });
-    }
-    void stop() {
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            quit = true;
-        }
-        cv.notify_all();
-        for (std::thread& th : threads) th.join();
-        threads.clear();
-        quit = false;
-    }
-};
-
 }  // namespace
 
 struct ssw_sweep {
@@ -859,6 +877,8 @@ struct ssw_sweep {
     hipStream_t stream = nullptr;
     bool fp32 = false;
     int mode_request = SSW_TRANSPORT_MODE_AUTO;
+    int rec_block = 64;                             // compiled replay block size
+    bool graphs_on = true;                          // capture replays as graphs
     bool compiled = false;
     // grid
     int64_t N = 0, F = 0, Nper = 0;
@@ -868,35 +888,34 @@ struct ssw_sweep {
     double4* d_geo = nullptr;
     double *d_dirs = nullptr, *d_vol = nullptr, *d_size = nullptr;
     double aref = 1.0;
-    int kmax = 0;                                   // deepest group
     double dw[3] = {0, 0, 0};
     int64_t ovf_total = 0;
-    // direction groups: each group's level sets run on its own stream
-    int G = 1;
-    int G_request = 0;                              // 0: library default
-    std::vector<uint32_t> gbase;                    // first task position of each group in a slot
-    std::vector<int> kmax_g;                        // depth bound of each group
-    uint8_t* d_dgroup = nullptr;
-    uint32_t* d_gbase = nullptr;
-    std::vector<hipStream_t> gstreams;
-    std::vector<hipEvent_t> gjoin;
-    hipEvent_t fork = nullptr;
-    SubmitPool pool;                                // G - 1 submitting threads
-    std::vector<std::vector<uint32_t>> full_start; // per group: level starts of the full graph
-    // schedule
+    // Kahn layering scratch (full graph at upload, restricted sets for the hook)
     uint4* d_flat = nullptr;
     uint32_t *d_lstart = nullptr, *d_ctr = nullptr, *d_cnt = nullptr, *d_need = nullptr;
-    uint4* d_hdr = nullptr;
+    int kcap = 0;
+    // full schedule
+    int kmax = 0;
+    std::vector<uint32_t> full_start;               // kmax + 1 level starts
+    uint32_t* d_fstart = nullptr;
+    uint4* d_recs = nullptr;                        // geometric
+    uint4* d_hdr = nullptr;                         // compiled
     void* d_A = nullptr;
     void* d_planes = nullptr;
     uint32_t* d_ovf_start = nullptr;
     void* d_ovf = nullptr;
-    int64_t compiled_bytes = 0;
-    int kcap = 0;
-    std::vector<int> replay_ok;                     // per slot: host copy of the level starts valid
-    std::vector<int> replays;                       // per slot: replays since its last build
-    std::vector<std::vector<uint32_t>> level_start; // per (slot, group): level starts of its last build
-    std::vector<hipGraphExec_t> graphs;             // per (slot, group): replay of its level sets
+    // compaction to active sets
+    uint32_t nchunks = 0;
+    uint4* d_chunks = nullptr;
+    uint32_t* d_first_chunk = nullptr;              // kmax: first chunk of each level set
+    uint32_t *d_ccount = nullptr, *d_coff = nullptr, *d_cstart = nullptr;
+    void* d_scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+    std::vector<uint32_t*> d_idx;                   // per sweep level >= 1: active positions
+    std::vector<std::vector<uint32_t>> cstart;      // per sweep level: level starts in idx
+    std::vector<int> list_ok;                       // per sweep level: idx / cstart current
+    std::vector<int> replays;                       // per sweep level: replays since the list was made
+    std::vector<hipGraphExec_t> graphs;             // per sweep level: captured replay
     // state
     double *d_rho = nullptr, *d_x = nullptr, *d_T = nullptr, *d_src = nullptr, *d_att = nullptr;
     double *d_sts = nullptr, *d_prev = nullptr, *d_change = nullptr, *d_src_t = nullptr;
@@ -957,10 +976,6 @@ Geo make_geo(ssw_sweep* s) {
     g.N = s->N;
     g.D = s->D;
     g.L = s->p.num_timestep_levels;
-    g.G = s->G;
-    g.Kcap = s->kcap;
-    g.dgroup = s->d_dgroup;
-    g.gbase = s->d_gbase;
     g.off = s->d_off;
     g.geo = s->d_geo;
     g.nk = s->d_nk;
@@ -972,7 +987,6 @@ Geo make_geo(ssw_sweep* s) {
     g.pfidx = s->d_pfidx;
     g.Nper = s->Nper;
     g.flat = s->d_flat;
-    g.flat_slot_stride = s->compiled ? 0 : (size_t)s->D * s->N;
     g.lstart = s->d_lstart;
     g.ctr = s->d_ctr;
     g.cnt = s->d_cnt;
@@ -994,227 +1008,262 @@ Flux<R> make_flux(ssw_sweep* s) {
     return f;
 }
 
-template <typename R>
-Comp<R> make_comp(ssw_sweep* s) {
-    Comp<R> c;
-    c.hdr = s->d_hdr;
-    c.A = (R*)s->d_A;
-    c.planes = (Entry<R>*)s->d_planes;
-    c.ovf_start = s->d_ovf_start;
-    c.ovf = (const Entry<R>*)s->d_ovf;
-    return c;
-}
-
 int set_need_all(ssw_sweep* s, uint32_t v) {
     k_fill_u32<<<1, kBlock, 0, s->stream>>>(s->d_need, v, s->p.num_timestep_levels);
     HIP_TRY(hipGetLastError());
     return SSW_OK;
 }
 
-void invalidate_replays(ssw_sweep* s) {
-    for (hipGraphExec_t& ge : s->graphs)
-        if (ge) {
-            (void)hipGraphExecDestroy(ge);
-            ge = nullptr;
-        }
-    std::fill(s->replay_ok.begin(), s->replay_ok.end(), 0);
-}
-
-hipStream_t gstream(ssw_sweep* s, int grp) { return s->G > 1 ? s->gstreams[grp] : s->stream; }
-
-// Blocks for building level set k of (slot, group): from the widths of its
-// last build, else of the full graph, with headroom; the kernel's grid-stride
-// loop covers wider level sets.
-unsigned build_blocks(ssw_sweep* s, int lvl, int grp, int k) {
-    const std::vector<uint32_t>& last = s->level_start[(size_t)lvl * s->G + grp];
-    const std::vector<uint32_t>& st = last.size() > (size_t)k + 1 ? last : s->full_start[grp];
-    int64_t w = 1024;
-    if (st.size() > (size_t)k + 1) w = (int64_t)(st[k + 1] - st[k]);
-    w = w + w / 4 + 64;
-    return (unsigned)std::min<int64_t>(16384, (w + kTasksPerBlock - 1) / kTasksPerBlock);
-}
-
-template <typename R>
-void launch_build(ssw_sweep* s, const Geo& g, int lvl, int grp, int k, bool transport, unsigned blocks,
-                  hipStream_t st) {
-    Flux<R> fl = make_flux<R>(s);
-    Comp<R> cp = make_comp<R>(s);
-    if (transport) {
-        if (s->compiled) k_sweep_build<R, true, true><<<blocks, kBlock, 0, st>>>(g, fl, cp, lvl, k, grp);
-        else k_sweep_build<R, true, false><<<blocks, kBlock, 0, st>>>(g, fl, cp, lvl, k, grp);
-    } else {
-        if (s->compiled) k_sweep_build<R, false, true><<<blocks, kBlock, 0, st>>>(g, fl, cp, lvl, k, grp);
-        else k_sweep_build<R, false, false><<<blocks, kBlock, 0, st>>>(g, fl, cp, lvl, k, grp);
+void drop_list(ssw_sweep* s, int lvl) {
+    if (lvl >= (int)s->list_ok.size()) return;
+    s->list_ok[lvl] = 0;
+    s->replays[lvl] = 0;
+    if (s->graphs[lvl]) {
+        (void)hipGraphExecDestroy(s->graphs[lvl]);
+        s->graphs[lvl] = nullptr;
     }
 }
 
-void launch_build_any(ssw_sweep* s, const Geo& g, int lvl, int grp, int k, bool transport, unsigned blocks,
-                      hipStream_t st) {
-    if (s->fp32) launch_build<float>(s, g, lvl, grp, k, transport, blocks, st);
-    else launch_build<double>(s, g, lvl, grp, k, transport, blocks, st);
+void drop_all_lists(ssw_sweep* s) {
+    for (int l = 0; l < (int)s->list_ok.size(); l++) drop_list(s, l);
 }
 
-// Builds the schedule of the full graph (all cells active) once per grid, with
-// no compiled output: the depth bound of every group and its level widths; a
-// graph that never empties its queue has a cycle. The level starts are sized
-// by a guess and the build restarts with more room if a group is deeper.
-int build_full_schedule(ssw_sweep* s) {
-    const int L = s->p.num_timestep_levels, G = s->G;
-    const bool was_compiled = s->compiled;
-    for (int kcap = 1024;; kcap *= 4) {
-        s->kcap = kcap;
-        TRY(dalloc(s, &s->d_lstart, (size_t)L * G * (kcap + 2)));
-        TRY(set_need_all(s, 1u));
-        s->compiled = false;  // records only
-        Geo g = make_geo(s);
-        g.flat_slot_stride = 0;
-        Flux<double> fl{};
-        Comp<double> cp{};
-        k_begin<double><<<1, kBlock, 0, s->stream>>>(g, fl, 0);
-        k_count<<<grid_for((int64_t)s->N * s->D), kBlock, 0, s->stream>>>(g, 0, 0);
+// Kahn layering of the active set {level >= lvl} into the scratch records:
+// counting, then one launch per level set until the queue is empty, reading
+// each level's width back (the depth and widths are not known in advance).
+// `expected` active tasks must all be placed, else the graph has a cycle.
+int kahn_layering(ssw_sweep* s, int lvl, int64_t expected, int* depth_out, std::vector<uint32_t>* starts) {
+    Geo g = make_geo(s);
+    k_kahn_reset<<<1, 64, 0, s->stream>>>(g);
+    k_count<<<grid_for((int64_t)s->N * s->D), kBlock, 0, s->stream>>>(g, lvl, 0);
+    HIP_TRY(hipGetLastError());
+    if (starts) starts->assign(1, 0u);
+    int64_t total = 0;
+    uint32_t c3[3];
+    for (int k = 0;; k++) {
+        HIP_TRY(hipMemcpyAsync(c3, s->d_ctr, sizeof(c3), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        const uint32_t width = c3[k % 3];
+        if (width == 0) {
+            if (total != expected)
+                return fail(SSW_ERR_CYCLE, "dependency cycle: " + std::to_string(expected - total) +
+                                               " cell-direction pairs never become ready");
+            if (k == 0) {  // no active task: one empty level set
+                k_kahn<<<1, kBlock, 0, s->stream>>>(g, 0);
+                HIP_TRY(hipGetLastError());
+                if (starts) starts->push_back(0u);
+            }
+            *depth_out = std::max(1, k);
+            return SSW_OK;
+        }
+        if (k + 2 > s->kcap) return fail(SSW_ERR_CYCLE, "level-set depth exceeds its bound");
+        total += width;
+        if (starts) starts->push_back((uint32_t)total);
+        const unsigned blocks = (unsigned)std::min<int64_t>(16384, (width + kTasksPerBlock - 1) / kTasksPerBlock);
+        k_kahn<<<blocks, kBlock, 0, s->stream>>>(g, k);
         HIP_TRY(hipGetLastError());
-        s->full_start.assign(G, std::vector<uint32_t>(1, 0u));
-        std::vector<int64_t> total(G, 0);
-        std::vector<int> depth(G, -1);
-        std::vector<uint32_t> c3(3 * G);
-        bool grow = false;
-        int rc = SSW_OK;
-        for (int k = 0;; k++) {
-            HIP_TRY(hipMemcpyAsync(c3.data(), s->d_ctr, 3 * G * 4, hipMemcpyDeviceToHost, s->stream));
-            HIP_TRY(hipStreamSynchronize(s->stream));
-            bool any = false;
-            for (int q = 0; q < G; q++) {
-                if (depth[q] >= 0) continue;
-                const uint32_t width = c3[3 * q + k % 3];
-                if (width == 0) {
-                    depth[q] = k;
-                    continue;
-                }
-                any = true;
-                total[q] += width;
-                s->full_start[q].push_back((uint32_t)total[q]);
-            }
-            if (!any) break;
-            if (k + 2 > kcap) {
-                grow = true;
-                break;
-            }
-            if (k > s->N + 1) {
-                rc = fail(SSW_ERR_CYCLE, "dependency graph deeper than the number of cells");
-                break;
-            }
-            for (int q = 0; q < G; q++) {
-                if (depth[q] >= 0) continue;
-                const int64_t width = (int64_t)(s->full_start[q][k + 1] - s->full_start[q][k]);
-                const unsigned blocks = (unsigned)std::min<int64_t>(16384, (width + kTasksPerBlock - 1) / kTasksPerBlock);
-                k_sweep_build<double, false, false><<<blocks, kBlock, 0, s->stream>>>(g, fl, cp, 0, k, q);
-            }
-            HIP_TRY(hipGetLastError());
-        }
-        s->compiled = was_compiled;
-        if (rc != SSW_OK) return rc;
-        if (grow) continue;
-        int64_t all = 0;
-        for (int q = 0; q < G; q++) all += total[q];
-        if (all != s->N * (int64_t)s->D)
-            return fail(SSW_ERR_CYCLE, "dependency cycle: " + std::to_string(s->N * (int64_t)s->D - all) +
-                                           " cell-direction pairs never become ready");
-        s->kmax_g.assign(G, 0);
-        s->kmax = 1;
-        for (int q = 0; q < G; q++) {
-            s->kmax_g[q] = std::max(1, depth[q]);
-            s->kmax = std::max(s->kmax, s->kmax_g[q]);
-        }
-        break;
     }
-    s->kcap = s->kmax;
-    TRY(dalloc(s, &s->d_lstart, (size_t)L * G * (s->kcap + 2)));
-    s->replay_ok.assign(L, 0);
+}
+
+// The full schedule, once per grid: Kahn layering of the whole graph, each
+// task's depth, a stable sort by depth of the tasks in (direction, cell)
+// order, then the compiled or geometric per-task data in that order, and the
+// chunk table of the compaction.
+int build_schedule(ssw_sweep* s) {
+    const size_t DN = (size_t)s->D * s->N;
+    s->kcap = (int)std::min<int64_t>(s->N + 2, 1 << 22);
+    TRY(dalloc(s, &s->d_lstart, (size_t)s->kcap + 2));
+    int depth = 0;
+    TRY(kahn_layering(s, 0, (int64_t)DN, &depth, &s->full_start));
+    s->kmax = depth;
+    // sort the tasks by depth, stable over t = d*N + c
+    uint32_t *d_depth = nullptr, *d_keys = nullptr, *d_vals = nullptr, *d_sorted = nullptr;
+    TRY(dalloc(s, &d_depth, DN));
+    TRY(dalloc(s, &d_keys, DN));
+    TRY(dalloc(s, &d_vals, DN));
+    TRY(dalloc(s, &d_sorted, DN));
+    Geo g = make_geo(s);
+    k_scatter_depth<<<grid_for(std::min<int64_t>((int64_t)DN, 1 << 22)), kBlock, 0, s->stream>>>(g, s->kmax,
+                                                                                                  d_depth);
+    k_iota<<<grid_for((int64_t)DN), kBlock, 0, s->stream>>>(d_vals, (uint32_t)DN);
+    HIP_TRY(hipGetLastError());
+    int bits = 1;
+    while ((1ll << bits) <= s->kmax) bits++;
+    size_t tmp_bytes = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, d_depth, d_keys, d_vals, d_sorted, (int)DN, 0,
+                                               bits, s->stream));
+    uint8_t* d_tmp = nullptr;
+    TRY(dalloc(s, &d_tmp, tmp_bytes));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp_bytes, d_depth, d_keys, d_vals, d_sorted, (int)DN, 0, bits,
+                                               s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    dfree(s, d_tmp);
+    dfree(s, d_keys);
+    dfree(s, d_vals);
+    dfree(s, d_depth);
+    TRY(dalloc(s, &s->d_fstart, s->full_start.size()));
+    HIP_TRY(hipMemcpyAsync(s->d_fstart, s->full_start.data(), s->full_start.size() * 4, hipMemcpyHostToDevice,
+                           s->stream));
+    const double inv_aref = 1.0 / s->aref;
+    if (s->compiled) {
+        TRY(dalloc(s, &s->d_hdr, DN));
+        TRY(dalloc(s, (uint8_t**)&s->d_A, DN * real_size(s)));
+        TRY(dalloc(s, (uint8_t**)&s->d_planes, (size_t)kPlanes * DN * entry_size(s)));
+        if (s->fp32)
+            k_schedule<float><<<grid_for((int64_t)DN), kBlock, 0, s->stream>>>(
+                g, inv_aref, d_sorted, s->d_fstart, s->kmax, 1, s->d_ovf_start, s->d_hdr, (float*)s->d_A,
+                (Entry<float>*)s->d_planes, nullptr);
+        else
+            k_schedule<double><<<grid_for((int64_t)DN), kBlock, 0, s->stream>>>(
+                g, inv_aref, d_sorted, s->d_fstart, s->kmax, 1, s->d_ovf_start, s->d_hdr, (double*)s->d_A,
+                (Entry<double>*)s->d_planes, nullptr);
+    } else {
+        TRY(dalloc(s, &s->d_recs, DN));
+        k_schedule<double><<<grid_for((int64_t)DN), kBlock, 0, s->stream>>>(
+            g, inv_aref, d_sorted, s->d_fstart, s->kmax, 0, nullptr, nullptr, nullptr, nullptr, s->d_recs);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    dfree(s, d_sorted);
+    // chunk table: chunks of at most kChunk tasks inside one level set
+    std::vector<uint4> chunks;
+    std::vector<uint32_t> first(s->kmax);
+    for (int k = 0; k < s->kmax; k++) {
+        first[k] = (uint32_t)chunks.size();
+        const uint32_t b = s->full_start[k], e = s->full_start[k + 1];
+        if (b == e) chunks.push_back(make_uint4((uint32_t)k, b, e, 0u));
+        for (uint32_t p = b; p < e; p += kChunk)
+            chunks.push_back(make_uint4((uint32_t)k, p, std::min(e, p + (uint32_t)kChunk), 0u));
+    }
+    s->nchunks = (uint32_t)chunks.size();
+    TRY(dalloc(s, &s->d_chunks, chunks.size()));
+    TRY(dalloc(s, &s->d_first_chunk, first.size()));
+    TRY(dalloc(s, &s->d_ccount, chunks.size()));
+    TRY(dalloc(s, &s->d_coff, chunks.size()));
+    TRY(dalloc(s, &s->d_cstart, (size_t)s->kmax + 1));
+    HIP_TRY(hipMemcpyAsync(s->d_chunks, chunks.data(), chunks.size() * sizeof(uint4), hipMemcpyHostToDevice,
+                           s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_first_chunk, first.data(), first.size() * 4, hipMemcpyHostToDevice, s->stream));
+    s->scan_tmp_bytes = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, s->scan_tmp_bytes, s->d_ccount, s->d_coff, (int)s->nchunks,
+                                             s->stream));
+    TRY(dalloc(s, (uint8_t**)&s->d_scan_tmp, s->scan_tmp_bytes));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const int L = s->p.num_timestep_levels;
+    for (uint32_t*& p : s->d_idx) dfree(s, p);
+    s->d_idx.assign(L, nullptr);
+    s->cstart.assign(L, std::vector<uint32_t>());
+    s->list_ok.assign(L, 0);
     s->replays.assign(L, 0);
-    s->level_start.assign((size_t)L * G, std::vector<uint32_t>());
     for (hipGraphExec_t ge : s->graphs)
         if (ge) (void)hipGraphExecDestroy(ge);
-    s->graphs.assign((size_t)L * G, nullptr);
+    s->graphs.assign(L, nullptr);
     return SSW_OK;
 }
 
-// Launches the replay of (slot, group)'s level sets on stream `st`, one
-// exact-size launch per non-empty level set; with `ev` each launch is
+const uint4* schedule_tasks(ssw_sweep* s) { return s->compiled ? s->d_hdr : s->d_recs; }
+
+// The active-set list of sweep level lvl >= 1: the positions of the full
+// schedule whose cell has level >= lvl, per level set, and its level starts
+// on the host. Sweep level 0 covers every task and needs no list.
+int make_list(ssw_sweep* s, int lvl) {
+    const size_t DN = (size_t)s->D * s->N;
+    if (!s->d_idx[lvl]) TRY(dalloc(s, &s->d_idx[lvl], DN));
+    Geo g = make_geo(s);
+    const uint4* tasks = schedule_tasks(s);
+    k_compact_count<<<s->nchunks, kBlock, 0, s->stream>>>(g, tasks, s->d_chunks, s->nchunks, lvl, s->d_ccount);
+    HIP_TRY(hipGetLastError());
+    size_t tb = s->scan_tmp_bytes;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(s->d_scan_tmp, tb, s->d_ccount, s->d_coff, (int)s->nchunks, s->stream));
+    k_compact_write<<<s->nchunks, kBlock, 0, s->stream>>>(g, tasks, s->d_chunks, s->nchunks, s->d_fstart, lvl,
+                                                          s->d_coff, s->d_idx[lvl]);
+    HIP_TRY(hipGetLastError());
+    // total = offset + count of the last chunk
+    uint32_t last[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(&last[0], s->d_coff + s->nchunks - 1, 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(&last[1], s->d_ccount + s->nchunks - 1, 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    k_compact_starts<<<grid_for((int64_t)s->kmax + 1), kBlock, 0, s->stream>>>(s->d_first_chunk, s->d_coff, s->kmax,
+                                                                               last[0] + last[1], s->d_cstart);
+    HIP_TRY(hipGetLastError());
+    std::vector<uint32_t>& cs = s->cstart[lvl];
+    cs.resize(s->kmax + 1);
+    HIP_TRY(hipMemcpyAsync(cs.data(), s->d_cstart, (s->kmax + 1) * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->list_ok[lvl] = 1;
+    s->replays[lvl] = 0;
+    if (s->graphs[lvl]) {
+        (void)hipGraphExecDestroy(s->graphs[lvl]);
+        s->graphs[lvl] = nullptr;
+    }
+    return SSW_OK;
+}
+
+// Launches the replay of sweep level lvl's level sets on the handle's stream,
+// one exact-size launch per non-empty level set; with `ev` each launch is
 // bracketed by events.
 template <typename R>
-void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, int grp, hipStream_t st,
-                     std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev,
-                     const std::function<hipEvent_t(hipStream_t)>* mark) {
+void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev,
+                     const std::function<hipEvent_t()>* mark) {
     Flux<R> fl = make_flux<R>(s);
-    Comp<R> cp = make_comp<R>(s);
-    const std::vector<uint32_t>& ls = s->level_start[(size_t)lvl * s->G + grp];
-    const size_t DN = (size_t)s->D * s->N;
-    const uint32_t gb = s->gbase[grp];
-    for (int k = 0; k < s->kmax_g[grp]; k++) {
-        const uint32_t beg = gb + ls[k], count = ls[k + 1] - ls[k];
+    const std::vector<uint32_t>& fs = s->full_start;
+    const bool all = lvl == 0;
+    const std::vector<uint32_t>& cs = all ? fs : s->cstart[lvl];
+    const uint32_t* idx = all ? nullptr : s->d_idx[lvl];
+    for (int k = 0; k < s->kmax; k++) {
+        const uint32_t count = cs[k + 1] - cs[k];
         if (count == 0) continue;
-        hipEvent_t a = ev ? (*mark)(st) : nullptr;
+        const uint32_t b = fs[k], W = fs[k + 1] - fs[k];
+        const uint32_t* ik = idx ? idx + cs[k] : nullptr;
+        hipEvent_t a = ev ? (*mark)() : nullptr;
         if (s->compiled) {
-            const size_t so = (size_t)lvl * DN + beg;
-            k_sweep_comp<R><<<(count + kRecBlock - 1) / kRecBlock, kRecBlock, 0, st>>>(
-                g, fl, cp.hdr + so, cp.A + so, cp.planes + (size_t)lvl * kPlanes * DN + (size_t)kPlanes * beg, cp.ovf,
-                count);
+            const uint4* hdr = s->d_hdr + b;
+            const R* A = (const R*)s->d_A + b;
+            const Entry<R>* planes = (const Entry<R>*)s->d_planes + (size_t)kPlanes * b;
+            const Entry<R>* ovf = (const Entry<R>*)s->d_ovf;
+            switch (s->rec_block) {
+                case 128:
+                    k_sweep_comp<R, 128><<<(count + 127) / 128, 128, 0, s->stream>>>(g, fl, hdr, A, planes, ovf, W,
+                                                                                   ik, count);
+                    break;
+                case 256:
+                    k_sweep_comp<R, 256><<<(count + 255) / 256, 256, 0, s->stream>>>(g, fl, hdr, A, planes, ovf, W,
+                                                                                   ik, count);
+                    break;
+                default:
+                    k_sweep_comp<R, 64><<<(count + 63) / 64, 64, 0, s->stream>>>(g, fl, hdr, A, planes, ovf, W, ik,
+                                                                               count);
+            }
         } else {
-            const uint4* recs = s->d_flat + (size_t)lvl * DN + beg;
-            k_sweep_rec<R><<<(count + kTasksPerBlock - 1) / kTasksPerBlock, kBlock, 0, st>>>(g, fl, recs, count);
+            k_sweep_rec<R><<<(count + kTasksPerBlock - 1) / kTasksPerBlock, kBlock, 0, s->stream>>>(
+                g, fl, s->d_recs + b, ik, count);
         }
-        if (ev) ev->push_back({a, (*mark)(st)});
+        if (ev) ev->push_back({a, (*mark)()});
     }
 }
 
-void launch_replay(ssw_sweep* s, const Geo& g, int lvl, int grp, hipStream_t st,
-                   std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev,
-                   const std::function<hipEvent_t(hipStream_t)>* mark = nullptr) {
-    if (s->fp32) launch_replay_t<float>(s, g, lvl, grp, st, ev, mark);
-    else launch_replay_t<double>(s, g, lvl, grp, st, ev, mark);
+void launch_replay(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev,
+                   const std::function<hipEvent_t()>* mark = nullptr) {
+    if (s->fp32) launch_replay_t<float>(s, g, lvl, ev, mark);
+    else launch_replay_t<double>(s, g, lvl, ev, mark);
 }
 
-// After a schedule is built: copies its level starts to the host. Its replay
-// graphs are captured on its second replay (a schedule that is rebuilt after
-// one use never pays for a capture).
-int build_replay_data(ssw_sweep* s, const Geo& g, int lvl) {
-    (void)g;
-    const int G = s->G;
-    std::vector<uint32_t> all((size_t)G * (s->kcap + 2));
-    HIP_TRY(hipMemcpyAsync(all.data(), s->d_lstart + (size_t)lvl * G * (s->kcap + 2), all.size() * 4,
-                           hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    for (int q = 0; q < G; q++) {
-        std::vector<uint32_t>& ls = s->level_start[(size_t)lvl * G + q];
-        ls.assign(all.begin() + (size_t)q * (s->kcap + 2), all.begin() + (size_t)q * (s->kcap + 2) + s->kmax_g[q] + 1);
-        hipGraphExec_t& slot = s->graphs[(size_t)lvl * G + q];
-        if (slot) {
-            (void)hipGraphExecDestroy(slot);
-            slot = nullptr;
-        }
-    }
-    s->replay_ok[lvl] = 1;
-    s->replays[lvl] = 0;
-    return SSW_OK;
-}
-
-// Captures the replay of (slot, group)'s level sets as one graph on the
-// group's stream (nothing is executed). A failed capture leaves no graph and
-// the level sets are launched one by one (counted in ssw_info).
-void capture_group(ssw_sweep* s, const Geo& g, int lvl, int q, int64_t* failures) {
-    const hipStream_t st = gstream(s, q);
+// Captures the replay of sweep level lvl as one graph (nothing is executed). A
+// failed capture leaves no graph; the level sets are then launched one by one
+// (counted in ssw_info).
+void capture(ssw_sweep* s, const Geo& g, int lvl) {
     hipGraph_t graph = nullptr;
-    if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
         (void)hipGetLastError();
-        (*failures)++;
+        s->capture_failures++;
         return;
     }
-    launch_replay(s, g, lvl, q, st, nullptr);
-    hipError_t ce = hipStreamEndCapture(st, &graph);
+    launch_replay(s, g, lvl, nullptr);
+    hipError_t ce = hipStreamEndCapture(s->stream, &graph);
     if (ce != hipSuccess || !graph) {
         if (graph) (void)hipGraphDestroy(graph);
         (void)hipGetLastError();
-        (*failures)++;
+        s->capture_failures++;
         return;
     }
     hipGraphExec_t ge = nullptr;
@@ -1222,22 +1271,10 @@ void capture_group(ssw_sweep* s, const Geo& g, int lvl, int q, int64_t* failures
     (void)hipGraphDestroy(graph);
     if (ie != hipSuccess) {
         (void)hipGetLastError();
-        (*failures)++;
+        s->capture_failures++;
         return;
     }
-    s->graphs[(size_t)lvl * s->G + q] = ge;
-}
-
-void drop_replay(ssw_sweep* s, int lvl) {
-    s->replay_ok[lvl] = 0;
-    s->replays[lvl] = 0;
-    for (int q = 0; q < s->G; q++) {
-        hipGraphExec_t& slot = s->graphs[(size_t)lvl * s->G + q];
-        if (slot) {
-            (void)hipGraphExecDestroy(slot);
-            slot = nullptr;
-        }
-    }
+    s->graphs[lvl] = ge;
 }
 
 template <typename R>
@@ -1267,86 +1304,53 @@ int run_step(ssw_sweep* s, double* elapsed) {
     const int L = s->p.num_timestep_levels;
     k_fill_u32<<<1, 2, 0, s->stream>>>((uint32_t*)s->d_active, 0u, 2);
     const unsigned gper = grid_for(std::max<int64_t>(s->Nper, 1) * s->D);
-    const unsigned gcount = grid_for((int64_t)s->N * s->D);
     const unsigned gcells = grid_for(s->N);
     s->sweeps_last = 0;
     s->builds_last = 0;
     s->t_transport_total = s->t_chem_total = s->t_build_total = 0;
     s->n_transport = 0;
     size_t evn = 0;
-    std::function<hipEvent_t(hipStream_t)> mark = [&](hipStream_t st) -> hipEvent_t {
+    std::function<hipEvent_t()> mark = [&](void) -> hipEvent_t {
         if (evn >= s->ev.size()) {
             hipEvent_t e;
             (void)hipEventCreate(&e);
             s->ev.push_back(e);
         }
         hipEvent_t e = s->ev[evn++];
-        (void)hipEventRecord(e, st);
+        (void)hipEventRecord(e, s->stream);
         return e;
     };
     std::vector<std::pair<hipEvent_t, hipEvent_t>> tr, ch, bu;
-    // Which schedules are (re)built in this step (device flags set by the level
-    // update of the previous step).
+    // sweep levels whose active set changed (device flags set by the level
+    // update of the previous step)
     std::vector<uint32_t> need(L, 1u);
-    HIP_TRY(hipMemcpyAsync(need.data(), s->d_need, L * 4, hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    for (int l = 0; l < L; l++)
-        if (need[l] && s->replay_ok[l]) drop_replay(s, l);
-    const int G = s->G;
+    if (L > 1) {
+        HIP_TRY(hipMemcpyAsync(need.data(), s->d_need, L * 4, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
+    for (int l = 1; l < L; l++)
+        if (need[l]) drop_list(s, l);
     for (int lvl : order) {
         k_begin<R><<<gper, kBlock, 0, s->stream>>>(g, fl, lvl);
-        if (!need[lvl] && !s->replay_ok[lvl]) TRY(build_replay_data(s, g, lvl));  // built by a parity hook
-        const bool rebuild = need[lvl] != 0;
-        hipEvent_t b0 = (s->timing && rebuild) ? mark(s->stream) : nullptr;
-        if (rebuild) k_count<<<gcount, kBlock, 0, s->stream>>>(g, lvl, 0);
-        if (G > 1) HIP_TRY(hipEventRecord(s->fork, s->stream));
-        const bool capture = !rebuild && !s->timing && s->replays[lvl] >= 1 && g_live_handles.load() == 1;
-        std::vector<int64_t> cap_fail(G, 0);
-        std::vector<hipError_t> herr(G, hipSuccess);
-        // group q's launches on its stream: replay or build-while-solving
-        auto submit = [&](int q) {
-            const hipStream_t st = gstream(s, q);
-            hipError_t e = hipSuccess;
-            if (G > 1) e = hipStreamWaitEvent(st, s->fork, 0);
-            if (!rebuild) {
-                hipGraphExec_t& ge = s->graphs[(size_t)lvl * G + q];
-                if (s->timing) {
-                    launch_replay(s, g, lvl, q, st, &tr, &mark);
-                } else {
-                    if (!ge && capture) capture_group(s, g, lvl, q, &cap_fail[q]);
-                    if (ge) e = e != hipSuccess ? e : hipGraphLaunch(ge, st);
-                    else launch_replay(s, g, lvl, q, st, nullptr);
-                }
-            } else {
-                for (int k = 0; k < s->kmax_g[q]; k++)
-                    launch_build<R>(s, g, lvl, q, k, true, build_blocks(s, lvl, q, k), st);
-            }
-            if (e == hipSuccess) e = hipGetLastError();  // launch errors are per host thread
-            if (G > 1 && e == hipSuccess) e = hipEventRecord(s->gjoin[q], st);
-            herr[q] = e;
-        };
-        if (G > 1 && !s->timing && !s->pool.threads.empty()) {
-            s->pool.run(submit);
-        } else {
-            for (int q = 0; q < G; q++) submit(q);
+        if (lvl > 0 && !s->list_ok[lvl]) {
+            hipEvent_t b0 = s->timing ? mark() : nullptr;
+            TRY(make_list(s, lvl));
+            if (s->timing) bu.push_back({b0, mark()});
+            s->builds_last++;
         }
-        for (int q = 0; q < G; q++) {
-            s->capture_failures += cap_fail[q];
-            HIP_TRY(herr[q]);
-            if (G > 1) HIP_TRY(hipStreamWaitEvent(s->stream, s->gjoin[q], 0));
+        if (s->timing) {
+            launch_replay(s, g, lvl, &tr, &mark);
+        } else {
+            if (s->graphs_on && !s->graphs[lvl] && s->replays[lvl] >= 1 && g_live_handles.load() == 1)
+                capture(s, g, lvl);
+            if (s->graphs[lvl]) HIP_TRY(hipGraphLaunch(s->graphs[lvl], s->stream));
+            else launch_replay(s, g, lvl, nullptr);
         }
         HIP_TRY(hipGetLastError());
-        if (rebuild) {
-            if (s->timing) bu.push_back({b0, mark(s->stream)});
-            TRY(build_replay_data(s, g, lvl));
-            need[lvl] = 0;
-            s->builds_last++;
-        } else {
-            s->replays[lvl]++;
-        }
-        hipEvent_t a = s->timing ? mark(s->stream) : nullptr;
+        s->replays[lvl]++;
+        hipEvent_t a = s->timing ? mark() : nullptr;
         k_chemistry<R><<<gcells, kBlock, 0, s->stream>>>(g, fl, ca, lvl);
-        if (s->timing) ch.push_back({a, mark(s->stream)});
+        if (s->timing) ch.push_back({a, mark()});
         s->sweeps_last++;
     }
     HIP_TRY(hipGetLastError());
@@ -1394,7 +1398,8 @@ void free_state(ssw_sweep* s) {
 }
 
 void free_grid(ssw_sweep* s) {
-    invalidate_replays(s);
+    drop_all_lists(s);
+    for (uint32_t*& p : s->d_idx) dfree(s, p);
     dfree(s, s->d_off);
     dfree(s, s->d_geo);
     dfree(s, s->d_nk);
@@ -1409,22 +1414,24 @@ void free_grid(ssw_sweep* s) {
     dfree(s, s->d_ctr);
     dfree(s, s->d_cnt);
     dfree(s, s->d_level);
+    dfree(s, s->d_fstart);
+    dfree(s, s->d_recs);
     dfree(s, s->d_hdr);
     dfree(s, s->d_A);
     dfree(s, s->d_planes);
     dfree(s, s->d_ovf_start);
     dfree(s, s->d_ovf);
-    dfree(s, s->d_dgroup);
-    dfree(s, s->d_gbase);
+    dfree(s, s->d_chunks);
+    dfree(s, s->d_first_chunk);
+    dfree(s, s->d_ccount);
+    dfree(s, s->d_coff);
+    dfree(s, s->d_cstart);
+    dfree(s, s->d_scan_tmp);
 }
 
-// Allocates the compiled schedule and the overflow entries (once per grid).
-int setup_compiled(ssw_sweep* s, uint32_t* d_ovf_count) {
+// Overflow entries of the compiled schedule (once per grid).
+int setup_overflow(ssw_sweep* s, uint32_t* d_ovf_count) {
     const size_t DN = (size_t)s->D * s->N;
-    const int L = s->p.num_timestep_levels;
-    TRY(dalloc(s, &s->d_hdr, (size_t)L * DN));
-    TRY(dalloc(s, (uint8_t**)&s->d_A, (size_t)L * DN * real_size(s)));
-    TRY(dalloc(s, (uint8_t**)&s->d_planes, (size_t)L * kPlanes * DN * entry_size(s)));
     TRY(dalloc(s, &s->d_ovf_start, DN));
     TRY(dalloc(s, (uint8_t**)&s->d_ovf, (size_t)std::max<int64_t>(1, s->ovf_total) * entry_size(s)));
     uint32_t* d_cursor = nullptr;
@@ -1442,8 +1449,6 @@ int setup_compiled(ssw_sweep* s, uint32_t* d_ovf_count) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->stream));
     dfree(s, d_cursor);
-    s->compiled_bytes = (int64_t)((size_t)L * DN * (16 + real_size(s) + kPlanes * entry_size(s)) + DN * 4 +
-                                  (size_t)std::max<int64_t>(1, s->ovf_total) * entry_size(s));
     return SSW_OK;
 }
 
@@ -1541,7 +1546,6 @@ int ssw_create(const ssw_sweep_parameters* params, ssw_sweep** out) {
 void ssw_destroy(ssw_sweep* s) {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    s->pool.stop();
     free_state(s);
     free_grid(s);
     dfree(s, s->d_dirs);
@@ -1549,9 +1553,6 @@ void ssw_destroy(ssw_sweep* s) {
     dfree(s, s->d_fail);
     dfree(s, s->d_active);
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
-    for (hipEvent_t e : s->gjoin) (void)hipEventDestroy(e);
-    if (s->fork) (void)hipEventDestroy(s->fork);
-    for (hipStream_t st : s->gstreams) (void)hipStreamDestroy(st);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     g_live_handles--;
@@ -1567,12 +1568,39 @@ int ssw_set_transport_mode(ssw_sweep* s, int32_t mode) {
     return SSW_OK;
 }
 
+int ssw_set_replay_block(ssw_sweep* s, int32_t block) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    if (block != 0 && block != 64 && block != 128 && block != 256)
+        return fail(SSW_ERR_INVALID, "replay block size must be 0, 64, 128 or 256");
+    s->rec_block = block ? block : 64;
+    for (hipGraphExec_t& ge : s->graphs)  // graphs hold the old launch shape
+        if (ge) {
+            (void)hipGraphExecDestroy(ge);
+            ge = nullptr;
+        }
+    std::fill(s->replays.begin(), s->replays.end(), 0);
+    return SSW_OK;
+}
+
+int ssw_set_graph_replay(ssw_sweep* s, int32_t enabled) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    s->graphs_on = enabled != 0;
+    if (!s->graphs_on)
+        for (hipGraphExec_t& ge : s->graphs)
+            if (ge) {
+                (void)hipGraphExecDestroy(ge);
+                ge = nullptr;
+            }
+    return SSW_OK;
+}
+
 int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* nbr, const uint8_t* kind,
                     const double* area, const double* normal, const double* volume, const double* size) {
     if (!s || !off || !nbr || !kind || !area || !normal || !volume || !size)
         return fail(SSW_ERR_INVALID, "null argument");
     if (n <= 0 || n > (int64_t)kCellMask) return fail(SSW_ERR_INVALID, "cell count must be in [1, 2^25 - 1]");
-    if ((int64_t)s->D * n >= (int64_t)0xFFFFFFFFll) return fail(SSW_ERR_INVALID, "cells x directions exceeds 2^32");
+    if ((int64_t)s->D * n >= (int64_t)0x7FFFFFFFll)
+        return fail(SSW_ERR_INVALID, "cells x directions must be below 2^31");
     if (off[0] != 0) return fail(SSW_ERR_INVALID, "face_offsets[0] must be 0");
     for (int64_t c = 0; c < n; c++) {
         if (off[c + 1] < off[c]) return fail(SSW_ERR_INVALID, "face_offsets must be non-decreasing");
@@ -1632,34 +1660,7 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     TRY(dalloc(s, &s->d_pfoff, pfoff.size()));
     TRY(dalloc(s, &s->d_pfidx, std::max<size_t>(1, pfidx.size())));
     TRY(dalloc(s, &s->d_level, n));
-    // direction groups: contiguous blocks of directions; each group's level
-    // sets run on a stream of their own
-    s->G = s->G_request > 0 ? std::min(s->G_request, s->D) : std::min(kDefaultGroups, s->D);
-    std::vector<uint8_t> dgroup(s->D);
-    s->gbase.assign(s->G, 0u);
-    for (int q = 0; q < s->G; q++) {
-        const int d0 = q * s->D / s->G, d1 = (q + 1) * s->D / s->G;
-        s->gbase[q] = (uint32_t)((int64_t)d0 * n);
-        for (int d = d0; d < d1; d++) dgroup[d] = (uint8_t)q;
-    }
-    while ((int)s->gstreams.size() < s->G) {
-        hipStream_t st = nullptr;
-        HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        s->gstreams.push_back(st);
-        hipEvent_t e = nullptr;
-        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        s->gjoin.push_back(e);
-    }
-    if (!s->fork) HIP_TRY(hipEventCreateWithFlags(&s->fork, hipEventDisableTiming));
-    if ((int)s->pool.threads.size() != s->G - 1) {
-        s->pool.stop();
-        if (s->G > 1) s->pool.start(s->G - 1);
-    }
-    TRY(dalloc(s, &s->d_dgroup, (size_t)s->D));
-    TRY(dalloc(s, &s->d_gbase, (size_t)s->G));
-    HIP_TRY(hipMemcpyAsync(s->d_dgroup, dgroup.data(), s->D, hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipMemcpyAsync(s->d_gbase, s->gbase.data(), s->G * 4, hipMemcpyHostToDevice, s->stream));
-    TRY(dalloc(s, &s->d_ctr, (size_t)L * s->G * 3));
+    TRY(dalloc(s, &s->d_ctr, 3));
     TRY(dalloc(s, &s->d_cnt, DN));
     HIP_TRY(hipMemcpyAsync(s->d_off, off32.data(), (n + 1) * 4, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_geo, geo.data(), F * sizeof(double4), hipMemcpyHostToDevice, s->stream));
@@ -1674,8 +1675,7 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     HIP_TRY(hipMemcpyAsync(s->d_pfoff, pfoff.data(), pfoff.size() * 4, hipMemcpyHostToDevice, s->stream));
     std::vector<uint8_t> lev(n, (uint8_t)(L - 1));
     HIP_TRY(hipMemcpyAsync(s->d_level, lev.data(), n, hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipMemsetAsync(s->d_ctr, 0, (size_t)L * s->G * 3 * 4, s->stream));
-    // mirror check, downwind statistics and overflow counts
+    // mirror check, downwind statistics and overflow counts (into cnt)
     unsigned long long* d_stats = nullptr;
     TRY(dalloc(s, &d_stats, 5));
     HIP_TRY(hipMemsetAsync(d_stats, 0, 5 * 8, s->stream));
@@ -1694,20 +1694,16 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     s->ovf_total = (int64_t)hb[4];
     // transport mode: compiled when its arrays fit (automatic: within half of
     // the free device memory), else geometric
-    const size_t comp_bytes = (size_t)L * DN * (16 + real_size(s) + kPlanes * entry_size(s)) + DN * (16 + 4) +
-                              (size_t)s->ovf_total * entry_size(s);
-    const size_t geom_bytes = (size_t)L * DN * 16;
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    const size_t comp_bytes =
+        DN * (16 + real_size(s) + kPlanes * entry_size(s) + 4) + (size_t)s->ovf_total * entry_size(s);
     if (s->mode_request == SSW_TRANSPORT_MODE_COMPILED) s->compiled = true;
     else if (s->mode_request == SSW_TRANSPORT_MODE_GEOMETRIC) s->compiled = false;
     else s->compiled = comp_bytes <= free_b / 2;
-    if (!s->compiled && geom_bytes > free_b)
-        return fail(SSW_ERR_OOM, "task records need " + std::to_string(geom_bytes) + " bytes, " +
-                                     std::to_string(free_b) + " free");
-    TRY(dalloc(s, &s->d_flat, s->compiled ? DN : (size_t)L * DN));
-    if (s->compiled) TRY(setup_compiled(s, s->d_cnt));
-    TRY(build_full_schedule(s));
+    TRY(dalloc(s, &s->d_flat, DN));
+    if (s->compiled) TRY(setup_overflow(s, s->d_cnt));
+    TRY(build_schedule(s));
     s->have_grid = true;
     return SSW_OK;
 }
@@ -1716,7 +1712,7 @@ int ssw_upload_state(ssw_sweep* s, const double* density, const double* xhii, co
                      const double* source) {
     if (!s || !density || !xhii || !temp || !source) return fail(SSW_ERR_INVALID, "null argument");
     if (!s->have_grid) return fail(SSW_ERR_INVALID, "ssw_upload_grid must be called first");
-    invalidate_replays(s);
+    drop_all_lists(s);
     free_state(s);
     const int64_t n = s->N;
     const int L = s->p.num_timestep_levels;
@@ -1828,37 +1824,21 @@ int ssw_download_levelsets(ssw_sweep* s, int32_t lvl, uint32_t* depth) {
     if (!s || !depth) return fail(SSW_ERR_INVALID, "null argument");
     if (!s->have_grid) return fail(SSW_ERR_INVALID, "ssw_upload_grid must be called first");
     if (lvl < 0 || lvl >= s->p.num_timestep_levels) return fail(SSW_ERR_INVALID, "sweep_level out of range");
-    Geo g = make_geo(s);
-    uint32_t need = 0;
-    HIP_TRY(hipMemcpyAsync(&need, s->d_need + lvl, 4, hipMemcpyDeviceToHost, s->stream));
+    // Kahn layering of the restricted graph in the scratch records (the replay
+    // schedule is not touched)
+    std::vector<uint8_t> lev(s->N);
+    HIP_TRY(hipMemcpyAsync(lev.data(), s->d_level, s->N, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    if (need) {  // build the schedule of this active set without transport
-        if (s->fp32) {
-            Flux<float> fl{};
-            k_begin<float><<<1, kBlock, 0, s->stream>>>(g, fl, lvl);
-        } else {
-            Flux<double> fl{};
-            k_begin<double><<<1, kBlock, 0, s->stream>>>(g, fl, lvl);
-        }
-        k_count<<<grid_for((int64_t)s->N * s->D), kBlock, 0, s->stream>>>(g, lvl, 0);
-        for (int q = 0; q < s->G; q++)
-            for (int k = 0; k < s->kmax_g[q]; k++)
-                launch_build_any(s, g, lvl, q, k, false, build_blocks(s, lvl, q, k), s->stream);
-        k_fill_u32<<<1, 1, 0, s->stream>>>(s->d_need + lvl, 0u, 1);
-        HIP_TRY(hipGetLastError());
-        if ((int)s->replay_ok.size() > lvl) drop_replay(s, lvl);
-    }
+    int64_t active = 0;
+    for (uint8_t v : lev) active += v >= lvl;
+    int k = 0;
+    TRY(kahn_layering(s, lvl, active * s->D, &k, nullptr));
     uint32_t* d_depth = nullptr;
     const int64_t DN = (int64_t)s->D * s->N;
     TRY(dalloc(s, &d_depth, (size_t)DN));
     k_fill_u32<<<grid_for(DN), kBlock, 0, s->stream>>>(d_depth, kNone, DN);
-    for (int q = 0; q < s->G; q++) {
-        const size_t base = (size_t)lvl * DN + s->gbase[q];
-        const uint4* tasks = s->compiled ? s->d_hdr + base : s->d_flat + base;
-        k_scatter_depth<<<grid_for(std::min<int64_t>(DN, 1 << 22)), kBlock, 0, s->stream>>>(
-            g, tasks, s->d_lstart + ((size_t)lvl * s->G + q) * (s->kcap + 2), s->kmax_g[q], s->compiled ? 1 : 0,
-            d_depth);
-    }
+    Geo g = make_geo(s);
+    k_scatter_depth<<<grid_for(std::min<int64_t>(DN, 1 << 22)), kBlock, 0, s->stream>>>(g, k, d_depth);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(depth, d_depth, (size_t)DN * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -1915,10 +1895,11 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info) {
     info->device_bytes = s->bytes;
     info->transport_mode = s->have_grid ? (s->compiled ? SSW_TRANSPORT_MODE_COMPILED : SSW_TRANSPORT_MODE_GEOMETRIC)
                                         : s->mode_request;
-    info->schedule_builds_last_step = s->builds_last;
+    info->list_builds_last_step = (int32_t)s->builds_last;
     info->overflow_entries = s->ovf_total;
     info->graph_capture_failures = s->capture_failures;
-    info->direction_groups = s->have_grid ? s->G : s->G_request;
+    info->replay_block = s->rec_block;
+    info->graph_replay = s->graphs_on ? 1 : 0;
     if (s->have_state) {
         unsigned long long f = 0, a = 0;
         HIP_TRY(hipMemcpyAsync(&f, s->d_fail, 8, hipMemcpyDeviceToHost, s->stream));
@@ -1965,21 +1946,15 @@ int64_t ssw_estimate_device_bytes(const ssw_sweep_parameters* p, int64_t n, int6
     const int64_t es = rs == 4 ? 8 : 16;
     const int64_t DN = D * n;
     int64_t b = 0;
-    b += (n + 1) * 4 + F * 32 + F * 4 + n * 16 + n * 4 + num_periodic_cells * 8 + F * 4 + n + L * 12 + DN * 4;
-    b += L * (n + 2) * 4;                          // level starts (depth < number of cells)
-    b += (mode == SSW_TRANSPORT_MODE_COMPILED) ? DN * 16 : L * DN * 16;
-    if (mode == SSW_TRANSPORT_MODE_COMPILED) b += L * DN * (16 + rs + kPlanes * es) + DN * 4 + DN * es / 16;
+    b += (n + 1) * 4 + F * 32 + F * 4 + n * 16 + n * 4 + num_periodic_cells * 8 + F * 4 + n + DN * 4;  // grid, counts
+    b += DN * 16 + (n + 4) * 4;                    // layering records and level starts
+    b += DN * 16;                                  // schedule build: depth, keys, values, sorted (transient)
+    b += (mode == SSW_TRANSPORT_MODE_COMPILED) ? DN * (16 + rs + kPlanes * es + 4) + DN * es / 16 : DN * 16;
+    b += (L - 1) * DN * 4;                         // active-set lists
+    b += DN / kChunk * 40 + (1 << 20);             // chunk table, scan scratch
     b += n * 8 * 9 + 2 * DN * rs + D * num_periodic_cells * rs;
     b += n * 8;                                    // photon-rate download
     return b + b / 20 + (1 << 20);
-}
-
-int ssw_set_direction_groups(ssw_sweep* s, int32_t groups) {
-    if (!s) return fail(SSW_ERR_INVALID, "null handle");
-    if (groups < 0 || groups > 16) return fail(SSW_ERR_INVALID, "direction groups must be in [0, 16]");
-    if (s->have_grid) return fail(SSW_ERR_INVALID, "direction groups must be set before ssw_upload_grid");
-    s->G_request = groups;
-    return SSW_OK;
 }
 
 int ssw_set_timing(ssw_sweep* s, int32_t enabled) {

@@ -65,7 +65,7 @@ class Server:
         self._pending = {}
 
     def submit(self, problem: Problem, steps: int, transport_mode: int = N.TRANSPORT_MODE_AUTO,
-               direction_groups: int = 0) -> int:
+               replay_block: int = 0) -> int:
         g = problem.grid
         n = g.num_cells
         arrays = list(g.arrays()) + [np.ascontiguousarray(a, dtype=np.float64) for a in problem.state()]
@@ -76,7 +76,7 @@ class Server:
         r.params = pc
         r.transport_mode = int(transport_mode)
         r.steps = int(steps)
-        r.direction_groups = int(direction_groups)
+        r.replay_block = int(replay_block)
         r.num_cells = n
         names = ["face_offsets", "face_neighbour", "face_kind", "face_area", "face_normal", "cell_volume",
                  "cell_size", "density", "ionized_fraction", "temperature", "source"]

@@ -83,7 +83,7 @@ class Sweep:
     """Device-resident sweep state of one problem (one HIP stream)."""
 
     def __init__(self, params: SweepParameters, grid: Grid, density, ionized_fraction, temperature, source,
-                 transport_mode: int = N.TRANSPORT_MODE_AUTO, direction_groups: int = 0):
+                 transport_mode: int = N.TRANSPORT_MODE_AUTO, replay_block: int = 0, graph_replay: bool = True):
         lib = N.sweep_lib()
         self._lib = lib
         self.params = params
@@ -95,7 +95,8 @@ class Sweep:
         self.num_cells = grid.num_cells
         try:
             N.check(lib.ssw_set_transport_mode(h, int(transport_mode)))
-            N.check(lib.ssw_set_direction_groups(h, int(direction_groups)))
+            N.check(lib.ssw_set_replay_block(h, int(replay_block)))
+            N.check(lib.ssw_set_graph_replay(h, 1 if graph_replay else 0))
             off, nbr, kind, area, normal, vol, size = grid.arrays()
             N.check(lib.ssw_upload_grid(h, grid.num_cells, N.ptr(off, C.c_int64), N.ptr(nbr, C.c_int32),
                                         N.ptr(kind, C.c_uint8), N.ptr(area, C.c_double), N.ptr(normal, C.c_double),

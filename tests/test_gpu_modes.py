@@ -1,13 +1,12 @@
-"""GPU: the two replay forms of the transport and the build-while-solving
-sweep evaluate the same arithmetic in the same order, so their results are
-bitwise equal:
+"""GPU: the two replay forms of the transport evaluate the same arithmetic in
+the same order, so their results are bitwise equal:
 
   * geometric replay (task records, face geometry re-read every sweep) and
     compiled replay (per-task upwind entries in level-position planes, with
-    overflow entries for tasks with more than 12 upwind faces);
-  * the first sweep of an active set (which builds the level sets while
-    solving) and a replay of the same sweep (schedule built beforehand
-    without transport, through the level-set parity hook).
+    overflow entries for tasks with more than 12 upwind faces), with one and
+    several timestep levels (restricted sweeps replay the full schedule
+    through active-set lists);
+  * the compiled replay at every block size.
 
 Results are also checked against the oracle in the parity tests; here the bar
 is bit-identity of the raw FP32/FP64 transport state and of x, T and levels.
@@ -23,14 +22,14 @@ from subsweep_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-def _run(pb, mode, steps, prebuild=False, groups=0):
-    s = Sweep(pb.params, pb.grid, *pb.state(), transport_mode=mode, direction_groups=groups)
+def _run(pb, mode, steps, prebuild=False, block=0):
+    s = Sweep(pb.params, pb.grid, *pb.state(), transport_mode=mode, replay_block=block)
     info = s.info()
-    if prebuild:
-        for lvl in range(pb.params.num_timestep_levels):
-            s.levelsets(lvl)
     states = []
     for _ in range(steps):
+        if prebuild:  # parity-hook layerings between the steps
+            for lvl in range(pb.params.num_timestep_levels):
+                s.levelsets(lvl)
         s.run_sweeps()
         x, i = s.transport_state()
         r = s.results(photon_rate=False)
@@ -59,36 +58,47 @@ def test_geometric_and_compiled_bitwise(precision, levels, hip_available):
     _assert_same(sg, sc, "geometric vs compiled")
 
 
-@pytest.mark.parametrize("mode", [N.TRANSPORT_MODE_GEOMETRIC, N.TRANSPORT_MODE_COMPILED])
-def test_build_equals_replay(mode, hip_available):
-    pb = W.jittered_problem(12, 1.0, "8", 16, levels=3, periodic=True, steps=4, precision=32)
+def test_graph_replay_bitwise(hip_available):
+    # one handle at a time: graphs are captured only while a single handle lives
+    pb = W.jittered_problem(10, 1.0, "6", 84, levels=3, periodic=True, steps=5, precision=32)
     pb = pb.with_params(max_timestep=1e-3 * 3.15576e13)
-    _, built = _run(pb, mode, pb.steps)
-    _, replayed = _run(pb, mode, pb.steps, prebuild=True)
-    _assert_same(built, replayed, "build-while-solving vs replay")
+    runs = []
+    for graphs in (True, False):
+        s = Sweep(pb.params, pb.grid, *pb.state(), graph_replay=graphs)
+        states = []
+        for _ in range(pb.steps):
+            s.run_sweeps()
+            states.append(s.transport_state())
+        info = s.info()
+        assert info.graph_replay == (1 if graphs else 0)
+        assert info.graph_capture_failures == 0
+        s.close()
+        runs.append(states)
+    for step, (a, b) in enumerate(zip(*runs)):
+        for u, v in zip(a, b):
+            assert np.array_equal(u.view(np.uint8), v.view(np.uint8)), f"step {step + 1}"
 
 
-@pytest.mark.parametrize("mode", [N.TRANSPORT_MODE_GEOMETRIC, N.TRANSPORT_MODE_COMPILED])
-def test_direction_groups_bitwise(mode, hip_available):
-    # 84 directions in 1, 3 (uneven blocks of 28) and 5 (16/17 directions) groups
+def test_replay_blocks_bitwise(hip_available):
     pb = W.jittered_problem(10, 1.0, "6", 84, levels=3, periodic=True, steps=4, precision=32)
     pb = pb.with_params(max_timestep=1e-3 * 3.15576e13)
-    i1, s1 = _run(pb, mode, pb.steps, groups=1)
-    assert i1.direction_groups == 1
-    for groups in (3, 5):
-        ig, sg = _run(pb, mode, pb.steps, groups=groups)
-        assert ig.direction_groups == groups
-        assert ig.max_depth == i1.max_depth
-        _assert_same(s1, sg, f"1 vs {groups} direction groups")
-    # the level-set parity hook is independent of the grouping too
-    for groups in (1, 4):
-        s = Sweep(pb.params, pb.grid, *pb.state(), transport_mode=mode, direction_groups=groups)
-        d = s.levelsets(0)
-        s.close()
-        if groups == 1:
-            d1 = d
-        else:
-            assert np.array_equal(d, d1)
+    i64, s64 = _run(pb, N.TRANSPORT_MODE_COMPILED, pb.steps, block=64)
+    assert i64.replay_block == 64
+    for block in (128, 256):
+        ib, sb = _run(pb, N.TRANSPORT_MODE_COMPILED, pb.steps, block=block)
+        assert ib.replay_block == block
+        _assert_same(s64, sb, f"block 64 vs {block}")
+
+
+@pytest.mark.parametrize("mode", [N.TRANSPORT_MODE_GEOMETRIC, N.TRANSPORT_MODE_COMPILED])
+def test_levelset_hook_leaves_the_replay_alone(mode, hip_available):
+    # the parity hook builds a restricted layering in scratch memory between
+    # steps; the replay schedule and its active-set lists must be unaffected
+    pb = W.jittered_problem(12, 1.0, "8", 16, levels=3, periodic=True, steps=4, precision=32)
+    pb = pb.with_params(max_timestep=1e-3 * 3.15576e13)
+    _, plain = _run(pb, mode, pb.steps)
+    _, hooked = _run(pb, mode, pb.steps, prebuild=True)
+    _assert_same(plain, hooked, "with and without level-set hook calls")
 
 
 def test_removed_modes_are_rejected(hip_available):
@@ -102,10 +112,9 @@ def test_removed_modes_are_rejected(hip_available):
 
 
 def test_autotune_picks_a_measured_candidate(hip_available):
-    from subsweep_amd.autotune import autotune
+    from subsweep_amd.autotune import autotune, candidates
     pb = W.jittered_problem(8, 1.0, "4", 16, levels=1, periodic=True, steps=2, precision=32)
-    r = autotune(pb, groups=(1, 2, 32), modes=(N.TRANSPORT_MODE_GEOMETRIC, N.TRANSPORT_MODE_COMPILED), steps=1,
-                 warmup=1)
-    assert len(r.table) == 4  # 32 groups exceed the 16 directions and are skipped
-    assert (r.direction_groups, r.transport_mode, r.ms_per_step) in [tuple(x) for x in r.table]
-    assert r.ms_per_step == min(x[2] for x in r.table)
+    r = autotune(pb, cands=candidates((64, 256)), steps=1, warmup=2)
+    assert len(r.table) == 4
+    assert (r.transport_mode, r.replay_block, r.graph_replay, r.ms_per_step) in [tuple(x) for x in r.table]
+    assert r.ms_per_step == min(x[3] for x in r.table)
