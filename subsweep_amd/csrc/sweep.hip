@@ -667,18 +667,18 @@ __global__ void __launch_bounds__(kRecBlock) k_sweep_comp(Geo g, Flux<R> fl, con
 // ---------------------------------------------------------------------------
 // per-sweep kernels
 
-// Periodic-source snapshot of the active periodic cells (the transport reads
-// the periodic source as it was when the sweep started). Thread t handles
-// (periodic cell t / D, direction t % D).
+// Periodic source of every periodic cell from the current X, after each
+// sweep: the chemistry of the sweep reads it (the reference's periodic
+// corrections have all arrived when update_chemistry runs), and the next
+// sweep's transport reads it as the value at its start (X does not change in
+// between). Thread t handles (periodic cell t / D, direction t % D).
 template <typename R>
-__global__ void k_begin(Geo g, Flux<R> fl, int lvl) {
+__global__ void k_periodic(Geo g, Flux<R> fl) {
     if (fl.P == nullptr) return;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t s = t / (uint32_t)g.D;
     const uint32_t d = t - s * (uint32_t)g.D;
     if ((int64_t)s >= g.Nper) return;
-    const uint32_t c = g.pcells[s];
-    if (g.level[c] < lvl) return;
     const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
     R acc = 0;
     for (uint32_t e = g.pfoff[s]; e < g.pfoff[s + 1]; e++) {
@@ -729,23 +729,13 @@ __global__ void k_chemistry(Geo g, Flux<R> fl, ChemArgs ca, int lvl) {
     const double dt = ca.max_ts * ssw::powi(0.5, lev);
     const uint32_t ps = g.pslot[c];
     const bool has_p = ps != kNone && fl.P;
+    const double src = ca.src[c];
     double rate = 0.0;
     for (int d = 0; d < g.D; d++) {
-        double inc = (double)fl.I[(size_t)d * g.N + c] * ca.lref;
-        double per = 0.0;
-        if (has_p) {  // current periodic source (the sweep has finished)
-            double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
-            R acc = 0;
-            for (uint32_t e = g.pfoff[ps]; e < g.pfoff[ps + 1]; e++) {
-                const uint32_t f = g.pfidx[e];
-                uint32_t nk = g.nk[f];
-                double4 a = g.geo[f];
-                double dd = fdot(a, dx, dy, dz);
-                if (dd < 0.0) acc += (R)(a.w * fl.inv_aref * -dd) * fl.X[(size_t)d * g.N + (nk & kNbrMask)];
-            }
-            per = (double)acc * ca.lref;
-        }
-        rate += (inc + ca.src[c]) + per;  // sum over d of Site::get_rate, in direction order
+        const double inc = (double)fl.I[(size_t)d * g.N + c] * ca.lref;
+        // current periodic source (k_periodic ran after the sweep)
+        const double per = has_p ? (double)fl.P[(size_t)d * g.Nper + ps] * ca.lref : 0.0;
+        rate += (inc + src) + per;  // sum over d of Site::get_rate, in direction order
     }
     double rel;
     if (fabs(rate) < fabs(ca.thr)) {
@@ -1331,7 +1321,6 @@ int run_step(ssw_sweep* s, double* elapsed) {
     for (int l = 1; l < L; l++)
         if (need[l]) drop_list(s, l);
     for (int lvl : order) {
-        k_begin<R><<<gper, kBlock, 0, s->stream>>>(g, fl, lvl);
         if (lvl > 0 && !s->list_ok[lvl]) {
             hipEvent_t b0 = s->timing ? mark() : nullptr;
             TRY(make_list(s, lvl));
@@ -1349,6 +1338,7 @@ int run_step(ssw_sweep* s, double* elapsed) {
         HIP_TRY(hipGetLastError());
         s->replays[lvl]++;
         hipEvent_t a = s->timing ? mark() : nullptr;
+        k_periodic<R><<<gper, kBlock, 0, s->stream>>>(g, fl);
         k_chemistry<R><<<gcells, kBlock, 0, s->stream>>>(g, fl, ca, lvl);
         if (s->timing) ch.push_back({a, mark()});
         s->sweeps_last++;
