@@ -9,16 +9,21 @@ synthetic and seeded; grid and state are resident on the device before the
 timed region starts.
 
 `value` is the FP32-transport + FP64-chemistry throughput; the FP64-transport
-throughput of the same workload is reported beside it. The roofline figure is
-the algorithmic byte count of the transport per cell-direction update (the
-per-update byte model: 8+8 B incoming/periodic reads, 16 B outgoing
+throughput of the same workload is reported beside it. The launch shape of
+each (replay form, block size, graph replay) is the autotuner's choice. The
+roofline figure is the algorithmic byte count of the transport per
+cell-direction update (8+8 B incoming/periodic reads, 16 B outgoing
 read/write, 16 B per-cell absorbed-rate accumulation, and per downwind face
-36/28/12 B for local/periodic/boundary faces in FP64, 24/16/8 B and 4+4+8 B
-in FP32) times the updates of one launch of the transport kernel, divided by
-its average duration from HIP events recorded on the library's stream around
-every launch, in extra steps run right after the timed steps (the timed steps
-themselves carry no per-launch events). cpu_baseline times the oracle (the single-threaded CPU
-restatement of the reference's sweep) on one step of the same workload.
+36/28/12 B for local/periodic/boundary faces in FP64, 24/16/8 B and 4+4+8 B in
+FP32) times the updates of one launch of the transport kernel, divided by its
+average launch duration: HIP events recorded on the library's stream around
+each sweep's whole replay in extra steps run right after the timed steps
+(the timed steps carry no events), divided by the launched level sets.
+`traffic` is the HBM bytes per launch from the committed rocprofv3
+FETCH_SIZE / WRITE_SIZE summary. cpu_baseline times the oracle (the
+single-threaded CPU restatement of the reference's sweep) on one step of the
+same workload. The 2097152-cell clustered workload (FP32 vs FP64) and the
+serving loop (p50/p99 latency) are reported in the same JSON line.
 """
 
 from __future__ import annotations

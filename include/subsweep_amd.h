@@ -150,6 +150,8 @@ typedef struct ssw_info {
                                        graph could not be captured            */
     int32_t replay_block;          /* compiled replay block size               */
     int32_t graph_replay;          /* 1 if replays are captured as graphs      */
+    int32_t locality_order;        /* 1 if cells are stored in position order  */
+    int32_t reserved1;
 } ssw_info;
 
 int ssw_get_info(ssw_sweep* s, ssw_info* info);
@@ -176,6 +178,15 @@ int ssw_set_transport_mode(ssw_sweep* s, int32_t mode);
  * results do not depend on it. No counterpart in the reference. */
 int ssw_set_replay_block(ssw_sweep* s, int32_t block);
 
+/* Optional cell positions (num_cells x 3 doubles, any units), given before
+ * ssw_upload_grid: the device then stores cells in the Morton (Z-curve) order
+ * of their positions, so that the upwind neighbours gathered by one wave of
+ * tasks share cache lines. Every array the API takes or returns stays in the
+ * caller's cell order, and results do not depend on it (bitwise). NULL or
+ * n <= 0 clears it. No counterpart in the reference (its cells carry a
+ * Position component, which is what a caller passes here). */
+int ssw_set_cell_positions(ssw_sweep* s, int64_t num_cells, const double* positions);
+
 /* Replays of an unchanged active set are captured as one HIP graph from their
  * second use on (enabled = 1, default) or launched one level set at a time
  * (0). Capture needs the process to hold no other handle at that moment
@@ -189,11 +200,11 @@ int ssw_set_graph_replay(ssw_sweep* s, int32_t enabled);
 int ssw_download_transport(ssw_sweep* s, void* X, void* I);
 
 /* Kernel timing of the last `ssw_step` when timing is enabled, from HIP events
- * recorded on the handle's stream: average duration in ms of each replay
- * launch of the transport kernel (one per level set), their total, the total
- * of the chemistry kernels, and the total of the schedule builds (counting
- * plus the level-by-level build-and-solve launches of sweeps whose active set
- * changed). */
+ * recorded on the handle's stream: one event pair around each sweep's whole
+ * transport replay, so transport_avg_ms is the replay time per launched
+ * level set (kernel time plus the gaps between the launches), together with
+ * the transport total and launch count, the total of the chemistry kernels,
+ * and the total of the active-set list builds. */
 int ssw_set_timing(ssw_sweep* s, int32_t enabled);
 int ssw_get_timing(ssw_sweep* s, double* transport_avg_ms, double* transport_total_ms,
                    int64_t* transport_launches, double* chemistry_total_ms, double* build_total_ms);
@@ -250,6 +261,7 @@ typedef struct ssw_request {
     const double* ionized_fraction;
     const double* temperature;
     const double* source;
+    const double* cell_position;   /* optional, as for ssw_set_cell_positions */
     double* out_ionized_fraction;  /* results, as for ssw_download            */
     double* out_temperature;
     double* out_timestep;

@@ -68,6 +68,8 @@ class InfoC(C.Structure):
         ("graph_capture_failures", C.c_int64),
         ("replay_block", C.c_int32),
         ("graph_replay", C.c_int32),
+        ("locality_order", C.c_int32),
+        ("reserved1", C.c_int32),
     ]
 
 
@@ -96,6 +98,7 @@ class RequestC(C.Structure):
         ("ionized_fraction", C.c_void_p),
         ("temperature", C.c_void_p),
         ("source", C.c_void_p),
+        ("cell_position", C.c_void_p),
         ("out_ionized_fraction", C.c_void_p),
         ("out_temperature", C.c_void_p),
         ("out_timestep", C.c_void_p),
@@ -143,6 +146,7 @@ SWEEP_SIGNATURES = {
     "ssw_set_timing": (C.c_int, [P, C.c_int32]),
     "ssw_set_replay_block": (C.c_int, [P, C.c_int32]),
     "ssw_set_graph_replay": (C.c_int, [P, C.c_int32]),
+    "ssw_set_cell_positions": (C.c_int, [P, C.c_int64, PD]),
     "ssw_get_timing": (C.c_int, [P, PD, PD, PI64, PD, PD]),
     "ssw_download_transport": (C.c_int, [P, P, P]),
     "ssw_timestep_level": (C.c_int32, [C.c_int32, C.c_double, C.c_double]),

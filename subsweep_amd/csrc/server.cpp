@@ -82,6 +82,7 @@ void run_request(Req* q) {
     rc = ssw_set_transport_mode(s, q->mode);
     if (rc == SSW_OK) rc = ssw_set_replay_block(s, r.replay_block);
     if (rc == SSW_OK) rc = ssw_set_graph_replay(s, 0);  // handles of other requests allocate concurrently
+    if (rc == SSW_OK && r.cell_position) rc = ssw_set_cell_positions(s, r.num_cells, r.cell_position);
     if (rc == SSW_OK)
         rc = ssw_upload_grid(s, r.num_cells, r.face_offsets, r.face_neighbour, r.face_kind, r.face_area, r.face_normal,
                              r.cell_volume, r.cell_size);

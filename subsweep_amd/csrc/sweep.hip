@@ -873,6 +873,11 @@ struct ssw_sweep {
     // grid
     int64_t N = 0, F = 0, Nper = 0;
     bool have_grid = false, have_state = false;
+    // internal cell order: internal index perm[c] for caller's cell c (Morton
+    // order of the positions given with ssw_set_cell_positions), inv = perm^-1
+    std::vector<double> pos;
+    std::vector<uint32_t> perm, inv;
+    bool permuted = false;
     uint32_t *d_off = nullptr, *d_nk = nullptr, *d_pslot = nullptr, *d_pcells = nullptr;
     uint32_t *d_pfoff = nullptr, *d_pfidx = nullptr;
     double4* d_geo = nullptr;
@@ -996,6 +1001,68 @@ Flux<R> make_flux(ssw_sweep* s) {
     f.inv_aref = 1.0 / s->aref;
     f.thr = s->p.significant_rate_threshold * f.inv_lref;
     return f;
+}
+
+// Morton (Z-order) code of a position quantised to 21 bits per axis.
+uint64_t morton3(const double* p, const double* lo, const double* inv_ext) {
+    uint64_t code = 0;
+    uint64_t q[3];
+    for (int k = 0; k < 3; k++) {
+        double u = (p[k] - lo[k]) * inv_ext[k];
+        u = std::min(std::max(u, 0.0), 1.0);
+        q[k] = (uint64_t)(u * 2097151.0);
+    }
+    for (int b = 0; b < 21; b++)
+        for (int k = 0; k < 3; k++) code |= ((q[k] >> b) & 1ull) << (3 * b + k);
+    return code;
+}
+
+// The internal cell order for the current grid: Morton order of the given
+// positions, or the caller's order.
+void make_order(ssw_sweep* s, int64_t n) {
+    s->permuted = (int64_t)s->pos.size() == 3 * n;
+    s->perm.resize(n);
+    s->inv.resize(n);
+    if (!s->permuted) {
+        for (int64_t c = 0; c < n; c++) s->perm[c] = s->inv[c] = (uint32_t)c;
+        return;
+    }
+    double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL}, ie[3];
+    for (int64_t c = 0; c < n; c++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], s->pos[3 * c + k]);
+            hi[k] = std::max(hi[k], s->pos[3 * c + k]);
+        }
+    for (int k = 0; k < 3; k++) ie[k] = hi[k] > lo[k] ? 1.0 / (hi[k] - lo[k]) : 0.0;
+    std::vector<std::pair<uint64_t, uint32_t>> key(n);
+    for (int64_t c = 0; c < n; c++) key[c] = {morton3(&s->pos[3 * c], lo, ie), (uint32_t)c};
+    std::sort(key.begin(), key.end());
+    for (int64_t i = 0; i < n; i++) {
+        s->inv[i] = key[i].second;
+        s->perm[key[i].second] = (uint32_t)i;
+    }
+}
+
+// Per-cell host array in the caller's order -> internal order.
+template <typename T>
+const T* to_internal(const ssw_sweep* s, const T* in, std::vector<T>& tmp) {
+    if (!s->permuted) return in;
+    tmp.resize(s->N);
+    for (int64_t i = 0; i < s->N; i++) tmp[i] = in[s->inv[i]];
+    return tmp.data();
+}
+
+// `blocks` consecutive per-cell arrays (e.g. one per direction) in internal
+// order -> the caller's order, in place.
+template <typename T>
+void to_caller(const ssw_sweep* s, T* a, int64_t blocks) {
+    if (!s->permuted) return;
+    std::vector<T> tmp(s->N);
+    for (int64_t b = 0; b < blocks; b++) {
+        T* x = a + b * s->N;
+        for (int64_t i = 0; i < s->N; i++) tmp[s->inv[i]] = x[i];
+        std::copy(tmp.begin(), tmp.end(), x);
+    }
 }
 
 int set_need_all(ssw_sweep* s, uint32_t v) {
@@ -1327,13 +1394,18 @@ int run_step(ssw_sweep* s, double* elapsed) {
             if (s->timing) bu.push_back({b0, mark()});
             s->builds_last++;
         }
+        // timing: one event pair around the sweep's whole replay (per-launch
+        // events would add their own packets between the launches); the
+        // average launch duration is the total over the non-empty level sets
+        hipEvent_t t0 = s->timing ? mark() : nullptr;
+        if (s->graphs_on && !s->graphs[lvl] && s->replays[lvl] >= 1 && g_live_handles.load() == 1)
+            capture(s, g, lvl);
+        if (s->graphs[lvl]) HIP_TRY(hipGraphLaunch(s->graphs[lvl], s->stream));
+        else launch_replay(s, g, lvl, nullptr);
         if (s->timing) {
-            launch_replay(s, g, lvl, &tr, &mark);
-        } else {
-            if (s->graphs_on && !s->graphs[lvl] && s->replays[lvl] >= 1 && g_live_handles.load() == 1)
-                capture(s, g, lvl);
-            if (s->graphs[lvl]) HIP_TRY(hipGraphLaunch(s->graphs[lvl], s->stream));
-            else launch_replay(s, g, lvl, nullptr);
+            tr.push_back({t0, mark()});
+            const std::vector<uint32_t>& cs = lvl == 0 ? s->full_start : s->cstart[lvl];
+            for (int k = 0; k < s->kmax; k++) s->n_transport += cs[k + 1] > cs[k] ? 1 : 0;
         }
         HIP_TRY(hipGetLastError());
         s->replays[lvl]++;
@@ -1355,7 +1427,6 @@ int run_step(ssw_sweep* s, double* elapsed) {
             float ms = 0;
             (void)hipEventElapsedTime(&ms, pr.first, pr.second);
             s->t_transport_total += ms;
-            s->n_transport++;
         }
         for (auto& pr : ch) {
             float ms = 0;
@@ -1572,6 +1643,17 @@ int ssw_set_replay_block(ssw_sweep* s, int32_t block) {
     return SSW_OK;
 }
 
+int ssw_set_cell_positions(ssw_sweep* s, int64_t n, const double* positions) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    if (s->have_grid) return fail(SSW_ERR_INVALID, "cell positions must be set before ssw_upload_grid");
+    if (!positions || n <= 0) {
+        s->pos.clear();
+        return SSW_OK;
+    }
+    s->pos.assign(positions, positions + 3 * n);
+    return SSW_OK;
+}
+
 int ssw_set_graph_replay(ssw_sweep* s, int32_t enabled) {
     if (!s) return fail(SSW_ERR_INVALID, "null handle");
     s->graphs_on = enabled != 0;
@@ -1603,33 +1685,40 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     s->have_grid = s->have_state = false;
     s->N = n;
     s->F = F;
-    // pack faces; a non-periodic sweep treats periodic faces as boundaries
+    make_order(s, n);
+    // pack faces in the internal cell order, each cell's faces in the caller's
+    // order; a non-periodic sweep treats periodic faces as boundaries
     std::vector<double4> geo(F);
     std::vector<uint32_t> nk(F);
     std::vector<uint32_t> off32(n + 1);
     std::vector<uint32_t> pslot(n, kNone), pcells, pfoff(1, 0), pfidx;
     double asum = 0.0;
     int64_t nfaces = 0;
-    for (int64_t c = 0; c <= n; c++) off32[c] = (uint32_t)off[c];
-    for (int64_t c = 0; c < n; c++) {
+    for (int64_t f = 0; f < F; f++) asum += area[f];  // caller's order: independent of the internal order
+    nfaces = F;
+    off32[0] = 0;
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t c = s->inv[i];
+        const int64_t fi0 = off32[i];
+        off32[i + 1] = (uint32_t)(fi0 + (off[c + 1] - off[c]));
         bool has_p = false;
         for (int64_t f = off[c]; f < off[c + 1]; f++) {
+            const int64_t fi = fi0 + (f - off[c]);
             uint32_t k = kind[f];
             if (k > 2) return fail(SSW_ERR_INVALID, "face_kind must be 0, 1 or 2");
             if (k == kKindPeriodic && !s->p.periodic) k = kKindBoundary;
             int64_t nb = (k == kKindBoundary) ? 0 : nbr[f];
             if (k != kKindBoundary && (nb < 0 || nb >= n)) return fail(SSW_ERR_INVALID, "face_neighbour out of range");
-            geo[f] = make_double4(normal[3 * f], normal[3 * f + 1], normal[3 * f + 2], area[f]);
-            nk[f] = (uint32_t)nb | (k << 30);
+            if (k != kKindBoundary) nb = s->perm[nb];
+            geo[fi] = make_double4(normal[3 * f], normal[3 * f + 1], normal[3 * f + 2], area[f]);
+            nk[fi] = (uint32_t)nb | (k << 30);
             if (k == kKindPeriodic) has_p = true;
-            asum += area[f];
-            nfaces++;
         }
         if (has_p) {
-            pslot[c] = (uint32_t)pcells.size();
-            pcells.push_back((uint32_t)c);
-            for (int64_t f = off[c]; f < off[c + 1]; f++)
-                if ((nk[f] >> 30) == kKindPeriodic) pfidx.push_back((uint32_t)f);
+            pslot[i] = (uint32_t)pcells.size();
+            pcells.push_back((uint32_t)i);
+            for (int64_t fi = fi0; fi < (int64_t)off32[i + 1]; fi++)
+                if ((nk[fi] >> 30) == kKindPeriodic) pfidx.push_back((uint32_t)fi);
             pfoff.push_back((uint32_t)pfidx.size());
         }
     }
@@ -1655,8 +1744,9 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     HIP_TRY(hipMemcpyAsync(s->d_off, off32.data(), (n + 1) * 4, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_geo, geo.data(), F * sizeof(double4), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_nk, nk.data(), F * 4, hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipMemcpyAsync(s->d_vol, volume, n * 8, hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipMemcpyAsync(s->d_size, size, n * 8, hipMemcpyHostToDevice, s->stream));
+    std::vector<double> vol_i, size_i;
+    HIP_TRY(hipMemcpyAsync(s->d_vol, to_internal(s, volume, vol_i), n * 8, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_size, to_internal(s, size, size_i), n * 8, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_pslot, pslot.data(), n * 4, hipMemcpyHostToDevice, s->stream));
     if (s->Nper) {
         HIP_TRY(hipMemcpyAsync(s->d_pcells, pcells.data(), s->Nper * 4, hipMemcpyHostToDevice, s->stream));
@@ -1706,6 +1796,11 @@ int ssw_upload_state(ssw_sweep* s, const double* density, const double* xhii, co
     free_state(s);
     const int64_t n = s->N;
     const int L = s->p.num_timestep_levels;
+    std::vector<double> rho_i, x_i, T_i, src_i;
+    density = to_internal(s, density, rho_i);
+    xhii = to_internal(s, xhii, x_i);
+    temp = to_internal(s, temp, T_i);
+    source = to_internal(s, source, src_i);
     std::vector<double> sod(n);
     double lmax = 0.0;
     for (int64_t c = 0; c < n; c++) {
@@ -1795,6 +1890,12 @@ int ssw_download(ssw_sweep* s, double* x, double* T, double* sts, double* photon
         dfree(s, d_pr);
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
+    if (x) to_caller(s, x, 1);
+    if (T) to_caller(s, T, 1);
+    if (sts) to_caller(s, sts, 1);
+    if (change) to_caller(s, change, 1);
+    if (level) to_caller(s, level, 1);
+    if (photon) to_caller(s, photon, 1);
     return SSW_OK;
 }
 
@@ -1807,6 +1908,7 @@ int ssw_download_counts(ssw_sweep* s, int32_t lvl, uint32_t* counts) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(counts, s->d_cnt, (size_t)s->D * s->N * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
+    to_caller(s, counts, s->D);
     return SSW_OK;
 }
 
@@ -1833,6 +1935,7 @@ int ssw_download_levelsets(ssw_sweep* s, int32_t lvl, uint32_t* depth) {
     HIP_TRY(hipMemcpyAsync(depth, d_depth, (size_t)DN * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     dfree(s, d_depth);
+    to_caller(s, depth, s->D);
     return SSW_OK;
 }
 
@@ -1858,6 +1961,9 @@ int ssw_download_fluxes(ssw_sweep* s, double* incoming, double* outgoing, double
     dfree(s, dI);
     dfree(s, dO);
     dfree(s, dP);
+    if (incoming) to_caller(s, incoming, s->D);
+    if (outgoing) to_caller(s, outgoing, s->D);
+    if (periodic) to_caller(s, periodic, s->D);
     return SSW_OK;
 }
 
@@ -1868,6 +1974,11 @@ int ssw_download_transport(ssw_sweep* s, void* X, void* I) {
     if (X) HIP_TRY(hipMemcpyAsync(X, s->d_X, m, hipMemcpyDeviceToHost, s->stream));
     if (I) HIP_TRY(hipMemcpyAsync(I, s->d_I, m, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
+    for (void* a : {X, I}) {
+        if (!a) continue;
+        if (s->fp32) to_caller(s, (float*)a, s->D);
+        else to_caller(s, (double*)a, s->D);
+    }
     return SSW_OK;
 }
 
@@ -1890,6 +2001,7 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info) {
     info->graph_capture_failures = s->capture_failures;
     info->replay_block = s->rec_block;
     info->graph_replay = s->graphs_on ? 1 : 0;
+    info->locality_order = s->permuted ? 1 : 0;
     if (s->have_state) {
         unsigned long long f = 0, a = 0;
         HIP_TRY(hipMemcpyAsync(&f, s->d_fail, 8, hipMemcpyDeviceToHost, s->stream));

@@ -82,6 +82,10 @@ class Server:
                  "cell_size", "density", "ionized_fraction", "temperature", "source"]
         for name, a in zip(names, arrays):
             setattr(r, name, a.ctypes.data)
+        if g.points is not None and len(g.points) == n:  # locality order from the generator positions
+            pts = np.ascontiguousarray(g.points, dtype=np.float64).reshape(-1, 3)
+            arrays.append(pts)
+            r.cell_position = pts.ctypes.data
         r.out_ionized_fraction = out["x"].ctypes.data
         r.out_temperature = out["T"].ctypes.data
         r.out_timestep = out["ts"].ctypes.data

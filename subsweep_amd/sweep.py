@@ -83,7 +83,8 @@ class Sweep:
     """Device-resident sweep state of one problem (one HIP stream)."""
 
     def __init__(self, params: SweepParameters, grid: Grid, density, ionized_fraction, temperature, source,
-                 transport_mode: int = N.TRANSPORT_MODE_AUTO, replay_block: int = 0, graph_replay: bool = True):
+                 transport_mode: int = N.TRANSPORT_MODE_AUTO, replay_block: int = 0, graph_replay: bool = True,
+                 locality_order: bool = True):
         lib = N.sweep_lib()
         self._lib = lib
         self.params = params
@@ -97,6 +98,10 @@ class Sweep:
             N.check(lib.ssw_set_transport_mode(h, int(transport_mode)))
             N.check(lib.ssw_set_replay_block(h, int(replay_block)))
             N.check(lib.ssw_set_graph_replay(h, 1 if graph_replay else 0))
+            if locality_order and grid.points is not None:
+                pts = np.ascontiguousarray(grid.points, dtype=np.float64).reshape(-1, 3)
+                if pts.shape[0] == grid.num_cells:
+                    N.check(lib.ssw_set_cell_positions(h, grid.num_cells, N.ptr(pts, C.c_double)))
             off, nbr, kind, area, normal, vol, size = grid.arrays()
             N.check(lib.ssw_upload_grid(h, grid.num_cells, N.ptr(off, C.c_int64), N.ptr(nbr, C.c_int32),
                                         N.ptr(kind, C.c_uint8), N.ptr(area, C.c_double), N.ptr(normal, C.c_double),

@@ -118,3 +118,26 @@ def test_autotune_picks_a_measured_candidate(hip_available):
     assert len(r.table) == 4
     assert (r.transport_mode, r.replay_block, r.graph_replay, r.ms_per_step) in [tuple(x) for x in r.table]
     assert r.ms_per_step == min(x[3] for x in r.table)
+
+
+@pytest.mark.parametrize("mode", [N.TRANSPORT_MODE_GEOMETRIC, N.TRANSPORT_MODE_COMPILED])
+def test_locality_order_bitwise(mode, hip_available):
+    # cells stored in the Morton order of their positions vs the caller's order:
+    # every output, in the caller's order, bitwise equal
+    pb = W.jittered_problem(10, 1.0, "6", 16, levels=3, periodic=True, steps=4, precision=32)
+    pb = pb.with_params(max_timestep=1e-3 * 3.15576e13)
+    out = []
+    for order in (True, False):
+        s = Sweep(pb.params, pb.grid, *pb.state(), transport_mode=mode, locality_order=order)
+        assert s.info().locality_order == (1 if order else 0)
+        seq = []
+        for _ in range(pb.steps):
+            s.run_sweeps()
+            r = s.results()
+            seq.append((*s.transport_state(), *s.fluxes(), r.ionized_hydrogen_fraction, r.temperature,
+                        r.photon_rate, r.level, r.change_timescale, s.counts(1), s.levelsets(2)))
+        s.close()
+        out.append(seq)
+    for step, (a, b) in enumerate(zip(*out)):
+        for k, (u, v) in enumerate(zip(a, b)):
+            assert np.array_equal(u.view(np.uint8), v.view(np.uint8)), f"step {step + 1}, field {k}"
