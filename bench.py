@@ -55,7 +55,7 @@ def bytes_per_update(info, precision: int) -> float:
 
 
 def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3, mode: int = 0, block: int = 0,
-            graphs: bool = True):
+            graphs: bool = True, progress: bool = False):
     from subsweep_amd import Sweep
     pp = pb.with_params(transport_precision=precision)
     s = Sweep(pp.params, pp.grid, *pp.state(), transport_mode=mode, replay_block=block, graph_replay=graphs)
@@ -65,9 +65,11 @@ def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3,
     # timed region: whole steps, no per-launch events
     updates = 0
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for k in range(steps):
         s.run_sweeps()
         updates += s.info().updates_last_step
+        if progress:
+            log(f"  step {k + 1}/{steps} at {time.perf_counter() - t0:.1f} s")
     s.synchronize()
     t1 = time.perf_counter()
     # kernel timing: the same steps again with HIP events around every launch
@@ -135,7 +137,7 @@ def run_largest(steps: int = 10):
                              steps=steps, precision=32, name="clustered_16m")
     log(f"largest grid: {pb.grid.num_cells} cells, {pb.grid.num_faces} faces, built in "
         f"{time.perf_counter() - t:.1f} s")
-    r = run_gpu(pb, 32, steps, 0, profile_steps=1)
+    r = run_gpu(pb, 32, steps, 0, profile_steps=1, progress=True)
     log(f"largest fp32: {r['updates_per_s'] / 1e9:.3f} G updates/s, {r['ms_per_step']:.1f} ms/step, "
         f"mode {r['transport_mode']}, {r['info'].device_bytes / 2**30:.1f} GiB on the device")
     return {

@@ -176,6 +176,7 @@ GRID_SIGNATURES = {
     "ssw_voronoi_get": (None, [P, PI64, PI32, PU8, PD, PD, PD, PD]),
     "ssw_voronoi_free": (None, [P]),
     "ssw_grid_last_error": (C.c_char_p, []),
+    "ssw_grid_set_progress": (None, [C.c_int]),
 }
 
 _sweep_lib = None

@@ -32,6 +32,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <unordered_map>
@@ -230,6 +231,7 @@ struct Grid {
 };
 
 thread_local std::string g_err;
+int g_progress = 0;  // print construction progress to stderr (long builds)
 
 struct Cand {
     double d2;
@@ -515,6 +517,7 @@ void* ssw_voronoi_build(int64_t n, const double* pts, const double* box, int mod
     std::vector<std::vector<BuiltFace>> cells(n);
     std::vector<double> vols(n, 0.0);
     std::atomic<int64_t> open_cells{0};
+    std::atomic<int64_t> done_cells{0};
 #ifdef _OPENMP
     if (num_threads > 0) omp_set_num_threads(num_threads);
 #endif
@@ -534,6 +537,14 @@ void* ssw_voronoi_build(int64_t n, const double* pts, const double* box, int mod
 #pragma omp for schedule(dynamic, 256)
             for (size_t ti = 0; ti < todo.size(); ti++) {
                 const int64_t i = todo[ti];
+                if (g_progress) {
+                    const int64_t k = ++done_cells;
+                    const int64_t step = std::max<int64_t>(1, (int64_t)todo.size() / 10);
+                    if (k % step == 0) {
+                        fprintf(stderr, "voronoi: %lld of %lld cells\n", (long long)k, (long long)todo.size());
+                        fflush(stderr);
+                    }
+                }
                 // work relative to the generator for accuracy
                 V3 pa{local[3 * i], local[3 * i + 1], local[3 * i + 2]};
                 V3 p{0.0, 0.0, 0.0};
@@ -709,6 +720,8 @@ void* ssw_voronoi_build(int64_t n, const double* pts, const double* box, int mod
     }
     return g;
 }
+
+void ssw_grid_set_progress(int enabled) { g_progress = enabled; }
 
 int64_t ssw_voronoi_num_faces(void* h) { return static_cast<Grid*>(h)->off.back(); }
 int64_t ssw_voronoi_dropped_faces(void* h) { return static_cast<Grid*>(h)->dropped_faces; }

@@ -93,6 +93,7 @@ def voronoi_grid(points: np.ndarray, box, threads: int = 0, free_space_box=None)
     if free_space_box is not None:
         mode = 1
         lohi = np.ascontiguousarray(np.concatenate([free_space_box[0], free_space_box[1]]), dtype=np.float64)
+    lib.ssw_grid_set_progress(1 if n >= (1 << 22) else 0)  # long builds report progress
     h = lib.ssw_voronoi_build(n, N.ptr(pts, C.c_double), N.ptr(boxa, C.c_double), mode,
                               N.ptr(lohi, C.c_double) if lohi is not None else None, threads)
     try:
