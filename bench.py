@@ -149,6 +149,7 @@ def run_largest(steps: int = 10):
         "updates_per_step": r["updates_per_step"],
         "transport_mode": r["transport_mode"],
         "device_gib": r["info"].device_bytes / 2**30,
+        "peak_device_gib": r["info"].peak_device_bytes / 2**30,
         "dag_depth": int(r["info"].max_depth),
     }
 

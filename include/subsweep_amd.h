@@ -152,6 +152,7 @@ typedef struct ssw_info {
     int32_t graph_replay;          /* 1 if replays are captured as graphs      */
     int32_t locality_order;        /* 1 if cells are stored in position order  */
     int32_t reserved1;
+    int64_t peak_device_bytes;     /* most device memory the handle has held   */
 } ssw_info;
 
 int ssw_get_info(ssw_sweep* s, ssw_info* info);
@@ -164,8 +165,9 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info);
  *  SSW_TRANSPORT_MODE_COMPILED   per task its upwind (neighbour, weight) list
  *      in level-position order (about 16 + 4 + 12 x 8 B per cell-direction in
  *      FP32, twice the entry size in FP64), one lane per task.
- *  SSW_TRANSPORT_MODE_AUTO       compiled when its arrays fit in half of the
- *      free device memory at ssw_upload_grid, else geometric (default).
+ *  SSW_TRANSPORT_MODE_AUTO       compiled when its peak need (schedule
+ *      arrays, then active-set lists and transport state) is at most 85% of
+ *      the free device memory at ssw_upload_grid, else geometric (default).
  * Values 1 and 2 selected a persistent one-launch-per-sweep kernel that did
  * not terminate on MI355X; it has been removed and they return SSW_ERR_INVALID. */
 #define SSW_TRANSPORT_MODE_AUTO 0
@@ -224,8 +226,11 @@ double ssw_chemistry_timestep(double* state, double density, double volume, doub
 
 /* Device memory a handle would hold for a problem of num_cells cells,
  * num_faces directed faces and num_periodic_cells cells with a periodic face,
- * in the given transport mode (GEOMETRIC or COMPILED): an upper estimate used
- * for admission against an HBM budget. No counterpart in the reference. */
+ * in the given transport mode (GEOMETRIC or COMPILED): an upper estimate of
+ * the handle's peak from ssw_upload_grid through ssw_step and ssw_download,
+ * used for admission against an HBM budget (the diagnostic downloads of
+ * counts, level sets, fluxes and transport state allocate transient arrays
+ * beyond it). No counterpart in the reference. */
 int64_t ssw_estimate_device_bytes(const ssw_sweep_parameters* params, int64_t num_cells, int64_t num_faces,
                                   int64_t num_periodic_cells, int32_t transport_mode);
 
@@ -277,7 +282,7 @@ typedef struct ssw_request_result {
     double start_s;                /* admitted                                */
     double end_s;                  /* results downloaded, memory released     */
     int64_t updates;               /* cell-direction updates of all steps     */
-    int64_t device_bytes;          /* device memory its handle held           */
+    int64_t device_bytes;          /* most device memory its handle held      */
     int64_t estimated_bytes;       /* the estimate it was admitted with       */
     char error[256];
 } ssw_request_result;

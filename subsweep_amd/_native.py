@@ -70,6 +70,7 @@ class InfoC(C.Structure):
         ("graph_replay", C.c_int32),
         ("locality_order", C.c_int32),
         ("reserved1", C.c_int32),
+        ("peak_device_bytes", C.c_int64),
     ]
 
 

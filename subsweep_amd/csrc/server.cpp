@@ -98,7 +98,7 @@ void run_request(Req* q) {
         rc = ssw_download(s, r.out_ionized_fraction, r.out_temperature, r.out_timestep, r.out_photon_rate, r.out_level,
                           r.out_change_timescale);
     if (rc == SSW_OK) rc = ssw_get_info(s, &info);
-    if (rc == SSW_OK) q->device_bytes = info.device_bytes;
+    if (rc == SSW_OK) q->device_bytes = info.peak_device_bytes;
     if (rc != SSW_OK) fail_with(rc);
     ssw_destroy(s);
 }

@@ -885,7 +885,8 @@ struct ssw_sweep {
     double aref = 1.0;
     double dw[3] = {0, 0, 0};
     int64_t ovf_total = 0;
-    // Kahn layering scratch (full graph at upload, restricted sets for the hook)
+    // Kahn layering scratch (full graph at upload, restricted sets for the
+    // hooks): flat and cnt are held only while a layering or count runs
     uint4* d_flat = nullptr;
     uint32_t *d_lstart = nullptr, *d_ctr = nullptr, *d_cnt = nullptr, *d_need = nullptr;
     int kcap = 0;
@@ -920,7 +921,7 @@ struct ssw_sweep {
     unsigned long long* d_fail = nullptr;
     double lref = 1.0;
     TimestepState ts;
-    int64_t bytes = 0;
+    int64_t bytes = 0, peak_bytes = 0;
     std::unordered_map<void*, size_t> alloc;
     int64_t sweeps_last = 0;
     int64_t builds_last = 0;
@@ -959,6 +960,7 @@ int dalloc(ssw_sweep* s, T** p, size_t count) {
                                      " bytes failed: " + hipGetErrorString(e));
     }
     s->bytes += (int64_t)(count * sizeof(T));
+    s->peak_bytes = std::max(s->peak_bytes, s->bytes);
     s->alloc[(void*)*p] = count * sizeof(T);
     return SSW_OK;
 }
@@ -1085,6 +1087,20 @@ void drop_all_lists(ssw_sweep* s) {
     for (int l = 0; l < (int)s->list_ok.size(); l++) drop_list(s, l);
 }
 
+// The layering scratch: counters (cnt) and, with `flat`, the level-major task
+// records of a layering.
+int ensure_layering_scratch(ssw_sweep* s, bool flat = true) {
+    const size_t DN = (size_t)s->D * s->N;
+    if (!s->d_cnt) TRY(dalloc(s, &s->d_cnt, DN));
+    if (flat && !s->d_flat) TRY(dalloc(s, &s->d_flat, DN));
+    return SSW_OK;
+}
+
+void release_layering_scratch(ssw_sweep* s) {
+    dfree(s, s->d_flat);
+    dfree(s, s->d_cnt);
+}
+
 // Kahn layering of the active set {level >= lvl} into the scratch records:
 // counting, then one launch per level set until the queue is empty, reading
 // each level's width back (the depth and widths are not known in advance).
@@ -1130,34 +1146,41 @@ int build_schedule(ssw_sweep* s) {
     const size_t DN = (size_t)s->D * s->N;
     s->kcap = (int)std::min<int64_t>(s->N + 2, 1 << 22);
     TRY(dalloc(s, &s->d_lstart, (size_t)s->kcap + 2));
+    TRY(ensure_layering_scratch(s));
     int depth = 0;
     TRY(kahn_layering(s, 0, (int64_t)DN, &depth, &s->full_start));
     s->kmax = depth;
-    // sort the tasks by depth, stable over t = d*N + c
-    uint32_t *d_depth = nullptr, *d_keys = nullptr, *d_vals = nullptr, *d_sorted = nullptr;
+    dfree(s, s->d_cnt);
+    // sort the tasks by depth, stable over t = d*N + c (double-buffered: four
+    // DN-word arrays and a small temporary)
+    uint32_t *d_depth = nullptr, *d_keys = nullptr, *d_vals = nullptr, *d_vals2 = nullptr;
     TRY(dalloc(s, &d_depth, DN));
-    TRY(dalloc(s, &d_keys, DN));
-    TRY(dalloc(s, &d_vals, DN));
-    TRY(dalloc(s, &d_sorted, DN));
     Geo g = make_geo(s);
     k_scatter_depth<<<grid_for(std::min<int64_t>((int64_t)DN, 1 << 22)), kBlock, 0, s->stream>>>(g, s->kmax,
                                                                                                   d_depth);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    dfree(s, s->d_flat);
+    TRY(dalloc(s, &d_keys, DN));
+    TRY(dalloc(s, &d_vals, DN));
+    TRY(dalloc(s, &d_vals2, DN));
     k_iota<<<grid_for((int64_t)DN), kBlock, 0, s->stream>>>(d_vals, (uint32_t)DN);
     HIP_TRY(hipGetLastError());
     int bits = 1;
     while ((1ll << bits) <= s->kmax) bits++;
+    hipcub::DoubleBuffer<uint32_t> kb(d_depth, d_keys), vb(d_vals, d_vals2);
     size_t tmp_bytes = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, d_depth, d_keys, d_vals, d_sorted, (int)DN, 0,
-                                               bits, s->stream));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, kb, vb, (int)DN, 0, bits, s->stream));
     uint8_t* d_tmp = nullptr;
     TRY(dalloc(s, &d_tmp, tmp_bytes));
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp_bytes, d_depth, d_keys, d_vals, d_sorted, (int)DN, 0, bits,
-                                               s->stream));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp_bytes, kb, vb, (int)DN, 0, bits, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
+    uint32_t* d_sorted = vb.Current();
     dfree(s, d_tmp);
     dfree(s, d_keys);
-    dfree(s, d_vals);
     dfree(s, d_depth);
+    if (d_sorted == d_vals) dfree(s, d_vals2);
+    else dfree(s, d_vals);
     TRY(dalloc(s, &s->d_fstart, s->full_start.size()));
     HIP_TRY(hipMemcpyAsync(s->d_fstart, s->full_start.data(), s->full_start.size() * 4, hipMemcpyHostToDevice,
                            s->stream));
@@ -1182,6 +1205,7 @@ int build_schedule(ssw_sweep* s) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->stream));
     dfree(s, d_sorted);
+    dfree(s, s->d_ovf_start);  // copied into the headers
     // chunk table: chunks of at most kChunk tasks inside one level set
     std::vector<uint4> chunks;
     std::vector<uint32_t> first(s->kmax);
@@ -1774,14 +1798,18 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     s->ovf_total = (int64_t)hb[4];
     // transport mode: compiled when its arrays fit (automatic: within half of
     // the free device memory), else geometric
+    // transport mode: compiled when, at its peak (schedule arrays, overflow
+    // starts and sorted order while the schedule is written; then the
+    // active-set lists and the transport state), it needs at most 85% of the
+    // free device memory (automatic), else geometric
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-    const size_t comp_bytes =
-        DN * (16 + real_size(s) + kPlanes * entry_size(s) + 4) + (size_t)s->ovf_total * entry_size(s);
+    const size_t rs = real_size(s);
+    const size_t comp_bytes = DN * (16 + rs + kPlanes * entry_size(s)) + (size_t)s->ovf_total * entry_size(s);
+    const size_t comp_need = comp_bytes + DN * std::max<size_t>(8, 4 * (size_t)(L - 1) + 2 * rs);
     if (s->mode_request == SSW_TRANSPORT_MODE_COMPILED) s->compiled = true;
     else if (s->mode_request == SSW_TRANSPORT_MODE_GEOMETRIC) s->compiled = false;
-    else s->compiled = comp_bytes <= free_b / 2;
-    TRY(dalloc(s, &s->d_flat, DN));
+    else s->compiled = (double)comp_need <= 0.85 * (double)free_b;
     if (s->compiled) TRY(setup_overflow(s, s->d_cnt));
     TRY(build_schedule(s));
     s->have_grid = true;
@@ -1903,11 +1931,13 @@ int ssw_download_counts(ssw_sweep* s, int32_t lvl, uint32_t* counts) {
     if (!s || !counts) return fail(SSW_ERR_INVALID, "null argument");
     if (!s->have_grid) return fail(SSW_ERR_INVALID, "ssw_upload_grid must be called first");
     if (lvl < 0 || lvl >= s->p.num_timestep_levels) return fail(SSW_ERR_INVALID, "sweep_level out of range");
+    TRY(ensure_layering_scratch(s, false));
     Geo g = make_geo(s);
     k_count<<<grid_for((int64_t)s->N * s->D), kBlock, 0, s->stream>>>(g, lvl, 1);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(counts, s->d_cnt, (size_t)s->D * s->N * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
+    release_layering_scratch(s);
     to_caller(s, counts, s->D);
     return SSW_OK;
 }
@@ -1924,6 +1954,7 @@ int ssw_download_levelsets(ssw_sweep* s, int32_t lvl, uint32_t* depth) {
     int64_t active = 0;
     for (uint8_t v : lev) active += v >= lvl;
     int k = 0;
+    TRY(ensure_layering_scratch(s));
     TRY(kahn_layering(s, lvl, active * s->D, &k, nullptr));
     uint32_t* d_depth = nullptr;
     const int64_t DN = (int64_t)s->D * s->N;
@@ -1935,6 +1966,7 @@ int ssw_download_levelsets(ssw_sweep* s, int32_t lvl, uint32_t* depth) {
     HIP_TRY(hipMemcpyAsync(depth, d_depth, (size_t)DN * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     dfree(s, d_depth);
+    release_layering_scratch(s);
     to_caller(s, depth, s->D);
     return SSW_OK;
 }
@@ -1994,6 +2026,7 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info) {
     info->downwind_periodic = s->dw[1];
     info->downwind_boundary = s->dw[2];
     info->device_bytes = s->bytes;
+    info->peak_device_bytes = s->peak_bytes;
     info->transport_mode = s->have_grid ? (s->compiled ? SSW_TRANSPORT_MODE_COMPILED : SSW_TRANSPORT_MODE_GEOMETRIC)
                                         : s->mode_request;
     info->list_builds_last_step = (int32_t)s->builds_last;
@@ -2047,15 +2080,20 @@ int64_t ssw_estimate_device_bytes(const ssw_sweep_parameters* p, int64_t n, int6
     const int64_t rs = p->transport_precision == SSW_TRANSPORT_FP32 ? 4 : 8;
     const int64_t es = rs == 4 ? 8 : 16;
     const int64_t DN = D * n;
-    int64_t b = 0;
-    b += (n + 1) * 4 + F * 32 + F * 4 + n * 16 + n * 4 + num_periodic_cells * 8 + F * 4 + n + DN * 4;  // grid, counts
-    b += DN * 16 + (n + 4) * 4;                    // layering records and level starts
-    b += DN * 16;                                  // schedule build: depth, keys, values, sorted (transient)
-    b += (mode == SSW_TRANSPORT_MODE_COMPILED) ? DN * (16 + rs + kPlanes * es + 4) + DN * es / 16 : DN * 16;
-    b += (L - 1) * DN * 4;                         // active-set lists
-    b += DN / kChunk * 40 + (1 << 20);             // chunk table, scan scratch
-    b += n * 8 * 9 + 2 * DN * rs + D * num_periodic_cells * rs;
-    b += n * 8;                                    // photon-rate download
+    const bool comp = mode == SSW_TRANSPORT_MODE_COMPILED;
+    const int64_t ovf = comp ? DN * es / 16 : 0;   // overflow entries (bound used for the estimate)
+    // held from the grid upload on: grid arrays, level starts, chunk table
+    const int64_t grid = (n + 1) * 4 + F * 32 + F * 4 + n * 16 + n * 4 + num_periodic_cells * 8 + F * 4 + n +
+                         (n + 4) * 4 + DN / kChunk * 40 + (1 << 20);
+    const int64_t sched = comp ? DN * (16 + rs + kPlanes * es) + ovf : DN * 16;
+    // transient peak of the schedule build: counters + layering records (+
+    // overflow starts), the four sort arrays, or the schedule being written
+    // with the sorted order and overflow starts
+    const int64_t ostart = comp ? DN * 4 + ovf : 0;
+    const int64_t build = std::max(DN * 20 + ostart, std::max(DN * 16 + ostart, sched + DN * 4 + ostart));
+    // afterwards: schedule, active-set lists, state, the photon-rate download
+    const int64_t run = sched + (L - 1) * DN * 4 + n * 8 * 9 + 2 * DN * rs + D * num_periodic_cells * rs + n * 8;
+    const int64_t b = grid + std::max(build, run);
     return b + b / 20 + (1 << 20);
 }
 

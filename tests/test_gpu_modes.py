@@ -141,3 +141,23 @@ def test_locality_order_bitwise(mode, hip_available):
     for step, (a, b) in enumerate(zip(*out)):
         for k, (u, v) in enumerate(zip(a, b)):
             assert np.array_equal(u.view(np.uint8), v.view(np.uint8)), f"step {step + 1}, field {k}"
+
+
+@pytest.mark.parametrize("mode", [N.TRANSPORT_MODE_GEOMETRIC, N.TRANSPORT_MODE_COMPILED])
+def test_device_memory_within_estimate(mode, hip_available):
+    # the handle's peak over upload, steps and the result download stays
+    # within the admission estimate; the layering scratch of the diagnostic
+    # hooks is held only while they run
+    from subsweep_amd.serve import estimate_device_bytes
+    pb = W.jittered_problem(10, 1.0, "6", 84, levels=3, periodic=True, steps=2, precision=32)
+    s = Sweep(pb.params, pb.grid, *pb.state(), transport_mode=mode)
+    for _ in range(2):
+        s.run_sweeps()
+    s.results()
+    info = s.info()
+    held = info.device_bytes
+    assert held <= info.peak_device_bytes <= estimate_device_bytes(pb, mode)
+    s.levelsets(1)
+    s.counts(0)
+    assert s.info().device_bytes == held
+    s.close()
