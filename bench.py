@@ -39,7 +39,9 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 KERNELS = {3: "k_sweep_rec (geometric replay: one launch per Kahn level set, all directions, 16 lanes per task)",
-           4: "k_sweep_comp (compiled replay: one launch per Kahn level set, all directions, one lane per task)"}
+           4: "k_sweep_comp (compiled replay: one launch per Kahn level set, all directions, one lane per task; "
+              "with narrow runs, k_sweep_comp_run solves each run of level sets of at most 1024 tasks in one "
+              "workgroup, and a launch is either)"}
 
 
 def log(*a):
@@ -54,11 +56,13 @@ def bytes_per_update(info, precision: int) -> float:
     return base + info.downwind_local * bl + info.downwind_periodic * bp + info.downwind_boundary * bb
 
 
-def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3, mode: int = 0, block: int = 0,
-            graphs: bool = True, progress: bool = False):
+def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3, shape: dict = None,
+            progress: bool = False):
+    """`shape`: Sweep launch-shape keywords (transport_mode, replay_block,
+    graph_replay, narrow_runs); defaults when None."""
     from subsweep_amd import Sweep
     pp = pb.with_params(transport_precision=precision)
-    s = Sweep(pp.params, pp.grid, *pp.state(), transport_mode=mode, replay_block=block, graph_replay=graphs)
+    s = Sweep(pp.params, pp.grid, *pp.state(), **(shape or {}))
     for _ in range(warmup):
         s.run_sweeps()
     s.synchronize()
@@ -72,8 +76,8 @@ def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3,
             log(f"  step {k + 1}/{steps} at {time.perf_counter() - t0:.1f} s")
     s.synchronize()
     t1 = time.perf_counter()
-    # kernel timing: the same steps again with HIP events around every launch
-    # of the transport kernel on the library's stream
+    # kernel timing: further steps with HIP events around each sweep's whole
+    # transport replay on the library's stream
     s.set_timing(True)
     kernel_ms, launches, prof_updates = 0.0, 0, 0
     for _ in range(profile_steps):
@@ -103,6 +107,7 @@ def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3,
         "kernel": KERNELS.get(info.transport_mode, "?"),
         "replay_block": int(info.replay_block),
         "graph_replay": bool(info.graph_replay),
+        "narrow_runs": bool(info.narrow_runs),
         "transport_mode": int(info.transport_mode),
         "info": info,
         "results": res,
@@ -155,16 +160,17 @@ def run_largest(steps: int = 10):
 
 
 def tuned(pb, precision: int, warmup: int, enabled: bool, mode: int, block: int):
-    """(mode, block, graphs) for a run: the autotuner's choice, or the command line's."""
+    """(launch shape, autotune table) for a run: the autotuner's choice, or the
+    command line's."""
     if not enabled:
-        return mode, block, True, None
+        return {"transport_mode": mode, "replay_block": block}, None
     from subsweep_amd.autotune import autotune
     t = time.perf_counter()
     r = autotune(pb.with_params(transport_precision=precision), warmup=warmup)
     log(f"autotune fp{precision}: mode {r.transport_mode}, block {r.replay_block}, graphs {r.graph_replay}, "
-        f"{r.ms_per_step:.2f} ms/step ({time.perf_counter() - t:.1f} s; " +
-        ", ".join(f"m{m}/b{b}/g{int(gr)} {ms:.2f}" for m, b, gr, ms in r.table) + ")")
-    return r.transport_mode, r.replay_block, r.graph_replay, [list(x) for x in r.table]
+        f"narrow runs {r.narrow_runs}, {r.ms_per_step:.2f} ms/step ({time.perf_counter() - t:.1f} s; " +
+        ", ".join(f"m{m}/b{b}/g{int(gr)}/n{int(nr)} {ms:.2f}" for m, b, gr, nr, ms in r.table) + ")")
+    return r.kwargs(), [list(x) for x in r.table]
 
 
 def run_clustered(steps: int = 4, warmup: int = 2, autotune: bool = True):
@@ -176,11 +182,11 @@ def run_clustered(steps: int = 4, warmup: int = 2, autotune: bool = True):
     pb = W.clustered_2m()
     log(f"clustered grid: {pb.grid.num_cells} cells, {pb.grid.num_faces} faces, built in "
         f"{time.perf_counter() - t:.1f} s")
-    m32, b32, gr32, t32 = tuned(pb, 32, warmup, autotune, 0, 0)
-    r32 = run_gpu(pb, 32, steps, warmup, profile_steps=1, mode=m32, block=b32, graphs=gr32)
+    sh32, t32 = tuned(pb, 32, warmup, autotune, 0, 0)
+    r32 = run_gpu(pb, 32, steps, warmup, profile_steps=1, shape=sh32)
     log(f"clustered fp32: {r32['updates_per_s'] / 1e9:.3f} G updates/s, {r32['ms_per_step']:.1f} ms/step")
-    m64, b64, gr64, t64 = tuned(pb, 64, warmup, autotune, 0, 0)
-    r64 = run_gpu(pb, 64, steps, warmup, profile_steps=1, mode=m64, block=b64, graphs=gr64)
+    sh64, t64 = tuned(pb, 64, warmup, autotune, 0, 0)
+    r64 = run_gpu(pb, 64, steps, warmup, profile_steps=1, shape=sh64)
     log(f"clustered fp64: {r64['updates_per_s'] / 1e9:.3f} G updates/s, {r64['ms_per_step']:.1f} ms/step")
     return {
         "workload": "2097152-cell clustered lognormal-field Voronoi grid (sigma_ln 2), 84 directions, "
@@ -376,13 +382,13 @@ def main():
     tune = not args.no_autotune and args.precision == 0
     tune32 = tune64 = None
     if args.precision in (0, 32):
-        m, b, gr, tune32 = tuned(pb, 32, args.warmup, tune, args.mode, args.block)
-        fp32 = run_gpu(pb, 32, args.steps, args.warmup, mode=m, block=b, graphs=gr)
+        sh, tune32 = tuned(pb, 32, args.warmup, tune, args.mode, args.block)
+        fp32 = run_gpu(pb, 32, args.steps, args.warmup, shape=sh)
         log(f"fp32 transport: {fp32['updates_per_s'] / 1e9:.3f} G updates/s, {fp32['ms_per_step']:.2f} ms/step, "
             f"kernel {fp32['avg_launch_ms']:.3f} ms/launch, {fp32['achieved_gbs']:.0f} GB/s")
     if args.precision in (0, 64):
-        m, b, gr, tune64 = tuned(pb, 64, args.warmup, tune, args.mode, args.block)
-        fp64 = run_gpu(pb, 64, args.steps, args.warmup, mode=m, block=b, graphs=gr)
+        sh, tune64 = tuned(pb, 64, args.warmup, tune, args.mode, args.block)
+        fp64 = run_gpu(pb, 64, args.steps, args.warmup, shape=sh)
         log(f"fp64 transport: {fp64['updates_per_s'] / 1e9:.3f} G updates/s, {fp64['ms_per_step']:.2f} ms/step, "
             f"kernel {fp64['avg_launch_ms']:.3f} ms/launch, {fp64['achieved_gbs']:.0f} GB/s")
     if args.precision != 0:  # profiling runs: no JSON line
@@ -455,9 +461,11 @@ def main():
             "fp64_bytes_per_update": fp64["bytes_per_update"],
         },
         "launch_shape": {"fp32": {"replay_block": fp32["replay_block"], "graph_replay": fp32["graph_replay"],
-                                  "transport_mode": fp32["transport_mode"], "autotune_ms_per_step": tune32},
+                                  "narrow_runs": fp32["narrow_runs"], "transport_mode": fp32["transport_mode"],
+                                  "autotune_ms_per_step": tune32},
                          "fp64": {"replay_block": fp64["replay_block"], "graph_replay": fp64["graph_replay"],
-                                  "transport_mode": fp64["transport_mode"], "autotune_ms_per_step": tune64}},
+                                  "narrow_runs": fp64["narrow_runs"], "transport_mode": fp64["transport_mode"],
+                                  "autotune_ms_per_step": tune64}},
         "cpu_baseline": cpu,
         "fp32_error_vs_fp64": field_errors(fp32["results"], fp64["results"]),
         "clustered_2m": clustered,

@@ -151,7 +151,8 @@ typedef struct ssw_info {
     int32_t replay_block;          /* compiled replay block size               */
     int32_t graph_replay;          /* 1 if replays are captured as graphs      */
     int32_t locality_order;        /* 1 if cells are stored in position order  */
-    int32_t reserved1;
+    int32_t narrow_runs;           /* 1 if runs of narrow level sets replay in
+                                      one workgroup (compiled replay)          */
     int64_t peak_device_bytes;     /* most device memory the handle has held   */
 } ssw_info;
 
@@ -195,6 +196,15 @@ int ssw_set_cell_positions(ssw_sweep* s, int64_t num_cells, const double* positi
  * (other threads' allocations invalidate a stream capture), so the request
  * loop turns it off. No counterpart in the reference. */
 int ssw_set_graph_replay(ssw_sweep* s, int32_t enabled);
+
+/* Compiled replay: each run of two or more consecutive level sets with at
+ * most 1024 active tasks each (the narrow ends of the dependency graph) is
+ * solved by one 1024-lane workgroup in one launch, level set after level set
+ * with a workgroup barrier in between, instead of one launch per level set
+ * (enabled = 1, default; 0 = one launch per level set). Same arithmetic per
+ * task, bitwise the same results. Set before ssw_upload_grid. No counterpart
+ * in the reference. */
+int ssw_set_narrow_runs(ssw_sweep* s, int32_t enabled);
 
 /* Raw transport state in its storage precision and scale (FP32 or FP64 per
  * the transport precision, direction-major): X = outgoing rate / downwind

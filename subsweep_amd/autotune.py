@@ -8,7 +8,9 @@ replayed:
 * replay block - the thread-block size of the compiled replay (64, 128 or
   256 tasks per block);
 * graph replay - level-set launches captured as one HIP graph, or launched
-  one at a time.
+  one at a time;
+* narrow runs - runs of narrow level sets solved by one workgroup per run,
+  or one launch per level set (compiled replay).
 
 `autotune` times a few steps of each candidate on the problem itself and
 returns the fastest. Every candidate computes bitwise the same results, so
@@ -30,30 +32,32 @@ class TuneResult:
     transport_mode: int
     replay_block: int
     graph_replay: bool
+    narrow_runs: bool
     ms_per_step: float
-    table: list  # (mode, block, graphs, ms per step) of every candidate
+    table: list  # (mode, block, graphs, narrow runs, ms per step) of every candidate
 
     def kwargs(self) -> dict:
         return {"transport_mode": self.transport_mode, "replay_block": self.replay_block,
-                "graph_replay": self.graph_replay}
+                "graph_replay": self.graph_replay, "narrow_runs": self.narrow_runs}
 
 
 def candidates(blocks=(64, 128, 256)):
-    out = [(N.TRANSPORT_MODE_COMPILED, b, True) for b in blocks]
-    out.append((N.TRANSPORT_MODE_COMPILED, blocks[0], False))
-    out.append((N.TRANSPORT_MODE_GEOMETRIC, 0, True))  # one launch shape (16 lanes per task)
+    out = [(N.TRANSPORT_MODE_COMPILED, b, True, True) for b in blocks]
+    out.append((N.TRANSPORT_MODE_COMPILED, blocks[0], False, True))
+    out.append((N.TRANSPORT_MODE_COMPILED, blocks[0], True, False))
+    out.append((N.TRANSPORT_MODE_GEOMETRIC, 0, True, False))  # one launch shape (16 lanes per task)
     return out
 
 
 def autotune(problem: Problem, cands=None, steps: int = 2, warmup: int = 2) -> TuneResult:
-    """Times `steps` steps after `warmup` steps of every (mode, block, graphs)
-    candidate on a fresh handle each; candidates that do not fit in device
-    memory are skipped."""
+    """Times `steps` steps after `warmup` steps of every (mode, block, graphs,
+    narrow runs) candidate on a fresh handle each; candidates that do not fit
+    in device memory are skipped."""
     table = []
-    for m, b, gr in (cands or candidates()):
+    for m, b, gr, nr in (cands or candidates()):
         try:
             s = Sweep(problem.params, problem.grid, *problem.state(), transport_mode=m, replay_block=b,
-                      graph_replay=gr)
+                      graph_replay=gr, narrow_runs=nr)
         except N.SweepError as e:
             if e.code == N.SSW_ERR_OOM:
                 continue
@@ -66,10 +70,10 @@ def autotune(problem: Problem, cands=None, steps: int = 2, warmup: int = 2) -> T
             for _ in range(steps):
                 s.run_sweeps()
             s.synchronize()
-            table.append((m, b, gr, (time.perf_counter() - t0) / steps * 1e3))
+            table.append((m, b, gr, nr, (time.perf_counter() - t0) / steps * 1e3))
         finally:
             s.close()
     if not table:
         raise RuntimeError("no autotuning candidate fits on the device")
-    m, b, gr, ms = min(table, key=lambda r: r[3])
-    return TuneResult(m, b, gr, ms, table)
+    m, b, gr, nr, ms = min(table, key=lambda r: r[4])
+    return TuneResult(m, b, gr, nr, ms, table)

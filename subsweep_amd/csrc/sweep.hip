@@ -80,7 +80,8 @@ constexpr uint32_t kCellMask = (1u << kCellBits) - 1u;
 constexpr int kMaxFaces = 127;             // faces per cell
 constexpr int kQueueCap = kTasksPerBlock * kMaxFaces;
 constexpr int kPlanes = 12;                // upwind entries per task held in planes
-constexpr int kChunk = 4096;               // compaction chunk (tasks, within one level set)
+constexpr int kChunk = 4096;
+constexpr int kRunBlock = 1024;            // workgroup of a run of narrow level sets (tasks per level set)               // compaction chunk (tasks, within one level set)
 
 thread_local std::string g_err;
 // Handles alive in the process. Replay graphs are captured only while a single
@@ -629,17 +630,13 @@ __global__ void __launch_bounds__(kBlock) k_sweep_rec(Geo g, Flux<R> fl, const u
     if (valid && sub == 0) finish_task<R>(fl, (size_t)d * g.N + c, inc, ci, (R)A);
 }
 
-// Compiled replay of one level set of width W: one lane per task. The header,
-// A and the plane entries of a task are independent loads; the upwind X
-// gathers and the cell inputs follow. With an index list, lane i solves the
-// task at position idx[i] of the level set.
-template <typename R, int kRecBlock>
-__global__ void __launch_bounds__(kRecBlock) k_sweep_comp(Geo g, Flux<R> fl, const uint4* hdr, const R* Av,
-                                                          const Entry<R>* planes, const Entry<R>* ovf, uint32_t W,
-                                                          const uint32_t* idx, uint32_t count) {
-    const uint32_t i = blockIdx.x * kRecBlock + threadIdx.x;
-    if (i >= count) return;
-    const uint32_t p = idx ? idx[i] : i;
+// One task of the compiled replay: the task at position p of a level set of
+// width W whose header, A and plane entries start at hdr, Av and planes. The
+// header, A and the plane entries are independent loads; the upwind X
+// gathers and the cell inputs follow.
+template <typename R>
+__device__ __forceinline__ void comp_task(const Geo& g, const Flux<R>& fl, const uint4* hdr, const R* Av,
+                                          const Entry<R>* planes, const Entry<R>* ovf, uint32_t W, uint32_t p) {
     // first memory step: the plane entries, the header and A (independent)
     Entry<R> e[kPlanes];
 #pragma unroll
@@ -662,6 +659,37 @@ __global__ void __launch_bounds__(kRecBlock) k_sweep_comp(Geo g, Flux<R> fl, con
         inc += o.w * fl.X[dN + o.nb];
     }
     finish_task<R>(fl, dN + c, inc, ci, A);
+}
+
+// Compiled replay of one level set of width W: one lane per task. With an
+// index list, lane i solves the task at position idx[i] of the level set.
+template <typename R, int kRecBlock>
+__global__ void __launch_bounds__(kRecBlock) k_sweep_comp(Geo g, Flux<R> fl, const uint4* hdr, const R* Av,
+                                                          const Entry<R>* planes, const Entry<R>* ovf, uint32_t W,
+                                                          const uint32_t* idx, uint32_t count) {
+    const uint32_t i = blockIdx.x * kRecBlock + threadIdx.x;
+    if (i >= count) return;
+    comp_task<R>(g, fl, hdr, Av, planes, ovf, W, idx ? idx[i] : i);
+}
+
+// Compiled replay of a run of consecutive narrow level sets (at most
+// kRunBlock active tasks each) by one workgroup, level set after level set,
+// with a workgroup barrier in between: every wave of the workgroup runs on
+// one CU, whose L1 is coherent within the workgroup, so the X stored for
+// one level set is seen by the next after the barrier. run[j] = {first
+// position b, width W, first index list entry, active count} of the run's
+// j-th level set; without an index list entry i is position i.
+template <typename R>
+__global__ void __launch_bounds__(kRunBlock) k_sweep_comp_run(Geo g, Flux<R> fl, const uint4* hdr, const R* Av,
+                                                              const Entry<R>* planes, const Entry<R>* ovf,
+                                                              const uint32_t* idx, const uint4* run, int levels) {
+    for (int j = 0; j < levels; j++) {
+        const uint4 r = run[j];
+        const uint32_t b = r.x, W = r.y, i = threadIdx.x;
+        if (i < r.w)
+            comp_task<R>(g, fl, hdr + b, Av + b, planes + (size_t)kPlanes * b, ovf, W, idx ? idx[r.z + i] : i);
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -869,6 +897,7 @@ struct ssw_sweep {
     int mode_request = SSW_TRANSPORT_MODE_AUTO;
     int rec_block = 64;                             // compiled replay block size
     bool graphs_on = true;                          // capture replays as graphs
+    bool narrow_runs = true;                        // runs of narrow level sets in one workgroup
     bool compiled = false;
     // grid
     int64_t N = 0, F = 0, Nper = 0;
@@ -912,6 +941,16 @@ struct ssw_sweep {
     std::vector<int> list_ok;                       // per sweep level: idx / cstart current
     std::vector<int> replays;                       // per sweep level: replays since the list was made
     std::vector<hipGraphExec_t> graphs;             // per sweep level: captured replay
+    // replay plan per sweep level: one launch per level set, or one launch per
+    // run of consecutive narrow level sets (compiled replay)
+    struct ReplayItem {
+        int k;          // level set (single launch) or first level set of the run
+        int levels;     // 0: single launch; else the run's non-empty level sets
+        uint32_t tab;   // the run's first entry in the run table
+    };
+    std::vector<std::vector<ReplayItem>> plan;
+    std::vector<std::vector<uint4>> runtab;         // host copies of the run tables
+    std::vector<uint4*> d_runtab;
     // state
     double *d_rho = nullptr, *d_x = nullptr, *d_T = nullptr, *d_src = nullptr, *d_att = nullptr;
     double *d_sts = nullptr, *d_prev = nullptr, *d_change = nullptr, *d_src_t = nullptr;
@@ -1138,6 +1177,57 @@ int kahn_layering(ssw_sweep* s, int lvl, int64_t expected, int* depth_out, std::
     }
 }
 
+// The replay plan of sweep level lvl from its level starts (the full
+// schedule's for level 0, the active-set list's otherwise): in the compiled
+// replay, each maximal run of at least two consecutive non-empty level sets of
+// at most kRunBlock active tasks (empty level sets inside a run are skipped)
+// becomes one single-workgroup launch over a run table entry per level set;
+// every other non-empty level set is one launch.
+int plan_replay(ssw_sweep* s, int lvl) {
+    const std::vector<uint32_t>& fs = s->full_start;
+    const std::vector<uint32_t>& cs = lvl == 0 ? fs : s->cstart[lvl];
+    std::vector<ssw_sweep::ReplayItem>& plan = s->plan[lvl];
+    std::vector<uint4>& tab = s->runtab[lvl];
+    plan.clear();
+    tab.clear();
+    const bool runs = s->compiled && s->narrow_runs;
+    int k = 0;
+    while (k < s->kmax) {
+        const uint32_t count = cs[k + 1] - cs[k];
+        if (count == 0) {
+            k++;
+            continue;
+        }
+        if (!runs || count > (uint32_t)kRunBlock) {
+            plan.push_back({k, 0, 0u});
+            k++;
+            continue;
+        }
+        const uint32_t t0 = (uint32_t)tab.size();
+        int k1 = k;
+        for (; k1 < s->kmax && cs[k1 + 1] - cs[k1] <= (uint32_t)kRunBlock; k1++) {
+            const uint32_t c = cs[k1 + 1] - cs[k1];
+            if (c) tab.push_back(make_uint4(fs[k1], fs[k1 + 1] - fs[k1], cs[k1], c));
+        }
+        const int n = (int)(tab.size() - t0);
+        if (n >= 2) {
+            plan.push_back({k, n, t0});
+        } else {
+            tab.resize(t0);
+            plan.push_back({k, 0, 0u});
+        }
+        k = k1;
+    }
+    if (!tab.empty()) {
+        TRY(dalloc(s, &s->d_runtab[lvl], tab.size()));
+        HIP_TRY(hipMemcpyAsync(s->d_runtab[lvl], tab.data(), tab.size() * sizeof(uint4), hipMemcpyHostToDevice,
+                               s->stream));
+    } else {
+        dfree(s, s->d_runtab[lvl]);
+    }
+    return SSW_OK;
+}
+
 // The full schedule, once per grid: Kahn layering of the whole graph, each
 // task's depth, a stable sort by depth of the tasks in (direction, cell)
 // order, then the compiled or geometric per-task data in that order, and the
@@ -1239,6 +1329,11 @@ int build_schedule(ssw_sweep* s) {
     for (hipGraphExec_t ge : s->graphs)
         if (ge) (void)hipGraphExecDestroy(ge);
     s->graphs.assign(L, nullptr);
+    for (uint4*& p : s->d_runtab) dfree(s, p);
+    s->d_runtab.assign(L, nullptr);
+    s->runtab.assign(L, std::vector<uint4>());
+    s->plan.assign(L, std::vector<ssw_sweep::ReplayItem>());
+    TRY(plan_replay(s, 0));
     return SSW_OK;
 }
 
@@ -1271,6 +1366,7 @@ int make_list(ssw_sweep* s, int lvl) {
     cs.resize(s->kmax + 1);
     HIP_TRY(hipMemcpyAsync(cs.data(), s->d_cstart, (s->kmax + 1) * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
+    TRY(plan_replay(s, lvl));
     s->list_ok[lvl] = 1;
     s->replays[lvl] = 0;
     if (s->graphs[lvl]) {
@@ -1291,17 +1387,24 @@ void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<
     const bool all = lvl == 0;
     const std::vector<uint32_t>& cs = all ? fs : s->cstart[lvl];
     const uint32_t* idx = all ? nullptr : s->d_idx[lvl];
-    for (int k = 0; k < s->kmax; k++) {
+    const Entry<R>* ovf = (const Entry<R>*)s->d_ovf;
+    for (const ssw_sweep::ReplayItem& it : s->plan[lvl]) {
+        hipEvent_t a = ev ? (*mark)() : nullptr;
+        if (it.levels > 0) {  // a run of narrow level sets, one workgroup
+            k_sweep_comp_run<R><<<1, kRunBlock, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A,
+                                                                 (const Entry<R>*)s->d_planes, ovf, idx,
+                                                                 s->d_runtab[lvl] + it.tab, it.levels);
+            if (ev) ev->push_back({a, (*mark)()});
+            continue;
+        }
+        const int k = it.k;
         const uint32_t count = cs[k + 1] - cs[k];
-        if (count == 0) continue;
         const uint32_t b = fs[k], W = fs[k + 1] - fs[k];
         const uint32_t* ik = idx ? idx + cs[k] : nullptr;
-        hipEvent_t a = ev ? (*mark)() : nullptr;
         if (s->compiled) {
             const uint4* hdr = s->d_hdr + b;
             const R* A = (const R*)s->d_A + b;
             const Entry<R>* planes = (const Entry<R>*)s->d_planes + (size_t)kPlanes * b;
-            const Entry<R>* ovf = (const Entry<R>*)s->d_ovf;
             switch (s->rec_block) {
                 case 128:
                     k_sweep_comp<R, 128><<<(count + 127) / 128, 128, 0, s->stream>>>(g, fl, hdr, A, planes, ovf, W,
@@ -1420,7 +1523,7 @@ int run_step(ssw_sweep* s, double* elapsed) {
         }
         // timing: one event pair around the sweep's whole replay (per-launch
         // events would add their own packets between the launches); the
-        // average launch duration is the total over the non-empty level sets
+        // average launch duration is the total over the replay's launches
         hipEvent_t t0 = s->timing ? mark() : nullptr;
         if (s->graphs_on && !s->graphs[lvl] && s->replays[lvl] >= 1 && g_live_handles.load() == 1)
             capture(s, g, lvl);
@@ -1428,8 +1531,7 @@ int run_step(ssw_sweep* s, double* elapsed) {
         else launch_replay(s, g, lvl, nullptr);
         if (s->timing) {
             tr.push_back({t0, mark()});
-            const std::vector<uint32_t>& cs = lvl == 0 ? s->full_start : s->cstart[lvl];
-            for (int k = 0; k < s->kmax; k++) s->n_transport += cs[k + 1] > cs[k] ? 1 : 0;
+            s->n_transport += (int64_t)s->plan[lvl].size();
         }
         HIP_TRY(hipGetLastError());
         s->replays[lvl]++;
@@ -1485,6 +1587,7 @@ void free_state(ssw_sweep* s) {
 void free_grid(ssw_sweep* s) {
     drop_all_lists(s);
     for (uint32_t*& p : s->d_idx) dfree(s, p);
+    for (uint4*& p : s->d_runtab) dfree(s, p);
     dfree(s, s->d_off);
     dfree(s, s->d_geo);
     dfree(s, s->d_nk);
@@ -1675,6 +1778,13 @@ int ssw_set_cell_positions(ssw_sweep* s, int64_t n, const double* positions) {
         return SSW_OK;
     }
     s->pos.assign(positions, positions + 3 * n);
+    return SSW_OK;
+}
+
+int ssw_set_narrow_runs(ssw_sweep* s, int32_t enabled) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    if (s->have_grid) return fail(SSW_ERR_INVALID, "ssw_set_narrow_runs must be called before ssw_upload_grid");
+    s->narrow_runs = enabled != 0;
     return SSW_OK;
 }
 
@@ -2034,6 +2144,7 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info) {
     info->graph_capture_failures = s->capture_failures;
     info->replay_block = s->rec_block;
     info->graph_replay = s->graphs_on ? 1 : 0;
+    info->narrow_runs = (s->compiled && s->narrow_runs) ? 1 : 0;
     info->locality_order = s->permuted ? 1 : 0;
     if (s->have_state) {
         unsigned long long f = 0, a = 0;

@@ -161,3 +161,25 @@ def test_device_memory_within_estimate(mode, hip_available):
     s.counts(0)
     assert s.info().device_bytes == held
     s.close()
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("levels", [1, 4])
+def test_narrow_runs_bitwise(precision, levels, hip_available):
+    # runs of narrow level sets in one workgroup vs one launch per level set;
+    # with 4 levels the restricted sweeps replay through active-set lists,
+    # whose level sets are mostly narrow
+    pb = W.jittered_problem(12, 1.0, "8", 84, levels=levels, periodic=True, steps=5, precision=precision)
+    pb = pb.with_params(max_timestep=1e-3 * 3.15576e13)
+    out = []
+    for narrow in (True, False):
+        s = Sweep(pb.params, pb.grid, *pb.state(), transport_mode=N.TRANSPORT_MODE_COMPILED, narrow_runs=narrow)
+        assert s.info().narrow_runs == (1 if narrow else 0)
+        seq = []
+        for _ in range(pb.steps):
+            s.run_sweeps()
+            r = s.results(photon_rate=False)
+            seq.append((*s.transport_state(), r.ionized_hydrogen_fraction, r.temperature, r.level))
+        s.close()
+        out.append(seq)
+    _assert_same(out[1], out[0], "narrow runs off vs on")
