@@ -115,9 +115,10 @@ def test_autotune_picks_a_measured_candidate(hip_available):
     from subsweep_amd.autotune import autotune, candidates
     pb = W.jittered_problem(8, 1.0, "4", 16, levels=1, periodic=True, steps=2, precision=32)
     r = autotune(pb, cands=candidates((64, 256)), steps=1, warmup=2)
-    assert len(r.table) == 4
-    assert (r.transport_mode, r.replay_block, r.graph_replay, r.ms_per_step) in [tuple(x) for x in r.table]
-    assert r.ms_per_step == min(x[3] for x in r.table)
+    assert len(r.table) == 5
+    assert (r.transport_mode, r.replay_block, r.graph_replay, r.narrow_runs, r.ms_per_step) in \
+        [tuple(x) for x in r.table]
+    assert r.ms_per_step == min(x[4] for x in r.table)
 
 
 @pytest.mark.parametrize("mode", [N.TRANSPORT_MODE_GEOMETRIC, N.TRANSPORT_MODE_COMPILED])
