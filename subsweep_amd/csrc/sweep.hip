@@ -80,8 +80,8 @@ constexpr uint32_t kCellMask = (1u << kCellBits) - 1u;
 constexpr int kMaxFaces = 127;             // faces per cell
 constexpr int kQueueCap = kTasksPerBlock * kMaxFaces;
 constexpr int kPlanes = 12;                // upwind entries per task held in planes
-constexpr int kChunk = 4096;
-constexpr int kRunBlock = 1024;            // workgroup of a run of narrow level sets (tasks per level set)               // compaction chunk (tasks, within one level set)
+constexpr int kChunk = 4096;               // compaction chunk (tasks, within one level set)
+constexpr int kRunBlock = 1024;            // workgroup of a run of narrow level sets (tasks per level set)
 
 thread_local std::string g_err;
 // Handles alive in the process. Replay graphs are captured only while a single
@@ -154,7 +154,8 @@ struct Geo {
 
 template <typename R>
 struct Flux {
-    R* X;                    // D x N
+    R* X;                    // D x N, at xidx(d, c)
+    const uint32_t* xpos;    // compiled replay: X of (c, d) at the task's schedule position xpos[d*N + c]
     R* I;                    // D x N
     R* P;                    // D x Nper
     const double* src_over_d;
@@ -163,6 +164,15 @@ struct Flux {
     double inv_lref;         // storage scale of rates
     double inv_aref;         // storage scale of areas
 };
+
+// Where X of (cell c, direction d) is stored: direction-major d*N + c, or in
+// the compiled replay the task's position in the schedule (level-major), so
+// that a level set's gathers read what the previous level sets wrote densely.
+template <typename R>
+__device__ __forceinline__ size_t xidx(const Flux<R>& fl, int64_t N, uint32_t d, uint32_t c) {
+    const size_t o = (size_t)d * N + c;
+    return fl.xpos ? (size_t)fl.xpos[o] : o;
+}
 
 __device__ __forceinline__ uint4 task_record(uint32_t c, uint32_t f0, uint32_t f1, uint32_t d, uint32_t ps) {
     return make_uint4(c | ((f1 - f0) << kCellBits), f0, d, ps);
@@ -211,13 +221,15 @@ __device__ __forceinline__ CellIn<R> cell_inputs(const Geo& g, const Flux<R>& fl
 
 // Outgoing rate of a solved task (Site::get_rate + HydrogenOnly::get_outgoing_rate,
 // src/sweep/site.rs:49-56, src/chemistry/hydrogen_only/mod.rs:72-88), stored as X = O / A.
+// X goes to ox, I to oi.
 template <typename R>
-__device__ __forceinline__ void finish_task(const Flux<R>& fl, size_t o, R inc, const CellIn<R>& ci, R A) {
+__device__ __forceinline__ void finish_task(const Flux<R>& fl, size_t ox, size_t oi, R inc, const CellIn<R>& ci,
+                                            R A) {
     if (inc < (R)0) inc = 0;  // Photons::make_positive
     R in = (inc + (R)ci.src) + ci.per;
     R out = (in < (R)fl.thr) ? (R)0 : in * (R)ci.att;
-    fl.X[o] = (A > (R)0) ? out / A : (R)0;
-    fl.I[o] = inc;
+    fl.X[ox] = (A > (R)0) ? out / A : (R)0;
+    fl.I[oi] = inc;
 }
 
 // The face loop of one task, kLanes lanes per task (one face each), shared by
@@ -347,38 +359,16 @@ __global__ void k_face_stats(Geo g, unsigned long long* out4, uint32_t* ovf_coun
     }
 }
 
-// Overflow entries (upwind entries q >= kPlanes) of every (cell, direction), in
-// face order, at ovf_start[d*N + c] (assigned here by an atomic cursor).
-template <typename R>
-__global__ void k_overflow(Geo g, double inv_aref, const uint32_t* ovf_count, uint32_t* ovf_start,
-                           Entry<R>* ovf, uint32_t* cursor) {
+// Start of the overflow entries (upwind entries q >= kPlanes, in face order)
+// of every (cell, direction) in the overflow list, ovf_start[d*N + c]
+// (assigned by an atomic cursor); the schedule writes the entries.
+__global__ void k_overflow(Geo g, const uint32_t* ovf_count, uint32_t* ovf_start, uint32_t* cursor) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t c, d;
     if (!cell_dir(g, t, c, d)) return;
     const size_t o = (size_t)d * g.N + c;
     const uint32_t extra = ovf_count[o];
-    if (extra == 0) {
-        ovf_start[o] = 0;
-        return;
-    }
-    const uint32_t s = atomicAdd(cursor, extra);
-    ovf_start[o] = s;
-    const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
-    uint32_t q = 0;
-    for (uint32_t f = g.off[c]; f < g.off[c + 1]; f++) {
-        const uint32_t nk = g.nk[f];
-        const double4 a = g.geo[f];
-        const double dd = fdot(a, dx, dy, dz);
-        if (!((nk >> 30) == kKindLocal && dd < 0.0)) continue;
-        if (q >= (uint32_t)kPlanes) {
-            Entry<R> e;
-            e.nb = nk & kNbrMask;
-            if constexpr (sizeof(R) == 8) e.pad = 0;
-            e.w = (R)(a.w * inv_aref * -dd);
-            ovf[s + q - kPlanes] = e;
-        }
-        q++;
-    }
+    ovf_start[o] = extra ? atomicAdd(cursor, extra) : 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -485,14 +475,21 @@ __global__ void k_iota(uint32_t* p, uint32_t n) {
     if (i < n) p[i] = i;
 }
 
+// Schedule position of every task: xpos[sorted[p]] = p.
+__global__ void k_invert(const uint32_t* sorted, uint32_t* xpos, uint32_t n) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) xpos[sorted[p]] = p;
+}
+
 // The full schedule from the tasks sorted by (depth, direction, cell): task t =
 // d*N + c at position p of level set k (start b, width W). Compiled: header,
-// A and plane entries (one thread per task, faces in order; A as the same
+// A, plane entries and overflow entries, each entry naming the upwind task's
+// schedule position (one thread per task, faces in order; A as the same
 // 16-lane butterfly the geometric kernels form). Geometric: the task record.
 template <typename R>
 __global__ void k_schedule(Geo g, double inv_aref, const uint32_t* sorted, const uint32_t* fstart, int kmax,
-                           int compiled, const uint32_t* ovf_start, uint4* hdr, R* Av, Entry<R>* planes,
-                           uint4* recs) {
+                           int compiled, const uint32_t* xpos, const uint32_t* ovf_start, uint4* hdr, R* Av,
+                           Entry<R>* planes, Entry<R>* ovf, uint4* recs) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if ((int64_t)p >= (int64_t)g.D * g.N) return;
     int a = 0, bb = kmax - 1;  // level set of position p
@@ -514,6 +511,7 @@ __global__ void k_schedule(Geo g, double inv_aref, const uint32_t* sorted, const
 #pragma unroll
     for (int l = 0; l < kLanes; l++) part[l] = 0.0;
     uint32_t q = 0;
+    const uint32_t os = ovf_start[(size_t)d * g.N + c];
     for (uint32_t f = f0; f < f1; f++) {
         const double4 fa = g.geo[f];
         const uint32_t nk = g.nk[f];
@@ -525,13 +523,12 @@ __global__ void k_schedule(Geo g, double inv_aref, const uint32_t* sorted, const
             for (int u = 0; u < kLanes; u++)
                 if ((uint32_t)u == l) part[u] += term;
         } else if (dd < 0.0 && (nk >> 30) == kKindLocal) {
-            if (q < (uint32_t)kPlanes) {
-                Entry<R> e;
-                e.nb = nk & kNbrMask;
-                if constexpr (sizeof(R) == 8) e.pad = 0;
-                e.w = (R)(fa.w * inv_aref * -dd);
-                planes[(size_t)kPlanes * b + (size_t)q * W + (p - b)] = e;
-            }
+            Entry<R> e;
+            e.nb = xpos[(size_t)d * g.N + (nk & kNbrMask)];
+            if constexpr (sizeof(R) == 8) e.pad = 0;
+            e.w = (R)(fa.w * inv_aref * -dd);
+            if (q < (uint32_t)kPlanes) planes[(size_t)kPlanes * b + (size_t)q * W + (p - b)] = e;
+            else ovf[os + q - kPlanes] = e;
             q++;
         }
     }
@@ -544,7 +541,7 @@ __global__ void k_schedule(Geo g, double inv_aref, const uint32_t* sorted, const
 #pragma unroll
         for (int l = 0; l < kLanes; l++) part[l] = nxt[l];
     }
-    hdr[p] = make_uint4(c | (d << kCellBits), ps, q > (uint32_t)kPlanes ? ovf_start[(size_t)d * g.N + c] : 0u, q);
+    hdr[p] = make_uint4(c | (d << kCellBits), ps, q > (uint32_t)kPlanes ? os : 0u, q);
     Av[p] = (R)part[0];
 }
 
@@ -627,16 +624,18 @@ __global__ void __launch_bounds__(kBlock) k_sweep_rec(Geo g, Flux<R> fl, const u
     double A;
     R inc;
     task_faces<R, true, false>(g, fl, valid, r, sub, nfmax, A, inc, nullptr, nullptr);
-    if (valid && sub == 0) finish_task<R>(fl, (size_t)d * g.N + c, inc, ci, (R)A);
+    if (valid && sub == 0) finish_task<R>(fl, (size_t)d * g.N + c, (size_t)d * g.N + c, inc, ci, (R)A);
 }
 
-// One task of the compiled replay: the task at position p of a level set of
-// width W whose header, A and plane entries start at hdr, Av and planes. The
-// header, A and the plane entries are independent loads; the upwind X
-// gathers and the cell inputs follow.
+// One task of the compiled replay: the task at position p of a level set that
+// starts at schedule position b and has width W, whose header, A and plane
+// entries start at hdr, Av and planes. The header, A and the plane entries
+// are independent loads; the upwind X gathers (at the upwind tasks' schedule
+// positions) and the cell inputs follow. The task's X goes to position b + p.
 template <typename R>
 __device__ __forceinline__ void comp_task(const Geo& g, const Flux<R>& fl, const uint4* hdr, const R* Av,
-                                          const Entry<R>* planes, const Entry<R>* ovf, uint32_t W, uint32_t p) {
+                                          const Entry<R>* planes, const Entry<R>* ovf, uint32_t b, uint32_t W,
+                                          uint32_t p) {
     // first memory step: the plane entries, the header and A (independent)
     Entry<R> e[kPlanes];
 #pragma unroll
@@ -646,30 +645,31 @@ __device__ __forceinline__ void comp_task(const Geo& g, const Flux<R>& fl, const
     const uint32_t c = h.x & kCellMask, d = h.x >> kCellBits, ps = h.y, nup = h.w;
     const size_t dN = (size_t)d * g.N;
     // second memory step: the upwind X (entries past the upwind count gather
-    // the cell's own X, a valid index, and are not summed), the cell inputs
+    // the task's own X, a valid index, and are not summed), the cell inputs
     const CellIn<R> ci = cell_inputs<R>(g, fl, c, d, ps);
+    const uint32_t self = b + p;
     R x[kPlanes];
 #pragma unroll
-    for (int q = 0; q < kPlanes; q++) x[q] = fl.X[dN + ((uint32_t)q < nup ? e[q].nb : c)];
+    for (int q = 0; q < kPlanes; q++) x[q] = fl.X[(uint32_t)q < nup ? e[q].nb : self];
     R inc = 0;
 #pragma unroll
     for (int q = 0; q < kPlanes; q++) inc += ((uint32_t)q < nup) ? e[q].w * x[q] : (R)0;
     for (uint32_t q = kPlanes; q < nup; q++) {
         const Entry<R> o = ovf[h.z + q - kPlanes];
-        inc += o.w * fl.X[dN + o.nb];
+        inc += o.w * fl.X[o.nb];
     }
-    finish_task<R>(fl, dN + c, inc, ci, A);
+    finish_task<R>(fl, self, dN + c, inc, ci, A);
 }
 
 // Compiled replay of one level set of width W: one lane per task. With an
 // index list, lane i solves the task at position idx[i] of the level set.
 template <typename R, int kRecBlock>
 __global__ void __launch_bounds__(kRecBlock) k_sweep_comp(Geo g, Flux<R> fl, const uint4* hdr, const R* Av,
-                                                          const Entry<R>* planes, const Entry<R>* ovf, uint32_t W,
-                                                          const uint32_t* idx, uint32_t count) {
+                                                          const Entry<R>* planes, const Entry<R>* ovf, uint32_t b,
+                                                          uint32_t W, const uint32_t* idx, uint32_t count) {
     const uint32_t i = blockIdx.x * kRecBlock + threadIdx.x;
     if (i >= count) return;
-    comp_task<R>(g, fl, hdr, Av, planes, ovf, W, idx ? idx[i] : i);
+    comp_task<R>(g, fl, hdr, Av, planes, ovf, b, W, idx ? idx[i] : i);
 }
 
 // Compiled replay of a run of consecutive narrow level sets (at most
@@ -687,7 +687,7 @@ __global__ void __launch_bounds__(kRunBlock) k_sweep_comp_run(Geo g, Flux<R> fl,
         const uint4 r = run[j];
         const uint32_t b = r.x, W = r.y, i = threadIdx.x;
         if (i < r.w)
-            comp_task<R>(g, fl, hdr + b, Av + b, planes + (size_t)kPlanes * b, ovf, W, idx ? idx[r.z + i] : i);
+            comp_task<R>(g, fl, hdr + b, Av + b, planes + (size_t)kPlanes * b, ovf, b, W, idx ? idx[r.z + i] : i);
         __syncthreads();
     }
 }
@@ -714,7 +714,7 @@ __global__ void k_periodic(Geo g, Flux<R> fl) {
         const uint32_t nk = g.nk[f];
         const double4 a = g.geo[f];
         const double dd = fdot(a, dx, dy, dz);
-        if (dd < 0.0) acc += (R)(a.w * fl.inv_aref * -dd) * fl.X[(size_t)d * g.N + (nk & kNbrMask)];
+        if (dd < 0.0) acc += (R)(a.w * fl.inv_aref * -dd) * fl.X[xidx(fl, g.N, d, nk & kNbrMask)];
     }
     fl.P[(size_t)d * g.Nper + s] = acc;
 }
@@ -827,7 +827,7 @@ __global__ void k_fluxes(Geo g, Flux<R> fl, double lref, double* I_out, double* 
             if (dd > 0.0) {
                 A += a.w * dd;
             } else if (dd < 0.0 && (nk >> 30) != kKindBoundary) {
-                R v = (R)(a.w * fl.inv_aref * -dd) * fl.X[(size_t)d * g.N + (nk & kNbrMask)];
+                R v = (R)(a.w * fl.inv_aref * -dd) * fl.X[xidx(fl, g.N, d, nk & kNbrMask)];
                 if ((nk >> 30) == kKindLocal) inc += v;
                 else per += v;
             }
@@ -837,9 +837,16 @@ __global__ void k_fluxes(Geo g, Flux<R> fl, double lref, double* I_out, double* 
         size_t o = (size_t)d * g.N + c;
         if (I_out) I_out[o] = inc_si;
         if (P_out) P_out[o] = (double)per * lref;
-        if (O_out) O_out[o] = (double)fl.X[o] * (A * fl.inv_aref) * lref;
+        if (O_out) O_out[o] = (double)fl.X[xidx(fl, g.N, (uint32_t)d, (uint32_t)c)] * (A * fl.inv_aref) * lref;
     }
     if (photon_rate) photon_rate[c] = total;
+}
+
+// Direction-major copy of X from the schedule-position order.
+template <typename R>
+__global__ void k_gather_x(const R* X, const uint32_t* xpos, R* out, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = X[xpos[i]];
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
@@ -929,6 +936,7 @@ struct ssw_sweep {
     void* d_planes = nullptr;
     uint32_t* d_ovf_start = nullptr;
     void* d_ovf = nullptr;
+    uint32_t* d_xpos = nullptr;                     // compiled: schedule position of each (d, c)
     // compaction to active sets
     uint32_t nchunks = 0;
     uint4* d_chunks = nullptr;
@@ -1034,6 +1042,7 @@ template <typename R>
 Flux<R> make_flux(ssw_sweep* s) {
     Flux<R> f;
     f.X = (R*)s->d_X;
+    f.xpos = s->compiled ? s->d_xpos : nullptr;
     f.I = (R*)s->d_I;
     f.P = (R*)s->d_P;
     f.src_over_d = s->fp32 ? s->d_src_t : s->d_src;
@@ -1276,21 +1285,24 @@ int build_schedule(ssw_sweep* s) {
                            s->stream));
     const double inv_aref = 1.0 / s->aref;
     if (s->compiled) {
+        TRY(dalloc(s, &s->d_xpos, DN));
+        k_invert<<<grid_for((int64_t)DN), kBlock, 0, s->stream>>>(d_sorted, s->d_xpos, (uint32_t)DN);
         TRY(dalloc(s, &s->d_hdr, DN));
         TRY(dalloc(s, (uint8_t**)&s->d_A, DN * real_size(s)));
         TRY(dalloc(s, (uint8_t**)&s->d_planes, (size_t)kPlanes * DN * entry_size(s)));
         if (s->fp32)
             k_schedule<float><<<grid_for((int64_t)DN), kBlock, 0, s->stream>>>(
-                g, inv_aref, d_sorted, s->d_fstart, s->kmax, 1, s->d_ovf_start, s->d_hdr, (float*)s->d_A,
-                (Entry<float>*)s->d_planes, nullptr);
+                g, inv_aref, d_sorted, s->d_fstart, s->kmax, 1, s->d_xpos, s->d_ovf_start, s->d_hdr,
+                (float*)s->d_A, (Entry<float>*)s->d_planes, (Entry<float>*)s->d_ovf, nullptr);
         else
             k_schedule<double><<<grid_for((int64_t)DN), kBlock, 0, s->stream>>>(
-                g, inv_aref, d_sorted, s->d_fstart, s->kmax, 1, s->d_ovf_start, s->d_hdr, (double*)s->d_A,
-                (Entry<double>*)s->d_planes, nullptr);
+                g, inv_aref, d_sorted, s->d_fstart, s->kmax, 1, s->d_xpos, s->d_ovf_start, s->d_hdr,
+                (double*)s->d_A, (Entry<double>*)s->d_planes, (Entry<double>*)s->d_ovf, nullptr);
     } else {
         TRY(dalloc(s, &s->d_recs, DN));
         k_schedule<double><<<grid_for((int64_t)DN), kBlock, 0, s->stream>>>(
-            g, inv_aref, d_sorted, s->d_fstart, s->kmax, 0, nullptr, nullptr, nullptr, nullptr, s->d_recs);
+            g, inv_aref, d_sorted, s->d_fstart, s->kmax, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+            s->d_recs);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -1407,16 +1419,16 @@ void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<
             const Entry<R>* planes = (const Entry<R>*)s->d_planes + (size_t)kPlanes * b;
             switch (s->rec_block) {
                 case 128:
-                    k_sweep_comp<R, 128><<<(count + 127) / 128, 128, 0, s->stream>>>(g, fl, hdr, A, planes, ovf, W,
-                                                                                   ik, count);
+                    k_sweep_comp<R, 128><<<(count + 127) / 128, 128, 0, s->stream>>>(g, fl, hdr, A, planes, ovf, b,
+                                                                                   W, ik, count);
                     break;
                 case 256:
-                    k_sweep_comp<R, 256><<<(count + 255) / 256, 256, 0, s->stream>>>(g, fl, hdr, A, planes, ovf, W,
-                                                                                   ik, count);
+                    k_sweep_comp<R, 256><<<(count + 255) / 256, 256, 0, s->stream>>>(g, fl, hdr, A, planes, ovf, b,
+                                                                                   W, ik, count);
                     break;
                 default:
-                    k_sweep_comp<R, 64><<<(count + 63) / 64, 64, 0, s->stream>>>(g, fl, hdr, A, planes, ovf, W, ik,
-                                                                               count);
+                    k_sweep_comp<R, 64><<<(count + 63) / 64, 64, 0, s->stream>>>(g, fl, hdr, A, planes, ovf, b, W,
+                                                                               ik, count);
             }
         } else {
             k_sweep_rec<R><<<(count + kTasksPerBlock - 1) / kTasksPerBlock, kBlock, 0, s->stream>>>(
@@ -1609,6 +1621,7 @@ void free_grid(ssw_sweep* s) {
     dfree(s, s->d_planes);
     dfree(s, s->d_ovf_start);
     dfree(s, s->d_ovf);
+    dfree(s, s->d_xpos);
     dfree(s, s->d_chunks);
     dfree(s, s->d_first_chunk);
     dfree(s, s->d_ccount);
@@ -1626,14 +1639,7 @@ int setup_overflow(ssw_sweep* s, uint32_t* d_ovf_count) {
     TRY(dalloc(s, &d_cursor, 1));
     HIP_TRY(hipMemsetAsync(d_cursor, 0, 4, s->stream));
     Geo g = make_geo(s);
-    const double inv_aref = 1.0 / s->aref;
-    const unsigned gt = grid_for((int64_t)DN);
-    if (s->fp32)
-        k_overflow<float><<<gt, kBlock, 0, s->stream>>>(g, inv_aref, d_ovf_count, s->d_ovf_start,
-                                                        (Entry<float>*)s->d_ovf, d_cursor);
-    else
-        k_overflow<double><<<gt, kBlock, 0, s->stream>>>(g, inv_aref, d_ovf_count, s->d_ovf_start,
-                                                         (Entry<double>*)s->d_ovf, d_cursor);
+    k_overflow<<<grid_for((int64_t)DN), kBlock, 0, s->stream>>>(g, d_ovf_count, s->d_ovf_start, d_cursor);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->stream));
     dfree(s, d_cursor);
@@ -1915,7 +1921,7 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
     const size_t rs = real_size(s);
-    const size_t comp_bytes = DN * (16 + rs + kPlanes * entry_size(s)) + (size_t)s->ovf_total * entry_size(s);
+    const size_t comp_bytes = DN * (16 + rs + kPlanes * entry_size(s) + 4) + (size_t)s->ovf_total * entry_size(s);
     const size_t comp_need = comp_bytes + DN * std::max<size_t>(8, 4 * (size_t)(L - 1) + 2 * rs);
     if (s->mode_request == SSW_TRANSPORT_MODE_COMPILED) s->compiled = true;
     else if (s->mode_request == SSW_TRANSPORT_MODE_GEOMETRIC) s->compiled = false;
@@ -2112,10 +2118,23 @@ int ssw_download_fluxes(ssw_sweep* s, double* incoming, double* outgoing, double
 int ssw_download_transport(ssw_sweep* s, void* X, void* I) {
     if (!s) return fail(SSW_ERR_INVALID, "null handle");
     if (!s->have_state) return fail(SSW_ERR_INVALID, "ssw_upload_state must be called first");
-    const size_t m = (size_t)s->D * s->N * real_size(s);
-    if (X) HIP_TRY(hipMemcpyAsync(X, s->d_X, m, hipMemcpyDeviceToHost, s->stream));
+    const int64_t DN = (int64_t)s->D * s->N;
+    const size_t m = (size_t)DN * real_size(s);
+    uint8_t* dtmp = nullptr;
+    if (X && s->compiled) {  // X is stored in schedule-position order
+        TRY(dalloc(s, &dtmp, m));
+        if (s->fp32)
+            k_gather_x<float><<<grid_for(DN), kBlock, 0, s->stream>>>((const float*)s->d_X, s->d_xpos,
+                                                                      (float*)dtmp, DN);
+        else
+            k_gather_x<double><<<grid_for(DN), kBlock, 0, s->stream>>>((const double*)s->d_X, s->d_xpos,
+                                                                       (double*)dtmp, DN);
+        HIP_TRY(hipGetLastError());
+    }
+    if (X) HIP_TRY(hipMemcpyAsync(X, dtmp ? (void*)dtmp : s->d_X, m, hipMemcpyDeviceToHost, s->stream));
     if (I) HIP_TRY(hipMemcpyAsync(I, s->d_I, m, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
+    dfree(s, dtmp);
     for (void* a : {X, I}) {
         if (!a) continue;
         if (s->fp32) to_caller(s, (float*)a, s->D);
@@ -2196,7 +2215,7 @@ int64_t ssw_estimate_device_bytes(const ssw_sweep_parameters* p, int64_t n, int6
     // held from the grid upload on: grid arrays, level starts, chunk table
     const int64_t grid = (n + 1) * 4 + F * 32 + F * 4 + n * 16 + n * 4 + num_periodic_cells * 8 + F * 4 + n +
                          (n + 4) * 4 + DN / kChunk * 40 + (1 << 20);
-    const int64_t sched = comp ? DN * (16 + rs + kPlanes * es) + ovf : DN * 16;
+    const int64_t sched = comp ? DN * (16 + rs + kPlanes * es + 4) + ovf : DN * 16;  // compiled: + positions
     // transient peak of the schedule build: counters + layering records (+
     // overflow starts), the four sort arrays, or the schedule being written
     // with the sorted order and overflow starts
