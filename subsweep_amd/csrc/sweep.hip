@@ -158,8 +158,7 @@ struct Flux {
     const uint32_t* xpos;    // compiled replay: X of (c, d) at the task's schedule position xpos[d*N + c]
     R* I;                    // D x N
     R* P;                    // D x Nper
-    const double* src_over_d;
-    const double* att;
+    const double2* sa;       // N: {source / D in the storage scale, attenuation} (one load per task)
     double thr;              // threshold in the storage scale
     double inv_lref;         // storage scale of rates
     double inv_aref;         // storage scale of areas
@@ -211,8 +210,9 @@ template <typename R>
 __device__ __forceinline__ CellIn<R> cell_inputs(const Geo& g, const Flux<R>& fl, uint32_t c, uint32_t d,
                                                  uint32_t ps) {
     CellIn<R> ci;
-    ci.src = fl.src_over_d[c];
-    ci.att = fl.att[c];
+    const double2 sa = fl.sa[c];
+    ci.src = sa.x;
+    ci.att = sa.y;
     // the periodic load uses slot 0 for cells without one (always a valid index)
     const R p = fl.P ? fl.P[(size_t)d * g.Nper + (ps != kNone ? ps : 0u)] : (R)0;
     ci.per = (ps != kNone) ? p : (R)0;
@@ -724,7 +724,7 @@ struct ChemArgs {
     double* x;
     double* T;
     double* src;             // source per direction bin (SI)
-    double* att;
+    double2* sa;             // .y: attenuation of the next sweep
     double* species_ts;
     double* prev_rate;
     double* change_ts;
@@ -790,7 +790,7 @@ __global__ void k_chemistry(Geo g, Flux<R> fl, ChemArgs ca, int lvl) {
     ca.T[c] = cell.T;
     ca.species_ts[c] = used;
     ca.change_ts[c] = (rate_ts < used) ? rate_ts : used;  // Timescale::min
-    ca.att[c] = ssw::non_absorbed_fraction(cell.rho, cell.x, cell.len);
+    ca.sa[c].y = ssw::non_absorbed_fraction(cell.rho, cell.x, cell.len);
 }
 
 // update_timestep_levels (src/sweep/mod.rs:576-589) for all cells. Marks the
@@ -854,9 +854,9 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
     if (i < n) p[i] = v;
 }
 
-__global__ void k_attenuation(const double* rho, const double* x, const double* size, double* att, int64_t n) {
+__global__ void k_attenuation(const double* rho, const double* x, const double* size, double2* sa, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) att[i] = ssw::non_absorbed_fraction(rho[i], x[i], size[i]);
+    if (i < n) sa[i].y = ssw::non_absorbed_fraction(rho[i], x[i], size[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -960,8 +960,9 @@ struct ssw_sweep {
     std::vector<std::vector<uint4>> runtab;         // host copies of the run tables
     std::vector<uint4*> d_runtab;
     // state
-    double *d_rho = nullptr, *d_x = nullptr, *d_T = nullptr, *d_src = nullptr, *d_att = nullptr;
-    double *d_sts = nullptr, *d_prev = nullptr, *d_change = nullptr, *d_src_t = nullptr;
+    double *d_rho = nullptr, *d_x = nullptr, *d_T = nullptr, *d_src = nullptr;
+    double2* d_sa = nullptr;                        // {transport source / D, attenuation}
+    double *d_sts = nullptr, *d_prev = nullptr, *d_change = nullptr;
     unsigned long long* d_active = nullptr;
     uint8_t* d_level = nullptr;
     void *d_X = nullptr, *d_I = nullptr, *d_P = nullptr;
@@ -1045,8 +1046,7 @@ Flux<R> make_flux(ssw_sweep* s) {
     f.xpos = s->compiled ? s->d_xpos : nullptr;
     f.I = (R*)s->d_I;
     f.P = (R*)s->d_P;
-    f.src_over_d = s->fp32 ? s->d_src_t : s->d_src;
-    f.att = s->d_att;
+    f.sa = s->d_sa;
     f.inv_lref = 1.0 / s->lref;
     f.inv_aref = 1.0 / s->aref;
     f.thr = s->p.significant_rate_threshold * f.inv_lref;
@@ -1482,7 +1482,7 @@ int run_step(ssw_sweep* s, double* elapsed) {
     ca.x = s->d_x;
     ca.T = s->d_T;
     ca.src = s->d_src;
-    ca.att = s->d_att;
+    ca.sa = s->d_sa;
     ca.species_ts = s->d_sts;
     ca.prev_rate = s->d_prev;
     ca.change_ts = s->d_change;
@@ -1586,11 +1586,10 @@ void free_state(ssw_sweep* s) {
     dfree(s, s->d_x);
     dfree(s, s->d_T);
     dfree(s, s->d_src);
-    dfree(s, s->d_att);
+    dfree(s, s->d_sa);
     dfree(s, s->d_sts);
     dfree(s, s->d_prev);
     dfree(s, s->d_change);
-    dfree(s, s->d_src_t);
     dfree(s, s->d_X);
     dfree(s, s->d_I);
     dfree(s, s->d_P);
@@ -1959,11 +1958,10 @@ int ssw_upload_state(ssw_sweep* s, const double* density, const double* xhii, co
     TRY(dalloc(s, &s->d_x, n));
     TRY(dalloc(s, &s->d_T, n));
     TRY(dalloc(s, &s->d_src, n));
-    TRY(dalloc(s, &s->d_att, n));
+    TRY(dalloc(s, &s->d_sa, n));
     TRY(dalloc(s, &s->d_sts, n));
     TRY(dalloc(s, &s->d_prev, n));
     TRY(dalloc(s, &s->d_change, n));
-    if (s->fp32) TRY(dalloc(s, &s->d_src_t, n));
     TRY(dalloc(s, (uint8_t**)&s->d_X, (size_t)s->D * n * es));
     TRY(dalloc(s, (uint8_t**)&s->d_I, (size_t)s->D * n * es));
     if (s->Nper) TRY(dalloc(s, (uint8_t**)&s->d_P, (size_t)s->D * s->Nper * es));
@@ -1971,12 +1969,11 @@ int ssw_upload_state(ssw_sweep* s, const double* density, const double* xhii, co
     HIP_TRY(hipMemcpyAsync(s->d_x, xhii, n * 8, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_T, temp, n * 8, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_src, sod.data(), n * 8, hipMemcpyHostToDevice, s->stream));
-    std::vector<double> sod_scaled;
-    if (s->fp32) {  // the FP32 transport reads source/D in its storage scale
-        sod_scaled.resize(n);
-        for (int64_t c = 0; c < n; c++) sod_scaled[c] = sod[c] / s->lref;
-        HIP_TRY(hipMemcpyAsync(s->d_src_t, sod_scaled.data(), n * 8, hipMemcpyHostToDevice, s->stream));
-    }
+    // the transport reads source/D in its storage scale (FP32: divided by the
+    // power of two lref, exact); the attenuation follows below
+    std::vector<double2> sa(n);
+    for (int64_t c = 0; c < n; c++) sa[c] = make_double2(s->fp32 ? sod[c] / s->lref : sod[c], 0.0);
+    HIP_TRY(hipMemcpyAsync(s->d_sa, sa.data(), n * sizeof(double2), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     HIP_TRY(hipMemsetAsync(s->d_sts, 0, n * 8, s->stream));
     HIP_TRY(hipMemsetAsync(s->d_prev, 0, n * 8, s->stream));
@@ -1988,7 +1985,7 @@ int ssw_upload_state(ssw_sweep* s, const double* density, const double* xhii, co
     HIP_TRY(hipMemsetAsync(s->d_active, 0, 8, s->stream));
     std::vector<uint8_t> lev(n, (uint8_t)(L - 1));  // src/sweep/mod.rs:206
     HIP_TRY(hipMemcpyAsync(s->d_level, lev.data(), n, hipMemcpyHostToDevice, s->stream));
-    k_attenuation<<<grid_for(n), kBlock, 0, s->stream>>>(s->d_rho, s->d_x, s->d_size, s->d_att, n);
+    k_attenuation<<<grid_for(n), kBlock, 0, s->stream>>>(s->d_rho, s->d_x, s->d_size, s->d_sa, n);
     HIP_TRY(hipGetLastError());
     TRY(set_need_all(s, 1u));
     HIP_TRY(hipStreamSynchronize(s->stream));
