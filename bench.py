@@ -108,6 +108,8 @@ def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3,
         "replay_block": int(info.replay_block),
         "graph_replay": bool(info.graph_replay),
         "narrow_runs": bool(info.narrow_runs),
+        "group_levels": int(info.group_levels),
+        "group_tasks_per_full_sweep": int(info.group_tasks),
         "transport_mode": int(info.transport_mode),
         "info": info,
         "results": res,
@@ -168,8 +170,9 @@ def tuned(pb, precision: int, warmup: int, enabled: bool, mode: int, block: int)
     t = time.perf_counter()
     r = autotune(pb.with_params(transport_precision=precision), warmup=warmup)
     log(f"autotune fp{precision}: mode {r.transport_mode}, block {r.replay_block}, graphs {r.graph_replay}, "
-        f"narrow runs {r.narrow_runs}, {r.ms_per_step:.2f} ms/step ({time.perf_counter() - t:.1f} s; " +
-        ", ".join(f"m{m}/b{b}/g{int(gr)}/n{int(nr)} {ms:.2f}" for m, b, gr, nr, ms in r.table) + ")")
+        f"narrow runs {r.narrow_runs}, group levels {r.group_levels}, {r.ms_per_step:.2f} ms/step "
+        f"({time.perf_counter() - t:.1f} s; " +
+        ", ".join(f"m{m}/b{b}/g{int(gr)}/n{int(nr)}/l{gl} {ms:.2f}" for m, b, gr, nr, gl, ms in r.table) + ")")
     return r.kwargs(), [list(x) for x in r.table]
 
 
@@ -203,6 +206,8 @@ def run_clustered(steps: int = 4, warmup: int = 2, autotune: bool = True):
         "fp64_transport_mode": r64["transport_mode"],
         "fp32_replay_block": r32["replay_block"],
         "fp64_replay_block": r64["replay_block"],
+        "fp32_group_levels": r32["group_levels"],
+        "fp64_group_levels": r64["group_levels"],
         "autotune_fp32_ms_per_step": t32,
         "autotune_fp64_ms_per_step": t64,
         "fp32_error_vs_fp64": field_errors(r32["results"], r64["results"]),
@@ -461,11 +466,11 @@ def main():
             "fp64_bytes_per_update": fp64["bytes_per_update"],
         },
         "launch_shape": {"fp32": {"replay_block": fp32["replay_block"], "graph_replay": fp32["graph_replay"],
-                                  "narrow_runs": fp32["narrow_runs"], "transport_mode": fp32["transport_mode"],
-                                  "autotune_ms_per_step": tune32},
+                                  "narrow_runs": fp32["narrow_runs"], "group_levels": fp32["group_levels"],
+                                  "transport_mode": fp32["transport_mode"], "autotune_ms_per_step": tune32},
                          "fp64": {"replay_block": fp64["replay_block"], "graph_replay": fp64["graph_replay"],
-                                  "narrow_runs": fp64["narrow_runs"], "transport_mode": fp64["transport_mode"],
-                                  "autotune_ms_per_step": tune64}},
+                                  "narrow_runs": fp64["narrow_runs"], "group_levels": fp64["group_levels"],
+                                  "transport_mode": fp64["transport_mode"], "autotune_ms_per_step": tune64}},
         "cpu_baseline": cpu,
         "fp32_error_vs_fp64": field_errors(fp32["results"], fp64["results"]),
         "clustered_2m": clustered,

@@ -154,6 +154,12 @@ typedef struct ssw_info {
     int32_t narrow_runs;           /* 1 if runs of narrow level sets replay in
                                       one workgroup (compiled replay)          */
     int64_t peak_device_bytes;     /* most device memory the handle has held   */
+    int32_t group_levels;          /* level sets per grouped launch of full
+                                      sweeps (1: one launch per level set)     */
+    int32_t group_fallback;        /* 1 if groups were requested but a cone did
+                                      not fit (full sweeps per level set)      */
+    int64_t group_tasks;           /* tasks a grouped full sweep solves, cones
+                                      overlapping (>= the sweep's tasks)       */
 } ssw_info;
 
 int ssw_get_info(ssw_sweep* s, ssw_info* info);
@@ -206,6 +212,18 @@ int ssw_set_graph_replay(ssw_sweep* s, int32_t enabled);
  * in the reference. */
 int ssw_set_narrow_runs(ssw_sweep* s, int32_t enabled);
 
+/* Compiled replay of full sweeps (every cell active): groups of `levels`
+ * consecutive level sets (2 to 4; 1 = one launch per level set, default) run
+ * as one launch each. A workgroup owns 128 root tasks of one level set of the
+ * group and solves their upwind cone inside the group level set by level set
+ * with a workgroup barrier in between; a task in several cones is solved by
+ * each of them from the same inputs with the same arithmetic, so results are
+ * bitwise the same as without groups. The cones are built once per grid at
+ * ssw_upload_grid; if one exceeds its bound the full sweeps stay per level
+ * set (ssw_info.group_fallback). Set before ssw_upload_grid. No counterpart in
+ * the reference. */
+int ssw_set_group_levels(ssw_sweep* s, int32_t levels);
+
 /* Raw transport state in its storage precision and scale (FP32 or FP64 per
  * the transport precision, direction-major): X = outgoing rate / downwind
  * effective area and I = incoming rate. For bitwise comparisons. */
@@ -238,7 +256,8 @@ double ssw_chemistry_timestep(double* state, double density, double volume, doub
  * num_faces directed faces and num_periodic_cells cells with a periodic face,
  * in the given transport mode (GEOMETRIC or COMPILED): an upper estimate of
  * the handle's peak from ssw_upload_grid through ssw_step and ssw_download,
- * used for admission against an HBM budget (the diagnostic downloads of
+ * used for admission against an HBM budget, for handles with the default
+ * launch shape (without grouped full sweeps; the diagnostic downloads of
  * counts, level sets, fluxes and transport state allocate transient arrays
  * beyond it). No counterpart in the reference. */
 int64_t ssw_estimate_device_bytes(const ssw_sweep_parameters* params, int64_t num_cells, int64_t num_faces,

@@ -82,6 +82,9 @@ constexpr int kQueueCap = kTasksPerBlock * kMaxFaces;
 constexpr int kPlanes = 12;                // upwind entries per task held in planes
 constexpr int kChunk = 4096;               // compaction chunk (tasks, within one level set)
 constexpr int kRunBlock = 1024;            // workgroup of a run of narrow level sets (tasks per level set)
+constexpr int kGroupT = 128;               // grouped replay: root tasks (and lanes) per workgroup
+constexpr int kConeCap = 4096;             // grouped replay build: upwind-cone entries per workgroup
+constexpr int kMaxGroup = 4;               // grouped replay: level sets per group
 
 thread_local std::string g_err;
 // Handles alive in the process. Replay graphs are captured only while a single
@@ -693,6 +696,190 @@ __global__ void __launch_bounds__(kRunBlock) k_sweep_comp_run(Geo g, Flux<R> fl,
 }
 
 // ---------------------------------------------------------------------------
+// grouped replay (full sweeps): m consecutive level sets per launch. Each
+// workgroup owns a block of root tasks (kGroupT consecutive tasks of one level
+// set of the group) and solves the roots' upwind cone inside the group, level
+// set by level set with a workgroup barrier in between, so that every
+// dependency inside the launch stays inside the workgroup (L1 coherent within
+// the workgroup). Cones of different workgroups may overlap: such a task is
+// solved by each of them from the same inputs with the same arithmetic and
+// stores the same values. The roots of level set j are the tasks of j in no
+// cone of a higher level set of the group.
+
+// Sorts a[0, n2) ascending (n2 a power of two) in LDS, all lanes taking part.
+__device__ void lds_sort(uint32_t* a, uint32_t n2) {
+    for (uint32_t k = 2; k <= n2; k <<= 1)
+        for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) {
+                const uint32_t ixj = i ^ jj;
+                if (ixj > i) {
+                    const uint32_t x = a[i], y = a[ixj];
+                    if ((x > y) == ((i & k) == 0)) {
+                        a[i] = y;
+                        a[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// Appends the upwind schedule positions in [lo, hi) of the task at position
+// P (level set starting at b, width W) to cand.
+template <typename R>
+__device__ void push_upwind(const uint4* hdr, const Entry<R>* planes, const Entry<R>* ovf, uint32_t P, uint32_t b,
+                            uint32_t W, uint32_t lo, uint32_t hi, uint32_t* cand, uint32_t* ncand, uint32_t* over) {
+    const uint4 h = hdr[P];
+    for (uint32_t q = 0; q < h.w; q++) {
+        const uint32_t v = q < (uint32_t)kPlanes ? planes[(size_t)kPlanes * b + (size_t)q * W + (P - b)].nb
+                                                 : ovf[h.z + q - kPlanes].nb;
+        if (v < lo || v >= hi) continue;
+        const uint32_t k = atomicAdd(ncand, 1u);
+        if (k < (uint32_t)kConeCap) cand[k] = v;
+        else *over = 1u;
+    }
+}
+
+// One workgroup per block of roots of level set j in the group starting at
+// level set k0: its cone per level set (sorted, unique), written with the
+// roots to the task list, level set k0 first; desc = {list offset, counts of
+// level sets k0 and k0+1, of k0+2 and k0+3 (16 bits each), level sets}.
+// Members and roots are marked done in todo. Roots are roots[r] or, without
+// a root list, positions root_base + r.
+template <typename R>
+__global__ void __launch_bounds__(kGroupT) k_cone(const uint4* hdr, const Entry<R>* planes, const Entry<R>* ovf,
+                                                  const uint32_t* fstart, int k0, int j, const uint32_t* roots,
+                                                  uint32_t root_base, uint32_t nroots, uint8_t* todo, uint32_t* list,
+                                                  uint32_t list_cap, uint32_t* cursor, uint4* desc, uint32_t* fail) {
+    __shared__ uint32_t cand[kConeCap], tmp[kConeCap], memb[kConeCap];
+    __shared__ uint32_t ncand, ntmp, nmemb, over, base;
+    __shared__ uint32_t cnt[kMaxGroup], psum[kGroupT];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t r0 = blockIdx.x * kGroupT;
+    const uint32_t n = min((uint32_t)kGroupT, nroots - r0);
+    if (tid == 0) {
+        ncand = 0;
+        nmemb = 0;
+        over = 0;
+    }
+    if (tid < (uint32_t)kMaxGroup) cnt[tid] = 0;
+    __syncthreads();
+    const uint32_t lo = fstart[k0];
+    uint32_t root = 0;
+    if (tid < n) {
+        root = roots ? roots[r0 + tid] : root_base + r0 + tid;
+        push_upwind<R>(hdr, planes, ovf, root, fstart[j], fstart[j + 1] - fstart[j], lo, fstart[j], cand, &ncand,
+                       &over);
+    }
+    __syncthreads();
+    for (int i = j - 1; i >= k0; i--) {
+        const uint32_t bi = fstart[i], ei = fstart[i + 1];
+        if (tid == 0) ntmp = 0;
+        __syncthreads();
+        const uint32_t nc = min(ncand, (uint32_t)kConeCap);
+        for (uint32_t c = tid; c < nc; c += kGroupT) {
+            const uint32_t v = cand[c];
+            if (v >= bi && v < ei) tmp[atomicAdd(&ntmp, 1u)] = v;
+        }
+        __syncthreads();
+        const uint32_t nt = ntmp;
+        uint32_t n2 = 1;
+        while (n2 < nt) n2 <<= 1;
+        for (uint32_t c = nt + tid; c < n2; c += kGroupT) tmp[c] = kNone;
+        __syncthreads();
+        lds_sort(tmp, n2);
+        // unique, in order: each lane compacts a contiguous chunk
+        const uint32_t per = (nt + kGroupT - 1) / kGroupT;
+        const uint32_t c0 = min(nt, tid * per), c1 = min(nt, c0 + per);
+        uint32_t u = 0;
+        for (uint32_t c = c0; c < c1; c++) u += (c == 0 || tmp[c] != tmp[c - 1]) ? 1u : 0u;
+        psum[tid] = u;
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t s = 0;
+            for (int l = 0; l < kGroupT; l++) {
+                const uint32_t x = psum[l];
+                psum[l] = s;
+                s += x;
+            }
+            cnt[i - k0] = s;
+            if (nmemb + s > (uint32_t)kConeCap) over = 1u;
+        }
+        __syncthreads();
+        if (over) break;
+        const uint32_t m0 = nmemb;
+        uint32_t w = m0 + psum[tid];
+        for (uint32_t c = c0; c < c1; c++)
+            if (c == 0 || tmp[c] != tmp[c - 1]) memb[w++] = tmp[c];
+        __syncthreads();
+        if (tid == 0) nmemb = m0 + cnt[i - k0];
+        __syncthreads();
+        const uint32_t m1 = nmemb;
+        if (i > k0)
+            for (uint32_t c = m0 + tid; c < m1; c += kGroupT)
+                push_upwind<R>(hdr, planes, ovf, memb[c], bi, ei - bi, lo, bi, cand, &ncand, &over);
+        __syncthreads();
+    }
+    if (over) {
+        if (tid == 0) atomicOr(fail, 1u);
+        return;
+    }
+    const uint32_t total = nmemb + n;
+    if (tid == 0) {
+        base = atomicAdd(cursor, total);
+        if (base + total > list_cap) {
+            atomicOr(fail, 2u);
+            base = kNone;
+        }
+    }
+    __syncthreads();
+    if (base == kNone) return;
+    // members were appended from level set j-1 down to k0; the list holds them
+    // from k0 up, then the roots
+    uint32_t src = nmemb, dst = 0;
+    for (int i = k0; i < j; i++) {
+        const uint32_t ci = cnt[i - k0];
+        src -= ci;
+        for (uint32_t c = tid; c < ci; c += kGroupT) {
+            const uint32_t v = memb[src + c];
+            list[base + dst + c] = v;
+            todo[v] = 0;
+        }
+        dst += ci;
+    }
+    if (tid < n) {
+        list[base + dst + tid] = root;
+        todo[root] = 0;
+    }
+    if (tid == 0) {
+        uint32_t c[kMaxGroup] = {0, 0, 0, 0};
+        for (int i = k0; i < j; i++) c[i - k0] = cnt[i - k0];
+        c[j - k0] = n;
+        desc[blockIdx.x] = make_uint4(base, c[0] | (c[1] << 16), c[2] | (c[3] << 16), (uint32_t)(j - k0 + 1));
+    }
+}
+
+// Grouped replay of the level sets k0 .. k0+phases-1: workgroup w solves the
+// tasks of its descriptor, level set by level set, barrier in between.
+template <typename R>
+__global__ void __launch_bounds__(kGroupT) k_sweep_group(Geo g, Flux<R> fl, const uint4* hdr, const R* Av,
+                                                         const Entry<R>* planes, const Entry<R>* ovf,
+                                                         const uint32_t* fstart, int k0, int phases,
+                                                         const uint4* desc, const uint32_t* list) {
+    const uint4 dsc = desc[blockIdx.x];
+    uint32_t off = dsc.x;
+    for (int ph = 0; ph < phases; ph++) {
+        const uint32_t word = ph < 2 ? dsc.y : dsc.z;
+        const uint32_t n = (ph & 1) ? (word >> 16) : (word & 0xFFFFu);
+        const uint32_t b = fstart[k0 + ph], W = fstart[k0 + ph + 1] - b;
+        for (uint32_t i = threadIdx.x; i < n; i += kGroupT)
+            comp_task<R>(g, fl, hdr + b, Av + b, planes + (size_t)kPlanes * b, ovf, b, W, list[off + i] - b);
+        off += n;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // per-sweep kernels
 
 // Periodic source of every periodic cell from the current X, after each
@@ -905,6 +1092,18 @@ struct ssw_sweep {
     int rec_block = 64;                             // compiled replay block size
     bool graphs_on = true;                          // capture replays as graphs
     bool narrow_runs = true;                        // runs of narrow level sets in one workgroup
+    int group_request = 1;                          // level sets per grouped launch of full sweeps (1: off)
+    // grouped replay of full sweeps (compiled): per group of level sets its
+    // first level set, level-set count and workgroup descriptors
+    struct Group {
+        int k0, levels;
+        uint32_t desc0, ndesc;
+    };
+    std::vector<Group> groups;
+    uint4* d_gdesc = nullptr;
+    uint32_t* d_glist = nullptr;
+    int64_t group_tasks = 0;                        // tasks solved per full sweep by the grouped replay
+    bool group_fallback = false;                    // groups requested but a cone did not fit
     bool compiled = false;
     // grid
     int64_t N = 0, F = 0, Nper = 0;
@@ -952,9 +1151,10 @@ struct ssw_sweep {
     // replay plan per sweep level: one launch per level set, or one launch per
     // run of consecutive narrow level sets (compiled replay)
     struct ReplayItem {
-        int k;          // level set (single launch) or first level set of the run
-        int levels;     // 0: single launch; else the run's non-empty level sets
-        uint32_t tab;   // the run's first entry in the run table
+        int k;          // level set (single launch) or first level set of the run / group
+        int levels;     // 0: single launch; else the run's non-empty level sets (or the group's level sets)
+        uint32_t tab;   // the run's first entry in the run table (the group's index)
+        int group;      // 1: a group of the grouped replay
     };
     std::vector<std::vector<ReplayItem>> plan;
     std::vector<std::vector<uint4>> runtab;         // host copies of the run tables
@@ -1199,6 +1399,12 @@ int plan_replay(ssw_sweep* s, int lvl) {
     std::vector<uint4>& tab = s->runtab[lvl];
     plan.clear();
     tab.clear();
+    if (lvl == 0 && !s->groups.empty()) {  // full sweeps: the grouped replay
+        for (size_t i = 0; i < s->groups.size(); i++)
+            if (s->groups[i].ndesc) plan.push_back({s->groups[i].k0, s->groups[i].levels, (uint32_t)i, 1});
+        dfree(s, s->d_runtab[lvl]);
+        return SSW_OK;
+    }
     const bool runs = s->compiled && s->narrow_runs;
     int k = 0;
     while (k < s->kmax) {
@@ -1208,7 +1414,7 @@ int plan_replay(ssw_sweep* s, int lvl) {
             continue;
         }
         if (!runs || count > (uint32_t)kRunBlock) {
-            plan.push_back({k, 0, 0u});
+            plan.push_back({k, 0, 0u, 0});
             k++;
             continue;
         }
@@ -1220,10 +1426,10 @@ int plan_replay(ssw_sweep* s, int lvl) {
         }
         const int n = (int)(tab.size() - t0);
         if (n >= 2) {
-            plan.push_back({k, n, t0});
+            plan.push_back({k, n, t0, 0});
         } else {
             tab.resize(t0);
-            plan.push_back({k, 0, 0u});
+            plan.push_back({k, 0, 0u, 0});
         }
         k = k1;
     }
@@ -1235,6 +1441,93 @@ int plan_replay(ssw_sweep* s, int lvl) {
         dfree(s, s->d_runtab[lvl]);
     }
     return SSW_OK;
+}
+
+// The grouped replay of full sweeps (compiled replay, group_request >= 2):
+// groups of m consecutive level sets; in each group, from its last level set
+// down to its first, the level set's tasks in no cone yet become roots, in
+// blocks of kGroupT, whose cones are built (k_cone). If a cone exceeds the
+// LDS capacity or the task list its bound, full sweeps keep the per-level
+// replay (no groups).
+template <typename R>
+int build_groups_t(ssw_sweep* s) {
+    const int m = s->group_request;
+    const size_t DN = (size_t)s->D * s->N;
+    const std::vector<uint32_t>& fs = s->full_start;
+    uint32_t maxw = 1;
+    for (int k = 0; k < s->kmax; k++) maxw = std::max(maxw, fs[k + 1] - fs[k]);
+    const size_t desc_cap = DN / kGroupT + (size_t)s->kmax + 1;
+    const size_t list_cap = DN + DN / 4 + (1u << 20);
+    uint8_t* todo = nullptr;
+    uint32_t *roots = nullptr, *ctl = nullptr;  // ctl: cursor, fail, selected count
+    uint8_t* tmp = nullptr;
+    TRY(dalloc(s, &todo, DN));
+    TRY(dalloc(s, &roots, maxw));
+    TRY(dalloc(s, &ctl, 3));
+    TRY(dalloc(s, &s->d_glist, list_cap));
+    TRY(dalloc(s, &s->d_gdesc, desc_cap));
+    HIP_TRY(hipMemsetAsync(todo, 1, DN, s->stream));
+    HIP_TRY(hipMemsetAsync(ctl, 0, 12, s->stream));
+    size_t tb = 0;
+    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, tb, hipcub::CountingInputIterator<uint32_t>(0u), todo, roots,
+                                          ctl + 2, (int)maxw, s->stream));
+    TRY(dalloc(s, &tmp, tb));
+    const uint4* hdr = s->d_hdr;
+    const Entry<R>* planes = (const Entry<R>*)s->d_planes;
+    const Entry<R>* ovf = (const Entry<R>*)s->d_ovf;
+    s->groups.clear();
+    uint32_t ndesc = 0;
+    for (int k0 = 0; k0 < s->kmax; k0 += m) {
+        const int k1 = std::min(s->kmax, k0 + m), L = k1 - 1;
+        ssw_sweep::Group gr{k0, k1 - k0, ndesc, 0u};
+        for (int j = L; j >= k0; j--) {
+            const uint32_t W = fs[j + 1] - fs[j];
+            if (W == 0) continue;
+            uint32_t n = W;
+            const uint32_t* rl = nullptr;
+            if (j < L) {
+                size_t tbj = tb;
+                HIP_TRY(hipcub::DeviceSelect::Flagged(tmp, tbj, hipcub::CountingInputIterator<uint32_t>(fs[j]),
+                                                      todo + fs[j], roots, ctl + 2, (int)W, s->stream));
+                HIP_TRY(hipMemcpyAsync(&n, ctl + 2, 4, hipMemcpyDeviceToHost, s->stream));
+                HIP_TRY(hipStreamSynchronize(s->stream));
+                if (n == 0) continue;
+                rl = roots;
+            }
+            const uint32_t nb = (n + kGroupT - 1) / kGroupT;
+            if ((size_t)ndesc + nb > desc_cap) return fail(SSW_ERR_HIP, "grouped replay: descriptor bound exceeded");
+            k_cone<R><<<nb, kGroupT, 0, s->stream>>>(hdr, planes, ovf, s->d_fstart, k0, j, rl, fs[j], n, todo,
+                                                     s->d_glist, (uint32_t)list_cap, ctl, s->d_gdesc + ndesc, ctl + 1);
+            HIP_TRY(hipGetLastError());
+            ndesc += nb;
+        }
+        gr.ndesc = ndesc - gr.desc0;
+        s->groups.push_back(gr);
+    }
+    uint32_t hctl[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(hctl, ctl, 8, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    dfree(s, tmp);
+    dfree(s, ctl);
+    dfree(s, roots);
+    dfree(s, todo);
+    if (hctl[1] != 0) {  // a cone or the list did not fit: per-level replay
+        s->groups.clear();
+        dfree(s, s->d_glist);
+        dfree(s, s->d_gdesc);
+        s->group_fallback = true;
+        return SSW_OK;
+    }
+    s->group_tasks = hctl[0];
+    return SSW_OK;
+}
+
+int build_groups(ssw_sweep* s) {
+    s->groups.clear();
+    s->group_tasks = 0;
+    s->group_fallback = false;
+    if (!s->compiled || s->group_request < 2) return SSW_OK;
+    return s->fp32 ? build_groups_t<float>(s) : build_groups_t<double>(s);
 }
 
 // The full schedule, once per grid: Kahn layering of the whole graph, each
@@ -1345,6 +1638,7 @@ int build_schedule(ssw_sweep* s) {
     s->d_runtab.assign(L, nullptr);
     s->runtab.assign(L, std::vector<uint4>());
     s->plan.assign(L, std::vector<ssw_sweep::ReplayItem>());
+    TRY(build_groups(s));
     TRY(plan_replay(s, 0));
     return SSW_OK;
 }
@@ -1402,6 +1696,15 @@ void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<
     const Entry<R>* ovf = (const Entry<R>*)s->d_ovf;
     for (const ssw_sweep::ReplayItem& it : s->plan[lvl]) {
         hipEvent_t a = ev ? (*mark)() : nullptr;
+        if (it.group) {  // a group of level sets, its cones one workgroup each
+            const ssw_sweep::Group& gr = s->groups[it.tab];
+            k_sweep_group<R><<<gr.ndesc, kGroupT, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A,
+                                                                  (const Entry<R>*)s->d_planes, ovf, s->d_fstart,
+                                                                  gr.k0, gr.levels, s->d_gdesc + gr.desc0,
+                                                                  s->d_glist);
+            if (ev) ev->push_back({a, (*mark)()});
+            continue;
+        }
         if (it.levels > 0) {  // a run of narrow level sets, one workgroup
             k_sweep_comp_run<R><<<1, kRunBlock, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A,
                                                                  (const Entry<R>*)s->d_planes, ovf, idx,
@@ -1599,6 +1902,9 @@ void free_grid(ssw_sweep* s) {
     drop_all_lists(s);
     for (uint32_t*& p : s->d_idx) dfree(s, p);
     for (uint4*& p : s->d_runtab) dfree(s, p);
+    dfree(s, s->d_gdesc);
+    dfree(s, s->d_glist);
+    s->groups.clear();
     dfree(s, s->d_off);
     dfree(s, s->d_geo);
     dfree(s, s->d_nk);
@@ -1786,6 +2092,14 @@ int ssw_set_cell_positions(ssw_sweep* s, int64_t n, const double* positions) {
     return SSW_OK;
 }
 
+int ssw_set_group_levels(ssw_sweep* s, int32_t levels) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    if (levels < 1 || levels > kMaxGroup) return fail(SSW_ERR_INVALID, "group levels must be 1 to 4");
+    if (s->have_grid) return fail(SSW_ERR_INVALID, "ssw_set_group_levels must be called before ssw_upload_grid");
+    s->group_request = levels;
+    return SSW_OK;
+}
+
 int ssw_set_narrow_runs(ssw_sweep* s, int32_t enabled) {
     if (!s) return fail(SSW_ERR_INVALID, "null handle");
     if (s->have_grid) return fail(SSW_ERR_INVALID, "ssw_set_narrow_runs must be called before ssw_upload_grid");
@@ -1921,7 +2235,8 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
     const size_t rs = real_size(s);
     const size_t comp_bytes = DN * (16 + rs + kPlanes * entry_size(s) + 4) + (size_t)s->ovf_total * entry_size(s);
-    const size_t comp_need = comp_bytes + DN * std::max<size_t>(8, 4 * (size_t)(L - 1) + 2 * rs);
+    const size_t comp_need = comp_bytes + DN * std::max<size_t>(8, 4 * (size_t)(L - 1) + 2 * rs) +
+                             (s->group_request >= 2 ? DN * 7 : 0);  // grouped replay: task list, build scratch
     if (s->mode_request == SSW_TRANSPORT_MODE_COMPILED) s->compiled = true;
     else if (s->mode_request == SSW_TRANSPORT_MODE_GEOMETRIC) s->compiled = false;
     else s->compiled = (double)comp_need <= 0.85 * (double)free_b;
@@ -2161,6 +2476,9 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info) {
     info->replay_block = s->rec_block;
     info->graph_replay = s->graphs_on ? 1 : 0;
     info->narrow_runs = (s->compiled && s->narrow_runs) ? 1 : 0;
+    info->group_levels = s->groups.empty() ? 1 : s->group_request;
+    info->group_fallback = s->group_fallback ? 1 : 0;
+    info->group_tasks = s->group_tasks;
     info->locality_order = s->permuted ? 1 : 0;
     if (s->have_state) {
         unsigned long long f = 0, a = 0;
