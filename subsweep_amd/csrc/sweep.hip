@@ -82,7 +82,8 @@ constexpr int kQueueCap = kTasksPerBlock * kMaxFaces;
 constexpr int kPlanes = 12;                // upwind entries per task held in planes
 constexpr int kChunk = 4096;               // compaction chunk (tasks, within one level set)
 constexpr int kRunBlock = 1024;            // workgroup of a run of narrow level sets (tasks per level set)
-constexpr int kGroupT = 128;               // grouped replay: root tasks (and lanes) per workgroup
+constexpr int kGroupT = 128;               // grouped replay build: root tasks (and lanes) per workgroup
+constexpr int kGroupExec = 256;            // grouped replay: lanes per workgroup
 constexpr int kConeCap = 4096;             // grouped replay build: upwind-cone entries per workgroup
 constexpr int kMaxGroup = 4;               // grouped replay: level sets per group
 
@@ -630,38 +631,73 @@ __global__ void __launch_bounds__(kBlock) k_sweep_rec(Geo g, Flux<R> fl, const u
     if (valid && sub == 0) finish_task<R>(fl, (size_t)d * g.N + c, (size_t)d * g.N + c, inc, ci, (R)A);
 }
 
-// One task of the compiled replay: the task at position p of a level set that
-// starts at schedule position b and has width W, whose header, A and plane
-// entries start at hdr, Av and planes. The header, A and the plane entries
-// are independent loads; the upwind X gathers (at the upwind tasks' schedule
-// positions) and the cell inputs follow. The task's X goes to position b + p.
+// One task of the compiled replay in three steps, so that the kernels that
+// solve several level sets in one workgroup can load the next level set's
+// schedule data while the current level set's gathers are in flight:
+//   load_task   the schedule data of the task at position p of a level set
+//               of width W whose header, A and plane entries start at hdr,
+//               Av and planes (independent loads, no dependence on X);
+//   issue_task  the upwind X gathers (at the upwind tasks' schedule
+//               positions; entries past the upwind count gather the task's
+//               own X, a valid index, and are not summed) and the cell inputs;
+//   finish_task_c  the upwind sum in entry order, the overflow entries, and
+//               the stores (X at the task's schedule position `self`).
+template <typename R>
+struct TaskS {
+    Entry<R> e[kPlanes];
+    uint4 h;
+    R A;
+};
+
+template <typename R>
+struct TaskG {
+    R x[kPlanes];
+    CellIn<R> ci;
+};
+
+template <typename R>
+__device__ __forceinline__ void load_task(TaskS<R>& t, const uint4* hdr, const R* Av, const Entry<R>* planes,
+                                          uint32_t W, uint32_t p) {
+#pragma unroll
+    for (int q = 0; q < kPlanes; q++) t.e[q] = planes[(size_t)q * W + p];
+    t.h = hdr[p];
+    t.A = Av[p];
+}
+
+template <typename R>
+__device__ __forceinline__ void issue_task(TaskG<R>& G, const Geo& g, const Flux<R>& fl, const TaskS<R>& t,
+                                           uint32_t self) {
+    const uint32_t c = t.h.x & kCellMask, d = t.h.x >> kCellBits, ps = t.h.y, nup = t.h.w;
+    G.ci = cell_inputs<R>(g, fl, c, d, ps);
+#pragma unroll
+    for (int q = 0; q < kPlanes; q++) G.x[q] = fl.X[(uint32_t)q < nup ? t.e[q].nb : self];
+}
+
+template <typename R>
+__device__ __forceinline__ void finish_task_c(const Geo& g, const Flux<R>& fl, const TaskS<R>& t, const TaskG<R>& G,
+                                              const Entry<R>* ovf, uint32_t self) {
+    const uint32_t c = t.h.x & kCellMask, d = t.h.x >> kCellBits, nup = t.h.w;
+    R inc = 0;
+#pragma unroll
+    for (int q = 0; q < kPlanes; q++) inc += ((uint32_t)q < nup) ? t.e[q].w * G.x[q] : (R)0;
+    for (uint32_t q = kPlanes; q < nup; q++) {
+        const Entry<R> o = ovf[t.h.z + q - kPlanes];
+        inc += o.w * fl.X[o.nb];
+    }
+    finish_task<R>(fl, self, (size_t)d * g.N + c, inc, G.ci, t.A);
+}
+
+// The three steps back to back: the task at position p of the level set that
+// starts at schedule position b and has width W.
 template <typename R>
 __device__ __forceinline__ void comp_task(const Geo& g, const Flux<R>& fl, const uint4* hdr, const R* Av,
                                           const Entry<R>* planes, const Entry<R>* ovf, uint32_t b, uint32_t W,
                                           uint32_t p) {
-    // first memory step: the plane entries, the header and A (independent)
-    Entry<R> e[kPlanes];
-#pragma unroll
-    for (int q = 0; q < kPlanes; q++) e[q] = planes[(size_t)q * W + p];
-    const uint4 h = hdr[p];
-    const R A = Av[p];
-    const uint32_t c = h.x & kCellMask, d = h.x >> kCellBits, ps = h.y, nup = h.w;
-    const size_t dN = (size_t)d * g.N;
-    // second memory step: the upwind X (entries past the upwind count gather
-    // the task's own X, a valid index, and are not summed), the cell inputs
-    const CellIn<R> ci = cell_inputs<R>(g, fl, c, d, ps);
-    const uint32_t self = b + p;
-    R x[kPlanes];
-#pragma unroll
-    for (int q = 0; q < kPlanes; q++) x[q] = fl.X[(uint32_t)q < nup ? e[q].nb : self];
-    R inc = 0;
-#pragma unroll
-    for (int q = 0; q < kPlanes; q++) inc += ((uint32_t)q < nup) ? e[q].w * x[q] : (R)0;
-    for (uint32_t q = kPlanes; q < nup; q++) {
-        const Entry<R> o = ovf[h.z + q - kPlanes];
-        inc += o.w * fl.X[o.nb];
-    }
-    finish_task<R>(fl, self, dN + c, inc, ci, A);
+    TaskS<R> t;
+    TaskG<R> G;
+    load_task<R>(t, hdr, Av, planes, W, p);
+    issue_task<R>(G, g, fl, t, b + p);
+    finish_task_c<R>(g, fl, t, G, ovf, b + p);
 }
 
 // Compiled replay of one level set of width W: one lane per task. With an
@@ -675,23 +711,61 @@ __global__ void __launch_bounds__(kRecBlock) k_sweep_comp(Geo g, Flux<R> fl, con
     comp_task<R>(g, fl, hdr, Av, planes, ovf, b, W, idx ? idx[i] : i);
 }
 
+// Lanes of the workgroup that solves a run of narrow level sets: at most this
+// many active tasks per level set (FP64 tasks hold twice the registers).
+template <typename R>
+constexpr int run_block() {
+    return sizeof(R) == 4 ? kRunBlock : kRunBlock / 2;
+}
+
 // Compiled replay of a run of consecutive narrow level sets (at most
-// kRunBlock active tasks each) by one workgroup, level set after level set,
-// with a workgroup barrier in between: every wave of the workgroup runs on
-// one CU, whose L1 is coherent within the workgroup, so the X stored for
+// run_block<R>() active tasks each) by one workgroup, level set after level
+// set, with a workgroup barrier in between: every wave of the workgroup runs
+// on one CU, whose L1 is coherent within the workgroup, so the X stored for
 // one level set is seen by the next after the barrier. run[j] = {first
 // position b, width W, first index list entry, active count} of the run's
 // j-th level set; without an index list entry i is position i.
+// Software pipelined: while level set j's gathers are in flight, level set
+// j+1's schedule data, level set j+2's index entries and run[j+3] are loaded,
+// none of which level set j waits for. Every load is unconditional (lanes
+// without a task, and level sets past the run's end, read a valid clamped
+// entry) so that the waits stay exact; only the stores are predicated.
 template <typename R>
-__global__ void __launch_bounds__(kRunBlock) k_sweep_comp_run(Geo g, Flux<R> fl, const uint4* hdr, const R* Av,
-                                                              const Entry<R>* planes, const Entry<R>* ovf,
-                                                              const uint32_t* idx, const uint4* run, int levels) {
+__global__ void __launch_bounds__(run_block<R>()) k_sweep_comp_run(Geo g, Flux<R> fl, const uint4* __restrict__ hdr,
+                                                                   const R* __restrict__ Av,
+                                                                   const Entry<R>* __restrict__ planes,
+                                                                   const Entry<R>* __restrict__ ovf,
+                                                                   const uint32_t* __restrict__ idx,
+                                                                   const uint4* __restrict__ run, int levels) {
+    const uint32_t i = threadIdx.x;
+    const int last = levels - 1;
+    // position of lane i in level set r (a valid position for every lane)
+    auto pos = [&](const uint4& r) -> uint32_t {
+        const uint32_t k = min(i, r.w - 1u);
+        return idx ? idx[r.z + k] : k;
+    };
+    uint4 r = run[0];
+    uint32_t p = pos(r);
+    TaskS<R> cur;
+    load_task<R>(cur, hdr + r.x, Av + r.x, planes + (size_t)kPlanes * r.x, r.y, p);
+    uint4 rn = run[min(1, last)];
+    uint32_t pn = pos(rn);
+    uint4 rnn = run[min(2, last)];
     for (int j = 0; j < levels; j++) {
-        const uint4 r = run[j];
-        const uint32_t b = r.x, W = r.y, i = threadIdx.x;
-        if (i < r.w)
-            comp_task<R>(g, fl, hdr + b, Av + b, planes + (size_t)kPlanes * b, ovf, b, W, idx ? idx[r.z + i] : i);
+        TaskG<R> G;
+        issue_task<R>(G, g, fl, cur, r.x + p);
+        TaskS<R> nxt;
+        load_task<R>(nxt, hdr + rn.x, Av + rn.x, planes + (size_t)kPlanes * rn.x, rn.y, pn);
+        const uint32_t pnn = pos(rnn);
+        const uint4 rnnn = run[min(j + 3, last)];
+        if (i < r.w) finish_task_c<R>(g, fl, cur, G, ovf, r.x + p);
         __syncthreads();
+        r = rn;
+        p = pn;
+        cur = nxt;
+        rn = rnn;
+        pn = pnn;
+        rnn = rnnn;
     }
 }
 
@@ -860,22 +934,53 @@ __global__ void __launch_bounds__(kGroupT) k_cone(const uint4* hdr, const Entry<
 }
 
 // Grouped replay of the level sets k0 .. k0+phases-1: workgroup w solves the
-// tasks of its descriptor, level set by level set, barrier in between.
+// tasks of its descriptor, level set by level set, barrier in between. Each
+// lane's first task of every level set is read from the list up front, and
+// the next level set's schedule data is loaded while the current level set's
+// gathers are in flight; loads are unconditional (lanes without a task read
+// the level set's first position) so that the waits stay exact. Tasks beyond
+// the first kGroupExec of a level set are solved without this overlap.
 template <typename R>
-__global__ void __launch_bounds__(kGroupT) k_sweep_group(Geo g, Flux<R> fl, const uint4* hdr, const R* Av,
-                                                         const Entry<R>* planes, const Entry<R>* ovf,
-                                                         const uint32_t* fstart, int k0, int phases,
-                                                         const uint4* desc, const uint32_t* list) {
+__global__ void __launch_bounds__(kGroupExec) k_sweep_group(Geo g, Flux<R> fl, const uint4* __restrict__ hdr,
+                                                            const R* __restrict__ Av,
+                                                            const Entry<R>* __restrict__ planes,
+                                                            const Entry<R>* __restrict__ ovf,
+                                                            const uint32_t* __restrict__ fstart, int k0, int phases,
+                                                            const uint4* __restrict__ desc,
+                                                            const uint32_t* __restrict__ list, uint32_t list_len) {
     const uint4 dsc = desc[blockIdx.x];
-    uint32_t off = dsc.x;
-    for (int ph = 0; ph < phases; ph++) {
-        const uint32_t word = ph < 2 ? dsc.y : dsc.z;
-        const uint32_t n = (ph & 1) ? (word >> 16) : (word & 0xFFFFu);
-        const uint32_t b = fstart[k0 + ph], W = fstart[k0 + ph + 1] - b;
-        for (uint32_t i = threadIdx.x; i < n; i += kGroupT)
-            comp_task<R>(g, fl, hdr + b, Av + b, planes + (size_t)kPlanes * b, ovf, b, W, list[off + i] - b);
-        off += n;
-        __syncthreads();
+    const uint32_t tid = threadIdx.x;
+    uint32_t n[kMaxGroup], off[kMaxGroup], bb[kMaxGroup], WW[kMaxGroup], P[kMaxGroup];
+    n[0] = dsc.y & 0xFFFFu;
+    n[1] = dsc.y >> 16;
+    n[2] = dsc.z & 0xFFFFu;
+    n[3] = dsc.z >> 16;
+#pragma unroll
+    for (int ph = 0; ph < kMaxGroup; ph++) {
+        off[ph] = ph == 0 ? dsc.x : off[ph - 1] + n[ph - 1];
+        const int k = k0 + min(ph, phases - 1);
+        bb[ph] = fstart[k];
+        WW[ph] = fstart[k + 1] - bb[ph];
+        const uint32_t v = list[min(off[ph] + tid, list_len - 1u)];
+        P[ph] = (ph < phases && tid < n[ph]) ? v : bb[ph];
+    }
+    TaskS<R> cur;
+    load_task<R>(cur, hdr + bb[0], Av + bb[0], planes + (size_t)kPlanes * bb[0], WW[0], P[0] - bb[0]);
+#pragma unroll
+    for (int ph = 0; ph < kMaxGroup; ph++) {
+        if (ph < phases) {
+            TaskG<R> G;
+            issue_task<R>(G, g, fl, cur, P[ph]);
+            TaskS<R> nxt;
+            const int q = ph + 1 < kMaxGroup ? ph + 1 : ph;
+            load_task<R>(nxt, hdr + bb[q], Av + bb[q], planes + (size_t)kPlanes * bb[q], WW[q], P[q] - bb[q]);
+            if (tid < n[ph]) finish_task_c<R>(g, fl, cur, G, ovf, P[ph]);
+            for (uint32_t i = tid + kGroupExec; i < n[ph]; i += kGroupExec)
+                comp_task<R>(g, fl, hdr + bb[ph], Av + bb[ph], planes + (size_t)kPlanes * bb[ph], ovf, bb[ph],
+                             WW[ph], list[off[ph] + i] - bb[ph]);
+            __syncthreads();
+            cur = nxt;
+        }
     }
 }
 
@@ -1103,6 +1208,7 @@ struct ssw_sweep {
     uint4* d_gdesc = nullptr;
     uint32_t* d_glist = nullptr;
     int64_t group_tasks = 0;                        // tasks solved per full sweep by the grouped replay
+    uint32_t glist_len = 0;                         // entries of the grouped replay's task list
     bool group_fallback = false;                    // groups requested but a cone did not fit
     bool compiled = false;
     // grid
@@ -1389,7 +1495,7 @@ int kahn_layering(ssw_sweep* s, int lvl, int64_t expected, int* depth_out, std::
 // The replay plan of sweep level lvl from its level starts (the full
 // schedule's for level 0, the active-set list's otherwise): in the compiled
 // replay, each maximal run of at least two consecutive non-empty level sets of
-// at most kRunBlock active tasks (empty level sets inside a run are skipped)
+// at most run_block<R>() active tasks (empty level sets inside a run are skipped)
 // becomes one single-workgroup launch over a run table entry per level set;
 // every other non-empty level set is one launch.
 int plan_replay(ssw_sweep* s, int lvl) {
@@ -1406,6 +1512,7 @@ int plan_replay(ssw_sweep* s, int lvl) {
         return SSW_OK;
     }
     const bool runs = s->compiled && s->narrow_runs;
+    const uint32_t rb = s->fp32 ? (uint32_t)run_block<float>() : (uint32_t)run_block<double>();
     int k = 0;
     while (k < s->kmax) {
         const uint32_t count = cs[k + 1] - cs[k];
@@ -1413,14 +1520,14 @@ int plan_replay(ssw_sweep* s, int lvl) {
             k++;
             continue;
         }
-        if (!runs || count > (uint32_t)kRunBlock) {
+        if (!runs || count > rb) {
             plan.push_back({k, 0, 0u, 0});
             k++;
             continue;
         }
         const uint32_t t0 = (uint32_t)tab.size();
         int k1 = k;
-        for (; k1 < s->kmax && cs[k1 + 1] - cs[k1] <= (uint32_t)kRunBlock; k1++) {
+        for (; k1 < s->kmax && cs[k1 + 1] - cs[k1] <= rb; k1++) {
             const uint32_t c = cs[k1 + 1] - cs[k1];
             if (c) tab.push_back(make_uint4(fs[k1], fs[k1 + 1] - fs[k1], cs[k1], c));
         }
@@ -1519,6 +1626,7 @@ int build_groups_t(ssw_sweep* s) {
         return SSW_OK;
     }
     s->group_tasks = hctl[0];
+    s->glist_len = std::max<uint32_t>(1u, hctl[0]);
     return SSW_OK;
 }
 
@@ -1698,15 +1806,15 @@ void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<
         hipEvent_t a = ev ? (*mark)() : nullptr;
         if (it.group) {  // a group of level sets, its cones one workgroup each
             const ssw_sweep::Group& gr = s->groups[it.tab];
-            k_sweep_group<R><<<gr.ndesc, kGroupT, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A,
+            k_sweep_group<R><<<gr.ndesc, kGroupExec, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A,
                                                                   (const Entry<R>*)s->d_planes, ovf, s->d_fstart,
                                                                   gr.k0, gr.levels, s->d_gdesc + gr.desc0,
-                                                                  s->d_glist);
+                                                                  s->d_glist, s->glist_len);
             if (ev) ev->push_back({a, (*mark)()});
             continue;
         }
         if (it.levels > 0) {  // a run of narrow level sets, one workgroup
-            k_sweep_comp_run<R><<<1, kRunBlock, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A,
+            k_sweep_comp_run<R><<<1, run_block<R>(), 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A,
                                                                  (const Entry<R>*)s->d_planes, ovf, idx,
                                                                  s->d_runtab[lvl] + it.tab, it.levels);
             if (ev) ev->push_back({a, (*mark)()});
