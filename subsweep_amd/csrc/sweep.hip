@@ -984,6 +984,57 @@ __global__ void __launch_bounds__(kGroupExec) k_sweep_group(Geo g, Flux<R> fl, c
     }
 }
 
+// The grouped replay of a restricted sweep (active set {level >= lvl}):
+// workgroup w filters descriptor w's task list to the active tasks, in order,
+// into the same place of list_out, and writes the filtered counts; the active
+// tasks of each group are summed into gcount (gmap: group of each
+// descriptor). The cones of the full graph contain the active cones, so every
+// dependency of an active task inside a launch stays inside its workgroup.
+__global__ void __launch_bounds__(kBlock) k_group_compact(Geo g, const uint4* __restrict__ hdr, int lvl,
+                                                          const uint4* __restrict__ desc_in,
+                                                          const uint32_t* __restrict__ list_in,
+                                                          const uint32_t* __restrict__ gmap, uint32_t ndesc,
+                                                          uint4* desc_out, uint32_t* list_out, uint32_t* gcount) {
+    const uint32_t w = blockIdx.x;
+    if (w >= ndesc) return;
+    __shared__ uint32_t wave_tot[kBlock / 64];
+    const uint4 d = desc_in[w];
+    const uint32_t n[kMaxGroup] = {d.y & 0xFFFFu, d.y >> 16, d.z & 0xFFFFu, d.z >> 16};
+    uint32_t m[kMaxGroup] = {0, 0, 0, 0};
+    const int lane = __lane_id(), wv = threadIdx.x / 64;
+    uint32_t src = d.x, dst = d.x;
+    for (int ph = 0; ph < kMaxGroup; ph++) {
+        const uint32_t base = dst;
+        for (uint32_t p0 = 0; p0 < n[ph]; p0 += kBlock) {
+            const uint32_t i = p0 + threadIdx.x;
+            uint32_t P = 0;
+            bool act = false;
+            if (i < n[ph]) {
+                P = list_in[src + i];
+                act = g.level[hdr[P].x & kCellMask] >= lvl;
+            }
+            const unsigned long long bal = __ballot(act);
+            if (lane == 0) wave_tot[wv] = (uint32_t)__popcll(bal);
+            __syncthreads();
+            uint32_t before = 0, all = 0;
+            for (int k = 0; k < kBlock / 64; k++) {
+                if (k < wv) before += wave_tot[k];
+                all += wave_tot[k];
+            }
+            if (act) list_out[dst + before + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = P;
+            dst += all;
+            __syncthreads();
+        }
+        m[ph] = dst - base;
+        src += n[ph];
+    }
+    if (threadIdx.x == 0) {
+        desc_out[w] = make_uint4(d.x, m[0] | (m[1] << 16), m[2] | (m[3] << 16), d.w);
+        const uint32_t total = m[0] + m[1] + m[2] + m[3];
+        if (total) atomicAdd(&gcount[gmap[w]], total);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // per-sweep kernels
 
@@ -1207,6 +1258,14 @@ struct ssw_sweep {
     std::vector<Group> groups;
     uint4* d_gdesc = nullptr;
     uint32_t* d_glist = nullptr;
+    uint32_t* d_gmap = nullptr;                     // group of each descriptor
+    uint32_t ndesc = 0;
+    // restricted sweeps (level >= 1): filtered descriptors and lists, active
+    // tasks per group
+    std::vector<uint4*> d_gdesc_l;
+    std::vector<uint32_t*> d_glist_l;
+    uint32_t* d_gcount = nullptr;
+    std::vector<std::vector<uint32_t>> gcount;
     int64_t group_tasks = 0;                        // tasks solved per full sweep by the grouped replay
     uint32_t glist_len = 0;                         // entries of the grouped replay's task list
     bool group_fallback = false;                    // groups requested but a cone did not fit
@@ -1505,9 +1564,10 @@ int plan_replay(ssw_sweep* s, int lvl) {
     std::vector<uint4>& tab = s->runtab[lvl];
     plan.clear();
     tab.clear();
-    if (lvl == 0 && !s->groups.empty()) {  // full sweeps: the grouped replay
+    if (!s->groups.empty()) {  // the grouped replay (restricted sweeps: groups with active tasks)
         for (size_t i = 0; i < s->groups.size(); i++)
-            if (s->groups[i].ndesc) plan.push_back({s->groups[i].k0, s->groups[i].levels, (uint32_t)i, 1});
+            if (s->groups[i].ndesc && (lvl == 0 || s->gcount[lvl][i] > 0))
+                plan.push_back({s->groups[i].k0, s->groups[i].levels, (uint32_t)i, 1});
         dfree(s, s->d_runtab[lvl]);
         return SSW_OK;
     }
@@ -1627,6 +1687,14 @@ int build_groups_t(ssw_sweep* s) {
     }
     s->group_tasks = hctl[0];
     s->glist_len = std::max<uint32_t>(1u, hctl[0]);
+    s->ndesc = ndesc;
+    std::vector<uint32_t> gmap(std::max<uint32_t>(1u, ndesc), 0u);
+    for (size_t i = 0; i < s->groups.size(); i++)
+        for (uint32_t k = 0; k < s->groups[i].ndesc; k++) gmap[s->groups[i].desc0 + k] = (uint32_t)i;
+    TRY(dalloc(s, &s->d_gmap, gmap.size()));
+    HIP_TRY(hipMemcpyAsync(s->d_gmap, gmap.data(), gmap.size() * 4, hipMemcpyHostToDevice, s->stream));
+    TRY(dalloc(s, &s->d_gcount, std::max<size_t>(1, s->groups.size())));
+    HIP_TRY(hipStreamSynchronize(s->stream));
     return SSW_OK;
 }
 
@@ -1744,6 +1812,11 @@ int build_schedule(ssw_sweep* s) {
     s->graphs.assign(L, nullptr);
     for (uint4*& p : s->d_runtab) dfree(s, p);
     s->d_runtab.assign(L, nullptr);
+    for (uint4*& p : s->d_gdesc_l) dfree(s, p);
+    for (uint32_t*& p : s->d_glist_l) dfree(s, p);
+    s->d_gdesc_l.assign(L, nullptr);
+    s->d_glist_l.assign(L, nullptr);
+    s->gcount.assign(L, std::vector<uint32_t>());
     s->runtab.assign(L, std::vector<uint4>());
     s->plan.assign(L, std::vector<ssw_sweep::ReplayItem>());
     TRY(build_groups(s));
@@ -1756,7 +1829,33 @@ const uint4* schedule_tasks(ssw_sweep* s) { return s->compiled ? s->d_hdr : s->d
 // The active-set list of sweep level lvl >= 1: the positions of the full
 // schedule whose cell has level >= lvl, per level set, and its level starts
 // on the host. Sweep level 0 covers every task and needs no list.
+int make_group_list(ssw_sweep* s, int lvl) {
+    if (!s->d_gdesc_l[lvl]) TRY(dalloc(s, &s->d_gdesc_l[lvl], std::max<uint32_t>(1u, s->ndesc)));
+    if (!s->d_glist_l[lvl]) TRY(dalloc(s, &s->d_glist_l[lvl], s->glist_len));
+    const size_t ng = std::max<size_t>(1, s->groups.size());
+    HIP_TRY(hipMemsetAsync(s->d_gcount, 0, ng * 4, s->stream));
+    Geo g = make_geo(s);
+    if (s->ndesc)
+        k_group_compact<<<s->ndesc, kBlock, 0, s->stream>>>(g, s->d_hdr, lvl, s->d_gdesc, s->d_glist, s->d_gmap,
+                                                            s->ndesc, s->d_gdesc_l[lvl], s->d_glist_l[lvl],
+                                                            s->d_gcount);
+    HIP_TRY(hipGetLastError());
+    std::vector<uint32_t>& gc = s->gcount[lvl];
+    gc.assign(ng, 0u);
+    HIP_TRY(hipMemcpyAsync(gc.data(), s->d_gcount, ng * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->list_ok[lvl] = 1;
+    s->replays[lvl] = 0;
+    if (s->graphs[lvl]) {
+        (void)hipGraphExecDestroy(s->graphs[lvl]);
+        s->graphs[lvl] = nullptr;
+    }
+    TRY(plan_replay(s, lvl));
+    return SSW_OK;
+}
+
 int make_list(ssw_sweep* s, int lvl) {
+    if (!s->groups.empty()) return make_group_list(s, lvl);
     const size_t DN = (size_t)s->D * s->N;
     if (!s->d_idx[lvl]) TRY(dalloc(s, &s->d_idx[lvl], DN));
     Geo g = make_geo(s);
@@ -1806,10 +1905,11 @@ void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<
         hipEvent_t a = ev ? (*mark)() : nullptr;
         if (it.group) {  // a group of level sets, its cones one workgroup each
             const ssw_sweep::Group& gr = s->groups[it.tab];
+            const uint4* dsc = (all ? s->d_gdesc : s->d_gdesc_l[lvl]) + gr.desc0;
+            const uint32_t* gl = all ? s->d_glist : s->d_glist_l[lvl];
             k_sweep_group<R><<<gr.ndesc, kGroupExec, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A,
                                                                   (const Entry<R>*)s->d_planes, ovf, s->d_fstart,
-                                                                  gr.k0, gr.levels, s->d_gdesc + gr.desc0,
-                                                                  s->d_glist, s->glist_len);
+                                                                  gr.k0, gr.levels, dsc, gl, s->glist_len);
             if (ev) ev->push_back({a, (*mark)()});
             continue;
         }
@@ -2012,6 +2112,10 @@ void free_grid(ssw_sweep* s) {
     for (uint4*& p : s->d_runtab) dfree(s, p);
     dfree(s, s->d_gdesc);
     dfree(s, s->d_glist);
+    dfree(s, s->d_gmap);
+    dfree(s, s->d_gcount);
+    for (uint4*& p : s->d_gdesc_l) dfree(s, p);
+    for (uint32_t*& p : s->d_glist_l) dfree(s, p);
     s->groups.clear();
     dfree(s, s->d_off);
     dfree(s, s->d_geo);
