@@ -109,6 +109,7 @@ def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3,
         "graph_replay": bool(info.graph_replay),
         "narrow_runs": bool(info.narrow_runs),
         "group_levels": int(info.group_levels),
+        "group_block": int(info.group_block),
         "group_tasks_per_full_sweep": int(info.group_tasks),
         "transport_mode": int(info.transport_mode),
         "info": info,
@@ -170,9 +171,10 @@ def tuned(pb, precision: int, warmup: int, enabled: bool, mode: int, block: int)
     t = time.perf_counter()
     r = autotune(pb.with_params(transport_precision=precision), warmup=warmup)
     log(f"autotune fp{precision}: mode {r.transport_mode}, block {r.replay_block}, graphs {r.graph_replay}, "
-        f"narrow runs {r.narrow_runs}, group levels {r.group_levels}, {r.ms_per_step:.2f} ms/step "
-        f"({time.perf_counter() - t:.1f} s; " +
-        ", ".join(f"m{m}/b{b}/g{int(gr)}/n{int(nr)}/l{gl} {ms:.2f}" for m, b, gr, nr, gl, ms in r.table) + ")")
+        f"narrow runs {r.narrow_runs}, group levels {r.group_levels}, group block {r.group_block}, "
+        f"{r.ms_per_step:.2f} ms/step ({time.perf_counter() - t:.1f} s; " +
+        ", ".join(f"m{m}/b{b}/g{int(gr)}/n{int(nr)}/l{gl}x{gb} {ms:.2f}" for m, b, gr, nr, gl, gb, ms in r.table) +
+        ")")
     return r.kwargs(), [list(x) for x in r.table]
 
 

@@ -160,6 +160,8 @@ typedef struct ssw_info {
                                       not fit (full sweeps per level set)      */
     int64_t group_tasks;           /* tasks a grouped full sweep solves, cones
                                       overlapping (>= the sweep's tasks)       */
+    int32_t group_block;           /* grouped replay: root tasks per workgroup */
+    int32_t reserved3;
 } ssw_info;
 
 int ssw_get_info(ssw_sweep* s, ssw_info* info);
@@ -212,17 +214,23 @@ int ssw_set_graph_replay(ssw_sweep* s, int32_t enabled);
  * in the reference. */
 int ssw_set_narrow_runs(ssw_sweep* s, int32_t enabled);
 
-/* Compiled replay of full sweeps (every cell active): groups of `levels`
- * consecutive level sets (2 to 4; 1 = one launch per level set, default) run
- * as one launch each. A workgroup owns 128 root tasks of one level set of the
- * group and solves their upwind cone inside the group level set by level set
- * with a workgroup barrier in between; a task in several cones is solved by
- * each of them from the same inputs with the same arithmetic, so results are
- * bitwise the same as without groups. The cones are built once per grid at
- * ssw_upload_grid; if one exceeds its bound the full sweeps stay per level
- * set (ssw_info.group_fallback). Set before ssw_upload_grid. No counterpart in
- * the reference. */
+/* Compiled replay: groups of `levels` consecutive level sets (2 to 8; 1 = one
+ * launch per level set, default) run as one launch each. A workgroup owns a
+ * block of root tasks (ssw_set_group_block) of one level set of the group and
+ * solves their upwind cone inside the group level set by level set with a
+ * workgroup barrier in between; a task in several cones is solved by each of
+ * them from the same inputs with the same arithmetic, so results are bitwise
+ * the same as without groups. The cones of the full graph are built once per
+ * grid at ssw_upload_grid; a restricted sweep filters them to its active set
+ * whenever that changes. If a cone exceeds its bound the sweeps stay per
+ * level set (ssw_info.group_fallback). Set before ssw_upload_grid. No
+ * counterpart in the reference. */
 int ssw_set_group_levels(ssw_sweep* s, int32_t levels);
+
+/* Root tasks per workgroup of the grouped replay: 64 or 128 (0 = 128,
+ * default); the workgroup has twice as many lanes. Set before
+ * ssw_upload_grid. No counterpart in the reference. */
+int ssw_set_group_block(ssw_sweep* s, int32_t roots);
 
 /* Raw transport state in its storage precision and scale (FP32 or FP64 per
  * the transport precision, direction-major): X = outgoing rate / downwind

@@ -74,6 +74,8 @@ class InfoC(C.Structure):
         ("group_levels", C.c_int32),
         ("group_fallback", C.c_int32),
         ("group_tasks", C.c_int64),
+        ("group_block", C.c_int32),
+        ("reserved3", C.c_int32),
     ]
 
 
@@ -152,6 +154,7 @@ SWEEP_SIGNATURES = {
     "ssw_set_graph_replay": (C.c_int, [P, C.c_int32]),
     "ssw_set_narrow_runs": (C.c_int, [P, C.c_int32]),
     "ssw_set_group_levels": (C.c_int, [P, C.c_int32]),
+    "ssw_set_group_block": (C.c_int, [P, C.c_int32]),
     "ssw_set_cell_positions": (C.c_int, [P, C.c_int64, PD]),
     "ssw_get_timing": (C.c_int, [P, PD, PD, PI64, PD, PD]),
     "ssw_download_transport": (C.c_int, [P, P, P]),

@@ -11,9 +11,9 @@ replayed:
   one at a time;
 * narrow runs - runs of narrow level sets solved by one workgroup per run,
   or one launch per level set (compiled replay);
-* group levels - full sweeps as launches of 2-4 consecutive level sets
-  (overlapping upwind cones, one workgroup each), or one launch per level
-  set (compiled replay).
+* group levels and group block - launches of 2-8 consecutive level sets
+  (overlapping upwind cones of 64 or 128 root tasks, one workgroup each), or
+  one launch per level set (compiled replay).
 
 `autotune` times a few steps of each candidate on the problem itself and
 returns the fastest. Every candidate computes bitwise the same results, so
@@ -37,34 +37,35 @@ class TuneResult:
     graph_replay: bool
     narrow_runs: bool
     group_levels: int
+    group_block: int
     ms_per_step: float
-    table: list  # (mode, block, graphs, narrow runs, group levels, ms per step) of every candidate
+    table: list  # (mode, block, graphs, narrow runs, group levels, group block, ms per step) per candidate
 
     def kwargs(self) -> dict:
         return {"transport_mode": self.transport_mode, "replay_block": self.replay_block,
                 "graph_replay": self.graph_replay, "narrow_runs": self.narrow_runs,
-                "group_levels": self.group_levels}
+                "group_levels": self.group_levels, "group_block": self.group_block}
 
 
-def candidates(blocks=(64, 128, 256), groups=(2, 3, 4)):
+def candidates(blocks=(64, 128), groups=(4, 6, 8), group_blocks=(128, 64)):
     C = N.TRANSPORT_MODE_COMPILED
-    out = [(C, b, True, True, 1) for b in blocks]
-    out.append((C, blocks[0], False, True, 1))
-    out.append((C, blocks[0], True, False, 1))
-    out += [(C, blocks[0], True, True, m) for m in groups]
-    out.append((N.TRANSPORT_MODE_GEOMETRIC, 0, True, False, 1))  # one launch shape (16 lanes per task)
+    out = [(C, b, True, True, 1, 0) for b in blocks]
+    out.append((C, blocks[0], False, True, 1, 0))
+    out.append((C, blocks[0], True, False, 1, 0))
+    out += [(C, blocks[0], True, True, m, gb) for m in groups for gb in group_blocks]
+    out.append((N.TRANSPORT_MODE_GEOMETRIC, 0, True, False, 1, 0))  # one launch shape (16 lanes per task)
     return out
 
 
 def autotune(problem: Problem, cands=None, steps: int = 2, warmup: int = 2) -> TuneResult:
     """Times `steps` steps after `warmup` steps of every (mode, block, graphs,
-    narrow runs, group levels) candidate on a fresh handle each; candidates
-    that do not fit in device memory are skipped."""
+    narrow runs, group levels, group block) candidate on a fresh handle each;
+    candidates that do not fit in device memory are skipped."""
     table = []
-    for m, b, gr, nr, gl in (cands or candidates()):
+    for m, b, gr, nr, gl, gb in (cands or candidates()):
         try:
             s = Sweep(problem.params, problem.grid, *problem.state(), transport_mode=m, replay_block=b,
-                      graph_replay=gr, narrow_runs=nr, group_levels=gl)
+                      graph_replay=gr, narrow_runs=nr, group_levels=gl, group_block=gb)
         except N.SweepError as e:
             if e.code == N.SSW_ERR_OOM:
                 continue
@@ -77,10 +78,11 @@ def autotune(problem: Problem, cands=None, steps: int = 2, warmup: int = 2) -> T
             for _ in range(steps):
                 s.run_sweeps()
             s.synchronize()
-            table.append((m, b, gr, nr, gl, (time.perf_counter() - t0) / steps * 1e3))
+            gb = s.info().group_block
+            table.append((m, b, gr, nr, gl, gb, (time.perf_counter() - t0) / steps * 1e3))
         finally:
             s.close()
     if not table:
         raise RuntimeError("no autotuning candidate fits on the device")
-    m, b, gr, nr, gl, ms = min(table, key=lambda r: r[5])
-    return TuneResult(m, b, gr, nr, gl, ms, table)
+    m, b, gr, nr, gl, gb, ms = min(table, key=lambda r: r[6])
+    return TuneResult(m, b, gr, nr, gl, gb, ms, table)

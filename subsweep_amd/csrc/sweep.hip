@@ -82,10 +82,10 @@ constexpr int kQueueCap = kTasksPerBlock * kMaxFaces;
 constexpr int kPlanes = 12;                // upwind entries per task held in planes
 constexpr int kChunk = 4096;               // compaction chunk (tasks, within one level set)
 constexpr int kRunBlock = 1024;            // workgroup of a run of narrow level sets (tasks per level set)
-constexpr int kGroupT = 128;               // grouped replay build: root tasks (and lanes) per workgroup
-constexpr int kGroupExec = 256;            // grouped replay: lanes per workgroup
-constexpr int kConeCap = 4096;             // grouped replay build: upwind-cone entries per workgroup
-constexpr int kMaxGroup = 4;               // grouped replay: level sets per group
+constexpr int kGroupT = 128;               // grouped replay: root tasks per workgroup (default; 64 or 128)
+constexpr int kConeCap = 3072;             // grouped replay build: cone members / pending entries per workgroup
+constexpr int kSortCap = 4096;             // grouped replay build: entries of one level set sorted in LDS
+constexpr int kMaxGroup = 8;               // grouped replay: level sets per group
 
 thread_local std::string g_err;
 // Handles alive in the process. Replay graphs are captured only while a single
@@ -814,23 +814,33 @@ __device__ void push_upwind(const uint4* hdr, const Entry<R>* planes, const Entr
     }
 }
 
-// One workgroup per block of roots of level set j in the group starting at
+// Workgroup descriptor of the grouped replay, two uint4: {list offset, level
+// sets, counts of level sets 0|1, 2|3} and {counts 4|5, 6|7, 0, 0} (16 bits
+// per count, level sets counted from the group's first).
+__device__ __forceinline__ uint32_t desc_count(const uint4& d0, const uint4& d1, int ph) {
+    const uint32_t w = ph < 2 ? d0.z : ph < 4 ? d0.w : ph < 6 ? d1.x : d1.y;
+    return (ph & 1) ? (w >> 16) : (w & 0xFFFFu);
+}
+
+// One workgroup per block of T roots of level set j in the group starting at
 // level set k0: its cone per level set (sorted, unique), written with the
-// roots to the task list, level set k0 first; desc = {list offset, counts of
-// level sets k0 and k0+1, of k0+2 and k0+3 (16 bits each), level sets}.
-// Members and roots are marked done in todo. Roots are roots[r] or, without
-// a root list, positions root_base + r.
-template <typename R>
-__global__ void __launch_bounds__(kGroupT) k_cone(const uint4* hdr, const Entry<R>* planes, const Entry<R>* ovf,
-                                                  const uint32_t* fstart, int k0, int j, const uint32_t* roots,
-                                                  uint32_t root_base, uint32_t nroots, uint8_t* todo, uint32_t* list,
-                                                  uint32_t list_cap, uint32_t* cursor, uint4* desc, uint32_t* fail) {
-    __shared__ uint32_t cand[kConeCap], tmp[kConeCap], memb[kConeCap];
-    __shared__ uint32_t ncand, ntmp, nmemb, over, base;
-    __shared__ uint32_t cnt[kMaxGroup], psum[kGroupT];
+// roots to the task list, level set k0 first, and its descriptor. Members
+// and roots are marked done in todo. Roots are roots[r] or, without a root
+// list, positions root_base + r. Pending upwind entries are kept per level
+// set: those of level set i are taken out when i is processed.
+template <typename R, int T>
+__global__ void __launch_bounds__(T) k_cone(const uint4* hdr, const Entry<R>* planes, const Entry<R>* ovf,
+                                            const uint32_t* fstart, int k0, int j, const uint32_t* roots,
+                                            uint32_t root_base, uint32_t nroots, uint8_t* todo, uint32_t* list,
+                                            uint32_t list_cap, uint32_t* cursor, uint4* desc, uint32_t* fail) {
+    __shared__ uint32_t candA[kConeCap], candB[kConeCap], tmp[kSortCap], memb[kConeCap];
+    __shared__ uint32_t ncand, nkeep, ntmp, nmemb, over, base;
+    __shared__ uint32_t cnt[kMaxGroup], psum[T];
+    uint32_t* cand = candA;
+    uint32_t* keep = candB;
     const uint32_t tid = threadIdx.x;
-    const uint32_t r0 = blockIdx.x * kGroupT;
-    const uint32_t n = min((uint32_t)kGroupT, nroots - r0);
+    const uint32_t r0 = blockIdx.x * T;
+    const uint32_t n = min((uint32_t)T, nroots - r0);
     if (tid == 0) {
         ncand = 0;
         nmemb = 0;
@@ -848,22 +858,33 @@ __global__ void __launch_bounds__(kGroupT) k_cone(const uint4* hdr, const Entry<
     __syncthreads();
     for (int i = j - 1; i >= k0; i--) {
         const uint32_t bi = fstart[i], ei = fstart[i + 1];
-        if (tid == 0) ntmp = 0;
-        __syncthreads();
-        const uint32_t nc = min(ncand, (uint32_t)kConeCap);
-        for (uint32_t c = tid; c < nc; c += kGroupT) {
-            const uint32_t v = cand[c];
-            if (v >= bi && v < ei) tmp[atomicAdd(&ntmp, 1u)] = v;
+        if (tid == 0) {
+            ntmp = 0;
+            nkeep = 0;
         }
         __syncthreads();
+        // pending entries of level set i to tmp, of lower level sets to keep
+        const uint32_t nc = min(ncand, (uint32_t)kConeCap);
+        for (uint32_t c = tid; c < nc; c += T) {
+            const uint32_t v = cand[c];
+            if (v >= bi) tmp[atomicAdd(&ntmp, 1u)] = v;  // v < ei: entries are below the level sets done
+            else keep[atomicAdd(&nkeep, 1u)] = v;
+        }
+        __syncthreads();
+        {
+            uint32_t* x = cand;
+            cand = keep;
+            keep = x;
+        }
+        if (tid == 0) ncand = nkeep;
         const uint32_t nt = ntmp;
         uint32_t n2 = 1;
         while (n2 < nt) n2 <<= 1;
-        for (uint32_t c = nt + tid; c < n2; c += kGroupT) tmp[c] = kNone;
+        for (uint32_t c = nt + tid; c < n2; c += T) tmp[c] = kNone;
         __syncthreads();
         lds_sort(tmp, n2);
         // unique, in order: each lane compacts a contiguous chunk
-        const uint32_t per = (nt + kGroupT - 1) / kGroupT;
+        const uint32_t per = (nt + T - 1) / T;
         const uint32_t c0 = min(nt, tid * per), c1 = min(nt, c0 + per);
         uint32_t u = 0;
         for (uint32_t c = c0; c < c1; c++) u += (c == 0 || tmp[c] != tmp[c - 1]) ? 1u : 0u;
@@ -871,7 +892,7 @@ __global__ void __launch_bounds__(kGroupT) k_cone(const uint4* hdr, const Entry<
         __syncthreads();
         if (tid == 0) {
             uint32_t s = 0;
-            for (int l = 0; l < kGroupT; l++) {
+            for (int l = 0; l < T; l++) {
                 const uint32_t x = psum[l];
                 psum[l] = s;
                 s += x;
@@ -890,7 +911,7 @@ __global__ void __launch_bounds__(kGroupT) k_cone(const uint4* hdr, const Entry<
         __syncthreads();
         const uint32_t m1 = nmemb;
         if (i > k0)
-            for (uint32_t c = m0 + tid; c < m1; c += kGroupT)
+            for (uint32_t c = m0 + tid; c < m1; c += T)
                 push_upwind<R>(hdr, planes, ovf, memb[c], bi, ei - bi, lo, bi, cand, &ncand, &over);
         __syncthreads();
     }
@@ -914,7 +935,7 @@ __global__ void __launch_bounds__(kGroupT) k_cone(const uint4* hdr, const Entry<
     for (int i = k0; i < j; i++) {
         const uint32_t ci = cnt[i - k0];
         src -= ci;
-        for (uint32_t c = tid; c < ci; c += kGroupT) {
+        for (uint32_t c = tid; c < ci; c += T) {
             const uint32_t v = memb[src + c];
             list[base + dst + c] = v;
             todo[v] = 0;
@@ -926,38 +947,37 @@ __global__ void __launch_bounds__(kGroupT) k_cone(const uint4* hdr, const Entry<
         todo[root] = 0;
     }
     if (tid == 0) {
-        uint32_t c[kMaxGroup] = {0, 0, 0, 0};
+        uint32_t c[kMaxGroup];
+        for (int k = 0; k < kMaxGroup; k++) c[k] = 0;
         for (int i = k0; i < j; i++) c[i - k0] = cnt[i - k0];
         c[j - k0] = n;
-        desc[blockIdx.x] = make_uint4(base, c[0] | (c[1] << 16), c[2] | (c[3] << 16), (uint32_t)(j - k0 + 1));
+        desc[2 * blockIdx.x] = make_uint4(base, (uint32_t)(j - k0 + 1), c[0] | (c[1] << 16), c[2] | (c[3] << 16));
+        desc[2 * blockIdx.x + 1] = make_uint4(c[4] | (c[5] << 16), c[6] | (c[7] << 16), 0u, 0u);
     }
 }
 
-// Grouped replay of the level sets k0 .. k0+phases-1: workgroup w solves the
-// tasks of its descriptor, level set by level set, barrier in between. Each
-// lane's first task of every level set is read from the list up front, and
-// the next level set's schedule data is loaded while the current level set's
-// gathers are in flight; loads are unconditional (lanes without a task read
-// the level set's first position) so that the waits stay exact. Tasks beyond
-// the first kGroupExec of a level set are solved without this overlap.
-template <typename R>
-__global__ void __launch_bounds__(kGroupExec) k_sweep_group(Geo g, Flux<R> fl, const uint4* __restrict__ hdr,
-                                                            const R* __restrict__ Av,
-                                                            const Entry<R>* __restrict__ planes,
-                                                            const Entry<R>* __restrict__ ovf,
-                                                            const uint32_t* __restrict__ fstart, int k0, int phases,
-                                                            const uint4* __restrict__ desc,
-                                                            const uint32_t* __restrict__ list, uint32_t list_len) {
-    const uint4 dsc = desc[blockIdx.x];
+// Grouped replay of the level sets k0 .. k0+phases-1 with E lanes per
+// workgroup: workgroup w solves the tasks of its descriptor, level set by
+// level set, barrier in between. Each lane's first task of every level set is
+// read from the list up front, and the next level set's schedule data is
+// loaded while the current level set's gathers are in flight; loads are
+// unconditional (lanes without a task read the level set's first position)
+// so that the waits stay exact. Tasks beyond the first E of a level set are
+// solved without this overlap.
+template <typename R, int E>
+__global__ void __launch_bounds__(E) k_sweep_group(Geo g, Flux<R> fl, const uint4* __restrict__ hdr,
+                                                   const R* __restrict__ Av, const Entry<R>* __restrict__ planes,
+                                                   const Entry<R>* __restrict__ ovf,
+                                                   const uint32_t* __restrict__ fstart, int k0, int phases,
+                                                   const uint4* __restrict__ desc, const uint32_t* __restrict__ list,
+                                                   uint32_t list_len) {
+    const uint4 d0 = desc[2 * blockIdx.x], d1 = desc[2 * blockIdx.x + 1];
     const uint32_t tid = threadIdx.x;
     uint32_t n[kMaxGroup], off[kMaxGroup], bb[kMaxGroup], WW[kMaxGroup], P[kMaxGroup];
-    n[0] = dsc.y & 0xFFFFu;
-    n[1] = dsc.y >> 16;
-    n[2] = dsc.z & 0xFFFFu;
-    n[3] = dsc.z >> 16;
 #pragma unroll
     for (int ph = 0; ph < kMaxGroup; ph++) {
-        off[ph] = ph == 0 ? dsc.x : off[ph - 1] + n[ph - 1];
+        n[ph] = desc_count(d0, d1, ph);
+        off[ph] = ph == 0 ? d0.x : off[ph - 1] + n[ph - 1];
         const int k = k0 + min(ph, phases - 1);
         bb[ph] = fstart[k];
         WW[ph] = fstart[k + 1] - bb[ph];
@@ -975,7 +995,7 @@ __global__ void __launch_bounds__(kGroupExec) k_sweep_group(Geo g, Flux<R> fl, c
             const int q = ph + 1 < kMaxGroup ? ph + 1 : ph;
             load_task<R>(nxt, hdr + bb[q], Av + bb[q], planes + (size_t)kPlanes * bb[q], WW[q], P[q] - bb[q]);
             if (tid < n[ph]) finish_task_c<R>(g, fl, cur, G, ovf, P[ph]);
-            for (uint32_t i = tid + kGroupExec; i < n[ph]; i += kGroupExec)
+            for (uint32_t i = tid + E; i < n[ph]; i += E)
                 comp_task<R>(g, fl, hdr + bb[ph], Av + bb[ph], planes + (size_t)kPlanes * bb[ph], ovf, bb[ph],
                              WW[ph], list[off[ph] + i] - bb[ph]);
             __syncthreads();
@@ -998,18 +1018,18 @@ __global__ void __launch_bounds__(kBlock) k_group_compact(Geo g, const uint4* __
     const uint32_t w = blockIdx.x;
     if (w >= ndesc) return;
     __shared__ uint32_t wave_tot[kBlock / 64];
-    const uint4 d = desc_in[w];
-    const uint32_t n[kMaxGroup] = {d.y & 0xFFFFu, d.y >> 16, d.z & 0xFFFFu, d.z >> 16};
-    uint32_t m[kMaxGroup] = {0, 0, 0, 0};
+    const uint4 d0 = desc_in[2 * w], d1 = desc_in[2 * w + 1];
+    uint32_t m[kMaxGroup];
     const int lane = __lane_id(), wv = threadIdx.x / 64;
-    uint32_t src = d.x, dst = d.x;
+    uint32_t src = d0.x, dst = d0.x;
     for (int ph = 0; ph < kMaxGroup; ph++) {
+        const uint32_t n = desc_count(d0, d1, ph);
         const uint32_t base = dst;
-        for (uint32_t p0 = 0; p0 < n[ph]; p0 += kBlock) {
+        for (uint32_t p0 = 0; p0 < n; p0 += kBlock) {
             const uint32_t i = p0 + threadIdx.x;
             uint32_t P = 0;
             bool act = false;
-            if (i < n[ph]) {
+            if (i < n) {
                 P = list_in[src + i];
                 act = g.level[hdr[P].x & kCellMask] >= lvl;
             }
@@ -1026,11 +1046,13 @@ __global__ void __launch_bounds__(kBlock) k_group_compact(Geo g, const uint4* __
             __syncthreads();
         }
         m[ph] = dst - base;
-        src += n[ph];
+        src += n;
     }
     if (threadIdx.x == 0) {
-        desc_out[w] = make_uint4(d.x, m[0] | (m[1] << 16), m[2] | (m[3] << 16), d.w);
-        const uint32_t total = m[0] + m[1] + m[2] + m[3];
+        desc_out[2 * w] = make_uint4(d0.x, d0.y, m[0] | (m[1] << 16), m[2] | (m[3] << 16));
+        desc_out[2 * w + 1] = make_uint4(m[4] | (m[5] << 16), m[6] | (m[7] << 16), 0u, 0u);
+        uint32_t total = 0;
+        for (int k = 0; k < kMaxGroup; k++) total += m[k];
         if (total) atomicAdd(&gcount[gmap[w]], total);
     }
 }
@@ -1248,7 +1270,8 @@ struct ssw_sweep {
     int rec_block = 64;                             // compiled replay block size
     bool graphs_on = true;                          // capture replays as graphs
     bool narrow_runs = true;                        // runs of narrow level sets in one workgroup
-    int group_request = 1;                          // level sets per grouped launch of full sweeps (1: off)
+    int group_request = 1;                          // level sets per grouped launch (1: off)
+    int group_block = kGroupT;                      // grouped replay: root tasks per workgroup
     // grouped replay of full sweeps (compiled): per group of level sets its
     // first level set, level-set count and workgroup descriptors
     struct Group {
@@ -1623,7 +1646,8 @@ int build_groups_t(ssw_sweep* s) {
     const std::vector<uint32_t>& fs = s->full_start;
     uint32_t maxw = 1;
     for (int k = 0; k < s->kmax; k++) maxw = std::max(maxw, fs[k + 1] - fs[k]);
-    const size_t desc_cap = DN / kGroupT + (size_t)s->kmax + 1;
+    const int T = s->group_block;
+    const size_t desc_cap = DN / T + (size_t)s->kmax + 1;  // at most ceil(width / T) per level set
     const size_t list_cap = DN + DN / 4 + (1u << 20);
     uint8_t* todo = nullptr;
     uint32_t *roots = nullptr, *ctl = nullptr;  // ctl: cursor, fail, selected count
@@ -1632,7 +1656,7 @@ int build_groups_t(ssw_sweep* s) {
     TRY(dalloc(s, &roots, maxw));
     TRY(dalloc(s, &ctl, 3));
     TRY(dalloc(s, &s->d_glist, list_cap));
-    TRY(dalloc(s, &s->d_gdesc, desc_cap));
+    TRY(dalloc(s, &s->d_gdesc, 2 * desc_cap));
     HIP_TRY(hipMemsetAsync(todo, 1, DN, s->stream));
     HIP_TRY(hipMemsetAsync(ctl, 0, 12, s->stream));
     size_t tb = 0;
@@ -1661,10 +1685,16 @@ int build_groups_t(ssw_sweep* s) {
                 if (n == 0) continue;
                 rl = roots;
             }
-            const uint32_t nb = (n + kGroupT - 1) / kGroupT;
+            const uint32_t nb = (n + T - 1) / T;
             if ((size_t)ndesc + nb > desc_cap) return fail(SSW_ERR_HIP, "grouped replay: descriptor bound exceeded");
-            k_cone<R><<<nb, kGroupT, 0, s->stream>>>(hdr, planes, ovf, s->d_fstart, k0, j, rl, fs[j], n, todo,
-                                                     s->d_glist, (uint32_t)list_cap, ctl, s->d_gdesc + ndesc, ctl + 1);
+            if (T == 64)
+                k_cone<R, 64><<<nb, 64, 0, s->stream>>>(hdr, planes, ovf, s->d_fstart, k0, j, rl, fs[j], n, todo,
+                                                        s->d_glist, (uint32_t)list_cap, ctl, s->d_gdesc + 2 * ndesc,
+                                                        ctl + 1);
+            else
+                k_cone<R, 128><<<nb, 128, 0, s->stream>>>(hdr, planes, ovf, s->d_fstart, k0, j, rl, fs[j], n, todo,
+                                                          s->d_glist, (uint32_t)list_cap, ctl,
+                                                          s->d_gdesc + 2 * ndesc, ctl + 1);
             HIP_TRY(hipGetLastError());
             ndesc += nb;
         }
@@ -1830,7 +1860,7 @@ const uint4* schedule_tasks(ssw_sweep* s) { return s->compiled ? s->d_hdr : s->d
 // schedule whose cell has level >= lvl, per level set, and its level starts
 // on the host. Sweep level 0 covers every task and needs no list.
 int make_group_list(ssw_sweep* s, int lvl) {
-    if (!s->d_gdesc_l[lvl]) TRY(dalloc(s, &s->d_gdesc_l[lvl], std::max<uint32_t>(1u, s->ndesc)));
+    if (!s->d_gdesc_l[lvl]) TRY(dalloc(s, &s->d_gdesc_l[lvl], 2 * (size_t)std::max<uint32_t>(1u, s->ndesc)));
     if (!s->d_glist_l[lvl]) TRY(dalloc(s, &s->d_glist_l[lvl], s->glist_len));
     const size_t ng = std::max<size_t>(1, s->groups.size());
     HIP_TRY(hipMemsetAsync(s->d_gcount, 0, ng * 4, s->stream));
@@ -1905,11 +1935,17 @@ void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<
         hipEvent_t a = ev ? (*mark)() : nullptr;
         if (it.group) {  // a group of level sets, its cones one workgroup each
             const ssw_sweep::Group& gr = s->groups[it.tab];
-            const uint4* dsc = (all ? s->d_gdesc : s->d_gdesc_l[lvl]) + gr.desc0;
+            const uint4* dsc = (all ? s->d_gdesc : s->d_gdesc_l[lvl]) + 2 * (size_t)gr.desc0;
             const uint32_t* gl = all ? s->d_glist : s->d_glist_l[lvl];
-            k_sweep_group<R><<<gr.ndesc, kGroupExec, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A,
-                                                                  (const Entry<R>*)s->d_planes, ovf, s->d_fstart,
-                                                                  gr.k0, gr.levels, dsc, gl, s->glist_len);
+            const Entry<R>* pl = (const Entry<R>*)s->d_planes;
+            if (s->group_block == 64)
+                k_sweep_group<R, 128><<<gr.ndesc, 128, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf,
+                                                                      s->d_fstart, gr.k0, gr.levels, dsc, gl,
+                                                                      s->glist_len);
+            else
+                k_sweep_group<R, 256><<<gr.ndesc, 256, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf,
+                                                                      s->d_fstart, gr.k0, gr.levels, dsc, gl,
+                                                                      s->glist_len);
             if (ev) ev->push_back({a, (*mark)()});
             continue;
         }
@@ -2306,9 +2342,17 @@ int ssw_set_cell_positions(ssw_sweep* s, int64_t n, const double* positions) {
 
 int ssw_set_group_levels(ssw_sweep* s, int32_t levels) {
     if (!s) return fail(SSW_ERR_INVALID, "null handle");
-    if (levels < 1 || levels > kMaxGroup) return fail(SSW_ERR_INVALID, "group levels must be 1 to 4");
+    if (levels < 1 || levels > kMaxGroup) return fail(SSW_ERR_INVALID, "group levels must be 1 to 8");
     if (s->have_grid) return fail(SSW_ERR_INVALID, "ssw_set_group_levels must be called before ssw_upload_grid");
     s->group_request = levels;
+    return SSW_OK;
+}
+
+int ssw_set_group_block(ssw_sweep* s, int32_t roots) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    if (roots != 0 && roots != 64 && roots != 128) return fail(SSW_ERR_INVALID, "group block must be 0, 64 or 128");
+    if (s->have_grid) return fail(SSW_ERR_INVALID, "ssw_set_group_block must be called before ssw_upload_grid");
+    s->group_block = roots ? roots : kGroupT;
     return SSW_OK;
 }
 
@@ -2691,6 +2735,7 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info) {
     info->group_levels = s->groups.empty() ? 1 : s->group_request;
     info->group_fallback = s->group_fallback ? 1 : 0;
     info->group_tasks = s->group_tasks;
+    info->group_block = s->group_block;
     info->locality_order = s->permuted ? 1 : 0;
     if (s->have_state) {
         unsigned long long f = 0, a = 0;

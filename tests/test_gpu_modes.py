@@ -114,11 +114,11 @@ def test_removed_modes_are_rejected(hip_available):
 def test_autotune_picks_a_measured_candidate(hip_available):
     from subsweep_amd.autotune import autotune, candidates
     pb = W.jittered_problem(8, 1.0, "4", 16, levels=1, periodic=True, steps=2, precision=32)
-    r = autotune(pb, cands=candidates((64, 256), groups=(2,)), steps=1, warmup=2)
+    r = autotune(pb, cands=candidates((64, 256), groups=(2,), group_blocks=(64,)), steps=1, warmup=2)
     assert len(r.table) == 6
-    assert (r.transport_mode, r.replay_block, r.graph_replay, r.narrow_runs, r.group_levels, r.ms_per_step) in \
-        [tuple(x) for x in r.table]
-    assert r.ms_per_step == min(x[5] for x in r.table)
+    assert (r.transport_mode, r.replay_block, r.graph_replay, r.narrow_runs, r.group_levels, r.group_block,
+            r.ms_per_step) in [tuple(x) for x in r.table]
+    assert r.ms_per_step == min(x[6] for x in r.table)
 
 
 @pytest.mark.parametrize("mode", [N.TRANSPORT_MODE_GEOMETRIC, N.TRANSPORT_MODE_COMPILED])
@@ -189,16 +189,18 @@ def test_narrow_runs_bitwise(precision, levels, hip_available):
 @pytest.mark.parametrize("precision", [64, 32])
 @pytest.mark.parametrize("levels", [1, 4])
 def test_grouped_full_sweeps_bitwise(precision, levels, hip_available):
-    # groups of 2, 3 and 4 level sets per launch (overlapping upwind cones,
-    # one workgroup each) vs one launch per level set; with 4 timestep levels
-    # only the full sweeps are grouped
+    # groups of 2 to 8 level sets per launch (overlapping upwind cones of 64 or
+    # 128 roots, one workgroup each) vs one launch per level set; with 4
+    # timestep levels the restricted sweeps replay filtered cones
     pb = W.jittered_problem(12, 1.0, "8", 84, levels=levels, periodic=True, steps=5, precision=precision)
     pb = pb.with_params(max_timestep=1e-3 * 3.15576e13)
     ref = None
-    for m in (1, 2, 3, 4):
-        s = Sweep(pb.params, pb.grid, *pb.state(), transport_mode=N.TRANSPORT_MODE_COMPILED, group_levels=m)
+    for m, gb in ((1, 0), (2, 128), (3, 64), (4, 128), (6, 64), (8, 128)):
+        s = Sweep(pb.params, pb.grid, *pb.state(), transport_mode=N.TRANSPORT_MODE_COMPILED, group_levels=m,
+                  group_block=gb)
         info = s.info()
         assert info.group_levels == m and info.group_fallback == 0
+        assert info.group_block == (gb or 128)
         if m > 1:
             assert info.group_tasks >= pb.grid.num_cells * info.num_directions
         seq = []
@@ -210,4 +212,4 @@ def test_grouped_full_sweeps_bitwise(precision, levels, hip_available):
         if ref is None:
             ref = seq
         else:
-            _assert_same(ref, seq, f"groups of {m} vs per level set")
+            _assert_same(ref, seq, f"groups of {m} (block {gb}) vs per level set")
