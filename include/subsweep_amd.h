@@ -214,8 +214,8 @@ int ssw_set_graph_replay(ssw_sweep* s, int32_t enabled);
  * in the reference. */
 int ssw_set_narrow_runs(ssw_sweep* s, int32_t enabled);
 
-/* Compiled replay: groups of `levels` consecutive level sets (2 to 8; 1 = one
- * launch per level set, default) run as one launch each. A workgroup owns a
+/* Compiled replay: groups of `levels` consecutive level sets (2 to 8, default
+ * 4; 1 = one launch per level set) run as one launch each. A workgroup owns a
  * block of root tasks (ssw_set_group_block) of one level set of the group and
  * solves their upwind cone inside the group level set by level set with a
  * workgroup barrier in between; a task in several cones is solved by each of
@@ -227,7 +227,7 @@ int ssw_set_narrow_runs(ssw_sweep* s, int32_t enabled);
  * counterpart in the reference. */
 int ssw_set_group_levels(ssw_sweep* s, int32_t levels);
 
-/* Root tasks per workgroup of the grouped replay: 64 or 128 (0 = 128,
+/* Root tasks per workgroup of the grouped replay: 64 or 128 (0 = 64, the
  * default); the workgroup has twice as many lanes. Set before
  * ssw_upload_grid. No counterpart in the reference. */
 int ssw_set_group_block(ssw_sweep* s, int32_t roots);
@@ -265,7 +265,8 @@ double ssw_chemistry_timestep(double* state, double density, double volume, doub
  * in the given transport mode (GEOMETRIC or COMPILED): an upper estimate of
  * the handle's peak from ssw_upload_grid through ssw_step and ssw_download,
  * used for admission against an HBM budget, for handles with the default
- * launch shape (without grouped full sweeps; the diagnostic downloads of
+ * launch shape (grouped replay of 4 level sets with 64 roots in the compiled
+ * mode; the diagnostic downloads of
  * counts, level sets, fluxes and transport state allocate transient arrays
  * beyond it). No counterpart in the reference. */
 int64_t ssw_estimate_device_bytes(const ssw_sweep_parameters* params, int64_t num_cells, int64_t num_faces,

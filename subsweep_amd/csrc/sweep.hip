@@ -82,7 +82,8 @@ constexpr int kQueueCap = kTasksPerBlock * kMaxFaces;
 constexpr int kPlanes = 12;                // upwind entries per task held in planes
 constexpr int kChunk = 4096;               // compaction chunk (tasks, within one level set)
 constexpr int kRunBlock = 1024;            // workgroup of a run of narrow level sets (tasks per level set)
-constexpr int kGroupT = 128;               // grouped replay: root tasks per workgroup (default; 64 or 128)
+constexpr int kGroupT = 64;                // grouped replay: root tasks per workgroup (default; 64 or 128)
+constexpr int kGroupLevels = 4;            // grouped replay: level sets per group (default)
 constexpr int kConeCap = 3072;             // grouped replay build: cone members / pending entries per workgroup
 constexpr int kSortCap = 4096;             // grouped replay build: entries of one level set sorted in LDS
 constexpr int kMaxGroup = 8;               // grouped replay: level sets per group
@@ -1270,7 +1271,7 @@ struct ssw_sweep {
     int rec_block = 64;                             // compiled replay block size
     bool graphs_on = true;                          // capture replays as graphs
     bool narrow_runs = true;                        // runs of narrow level sets in one workgroup
-    int group_request = 1;                          // level sets per grouped launch (1: off)
+    int group_request = kGroupLevels;               // level sets per grouped launch (1: off)
     int group_block = kGroupT;                      // grouped replay: root tasks per workgroup
     // grouped replay of full sweeps (compiled): per group of level sets its
     // first level set, level-set count and workgroup descriptors
@@ -1572,6 +1573,17 @@ int kahn_layering(ssw_sweep* s, int lvl, int64_t expected, int* depth_out, std::
         k_kahn<<<blocks, kBlock, 0, s->stream>>>(g, k);
         HIP_TRY(hipGetLastError());
     }
+}
+
+// Device bytes of the grouped replay for DN tasks, n cells and L timestep
+// levels with T roots per workgroup: the full graph's task list (bound: DN +
+// DN/4 + 2^20 entries), descriptors and group map, and per restricted sweep
+// level a filtered list and descriptors (replacing the active-set lists).
+int64_t group_bytes(int64_t DN, int64_t n, int64_t L, int64_t T) {
+    const int64_t kmx = std::min<int64_t>(n + 2, 1 << 22);
+    const int64_t dcap = DN / T + kmx + 1;
+    const int64_t list = (DN + DN / 4 + (1 << 20)) * 4;
+    return list + dcap * 36 + kmx * 4 + (L - 1) * (list + dcap * 32);
 }
 
 // The replay plan of sweep level lvl from its level starts (the full
@@ -2491,8 +2503,10 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
     const size_t rs = real_size(s);
     const size_t comp_bytes = DN * (16 + rs + kPlanes * entry_size(s) + 4) + (size_t)s->ovf_total * entry_size(s);
-    const size_t comp_need = comp_bytes + DN * std::max<size_t>(8, 4 * (size_t)(L - 1) + 2 * rs) +
-                             (s->group_request >= 2 ? DN * 7 : 0);  // grouped replay: task list, build scratch
+    const bool grouped = s->group_request >= 2;
+    const size_t lists = grouped ? (size_t)group_bytes((int64_t)DN, n, L, s->group_block) + DN * 5
+                                 : DN * 4 * (size_t)(L - 1);  // grouped: + build scratch
+    const size_t comp_need = comp_bytes + std::max<size_t>(DN * 8, lists + DN * 2 * rs);
     if (s->mode_request == SSW_TRANSPORT_MODE_COMPILED) s->compiled = true;
     else if (s->mode_request == SSW_TRANSPORT_MODE_GEOMETRIC) s->compiled = false;
     else s->compiled = (double)comp_need <= 0.85 * (double)free_b;
@@ -2788,13 +2802,18 @@ int64_t ssw_estimate_device_bytes(const ssw_sweep_parameters* p, int64_t n, int6
     const int64_t grid = (n + 1) * 4 + F * 32 + F * 4 + n * 16 + n * 4 + num_periodic_cells * 8 + F * 4 + n +
                          (n + 4) * 4 + DN / kChunk * 40 + (1 << 20);
     const int64_t sched = comp ? DN * (16 + rs + kPlanes * es + 4) + ovf : DN * 16;  // compiled: + positions
+    // compiled replay with the default grouped replay (4 level sets, 64 roots):
+    // group lists instead of active-set lists, and the cone build's scratch
+    const int64_t grp = comp ? group_bytes(DN, n, L, kGroupT) : 0;
+    const int64_t lists = comp ? grp : (L - 1) * DN * 4;
     // transient peak of the schedule build: counters + layering records (+
     // overflow starts), the four sort arrays, or the schedule being written
     // with the sorted order and overflow starts
     const int64_t ostart = comp ? DN * 4 + ovf : 0;
-    const int64_t build = std::max(DN * 20 + ostart, std::max(DN * 16 + ostart, sched + DN * 4 + ostart));
-    // afterwards: schedule, active-set lists, state, the photon-rate download
-    const int64_t run = sched + (L - 1) * DN * 4 + n * 8 * 9 + 2 * DN * rs + D * num_periodic_cells * rs + n * 8;
+    int64_t build = std::max(DN * 20 + ostart, std::max(DN * 16 + ostart, sched + DN * 4 + ostart));
+    if (comp) build = std::max(build, sched + grp + DN * 5);
+    // afterwards: schedule, active-set (or group) lists, state, the photon-rate download
+    const int64_t run = sched + lists + n * 8 * 9 + 2 * DN * rs + D * num_periodic_cells * rs + n * 8;
     const int64_t b = grid + std::max(build, run);
     return b + b / 20 + (1 << 20);
 }

@@ -84,7 +84,7 @@ class Sweep:
 
     def __init__(self, params: SweepParameters, grid: Grid, density, ionized_fraction, temperature, source,
                  transport_mode: int = N.TRANSPORT_MODE_AUTO, replay_block: int = 0, graph_replay: bool = True,
-                 locality_order: bool = True, narrow_runs: bool = True, group_levels: int = 1,
+                 locality_order: bool = True, narrow_runs: bool = True, group_levels: int = 0,
                  group_block: int = 0):
         lib = N.sweep_lib()
         self._lib = lib
@@ -100,7 +100,8 @@ class Sweep:
             N.check(lib.ssw_set_replay_block(h, int(replay_block)))
             N.check(lib.ssw_set_graph_replay(h, 1 if graph_replay else 0))
             N.check(lib.ssw_set_narrow_runs(h, 1 if narrow_runs else 0))
-            N.check(lib.ssw_set_group_levels(h, int(group_levels)))
+            if group_levels:  # 0: the library's default
+                N.check(lib.ssw_set_group_levels(h, int(group_levels)))
             N.check(lib.ssw_set_group_block(h, int(group_block)))
             if locality_order and grid.points is not None:
                 pts = np.ascontiguousarray(grid.points, dtype=np.float64).reshape(-1, 3)

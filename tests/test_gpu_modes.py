@@ -22,8 +22,8 @@ from subsweep_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-def _run(pb, mode, steps, prebuild=False, block=0):
-    s = Sweep(pb.params, pb.grid, *pb.state(), transport_mode=mode, replay_block=block)
+def _run(pb, mode, steps, prebuild=False, block=0, group_levels=0):
+    s = Sweep(pb.params, pb.grid, *pb.state(), transport_mode=mode, replay_block=block, group_levels=group_levels)
     info = s.info()
     states = []
     for _ in range(steps):
@@ -51,7 +51,10 @@ def test_geometric_and_compiled_bitwise(precision, levels, hip_available):
     pb = W.jittered_problem(12, 1.0, "8", 84, levels=levels, periodic=True, steps=5, precision=precision)
     pb = pb.with_params(max_timestep=1e-3 * 3.15576e13)
     ig, sg = _run(pb, N.TRANSPORT_MODE_GEOMETRIC, pb.steps)
-    ic, sc = _run(pb, N.TRANSPORT_MODE_COMPILED, pb.steps)
+    ic, sc = _run(pb, N.TRANSPORT_MODE_COMPILED, pb.steps)  # grouped replay (default)
+    assert ic.group_levels == 4 and ig.group_levels == 1
+    _assert_same(sg, sc, "geometric vs compiled (grouped)")
+    ic, sc = _run(pb, N.TRANSPORT_MODE_COMPILED, pb.steps, group_levels=1)
     assert ig.transport_mode == N.TRANSPORT_MODE_GEOMETRIC
     assert ic.transport_mode == N.TRANSPORT_MODE_COMPILED
     assert ic.overflow_entries > 0, "the grid should exercise the overflow entries"
@@ -82,10 +85,11 @@ def test_graph_replay_bitwise(hip_available):
 def test_replay_blocks_bitwise(hip_available):
     pb = W.jittered_problem(10, 1.0, "6", 84, levels=3, periodic=True, steps=4, precision=32)
     pb = pb.with_params(max_timestep=1e-3 * 3.15576e13)
-    i64, s64 = _run(pb, N.TRANSPORT_MODE_COMPILED, pb.steps, block=64)
+    # one launch per level set (no groups), where the replay block applies
+    i64, s64 = _run(pb, N.TRANSPORT_MODE_COMPILED, pb.steps, block=64, group_levels=1)
     assert i64.replay_block == 64
     for block in (128, 256):
-        ib, sb = _run(pb, N.TRANSPORT_MODE_COMPILED, pb.steps, block=block)
+        ib, sb = _run(pb, N.TRANSPORT_MODE_COMPILED, pb.steps, block=block, group_levels=1)
         assert ib.replay_block == block
         _assert_same(s64, sb, f"block 64 vs {block}")
 
@@ -174,7 +178,8 @@ def test_narrow_runs_bitwise(precision, levels, hip_available):
     pb = pb.with_params(max_timestep=1e-3 * 3.15576e13)
     out = []
     for narrow in (True, False):
-        s = Sweep(pb.params, pb.grid, *pb.state(), transport_mode=N.TRANSPORT_MODE_COMPILED, narrow_runs=narrow)
+        s = Sweep(pb.params, pb.grid, *pb.state(), transport_mode=N.TRANSPORT_MODE_COMPILED, narrow_runs=narrow,
+                  group_levels=1)
         assert s.info().narrow_runs == (1 if narrow else 0)
         seq = []
         for _ in range(pb.steps):
