@@ -205,7 +205,7 @@ def test_grouped_full_sweeps_bitwise(precision, levels, hip_available):
                   group_block=gb)
         info = s.info()
         assert info.group_levels == m and info.group_fallback == 0
-        assert info.group_block == (gb or 128)
+        assert info.group_block == (gb or 64)
         if m > 1:
             assert info.group_tasks >= pb.grid.num_cells * info.num_directions
         seq = []
