@@ -39,9 +39,9 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 KERNELS = {3: "k_sweep_rec (geometric replay: one launch per Kahn level set, all directions, 16 lanes per task)",
-           4: "k_sweep_comp (compiled replay: one launch per Kahn level set, all directions, one lane per task; "
-              "with narrow runs, k_sweep_comp_run solves each run of level sets of at most 1024 tasks in one "
-              "workgroup, and a launch is either)"}
+           4: "k_sweep_group (compiled replay grouped: one launch per 4 consecutive Kahn level sets, one workgroup per "
+              "block of 64 root tasks solving their upwind cone level set by level set; with group levels 1, "
+              "k_sweep_comp: one launch per level set, one lane per task)"}
 
 
 def log(*a):
