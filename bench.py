@@ -162,6 +162,41 @@ def run_largest(steps: int = 10):
     }
 
 
+def run_shard(gpus: int, steps: int = 4, warmup: int = 2):
+    """One GPU's share of a direction-sharded run over `gpus` GPUs: the
+    2097152-cell clustered grid (4 timestep levels, 1024 sources) with the
+    first ceil(84 / gpus) directions of the 84-direction table as explicit
+    directions, FP32 transport. Chemistry runs on the partial rate sum of
+    these directions (a sharded run would reduce the partial sums over the
+    GPUs once per sweep: N doubles, 16 MB here)."""
+    import math
+    from subsweep_amd import direction_table
+    from subsweep_amd import workloads as W
+    k = math.ceil(84 / gpus)
+    t = time.perf_counter()
+    pb = W.clustered_2m()
+    pb = pb.with_params(directions=direction_table(84)[:k].tolist())
+    log(f"shard grid: {pb.grid.num_cells} cells, built in {time.perf_counter() - t:.1f} s; {k} of 84 directions")
+    r = run_gpu(pb, 32, steps, warmup, profile_steps=1)
+    log(f"shard of {gpus}: {r['updates_per_s'] / 1e9:.3f} G updates/s, {r['ms_per_step']:.1f} ms/step, "
+        f"transport {r['kernel_ms_per_step']:.1f} ms/step")
+    n = pb.grid.num_cells
+    return {
+        "workload": f"one GPU's share of {gpus}: 2097152-cell clustered lognormal-field Voronoi grid (sigma_ln 2), "
+                    f"{k} of 84 directions, 4 timestep levels, 1024 sources, FP32 transport",
+        "gpus": gpus,
+        "directions": k,
+        "steps": steps,
+        "warmup": warmup,
+        "updates_per_s": r["updates_per_s"],
+        "ms_per_step": r["ms_per_step"],
+        "transport_ms_per_step": r["kernel_ms_per_step"],
+        "dag_depth": int(r["info"].max_depth),
+        "sweeps_last_step": int(r["info"].sweeps_last_step),
+        "partial_rate_bytes_per_sweep": n * 8,
+    }
+
+
 def tuned(pb, precision: int, warmup: int, enabled: bool, mode: int, block: int):
     """(launch shape, autotune table) for a run: the autotuner's choice, or the
     command line's."""
@@ -359,6 +394,8 @@ def main():
     ap.add_argument("--mode", type=int, default=0, choices=(0, 3, 4),
                     help="transport replay form: 0 automatic, 3 geometric, 4 compiled")
     ap.add_argument("--block", type=int, default=0, help="compiled replay block size (0: library default)")
+    ap.add_argument("--shard", type=int, default=0, metavar="G",
+                    help="measure one GPU's share of a direction-sharded run over G GPUs and print its JSON only")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use --mode/--block instead of the autotuner's choice")
     args = ap.parse_args()
@@ -379,6 +416,12 @@ def main():
 
     if args.largest:
         print(json.dumps(run_largest()), flush=True)
+        return
+    if args.shard:
+        if not 1 < args.shard <= 84:
+            print("--shard G needs 1 < G <= 84", file=sys.stderr)
+            sys.exit(2)
+        print(json.dumps(run_shard(args.shard)), flush=True)
         return
 
     from subsweep_amd import workloads as W

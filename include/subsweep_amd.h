@@ -71,6 +71,13 @@ typedef struct ssw_sweep_parameters {
 
 typedef struct ssw_sweep ssw_sweep;
 
+/* The direction table used for DirectionsSpecification::Num (HEALPix bins,
+ * used as tabulated, src/sweep/direction/mod.rs:58-75,
+ * src/sweep/direction/healpix.rs:1-231): num_directions x 3 doubles into out.
+ * A caller that shards by direction passes a block of rows as explicit
+ * directions. */
+int ssw_direction_table(int32_t num_directions, double* out);
+
 /* Fills *p with the reference defaults (src/sweep/parameters.rs:64-78):
  * threshold 0, safety factors 0.1, prevent_cooling true, FP64 transport,
  * scale factor 1. num_directions/levels/max_timestep are left for the caller. */

@@ -2360,6 +2360,24 @@ int ssw_set_group_levels(ssw_sweep* s, int32_t levels) {
     return SSW_OK;
 }
 
+int ssw_direction_table(int32_t num_directions, double* out) {
+    if (!out) return fail(SSW_ERR_INVALID, "null argument");
+    const double(*bins)[3] = nullptr;
+    static const double one[1][3] = {{1.0, 0.0, 0.0}};
+    switch (num_directions) {  // Directions::from_num (src/sweep/direction/mod.rs:58-75)
+        case 1: bins = one; break;
+        case 16: bins = SSW_DIRECTION_BINS_16; break;
+        case 21: bins = SSW_DIRECTION_BINS_21; break;
+        case 32: bins = SSW_DIRECTION_BINS_32; break;
+        case 64: bins = SSW_DIRECTION_BINS_64; break;
+        case 84: bins = SSW_DIRECTION_BINS_84; break;
+        default: return fail(SSW_ERR_INVALID, "num_directions must be 1, 16, 21, 32, 64 or 84");
+    }
+    for (int d = 0; d < num_directions; d++)
+        for (int k = 0; k < 3; k++) out[3 * d + k] = bins[d][k];
+    return SSW_OK;
+}
+
 int ssw_set_group_block(ssw_sweep* s, int32_t roots) {
     if (!s) return fail(SSW_ERR_INVALID, "null handle");
     if (roots != 0 && roots != 64 && roots != 128) return fail(SSW_ERR_INVALID, "group block must be 0, 64 or 128");

@@ -69,6 +69,14 @@ class SweepParameters:
         return p, keep
 
 
+def direction_table(num_directions: int) -> np.ndarray:
+    """The [num_directions, 3] direction table of DirectionsSpecification::Num
+    (src/sweep/direction/mod.rs:58-75), as the library uses it."""
+    out = np.zeros((int(num_directions), 3), dtype=np.float64)
+    N.check(N.sweep_lib().ssw_direction_table(int(num_directions), N.ptr(out, C.c_double)))
+    return out
+
+
 @dataclass
 class SweepResults:
     ionized_hydrogen_fraction: np.ndarray

@@ -52,3 +52,20 @@ def test_invalid_parameters_are_rejected():
     assert lib.ssw_create(C.byref(p), C.byref(h)) == N.SSW_ERR_INVALID
     assert b"num_directions" in lib.ssw_last_error()
     assert not h.value
+
+
+def test_direction_table_matches_the_oracle():
+    # host-only call: the tables the library uses for DirectionsSpecification::Num
+    # against the oracle's own copy (a tiny oracle problem per direction count)
+    import numpy as np
+    from subsweep_amd import direction_table
+    from subsweep_amd import workloads as W
+    from oracle.oracle import OracleSweep
+    for num in (1, 16, 21, 32, 64, 84):
+        pb = W.cartesian_problem(2, num, levels=1, periodic=False, source_cells=(0,))
+        o = OracleSweep(pb.params, pb.grid, *pb.state())
+        ref = np.asarray(o.directions()).reshape(-1, 3)
+        o.close()
+        assert np.array_equal(direction_table(num), ref), num
+    with __import__("pytest").raises(N.SweepError):
+        direction_table(5)
