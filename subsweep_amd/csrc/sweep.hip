@@ -1829,6 +1829,17 @@ int64_t group_bytes(int64_t DN, int64_t n, int64_t L, int64_t T) {
     return list + dcap * 36 + kmx * 4 + (L - 1) * (list + dcap * 32);
 }
 
+// Device bytes the staged replay may take for DN tasks, F directed faces and
+// D directions: task records for 1.25 x DN slots and their entries, with the
+// upwind entries counted as half of the (face, direction) pairs, plus the
+// per-descriptor offsets and counts. A handle whose staged data would exceed
+// this keeps the unstaged grouped kernel, so the serving estimate holds.
+int64_t stage_allowance(int64_t DN, int64_t F, int64_t D, int64_t n, int64_t es, int64_t T) {
+    const int64_t kmx = std::min<int64_t>(n + 2, 1 << 22);
+    const int64_t slots = DN + DN / 4;
+    return slots * kRecW * 4 + (F * D / 2 + F * D / 8) * es + (DN / T + kmx + 1) * 8 + (1 << 20);
+}
+
 // The replay plan of sweep level lvl from its level starts (the full
 // schedule's for level 0, the active-set list's otherwise): in the compiled
 // replay, each maximal run of at least two consecutive non-empty level sets of
@@ -2032,7 +2043,10 @@ int build_stage_t(ssw_sweep* s) {
     }
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-    if (words == 0 || words >= 0xFFFFFFF0ull || (double)words * 4.0 > 0.8 * (double)free_b) {
+    const int64_t allow = stage_allowance((int64_t)s->D * s->N, s->F, s->D, s->N, (int64_t)sizeof(Entry<R>),
+                                          s->group_block);
+    if (words == 0 || words >= 0xFFFFFFF0ull || (double)words * 4.0 > 0.8 * (double)free_b ||
+        (int64_t)words * 4 + (int64_t)nd * 8 > allow) {
         for (ssw_sweep::Group& gr : s->groups) gr.lds_bytes = 0;
         dfree(s, s->d_ecount);
         return SSW_OK;
@@ -2874,8 +2888,12 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     const size_t rs = real_size(s);
     const size_t comp_bytes = DN * (16 + rs + kPlanes * entry_size(s) + 4) + (size_t)s->ovf_total * entry_size(s);
     const bool grouped = s->group_request >= 2;
-    const size_t lists = grouped ? (size_t)group_bytes((int64_t)DN, n, L, s->group_block) + DN * 5
-                                 : DN * 4 * (size_t)(L - 1);  // grouped: + build scratch
+    const size_t lists = grouped ? (size_t)group_bytes((int64_t)DN, n, L, s->group_block) + DN * 5 +
+                                       (s->group_stage ? (size_t)stage_allowance((int64_t)DN, F, s->D, n,
+                                                                                 (int64_t)entry_size(s),
+                                                                                 s->group_block)
+                                                       : 0)
+                                 : DN * 4 * (size_t)(L - 1);  // grouped: + build scratch (+ staged regions)
     const size_t comp_need = comp_bytes + std::max<size_t>(DN * 8, lists + DN * 2 * rs);
     if (s->mode_request == SSW_TRANSPORT_MODE_COMPILED) s->compiled = true;
     else if (s->mode_request == SSW_TRANSPORT_MODE_GEOMETRIC) s->compiled = false;
@@ -3175,7 +3193,7 @@ int64_t ssw_estimate_device_bytes(const ssw_sweep_parameters* p, int64_t n, int6
     const int64_t sched = comp ? DN * (16 + rs + kPlanes * es + 4) + ovf : DN * 16;  // compiled: + positions
     // compiled replay with the default grouped replay (4 level sets, 64 roots):
     // group lists instead of active-set lists, and the cone build's scratch
-    const int64_t grp = comp ? group_bytes(DN, n, L, kGroupT) : 0;
+    const int64_t grp = comp ? group_bytes(DN, n, L, kGroupT) + stage_allowance(DN, F, D, n, es, kGroupT) : 0;
     const int64_t lists = comp ? grp : (L - 1) * DN * 4;
     // transient peak of the schedule build: counters + layering records (+
     // overflow starts), the four sort arrays, or the schedule being written
