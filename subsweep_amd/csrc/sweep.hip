@@ -1234,9 +1234,23 @@ __global__ void __launch_bounds__(E_LANES) k_sweep_staged(Geo g, Flux<R> fl, con
     constexpr uint32_t ew = entry_words<R>();
     const uint32_t words = region_words(S, E, ew);
     const uint32_t* src = region + roff[w];
-    // 1. the region into LDS (16 B per lane and step; regions start 16 B aligned)
-    for (uint32_t i = tid; i < words / 4; i += E_LANES)
-        reinterpret_cast<uint4*>(lds)[i] = reinterpret_cast<const uint4*>(src)[i];
+    // 1. the region into LDS, 16 B per lane and element; kBatch loads in flight
+    // per lane before their LDS stores (regions start 16 B aligned)
+    constexpr uint32_t kBatch = 16;
+    const uint32_t n4 = words / 4;
+    for (uint32_t i0 = 0; i0 < n4; i0 += kBatch * E_LANES) {
+        uint4 buf[kBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; j++) {
+            const uint32_t i = i0 + j * E_LANES + tid;
+            buf[j] = reinterpret_cast<const uint4*>(src)[min(i, n4 - 1u)];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; j++) {
+            const uint32_t i = i0 + j * E_LANES + tid;
+            if (i < n4) reinterpret_cast<uint4*>(lds)[i] = buf[j];
+        }
+    }
     __syncthreads();
     const uint32_t* rec = lds;
     const Entry<R>* ent = reinterpret_cast<const Entry<R>*>(lds + (size_t)S * kRecW);
@@ -1255,10 +1269,19 @@ __global__ void __launch_bounds__(E_LANES) k_sweep_staged(Geo g, Flux<R> fl, con
         const R p = fl.P ? fl.P[(size_t)d * g.Nper + (ps != kNone ? ps : 0u)] : (R)0;
         per[s] = (ps != kNone) ? p : (R)0;
     }
-#pragma unroll 8
-    for (uint32_t e = tid; e < E; e += E_LANES) {
-        const uint32_t code = ent[e].nb;
-        xin[e] = fl.X[(code & kLocalFlag) ? 0u : code];
+    for (uint32_t e0 = 0; e0 < E; e0 += kBatch * E_LANES) {
+        R buf[kBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; j++) {
+            const uint32_t e = min(e0 + j * E_LANES + tid, E - 1u);
+            const uint32_t code = ent[e].nb;
+            buf[j] = fl.X[(code & kLocalFlag) ? 0u : code];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; j++) {
+            const uint32_t e = e0 + j * E_LANES + tid;
+            if (e < E) xin[e] = buf[j];
+        }
     }
     __syncthreads();
     // 3. the level sets, X passed on through LDS
@@ -1269,11 +1292,24 @@ __global__ void __launch_bounds__(E_LANES) k_sweep_staged(Geo g, Flux<R> fl, con
             const uint32_t* r = rec + (size_t)s * kRecW;
             const uint32_t cd = r[0], P = r[2], eoff = r[3], nup = r[4];
             const R A = *reinterpret_cast<const R*>(r + 6);
+            // the first kPlanes entries and their X read as two batches of
+            // independent LDS reads (entries past the count read entry eoff
+            // and are not summed), then summed in order; further entries after
+            Entry<R> e[kPlanes];
+#pragma unroll
+            for (int q = 0; q < kPlanes; q++) e[q] = ent[eoff + ((uint32_t)q < nup ? (uint32_t)q : 0u)];
+            R x[kPlanes];
+#pragma unroll
+            for (int q = 0; q < kPlanes; q++) {
+                const uint32_t qq = (uint32_t)q < nup ? (uint32_t)q : 0u;
+                x[q] = (e[q].nb & kLocalFlag) ? xl[e[q].nb & ~kLocalFlag] : xin[eoff + qq];
+            }
             R inc = 0;
-            for (uint32_t q = 0; q < nup; q++) {
-                const Entry<R> e = ent[eoff + q];
-                const R x = (e.nb & kLocalFlag) ? xl[e.nb & ~kLocalFlag] : xin[eoff + q];
-                inc += e.w * x;
+#pragma unroll
+            for (int q = 0; q < kPlanes; q++) inc += ((uint32_t)q < nup) ? e[q].w * x[q] : (R)0;
+            for (uint32_t q = kPlanes; q < nup; q++) {
+                const Entry<R> o = ent[eoff + q];
+                inc += o.w * ((o.nb & kLocalFlag) ? xl[o.nb & ~kLocalFlag] : xin[eoff + q]);
             }
             const uint32_t c = cd & kCellMask, d = cd >> kCellBits;
             CellIn<R> ci;
@@ -1528,7 +1564,7 @@ struct ssw_sweep {
     uint32_t* d_gcount = nullptr;
     std::vector<std::vector<uint32_t>> gcount;
     int64_t group_tasks = 0;                        // tasks solved per full sweep by the grouped replay
-    bool group_stage = true;                        // staged (LDS) replay of grouped full sweeps
+    bool group_stage = false;                       // staged (LDS) replay of grouped full sweeps
     uint32_t* d_roff = nullptr;                     // staged: region offset (words) of each descriptor
     uint32_t* d_ecount = nullptr;                   // staged: entries of each descriptor
     uint32_t* d_region = nullptr;                   // staged: task records and entries
@@ -3193,7 +3229,7 @@ int64_t ssw_estimate_device_bytes(const ssw_sweep_parameters* p, int64_t n, int6
     const int64_t sched = comp ? DN * (16 + rs + kPlanes * es + 4) + ovf : DN * 16;  // compiled: + positions
     // compiled replay with the default grouped replay (4 level sets, 64 roots):
     // group lists instead of active-set lists, and the cone build's scratch
-    const int64_t grp = comp ? group_bytes(DN, n, L, kGroupT) + stage_allowance(DN, F, D, n, es, kGroupT) : 0;
+    const int64_t grp = comp ? group_bytes(DN, n, L, kGroupT) : 0;
     const int64_t lists = comp ? grp : (L - 1) * DN * 4;
     // transient peak of the schedule build: counters + layering records (+
     // overflow starts), the four sort arrays, or the schedule being written

@@ -93,7 +93,7 @@ class Sweep:
     def __init__(self, params: SweepParameters, grid: Grid, density, ionized_fraction, temperature, source,
                  transport_mode: int = N.TRANSPORT_MODE_AUTO, replay_block: int = 0, graph_replay: bool = True,
                  locality_order: bool = True, narrow_runs: bool = True, group_levels: int = 0,
-                 group_block: int = 0, group_staging: bool = True):
+                 group_block: int = 0, group_staging: bool = False):
         lib = N.sweep_lib()
         self._lib = lib
         self.params = params

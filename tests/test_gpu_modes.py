@@ -228,10 +228,11 @@ def test_staged_groups_bitwise(precision, levels, hip_available):
     pb = W.jittered_problem(12, 1.0, "8", 84, levels=levels, periodic=True, steps=5, precision=precision)
     pb = pb.with_params(max_timestep=1e-3 * 3.15576e13)
     runs = []
-    for kw in ({"group_levels": 1}, {"group_staging": False}, {}, {"group_levels": 8, "group_block": 128}):
+    for kw in ({"group_levels": 1}, {}, {"group_staging": True},
+               {"group_staging": True, "group_levels": 8, "group_block": 128}):
         s = Sweep(pb.params, pb.grid, *pb.state(), transport_mode=N.TRANSPORT_MODE_COMPILED, **kw)
         info = s.info()
-        staged = kw.get("group_staging", True) and kw.get("group_levels", 4) > 1
+        staged = kw.get("group_staging", False) and kw.get("group_levels", 4) > 1
         assert (info.staged_groups > 0) == staged, (kw, info.staged_groups)
         seq = []
         for _ in range(pb.steps):
