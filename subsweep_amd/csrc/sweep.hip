@@ -88,7 +88,6 @@ constexpr int kConeCap = 3072;             // grouped replay build: cone members
 constexpr int kSortCap = 4096;             // grouped replay build: entries of one level set sorted in LDS
 constexpr int kMaxGroup = 8;               // grouped replay: level sets per group
 constexpr uint32_t kLocalFlag = 0x80000000u;  // staged grouped replay: entry names a slot of the workgroup
-constexpr int kRecW = 8;                   // staged grouped replay: words per task record
 constexpr int kSlotCap = kConeCap + 128;   // staged grouped replay build: tasks per workgroup
 constexpr int kLdsMax = 65536;             // staged grouped replay: LDS bytes per workgroup
 
@@ -1064,34 +1063,55 @@ __global__ void __launch_bounds__(kBlock) k_group_compact(Geo g, const uint4* __
 
 // ---------------------------------------------------------------------------
 // staged grouped replay (full sweeps): each workgroup's tasks are stored as one
-// contiguous region, task records (kRecW words: cell | direction << 25,
-// periodic slot, schedule position, first entry, upwind count, A) then the
-// upwind entries of all its tasks in face order; an entry names either a task
-// of the same workgroup (kLocalFlag | slot) or a schedule position outside the
-// group. The kernel copies the region into LDS, gathers the cell inputs and
-// the X of every outside entry at once, then solves the level sets from LDS,
-// passing X between them through LDS with LDS-only barriers; its global
-// stores are waited for only at the kernel's end.
+// contiguous region of arrays over its S task slots (cell | direction << 25,
+// periodic slot, schedule position, upwind count, overflow offset, A), then
+// the upwind entries in kPlanes planes (entry q of slot s at q * S + s, unused
+// ones zero) and the overflow entries of the slots with more. An entry names
+// either a task of the same workgroup (kLocalFlag | slot) or a schedule
+// position outside the group. The kernel copies the region into LDS, gathers
+// the cell inputs and the X of every outside entry at once, then solves the
+// level sets from LDS, passing X between them through LDS with LDS-only
+// barriers; its global stores are waited for only at the kernel's end. Arrays
+// over slots keep consecutive lanes on consecutive LDS words.
 
 template <typename R>
 constexpr uint32_t entry_words() {
     return (uint32_t)(sizeof(Entry<R>) / 4);
 }
 
-__host__ __device__ __forceinline__ uint32_t region_words(uint32_t S, uint32_t E, uint32_t ew) {
-    return S * kRecW + ((E * ew + 3u) & ~3u);
+// Word offsets of a region's arrays (each 16 B aligned).
+struct StageLayout {
+    uint32_t cd, ps, P, nup, ovo, A, pl, ov, words;
+};
+
+template <typename R>
+__host__ __device__ __forceinline__ StageLayout stage_layout(uint32_t S, uint32_t Eo) {
+    const uint32_t rw = (uint32_t)(sizeof(R) / 4), ew = entry_words<R>();
+    auto a4 = [](uint32_t x) { return (x + 3u) & ~3u; };
+    StageLayout L;
+    L.cd = 0;
+    L.ps = L.cd + a4(S);
+    L.P = L.ps + a4(S);
+    L.nup = L.P + a4(S);
+    L.ovo = L.nup + a4(S);
+    L.A = L.ovo + a4(S);
+    L.pl = L.A + a4(S * rw);
+    L.ov = L.pl + a4((uint32_t)kPlanes * S * ew);
+    L.words = L.ov + a4(Eo * ew);
+    return L;
 }
 
-// LDS bytes of a workgroup with S tasks and E entries: its region, the
-// gathered X per entry, X per task, cell inputs per task (16 B + periodic).
+// LDS bytes of a workgroup with S tasks and Eo overflow entries: its region,
+// the gathered X per plane entry and overflow entry, X per task, cell inputs.
 template <typename R>
-__host__ __device__ __forceinline__ uint32_t staged_lds_bytes(uint32_t S, uint32_t E) {
+__host__ __device__ __forceinline__ uint32_t staged_lds_bytes(uint32_t S, uint32_t Eo) {
     const uint32_t rs = (uint32_t)sizeof(R);
     auto al = [](uint32_t x) { return (x + 15u) & ~15u; };
-    return region_words(S, E, entry_words<R>()) * 4u + al(E * rs) + al(S * rs) + S * 16u + al(S * rs);
+    return stage_layout<R>(S, Eo).words * 4u + al((uint32_t)kPlanes * S * rs) + al(Eo * rs) + al(S * rs) +
+           S * 16u + al(S * rs);
 }
 
-// Per workgroup descriptor: its tasks S (from the counts) and entries E.
+// Per workgroup descriptor: its overflow entries (upwind entries past kPlanes).
 __global__ void __launch_bounds__(kBlock) k_stage_size(const uint4* __restrict__ hdr, const uint4* __restrict__ desc,
                                                        const uint32_t* __restrict__ list, uint32_t ndesc,
                                                        uint32_t* E_out) {
@@ -1104,16 +1124,19 @@ __global__ void __launch_bounds__(kBlock) k_stage_size(const uint4* __restrict__
     if (threadIdx.x == 0) tot = 0;
     __syncthreads();
     uint32_t e = 0;
-    for (uint32_t s = threadIdx.x; s < S; s += kBlock) e += hdr[list[d0.x + s]].w;
+    for (uint32_t s = threadIdx.x; s < S; s += kBlock) {
+        const uint32_t nup = hdr[list[d0.x + s]].w;
+        e += nup > (uint32_t)kPlanes ? nup - kPlanes : 0u;
+    }
     for (int o = 32; o > 0; o >>= 1) e += __shfl_down(e, o);
     if (__lane_id() == 0 && e) atomicAdd(&tot, e);
     __syncthreads();
     if (threadIdx.x == 0) E_out[w] = tot;
 }
 
-// Writes workgroup w's region at roff[w] (words): task records and entries,
-// an entry inside the group resolved to its slot by binary search in the
-// workgroup's sorted task list of that level set (the cone contains it).
+// Writes workgroup w's region at roff[w] (words), an entry inside the group
+// resolved to its slot by binary search in the workgroup's sorted task list
+// of that level set (the cone contains it).
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_stage_write(const uint4* __restrict__ hdr, const R* __restrict__ Av,
                                                         const Entry<R>* __restrict__ planes,
@@ -1122,7 +1145,8 @@ __global__ void __launch_bounds__(kBlock) k_stage_write(const uint4* __restrict_
                                                         const int* __restrict__ gk0, const uint32_t* __restrict__ gmap,
                                                         const uint4* __restrict__ desc,
                                                         const uint32_t* __restrict__ list, uint32_t ndesc,
-                                                        const uint32_t* __restrict__ roff, uint32_t* region,
+                                                        const uint32_t* __restrict__ roff,
+                                                        const uint32_t* __restrict__ ecount, uint32_t* region,
                                                         uint32_t* fail) {
     const uint32_t w = blockIdx.x;
     if (w >= ndesc || roff[w] == kNone) return;  // kNone: the descriptor's group is not staged
@@ -1142,10 +1166,11 @@ __global__ void __launch_bounds__(kBlock) k_stage_write(const uint4* __restrict_
     }
     for (uint32_t s = threadIdx.x; s < S; s += kBlock) {
         pos[s] = list[d0.x + s];
-        eo[s] = hdr[pos[s]].w;
+        const uint32_t nup = hdr[pos[s]].w;
+        eo[s] = nup > (uint32_t)kPlanes ? nup - kPlanes : 0u;
     }
     __syncthreads();
-    // exclusive prefix of the upwind counts: each lane a contiguous chunk
+    // exclusive prefix of the overflow counts: each lane a contiguous chunk
     const uint32_t per = (S + kBlock - 1) / kBlock;
     const uint32_t c0 = min(S, threadIdx.x * per), c1 = min(S, c0 + per);
     uint32_t u = 0;
@@ -1168,40 +1193,46 @@ __global__ void __launch_bounds__(kBlock) k_stage_write(const uint4* __restrict_
         acc += x;
     }
     __syncthreads();
-    const uint32_t base = roff[w];
+    uint32_t* rg = region + roff[w];
+    const StageLayout L = stage_layout<R>(S, ecount[w]);
     const uint32_t lo = fstart[k0];
-    constexpr uint32_t ew = entry_words<R>();
+    Entry<R>* pl = reinterpret_cast<Entry<R>*>(rg + L.pl);
+    Entry<R>* ov = reinterpret_cast<Entry<R>*>(rg + L.ov);
     for (uint32_t s = threadIdx.x; s < S; s += kBlock) {
         int ph = 0;
         while (ph + 1 < phases && s >= seg[ph + 1]) ph++;
         const uint32_t P = pos[s];
         const uint32_t b = fstart[k0 + ph], W = fstart[k0 + ph + 1] - b;
         const uint4 h = hdr[P];
-        uint32_t* rec = region + base + (size_t)s * kRecW;
-        rec[0] = h.x;
-        rec[1] = h.y;
-        rec[2] = P;
-        rec[3] = eo[s];
-        rec[4] = h.w;
-        rec[5] = 0;
-        *reinterpret_cast<R*>(rec + 6) = Av[P];
-        if (sizeof(R) == 4) rec[7] = 0;
-        for (uint32_t q = 0; q < h.w; q++) {
-            Entry<R> e = q < (uint32_t)kPlanes ? planes[(size_t)kPlanes * b + (size_t)q * W + (P - b)]
-                                               : ovf[h.z + q - kPlanes];
-            if (e.nb >= lo) {  // inside the group: an earlier level set of this workgroup
-                int pj = 0;
-                while (pj + 1 < ph && e.nb >= fstart[k0 + pj + 1]) pj++;
-                uint32_t a = seg[pj], z = seg[pj + 1];
-                while (a < z) {
-                    const uint32_t m = (a + z) >> 1;
-                    if (pos[m] < e.nb) a = m + 1;
-                    else z = m;
+        rg[L.cd + s] = h.x;
+        rg[L.ps + s] = h.y;
+        rg[L.P + s] = P;
+        rg[L.nup + s] = h.w;
+        rg[L.ovo + s] = eo[s];
+        reinterpret_cast<R*>(rg + L.A)[s] = Av[P];
+        for (uint32_t q = 0; q < max(h.w, (uint32_t)kPlanes); q++) {
+            Entry<R> e;
+            e.nb = 0;
+            if constexpr (sizeof(R) == 8) e.pad = 0;
+            e.w = (R)0;
+            if (q < h.w) {
+                e = q < (uint32_t)kPlanes ? planes[(size_t)kPlanes * b + (size_t)q * W + (P - b)]
+                                          : ovf[h.z + q - kPlanes];
+                if (e.nb >= lo) {  // inside the group: an earlier level set of this workgroup
+                    int pj = 0;
+                    while (pj + 1 < ph && e.nb >= fstart[k0 + pj + 1]) pj++;
+                    uint32_t a = seg[pj], z = seg[pj + 1];
+                    while (a < z) {
+                        const uint32_t m = (a + z) >> 1;
+                        if (pos[m] < e.nb) a = m + 1;
+                        else z = m;
+                    }
+                    if (a < seg[pj + 1] && pos[a] == e.nb) e.nb = kLocalFlag | a;
+                    else atomicOr(fail, 8u);
                 }
-                if (a < seg[pj + 1] && pos[a] == e.nb) e.nb = kLocalFlag | a;
-                else atomicOr(fail, 8u);
             }
-            *reinterpret_cast<Entry<R>*>(region + base + (size_t)S * kRecW + (size_t)(eo[s] + q) * ew) = e;
+            if (q < (uint32_t)kPlanes) pl[(size_t)q * S + s] = e;
+            else ov[eo[s] + q - kPlanes] = e;
         }
     }
 }
@@ -1215,10 +1246,11 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 // Staged grouped replay of one group (full sweep): workgroup w copies its
-// region (roff[w] words, S tasks, E entries) into LDS, gathers the cell inputs
-// of its tasks and the X of its outside entries, then solves its level sets
-// one after the other from LDS. The upwind sum runs over the entries in face
-// order like every other replay, so results are bitwise the same.
+// region (roff[w] words; S tasks, ecount[w] overflow entries) into LDS,
+// gathers the cell inputs of its tasks and the X of its outside entries, then
+// solves its level sets one after the other from LDS. The upwind sum runs over
+// the entries in face order like every other replay, so results are bitwise
+// the same.
 template <typename R, int E_LANES>
 __global__ void __launch_bounds__(E_LANES) k_sweep_staged(Geo g, Flux<R> fl, const uint4* __restrict__ desc,
                                                           const uint32_t* __restrict__ roff,
@@ -1230,14 +1262,13 @@ __global__ void __launch_bounds__(E_LANES) k_sweep_staged(Geo g, Flux<R> fl, con
     const int phases = (int)d0.y;
     uint32_t S = 0;
     for (int ph = 0; ph < kMaxGroup; ph++) S += desc_count(d0, d1, ph);
-    const uint32_t E = ecount[w];
-    constexpr uint32_t ew = entry_words<R>();
-    const uint32_t words = region_words(S, E, ew);
+    const uint32_t Eo = ecount[w];
+    const StageLayout L = stage_layout<R>(S, Eo);
     const uint32_t* src = region + roff[w];
     // 1. the region into LDS, 16 B per lane and element; kBatch loads in flight
     // per lane before their LDS stores (regions start 16 B aligned)
     constexpr uint32_t kBatch = 16;
-    const uint32_t n4 = words / 4;
+    const uint32_t n4 = L.words / 4;
     for (uint32_t i0 = 0; i0 < n4; i0 += kBatch * E_LANES) {
         uint4 buf[kBatch];
 #pragma unroll
@@ -1252,36 +1283,50 @@ __global__ void __launch_bounds__(E_LANES) k_sweep_staged(Geo g, Flux<R> fl, con
         }
     }
     __syncthreads();
-    const uint32_t* rec = lds;
-    const Entry<R>* ent = reinterpret_cast<const Entry<R>*>(lds + (size_t)S * kRecW);
+    const uint32_t* cdA = lds + L.cd;
+    const uint32_t* psA = lds + L.ps;
+    const uint32_t* PA = lds + L.P;
+    const uint32_t* nupA = lds + L.nup;
+    const uint32_t* ovoA = lds + L.ovo;
+    const R* AA = reinterpret_cast<const R*>(lds + L.A);
+    const Entry<R>* pl = reinterpret_cast<const Entry<R>*>(lds + L.pl);
+    const Entry<R>* ov = reinterpret_cast<const Entry<R>*>(lds + L.ov);
     auto al = [](uint32_t x) { return (x + 15u) & ~15u; };
-    uint8_t* free_lds = reinterpret_cast<uint8_t*>(lds) + (size_t)words * 4;
-    R* xin = reinterpret_cast<R*>(free_lds);
-    R* xl = reinterpret_cast<R*>(free_lds + al(E * sizeof(R)));
-    double2* cin = reinterpret_cast<double2*>(free_lds + al(E * sizeof(R)) + al(S * sizeof(R)));
-    R* per = reinterpret_cast<R*>(reinterpret_cast<uint8_t*>(cin) + (size_t)S * 16);
+    uint8_t* fr = reinterpret_cast<uint8_t*>(lds) + (size_t)L.words * 4;
+    R* xin = reinterpret_cast<R*>(fr);
+    fr += al((uint32_t)kPlanes * S * sizeof(R));
+    R* xov = reinterpret_cast<R*>(fr);
+    fr += al(Eo * sizeof(R));
+    R* xl = reinterpret_cast<R*>(fr);
+    fr += al(S * sizeof(R));
+    double2* cin = reinterpret_cast<double2*>(fr);
+    R* per = reinterpret_cast<R*>(fr + (size_t)S * 16);
     // 2. cell inputs and the X of entries outside the group (no dependence
-    // on this launch's results)
+    // on this launch's results); plane entries past a task's count read X[0]
     for (uint32_t s = tid; s < S; s += E_LANES) {
-        const uint32_t cd = rec[s * kRecW], ps = rec[s * kRecW + 1];
+        const uint32_t cd = cdA[s], ps = psA[s];
         const uint32_t c = cd & kCellMask, d = cd >> kCellBits;
         cin[s] = fl.sa[c];
         const R p = fl.P ? fl.P[(size_t)d * g.Nper + (ps != kNone ? ps : 0u)] : (R)0;
         per[s] = (ps != kNone) ? p : (R)0;
     }
-    for (uint32_t e0 = 0; e0 < E; e0 += kBatch * E_LANES) {
+    const uint32_t np = (uint32_t)kPlanes * S;
+    for (uint32_t e0 = 0; e0 < np; e0 += kBatch * E_LANES) {
         R buf[kBatch];
 #pragma unroll
         for (uint32_t j = 0; j < kBatch; j++) {
-            const uint32_t e = min(e0 + j * E_LANES + tid, E - 1u);
-            const uint32_t code = ent[e].nb;
+            const uint32_t code = pl[min(e0 + j * E_LANES + tid, np - 1u)].nb;
             buf[j] = fl.X[(code & kLocalFlag) ? 0u : code];
         }
 #pragma unroll
         for (uint32_t j = 0; j < kBatch; j++) {
             const uint32_t e = e0 + j * E_LANES + tid;
-            if (e < E) xin[e] = buf[j];
+            if (e < np) xin[e] = buf[j];
         }
+    }
+    for (uint32_t e = tid; e < Eo; e += E_LANES) {
+        const uint32_t code = ov[e].nb;
+        xov[e] = fl.X[(code & kLocalFlag) ? 0u : code];
     }
     __syncthreads();
     // 3. the level sets, X passed on through LDS
@@ -1289,29 +1334,25 @@ __global__ void __launch_bounds__(E_LANES) k_sweep_staged(Geo g, Flux<R> fl, con
     for (int ph = 0; ph < phases; ph++) {
         const uint32_t n = desc_count(d0, d1, ph);
         for (uint32_t s = s0 + tid; s < s0 + n; s += E_LANES) {
-            const uint32_t* r = rec + (size_t)s * kRecW;
-            const uint32_t cd = r[0], P = r[2], eoff = r[3], nup = r[4];
-            const R A = *reinterpret_cast<const R*>(r + 6);
-            // the first kPlanes entries and their X read as two batches of
-            // independent LDS reads (entries past the count read entry eoff
-            // and are not summed), then summed in order; further entries after
+            const uint32_t nup = nupA[s];
             Entry<R> e[kPlanes];
 #pragma unroll
-            for (int q = 0; q < kPlanes; q++) e[q] = ent[eoff + ((uint32_t)q < nup ? (uint32_t)q : 0u)];
+            for (int q = 0; q < kPlanes; q++) e[q] = pl[(uint32_t)q * S + s];
             R x[kPlanes];
 #pragma unroll
-            for (int q = 0; q < kPlanes; q++) {
-                const uint32_t qq = (uint32_t)q < nup ? (uint32_t)q : 0u;
-                x[q] = (e[q].nb & kLocalFlag) ? xl[e[q].nb & ~kLocalFlag] : xin[eoff + qq];
-            }
+            for (int q = 0; q < kPlanes; q++)
+                x[q] = (e[q].nb & kLocalFlag) ? xl[e[q].nb & ~kLocalFlag] : xin[(uint32_t)q * S + s];
             R inc = 0;
 #pragma unroll
             for (int q = 0; q < kPlanes; q++) inc += ((uint32_t)q < nup) ? e[q].w * x[q] : (R)0;
             for (uint32_t q = kPlanes; q < nup; q++) {
-                const Entry<R> o = ent[eoff + q];
-                inc += o.w * ((o.nb & kLocalFlag) ? xl[o.nb & ~kLocalFlag] : xin[eoff + q]);
+                const uint32_t o = ovoA[s] + q - kPlanes;
+                const Entry<R> eo = ov[o];
+                inc += eo.w * ((eo.nb & kLocalFlag) ? xl[eo.nb & ~kLocalFlag] : xov[o]);
             }
+            const uint32_t cd = cdA[s];
             const uint32_t c = cd & kCellMask, d = cd >> kCellBits;
+            const R A = AA[s];
             CellIn<R> ci;
             const double2 sa = cin[s];
             ci.src = sa.x;
@@ -1322,7 +1363,7 @@ __global__ void __launch_bounds__(E_LANES) k_sweep_staged(Geo g, Flux<R> fl, con
             const R out = (in < (R)fl.thr) ? (R)0 : in * (R)ci.att;
             const R xv = (A > (R)0) ? out / A : (R)0;
             xl[s] = xv;
-            fl.X[P] = xv;
+            fl.X[PA[s]] = xv;
             fl.I[(size_t)d * g.N + c] = inc;
         }
         s0 += n;
@@ -1865,15 +1906,14 @@ int64_t group_bytes(int64_t DN, int64_t n, int64_t L, int64_t T) {
     return list + dcap * 36 + kmx * 4 + (L - 1) * (list + dcap * 32);
 }
 
-// Device bytes the staged replay may take for DN tasks, F directed faces and
-// D directions: task records for 1.25 x DN slots and their entries, with the
-// upwind entries counted as half of the (face, direction) pairs, plus the
+// Device bytes the staged replay may take for DN tasks: arrays and entry
+// planes for 1.25 x DN slots, overflow entries for at most DN, plus the
 // per-descriptor offsets and counts. A handle whose staged data would exceed
 // this keeps the unstaged grouped kernel, so the serving estimate holds.
 int64_t stage_allowance(int64_t DN, int64_t F, int64_t D, int64_t n, int64_t es, int64_t T) {
     const int64_t kmx = std::min<int64_t>(n + 2, 1 << 22);
     const int64_t slots = DN + DN / 4;
-    return slots * kRecW * 4 + (F * D / 2 + F * D / 8) * es + (DN / T + kmx + 1) * 8 + (1 << 20);
+    return slots * (24 + es / 2 + kPlanes * es) + DN * es + (DN / T + kmx + 1) * 8 + (1 << 20);
 }
 
 // The replay plan of sweep level lvl from its level starts (the full
@@ -2050,31 +2090,28 @@ int build_stage_t(ssw_sweep* s) {
     HIP_TRY(hipMemcpyAsync(desc.data(), s->d_gdesc, desc.size() * sizeof(uint4), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     (void)d_se;
-    constexpr uint32_t ew = entry_words<R>();
     std::vector<uint32_t> roff(nd, kNone);
     uint64_t words = 0;
+    auto slots_of = [&](uint32_t w) {
+        const uint4 d0 = desc[2 * w], d1 = desc[2 * w + 1];
+        const uint32_t cnt[kMaxGroup] = {d0.z & 0xFFFFu, d0.z >> 16, d0.w & 0xFFFFu, d0.w >> 16,
+                                         d1.x & 0xFFFFu, d1.x >> 16, d1.y & 0xFFFFu, d1.y >> 16};
+        uint32_t S = 0;
+        for (uint32_t c : cnt) S += c;
+        return S;
+    };
     for (ssw_sweep::Group& gr : s->groups) {
         uint32_t need = 0;
         for (uint32_t k = 0; k < gr.ndesc; k++) {
             const uint32_t w = gr.desc0 + k;
-            const uint4 d0 = desc[2 * w], d1 = desc[2 * w + 1];
-            const uint32_t cnt[kMaxGroup] = {d0.z & 0xFFFFu, d0.z >> 16, d0.w & 0xFFFFu, d0.w >> 16,
-                                             d1.x & 0xFFFFu, d1.x >> 16, d1.y & 0xFFFFu, d1.y >> 16};
-            uint32_t S = 0;
-            for (uint32_t c : cnt) S += c;
-            need = std::max(need, staged_lds_bytes<R>(S, E[w]));
+            need = std::max(need, staged_lds_bytes<R>(slots_of(w), E[w]));
         }
         gr.lds_bytes = (need <= (uint32_t)kLdsMax && gr.ndesc) ? need : 0u;
         if (!gr.lds_bytes) continue;
         for (uint32_t k = 0; k < gr.ndesc; k++) {
             const uint32_t w = gr.desc0 + k;
-            const uint4 d0 = desc[2 * w], d1 = desc[2 * w + 1];
-            const uint32_t cnt[kMaxGroup] = {d0.z & 0xFFFFu, d0.z >> 16, d0.w & 0xFFFFu, d0.w >> 16,
-                                             d1.x & 0xFFFFu, d1.x >> 16, d1.y & 0xFFFFu, d1.y >> 16};
-            uint32_t S = 0;
-            for (uint32_t c : cnt) S += c;
             roff[w] = (uint32_t)words;
-            words += region_words(S, E[w], ew);
+            words += stage_layout<R>(slots_of(w), E[w]).words;
         }
     }
     size_t free_b = 0, total_b = 0;
@@ -2099,7 +2136,8 @@ int build_stage_t(ssw_sweep* s) {
     HIP_TRY(hipMemsetAsync(d_fail, 0, 4, s->stream));
     k_stage_write<R><<<nd, kBlock, 0, s->stream>>>(s->d_hdr, (const R*)s->d_A, (const Entry<R>*)s->d_planes,
                                                    (const Entry<R>*)s->d_ovf, s->d_fstart, s->d_gk0, s->d_gmap,
-                                                   s->d_gdesc, s->d_glist, nd, s->d_roff, s->d_region, d_fail);
+                                                   s->d_gdesc, s->d_glist, nd, s->d_roff, s->d_ecount, s->d_region,
+                                                   d_fail);
     HIP_TRY(hipGetLastError());
     uint32_t hf = 0;
     HIP_TRY(hipMemcpyAsync(&hf, d_fail, 4, hipMemcpyDeviceToHost, s->stream));
