@@ -168,7 +168,7 @@ typedef struct ssw_info {
     int64_t group_tasks;           /* tasks a grouped full sweep solves, cones
                                       overlapping (>= the sweep's tasks)       */
     int32_t group_block;           /* grouped replay: root tasks per workgroup */
-    int32_t staged_groups;         /* groups whose full sweeps run staged in LDS */
+    int32_t reserved3;
 } ssw_info;
 
 int ssw_get_info(ssw_sweep* s, ssw_info* info);
@@ -233,20 +233,6 @@ int ssw_set_narrow_runs(ssw_sweep* s, int32_t enabled);
  * level set (ssw_info.group_fallback). Set before ssw_upload_grid. No
  * counterpart in the reference. */
 int ssw_set_group_levels(ssw_sweep* s, int32_t levels);
-
-/* Staged grouped replay of full sweeps (enabled = 1; default 0): each
- * workgroup's task records and upwind entries are stored contiguously
- * (entries inside the group resolved to the workgroup's own task slots); the
- * kernel copies them into LDS, gathers the cell inputs and the X from outside
- * the group at once, and passes X between the group's level sets through LDS
- * with LDS-only barriers. Groups whose workgroups need more than 64 KB of LDS,
- * and grids whose staged data does not fit in device memory, keep the
- * unstaged grouped kernel. Same arithmetic in the same order: bitwise the
- * same results. On the 262144-cell, 84-direction grid it measured slower than
- * the pipelined unstaged kernel (4.77 vs 4.29 ms per FP32 step), hence off.
- * The serving estimate does not include its regions. Set before
- * ssw_upload_grid. No counterpart in the reference. */
-int ssw_set_group_staging(ssw_sweep* s, int32_t enabled);
 
 /* Root tasks per workgroup of the grouped replay: 64 or 128 (0 = 64, the
  * default); the workgroup has twice as many lanes. Set before

@@ -87,9 +87,6 @@ constexpr int kGroupLevels = 4;            // grouped replay: level sets per gro
 constexpr int kConeCap = 3072;             // grouped replay build: cone members / pending entries per workgroup
 constexpr int kSortCap = 4096;             // grouped replay build: entries of one level set sorted in LDS
 constexpr int kMaxGroup = 8;               // grouped replay: level sets per group
-constexpr uint32_t kLocalFlag = 0x80000000u;  // staged grouped replay: entry names a slot of the workgroup
-constexpr int kSlotCap = kConeCap + 128;   // staged grouped replay build: tasks per workgroup
-constexpr int kLdsMax = 65536;             // staged grouped replay: LDS bytes per workgroup
 
 thread_local std::string g_err;
 // Handles alive in the process. Replay graphs are captured only while a single
@@ -1062,316 +1059,6 @@ __global__ void __launch_bounds__(kBlock) k_group_compact(Geo g, const uint4* __
 }
 
 // ---------------------------------------------------------------------------
-// staged grouped replay (full sweeps): each workgroup's tasks are stored as one
-// contiguous region of arrays over its S task slots (cell | direction << 25,
-// periodic slot, schedule position, upwind count, overflow offset, A), then
-// the upwind entries in kPlanes planes (entry q of slot s at q * S + s, unused
-// ones zero) and the overflow entries of the slots with more. An entry names
-// either a task of the same workgroup (kLocalFlag | slot) or a schedule
-// position outside the group. The kernel copies the region into LDS, gathers
-// the cell inputs and the X of every outside entry at once, then solves the
-// level sets from LDS, passing X between them through LDS with LDS-only
-// barriers; its global stores are waited for only at the kernel's end. Arrays
-// over slots keep consecutive lanes on consecutive LDS words.
-
-template <typename R>
-constexpr uint32_t entry_words() {
-    return (uint32_t)(sizeof(Entry<R>) / 4);
-}
-
-// Word offsets of a region's arrays (each 16 B aligned).
-struct StageLayout {
-    uint32_t cd, ps, P, nup, ovo, A, pl, ov, words;
-};
-
-template <typename R>
-__host__ __device__ __forceinline__ StageLayout stage_layout(uint32_t S, uint32_t Eo) {
-    const uint32_t rw = (uint32_t)(sizeof(R) / 4), ew = entry_words<R>();
-    auto a4 = [](uint32_t x) { return (x + 3u) & ~3u; };
-    StageLayout L;
-    L.cd = 0;
-    L.ps = L.cd + a4(S);
-    L.P = L.ps + a4(S);
-    L.nup = L.P + a4(S);
-    L.ovo = L.nup + a4(S);
-    L.A = L.ovo + a4(S);
-    L.pl = L.A + a4(S * rw);
-    L.ov = L.pl + a4((uint32_t)kPlanes * S * ew);
-    L.words = L.ov + a4(Eo * ew);
-    return L;
-}
-
-// LDS bytes of a workgroup with S tasks and Eo overflow entries: its region,
-// the gathered X per plane entry and overflow entry, X per task, cell inputs.
-template <typename R>
-__host__ __device__ __forceinline__ uint32_t staged_lds_bytes(uint32_t S, uint32_t Eo) {
-    const uint32_t rs = (uint32_t)sizeof(R);
-    auto al = [](uint32_t x) { return (x + 15u) & ~15u; };
-    return stage_layout<R>(S, Eo).words * 4u + al((uint32_t)kPlanes * S * rs) + al(Eo * rs) + al(S * rs) +
-           S * 16u + al(S * rs);
-}
-
-// Per workgroup descriptor: its overflow entries (upwind entries past kPlanes).
-__global__ void __launch_bounds__(kBlock) k_stage_size(const uint4* __restrict__ hdr, const uint4* __restrict__ desc,
-                                                       const uint32_t* __restrict__ list, uint32_t ndesc,
-                                                       uint32_t* E_out) {
-    const uint32_t w = blockIdx.x;
-    if (w >= ndesc) return;
-    __shared__ uint32_t tot;
-    const uint4 d0 = desc[2 * w], d1 = desc[2 * w + 1];
-    uint32_t S = 0;
-    for (int ph = 0; ph < kMaxGroup; ph++) S += desc_count(d0, d1, ph);
-    if (threadIdx.x == 0) tot = 0;
-    __syncthreads();
-    uint32_t e = 0;
-    for (uint32_t s = threadIdx.x; s < S; s += kBlock) {
-        const uint32_t nup = hdr[list[d0.x + s]].w;
-        e += nup > (uint32_t)kPlanes ? nup - kPlanes : 0u;
-    }
-    for (int o = 32; o > 0; o >>= 1) e += __shfl_down(e, o);
-    if (__lane_id() == 0 && e) atomicAdd(&tot, e);
-    __syncthreads();
-    if (threadIdx.x == 0) E_out[w] = tot;
-}
-
-// Writes workgroup w's region at roff[w] (words), an entry inside the group
-// resolved to its slot by binary search in the workgroup's sorted task list
-// of that level set (the cone contains it).
-template <typename R>
-__global__ void __launch_bounds__(kBlock) k_stage_write(const uint4* __restrict__ hdr, const R* __restrict__ Av,
-                                                        const Entry<R>* __restrict__ planes,
-                                                        const Entry<R>* __restrict__ ovf,
-                                                        const uint32_t* __restrict__ fstart,
-                                                        const int* __restrict__ gk0, const uint32_t* __restrict__ gmap,
-                                                        const uint4* __restrict__ desc,
-                                                        const uint32_t* __restrict__ list, uint32_t ndesc,
-                                                        const uint32_t* __restrict__ roff,
-                                                        const uint32_t* __restrict__ ecount, uint32_t* region,
-                                                        uint32_t* fail) {
-    const uint32_t w = blockIdx.x;
-    if (w >= ndesc || roff[w] == kNone) return;  // kNone: the descriptor's group is not staged
-    __shared__ uint32_t pos[kSlotCap], eo[kSlotCap], psum[kBlock], seg[kMaxGroup + 1];
-    const uint4 d0 = desc[2 * w], d1 = desc[2 * w + 1];
-    const int k0 = gk0[gmap[w]];
-    const int phases = (int)d0.y;
-    uint32_t S = 0;
-    if (threadIdx.x == 0) {
-        seg[0] = 0;
-        for (int ph = 0; ph < kMaxGroup; ph++) seg[ph + 1] = seg[ph] + desc_count(d0, d1, ph);
-    }
-    for (int ph = 0; ph < kMaxGroup; ph++) S += desc_count(d0, d1, ph);
-    if (S > (uint32_t)kSlotCap) {
-        if (threadIdx.x == 0) atomicOr(fail, 4u);
-        return;
-    }
-    for (uint32_t s = threadIdx.x; s < S; s += kBlock) {
-        pos[s] = list[d0.x + s];
-        const uint32_t nup = hdr[pos[s]].w;
-        eo[s] = nup > (uint32_t)kPlanes ? nup - kPlanes : 0u;
-    }
-    __syncthreads();
-    // exclusive prefix of the overflow counts: each lane a contiguous chunk
-    const uint32_t per = (S + kBlock - 1) / kBlock;
-    const uint32_t c0 = min(S, threadIdx.x * per), c1 = min(S, c0 + per);
-    uint32_t u = 0;
-    for (uint32_t c = c0; c < c1; c++) u += eo[c];
-    psum[threadIdx.x] = u;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (int l = 0; l < kBlock; l++) {
-            const uint32_t x = psum[l];
-            psum[l] = acc;
-            acc += x;
-        }
-    }
-    __syncthreads();
-    uint32_t acc = psum[threadIdx.x];
-    for (uint32_t c = c0; c < c1; c++) {
-        const uint32_t x = eo[c];
-        eo[c] = acc;
-        acc += x;
-    }
-    __syncthreads();
-    uint32_t* rg = region + roff[w];
-    const StageLayout L = stage_layout<R>(S, ecount[w]);
-    const uint32_t lo = fstart[k0];
-    Entry<R>* pl = reinterpret_cast<Entry<R>*>(rg + L.pl);
-    Entry<R>* ov = reinterpret_cast<Entry<R>*>(rg + L.ov);
-    for (uint32_t s = threadIdx.x; s < S; s += kBlock) {
-        int ph = 0;
-        while (ph + 1 < phases && s >= seg[ph + 1]) ph++;
-        const uint32_t P = pos[s];
-        const uint32_t b = fstart[k0 + ph], W = fstart[k0 + ph + 1] - b;
-        const uint4 h = hdr[P];
-        rg[L.cd + s] = h.x;
-        rg[L.ps + s] = h.y;
-        rg[L.P + s] = P;
-        rg[L.nup + s] = h.w;
-        rg[L.ovo + s] = eo[s];
-        reinterpret_cast<R*>(rg + L.A)[s] = Av[P];
-        for (uint32_t q = 0; q < max(h.w, (uint32_t)kPlanes); q++) {
-            Entry<R> e;
-            e.nb = 0;
-            if constexpr (sizeof(R) == 8) e.pad = 0;
-            e.w = (R)0;
-            if (q < h.w) {
-                e = q < (uint32_t)kPlanes ? planes[(size_t)kPlanes * b + (size_t)q * W + (P - b)]
-                                          : ovf[h.z + q - kPlanes];
-                if (e.nb >= lo) {  // inside the group: an earlier level set of this workgroup
-                    int pj = 0;
-                    while (pj + 1 < ph && e.nb >= fstart[k0 + pj + 1]) pj++;
-                    uint32_t a = seg[pj], z = seg[pj + 1];
-                    while (a < z) {
-                        const uint32_t m = (a + z) >> 1;
-                        if (pos[m] < e.nb) a = m + 1;
-                        else z = m;
-                    }
-                    if (a < seg[pj + 1] && pos[a] == e.nb) e.nb = kLocalFlag | a;
-                    else atomicOr(fail, 8u);
-                }
-            }
-            if (q < (uint32_t)kPlanes) pl[(size_t)q * S + s] = e;
-            else ov[eo[s] + q - kPlanes] = e;
-        }
-    }
-}
-
-// LDS-only workgroup barrier: orders LDS accesses (s_waitcnt lgkmcnt(0) +
-// s_barrier), without waiting for the workgroup's outstanding global stores.
-__device__ __forceinline__ void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-// Staged grouped replay of one group (full sweep): workgroup w copies its
-// region (roff[w] words; S tasks, ecount[w] overflow entries) into LDS,
-// gathers the cell inputs of its tasks and the X of its outside entries, then
-// solves its level sets one after the other from LDS. The upwind sum runs over
-// the entries in face order like every other replay, so results are bitwise
-// the same.
-template <typename R, int E_LANES>
-__global__ void __launch_bounds__(E_LANES) k_sweep_staged(Geo g, Flux<R> fl, const uint4* __restrict__ desc,
-                                                          const uint32_t* __restrict__ roff,
-                                                          const uint32_t* __restrict__ ecount,
-                                                          const uint32_t* __restrict__ region) {
-    extern __shared__ __align__(16) uint32_t lds[];
-    const uint32_t w = blockIdx.x, tid = threadIdx.x;
-    const uint4 d0 = desc[2 * w], d1 = desc[2 * w + 1];
-    const int phases = (int)d0.y;
-    uint32_t S = 0;
-    for (int ph = 0; ph < kMaxGroup; ph++) S += desc_count(d0, d1, ph);
-    const uint32_t Eo = ecount[w];
-    const StageLayout L = stage_layout<R>(S, Eo);
-    const uint32_t* src = region + roff[w];
-    // 1. the region into LDS, 16 B per lane and element; kBatch loads in flight
-    // per lane before their LDS stores (regions start 16 B aligned)
-    constexpr uint32_t kBatch = 16;
-    const uint32_t n4 = L.words / 4;
-    for (uint32_t i0 = 0; i0 < n4; i0 += kBatch * E_LANES) {
-        uint4 buf[kBatch];
-#pragma unroll
-        for (uint32_t j = 0; j < kBatch; j++) {
-            const uint32_t i = i0 + j * E_LANES + tid;
-            buf[j] = reinterpret_cast<const uint4*>(src)[min(i, n4 - 1u)];
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < kBatch; j++) {
-            const uint32_t i = i0 + j * E_LANES + tid;
-            if (i < n4) reinterpret_cast<uint4*>(lds)[i] = buf[j];
-        }
-    }
-    __syncthreads();
-    const uint32_t* cdA = lds + L.cd;
-    const uint32_t* psA = lds + L.ps;
-    const uint32_t* PA = lds + L.P;
-    const uint32_t* nupA = lds + L.nup;
-    const uint32_t* ovoA = lds + L.ovo;
-    const R* AA = reinterpret_cast<const R*>(lds + L.A);
-    const Entry<R>* pl = reinterpret_cast<const Entry<R>*>(lds + L.pl);
-    const Entry<R>* ov = reinterpret_cast<const Entry<R>*>(lds + L.ov);
-    auto al = [](uint32_t x) { return (x + 15u) & ~15u; };
-    uint8_t* fr = reinterpret_cast<uint8_t*>(lds) + (size_t)L.words * 4;
-    R* xin = reinterpret_cast<R*>(fr);
-    fr += al((uint32_t)kPlanes * S * sizeof(R));
-    R* xov = reinterpret_cast<R*>(fr);
-    fr += al(Eo * sizeof(R));
-    R* xl = reinterpret_cast<R*>(fr);
-    fr += al(S * sizeof(R));
-    double2* cin = reinterpret_cast<double2*>(fr);
-    R* per = reinterpret_cast<R*>(fr + (size_t)S * 16);
-    // 2. cell inputs and the X of entries outside the group (no dependence
-    // on this launch's results); plane entries past a task's count read X[0]
-    for (uint32_t s = tid; s < S; s += E_LANES) {
-        const uint32_t cd = cdA[s], ps = psA[s];
-        const uint32_t c = cd & kCellMask, d = cd >> kCellBits;
-        cin[s] = fl.sa[c];
-        const R p = fl.P ? fl.P[(size_t)d * g.Nper + (ps != kNone ? ps : 0u)] : (R)0;
-        per[s] = (ps != kNone) ? p : (R)0;
-    }
-    const uint32_t np = (uint32_t)kPlanes * S;
-    for (uint32_t e0 = 0; e0 < np; e0 += kBatch * E_LANES) {
-        R buf[kBatch];
-#pragma unroll
-        for (uint32_t j = 0; j < kBatch; j++) {
-            const uint32_t code = pl[min(e0 + j * E_LANES + tid, np - 1u)].nb;
-            buf[j] = fl.X[(code & kLocalFlag) ? 0u : code];
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < kBatch; j++) {
-            const uint32_t e = e0 + j * E_LANES + tid;
-            if (e < np) xin[e] = buf[j];
-        }
-    }
-    for (uint32_t e = tid; e < Eo; e += E_LANES) {
-        const uint32_t code = ov[e].nb;
-        xov[e] = fl.X[(code & kLocalFlag) ? 0u : code];
-    }
-    __syncthreads();
-    // 3. the level sets, X passed on through LDS
-    uint32_t s0 = 0;
-    for (int ph = 0; ph < phases; ph++) {
-        const uint32_t n = desc_count(d0, d1, ph);
-        for (uint32_t s = s0 + tid; s < s0 + n; s += E_LANES) {
-            const uint32_t nup = nupA[s];
-            Entry<R> e[kPlanes];
-#pragma unroll
-            for (int q = 0; q < kPlanes; q++) e[q] = pl[(uint32_t)q * S + s];
-            R x[kPlanes];
-#pragma unroll
-            for (int q = 0; q < kPlanes; q++)
-                x[q] = (e[q].nb & kLocalFlag) ? xl[e[q].nb & ~kLocalFlag] : xin[(uint32_t)q * S + s];
-            R inc = 0;
-#pragma unroll
-            for (int q = 0; q < kPlanes; q++) inc += ((uint32_t)q < nup) ? e[q].w * x[q] : (R)0;
-            for (uint32_t q = kPlanes; q < nup; q++) {
-                const uint32_t o = ovoA[s] + q - kPlanes;
-                const Entry<R> eo = ov[o];
-                inc += eo.w * ((eo.nb & kLocalFlag) ? xl[eo.nb & ~kLocalFlag] : xov[o]);
-            }
-            const uint32_t cd = cdA[s];
-            const uint32_t c = cd & kCellMask, d = cd >> kCellBits;
-            const R A = AA[s];
-            CellIn<R> ci;
-            const double2 sa = cin[s];
-            ci.src = sa.x;
-            ci.att = sa.y;
-            ci.per = per[s];
-            if (inc < (R)0) inc = 0;  // Photons::make_positive
-            const R in = (inc + (R)ci.src) + ci.per;
-            const R out = (in < (R)fl.thr) ? (R)0 : in * (R)ci.att;
-            const R xv = (A > (R)0) ? out / A : (R)0;
-            xl[s] = xv;
-            fl.X[PA[s]] = xv;
-            fl.I[(size_t)d * g.N + c] = inc;
-        }
-        s0 += n;
-        lds_barrier();
-    }
-}
-
-// ---------------------------------------------------------------------------
 // per-sweep kernels
 
 // Periodic source of every periodic cell from the current X, after each
@@ -1591,7 +1278,6 @@ struct ssw_sweep {
     struct Group {
         int k0, levels;
         uint32_t desc0, ndesc;
-        uint32_t lds_bytes;  // staged replay of full sweeps: dynamic LDS per workgroup (0: not staged)
     };
     std::vector<Group> groups;
     uint4* d_gdesc = nullptr;
@@ -1605,12 +1291,6 @@ struct ssw_sweep {
     uint32_t* d_gcount = nullptr;
     std::vector<std::vector<uint32_t>> gcount;
     int64_t group_tasks = 0;                        // tasks solved per full sweep by the grouped replay
-    bool group_stage = false;                       // staged (LDS) replay of grouped full sweeps
-    uint32_t* d_roff = nullptr;                     // staged: region offset (words) of each descriptor
-    uint32_t* d_ecount = nullptr;                   // staged: entries of each descriptor
-    uint32_t* d_region = nullptr;                   // staged: task records and entries
-    int* d_gk0 = nullptr;                           // first level set of each group
-    int staged_groups = 0;
     uint32_t glist_len = 0;                         // entries of the grouped replay's task list
     bool group_fallback = false;                    // groups requested but a cone did not fit
     bool compiled = false;
@@ -1906,16 +1586,6 @@ int64_t group_bytes(int64_t DN, int64_t n, int64_t L, int64_t T) {
     return list + dcap * 36 + kmx * 4 + (L - 1) * (list + dcap * 32);
 }
 
-// Device bytes the staged replay may take for DN tasks: arrays and entry
-// planes for 1.25 x DN slots, overflow entries for at most DN, plus the
-// per-descriptor offsets and counts. A handle whose staged data would exceed
-// this keeps the unstaged grouped kernel, so the serving estimate holds.
-int64_t stage_allowance(int64_t DN, int64_t F, int64_t D, int64_t n, int64_t es, int64_t T) {
-    const int64_t kmx = std::min<int64_t>(n + 2, 1 << 22);
-    const int64_t slots = DN + DN / 4;
-    return slots * (24 + es / 2 + kPlanes * es) + DN * es + (DN / T + kmx + 1) * 8 + (1 << 20);
-}
-
 // The replay plan of sweep level lvl from its level starts (the full
 // schedule's for level 0, the active-set list's otherwise): in the compiled
 // replay, each maximal run of at least two consecutive non-empty level sets of
@@ -2012,7 +1682,7 @@ int build_groups_t(ssw_sweep* s) {
     uint32_t ndesc = 0;
     for (int k0 = 0; k0 < s->kmax; k0 += m) {
         const int k1 = std::min(s->kmax, k0 + m), L = k1 - 1;
-        ssw_sweep::Group gr{k0, k1 - k0, ndesc, 0u, 0u};
+        ssw_sweep::Group gr{k0, k1 - k0, ndesc, 0u};
         for (int j = L; j >= k0; j--) {
             const uint32_t W = fs[j + 1] - fs[j];
             if (W == 0) continue;
@@ -2070,99 +1740,12 @@ int build_groups_t(ssw_sweep* s) {
     return SSW_OK;
 }
 
-// The staged replay of the grouped full sweeps: per descriptor its task count
-// S and entry count E (k_stage_size), the regions laid out on the host, and
-// the regions written (k_stage_write). Groups whose largest workgroup needs
-// more than kLdsMax bytes of LDS keep the unstaged grouped kernel; without
-// room for the regions in device memory nothing is staged.
-template <typename R>
-int build_stage_t(ssw_sweep* s) {
-    s->staged_groups = 0;
-    const uint32_t nd = s->ndesc;
-    if (!s->group_stage || s->groups.empty() || nd == 0) return SSW_OK;
-    uint32_t* d_se = nullptr;
-    TRY(dalloc(s, &s->d_ecount, nd));
-    k_stage_size<<<nd, kBlock, 0, s->stream>>>(s->d_hdr, s->d_gdesc, s->d_glist, nd, s->d_ecount);
-    HIP_TRY(hipGetLastError());
-    std::vector<uint32_t> E(nd);
-    std::vector<uint4> desc(2 * (size_t)nd);
-    HIP_TRY(hipMemcpyAsync(E.data(), s->d_ecount, (size_t)nd * 4, hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipMemcpyAsync(desc.data(), s->d_gdesc, desc.size() * sizeof(uint4), hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    (void)d_se;
-    std::vector<uint32_t> roff(nd, kNone);
-    uint64_t words = 0;
-    auto slots_of = [&](uint32_t w) {
-        const uint4 d0 = desc[2 * w], d1 = desc[2 * w + 1];
-        const uint32_t cnt[kMaxGroup] = {d0.z & 0xFFFFu, d0.z >> 16, d0.w & 0xFFFFu, d0.w >> 16,
-                                         d1.x & 0xFFFFu, d1.x >> 16, d1.y & 0xFFFFu, d1.y >> 16};
-        uint32_t S = 0;
-        for (uint32_t c : cnt) S += c;
-        return S;
-    };
-    for (ssw_sweep::Group& gr : s->groups) {
-        uint32_t need = 0;
-        for (uint32_t k = 0; k < gr.ndesc; k++) {
-            const uint32_t w = gr.desc0 + k;
-            need = std::max(need, staged_lds_bytes<R>(slots_of(w), E[w]));
-        }
-        gr.lds_bytes = (need <= (uint32_t)kLdsMax && gr.ndesc) ? need : 0u;
-        if (!gr.lds_bytes) continue;
-        for (uint32_t k = 0; k < gr.ndesc; k++) {
-            const uint32_t w = gr.desc0 + k;
-            roff[w] = (uint32_t)words;
-            words += stage_layout<R>(slots_of(w), E[w]).words;
-        }
-    }
-    size_t free_b = 0, total_b = 0;
-    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-    const int64_t allow = stage_allowance((int64_t)s->D * s->N, s->F, s->D, s->N, (int64_t)sizeof(Entry<R>),
-                                          s->group_block);
-    if (words == 0 || words >= 0xFFFFFFF0ull || (double)words * 4.0 > 0.8 * (double)free_b ||
-        (int64_t)words * 4 + (int64_t)nd * 8 > allow) {
-        for (ssw_sweep::Group& gr : s->groups) gr.lds_bytes = 0;
-        dfree(s, s->d_ecount);
-        return SSW_OK;
-    }
-    std::vector<int> gk0(s->groups.size());
-    for (size_t i = 0; i < s->groups.size(); i++) gk0[i] = s->groups[i].k0;
-    uint32_t* d_fail = nullptr;
-    TRY(dalloc(s, &s->d_region, (size_t)words));
-    TRY(dalloc(s, &s->d_roff, nd));
-    TRY(dalloc(s, &s->d_gk0, gk0.size()));
-    TRY(dalloc(s, &d_fail, 1));
-    HIP_TRY(hipMemcpyAsync(s->d_roff, roff.data(), (size_t)nd * 4, hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipMemcpyAsync(s->d_gk0, gk0.data(), gk0.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipMemsetAsync(d_fail, 0, 4, s->stream));
-    k_stage_write<R><<<nd, kBlock, 0, s->stream>>>(s->d_hdr, (const R*)s->d_A, (const Entry<R>*)s->d_planes,
-                                                   (const Entry<R>*)s->d_ovf, s->d_fstart, s->d_gk0, s->d_gmap,
-                                                   s->d_gdesc, s->d_glist, nd, s->d_roff, s->d_ecount, s->d_region,
-                                                   d_fail);
-    HIP_TRY(hipGetLastError());
-    uint32_t hf = 0;
-    HIP_TRY(hipMemcpyAsync(&hf, d_fail, 4, hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    dfree(s, d_fail);
-    if (hf) {  // a workgroup could not be staged: keep the unstaged grouped kernel
-        for (ssw_sweep::Group& gr : s->groups) gr.lds_bytes = 0;
-        dfree(s, s->d_region);
-        dfree(s, s->d_roff);
-        dfree(s, s->d_ecount);
-        return SSW_OK;
-    }
-    for (const ssw_sweep::Group& gr : s->groups) s->staged_groups += gr.lds_bytes ? 1 : 0;
-    return SSW_OK;
-}
-
 int build_groups(ssw_sweep* s) {
     s->groups.clear();
     s->group_tasks = 0;
     s->group_fallback = false;
-    s->staged_groups = 0;
     if (!s->compiled || s->group_request < 2) return SSW_OK;
-    TRY(s->fp32 ? build_groups_t<float>(s) : build_groups_t<double>(s));
-    if (s->groups.empty()) return SSW_OK;
-    return s->fp32 ? build_stage_t<float>(s) : build_stage_t<double>(s);
+    return s->fp32 ? build_groups_t<float>(s) : build_groups_t<double>(s);
 }
 
 // The full schedule, once per grid: Kahn layering of the whole graph, each
@@ -2364,17 +1947,6 @@ void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<
         hipEvent_t a = ev ? (*mark)() : nullptr;
         if (it.group) {  // a group of level sets, its cones one workgroup each
             const ssw_sweep::Group& gr = s->groups[it.tab];
-            if (all && gr.lds_bytes) {  // full sweep, staged in LDS
-                const uint4* d = s->d_gdesc + 2 * (size_t)gr.desc0;
-                if (s->group_block == 64)
-                    k_sweep_staged<R, 128><<<gr.ndesc, 128, gr.lds_bytes, s->stream>>>(
-                        g, fl, d, s->d_roff + gr.desc0, s->d_ecount + gr.desc0, s->d_region);
-                else
-                    k_sweep_staged<R, 256><<<gr.ndesc, 256, gr.lds_bytes, s->stream>>>(
-                        g, fl, d, s->d_roff + gr.desc0, s->d_ecount + gr.desc0, s->d_region);
-                if (ev) ev->push_back({a, (*mark)()});
-                continue;
-            }
             const uint4* dsc = (all ? s->d_gdesc : s->d_gdesc_l[lvl]) + 2 * (size_t)gr.desc0;
             const uint32_t* gl = all ? s->d_glist : s->d_glist_l[lvl];
             const Entry<R>* pl = (const Entry<R>*)s->d_planes;
@@ -2590,11 +2162,6 @@ void free_grid(ssw_sweep* s) {
     dfree(s, s->d_glist);
     dfree(s, s->d_gmap);
     dfree(s, s->d_gcount);
-    dfree(s, s->d_roff);
-    dfree(s, s->d_ecount);
-    dfree(s, s->d_region);
-    dfree(s, s->d_gk0);
-    s->staged_groups = 0;
     for (uint4*& p : s->d_gdesc_l) dfree(s, p);
     for (uint32_t*& p : s->d_glist_l) dfree(s, p);
     s->groups.clear();
@@ -2811,13 +2378,6 @@ int ssw_direction_table(int32_t num_directions, double* out) {
     return SSW_OK;
 }
 
-int ssw_set_group_staging(ssw_sweep* s, int32_t enabled) {
-    if (!s) return fail(SSW_ERR_INVALID, "null handle");
-    if (s->have_grid) return fail(SSW_ERR_INVALID, "ssw_set_group_staging must be called before ssw_upload_grid");
-    s->group_stage = enabled != 0;
-    return SSW_OK;
-}
-
 int ssw_set_group_block(ssw_sweep* s, int32_t roots) {
     if (!s) return fail(SSW_ERR_INVALID, "null handle");
     if (roots != 0 && roots != 64 && roots != 128) return fail(SSW_ERR_INVALID, "group block must be 0, 64 or 128");
@@ -2962,12 +2522,8 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     const size_t rs = real_size(s);
     const size_t comp_bytes = DN * (16 + rs + kPlanes * entry_size(s) + 4) + (size_t)s->ovf_total * entry_size(s);
     const bool grouped = s->group_request >= 2;
-    const size_t lists = grouped ? (size_t)group_bytes((int64_t)DN, n, L, s->group_block) + DN * 5 +
-                                       (s->group_stage ? (size_t)stage_allowance((int64_t)DN, F, s->D, n,
-                                                                                 (int64_t)entry_size(s),
-                                                                                 s->group_block)
-                                                       : 0)
-                                 : DN * 4 * (size_t)(L - 1);  // grouped: + build scratch (+ staged regions)
+    const size_t lists = grouped ? (size_t)group_bytes((int64_t)DN, n, L, s->group_block) + DN * 5
+                                 : DN * 4 * (size_t)(L - 1);  // grouped: + build scratch
     const size_t comp_need = comp_bytes + std::max<size_t>(DN * 8, lists + DN * 2 * rs);
     if (s->mode_request == SSW_TRANSPORT_MODE_COMPILED) s->compiled = true;
     else if (s->mode_request == SSW_TRANSPORT_MODE_GEOMETRIC) s->compiled = false;
@@ -3212,7 +2768,6 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info) {
     info->group_fallback = s->group_fallback ? 1 : 0;
     info->group_tasks = s->group_tasks;
     info->group_block = s->group_block;
-    info->staged_groups = s->staged_groups;
     info->locality_order = s->permuted ? 1 : 0;
     if (s->have_state) {
         unsigned long long f = 0, a = 0;

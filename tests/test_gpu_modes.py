@@ -218,28 +218,3 @@ def test_grouped_full_sweeps_bitwise(precision, levels, hip_available):
             ref = seq
         else:
             _assert_same(ref, seq, f"groups of {m} (block {gb}) vs per level set")
-
-
-@pytest.mark.parametrize("precision", [64, 32])
-@pytest.mark.parametrize("levels", [1, 4])
-def test_staged_groups_bitwise(precision, levels, hip_available):
-    # full sweeps staged in LDS (X passed between the group's level sets in
-    # LDS) vs the unstaged grouped kernel vs one launch per level set
-    pb = W.jittered_problem(12, 1.0, "8", 84, levels=levels, periodic=True, steps=5, precision=precision)
-    pb = pb.with_params(max_timestep=1e-3 * 3.15576e13)
-    runs = []
-    for kw in ({"group_levels": 1}, {}, {"group_staging": True},
-               {"group_staging": True, "group_levels": 8, "group_block": 128}):
-        s = Sweep(pb.params, pb.grid, *pb.state(), transport_mode=N.TRANSPORT_MODE_COMPILED, **kw)
-        info = s.info()
-        staged = kw.get("group_staging", False) and kw.get("group_levels", 4) > 1
-        assert (info.staged_groups > 0) == staged, (kw, info.staged_groups)
-        seq = []
-        for _ in range(pb.steps):
-            s.run_sweeps()
-            r = s.results(photon_rate=False)
-            seq.append((*s.transport_state(), r.ionized_hydrogen_fraction, r.temperature, r.level))
-        s.close()
-        runs.append((kw, seq))
-    for kw, seq in runs[1:]:
-        _assert_same(runs[0][1], seq, f"{kw} vs one launch per level set")
