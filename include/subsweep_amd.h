@@ -173,6 +173,24 @@ typedef struct ssw_info {
 
 int ssw_get_info(ssw_sweep* s, ssw_info* info);
 
+/* Checkpoint of the dynamic state: density, x, T, source, attenuation,
+ * species timestep, previous rate, change timescale and level per cell, the
+ * transport state X and I per (direction, cell) and the periodic sources, all
+ * in the caller's cell order, plus the timestep-level cadence. A handle of the
+ * same problem (grid, directions, levels, precision, maximum timestep; any
+ * transport mode, launch shape or cell order) continues from it with bitwise
+ * the same results. No counterpart in the reference (it restarts from
+ * snapshots of the ECS components, src/io/output/mod.rs); for restart files
+ * and for moving a problem between handles.
+ *   ssw_checkpoint_bytes  size of the checkpoint (0 before ssw_upload_state)
+ *   ssw_save_checkpoint   writes it into buf (at least that many bytes)
+ *   ssw_load_checkpoint   replaces the state of a handle after its
+ *                         ssw_upload_state; SSW_ERR_INVALID for a checkpoint
+ *                         of another problem */
+int64_t ssw_checkpoint_bytes(ssw_sweep* s);
+int ssw_save_checkpoint(ssw_sweep* s, void* buf, int64_t bytes);
+int ssw_load_checkpoint(ssw_sweep* s, const void* buf, int64_t bytes);
+
 /* Replay form of the transport (no counterpart in the reference, which has one
  * CPU code path); set before ssw_upload_grid. Both forms solve the same level
  * sets with the same arithmetic and give bitwise the same results.

@@ -156,6 +156,9 @@ SWEEP_SIGNATURES = {
     "ssw_set_group_levels": (C.c_int, [P, C.c_int32]),
     "ssw_set_group_block": (C.c_int, [P, C.c_int32]),
     "ssw_direction_table": (C.c_int, [C.c_int32, C.POINTER(C.c_double)]),
+    "ssw_checkpoint_bytes": (C.c_int64, [P]),
+    "ssw_save_checkpoint": (C.c_int, [P, C.c_void_p, C.c_int64]),
+    "ssw_load_checkpoint": (C.c_int, [P, C.c_void_p, C.c_int64]),
     "ssw_set_cell_positions": (C.c_int, [P, C.c_int64, PD]),
     "ssw_get_timing": (C.c_int, [P, PD, PD, PI64, PD, PD]),
     "ssw_download_transport": (C.c_int, [P, P, P]),

@@ -1215,6 +1215,13 @@ __global__ void k_gather_x(const R* X, const uint32_t* xpos, R* out, int64_t n) 
     if (i < n) out[i] = X[xpos[i]];
 }
 
+// X from the direction-major order into the schedule-position order.
+template <typename R>
+__global__ void k_scatter_x(R* X, const uint32_t* xpos, const R* in, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) X[xpos[i]] = in[i];
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1303,6 +1310,7 @@ struct ssw_sweep {
     std::vector<uint32_t> perm, inv;
     bool permuted = false;
     uint32_t *d_off = nullptr, *d_nk = nullptr, *d_pslot = nullptr, *d_pcells = nullptr;
+    std::vector<uint32_t> h_pcells;                 // internal cell of each periodic slot
     uint32_t *d_pfoff = nullptr, *d_pfidx = nullptr;
     double4* d_geo = nullptr;
     double *d_dirs = nullptr, *d_vol = nullptr, *d_size = nullptr;
@@ -2466,6 +2474,7 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     // by a power of two is exact.
     s->aref = (nfaces > 0 && asum > 0.0) ? std::ldexp(1.0, (int)std::lround(std::log2(asum / (double)nfaces))) : 1.0;
     s->Nper = (int64_t)pcells.size();
+    s->h_pcells = pcells;
     const int L = s->p.num_timestep_levels;
     const size_t DN = (size_t)s->D * n;
     TRY(dalloc(s, &s->d_off, n + 1));
@@ -2741,6 +2750,192 @@ int ssw_download_transport(ssw_sweep* s, void* X, void* I) {
         else to_caller(s, (double*)a, s->D);
     }
     return SSW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// checkpoint of the dynamic state (no counterpart in the reference, which
+// restarts from snapshots of the ECS components): a header, then per cell in
+// the caller's order density, x, T, source per direction, attenuation,
+// species timestep, previous rate, change timescale and level, then X and I
+// direction-major in the caller's cell order and the periodic sources per
+// direction for the periodic cells in increasing caller index, all in the
+// storage precision and scale of the transport.
+
+}  // extern "C"
+
+namespace {
+
+struct CkptHeader {
+    uint64_t magic;
+    uint32_t version, precision;
+    int64_t N, Nper;
+    int32_t D, L;
+    double lref, max_ts;
+    int32_t lowest, first_done;
+};
+constexpr uint64_t kCkptMagic = 0x31544b5057535353ull;  // "SSSWPKT1"
+
+int64_t ckpt_bytes(const ssw_sweep* s) {
+    const int64_t n = s->N, DN = (int64_t)s->D * n, rs = (int64_t)real_size(s);
+    return (int64_t)sizeof(CkptHeader) + n * 8 * 8 + ((n + 7) / 8) * 8 + 2 * DN * rs + (int64_t)s->D * s->Nper * rs;
+}
+
+// Periodic slots ordered by the caller index of their cell.
+std::vector<uint32_t> slots_by_caller(const ssw_sweep* s) {
+    std::vector<std::pair<uint32_t, uint32_t>> v(s->h_pcells.size());
+    for (size_t k = 0; k < v.size(); k++) {
+        const uint32_t ci = s->h_pcells[k];
+        v[k] = {s->permuted ? s->inv[ci] : ci, (uint32_t)k};
+    }
+    std::sort(v.begin(), v.end());
+    std::vector<uint32_t> out(v.size());
+    for (size_t k = 0; k < v.size(); k++) out[k] = v[k].second;
+    return out;
+}
+
+template <typename R>
+int save_t(ssw_sweep* s, uint8_t* buf) {
+    const int64_t n = s->N, DN = (int64_t)s->D * n;
+    CkptHeader h{kCkptMagic, 1u, (uint32_t)(s->fp32 ? 32 : 64), n, s->Nper, s->D, s->p.num_timestep_levels,
+                 s->lref, s->p.max_timestep, s->ts.lowest, s->ts.first_done ? 1 : 0};
+    std::memcpy(buf, &h, sizeof(h));
+    uint8_t* p = buf + sizeof(h);
+    std::vector<double2> sa(n);
+    double* cell = reinterpret_cast<double*>(p);
+    HIP_TRY(hipMemcpyAsync(cell + 0 * n, s->d_rho, n * 8, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(cell + 1 * n, s->d_x, n * 8, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(cell + 2 * n, s->d_T, n * 8, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(cell + 3 * n, s->d_src, n * 8, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(sa.data(), s->d_sa, n * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(cell + 5 * n, s->d_sts, n * 8, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(cell + 6 * n, s->d_prev, n * 8, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(cell + 7 * n, s->d_change, n * 8, hipMemcpyDeviceToHost, s->stream));
+    uint8_t* lev = p + n * 8 * 8;
+    HIP_TRY(hipMemcpyAsync(lev, s->d_level, n, hipMemcpyDeviceToHost, s->stream));
+    R* X = reinterpret_cast<R*>(lev + ((n + 7) / 8) * 8);
+    R* I = X + DN;
+    R* P = I + DN;
+    R* tmp = nullptr;
+    if (s->compiled) {
+        TRY(dalloc(s, &tmp, (size_t)DN));
+        k_gather_x<R><<<grid_for(DN), kBlock, 0, s->stream>>>((const R*)s->d_X, s->d_xpos, tmp, DN);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemcpyAsync(X, tmp ? (void*)tmp : s->d_X, DN * sizeof(R), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(I, s->d_I, DN * sizeof(R), hipMemcpyDeviceToHost, s->stream));
+    std::vector<R> Pd((size_t)s->D * s->Nper);
+    if (s->Nper)
+        HIP_TRY(hipMemcpyAsync(Pd.data(), s->d_P, Pd.size() * sizeof(R), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    dfree(s, tmp);
+    for (int64_t c = 0; c < n; c++) cell[4 * n + c] = sa[c].y;
+    for (int k = 0; k < 8; k++) to_caller(s, cell + k * n, 1);
+    to_caller(s, lev, 1);
+    to_caller(s, X, s->D);
+    to_caller(s, I, s->D);
+    const std::vector<uint32_t> order = slots_by_caller(s);
+    for (int d = 0; d < s->D; d++)
+        for (size_t j = 0; j < order.size(); j++) P[(size_t)d * s->Nper + j] = Pd[(size_t)d * s->Nper + order[j]];
+    return SSW_OK;
+}
+
+template <typename R>
+int load_t(ssw_sweep* s, const uint8_t* buf) {
+    const int64_t n = s->N, DN = (int64_t)s->D * n;
+    CkptHeader h;
+    std::memcpy(&h, buf, sizeof(h));
+    const uint8_t* p = buf + sizeof(h);
+    std::vector<double> cell((size_t)n * 8);
+    std::memcpy(cell.data(), p, (size_t)n * 8 * 8);
+    std::vector<uint8_t> lev(p + n * 8 * 8, p + n * 8 * 9);
+    const R* Xc = reinterpret_cast<const R*>(p + n * 8 * 8 + ((n + 7) / 8) * 8);
+    std::vector<R> X(Xc, Xc + DN), I(Xc + DN, Xc + 2 * DN);
+    const R* Pc = Xc + 2 * DN;
+    std::vector<double> tmpc;
+    for (int k = 0; k < 8; k++) {
+        const double* in = to_internal(s, cell.data() + k * n, tmpc);
+        if (in != cell.data() + k * n) std::copy(in, in + n, cell.begin() + k * n);
+    }
+    {
+        std::vector<uint8_t> tl;
+        const uint8_t* in = to_internal(s, lev.data(), tl);
+        if (in != lev.data()) lev.assign(in, in + n);
+    }
+    for (int d = 0; d < s->D; d++) {
+        std::vector<R> tr;
+        const R* inx = to_internal(s, X.data() + (size_t)d * n, tr);
+        if (inx != X.data() + (size_t)d * n) std::copy(inx, inx + n, X.begin() + (size_t)d * n);
+        const R* ini = to_internal(s, I.data() + (size_t)d * n, tr);
+        if (ini != I.data() + (size_t)d * n) std::copy(ini, ini + n, I.begin() + (size_t)d * n);
+    }
+    const std::vector<uint32_t> order = slots_by_caller(s);
+    std::vector<R> Pd((size_t)s->D * s->Nper);
+    for (int d = 0; d < s->D; d++)
+        for (size_t j = 0; j < order.size(); j++) Pd[(size_t)d * s->Nper + order[j]] = Pc[(size_t)d * s->Nper + j];
+    s->lref = h.lref;
+    std::vector<double2> sa(n);
+    for (int64_t c = 0; c < n; c++)
+        sa[c] = make_double2(s->fp32 ? cell[3 * n + c] / s->lref : cell[3 * n + c], cell[4 * n + c]);
+    HIP_TRY(hipMemcpyAsync(s->d_rho, cell.data() + 0 * n, n * 8, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_x, cell.data() + 1 * n, n * 8, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_T, cell.data() + 2 * n, n * 8, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_src, cell.data() + 3 * n, n * 8, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_sa, sa.data(), n * sizeof(double2), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_sts, cell.data() + 5 * n, n * 8, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_prev, cell.data() + 6 * n, n * 8, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_change, cell.data() + 7 * n, n * 8, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_level, lev.data(), n, hipMemcpyHostToDevice, s->stream));
+    R* tmp = nullptr;
+    if (s->compiled) {
+        TRY(dalloc(s, &tmp, (size_t)DN));
+        HIP_TRY(hipMemcpyAsync(tmp, X.data(), DN * sizeof(R), hipMemcpyHostToDevice, s->stream));
+        k_scatter_x<R><<<grid_for(DN), kBlock, 0, s->stream>>>((R*)s->d_X, s->d_xpos, tmp, DN);
+        HIP_TRY(hipGetLastError());
+    } else {
+        HIP_TRY(hipMemcpyAsync(s->d_X, X.data(), DN * sizeof(R), hipMemcpyHostToDevice, s->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(s->d_I, I.data(), DN * sizeof(R), hipMemcpyHostToDevice, s->stream));
+    if (s->Nper)
+        HIP_TRY(hipMemcpyAsync(s->d_P, Pd.data(), Pd.size() * sizeof(R), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    dfree(s, tmp);
+    s->ts.lowest = h.lowest;
+    s->ts.first_done = h.first_done != 0;
+    drop_all_lists(s);
+    TRY(set_need_all(s, 1u));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SSW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t ssw_checkpoint_bytes(ssw_sweep* s) {
+    if (!s || !s->have_state) return 0;
+    return ckpt_bytes(s);
+}
+
+int ssw_save_checkpoint(ssw_sweep* s, void* buf, int64_t bytes) {
+    if (!s || !buf) return fail(SSW_ERR_INVALID, "null argument");
+    if (!s->have_state) return fail(SSW_ERR_INVALID, "ssw_upload_state must be called first");
+    if (bytes < ckpt_bytes(s)) return fail(SSW_ERR_INVALID, "checkpoint buffer too small");
+    return s->fp32 ? save_t<float>(s, (uint8_t*)buf) : save_t<double>(s, (uint8_t*)buf);
+}
+
+int ssw_load_checkpoint(ssw_sweep* s, const void* buf, int64_t bytes) {
+    if (!s || !buf) return fail(SSW_ERR_INVALID, "null argument");
+    if (!s->have_state) return fail(SSW_ERR_INVALID, "ssw_upload_state must be called first");
+    if (bytes < (int64_t)sizeof(CkptHeader)) return fail(SSW_ERR_INVALID, "checkpoint too small");
+    CkptHeader h;
+    std::memcpy(&h, buf, sizeof(h));
+    if (h.magic != kCkptMagic || h.version != 1u) return fail(SSW_ERR_INVALID, "not a checkpoint of this library");
+    if (h.N != s->N || h.D != s->D || h.L != s->p.num_timestep_levels || h.Nper != s->Nper ||
+        h.precision != (uint32_t)(s->fp32 ? 32 : 64) || h.max_ts != s->p.max_timestep)
+        return fail(SSW_ERR_INVALID, "checkpoint of a different problem (cells, directions, levels, periodic "
+                                     "cells, precision or maximum timestep differ)");
+    if (bytes < ckpt_bytes(s)) return fail(SSW_ERR_INVALID, "checkpoint truncated");
+    return s->fp32 ? load_t<float>(s, (const uint8_t*)buf) : load_t<double>(s, (const uint8_t*)buf);
 }
 
 int ssw_get_info(ssw_sweep* s, ssw_info* info) {

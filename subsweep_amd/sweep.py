@@ -200,6 +200,18 @@ class Sweep:
         return {"transport_avg_ms": avg.value, "transport_total_ms": tot.value, "transport_launches": n.value,
                 "chemistry_total_ms": chem.value, "build_total_ms": build.value}
 
+    def save_checkpoint(self) -> bytes:
+        """The dynamic state as a checkpoint (`ssw_save_checkpoint`)."""
+        n = int(self._lib.ssw_checkpoint_bytes(self._h))
+        buf = np.zeros(n, dtype=np.uint8)
+        N.check(self._lib.ssw_save_checkpoint(self._h, buf.ctypes.data_as(C.c_void_p), n))
+        return buf.tobytes()
+
+    def load_checkpoint(self, data: bytes) -> None:
+        """Replaces the dynamic state with a checkpoint of the same problem."""
+        buf = np.frombuffer(data, dtype=np.uint8).copy()
+        N.check(self._lib.ssw_load_checkpoint(self._h, buf.ctypes.data_as(C.c_void_p), buf.size))
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             self._lib.ssw_destroy(self._h)
