@@ -168,8 +168,7 @@ typedef struct ssw_info {
     int64_t group_tasks;           /* tasks a grouped full sweep solves, cones
                                       overlapping (>= the sweep's tasks)       */
     int32_t group_block;           /* grouped replay: root tasks per workgroup */
-    int32_t group_producers;       /* 1 if producer waves prefetch each level
-                                      set's schedule data into LDS            */
+    int32_t reserved3;
 } ssw_info;
 
 int ssw_get_info(ssw_sweep* s, ssw_info* info);
@@ -252,14 +251,6 @@ int ssw_set_narrow_runs(ssw_sweep* s, int32_t enabled);
  * level set (ssw_info.group_fallback). Set before ssw_upload_grid. No
  * counterpart in the reference. */
 int ssw_set_group_levels(ssw_sweep* s, int32_t levels);
-
-/* Grouped replay with 64 roots per workgroup: one producer wave per
- * workgroup loads the next level set's schedule data into LDS while the
- * consumer waves solve the current one, so the prefetch stays in flight
- * across the level-set barriers (enabled = 1, default; 0 = the consumers
- * prefetch into registers). Bitwise the same results. May be changed at any
- * time. No counterpart in the reference. */
-int ssw_set_group_producers(ssw_sweep* s, int32_t enabled);
 
 /* Root tasks per workgroup of the grouped replay: 64 or 128 (0 = 64, the
  * default); the workgroup has twice as many lanes. Set before

@@ -75,7 +75,7 @@ class InfoC(C.Structure):
         ("group_fallback", C.c_int32),
         ("group_tasks", C.c_int64),
         ("group_block", C.c_int32),
-        ("group_producers", C.c_int32),
+        ("reserved3", C.c_int32),
     ]
 
 
@@ -155,7 +155,6 @@ SWEEP_SIGNATURES = {
     "ssw_set_narrow_runs": (C.c_int, [P, C.c_int32]),
     "ssw_set_group_levels": (C.c_int, [P, C.c_int32]),
     "ssw_set_group_block": (C.c_int, [P, C.c_int32]),
-    "ssw_set_group_producers": (C.c_int, [P, C.c_int32]),
     "ssw_direction_table": (C.c_int, [C.c_int32, C.POINTER(C.c_double)]),
     "ssw_checkpoint_bytes": (C.c_int64, [P]),
     "ssw_save_checkpoint": (C.c_int, [P, C.c_void_p, C.c_int64]),

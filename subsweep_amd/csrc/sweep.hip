@@ -1005,129 +1005,6 @@ __global__ void __launch_bounds__(E) k_sweep_group(Geo g, Flux<R> fl, const uint
     }
 }
 
-// LDS-only workgroup barrier: orders LDS accesses (s_waitcnt lgkmcnt(0) +
-// s_barrier) without waiting for the wave's outstanding global loads.
-__device__ __forceinline__ void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-// Grouped replay with producer waves: E consumer lanes solve the tasks, and
-// PW producer waves load the schedule data (header, A, plane entries,
-// position) of level set ph+1 into an LDS buffer while the consumers solve
-// level set ph. The consumers wait at each barrier for their own gathers and
-// stores (a full workgroup barrier); the producers' barrier waits only for
-// their LDS writes, so their global loads for level set ph+2 stay in flight
-// across it. Level set 0 is loaded by the consumers themselves; tasks beyond
-// the first E of a level set are solved without the LDS buffer. Same
-// arithmetic per task as every other replay.
-template <typename R, int E, int PW>
-__global__ void __launch_bounds__(E + 64 * PW) k_sweep_group_p(Geo g, Flux<R> fl, const uint4* __restrict__ hdr,
-                                                               const R* __restrict__ Av,
-                                                               const Entry<R>* __restrict__ planes,
-                                                               const Entry<R>* __restrict__ ovf,
-                                                               const uint32_t* __restrict__ fstart, int k0,
-                                                               int phases, const uint4* __restrict__ desc,
-                                                               const uint32_t* __restrict__ list, uint32_t list_len) {
-    constexpr int kPL = 64 * PW;                 // producer lanes
-    constexpr int kPer = (E + kPL - 1) / kPL;    // buffered tasks per producer lane
-    __shared__ uint4 sh_h[2][E];
-    __shared__ R sh_A[2][E];
-    __shared__ Entry<R> sh_e[2][kPlanes][E];
-    __shared__ uint32_t sh_P[2][E];
-    const uint4 d0 = desc[2 * blockIdx.x], d1 = desc[2 * blockIdx.x + 1];
-    const uint32_t tid = threadIdx.x;
-    // wave-uniform role (E is a multiple of 64): a scalar branch, so that each
-    // wave executes the barriers of its own role only
-    const bool producer = __builtin_amdgcn_readfirstlane(tid) >= (uint32_t)E;
-    uint32_t n[kMaxGroup], off[kMaxGroup], bb[kMaxGroup], WW[kMaxGroup];
-#pragma unroll
-    for (int ph = 0; ph < kMaxGroup; ph++) {
-        n[ph] = desc_count(d0, d1, ph);
-        off[ph] = ph == 0 ? d0.x : off[ph - 1] + n[ph - 1];
-        const int k = k0 + min(ph, phases - 1);
-        bb[ph] = fstart[k];
-        WW[ph] = fstart[k + 1] - bb[ph];
-    }
-    if (!producer) {
-        // consumers: level set 0 from global memory, then from the LDS buffers
-        uint32_t P0 = list[min(off[0] + tid, list_len - 1u)];
-        P0 = tid < n[0] ? P0 : bb[0];
-        TaskS<R> cur;
-        load_task<R>(cur, hdr + bb[0], Av + bb[0], planes + (size_t)kPlanes * bb[0], WW[0], P0 - bb[0]);
-        uint32_t P = P0;
-        for (int ph = 0; ph < phases; ph++) {
-            if (ph > 0) {
-                const int bf = ph & 1;
-                const uint32_t i = tid < n[ph] ? tid : 0u;  // lanes without a task read slot 0
-                cur.h = sh_h[bf][i];
-                cur.A = sh_A[bf][i];
-#pragma unroll
-                for (int q = 0; q < kPlanes; q++) cur.e[q] = sh_e[bf][q][i];
-                P = sh_P[bf][i];
-            }
-            if (tid < n[ph]) {
-                TaskG<R> G;
-                issue_task<R>(G, g, fl, cur, P);
-                finish_task_c<R>(g, fl, cur, G, ovf, P);
-            }
-            for (uint32_t i = tid + E; i < n[ph]; i += E)
-                comp_task<R>(g, fl, hdr + bb[ph], Av + bb[ph], planes + (size_t)kPlanes * bb[ph], ovf, bb[ph],
-                             WW[ph], list[off[ph] + i] - bb[ph]);
-            __syncthreads();
-        }
-    } else {
-        // producers: slots j, j + kPL, ... (< E) of level set ph+1 into LDS
-        // buffer (ph+1)&1 during level set ph; loads for ph+2 issued after.
-        // Positions of every level set read up front; loops unrolled over
-        // kMaxGroup so that every register array index is a constant.
-        const uint32_t j = tid - E;
-        uint32_t pos[kMaxGroup][kPer];
-#pragma unroll
-        for (int ph = 0; ph < kMaxGroup; ph++)
-#pragma unroll
-            for (int k = 0; k < kPer; k++) {
-                const uint32_t slot = j + (uint32_t)k * kPL;
-                const uint32_t v = list[min(off[ph] + slot, list_len - 1u)];
-                pos[ph][k] = (ph < phases && slot < (uint32_t)E && slot < n[ph]) ? v : bb[ph];
-            }
-        TaskS<R> buf[kPer];
-        if (phases > 1)
-#pragma unroll
-            for (int k = 0; k < kPer; k++)
-                load_task<R>(buf[k], hdr + bb[1], Av + bb[1], planes + (size_t)kPlanes * bb[1], WW[1],
-                             pos[1][k] - bb[1]);
-#pragma unroll
-        for (int ph = 0; ph < kMaxGroup; ph++) {
-            if (ph < phases) {
-                const int q1 = ph + 1 < kMaxGroup ? ph + 1 : ph;
-                const int q2 = ph + 2 < kMaxGroup ? ph + 2 : ph;
-                if (ph + 1 < phases) {
-                    const int bf = (ph + 1) & 1;
-#pragma unroll
-                    for (int k = 0; k < kPer; k++) {
-                        const uint32_t slot = j + (uint32_t)k * kPL;
-                        if (slot < (uint32_t)E && slot < n[q1]) {
-                            sh_h[bf][slot] = buf[k].h;
-                            sh_A[bf][slot] = buf[k].A;
-#pragma unroll
-                            for (int q = 0; q < kPlanes; q++) sh_e[bf][q][slot] = buf[k].e[q];
-                            sh_P[bf][slot] = pos[q1][k];
-                        }
-                    }
-                    if (ph + 2 < phases)
-#pragma unroll
-                        for (int k = 0; k < kPer; k++)
-                            load_task<R>(buf[k], hdr + bb[q2], Av + bb[q2], planes + (size_t)kPlanes * bb[q2], WW[q2],
-                                         pos[q2][k] - bb[q2]);
-                }
-                lds_barrier();
-            }
-        }
-    }
-}
-
 // The grouped replay of a restricted sweep (active set {level >= lvl}):
 // workgroup w filters descriptor w's task list to the active tasks, in order,
 // into the same place of list_out, and writes the filtered counts; the active
@@ -1403,7 +1280,6 @@ struct ssw_sweep {
     bool narrow_runs = true;                        // runs of narrow level sets in one workgroup
     int group_request = kGroupLevels;               // level sets per grouped launch (1: off)
     int group_block = kGroupT;                      // grouped replay: root tasks per workgroup
-    bool group_producers = true;                    // grouped replay: producer waves (64 roots)
     // grouped replay of full sweeps (compiled): per group of level sets its
     // first level set, level-set count and workgroup descriptors
     struct Group {
@@ -2082,11 +1958,7 @@ void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<
             const uint4* dsc = (all ? s->d_gdesc : s->d_gdesc_l[lvl]) + 2 * (size_t)gr.desc0;
             const uint32_t* gl = all ? s->d_glist : s->d_glist_l[lvl];
             const Entry<R>* pl = (const Entry<R>*)s->d_planes;
-            if (s->group_block == 64 && s->group_producers)
-                k_sweep_group_p<R, 128, 1><<<gr.ndesc, 192, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl,
-                                                                           ovf, s->d_fstart, gr.k0, gr.levels, dsc,
-                                                                           gl, s->glist_len);
-            else if (s->group_block == 64)
+            if (s->group_block == 64)
                 k_sweep_group<R, 128><<<gr.ndesc, 128, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf,
                                                                       s->d_fstart, gr.k0, gr.levels, dsc, gl,
                                                                       s->glist_len);
@@ -2511,18 +2383,6 @@ int ssw_direction_table(int32_t num_directions, double* out) {
     }
     for (int d = 0; d < num_directions; d++)
         for (int k = 0; k < 3; k++) out[3 * d + k] = bins[d][k];
-    return SSW_OK;
-}
-
-int ssw_set_group_producers(ssw_sweep* s, int32_t enabled) {
-    if (!s) return fail(SSW_ERR_INVALID, "null handle");
-    s->group_producers = enabled != 0;
-    for (hipGraphExec_t& ge : s->graphs)  // graphs hold the old launch shape
-        if (ge) {
-            (void)hipGraphExecDestroy(ge);
-            ge = nullptr;
-        }
-    std::fill(s->replays.begin(), s->replays.end(), 0);
     return SSW_OK;
 }
 
@@ -3103,7 +2963,6 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info) {
     info->group_fallback = s->group_fallback ? 1 : 0;
     info->group_tasks = s->group_tasks;
     info->group_block = s->group_block;
-    info->group_producers = (s->group_producers && s->group_block == 64 && !s->groups.empty()) ? 1 : 0;
     info->locality_order = s->permuted ? 1 : 0;
     if (s->have_state) {
         unsigned long long f = 0, a = 0;

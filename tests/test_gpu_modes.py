@@ -200,14 +200,12 @@ def test_grouped_full_sweeps_bitwise(precision, levels, hip_available):
     pb = W.jittered_problem(12, 1.0, "8", 84, levels=levels, periodic=True, steps=5, precision=precision)
     pb = pb.with_params(max_timestep=1e-3 * 3.15576e13)
     ref = None
-    for m, gb, prod in ((1, 0, True), (2, 128, True), (3, 64, True), (3, 64, False), (4, 128, True),
-                        (4, 64, False), (6, 64, True), (8, 128, True)):
+    for m, gb in ((1, 0), (2, 128), (3, 64), (4, 128), (6, 64), (8, 128)):
         s = Sweep(pb.params, pb.grid, *pb.state(), transport_mode=N.TRANSPORT_MODE_COMPILED, group_levels=m,
-                  group_block=gb, group_producers=prod)
+                  group_block=gb)
         info = s.info()
         assert info.group_levels == m and info.group_fallback == 0
         assert info.group_block == (gb or 64)
-        assert info.group_producers == (1 if (prod and m > 1 and (gb or 64) == 64) else 0)
         if m > 1:
             assert info.group_tasks >= pb.grid.num_cells * info.num_directions
         seq = []
@@ -219,4 +217,4 @@ def test_grouped_full_sweeps_bitwise(precision, levels, hip_available):
         if ref is None:
             ref = seq
         else:
-            _assert_same(ref, seq, f"groups of {m} (block {gb}, producers {prod}) vs per level set")
+            _assert_same(ref, seq, f"groups of {m} (block {gb}) vs per level set")
