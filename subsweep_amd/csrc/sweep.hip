@@ -1007,15 +1007,20 @@ __global__ void __launch_bounds__(E) k_sweep_group(Geo g, Flux<R> fl, const uint
 
 // The grouped replay of a restricted sweep (active set {level >= lvl}):
 // workgroup w filters descriptor w's task list to the active tasks, in order,
-// into the same place of list_out, and writes the filtered counts; the active
-// tasks of each group are summed into gcount (gmap: group of each
+// into the same place of list_out, and, if any is left, appends its filtered
+// descriptor to its group's descriptors (from gbase[group], slot counted in
+// gblocks), so that a launch starts only workgroups with active tasks; the
+// active tasks of each group are summed into gcount (gmap: group of each
 // descriptor). The cones of the full graph contain the active cones, so every
-// dependency of an active task inside a launch stays inside its workgroup.
+// dependency of an active task inside a launch stays inside its workgroup;
+// the workgroups of a launch are independent, so their order is free.
 __global__ void __launch_bounds__(kBlock) k_group_compact(Geo g, const uint4* __restrict__ hdr, int lvl,
                                                           const uint4* __restrict__ desc_in,
                                                           const uint32_t* __restrict__ list_in,
-                                                          const uint32_t* __restrict__ gmap, uint32_t ndesc,
-                                                          uint4* desc_out, uint32_t* list_out, uint32_t* gcount) {
+                                                          const uint32_t* __restrict__ gmap,
+                                                          const uint32_t* __restrict__ gbase, uint32_t ndesc,
+                                                          uint4* desc_out, uint32_t* list_out, uint32_t* gcount,
+                                                          uint32_t* gblocks) {
     const uint32_t w = blockIdx.x;
     if (w >= ndesc) return;
     __shared__ uint32_t wave_tot[kBlock / 64];
@@ -1050,11 +1055,15 @@ __global__ void __launch_bounds__(kBlock) k_group_compact(Geo g, const uint4* __
         src += n;
     }
     if (threadIdx.x == 0) {
-        desc_out[2 * w] = make_uint4(d0.x, d0.y, m[0] | (m[1] << 16), m[2] | (m[3] << 16));
-        desc_out[2 * w + 1] = make_uint4(m[4] | (m[5] << 16), m[6] | (m[7] << 16), 0u, 0u);
         uint32_t total = 0;
         for (int k = 0; k < kMaxGroup; k++) total += m[k];
-        if (total) atomicAdd(&gcount[gmap[w]], total);
+        if (total) {
+            const uint32_t gr = gmap[w];
+            atomicAdd(&gcount[gr], total);
+            const uint32_t o = gbase[gr] + atomicAdd(&gblocks[gr], 1u);
+            desc_out[2 * o] = make_uint4(d0.x, d0.y, m[0] | (m[1] << 16), m[2] | (m[3] << 16));
+            desc_out[2 * o + 1] = make_uint4(m[4] | (m[5] << 16), m[6] | (m[7] << 16), 0u, 0u);
+        }
     }
 }
 
@@ -1296,7 +1305,9 @@ struct ssw_sweep {
     std::vector<uint4*> d_gdesc_l;
     std::vector<uint32_t*> d_glist_l;
     uint32_t* d_gcount = nullptr;
-    std::vector<std::vector<uint32_t>> gcount;
+    uint32_t* d_gbase = nullptr;                    // first descriptor of each group
+    uint32_t* d_gblocks = nullptr;                  // restricted sweep: workgroups with active tasks per group
+    std::vector<std::vector<uint32_t>> gcount, gblocks;
     int64_t group_tasks = 0;                        // tasks solved per full sweep by the grouped replay
     uint32_t glist_len = 0;                         // entries of the grouped replay's task list
     bool group_fallback = false;                    // groups requested but a cone did not fit
@@ -1744,6 +1755,11 @@ int build_groups_t(ssw_sweep* s) {
     TRY(dalloc(s, &s->d_gmap, gmap.size()));
     HIP_TRY(hipMemcpyAsync(s->d_gmap, gmap.data(), gmap.size() * 4, hipMemcpyHostToDevice, s->stream));
     TRY(dalloc(s, &s->d_gcount, std::max<size_t>(1, s->groups.size())));
+    TRY(dalloc(s, &s->d_gblocks, std::max<size_t>(1, s->groups.size())));
+    std::vector<uint32_t> gbase(std::max<size_t>(1, s->groups.size()), 0u);
+    for (size_t i = 0; i < s->groups.size(); i++) gbase[i] = s->groups[i].desc0;
+    TRY(dalloc(s, &s->d_gbase, gbase.size()));
+    HIP_TRY(hipMemcpyAsync(s->d_gbase, gbase.data(), gbase.size() * 4, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return SSW_OK;
 }
@@ -1867,6 +1883,7 @@ int build_schedule(ssw_sweep* s) {
     s->d_gdesc_l.assign(L, nullptr);
     s->d_glist_l.assign(L, nullptr);
     s->gcount.assign(L, std::vector<uint32_t>());
+    s->gblocks.assign(L, std::vector<uint32_t>());
     s->runtab.assign(L, std::vector<uint4>());
     s->plan.assign(L, std::vector<ssw_sweep::ReplayItem>());
     TRY(build_groups(s));
@@ -1884,15 +1901,19 @@ int make_group_list(ssw_sweep* s, int lvl) {
     if (!s->d_glist_l[lvl]) TRY(dalloc(s, &s->d_glist_l[lvl], s->glist_len));
     const size_t ng = std::max<size_t>(1, s->groups.size());
     HIP_TRY(hipMemsetAsync(s->d_gcount, 0, ng * 4, s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_gblocks, 0, ng * 4, s->stream));
     Geo g = make_geo(s);
     if (s->ndesc)
         k_group_compact<<<s->ndesc, kBlock, 0, s->stream>>>(g, s->d_hdr, lvl, s->d_gdesc, s->d_glist, s->d_gmap,
-                                                            s->ndesc, s->d_gdesc_l[lvl], s->d_glist_l[lvl],
-                                                            s->d_gcount);
+                                                            s->d_gbase, s->ndesc, s->d_gdesc_l[lvl],
+                                                            s->d_glist_l[lvl], s->d_gcount, s->d_gblocks);
     HIP_TRY(hipGetLastError());
     std::vector<uint32_t>& gc = s->gcount[lvl];
     gc.assign(ng, 0u);
     HIP_TRY(hipMemcpyAsync(gc.data(), s->d_gcount, ng * 4, hipMemcpyDeviceToHost, s->stream));
+    std::vector<uint32_t>& gb = s->gblocks[lvl];
+    gb.assign(ng, 0u);
+    HIP_TRY(hipMemcpyAsync(gb.data(), s->d_gblocks, ng * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->list_ok[lvl] = 1;
     s->replays[lvl] = 0;
@@ -1957,13 +1978,14 @@ void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<
             const ssw_sweep::Group& gr = s->groups[it.tab];
             const uint4* dsc = (all ? s->d_gdesc : s->d_gdesc_l[lvl]) + 2 * (size_t)gr.desc0;
             const uint32_t* gl = all ? s->d_glist : s->d_glist_l[lvl];
+            const uint32_t nblk = all ? gr.ndesc : s->gblocks[lvl][it.tab];  // restricted: workgroups with active tasks
             const Entry<R>* pl = (const Entry<R>*)s->d_planes;
             if (s->group_block == 64)
-                k_sweep_group<R, 128><<<gr.ndesc, 128, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf,
+                k_sweep_group<R, 128><<<nblk, 128, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf,
                                                                       s->d_fstart, gr.k0, gr.levels, dsc, gl,
                                                                       s->glist_len);
             else
-                k_sweep_group<R, 256><<<gr.ndesc, 256, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf,
+                k_sweep_group<R, 256><<<nblk, 256, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf,
                                                                       s->d_fstart, gr.k0, gr.levels, dsc, gl,
                                                                       s->glist_len);
             if (ev) ev->push_back({a, (*mark)()});
@@ -2170,6 +2192,8 @@ void free_grid(ssw_sweep* s) {
     dfree(s, s->d_glist);
     dfree(s, s->d_gmap);
     dfree(s, s->d_gcount);
+    dfree(s, s->d_gbase);
+    dfree(s, s->d_gblocks);
     for (uint4*& p : s->d_gdesc_l) dfree(s, p);
     for (uint32_t*& p : s->d_glist_l) dfree(s, p);
     s->groups.clear();
