@@ -823,25 +823,27 @@ __device__ __forceinline__ uint32_t desc_count(const uint4& d0, const uint4& d1,
     return (ph & 1) ? (w >> 16) : (w & 0xFFFFu);
 }
 
-// One workgroup per block of T roots of level set j in the group starting at
-// level set k0: its cone per level set (sorted, unique), written with the
-// roots to the task list, level set k0 first, and its descriptor. Members
-// and roots are marked done in todo. Roots are roots[r] or, without a root
-// list, positions root_base + r. Pending upwind entries are kept per level
-// set: those of level set i are taken out when i is processed.
+// One workgroup per block of at most T roots of one level set j of a group
+// starting at level set k0, meta[block] = {k0, j, first root, roots}: its cone
+// per level set (sorted, unique), written with the roots to the task list,
+// level set k0 first, and its descriptor. Members and roots are marked done
+// in todo. Roots are roots[first + r] or, without a root list, positions
+// first + r. Pending upwind entries are kept per level set: those of level
+// set i are taken out when i is processed.
 template <typename R, int T>
 __global__ void __launch_bounds__(T) k_cone(const uint4* hdr, const Entry<R>* planes, const Entry<R>* ovf,
-                                            const uint32_t* fstart, int k0, int j, const uint32_t* roots,
-                                            uint32_t root_base, uint32_t nroots, uint8_t* todo, uint32_t* list,
-                                            uint32_t list_cap, uint32_t* cursor, uint4* desc, uint32_t* fail) {
+                                            const uint32_t* fstart, const uint4* meta, const uint32_t* roots,
+                                            uint8_t* todo, uint32_t* list, uint32_t list_cap, uint32_t* cursor,
+                                            uint4* desc, uint32_t* fail) {
     __shared__ uint32_t candA[kConeCap], candB[kConeCap], tmp[kSortCap], memb[kConeCap];
     __shared__ uint32_t ncand, nkeep, ntmp, nmemb, over, base;
     __shared__ uint32_t cnt[kMaxGroup], psum[T];
     uint32_t* cand = candA;
     uint32_t* keep = candB;
     const uint32_t tid = threadIdx.x;
-    const uint32_t r0 = blockIdx.x * T;
-    const uint32_t n = min((uint32_t)T, nroots - r0);
+    const uint4 mb = meta[blockIdx.x];
+    const int k0 = (int)mb.x, j = (int)mb.y;
+    const uint32_t n = min((uint32_t)T, mb.w);
     if (tid == 0) {
         ncand = 0;
         nmemb = 0;
@@ -852,7 +854,7 @@ __global__ void __launch_bounds__(T) k_cone(const uint4* hdr, const Entry<R>* pl
     const uint32_t lo = fstart[k0];
     uint32_t root = 0;
     if (tid < n) {
-        root = roots ? roots[r0 + tid] : root_base + r0 + tid;
+        root = roots ? roots[mb.z + tid] : mb.z + tid;
         push_upwind<R>(hdr, planes, ovf, root, fstart[j], fstart[j + 1] - fstart[j], lo, fstart[j], cand, &ncand,
                        &over);
     }
@@ -955,6 +957,48 @@ __global__ void __launch_bounds__(T) k_cone(const uint4* hdr, const Entry<R>* pl
         desc[2 * blockIdx.x] = make_uint4(base, (uint32_t)(j - k0 + 1), c[0] | (c[1] << 16), c[2] | (c[3] << 16));
         desc[2 * blockIdx.x + 1] = make_uint4(c[4] | (c[5] << 16), c[6] | (c[7] << 16), 0u, 0u);
     }
+}
+
+// Cone build: flags[p] = todo[p] for the positions of the level sets marked
+// in lvlflag (binary search of p's level set in fstart), else 0.
+__global__ void k_cone_flags(const uint32_t* fstart, int kmax, const uint8_t* lvlflag, const uint8_t* todo,
+                             uint8_t* flags, uint32_t n) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int a = 0, b = kmax - 1;
+    while (a < b) {
+        const int m = (a + b + 1) >> 1;
+        if (fstart[m] <= p) a = m;
+        else b = m - 1;
+    }
+    flags[p] = lvlflag[a] ? todo[p] : (uint8_t)0;
+}
+
+// Cone build: for level set lv[i], the range [lo, hi) of the sorted root list
+// (nsel entries) holding its positions.
+__global__ void k_cone_bounds(const uint32_t* roots, const uint32_t* nsel, const uint32_t* fstart, const int* lv,
+                              int nlv, uint32_t* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nlv) return;
+    const uint32_t n = *nsel;
+    for (int e = 0; e < 2; e++) {
+        const uint32_t key = fstart[lv[i] + e];
+        uint32_t a = 0, z = n;
+        while (a < z) {
+            const uint32_t m = (a + z) >> 1;
+            if (roots[m] < key) a = m + 1;
+            else z = m;
+        }
+        out[2 * i + e] = a;
+    }
+}
+
+// Descriptors into group order: out[i] = in[perm[i]] (two uint4 each).
+__global__ void k_permute_desc(const uint4* in, const uint32_t* perm, uint4* out, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[2 * i] = in[2 * perm[i]];
+    out[2 * i + 1] = in[2 * perm[i] + 1];
 }
 
 // Grouped replay of the level sets k0 .. k0+phases-1 with E lanes per
@@ -1558,39 +1602,59 @@ void release_layering_scratch(ssw_sweep* s) {
 }
 
 // Kahn layering of the active set {level >= lvl} into the scratch records:
-// counting, then one launch per level set until the queue is empty, reading
-// each level's width back (the depth and widths are not known in advance).
+// counting, then one launch per level set until the queue is empty. The
+// launches go out in batches of kKahnBatch level sets between host reads of
+// the level starts (a launch past the last level set finds it empty and only
+// repeats its start); the grid of a batch is sized from the last width read.
 // `expected` active tasks must all be placed, else the graph has a cycle.
+constexpr int kKahnBatch = 32;
+
 int kahn_layering(ssw_sweep* s, int lvl, int64_t expected, int* depth_out, std::vector<uint32_t>* starts) {
     Geo g = make_geo(s);
     k_kahn_reset<<<1, 64, 0, s->stream>>>(g);
     k_count<<<grid_for((int64_t)s->N * s->D), kBlock, 0, s->stream>>>(g, lvl, 0);
     HIP_TRY(hipGetLastError());
     if (starts) starts->assign(1, 0u);
-    int64_t total = 0;
     uint32_t c3[3];
-    for (int k = 0;; k++) {
-        HIP_TRY(hipMemcpyAsync(c3, s->d_ctr, sizeof(c3), hipMemcpyDeviceToHost, s->stream));
-        HIP_TRY(hipStreamSynchronize(s->stream));
-        const uint32_t width = c3[k % 3];
-        if (width == 0) {
-            if (total != expected)
-                return fail(SSW_ERR_CYCLE, "dependency cycle: " + std::to_string(expected - total) +
-                                               " cell-direction pairs never become ready");
-            if (k == 0) {  // no active task: one empty level set
-                k_kahn<<<1, kBlock, 0, s->stream>>>(g, 0);
-                HIP_TRY(hipGetLastError());
-                if (starts) starts->push_back(0u);
-            }
-            *depth_out = std::max(1, k);
-            return SSW_OK;
-        }
-        if (k + 2 > s->kcap) return fail(SSW_ERR_CYCLE, "level-set depth exceeds its bound");
-        total += width;
-        if (starts) starts->push_back((uint32_t)total);
-        const unsigned blocks = (unsigned)std::min<int64_t>(16384, (width + kTasksPerBlock - 1) / kTasksPerBlock);
-        k_kahn<<<blocks, kBlock, 0, s->stream>>>(g, k);
+    HIP_TRY(hipMemcpyAsync(c3, s->d_ctr, sizeof(c3), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (c3[0] == 0) {
+        if (expected != 0)
+            return fail(SSW_ERR_CYCLE, "dependency cycle: " + std::to_string(expected) +
+                                           " cell-direction pairs never become ready");
+        k_kahn<<<1, kBlock, 0, s->stream>>>(g, 0);  // no active task: one empty level set
         HIP_TRY(hipGetLastError());
+        if (starts) starts->push_back(0u);
+        *depth_out = 1;
+        return SSW_OK;
+    }
+    uint32_t hint = c3[0], prev = 0;
+    std::vector<uint32_t> part(kKahnBatch);
+    for (int k = 0;;) {
+        const int kend = std::min(k + kKahnBatch, s->kcap - 1);
+        if (k >= kend) return fail(SSW_ERR_CYCLE, "level-set depth exceeds its bound");
+        const unsigned blocks =
+            (unsigned)std::max<int64_t>(1, std::min<int64_t>(16384, (2 * (int64_t)hint + kTasksPerBlock - 1) /
+                                                                        kTasksPerBlock));
+        for (int kk = k; kk < kend; kk++) k_kahn<<<blocks, kBlock, 0, s->stream>>>(g, kk);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(part.data(), s->d_lstart + k + 1, (size_t)(kend - k) * 4, hipMemcpyDeviceToHost,
+                               s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        for (int kk = k; kk < kend; kk++) {
+            const uint32_t cur = part[kk - k];
+            if (cur == prev) {  // level set kk is empty: the layering ended with kk level sets
+                if ((int64_t)prev != expected)
+                    return fail(SSW_ERR_CYCLE, "dependency cycle: " + std::to_string(expected - (int64_t)prev) +
+                                                   " cell-direction pairs never become ready");
+                *depth_out = std::max(1, kk);
+                return SSW_OK;
+            }
+            if (starts) starts->push_back(cur);
+            hint = cur - prev;
+            prev = cur;
+        }
+        k = kend;
     }
 }
 
@@ -1675,77 +1739,151 @@ int build_groups_t(ssw_sweep* s) {
     const int m = s->group_request;
     const size_t DN = (size_t)s->D * s->N;
     const std::vector<uint32_t>& fs = s->full_start;
-    uint32_t maxw = 1;
-    for (int k = 0; k < s->kmax; k++) maxw = std::max(maxw, fs[k + 1] - fs[k]);
     const int T = s->group_block;
     const size_t desc_cap = DN / T + (size_t)s->kmax + 1;  // at most ceil(width / T) per level set
     const size_t list_cap = DN + DN / 4 + (1u << 20);
-    uint8_t* todo = nullptr;
-    uint32_t *roots = nullptr, *ctl = nullptr;  // ctl: cursor, fail, selected count
-    uint8_t* tmp = nullptr;
+    // groups, and per round r the level set k0 + levels - 1 - r of each
+    // group that has it (round 0: the last level sets, all tasks roots)
+    s->groups.clear();
+    for (int k0 = 0; k0 < s->kmax; k0 += m) s->groups.push_back({k0, std::min(s->kmax, k0 + m) - k0, 0u, 0u});
+    const size_t ng = s->groups.size();
+    size_t round_roots = 1;
+    for (int r = 1; r < m; r++) {
+        size_t w = 0;
+        for (const ssw_sweep::Group& gr : s->groups)
+            if (gr.levels > r) w += fs[gr.k0 + gr.levels - r] - fs[gr.k0 + gr.levels - 1 - r];
+        round_roots = std::max(round_roots, w);
+    }
+    uint8_t *todo = nullptr, *flags = nullptr, *lvlflag = nullptr, *tmp = nullptr;
+    uint32_t *roots = nullptr, *ctl = nullptr, *bounds = nullptr, *perm = nullptr;  // ctl: cursor, fail, selected
+    int* lv = nullptr;
+    uint4 *meta = nullptr, *desc_rm = nullptr;
     TRY(dalloc(s, &todo, DN));
-    TRY(dalloc(s, &roots, maxw));
     TRY(dalloc(s, &ctl, 3));
     TRY(dalloc(s, &s->d_glist, list_cap));
-    TRY(dalloc(s, &s->d_gdesc, 2 * desc_cap));
+    TRY(dalloc(s, &desc_rm, 2 * desc_cap));
     HIP_TRY(hipMemsetAsync(todo, 1, DN, s->stream));
     HIP_TRY(hipMemsetAsync(ctl, 0, 12, s->stream));
     size_t tb = 0;
-    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, tb, hipcub::CountingInputIterator<uint32_t>(0u), todo, roots,
-                                          ctl + 2, (int)maxw, s->stream));
-    TRY(dalloc(s, &tmp, tb));
+    if (m > 1) {
+        TRY(dalloc(s, &flags, DN));
+        TRY(dalloc(s, &lvlflag, (size_t)s->kmax));
+        TRY(dalloc(s, &roots, round_roots));
+        TRY(dalloc(s, &lv, ng));
+        TRY(dalloc(s, &bounds, 2 * ng));
+        HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, tb, hipcub::CountingInputIterator<uint32_t>(0u), flags, roots,
+                                              ctl + 2, (int)DN, s->stream));
+        TRY(dalloc(s, &tmp, tb));
+    }
     const uint4* hdr = s->d_hdr;
     const Entry<R>* planes = (const Entry<R>*)s->d_planes;
     const Entry<R>* ovf = (const Entry<R>*)s->d_ovf;
-    s->groups.clear();
+    // blocks of group g in round r: [rstart[r][g], rstart[r][g] + rcount[r][g]) of the round-major descriptors
+    std::vector<std::vector<uint32_t>> rstart(m, std::vector<uint32_t>(ng, 0u)), rcount(rstart);
     uint32_t ndesc = 0;
-    for (int k0 = 0; k0 < s->kmax; k0 += m) {
-        const int k1 = std::min(s->kmax, k0 + m), L = k1 - 1;
-        ssw_sweep::Group gr{k0, k1 - k0, ndesc, 0u};
-        for (int j = L; j >= k0; j--) {
-            const uint32_t W = fs[j + 1] - fs[j];
-            if (W == 0) continue;
-            uint32_t n = W;
-            const uint32_t* rl = nullptr;
-            if (j < L) {
-                size_t tbj = tb;
-                HIP_TRY(hipcub::DeviceSelect::Flagged(tmp, tbj, hipcub::CountingInputIterator<uint32_t>(fs[j]),
-                                                      todo + fs[j], roots, ctl + 2, (int)W, s->stream));
-                HIP_TRY(hipMemcpyAsync(&n, ctl + 2, 4, hipMemcpyDeviceToHost, s->stream));
-                HIP_TRY(hipStreamSynchronize(s->stream));
-                if (n == 0) continue;
-                rl = roots;
-            }
-            const uint32_t nb = (n + T - 1) / T;
-            if ((size_t)ndesc + nb > desc_cap) return fail(SSW_ERR_HIP, "grouped replay: descriptor bound exceeded");
-            if (T == 64)
-                k_cone<R, 64><<<nb, 64, 0, s->stream>>>(hdr, planes, ovf, s->d_fstart, k0, j, rl, fs[j], n, todo,
-                                                        s->d_glist, (uint32_t)list_cap, ctl, s->d_gdesc + 2 * ndesc,
-                                                        ctl + 1);
-            else
-                k_cone<R, 128><<<nb, 128, 0, s->stream>>>(hdr, planes, ovf, s->d_fstart, k0, j, rl, fs[j], n, todo,
-                                                          s->d_glist, (uint32_t)list_cap, ctl,
-                                                          s->d_gdesc + 2 * ndesc, ctl + 1);
+    std::vector<uint4> hmeta;
+    std::vector<uint8_t> hflag(s->kmax);
+    std::vector<int> hlv;
+    std::vector<uint32_t> hb;
+    for (int r = 0; r < m; r++) {
+        hmeta.clear();
+        hlv.clear();
+        const uint32_t* rl = nullptr;
+        if (r > 0) {  // roots: the tasks of the round's level sets in no cone yet, in position order
+            std::fill(hflag.begin(), hflag.end(), 0);
+            for (const ssw_sweep::Group& gr : s->groups)
+                if (gr.levels > r) {
+                    hflag[gr.k0 + gr.levels - 1 - r] = 1;
+                    hlv.push_back(gr.k0 + gr.levels - 1 - r);
+                }
+            if (hlv.empty()) break;
+            HIP_TRY(hipMemcpyAsync(lvlflag, hflag.data(), hflag.size(), hipMemcpyHostToDevice, s->stream));
+            HIP_TRY(hipMemcpyAsync(lv, hlv.data(), hlv.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
+            k_cone_flags<<<grid_for((int64_t)DN), kBlock, 0, s->stream>>>(s->d_fstart, s->kmax, lvlflag, todo, flags,
+                                                                          (uint32_t)DN);
+            size_t tbr = tb;
+            HIP_TRY(hipcub::DeviceSelect::Flagged(tmp, tbr, hipcub::CountingInputIterator<uint32_t>(0u), flags, roots,
+                                                  ctl + 2, (int)DN, s->stream));
+            k_cone_bounds<<<grid_for((int64_t)hlv.size()), kBlock, 0, s->stream>>>(roots, ctl + 2, s->d_fstart, lv,
+                                                                                    (int)hlv.size(), bounds);
             HIP_TRY(hipGetLastError());
-            ndesc += nb;
+            hb.resize(2 * hlv.size());
+            HIP_TRY(hipMemcpyAsync(hb.data(), bounds, hb.size() * 4, hipMemcpyDeviceToHost, s->stream));
+            HIP_TRY(hipStreamSynchronize(s->stream));
+            rl = roots;
         }
-        gr.ndesc = ndesc - gr.desc0;
-        s->groups.push_back(gr);
+        size_t li = 0;
+        for (size_t gi = 0; gi < ng; gi++) {
+            const ssw_sweep::Group& gr = s->groups[gi];
+            if (gr.levels <= r) continue;
+            const int j = gr.k0 + gr.levels - 1 - r;
+            uint32_t first, n;
+            if (r == 0) {
+                first = fs[j];
+                n = fs[j + 1] - fs[j];
+            } else {
+                first = hb[2 * li];
+                n = hb[2 * li + 1] - hb[2 * li];
+                li++;
+            }
+            rstart[r][gi] = ndesc + (uint32_t)hmeta.size();
+            for (uint32_t b = 0; b < n; b += (uint32_t)T)
+                hmeta.push_back(make_uint4((uint32_t)gr.k0, (uint32_t)j, first + b, std::min<uint32_t>(T, n - b)));
+            rcount[r][gi] = ndesc + (uint32_t)hmeta.size() - rstart[r][gi];
+        }
+        if (hmeta.empty()) continue;
+        if ((size_t)ndesc + hmeta.size() > desc_cap) return fail(SSW_ERR_HIP, "grouped replay: descriptor bound exceeded");
+        TRY(dalloc(s, &meta, hmeta.size()));
+        HIP_TRY(hipMemcpyAsync(meta, hmeta.data(), hmeta.size() * sizeof(uint4), hipMemcpyHostToDevice, s->stream));
+        const unsigned nb = (unsigned)hmeta.size();
+        if (T == 64)
+            k_cone<R, 64><<<nb, 64, 0, s->stream>>>(hdr, planes, ovf, s->d_fstart, meta, rl, todo, s->d_glist,
+                                                    (uint32_t)list_cap, ctl, desc_rm + 2 * (size_t)ndesc, ctl + 1);
+        else
+            k_cone<R, 128><<<nb, 128, 0, s->stream>>>(hdr, planes, ovf, s->d_fstart, meta, rl, todo, s->d_glist,
+                                                      (uint32_t)list_cap, ctl, desc_rm + 2 * (size_t)ndesc, ctl + 1);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(s->stream));  // meta is reallocated by the next round
+        ndesc += nb;
     }
     uint32_t hctl[2] = {0, 0};
     HIP_TRY(hipMemcpyAsync(hctl, ctl, 8, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
+    dfree(s, meta);
     dfree(s, tmp);
-    dfree(s, ctl);
+    dfree(s, bounds);
+    dfree(s, lv);
     dfree(s, roots);
+    dfree(s, lvlflag);
+    dfree(s, flags);
+    dfree(s, ctl);
     dfree(s, todo);
     if (hctl[1] != 0) {  // a cone or the list did not fit: per-level replay
         s->groups.clear();
+        dfree(s, desc_rm);
         dfree(s, s->d_glist);
-        dfree(s, s->d_gdesc);
         s->group_fallback = true;
         return SSW_OK;
     }
+    // descriptors in group order (a group's workgroups contiguous)
+    std::vector<uint32_t> hperm;
+    hperm.reserve(ndesc);
+    for (size_t gi = 0; gi < ng; gi++) {
+        s->groups[gi].desc0 = (uint32_t)hperm.size();
+        for (int r = 0; r < m; r++)
+            for (uint32_t k = 0; k < rcount[r][gi]; k++) hperm.push_back(rstart[r][gi] + k);
+        s->groups[gi].ndesc = (uint32_t)hperm.size() - s->groups[gi].desc0;
+    }
+    TRY(dalloc(s, &s->d_gdesc, 2 * std::max<size_t>(1, ndesc)));
+    TRY(dalloc(s, &perm, std::max<size_t>(1, ndesc)));
+    if (ndesc) {
+        HIP_TRY(hipMemcpyAsync(perm, hperm.data(), hperm.size() * 4, hipMemcpyHostToDevice, s->stream));
+        k_permute_desc<<<grid_for(ndesc), kBlock, 0, s->stream>>>(desc_rm, perm, s->d_gdesc, ndesc);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    dfree(s, perm);
+    dfree(s, desc_rm);
     s->group_tasks = hctl[0];
     s->glist_len = std::max<uint32_t>(1u, hctl[0]);
     s->ndesc = ndesc;
