@@ -410,7 +410,7 @@ def main():
         print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build keeps the whole domain on a single MI355X; "
+        print("bench.py measures one GPU: the library keeps the whole domain on a single MI355X; "
               "--gpus above 1 is not supported", file=sys.stderr)
         sys.exit(2)
 

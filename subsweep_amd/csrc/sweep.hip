@@ -1742,17 +1742,17 @@ int build_groups_t(ssw_sweep* s) {
     const int T = s->group_block;
     const size_t desc_cap = DN / T + (size_t)s->kmax + 1;  // at most ceil(width / T) per level set
     const size_t list_cap = DN + DN / 4 + (1u << 20);
-    // groups, and per round r the level set k0 + levels - 1 - r of each
-    // group that has it (round 0: the last level sets, all tasks roots)
+    // groups, and per pass r the level set k0 + levels - 1 - r of each
+    // group that has it (pass 0: the last level sets, all tasks roots)
     s->groups.clear();
     for (int k0 = 0; k0 < s->kmax; k0 += m) s->groups.push_back({k0, std::min(s->kmax, k0 + m) - k0, 0u, 0u});
     const size_t ng = s->groups.size();
-    size_t round_roots = 1;
+    size_t pass_roots = 1;
     for (int r = 1; r < m; r++) {
         size_t w = 0;
         for (const ssw_sweep::Group& gr : s->groups)
             if (gr.levels > r) w += fs[gr.k0 + gr.levels - r] - fs[gr.k0 + gr.levels - 1 - r];
-        round_roots = std::max(round_roots, w);
+        pass_roots = std::max(pass_roots, w);
     }
     uint8_t *todo = nullptr, *flags = nullptr, *lvlflag = nullptr, *tmp = nullptr;
     uint32_t *roots = nullptr, *ctl = nullptr, *bounds = nullptr, *perm = nullptr;  // ctl: cursor, fail, selected
@@ -1768,7 +1768,7 @@ int build_groups_t(ssw_sweep* s) {
     if (m > 1) {
         TRY(dalloc(s, &flags, DN));
         TRY(dalloc(s, &lvlflag, (size_t)s->kmax));
-        TRY(dalloc(s, &roots, round_roots));
+        TRY(dalloc(s, &roots, pass_roots));
         TRY(dalloc(s, &lv, ng));
         TRY(dalloc(s, &bounds, 2 * ng));
         HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, tb, hipcub::CountingInputIterator<uint32_t>(0u), flags, roots,
@@ -1778,7 +1778,7 @@ int build_groups_t(ssw_sweep* s) {
     const uint4* hdr = s->d_hdr;
     const Entry<R>* planes = (const Entry<R>*)s->d_planes;
     const Entry<R>* ovf = (const Entry<R>*)s->d_ovf;
-    // blocks of group g in round r: [rstart[r][g], rstart[r][g] + rcount[r][g]) of the round-major descriptors
+    // blocks of group g in pass r: [rstart[r][g], rstart[r][g] + rcount[r][g]) of the pass-major descriptors
     std::vector<std::vector<uint32_t>> rstart(m, std::vector<uint32_t>(ng, 0u)), rcount(rstart);
     uint32_t ndesc = 0;
     std::vector<uint4> hmeta;
@@ -1789,7 +1789,7 @@ int build_groups_t(ssw_sweep* s) {
         hmeta.clear();
         hlv.clear();
         const uint32_t* rl = nullptr;
-        if (r > 0) {  // roots: the tasks of the round's level sets in no cone yet, in position order
+        if (r > 0) {  // roots: the tasks of the pass's level sets in no cone yet, in position order
             std::fill(hflag.begin(), hflag.end(), 0);
             for (const ssw_sweep::Group& gr : s->groups)
                 if (gr.levels > r) {
@@ -1843,7 +1843,7 @@ int build_groups_t(ssw_sweep* s) {
             k_cone<R, 128><<<nb, 128, 0, s->stream>>>(hdr, planes, ovf, s->d_fstart, meta, rl, todo, s->d_glist,
                                                       (uint32_t)list_cap, ctl, desc_rm + 2 * (size_t)ndesc, ctl + 1);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipStreamSynchronize(s->stream));  // meta is reallocated by the next round
+        HIP_TRY(hipStreamSynchronize(s->stream));  // meta is reallocated by the next pass
         ndesc += nb;
     }
     uint32_t hctl[2] = {0, 0};
