@@ -119,3 +119,23 @@ def test_cartesian_grid_semantics():
         assert np.all((kinds[:, 1] != 0) == (ix == n - 1))
         assert set(np.unique(g.kind[g.kind != 0])) == ({1} if periodic else {2})
         _mirror_ok(g)
+
+
+def test_source_assignment_is_nearest_cell_summed():
+    # set_source_terms_system (src/source_systems.rs:48-80): each source's rate
+    # goes to the nearest cell by plain Euclidean distance (no periodic
+    # wrapping), rates of sources in one cell add up; brute force on random
+    # points (no ties)
+    import numpy as np
+    from subsweep_amd import assign_sources
+    rng = np.random.default_rng(7)
+    pts = rng.uniform(0.0, 10.0, (500, 3))
+    pos = np.vstack([rng.uniform(-1.0, 11.0, (300, 3)), pts[:5] + 1e-9])  # some outside the box, some on cells
+    rates = rng.uniform(1e48, 1e50, pos.shape[0])
+    src, cells = assign_sources(pts, pos, rates)
+    d2 = ((pos[:, None, :] - pts[None, :, :]) ** 2).sum(axis=2)
+    want = d2.argmin(axis=1)
+    assert np.array_equal(np.asarray(cells), want)
+    ref = np.zeros(pts.shape[0])
+    np.add.at(ref, want, rates)
+    np.testing.assert_allclose(src, ref, rtol=1e-15)
