@@ -247,9 +247,9 @@ class SweepPlugin:
             return None
         self.time += self.sweep.run_sweeps()
         res = self.sweep.results()
-        if self.ionization_time is None:
-            self.ionization_time = np.zeros(res.ionized_hydrogen_fraction.shape)
-        newly = (res.ionized_hydrogen_fraction > 0.5) & (self.ionization_time == 0.0)
+        if self.ionization_time is None:  # IonizationTime::default() is +inf (src/components.rs:79-82)
+            self.ionization_time = np.full(res.ionized_hydrogen_fraction.shape, np.inf)
+        newly = (res.ionized_hydrogen_fraction > 0.5) & (self.ionization_time == np.inf)
         self.ionization_time[newly] = self.time
         self.last = res
         return res

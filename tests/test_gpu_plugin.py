@@ -1,0 +1,40 @@
+"""GPU: the plugin's step system (run_sweep_system, src/sweep/mod.rs:694-739):
+the first call only clears IsFirstTime; later calls equal a direct Sweep
+step by step; the ionisation time of a cell is the simulation time at the
+first step that leaves its ionised fraction above 0.5 and +inf (the
+component's default) until then."""
+
+import numpy as np
+import pytest
+
+from subsweep_amd import Sweep, SweepPlugin
+from subsweep_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("ionized_hydrogen_fraction", "temperature", "timestep", "photon_rate", "level", "change_timescale")
+
+
+def test_plugin_step_system(hip_available):
+    # a 6^3 box of 1e20 m with 1e-24 kg m^-3 gas: the ionised region grows
+    # from the source cell over the six steps
+    pb = W.cartesian_problem(6, 16, levels=2, periodic=False, source_cells=(0,), steps=6, max_timestep=3e12,
+                             density=1e-24, box=1e20, temperature=1e4)
+    plugin = SweepPlugin(Sweep(pb.params, pb.grid, *pb.state()))
+    direct = Sweep(pb.params, pb.grid, *pb.state())
+    assert plugin.run_sweep_system() is None  # the first call outputs the initial conditions
+    t = 0.0
+    first = np.full(pb.grid.num_cells, np.inf)
+    for _ in range(pb.steps):
+        got = plugin.run_sweep_system()
+        t += direct.run_sweeps()
+        want = direct.results()
+        for f in FIELDS:
+            assert np.array_equal(getattr(got, f), getattr(want, f)), f
+        first[(want.ionized_hydrogen_fraction > 0.5) & (first == np.inf)] = t
+        assert plugin.time == t
+        assert np.array_equal(plugin.ionization_time, first)
+    assert np.isfinite(first).any(), "no cell ionised: the test does not exercise the ionisation time"
+    assert np.isinf(first).any(), "every cell ionised: the default value is not exercised"
+    plugin.sweep.close()
+    direct.close()
