@@ -199,3 +199,28 @@ def test_precision_64_fp32_against_oracle(hip_available):
     _rel_ok(r32.ionized_hydrogen_fraction, ro.ionized_hydrogen_fraction, 1e-5, name="x_HII fp32")
     _rel_ok(r32.temperature, ro.temperature, 1e-5, name="T fp32")
     assert np.array_equal(r32.level, ro.level)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_ionisation_front_against_oracle(precision, hip_available):
+    """An ionisation front crossing a 6^3 box (1e20 m, 1e-24 kg m^-3, 16
+    directions, 2 timestep levels, 6 steps): FP64 transport within 1e-10 of
+    the oracle in every field; FP32 transport's ionised fraction and
+    temperature within 1e-5 relative, timestep levels equal."""
+    pb = W.cartesian_problem(6, 16, levels=2, periodic=False, source_cells=(0,), steps=6, max_timestep=3e12,
+                             density=1e-24, box=1e20, temperature=1e4)
+    o = OracleSweep(pb.params, pb.grid, *pb.state())
+    g = Sweep(pb.with_params(transport_precision=precision).params, pb.grid, *pb.state())
+    for _ in range(pb.steps):
+        assert g.run_sweeps() == o.run_sweeps()
+    ro = o.results()
+    assert (ro.ionized_hydrogen_fraction > 0.5).sum() > 10, "the front should have crossed several cells"
+    if precision == 64:
+        _compare_fields(g, o, 1e-10)
+    else:
+        rg = g.results()
+        _rel_ok(rg.ionized_hydrogen_fraction, ro.ionized_hydrogen_fraction, 1e-5, name="x_HII fp32")
+        _rel_ok(rg.temperature, ro.temperature, 1e-5, name="T fp32")
+        assert np.array_equal(rg.level, ro.level)
+    g.close()
+    o.close()
