@@ -240,7 +240,8 @@ int ssw_set_graph_replay(ssw_sweep* s, int32_t enabled);
 int ssw_set_narrow_runs(ssw_sweep* s, int32_t enabled);
 
 /* Compiled replay: groups of `levels` consecutive level sets (2 to 8, default
- * 4; 1 = one launch per level set) run as one launch each. A workgroup owns a
+ * 4; 8 where 8 consecutive level sets hold at most 2048 tasks each; 1 = one
+ * launch per level set) run as one launch each. A workgroup owns a
  * block of root tasks (ssw_set_group_block) of one level set of the group and
  * solves their upwind cone inside the group level set by level set with a
  * workgroup barrier in between; a task in several cones is solved by each of

@@ -87,6 +87,7 @@ constexpr int kGroupLevels = 4;            // grouped replay: level sets per gro
 constexpr int kConeCap = 3072;             // grouped replay build: cone members / pending entries per workgroup
 constexpr int kSortCap = 4096;             // grouped replay build: entries of one level set sorted in LDS
 constexpr int kMaxGroup = 8;               // grouped replay: level sets per group
+constexpr uint32_t kNarrowGroup = 2048;    // grouped replay: level sets this narrow go kMaxGroup to a group
 
 thread_local std::string g_err;
 // Handles alive in the process. Replay graphs are captured only while a single
@@ -1742,13 +1743,28 @@ int build_groups_t(ssw_sweep* s) {
     const int T = s->group_block;
     const size_t desc_cap = DN / T + (size_t)s->kmax + 1;  // at most ceil(width / T) per level set
     const size_t list_cap = DN + DN / 4 + (1u << 20);
-    // groups, and per pass r the level set k0 + levels - 1 - r of each
-    // group that has it (pass 0: the last level sets, all tasks roots)
+    // groups of m level sets, of kMaxGroup where that many consecutive level
+    // sets hold at most kNarrowGroup tasks each (the narrow ends of the
+    // graph, whose cones stay small); per pass r the level set
+    // k0 + levels - 1 - r of each group that has it (pass 0: the last level
+    // sets, all tasks roots)
     s->groups.clear();
-    for (int k0 = 0; k0 < s->kmax; k0 += m) s->groups.push_back({k0, std::min(s->kmax, k0 + m) - k0, 0u, 0u});
+    int mmax = 0;
+    for (int k0 = 0; k0 < s->kmax;) {
+        int mk = m;
+        if (m < kMaxGroup && k0 + kMaxGroup <= s->kmax) {
+            bool narrow = true;
+            for (int k = k0; narrow && k < k0 + kMaxGroup; k++) narrow = fs[k + 1] - fs[k] <= kNarrowGroup;
+            if (narrow) mk = kMaxGroup;
+        }
+        const int lv = std::min(s->kmax, k0 + mk) - k0;
+        s->groups.push_back({k0, lv, 0u, 0u});
+        mmax = std::max(mmax, lv);
+        k0 += mk;
+    }
     const size_t ng = s->groups.size();
     size_t pass_roots = 1;
-    for (int r = 1; r < m; r++) {
+    for (int r = 1; r < mmax; r++) {
         size_t w = 0;
         for (const ssw_sweep::Group& gr : s->groups)
             if (gr.levels > r) w += fs[gr.k0 + gr.levels - r] - fs[gr.k0 + gr.levels - 1 - r];
@@ -1765,7 +1781,7 @@ int build_groups_t(ssw_sweep* s) {
     HIP_TRY(hipMemsetAsync(todo, 1, DN, s->stream));
     HIP_TRY(hipMemsetAsync(ctl, 0, 12, s->stream));
     size_t tb = 0;
-    if (m > 1) {
+    if (mmax > 1) {
         TRY(dalloc(s, &flags, DN));
         TRY(dalloc(s, &lvlflag, (size_t)s->kmax));
         TRY(dalloc(s, &roots, pass_roots));
@@ -1779,13 +1795,13 @@ int build_groups_t(ssw_sweep* s) {
     const Entry<R>* planes = (const Entry<R>*)s->d_planes;
     const Entry<R>* ovf = (const Entry<R>*)s->d_ovf;
     // blocks of group g in pass r: [rstart[r][g], rstart[r][g] + rcount[r][g]) of the pass-major descriptors
-    std::vector<std::vector<uint32_t>> rstart(m, std::vector<uint32_t>(ng, 0u)), rcount(rstart);
+    std::vector<std::vector<uint32_t>> rstart(mmax, std::vector<uint32_t>(ng, 0u)), rcount(rstart);
     uint32_t ndesc = 0;
     std::vector<uint4> hmeta;
     std::vector<uint8_t> hflag(s->kmax);
     std::vector<int> hlv;
     std::vector<uint32_t> hb;
-    for (int r = 0; r < m; r++) {
+    for (int r = 0; r < mmax; r++) {
         hmeta.clear();
         hlv.clear();
         const uint32_t* rl = nullptr;
@@ -1870,7 +1886,7 @@ int build_groups_t(ssw_sweep* s) {
     hperm.reserve(ndesc);
     for (size_t gi = 0; gi < ng; gi++) {
         s->groups[gi].desc0 = (uint32_t)hperm.size();
-        for (int r = 0; r < m; r++)
+        for (int r = 0; r < mmax; r++)
             for (uint32_t k = 0; k < rcount[r][gi]; k++) hperm.push_back(rstart[r][gi] + k);
         s->groups[gi].ndesc = (uint32_t)hperm.size() - s->groups[gi].desc0;
     }
