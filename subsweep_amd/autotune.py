@@ -47,7 +47,7 @@ class TuneResult:
                 "group_levels": self.group_levels, "group_block": self.group_block}
 
 
-def candidates(blocks=(64, 128), groups=(4, 5), group_blocks=(64, 128)):
+def candidates(blocks=(64, 128), groups=(4,), group_blocks=(64, 128)):
     C = N.TRANSPORT_MODE_COMPILED
     out = [(C, b, True, True, 1, 0) for b in blocks]
     out.append((C, blocks[0], False, True, 1, 0))
