@@ -158,13 +158,27 @@ struct Geo {
     uint32_t* need;          // L: the active set of a sweep level changed
 };
 
+// Per-cell {transport source / D in the storage scale, attenuation} as the
+// transport reads them: in its storage precision (the FP32 transport rounds
+// both to float before use, so storing them rounded changes no result).
+template <typename R>
+struct SAOf {
+    using T = double2;
+};
+template <>
+struct SAOf<float> {
+    using T = float2;
+};
+template <typename R>
+using SA = typename SAOf<R>::T;
+
 template <typename R>
 struct Flux {
     R* X;                    // D x N, at xidx(d, c)
     const uint32_t* xpos;    // compiled replay: X of (c, d) at the task's schedule position xpos[d*N + c]
     R* I;                    // D x N
     R* P;                    // D x Nper
-    const double2* sa;       // N: {source / D in the storage scale, attenuation} (one load per task)
+    const SA<R>* sa;         // N: {source / D in the storage scale, attenuation} (one load per task)
     double thr;              // threshold in the storage scale
     double inv_lref;         // storage scale of rates
     double inv_aref;         // storage scale of areas
@@ -216,7 +230,7 @@ template <typename R>
 __device__ __forceinline__ CellIn<R> cell_inputs(const Geo& g, const Flux<R>& fl, uint32_t c, uint32_t d,
                                                  uint32_t ps) {
     CellIn<R> ci;
-    const double2 sa = fl.sa[c];
+    const SA<R> sa = fl.sa[c];
     ci.src = sa.x;
     ci.att = sa.y;
     // the periodic load uses slot 0 for cells without one (always a valid index)
@@ -1144,7 +1158,7 @@ struct ChemArgs {
     double* x;
     double* T;
     double* src;             // source per direction bin (SI)
-    double2* sa;             // .y: attenuation of the next sweep
+    void* sa;                // SA<R>[N], .y: attenuation of the next sweep
     double* species_ts;
     double* prev_rate;
     double* change_ts;
@@ -1210,7 +1224,7 @@ __global__ void k_chemistry(Geo g, Flux<R> fl, ChemArgs ca, int lvl) {
     ca.T[c] = cell.T;
     ca.species_ts[c] = used;
     ca.change_ts[c] = (rate_ts < used) ? rate_ts : used;  // Timescale::min
-    ca.sa[c].y = ssw::non_absorbed_fraction(cell.rho, cell.x, cell.len);
+    ((SA<R>*)ca.sa)[c].y = (R)ssw::non_absorbed_fraction(cell.rho, cell.x, cell.len);
 }
 
 // update_timestep_levels (src/sweep/mod.rs:576-589) for all cells. Marks the
@@ -1281,9 +1295,10 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
     if (i < n) p[i] = v;
 }
 
-__global__ void k_attenuation(const double* rho, const double* x, const double* size, double2* sa, int64_t n) {
+template <typename R>
+__global__ void k_attenuation(const double* rho, const double* x, const double* size, SA<R>* sa, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) sa[i].y = ssw::non_absorbed_fraction(rho[i], x[i], size[i]);
+    if (i < n) sa[i].y = (R)ssw::non_absorbed_fraction(rho[i], x[i], size[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1414,7 +1429,7 @@ struct ssw_sweep {
     std::vector<uint4*> d_runtab;
     // state
     double *d_rho = nullptr, *d_x = nullptr, *d_T = nullptr, *d_src = nullptr;
-    double2* d_sa = nullptr;                        // {transport source / D, attenuation}
+    void* d_sa = nullptr;                           // SA<R>: {transport source / D, attenuation}
     double *d_sts = nullptr, *d_prev = nullptr, *d_change = nullptr;
     unsigned long long* d_active = nullptr;
     uint8_t* d_level = nullptr;
@@ -1468,6 +1483,7 @@ int dalloc(ssw_sweep* s, T** p, size_t count) {
 
 size_t real_size(const ssw_sweep* s) { return s->fp32 ? 4 : 8; }
 size_t entry_size(const ssw_sweep* s) { return s->fp32 ? sizeof(Entry<float>) : sizeof(Entry<double>); }
+size_t sa_size(const ssw_sweep* s) { return s->fp32 ? sizeof(float2) : sizeof(double2); }
 
 Geo make_geo(ssw_sweep* s) {
     Geo g;
@@ -1499,7 +1515,7 @@ Flux<R> make_flux(ssw_sweep* s) {
     f.xpos = s->compiled ? s->d_xpos : nullptr;
     f.I = (R*)s->d_I;
     f.P = (R*)s->d_P;
-    f.sa = s->d_sa;
+    f.sa = (const SA<R>*)s->d_sa;
     f.inv_lref = 1.0 / s->lref;
     f.inv_aref = 1.0 / s->aref;
     f.thr = s->p.significant_rate_threshold * f.inv_lref;
@@ -1571,6 +1587,36 @@ void to_caller(const ssw_sweep* s, T* a, int64_t blocks) {
 int set_need_all(ssw_sweep* s, uint32_t v) {
     k_fill_u32<<<1, kBlock, 0, s->stream>>>(s->d_need, v, s->p.num_timestep_levels);
     HIP_TRY(hipGetLastError());
+    return SSW_OK;
+}
+
+// The per-cell {source / D, attenuation} pairs to and from the device, in
+// the transport's storage precision there (synchronous).
+int upload_sa(ssw_sweep* s, const std::vector<double2>& sa) {
+    const size_t n = sa.size();
+    if (s->fp32) {
+        std::vector<float2> f(n);
+        for (size_t c = 0; c < n; c++) f[c] = make_float2((float)sa[c].x, (float)sa[c].y);
+        HIP_TRY(hipMemcpyAsync(s->d_sa, f.data(), n * sizeof(float2), hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    } else {
+        HIP_TRY(hipMemcpyAsync(s->d_sa, sa.data(), n * sizeof(double2), hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
+    return SSW_OK;
+}
+
+int download_sa(ssw_sweep* s, std::vector<double2>& sa) {
+    const size_t n = sa.size();
+    if (s->fp32) {
+        std::vector<float2> f(n);
+        HIP_TRY(hipMemcpyAsync(f.data(), s->d_sa, n * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        for (size_t c = 0; c < n; c++) sa[c] = make_double2((double)f[c].x, (double)f[c].y);
+    } else {
+        HIP_TRY(hipMemcpyAsync(sa.data(), s->d_sa, n * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
     return SSW_OK;
 }
 
@@ -2748,7 +2794,7 @@ int ssw_upload_state(ssw_sweep* s, const double* density, const double* xhii, co
     TRY(dalloc(s, &s->d_x, n));
     TRY(dalloc(s, &s->d_T, n));
     TRY(dalloc(s, &s->d_src, n));
-    TRY(dalloc(s, &s->d_sa, n));
+    TRY(dalloc(s, (uint8_t**)&s->d_sa, n * sa_size(s)));
     TRY(dalloc(s, &s->d_sts, n));
     TRY(dalloc(s, &s->d_prev, n));
     TRY(dalloc(s, &s->d_change, n));
@@ -2763,8 +2809,7 @@ int ssw_upload_state(ssw_sweep* s, const double* density, const double* xhii, co
     // power of two lref, exact); the attenuation follows below
     std::vector<double2> sa(n);
     for (int64_t c = 0; c < n; c++) sa[c] = make_double2(s->fp32 ? sod[c] / s->lref : sod[c], 0.0);
-    HIP_TRY(hipMemcpyAsync(s->d_sa, sa.data(), n * sizeof(double2), hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
+    TRY(upload_sa(s, sa));
     HIP_TRY(hipMemsetAsync(s->d_sts, 0, n * 8, s->stream));
     HIP_TRY(hipMemsetAsync(s->d_prev, 0, n * 8, s->stream));
     HIP_TRY(hipMemsetAsync(s->d_change, 0, n * 8, s->stream));
@@ -2775,7 +2820,11 @@ int ssw_upload_state(ssw_sweep* s, const double* density, const double* xhii, co
     HIP_TRY(hipMemsetAsync(s->d_active, 0, 8, s->stream));
     std::vector<uint8_t> lev(n, (uint8_t)(L - 1));  // src/sweep/mod.rs:206
     HIP_TRY(hipMemcpyAsync(s->d_level, lev.data(), n, hipMemcpyHostToDevice, s->stream));
-    k_attenuation<<<grid_for(n), kBlock, 0, s->stream>>>(s->d_rho, s->d_x, s->d_size, s->d_sa, n);
+    if (s->fp32)
+        k_attenuation<float><<<grid_for(n), kBlock, 0, s->stream>>>(s->d_rho, s->d_x, s->d_size, (float2*)s->d_sa, n);
+    else
+        k_attenuation<double><<<grid_for(n), kBlock, 0, s->stream>>>(s->d_rho, s->d_x, s->d_size, (double2*)s->d_sa,
+                                                                     n);
     HIP_TRY(hipGetLastError());
     TRY(set_need_all(s, 1u));
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -2984,7 +3033,7 @@ int save_t(ssw_sweep* s, uint8_t* buf) {
     HIP_TRY(hipMemcpyAsync(cell + 1 * n, s->d_x, n * 8, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipMemcpyAsync(cell + 2 * n, s->d_T, n * 8, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipMemcpyAsync(cell + 3 * n, s->d_src, n * 8, hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipMemcpyAsync(sa.data(), s->d_sa, n * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+    TRY(download_sa(s, sa));
     HIP_TRY(hipMemcpyAsync(cell + 5 * n, s->d_sts, n * 8, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipMemcpyAsync(cell + 6 * n, s->d_prev, n * 8, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipMemcpyAsync(cell + 7 * n, s->d_change, n * 8, hipMemcpyDeviceToHost, s->stream));
@@ -3058,7 +3107,7 @@ int load_t(ssw_sweep* s, const uint8_t* buf) {
     HIP_TRY(hipMemcpyAsync(s->d_x, cell.data() + 1 * n, n * 8, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_T, cell.data() + 2 * n, n * 8, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_src, cell.data() + 3 * n, n * 8, hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipMemcpyAsync(s->d_sa, sa.data(), n * sizeof(double2), hipMemcpyHostToDevice, s->stream));
+    TRY(upload_sa(s, sa));
     HIP_TRY(hipMemcpyAsync(s->d_sts, cell.data() + 5 * n, n * 8, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_prev, cell.data() + 6 * n, n * 8, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_change, cell.data() + 7 * n, n * 8, hipMemcpyHostToDevice, s->stream));
