@@ -665,9 +665,14 @@ struct TaskS {
     R A;
 };
 
+// Overflow entries of a task read together with its upwind gathers (the
+// rest, rarer, one after the other).
+constexpr int kEarlyOvf = 2;
+
 template <typename R>
 struct TaskG {
     R x[kPlanes];
+    Entry<R> o[kEarlyOvf];  // first overflow entries (tasks with more than kPlanes upwind faces)
     CellIn<R> ci;
 };
 
@@ -682,8 +687,13 @@ __device__ __forceinline__ void load_task(TaskS<R>& t, const uint4* hdr, const R
 
 template <typename R>
 __device__ __forceinline__ void issue_task(TaskG<R>& G, const Geo& g, const Flux<R>& fl, const TaskS<R>& t,
-                                           uint32_t self) {
+                                           const Entry<R>* ovf, uint32_t self) {
     const uint32_t c = t.h.x & kCellMask, d = t.h.x >> kCellBits, ps = t.h.y, nup = t.h.w;
+    if (nup > (uint32_t)kPlanes) {  // indexes clamped to the task's own overflow entries
+        const uint32_t last = nup - kPlanes - 1u;
+#pragma unroll
+        for (int k = 0; k < kEarlyOvf; k++) G.o[k] = ovf[t.h.z + min((uint32_t)k, last)];
+    }
     G.ci = cell_inputs<R>(g, fl, c, d, ps);
 #pragma unroll
     for (int q = 0; q < kPlanes; q++) G.x[q] = fl.X[(uint32_t)q < nup ? t.e[q].nb : self];
@@ -696,9 +706,17 @@ __device__ __forceinline__ void finish_task_c(const Geo& g, const Flux<R>& fl, c
     R inc = 0;
 #pragma unroll
     for (int q = 0; q < kPlanes; q++) inc += ((uint32_t)q < nup) ? t.e[q].w * G.x[q] : (R)0;
-    for (uint32_t q = kPlanes; q < nup; q++) {
-        const Entry<R> o = ovf[t.h.z + q - kPlanes];
-        inc += o.w * fl.X[o.nb];
+    if (nup > (uint32_t)kPlanes) {  // the overflow entries, in entry order
+        R xo[kEarlyOvf];
+#pragma unroll
+        for (int k = 0; k < kEarlyOvf; k++) xo[k] = fl.X[G.o[k].nb];
+#pragma unroll
+        for (int k = 0; k < kEarlyOvf; k++)
+            if ((uint32_t)(kPlanes + k) < nup) inc += G.o[k].w * xo[k];
+        for (uint32_t q = kPlanes + kEarlyOvf; q < nup; q++) {
+            const Entry<R> o = ovf[t.h.z + q - kPlanes];
+            inc += o.w * fl.X[o.nb];
+        }
     }
     finish_task<R>(fl, self, (size_t)d * g.N + c, inc, G.ci, t.A);
 }
@@ -712,7 +730,7 @@ __device__ __forceinline__ void comp_task(const Geo& g, const Flux<R>& fl, const
     TaskS<R> t;
     TaskG<R> G;
     load_task<R>(t, hdr, Av, planes, W, p);
-    issue_task<R>(G, g, fl, t, b + p);
+    issue_task<R>(G, g, fl, t, ovf, b + p);
     finish_task_c<R>(g, fl, t, G, ovf, b + p);
 }
 
@@ -769,7 +787,7 @@ __global__ void __launch_bounds__(run_block<R>()) k_sweep_comp_run(Geo g, Flux<R
     uint4 rnn = run[min(2, last)];
     for (int j = 0; j < levels; j++) {
         TaskG<R> G;
-        issue_task<R>(G, g, fl, cur, r.x + p);
+        issue_task<R>(G, g, fl, cur, ovf, r.x + p);
         TaskS<R> nxt;
         load_task<R>(nxt, hdr + rn.x, Av + rn.x, planes + (size_t)kPlanes * rn.x, rn.y, pn);
         const uint32_t pnn = pos(rnn);
@@ -1050,7 +1068,7 @@ __global__ void __launch_bounds__(E) k_sweep_group(Geo g, Flux<R> fl, const uint
     for (int ph = 0; ph < kMaxGroup; ph++) {
         if (ph < phases) {
             TaskG<R> G;
-            issue_task<R>(G, g, fl, cur, P[ph]);
+            issue_task<R>(G, g, fl, cur, ovf, P[ph]);
             TaskS<R> nxt;
             const int q = ph + 1 < kMaxGroup ? ph + 1 : ph;
             load_task<R>(nxt, hdr + bb[q], Av + bb[q], planes + (size_t)kPlanes * bb[q], WW[q], P[q] - bb[q]);
