@@ -120,6 +120,18 @@ int ssw_download(ssw_sweep* s, double* ionized_fraction, double* temperature,
                  double* species_timestep, double* photon_rate, uint8_t* level,
                  double* change_timescale);
 
+/* The optional chemistry outputs of sweep_optional_output_system
+ * (src/sweep/chemistry_output.rs:25-67): per cell, the Solver of
+ * Sweep::get_solver (src/sweep/mod.rs:612-632; rate = sum over directions of
+ * incoming + source/D + periodic, current x and T, no floor) evaluated with
+ * a timestep of 1 yr: PhotoionizationRate (s^-1), HeatingRate (photoheating
+ * minus cooling, W m^-3), RecombinationRate (alpha_B n_e x, s^-1) and
+ * CollisionalIonizationRate (beta n_e (1 - x), s^-1). SI units, caller's
+ * cell order; any pointer may be NULL. The components they fill are
+ * src/components.rs:50-68. */
+int ssw_download_chemistry_outputs(ssw_sweep* s, double* photoionization_rate, double* heating_rate,
+                                   double* recombination_rate, double* collisional_ionization_rate);
+
 /* Waits until all work queued on the handle's stream has finished. */
 int ssw_synchronize(ssw_sweep* s);
 

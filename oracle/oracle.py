@@ -44,6 +44,8 @@ def lib():
             "orc_download_counts": (C.c_int, [P, C.c_int32, PU32]),
             "orc_download_levelsets": (C.c_int, [P, C.c_int32, PU32]),
             "orc_download_fluxes": (C.c_int, [P, PD, PD, PD]),
+            "orc_download_chemistry_outputs": (C.c_int, [P, PD, PD, PD, PD]),
+            "orc_chemistry_outputs_cell": (None, [C.c_double] * 7 + [PD]),
             "orc_updates_last_step": (C.c_int64, [P]),
             "orc_chemistry_failures": (C.c_int64, [P]),
             "orc_num_directions": (C.c_int, [P]),
@@ -90,6 +92,17 @@ def solver_rates(temperature, xhii=0.0, density=0.0, scale_factor=1.0) -> dict:
     out = np.empty(16)
     lib().orc_solver_rates(temperature, xhii, density, scale_factor, _p(out, C.c_double))
     return dict(zip(RATE_NAMES, out))
+
+
+CHEMISTRY_OUTPUT_NAMES = ["photoionization_rate", "heating_rate", "recombination_rate", "collisional_ionization_rate"]
+
+
+def chemistry_outputs_cell(x, T, density, volume, length, rate, scale_factor=1.0) -> dict:
+    """The optional chemistry outputs of one cell (src/sweep/chemistry_output.rs:25-56,
+    solver without floor, timestep 1 yr)."""
+    out = np.empty(4)
+    lib().orc_chemistry_outputs_cell(x, T, density, volume, length, rate, scale_factor, _p(out, C.c_double))
+    return dict(zip(CHEMISTRY_OUTPUT_NAMES, out))
 
 
 def perform_timestep(x, T, density, volume, length, rate, timestep, safety=0.1, scale_factor=1.0, floor=False):
@@ -162,6 +175,11 @@ class OracleSweep:
         _check(lib().orc_download_fluxes(self._h, _p(i, C.c_double), _p(o, C.c_double), _p(p, C.c_double)))
         s = (self.num_directions, self.num_cells)
         return i.reshape(s), o.reshape(s), p.reshape(s)
+
+    def chemistry_outputs(self) -> dict:
+        out = {k: np.empty(self.num_cells) for k in CHEMISTRY_OUTPUT_NAMES}
+        _check(lib().orc_download_chemistry_outputs(self._h, *(_p(v, C.c_double) for v in out.values())))
+        return out
 
     def directions(self):
         out = np.empty(3 * self.num_directions)

@@ -751,6 +751,48 @@ int orc_download_fluxes(void* h, double* incoming, double* outgoing, double* per
     return SSW_OK;
 }
 
+// The optional chemistry outputs (src/sweep/chemistry_output.rs:25-56) of one
+// cell from the Solver of Sweep::get_solver (src/sweep/mod.rs:612-632: no
+// floor), timestep 1 yr (chemistry_output.rs:15-19; src/units/mod.rs:48).
+// out[0..3]: photoionization, heating (photoheating - cooling), recombination
+// (alpha_B n_e x), collisional ionization (beta n_e (1 - x)).
+void orc_chemistry_outputs_cell(double xhii, double temperature, double density, double volume, double length,
+                                double rate, double scale_factor, double* out) {
+    Solver s{};
+    s.ionized_hydrogen_fraction = xhii;
+    s.temperature = temperature;
+    s.density = density;
+    s.volume = volume;
+    s.length = length;
+    s.rate = rate;
+    s.scale_factor = scale_factor;
+    s.has_floor = false;
+    const double timestep = 3.15576e7;
+    out[0] = s.photoionization_rate(timestep);
+    out[1] = s.photoheating_rate(timestep) - s.cooling_rate();
+    out[2] = s.case_b_recombination_rate() * s.electron_number_density() * s.ionized_hydrogen_fraction;
+    out[3] = s.collisional_ionization_rate() * s.electron_number_density() * (1.0 - s.ionized_hydrogen_fraction);
+}
+
+// The same for every cell of the oracle's current state, rate = sum over
+// directions of Site::get_rate (src/sweep/site.rs:53-56) in direction order.
+int orc_download_chemistry_outputs(void* h, double* photoionization, double* heating, double* recombination,
+                                   double* collisional_ionization) {
+    Oracle* o = static_cast<Oracle*>(h);
+    if (!o || !o->have_state) return fail(SSW_ERR_INVALID, "state first");
+    double* outs[4] = {photoionization, heating, recombination, collisional_ionization};
+    for (int64_t c = 0; c < o->N; c++) {
+        double rate = 0.0;
+        for (int d = 0; d < o->D; d++) rate += (o->I[c * o->D + d] + o->source_per_dir(c)) + o->P[c * o->D + d];
+        double v[4];
+        orc_chemistry_outputs_cell(o->x[c], o->T[c], o->density[c], o->volume[c], o->size[c], rate,
+                                   o->params.scale_factor, v);
+        for (int k = 0; k < 4; k++)
+            if (outs[k]) outs[k][c] = v[k];
+    }
+    return SSW_OK;
+}
+
 int64_t orc_updates_last_step(void* h) { return static_cast<Oracle*>(h)->updates_last_step; }
 int64_t orc_chemistry_failures(void* h) { return static_cast<Oracle*>(h)->chemistry_failures; }
 int orc_num_directions(void* h) { return static_cast<Oracle*>(h)->D; }

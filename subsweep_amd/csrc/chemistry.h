@@ -179,6 +179,27 @@ SSW_HD double delta_ionized(const Cell& c, double dt) {  // :334-354 (Rosdahl+15
     return dt * (cc - x * (cc + dd)) / (1.0 - j * dt);
 }
 
+// --- optional outputs of the solver (src/sweep/chemistry_output.rs:25-56) -------
+// evaluated with the fixed timestep of :15-19 (1 yr = 3.15576e7 s,
+// src/units/mod.rs:48) on the state after the step, without a floor
+constexpr double kYear = 3.15576e7;
+SSW_HD double photoionization_rate(const Cell& c, double dt) {  // hydrogen_only/mod.rs:327-332
+    return newly_ionized(c, dt) / (n_hi(c) * c.vol) / dt;
+}
+SSW_HD double photoheating_rate(const Cell& c, double dt) {  // :321-325
+    return newly_ionized(c, dt) / c.vol * (kPhotonEnergy - kRydberg) / dt;
+}
+// which: 0 PhotoionizationRate, 1 HeatingRate (photoheating - cooling),
+// 2 RecombinationRate (alpha_B n_e x), 3 CollisionalIonizationRate (beta n_e (1 - x))
+SSW_HD double chemistry_output(const Cell& c, int which) {
+    switch (which) {
+        case 0: return photoionization_rate(c, kYear);
+        case 1: return photoheating_rate(c, kYear) - cooling(c);
+        case 2: return alpha_b(c.T) * n_hii(c) * c.x;
+        default: return beta_ci(c.T) * n_hii(c) * (1.0 - c.x);
+    }
+}
+
 struct Floor {
     bool on;
     double T, x;

@@ -1245,6 +1245,34 @@ __global__ void k_chemistry(Geo g, Flux<R> fl, ChemArgs ca, int lvl) {
     ((SA<R>*)ca.sa)[c].y = (R)ssw::non_absorbed_fraction(cell.rho, cell.x, cell.len);
 }
 
+// The optional chemistry outputs (src/sweep/chemistry_output.rs:25-67) of
+// every cell: Sweep::get_solver (src/sweep/mod.rs:612-632) with the rate
+// summed over directions in direction order as in update_chemistry, the
+// current x and T, and no floor; output `which` of ssw::chemistry_output.
+template <typename R>
+__global__ void k_chem_outputs(Geo g, Flux<R> fl, ChemArgs ca, int which, double* out) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    const uint32_t ps = g.pslot[c];
+    const bool has_p = ps != kNone && fl.P;
+    const double src = ca.src[c];
+    double rate = 0.0;
+    for (int d = 0; d < g.D; d++) {
+        const double inc = (double)fl.I[(size_t)d * g.N + c] * ca.lref;
+        const double per = has_p ? (double)fl.P[(size_t)d * g.Nper + ps] * ca.lref : 0.0;
+        rate += (inc + src) + per;  // Site::get_rate (src/sweep/site.rs:53-56)
+    }
+    ssw::Cell cell;
+    cell.x = ca.x[c];
+    cell.T = ca.T[c];
+    cell.rho = ca.rho[c];
+    cell.vol = ca.vol[c];
+    cell.len = ca.size[c];
+    cell.rate = rate;
+    cell.a = ca.scale_factor;
+    out[c] = ssw::chemistry_output(cell, which);
+}
+
 // update_timestep_levels (src/sweep/mod.rs:576-589) for all cells. Marks the
 // sweep levels whose active set changed.
 __global__ void k_levels(Geo g, const double* change_ts, uint8_t* level, double max_ts, double safety,
@@ -2280,10 +2308,7 @@ void capture(ssw_sweep* s, const Geo& g, int lvl) {
     s->graphs[lvl] = ge;
 }
 
-template <typename R>
-int run_step(ssw_sweep* s, double* elapsed) {
-    Geo g = make_geo(s);
-    Flux<R> fl = make_flux<R>(s);
+ChemArgs chem_args(ssw_sweep* s) {
     ChemArgs ca;
     ca.rho = s->d_rho;
     ca.x = s->d_x;
@@ -2303,6 +2328,14 @@ int run_step(ssw_sweep* s, double* elapsed) {
     ca.lref = s->lref;
     ca.failures = s->d_fail;
     ca.active = s->d_active;
+    return ca;
+}
+
+template <typename R>
+int run_step(ssw_sweep* s, double* elapsed) {
+    Geo g = make_geo(s);
+    Flux<R> fl = make_flux<R>(s);
+    const ChemArgs ca = chem_args(s);
     const std::vector<int> order = s->ts.sweep_order();
     const int L = s->p.num_timestep_levels;
     k_fill_u32<<<1, 2, 0, s->stream>>>((uint32_t*)s->d_active, 0u, 2);
@@ -2894,6 +2927,31 @@ int ssw_download(ssw_sweep* s, double* x, double* T, double* sts, double* photon
     if (change) to_caller(s, change, 1);
     if (level) to_caller(s, level, 1);
     if (photon) to_caller(s, photon, 1);
+    return SSW_OK;
+}
+
+int ssw_download_chemistry_outputs(ssw_sweep* s, double* photoionization, double* heating, double* recombination,
+                                   double* collisional_ionization) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    if (!s->have_state) return fail(SSW_ERR_INVALID, "ssw_upload_state must be called first");
+    const int64_t n = s->N;
+    double* outs[4] = {photoionization, heating, recombination, collisional_ionization};
+    double* d_out = nullptr;
+    TRY(dalloc(s, &d_out, n));
+    const Geo g = make_geo(s);
+    const ChemArgs ca = chem_args(s);
+    for (int k = 0; k < 4; k++) {
+        if (!outs[k]) continue;
+        if (s->fp32)
+            k_chem_outputs<float><<<grid_for(n), kBlock, 0, s->stream>>>(g, make_flux<float>(s), ca, k, d_out);
+        else
+            k_chem_outputs<double><<<grid_for(n), kBlock, 0, s->stream>>>(g, make_flux<double>(s), ca, k, d_out);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(outs[k], d_out, n * 8, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        to_caller(s, outs[k], 1);
+    }
+    dfree(s, d_out);
     return SSW_OK;
 }
 

@@ -146,6 +146,17 @@ class Sweep:
                                        N.ptr(ch, C.c_double)))
         return SweepResults(x, T, ts, pr if photon_rate else None, lev, ch)
 
+    def chemistry_outputs(self) -> dict:
+        """The optional outputs of sweep_optional_output_system
+        (src/sweep/chemistry_output.rs:25-67) for every cell: photoionization,
+        heating, recombination and collisional ionization rates (SI), from the
+        solver of Sweep::get_solver (src/sweep/mod.rs:612-632) with dt = 1 yr."""
+        n = self.num_cells
+        out = {k: np.empty(n) for k in ("photoionization_rate", "heating_rate", "recombination_rate",
+                                         "collisional_ionization_rate")}
+        N.check(self._lib.ssw_download_chemistry_outputs(self._h, *(N.ptr(v, C.c_double) for v in out.values())))
+        return out
+
     def counts(self, sweep_level: int) -> np.ndarray:
         out = np.empty(self.num_directions * self.num_cells, np.uint32)
         N.check(self._lib.ssw_download_counts(self._h, sweep_level, N.ptr(out, C.c_uint32)))
@@ -230,16 +241,32 @@ class Sweep:
         self.close()
 
 
+CHEMISTRY_OUTPUTS = ("photoionization_rate", "heating_rate", "recombination_rate", "collisional_ionization_rate")
+
+
 @dataclass
 class SweepPlugin:
     """The plugin's step system over a `Sweep`: run_sweep_system
-    (src/sweep/mod.rs:694-739). The first call only marks IsFirstTime."""
+    (src/sweep/mod.rs:694-739). The first call only marks IsFirstTime.
+
+    `optional_outputs` names the optional chemistry components
+    (src/components.rs:50-68) to evaluate after each step, as the
+    sweep_optional_output_system instances that the plugin registers for
+    the desired output fields (src/sweep/mod.rs:162-165, 770-781); their
+    values land in `chemistry`."""
 
     sweep: Sweep
     time: float = 0.0
     is_first_time: bool = True
     ionization_time: np.ndarray | None = None
     last: SweepResults | None = field(default=None, repr=False)
+    optional_outputs: tuple = ()
+    chemistry: dict = field(default_factory=dict, repr=False)
+
+    def __post_init__(self):
+        unknown = [k for k in self.optional_outputs if k not in CHEMISTRY_OUTPUTS]
+        if unknown:
+            raise ValueError(f"unknown optional chemistry outputs {unknown}; known: {CHEMISTRY_OUTPUTS}")
 
     def run_sweep_system(self) -> SweepResults | None:
         if self.is_first_time:
@@ -252,4 +279,7 @@ class SweepPlugin:
         newly = (res.ionized_hydrogen_fraction > 0.5) & (self.ionization_time == np.inf)
         self.ionization_time[newly] = self.time
         self.last = res
+        if self.optional_outputs:
+            out = self.sweep.chemistry_outputs()
+            self.chemistry = {k: out[k] for k in self.optional_outputs}
         return res

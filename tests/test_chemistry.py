@@ -7,7 +7,12 @@ is pinned by:
   * the collisional equilibrium x* = beta / (alpha + beta), the fixed point of
     the implicit ionised-fraction update (mod.rs:334-354),
   * the reference's no-panic extreme states (mod.rs:948-984) and the clamp
-    bounds (mod.rs:356-369).
+    bounds (mod.rs:356-369),
+  * for the optional chemistry outputs (src/sweep/chemistry_output.rs:25-56):
+    recombinations balance collisional ionisations at x*, heating is minus
+    the cooling without photons and the photoheating per photoionisation is
+    E_photon - E_Rydberg, every photon is absorbed in the optically thick
+    limit and the rate stops depending on the density in the thin one.
 The library's single-cell chemistry (the code the device kernel runs,
 compiled for the host) must then agree with the oracle bit for bit on random
 states: both are transcriptions of the same expressions with the same libm.
@@ -127,3 +132,52 @@ def test_library_extreme_states():
                         e["length_kpc"] * KPC, e["rate_per_s"], e["scale_factor"], False, MYR, 0.1)
         assert math.isfinite(out["x"]) and math.isfinite(out["T"])
         assert 1e-10 <= out["x"] <= 1 - 1e-10
+
+
+PROTON_MASS = 1.67262192369e-27
+EV = 1.602176634e-19
+
+
+def test_chemistry_outputs_pinned():
+    T = 2e4
+    rho = 1e-24 * GRAMS_PER_CM3
+    size = 0.5 * KPC
+    vol = 4.0 / 3.0 * math.pi * size ** 3
+    r = O.solver_rates(T)
+    beta, alpha = r["collisional_ionization_rate"], r["case_b_recombination_rate"]
+    x_star = beta / (alpha + beta)
+    # no photons, collisional equilibrium: recombinations balance collisional
+    # ionisations, nothing is photoionised and the heating is minus the cooling
+    out = O.chemistry_outputs_cell(x_star, T, rho, vol, size, 0.0)
+    assert out["recombination_rate"] > 0
+    assert out["recombination_rate"] == pytest.approx(out["collisional_ionization_rate"], rel=1e-12, abs=0)
+    assert out["photoionization_rate"] == 0.0
+    assert out["heating_rate"] == pytest.approx(-O.solver_rates(T, x_star, rho)["cooling_rate"], rel=1e-12, abs=0)
+    # optically thick (tau ~ 3e3 here): every photon of the rate R ionises one
+    # of the n_HI V neutral atoms, Gamma n_HI V = R; each photoionisation
+    # heats by E_photon - E_Rydberg = 18.0284 - 13.6569 eV
+    x, R = 0.3, 1e48
+    out = O.chemistry_outputs_cell(x, T, rho, vol, size, R)
+    n_hi = rho / PROTON_MASS * (1 - x)
+    assert out["photoionization_rate"] * n_hi * vol == pytest.approx(R, rel=1e-12, abs=0)
+    cool = O.solver_rates(T, x, rho)["cooling_rate"]
+    per_ionisation = (out["heating_rate"] + cool) / (out["photoionization_rate"] * n_hi)
+    assert per_ionisation == pytest.approx((18.028356312818811 - 13.65693) * EV, rel=1e-9, abs=0)
+    # optically thin (tau ~ 6e-7, where 1 - exp(-tau) still holds ~10 digits):
+    # Gamma = R sigma L / V does not depend on the density (to ~tau / 2);
+    # sigma is the number-weighted cross-section, ~2.96e-18 cm^2
+    thin = [O.chemistry_outputs_cell(x, T, d, vol, size, R)["photoionization_rate"] for d in (3e-31, 6e-31)]
+    assert thin[0] == pytest.approx(thin[1], rel=1e-6, abs=0)
+    sigma = thin[0] * vol / (R * size)
+    assert sigma == pytest.approx(2.958e-18 * 1e-4, rel=1e-3, abs=0)
+
+
+def test_a_dropped_factor_in_the_outputs_fails_the_pins():
+    # the balance above tells apart the recombination rate with and without its x
+    T = 2e4
+    rho = 1e-24 * GRAMS_PER_CM3
+    r = O.solver_rates(T)
+    x_star = r["collisional_ionization_rate"] / (r["collisional_ionization_rate"] + r["case_b_recombination_rate"])
+    out = O.chemistry_outputs_cell(x_star, T, rho, 1.0, 1.0, 0.0)
+    planted = out["recombination_rate"] / x_star  # alpha n_e without x
+    assert planted != pytest.approx(out["collisional_ionization_rate"], rel=1e-3, abs=0)

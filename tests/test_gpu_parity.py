@@ -224,3 +224,24 @@ def test_ionisation_front_against_oracle(precision, hip_available):
         assert np.array_equal(rg.level, ro.level)
     g.close()
     o.close()
+
+
+def test_chemistry_outputs_against_oracle(hip_available):
+    """The optional chemistry outputs (src/sweep/chemistry_output.rs:25-67)
+    after the ionisation front of the test above (FP64 transport): every
+    cell's photoionisation, heating, recombination and collisional
+    ionisation rate within 1e-10 of the oracle's (the heating, a difference
+    of photoheating and cooling, with 1e-10 of its largest magnitude as the
+    absolute floor)."""
+    pb = W.cartesian_problem(6, 16, levels=2, periodic=False, source_cells=(0,), steps=6, max_timestep=3e12,
+                             density=1e-24, box=1e20, temperature=1e4)
+    g, o = _pair(pb)
+    for _ in range(pb.steps):
+        assert g.run_sweeps() == o.run_sweeps()
+    og, oo = g.chemistry_outputs(), o.chemistry_outputs()
+    assert (oo["photoionization_rate"] > 0).sum() > 10, "the front should photoionise several cells"
+    for k in ("photoionization_rate", "recombination_rate", "collisional_ionization_rate"):
+        _rel_ok(og[k], oo[k], 1e-10, name=k)
+    _rel_ok(og["heating_rate"], oo["heating_rate"], 1e-10, atol_frac=1e-10, name="heating_rate")
+    g.close()
+    o.close()

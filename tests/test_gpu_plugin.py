@@ -38,3 +38,26 @@ def test_plugin_step_system(hip_available):
     assert np.isinf(first).any(), "every cell ionised: the default value is not exercised"
     plugin.sweep.close()
     direct.close()
+
+
+def test_plugin_optional_chemistry_outputs(hip_available):
+    # the optional outputs named in optional_outputs are evaluated after each
+    # step, as the plugin's sweep_optional_output_system instances
+    pb = W.cartesian_problem(6, 16, levels=1, periodic=False, source_cells=(0,), steps=2, max_timestep=3e12,
+                             density=1e-24, box=1e20, temperature=1e4)
+    names = ("heating_rate", "photoionization_rate")
+    plugin = SweepPlugin(Sweep(pb.params, pb.grid, *pb.state()), optional_outputs=names)
+    direct = Sweep(pb.params, pb.grid, *pb.state())
+    plugin.run_sweep_system()
+    for _ in range(pb.steps):
+        plugin.run_sweep_system()
+        direct.run_sweeps()
+        want = direct.chemistry_outputs()
+        assert tuple(plugin.chemistry) == names
+        for k in names:
+            assert np.array_equal(plugin.chemistry[k], want[k]), k
+    assert (plugin.chemistry["photoionization_rate"] > 0).any()
+    with pytest.raises(ValueError):
+        SweepPlugin(direct, optional_outputs=("cooling",))
+    plugin.sweep.close()
+    direct.close()
