@@ -61,3 +61,33 @@ def test_plugin_optional_chemistry_outputs(hip_available):
         SweepPlugin(direct, optional_outputs=("cooling",))
     plugin.sweep.close()
     direct.close()
+
+
+def test_plugin_time_series_against_oracle(tmp_path, hip_available):
+    # the time series written after each step (src/sweep/time_series.rs:61-188)
+    # equal, within 1e-10, the same series computed from the oracle's state
+    from oracle.oracle import OracleSweep
+    from subsweep_amd import time_series as TS
+    pb = W.cartesian_problem(6, 16, levels=2, periodic=False, source_cells=(0,), steps=4, max_timestep=3e12,
+                             density=1e-24, box=1e20, temperature=1e4)
+    density = pb.state()[0]
+    volume = pb.grid.arrays()[5]
+    plugin = SweepPlugin(Sweep(pb.params, pb.grid, *pb.state()))
+    out = TS.TimeSeriesOutput(str(tmp_path), volume, density, pb.params.max_timestep, pb.params.num_timestep_levels)
+    o = OracleSweep(pb.params, pb.grid, *pb.state())
+    plugin.run_sweep_system()
+    assert out.write_plugin_step(plugin) is None  # the skipped first call writes nothing
+    for _ in range(pb.steps):
+        plugin.run_sweep_system()
+        got = out.write_plugin_step(plugin)
+        o.run_sweeps()
+        ro = o.results()
+        want = TS.compute(ro.ionized_hydrogen_fraction, ro.temperature, o.chemistry_outputs()["photoionization_rate"],
+                          ro.level, volume, density, pb.params.max_timestep, pb.params.num_timestep_levels)
+        for k in TS.AVERAGES:
+            assert got[k] == pytest.approx(want[k], rel=1e-10, abs=0), k
+        assert got[TS.LEVELS] == want[TS.LEVELS]
+    assert got["photoionization_rate_volume_average"] > 0
+    assert len(TS.read(str(tmp_path), TS.LEVELS)) == pb.steps
+    plugin.sweep.close()
+    o.close()
