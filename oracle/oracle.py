@@ -1,7 +1,7 @@
 """ctypes wrapper of the oracle (liboracle_sweep.so): the CPU restatement of
 the reference's sweep + chemistry. TEST INFRASTRUCTURE ONLY - imported by
-tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker,
-never by the product.
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg to compare
+results against, never by the product.
 
 `OracleSweep` has the same methods as subsweep_amd.Sweep so that a test can
 drive both with the same calls."""

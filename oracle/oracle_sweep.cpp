@@ -1,7 +1,8 @@
 // Oracle: a single-threaded CPU restatement of subsweep's sweep radiative
-// transfer and hydrogen-only chemistry, used ONLY as test infrastructure (the
-// checker in tests/, __graft_entry__.smoke() and the cpu_baseline leg of
-// bench.py). Nothing in the product links or calls it.
+// transfer and hydrogen-only chemistry, used ONLY as test infrastructure (what
+// the results are compared with in tests/ and __graft_entry__.smoke(), and the
+// CPU figure of bench.py's cpu_baseline leg). Nothing in the product links or
+// calls it.
 //
 // It follows the reference's Rust code operation by operation:
 //   sweep driver      src/sweep/mod.rs:258-289, 346-398, 412-513, 549-589
