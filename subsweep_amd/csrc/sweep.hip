@@ -88,6 +88,7 @@ constexpr int kConeCap = 3072;             // grouped replay build: cone members
 constexpr int kSortCap = 4096;             // grouped replay build: entries of one level set sorted in LDS
 constexpr int kMaxGroup = 8;               // grouped replay: level sets per group
 constexpr uint32_t kNarrowGroup = 2048;    // grouped replay: level sets this narrow go kMaxGroup to a group
+constexpr uint32_t kXcds = 8;              // MI355X: accelerator dies, each with its own L2
 
 thread_local std::string g_err;
 // Handles alive in the process. Replay graphs are captured only while a single
@@ -191,6 +192,29 @@ template <typename R>
 __device__ __forceinline__ size_t xidx(const Flux<R>& fl, int64_t N, uint32_t d, uint32_t c) {
     const size_t o = (size_t)d * N + c;
     return fl.xpos ? (size_t)fl.xpos[o] : o;
+}
+
+// Descriptor ranges of a grouped launch, one per build pass (the group's
+// last level set's roots first): workgroups [off[r], off[r+1]) solve pass r's
+// descriptors. n = 0: no ranges (the workgroup order is the descriptor order).
+struct PassTab {
+    uint32_t off[kMaxGroup + 1];
+    int n;
+};
+
+// The descriptor of workgroup b: the hardware deals workgroups to the dies
+// round-robin (b mod kXcds); inside each pass range, die x is given a
+// contiguous run of the range's descriptors ([x*q + min(x, r), ...) of the
+// range's c = 8q + r, taken in order), so that neighbouring cones, which read
+// the same schedule and X lines, share one die's L2, while every die still
+// gets an eighth of every pass.
+__device__ __forceinline__ uint32_t pass_item(const PassTab& pt, uint32_t b) {
+    if (pt.n == 0) return b;
+    int r = 0;
+    while (r + 1 < pt.n && b >= pt.off[r + 1]) r++;
+    const uint32_t o = pt.off[r], c = pt.off[r + 1] - o, i = b - o;
+    const uint32_t x = i % kXcds, q = c / kXcds, rem = c % kXcds;
+    return o + x * q + min(x, rem) + i / kXcds;
 }
 
 __device__ __forceinline__ uint4 task_record(uint32_t c, uint32_t f0, uint32_t f1, uint32_t d, uint32_t ps) {
@@ -1048,8 +1072,9 @@ __global__ void __launch_bounds__(E) k_sweep_group(Geo g, Flux<R> fl, const uint
                                                    const Entry<R>* __restrict__ ovf,
                                                    const uint32_t* __restrict__ fstart, int k0, int phases,
                                                    const uint4* __restrict__ desc, const uint32_t* __restrict__ list,
-                                                   uint32_t list_len) {
-    const uint4 d0 = desc[2 * blockIdx.x], d1 = desc[2 * blockIdx.x + 1];
+                                                   uint32_t list_len, PassTab pt) {
+    const uint32_t w = pass_item(pt, blockIdx.x);
+    const uint4 d0 = desc[2 * w], d1 = desc[2 * w + 1];
     const uint32_t tid = threadIdx.x;
     uint32_t n[kMaxGroup], off[kMaxGroup], bb[kMaxGroup], WW[kMaxGroup], P[kMaxGroup];
 #pragma unroll
@@ -1400,6 +1425,7 @@ struct ssw_sweep {
     struct Group {
         int k0, levels;
         uint32_t desc0, ndesc;
+        PassTab passes;  // descriptor ranges of the build passes, relative to desc0
     };
     std::vector<Group> groups;
     uint4* d_gdesc = nullptr;
@@ -1850,7 +1876,7 @@ int build_groups_t(ssw_sweep* s) {
             if (narrow) mk = kMaxGroup;
         }
         const int lv = std::min(s->kmax, k0 + mk) - k0;
-        s->groups.push_back({k0, lv, 0u, 0u});
+        s->groups.push_back({k0, lv, 0u, 0u, PassTab{}});
         mmax = std::max(mmax, lv);
         k0 += mk;
     }
@@ -1977,10 +2003,16 @@ int build_groups_t(ssw_sweep* s) {
     std::vector<uint32_t> hperm;
     hperm.reserve(ndesc);
     for (size_t gi = 0; gi < ng; gi++) {
-        s->groups[gi].desc0 = (uint32_t)hperm.size();
-        for (int r = 0; r < mmax; r++)
+        ssw_sweep::Group& gr = s->groups[gi];
+        gr.desc0 = (uint32_t)hperm.size();
+        gr.passes = PassTab{};
+        for (int r = 0; r < mmax; r++) {
+            if (rcount[r][gi] == 0) continue;
+            gr.passes.off[gr.passes.n++] = (uint32_t)hperm.size() - gr.desc0;
             for (uint32_t k = 0; k < rcount[r][gi]; k++) hperm.push_back(rstart[r][gi] + k);
-        s->groups[gi].ndesc = (uint32_t)hperm.size() - s->groups[gi].desc0;
+        }
+        gr.ndesc = (uint32_t)hperm.size() - gr.desc0;
+        gr.passes.off[gr.passes.n] = gr.ndesc;
     }
     TRY(dalloc(s, &s->d_gdesc, 2 * std::max<size_t>(1, ndesc)));
     TRY(dalloc(s, &perm, std::max<size_t>(1, ndesc)));
@@ -2226,14 +2258,17 @@ void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<
             const uint32_t* gl = all ? s->d_glist : s->d_glist_l[lvl];
             const uint32_t nblk = all ? gr.ndesc : s->gblocks[lvl][it.tab];  // restricted: workgroups with active tasks
             const Entry<R>* pl = (const Entry<R>*)s->d_planes;
+            // full sweeps: die-aware order inside each pass; restricted sweeps
+            // launch their filtered descriptors in order
+            const PassTab pt = all ? gr.passes : PassTab{};
             if (s->group_block == 64)
                 k_sweep_group<R, 128><<<nblk, 128, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf,
                                                                       s->d_fstart, gr.k0, gr.levels, dsc, gl,
-                                                                      s->glist_len);
+                                                                      s->glist_len, pt);
             else
                 k_sweep_group<R, 256><<<nblk, 256, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf,
                                                                       s->d_fstart, gr.k0, gr.levels, dsc, gl,
-                                                                      s->glist_len);
+                                                                      s->glist_len, pt);
             if (ev) ev->push_back({a, (*mark)()});
             continue;
         }
