@@ -1486,6 +1486,9 @@ struct ssw_sweep {
     std::vector<uint32_t*> d_idx;                   // per sweep level >= 1: active positions
     std::vector<std::vector<uint32_t>> cstart;      // per sweep level: level starts in idx
     std::vector<int> list_ok;                       // per sweep level: idx / cstart current
+    // per sweep level: the grouped replay's filter kept every task (the
+    // active set is all cells), so the level replays the full sweep's launches
+    std::vector<int> list_full;
     std::vector<int> replays;                       // per sweep level: replays since the list was made
     std::vector<hipGraphExec_t> graphs;             // per sweep level: captured replay
     // replay plan per sweep level: one launch per level set, or one launch per
@@ -1695,6 +1698,7 @@ int download_sa(ssw_sweep* s, std::vector<double2>& sa) {
 void drop_list(ssw_sweep* s, int lvl) {
     if (lvl >= (int)s->list_ok.size()) return;
     s->list_ok[lvl] = 0;
+    s->list_full[lvl] = 0;
     s->replays[lvl] = 0;
     if (s->graphs[lvl]) {
         (void)hipGraphExecDestroy(s->graphs[lvl]);
@@ -2150,6 +2154,7 @@ int build_schedule(ssw_sweep* s) {
     s->d_idx.assign(L, nullptr);
     s->cstart.assign(L, std::vector<uint32_t>());
     s->list_ok.assign(L, 0);
+    s->list_full.assign(L, 0);
     s->replays.assign(L, 0);
     for (hipGraphExec_t ge : s->graphs)
         if (ge) (void)hipGraphExecDestroy(ge);
@@ -2193,6 +2198,11 @@ int make_group_list(ssw_sweep* s, int lvl) {
     gb.assign(ng, 0u);
     HIP_TRY(hipMemcpyAsync(gb.data(), s->d_gblocks, ng * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
+    int64_t kept = 0;
+    for (uint32_t v : gc) kept += v;
+    // every list entry kept: the same tasks in the same per-workgroup order as
+    // the full sweep, whose launches (in their die-aware order) are then used
+    s->list_full[lvl] = (kept == s->group_tasks) ? 1 : 0;
     s->list_ok[lvl] = 1;
     s->replays[lvl] = 0;
     if (s->graphs[lvl]) {
@@ -2246,7 +2256,7 @@ void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<
                      const std::function<hipEvent_t()>* mark) {
     Flux<R> fl = make_flux<R>(s);
     const std::vector<uint32_t>& fs = s->full_start;
-    const bool all = lvl == 0;
+    const bool all = lvl == 0 || s->list_full[lvl];
     const std::vector<uint32_t>& cs = all ? fs : s->cstart[lvl];
     const uint32_t* idx = all ? nullptr : s->d_idx[lvl];
     const Entry<R>* ovf = (const Entry<R>*)s->d_ovf;
