@@ -1109,20 +1109,20 @@ __global__ void __launch_bounds__(E) k_sweep_group(Geo g, Flux<R> fl, const uint
 
 // The grouped replay of a restricted sweep (active set {level >= lvl}):
 // workgroup w filters descriptor w's task list to the active tasks, in order,
-// into the same place of list_out, and, if any is left, appends its filtered
-// descriptor to its group's descriptors (from gbase[group], slot counted in
-// gblocks), so that a launch starts only workgroups with active tasks; the
-// active tasks of each group are summed into gcount (gmap: group of each
+// into the same place of list_out, and, if any is left, writes its filtered
+// descriptor to desc_tmp[w] and kept[w] = 1 (k_place_desc then packs the kept
+// descriptors of each group in order), so that a launch starts only
+// workgroups with active tasks; the active tasks and the kept descriptors of
+// each group are summed into gcount and gblocks (gmap: group of each
 // descriptor). The cones of the full graph contain the active cones, so every
 // dependency of an active task inside a launch stays inside its workgroup;
 // the workgroups of a launch are independent, so their order is free.
 __global__ void __launch_bounds__(kBlock) k_group_compact(Geo g, const uint4* __restrict__ hdr, int lvl,
                                                           const uint4* __restrict__ desc_in,
                                                           const uint32_t* __restrict__ list_in,
-                                                          const uint32_t* __restrict__ gmap,
-                                                          const uint32_t* __restrict__ gbase, uint32_t ndesc,
-                                                          uint4* desc_out, uint32_t* list_out, uint32_t* gcount,
-                                                          uint32_t* gblocks) {
+                                                          const uint32_t* __restrict__ gmap, uint32_t ndesc,
+                                                          uint4* desc_tmp, uint32_t* kept, uint32_t* list_out,
+                                                          uint32_t* gcount, uint32_t* gblocks) {
     const uint32_t w = blockIdx.x;
     if (w >= ndesc) return;
     __shared__ uint32_t wave_tot[kBlock / 64];
@@ -1159,14 +1159,36 @@ __global__ void __launch_bounds__(kBlock) k_group_compact(Geo g, const uint4* __
     if (threadIdx.x == 0) {
         uint32_t total = 0;
         for (int k = 0; k < kMaxGroup; k++) total += m[k];
+        kept[w] = total ? 1u : 0u;
         if (total) {
             const uint32_t gr = gmap[w];
             atomicAdd(&gcount[gr], total);
-            const uint32_t o = gbase[gr] + atomicAdd(&gblocks[gr], 1u);
-            desc_out[2 * o] = make_uint4(d0.x, d0.y, m[0] | (m[1] << 16), m[2] | (m[3] << 16));
-            desc_out[2 * o + 1] = make_uint4(m[4] | (m[5] << 16), m[6] | (m[7] << 16), 0u, 0u);
+            atomicAdd(&gblocks[gr], 1u);
+            desc_tmp[2 * w] = make_uint4(d0.x, d0.y, m[0] | (m[1] << 16), m[2] | (m[3] << 16));
+            desc_tmp[2 * w + 1] = make_uint4(m[4] | (m[5] << 16), m[6] | (m[7] << 16), 0u, 0u);
         }
     }
+}
+
+// Packs the kept filtered descriptors: descriptor w of group g (whose
+// descriptors start at gbase[g]) goes to gbase[g] + (pos[w] - pos[gbase[g]]),
+// pos the exclusive prefix sum of kept, so each group's kept descriptors keep
+// their order (and with it the pass ranges of the die-aware launch order).
+__global__ void k_place_desc(const uint4* __restrict__ desc_tmp, const uint32_t* __restrict__ kept,
+                             const uint32_t* __restrict__ pos, const uint32_t* __restrict__ gmap,
+                             const uint32_t* __restrict__ gbase, uint4* desc_out, uint32_t ndesc) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= ndesc || !kept[w]) return;
+    const uint32_t b = gbase[gmap[w]];
+    const uint32_t o = b + (pos[w] - pos[b]);
+    desc_out[2 * o] = desc_tmp[2 * w];
+    desc_out[2 * o + 1] = desc_tmp[2 * w + 1];
+}
+
+__global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx, uint32_t* out,
+                             uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = src[idx[i]];
 }
 
 // ---------------------------------------------------------------------------
@@ -1439,6 +1461,12 @@ struct ssw_sweep {
     uint32_t* d_gcount = nullptr;
     uint32_t* d_gbase = nullptr;                    // first descriptor of each group
     uint32_t* d_gblocks = nullptr;                  // restricted sweep: workgroups with active tasks per group
+    // global descriptor index of every pass boundary of every group (group
+    // gi's npass + 1 boundaries from pbound_start[gi]); per restricted sweep
+    // level the pass ranges of each group's kept descriptors
+    uint32_t* d_pbound = nullptr;
+    std::vector<uint32_t> pbound_start;
+    std::vector<std::vector<PassTab>> gpass_l;
     std::vector<std::vector<uint32_t>> gcount, gblocks;
     int64_t group_tasks = 0;                        // tasks solved per full sweep by the grouped replay
     uint32_t glist_len = 0;                         // entries of the grouped replay's task list
@@ -1789,7 +1817,9 @@ int64_t group_bytes(int64_t DN, int64_t n, int64_t L, int64_t T) {
     const int64_t kmx = std::min<int64_t>(n + 2, 1 << 22);
     const int64_t dcap = DN / T + kmx + 1;
     const int64_t list = (DN + DN / 4 + (1 << 20)) * 4;
-    return list + dcap * 36 + kmx * 4 + (L - 1) * (list + dcap * 32);
+    // + the transient scratch of a restricted sweep's filter (descriptors in
+    // place, kept flags, their prefix sum)
+    return list + dcap * 36 + kmx * 4 + (L - 1) * (list + dcap * 32) + (L > 1 ? dcap * 40 + (1 << 20) : 0);
 }
 
 // The replay plan of sweep level lvl from its level starts (the full
@@ -2042,6 +2072,16 @@ int build_groups_t(ssw_sweep* s) {
     for (size_t i = 0; i < s->groups.size(); i++) gbase[i] = s->groups[i].desc0;
     TRY(dalloc(s, &s->d_gbase, gbase.size()));
     HIP_TRY(hipMemcpyAsync(s->d_gbase, gbase.data(), gbase.size() * 4, hipMemcpyHostToDevice, s->stream));
+    std::vector<uint32_t> pb;
+    s->pbound_start.assign(s->groups.size(), 0u);
+    for (size_t i = 0; i < s->groups.size(); i++) {
+        const ssw_sweep::Group& gr = s->groups[i];
+        s->pbound_start[i] = (uint32_t)pb.size();
+        for (int r = 0; r <= gr.passes.n; r++) pb.push_back(gr.desc0 + gr.passes.off[r]);
+    }
+    if (pb.empty()) pb.push_back(0u);
+    TRY(dalloc(s, &s->d_pbound, pb.size()));
+    HIP_TRY(hipMemcpyAsync(s->d_pbound, pb.data(), pb.size() * 4, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return SSW_OK;
 }
@@ -2186,11 +2226,40 @@ int make_group_list(ssw_sweep* s, int lvl) {
     HIP_TRY(hipMemsetAsync(s->d_gcount, 0, ng * 4, s->stream));
     HIP_TRY(hipMemsetAsync(s->d_gblocks, 0, ng * 4, s->stream));
     Geo g = make_geo(s);
-    if (s->ndesc)
-        k_group_compact<<<s->ndesc, kBlock, 0, s->stream>>>(g, s->d_hdr, lvl, s->d_gdesc, s->d_glist, s->d_gmap,
-                                                            s->d_gbase, s->ndesc, s->d_gdesc_l[lvl],
-                                                            s->d_glist_l[lvl], s->d_gcount, s->d_gblocks);
-    HIP_TRY(hipGetLastError());
+    const uint32_t nd = s->ndesc;
+    const size_t npb = std::max<size_t>(1, s->pbound_start.empty() ? 1 : (size_t)s->pbound_start.back() +
+                                                                          s->groups.back().passes.n + 1);
+    std::vector<uint32_t> pbv(npb, 0u);
+    if (nd) {
+        // filter, then pack each group's kept descriptors in order (transient scratch)
+        uint4* tmp_desc = nullptr;
+        uint32_t *kept = nullptr, *pos = nullptr, *pbval = nullptr;
+        uint8_t* scan_tmp = nullptr;
+        size_t tb = 0;
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, kept, pos, (int)nd + 1, s->stream));
+        TRY(dalloc(s, &tmp_desc, 2 * (size_t)nd));
+        TRY(dalloc(s, &kept, (size_t)nd + 1));
+        TRY(dalloc(s, &pos, (size_t)nd + 1));
+        TRY(dalloc(s, &pbval, npb));
+        TRY(dalloc(s, &scan_tmp, std::max<size_t>(tb, 1)));
+        k_fill_u32<<<1, 1, 0, s->stream>>>(kept + nd, 0u, 1);
+        k_group_compact<<<nd, kBlock, 0, s->stream>>>(g, s->d_hdr, lvl, s->d_gdesc, s->d_glist, s->d_gmap, nd,
+                                                       tmp_desc, kept, s->d_glist_l[lvl], s->d_gcount,
+                                                       s->d_gblocks);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(scan_tmp, tb, kept, pos, (int)nd + 1, s->stream));
+        k_place_desc<<<grid_for(nd), kBlock, 0, s->stream>>>(tmp_desc, kept, pos, s->d_gmap, s->d_gbase,
+                                                             s->d_gdesc_l[lvl], nd);
+        k_gather_u32<<<grid_for((int64_t)npb), kBlock, 0, s->stream>>>(pos, s->d_pbound, pbval, (uint32_t)npb);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(pbv.data(), pbval, npb * 4, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        dfree(s, scan_tmp);
+        dfree(s, pbval);
+        dfree(s, pos);
+        dfree(s, kept);
+        dfree(s, tmp_desc);
+    }
     std::vector<uint32_t>& gc = s->gcount[lvl];
     gc.assign(ng, 0u);
     HIP_TRY(hipMemcpyAsync(gc.data(), s->d_gcount, ng * 4, hipMemcpyDeviceToHost, s->stream));
@@ -2198,6 +2267,16 @@ int make_group_list(ssw_sweep* s, int lvl) {
     gb.assign(ng, 0u);
     HIP_TRY(hipMemcpyAsync(gb.data(), s->d_gblocks, ng * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
+    // pass ranges of each group's kept descriptors
+    if ((int)s->gpass_l.size() <= lvl) s->gpass_l.resize(lvl + 1);
+    s->gpass_l[lvl].assign(s->groups.size(), PassTab{});
+    for (size_t i = 0; i < s->groups.size(); i++) {
+        const ssw_sweep::Group& gr = s->groups[i];
+        PassTab& pt = s->gpass_l[lvl][i];
+        pt.n = gr.passes.n;
+        const uint32_t b = pbv[s->pbound_start[i]];
+        for (int r = 0; r <= gr.passes.n; r++) pt.off[r] = pbv[s->pbound_start[i] + r] - b;
+    }
     int64_t kept = 0;
     for (uint32_t v : gc) kept += v;
     // every list entry kept: the same tasks in the same per-workgroup order as
@@ -2270,7 +2349,7 @@ void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<
             const Entry<R>* pl = (const Entry<R>*)s->d_planes;
             // full sweeps: die-aware order inside each pass; restricted sweeps
             // launch their filtered descriptors in order
-            const PassTab pt = all ? gr.passes : PassTab{};
+            const PassTab pt = all ? gr.passes : s->gpass_l[lvl][it.tab];
             if (s->group_block == 64)
                 k_sweep_group<R, 128><<<nblk, 128, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf,
                                                                       s->d_fstart, gr.k0, gr.levels, dsc, gl,
@@ -2490,6 +2569,7 @@ void free_grid(ssw_sweep* s) {
     dfree(s, s->d_gcount);
     dfree(s, s->d_gbase);
     dfree(s, s->d_gblocks);
+    dfree(s, s->d_pbound);
     for (uint4*& p : s->d_gdesc_l) dfree(s, p);
     for (uint32_t*& p : s->d_glist_l) dfree(s, p);
     s->groups.clear();
