@@ -217,6 +217,15 @@ __device__ __forceinline__ uint32_t pass_item(const PassTab& pt, uint32_t b) {
     return o + x * q + min(x, rem) + i / kXcds;
 }
 
+// Block b of nb equal blocks in die order: the hardware deals workgroups to
+// the dies round-robin (b mod kXcds); die x takes the contiguous run of
+// blocks [x*q + min(x, r), ...) (q = nb / kXcds, r = nb mod kXcds), so that
+// neighbouring blocks, whose gathers share cache lines, share its L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t x = b % kXcds, q = nb / kXcds, r = nb % kXcds;
+    return x * q + min(x, r) + b / kXcds;
+}
+
 __device__ __forceinline__ uint4 task_record(uint32_t c, uint32_t f0, uint32_t f1, uint32_t d, uint32_t ps) {
     return make_uint4(c | ((f1 - f0) << kCellBits), f0, d, ps);
 }
@@ -656,7 +665,7 @@ __global__ void k_compact_starts(const uint32_t* first_chunk, const uint32_t* of
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_sweep_rec(Geo g, Flux<R> fl, const uint4* recs, const uint32_t* idx,
                                                       uint32_t count) {
-    const uint32_t i = blockIdx.x * kTasksPerBlock + threadIdx.x / kLanes;
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kTasksPerBlock + threadIdx.x / kLanes;
     const uint32_t sub = threadIdx.x % kLanes;
     const bool valid = i < count;
     const uint32_t p = valid ? (idx ? idx[i] : i) : 0u;
@@ -764,7 +773,7 @@ template <typename R, int kRecBlock>
 __global__ void __launch_bounds__(kRecBlock) k_sweep_comp(Geo g, Flux<R> fl, const uint4* hdr, const R* Av,
                                                           const Entry<R>* planes, const Entry<R>* ovf, uint32_t b,
                                                           uint32_t W, const uint32_t* idx, uint32_t count) {
-    const uint32_t i = blockIdx.x * kRecBlock + threadIdx.x;
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kRecBlock + threadIdx.x;
     if (i >= count) return;
     comp_task<R>(g, fl, hdr, Av, planes, ovf, b, W, idx ? idx[i] : i);
 }
