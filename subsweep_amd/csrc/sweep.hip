@@ -203,11 +203,12 @@ struct PassTab {
 };
 
 // The descriptor of workgroup b: the hardware deals workgroups to the dies
-// round-robin (b mod kXcds); inside each pass range, die x is given a
-// contiguous run of the range's descriptors ([x*q + min(x, r), ...) of the
-// range's c = 8q + r, taken in order), so that neighbouring cones, which read
-// the same schedule and X lines, share one die's L2, while every die still
-// gets an eighth of every pass.
+// round-robin (b mod kXcds). Inside the pass range [o, o + c) that holds b,
+// the workgroups of one die are those with the same x = (b - o) mod kXcds;
+// they are given the contiguous run of the range's descriptors that starts
+// at o + x*(c / kXcds) + min(x, c mod kXcds), in order. Neighbouring cones,
+// which read the same schedule and X lines, then share one die's L2, and
+// every die still gets an eighth of every pass.
 __device__ __forceinline__ uint32_t pass_item(const PassTab& pt, uint32_t b) {
     if (pt.n == 0) return b;
     int r = 0;
