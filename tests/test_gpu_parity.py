@@ -245,3 +245,22 @@ def test_chemistry_outputs_against_oracle(hip_available):
     _rel_ok(og["heating_rate"], oo["heating_rate"], 1e-10, atol_frac=1e-10, name="heating_rate")
     g.close()
     o.close()
+
+
+def test_chemistry_outputs_fp32_transport(hip_available):
+    """The optional outputs with FP32 transport (incoming rates stored in a
+    power-of-two scale and converted back): within 1e-5 of the oracle's FP64
+    values, like the ionised fraction, after the same ionisation front."""
+    pb = W.cartesian_problem(6, 16, levels=2, periodic=False, source_cells=(0,), steps=6, max_timestep=3e12,
+                             density=1e-24, box=1e20, temperature=1e4)
+    o = OracleSweep(pb.params, pb.grid, *pb.state())
+    g = Sweep(pb.with_params(transport_precision=32).params, pb.grid, *pb.state())
+    for _ in range(pb.steps):
+        assert g.run_sweeps() == o.run_sweeps()
+    og, oo = g.chemistry_outputs(), o.chemistry_outputs()
+    assert (oo["photoionization_rate"] > 0).sum() > 10
+    for k in ("photoionization_rate", "recombination_rate", "collisional_ionization_rate"):
+        _rel_ok(og[k], oo[k], 1e-5, atol_frac=1e-7, name=k + " fp32")
+    _rel_ok(og["heating_rate"], oo["heating_rate"], 1e-5, atol_frac=1e-5, name="heating_rate fp32")
+    g.close()
+    o.close()
