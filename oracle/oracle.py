@@ -4,7 +4,14 @@ tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg to compare
 results against, never by the product.
 
 `OracleSweep` has the same methods as subsweep_amd.Sweep so that a test can
-drive both with the same calls."""
+drive both with the same calls. The direction bins of
+DirectionsSpecification::Num come from tests/golden/reference_tables.json
+(the reference's tables, src/sweep/direction/healpix.rs, read as data), not
+from the product.
+
+`threads` spreads a sweep's directions and the per-cell loops over threads;
+each direction and each cell is computed exactly as with one thread, so the
+results are bitwise the same for every count."""
 
 from __future__ import annotations
 
@@ -15,6 +22,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_sweep.so")
+TABLES_PATH = os.path.join(os.path.dirname(HERE), "tests", "golden", "reference_tables.json")
 
 PD = C.POINTER(C.c_double)
 PI64 = C.POINTER(C.c_int64)
@@ -35,7 +43,13 @@ def lib():
         from subsweep_amd._native import SweepParametersC
         L = C.CDLL(LIB_PATH)
         sig = {
-            "orc_create": (P, [C.POINTER(SweepParametersC)]),
+            "orc_create": (P, [C.POINTER(SweepParametersC), PD]),
+            "orc_set_threads": (None, [P, C.c_int]),
+            "orc_set_source": (C.c_int, [P, PD]),
+            "orc_limiting_processes": (None, [P, PI64]),
+            "orc_solver_terms": (None, [C.c_double] * 8 + [PD]),
+            "orc_solver_try_timestep": (C.c_double, [PD] + [C.c_double] * 5 + [C.c_int] + [C.c_double] * 4 +
+                                        [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             "orc_destroy": (None, [P]),
             "orc_upload_grid": (C.c_int, [P, C.c_int64, PI64, PI32, PU8, PD, PD, PD, PD]),
             "orc_upload_state": (C.c_int, [P, PD, PD, PD, PD]),
@@ -65,6 +79,31 @@ def lib():
             f.argtypes = a
         _lib = L
     return _lib
+
+
+_tables = None
+
+
+def reference_tables() -> dict:
+    """tests/golden/reference_tables.json: the reference's direction bins,
+    BPASS table and test values, as data."""
+    global _tables
+    if _tables is None:
+        import json
+        with open(TABLES_PATH) as f:
+            _tables = json.load(f)
+    return _tables
+
+
+def tabulated_directions(num: int) -> np.ndarray:
+    """The bins of Directions::from_num(num) (src/sweep/direction/mod.rs:58-75):
+    [1, 0, 0] for one direction, else the HEALPix table of the reference."""
+    if num == 1:
+        return np.array([[1.0, 0.0, 0.0]])
+    t = reference_tables()["directions"]
+    if str(num) not in t:
+        raise ValueError(f"no direction table for {num} directions")
+    return np.array(t[str(num)]["bins"], dtype=np.float64)
 
 
 def _p(a, t):
@@ -114,6 +153,30 @@ def perform_timestep(x, T, density, volume, length, rate, timestep, safety=0.1, 
     return {"x": st[0], "T": st[1], "timescale": ts, "process": proc.value, "failed": bool(failed.value)}
 
 
+TERM_NAMES = ["temperature_change", "photoheating_rate", "cooling_rate", "cooling_rate_derivative", "mu",
+              "ionized_fraction_change", "photoionization_rate"]
+
+
+def solver_terms(x, T, density, volume, length, rate, timestep, scale_factor=1.0) -> dict:
+    """The terms of one implicit Solver update (no floor) at (x, T)."""
+    out = np.empty(7)
+    lib().orc_solver_terms(x, T, density, volume, length, rate, scale_factor, timestep, _p(out, C.c_double))
+    return dict(zip(TERM_NAMES, out))
+
+
+def try_timestep(x, T, density, volume, length, rate, timestep, safety=0.1, scale_factor=1.0, floor=None):
+    """One attempt of the sub-stepping at depth 0 (clamp + try_timestep_update):
+    {accepted, x, T, timescale, process}; on rejection x, T are the clamped
+    starting state. floor: None or (floor_x, floor_T)."""
+    st = np.array([x, T], dtype=np.float64)
+    acc, proc = C.c_int(), C.c_int()
+    fx, fT = floor if floor is not None else (0.0, 0.0)
+    ts = lib().orc_solver_try_timestep(_p(st, C.c_double), density, volume, length, rate, scale_factor,
+                                       0 if floor is None else 1, fx, fT, timestep, safety, C.byref(acc),
+                                       C.byref(proc))
+    return {"accepted": bool(acc.value), "x": st[0], "T": st[1], "timescale": ts, "process": proc.value}
+
+
 def timestep_level(max_num_levels, max_timestep, desired):
     return lib().orc_timestep_level(max_num_levels, max_timestep, desired)
 
@@ -130,12 +193,16 @@ def levels_in_sweep_order(num_levels: int, calls: int):
 
 
 class OracleSweep:
-    def __init__(self, params, grid, density, ionized_fraction, temperature, source):
+    def __init__(self, params, grid, density, ionized_fraction, temperature, source, threads: int = 1):
         L = lib()
         pc, keep = params.to_c()
-        self._h = L.orc_create(C.byref(pc))
+        table = None
+        if isinstance(params.directions, (int, np.integer)):
+            table = np.ascontiguousarray(tabulated_directions(int(params.directions)))
+        self._h = L.orc_create(C.byref(pc), _p(table, C.c_double))
         if not self._h:
             raise RuntimeError("oracle: bad parameters")
+        L.orc_set_threads(self._h, int(threads))
         self.num_cells = grid.num_cells
         self.num_directions = L.orc_num_directions(self._h)
         off, nbr, kind, area, normal, vol, size = grid.arrays()
@@ -180,6 +247,22 @@ class OracleSweep:
         out = {k: np.empty(self.num_cells) for k in CHEMISTRY_OUTPUT_NAMES}
         _check(lib().orc_download_chemistry_outputs(self._h, *(_p(v, C.c_double) for v in out.values())))
         return out
+
+    def set_source(self, source):
+        """Replaces the source rates (test hook; the reference sets sources
+        once at startup)."""
+        a = np.ascontiguousarray(source, dtype=np.float64)
+        _check(lib().orc_set_source(self._h, _p(a, C.c_double)))
+
+    def set_threads(self, threads: int):
+        lib().orc_set_threads(self._h, int(threads))
+
+    def limiting_processes(self) -> dict:
+        """TimescaleCounter of the last step: cells per limiting process whose
+        change timescale is below the maximum timestep."""
+        out = np.zeros(3, np.int64)
+        lib().orc_limiting_processes(self._h, _p(out, C.c_int64))
+        return {"temperature": int(out[0]), "ionization_fraction": int(out[1]), "photon_rate": int(out[2])}
 
     def directions(self):
         out = np.empty(3 * self.num_directions)

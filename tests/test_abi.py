@@ -54,18 +54,27 @@ def test_invalid_parameters_are_rejected():
     assert not h.value
 
 
-def test_direction_table_matches_the_oracle():
+def test_direction_tables_match_the_reference():
     # host-only call: the tables the library uses for DirectionsSpecification::Num
-    # against the oracle's own copy (a tiny oracle problem per direction count)
+    # against the reference's own (src/sweep/direction/healpix.rs, read as data
+    # into tests/golden/reference_tables.json), number for number; the oracle
+    # takes its bins from the same data
     import numpy as np
+    import pytest
+    from oracle.oracle import OracleSweep, reference_tables
     from subsweep_amd import direction_table
     from subsweep_amd import workloads as W
-    from oracle.oracle import OracleSweep
-    for num in (1, 16, 21, 32, 64, 84):
+    t = reference_tables()["directions"]
+    assert sorted(int(k) for k in t) == [16, 21, 32, 64, 84]
+    for num in (16, 21, 32, 64, 84):
+        ref = np.array(t[str(num)]["bins"])
+        assert ref.shape == (num, 3)
+        assert np.array_equal(direction_table(num), ref), num
         pb = W.cartesian_problem(2, num, levels=1, periodic=False, source_cells=(0,))
         o = OracleSweep(pb.params, pb.grid, *pb.state())
-        ref = np.asarray(o.directions()).reshape(-1, 3)
+        assert np.array_equal(o.directions(), ref), num
         o.close()
-        assert np.array_equal(direction_table(num), ref), num
-    with __import__("pytest").raises(N.SweepError):
+    # one direction is [1, 0, 0] (src/sweep/direction/mod.rs:62)
+    assert np.array_equal(direction_table(1), [[1.0, 0.0, 0.0]])
+    with pytest.raises(N.SweepError):
         direction_table(5)
