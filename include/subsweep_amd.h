@@ -181,6 +181,14 @@ typedef struct ssw_info {
                                       overlapping (>= the sweep's tasks)       */
     int32_t group_block;           /* grouped replay: root tasks per workgroup */
     int32_t reserved3;
+    /* TimescaleCounter of the last step (src/chemistry/timescale.rs:70-111,
+     * counted in update_chemistry, src/sweep/mod.rs:572, reset per run_sweeps,
+     * :267): chemistry updates whose change timescale is below max_timestep,
+     * by limiting process (a tie of the photon-rate and chemistry timescales
+     * goes to the chemistry, src/chemistry/timescale.rs:32-38). */
+    int64_t limiting_temperature;
+    int64_t limiting_ionization_fraction;
+    int64_t limiting_photon_rate;
 } ssw_info;
 
 int ssw_get_info(ssw_sweep* s, ssw_info* info);

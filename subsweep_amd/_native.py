@@ -76,6 +76,9 @@ class InfoC(C.Structure):
         ("group_tasks", C.c_int64),
         ("group_block", C.c_int32),
         ("reserved3", C.c_int32),
+        ("limiting_temperature", C.c_int64),
+        ("limiting_ionization_fraction", C.c_int64),
+        ("limiting_photon_rate", C.c_int64),
     ]
 
 

@@ -281,7 +281,9 @@ __device__ __forceinline__ void finish_task(const Flux<R>& fl, size_t ox, size_t
                                             R A) {
     if (inc < (R)0) inc = 0;  // Photons::make_positive
     R in = (inc + (R)ci.src) + ci.per;
-    R out = (in < (R)fl.thr) ? (R)0 : in * (R)ci.att;
+    // the threshold in the storage scale is compared in double: in FP32 it is
+    // threshold / lref (e.g. 1e-5 / 2^160), far below the float range
+    R out = ((double)in < fl.thr) ? (R)0 : in * (R)ci.att;
     fl.X[ox] = (A > (R)0) ? out / A : (R)0;
     fl.I[oi] = inc;
 }
@@ -1247,7 +1249,15 @@ struct ChemArgs {
     double lref;             // storage scale of I (rate = I * lref)
     unsigned long long* failures;
     unsigned long long* active;
+    unsigned long long* limiting;  // [3] TimescaleCounter: temperature, ionisation fraction, photon rate
 };
+
+// Adds the number of lanes with `pred` to *counter with one atomic per wave
+// (lanes outside the current control flow take no part).
+__device__ __forceinline__ void wave_count(bool pred, unsigned long long* counter) {
+    const unsigned long long m = __ballot(pred);
+    if (m && __lane_id() == __ffsll((long long)__ballot(1)) - 1) atomicAdd(counter, (unsigned long long)__popcll(m));
+}
 
 // update_chemistry (src/sweep/mod.rs:549-574) for the active cells, then the
 // attenuation of the updated cell for the next sweep.
@@ -1298,7 +1308,16 @@ __global__ void k_chemistry(Geo g, Flux<R> fl, ChemArgs ca, int lvl) {
     ca.x[c] = cell.x;
     ca.T[c] = cell.T;
     ca.species_ts[c] = used;
-    ca.change_ts[c] = (rate_ts < used) ? rate_ts : used;  // Timescale::min
+    // Timescale::min (src/chemistry/timescale.rs:32-38): a tie goes to the chemistry
+    const bool rate_limits = rate_ts < used;
+    const double change = rate_limits ? rate_ts : used;
+    ca.change_ts[c] = change;
+    // TimescaleCounter::count (src/chemistry/timescale.rs:80-87)
+    const int process = rate_limits ? 2 : proc;
+    const bool counted = change < ca.max_ts;
+    wave_count(counted && process == 0, &ca.limiting[0]);
+    wave_count(counted && process == 1, &ca.limiting[1]);
+    wave_count(counted && process == 2, &ca.limiting[2]);
     ((SA<R>*)ca.sa)[c].y = (R)ssw::non_absorbed_fraction(cell.rho, cell.x, cell.len);
 }
 
@@ -1476,6 +1495,13 @@ struct ssw_sweep {
     // level the pass ranges of each group's kept descriptors
     uint32_t* d_pbound = nullptr;
     std::vector<uint32_t> pbound_start;
+    // restricted sweeps' filter scratch, held from the cone build on: filtered
+    // descriptors in place, kept flags, their prefix sum, the pass bounds'
+    // packed positions and the scan's temporary storage
+    uint4* d_gf_desc = nullptr;
+    uint32_t *d_gf_kept = nullptr, *d_gf_pos = nullptr, *d_gf_pbval = nullptr;
+    uint8_t* d_gf_scan = nullptr;
+    size_t gf_scan_bytes = 0;
     std::vector<std::vector<PassTab>> gpass_l;
     std::vector<std::vector<uint32_t>> gcount, gblocks;
     int64_t group_tasks = 0;                        // tasks solved per full sweep by the grouped replay
@@ -1545,6 +1571,7 @@ struct ssw_sweep {
     void* d_sa = nullptr;                           // SA<R>: {transport source / D, attenuation}
     double *d_sts = nullptr, *d_prev = nullptr, *d_change = nullptr;
     unsigned long long* d_active = nullptr;
+    unsigned long long* d_limit = nullptr;          // [3] limiting processes of the last step
     uint8_t* d_level = nullptr;
     void *d_X = nullptr, *d_I = nullptr, *d_P = nullptr;
     unsigned long long* d_fail = nullptr;
@@ -1593,6 +1620,13 @@ int dalloc(ssw_sweep* s, T** p, size_t count) {
     s->alloc[(void*)*p] = count * sizeof(T);
     return SSW_OK;
 }
+
+// Runs its function when the scope ends, also on an early error return: the
+// transient device buffers of a call are freed on every path.
+struct Defer {
+    std::function<void()> f;
+    ~Defer() { f(); }
+};
 
 size_t real_size(const ssw_sweep* s) { return s->fp32 ? 4 : 8; }
 size_t entry_size(const ssw_sweep* s) { return s->fp32 ? sizeof(Entry<float>) : sizeof(Entry<double>); }
@@ -1936,6 +1970,13 @@ int build_groups_t(ssw_sweep* s) {
     uint32_t *roots = nullptr, *ctl = nullptr, *bounds = nullptr, *perm = nullptr;  // ctl: cursor, fail, selected
     int* lv = nullptr;
     uint4 *meta = nullptr, *desc_rm = nullptr;
+    Defer cleanup{[&] {
+        for (uint8_t** p : {&todo, &flags, &lvlflag, &tmp}) dfree(s, *p);
+        for (uint32_t** p : {&roots, &ctl, &bounds, &perm}) dfree(s, *p);
+        dfree(s, lv);
+        dfree(s, meta);
+        dfree(s, desc_rm);
+    }};
     TRY(dalloc(s, &todo, DN));
     TRY(dalloc(s, &ctl, 3));
     TRY(dalloc(s, &s->d_glist, list_cap));
@@ -2092,6 +2133,16 @@ int build_groups_t(ssw_sweep* s) {
     if (pb.empty()) pb.push_back(0u);
     TRY(dalloc(s, &s->d_pbound, pb.size()));
     HIP_TRY(hipMemcpyAsync(s->d_pbound, pb.data(), pb.size() * 4, hipMemcpyHostToDevice, s->stream));
+    if (s->p.num_timestep_levels > 1 && ndesc) {  // restricted sweeps filter the cones
+        s->gf_scan_bytes = 0;
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, s->gf_scan_bytes, s->d_gf_kept, s->d_gf_pos, (int)ndesc + 1,
+                                                 s->stream));
+        TRY(dalloc(s, &s->d_gf_desc, 2 * (size_t)ndesc));
+        TRY(dalloc(s, &s->d_gf_kept, (size_t)ndesc + 1));
+        TRY(dalloc(s, &s->d_gf_pos, (size_t)ndesc + 1));
+        TRY(dalloc(s, &s->d_gf_pbval, pb.size()));
+        TRY(dalloc(s, &s->d_gf_scan, std::max<size_t>(s->gf_scan_bytes, 1)));
+    }
     HIP_TRY(hipStreamSynchronize(s->stream));
     return SSW_OK;
 }
@@ -2120,6 +2171,11 @@ int build_schedule(ssw_sweep* s) {
     // sort the tasks by depth, stable over t = d*N + c (double-buffered: four
     // DN-word arrays and a small temporary)
     uint32_t *d_depth = nullptr, *d_keys = nullptr, *d_vals = nullptr, *d_vals2 = nullptr;
+    uint8_t* d_tmp = nullptr;
+    Defer cleanup{[&] {
+        for (uint32_t** p : {&d_depth, &d_keys, &d_vals, &d_vals2}) dfree(s, *p);
+        dfree(s, d_tmp);
+    }};
     TRY(dalloc(s, &d_depth, DN));
     Geo g = make_geo(s);
     k_scatter_depth<<<grid_for(std::min<int64_t>((int64_t)DN, 1 << 22)), kBlock, 0, s->stream>>>(g, s->kmax,
@@ -2137,7 +2193,6 @@ int build_schedule(ssw_sweep* s) {
     hipcub::DoubleBuffer<uint32_t> kb(d_depth, d_keys), vb(d_vals, d_vals2);
     size_t tmp_bytes = 0;
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, kb, vb, (int)DN, 0, bits, s->stream));
-    uint8_t* d_tmp = nullptr;
     TRY(dalloc(s, &d_tmp, tmp_bytes));
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp_bytes, kb, vb, (int)DN, 0, bits, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -2241,17 +2296,14 @@ int make_group_list(ssw_sweep* s, int lvl) {
                                                                           s->groups.back().passes.n + 1);
     std::vector<uint32_t> pbv(npb, 0u);
     if (nd) {
-        // filter, then pack each group's kept descriptors in order (transient scratch)
-        uint4* tmp_desc = nullptr;
-        uint32_t *kept = nullptr, *pos = nullptr, *pbval = nullptr;
-        uint8_t* scan_tmp = nullptr;
-        size_t tb = 0;
-        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, kept, pos, (int)nd + 1, s->stream));
-        TRY(dalloc(s, &tmp_desc, 2 * (size_t)nd));
-        TRY(dalloc(s, &kept, (size_t)nd + 1));
-        TRY(dalloc(s, &pos, (size_t)nd + 1));
-        TRY(dalloc(s, &pbval, npb));
-        TRY(dalloc(s, &scan_tmp, std::max<size_t>(tb, 1)));
+        // filter, then pack each group's kept descriptors in order (scratch
+        // held since the cone build)
+        uint4* tmp_desc = s->d_gf_desc;
+        uint32_t *kept = s->d_gf_kept, *pos = s->d_gf_pos, *pbval = s->d_gf_pbval;
+        uint8_t* scan_tmp = s->d_gf_scan;
+        size_t tb = s->gf_scan_bytes;
+        if (!tmp_desc || !kept || !pos || !pbval || !scan_tmp)
+            return fail(SSW_ERR_INVALID, "grouped replay: restricted-sweep scratch missing");
         k_fill_u32<<<1, 1, 0, s->stream>>>(kept + nd, 0u, 1);
         k_group_compact<<<nd, kBlock, 0, s->stream>>>(g, s->d_hdr, lvl, s->d_gdesc, s->d_glist, s->d_gmap, nd,
                                                        tmp_desc, kept, s->d_glist_l[lvl], s->d_gcount,
@@ -2264,11 +2316,6 @@ int make_group_list(ssw_sweep* s, int lvl) {
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(pbv.data(), pbval, npb * 4, hipMemcpyDeviceToHost, s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
-        dfree(s, scan_tmp);
-        dfree(s, pbval);
-        dfree(s, pos);
-        dfree(s, kept);
-        dfree(s, tmp_desc);
     }
     std::vector<uint32_t>& gc = s->gcount[lvl];
     gc.assign(ng, 0u);
@@ -2462,6 +2509,7 @@ ChemArgs chem_args(ssw_sweep* s) {
     ca.lref = s->lref;
     ca.failures = s->d_fail;
     ca.active = s->d_active;
+    ca.limiting = s->d_limit;
     return ca;
 }
 
@@ -2473,6 +2521,7 @@ int run_step(ssw_sweep* s, double* elapsed) {
     const std::vector<int> order = s->ts.sweep_order();
     const int L = s->p.num_timestep_levels;
     k_fill_u32<<<1, 2, 0, s->stream>>>((uint32_t*)s->d_active, 0u, 2);
+    k_fill_u32<<<1, 6, 0, s->stream>>>((uint32_t*)s->d_limit, 0u, 6);
     const unsigned gper = grid_for(std::max<int64_t>(s->Nper, 1) * s->D);
     const unsigned gcells = grid_for(s->N);
     s->sweeps_last = 0;
@@ -2580,6 +2629,12 @@ void free_grid(ssw_sweep* s) {
     dfree(s, s->d_gbase);
     dfree(s, s->d_gblocks);
     dfree(s, s->d_pbound);
+    dfree(s, s->d_gf_desc);
+    dfree(s, s->d_gf_kept);
+    dfree(s, s->d_gf_pos);
+    dfree(s, s->d_gf_pbval);
+    dfree(s, s->d_gf_scan);
+    s->gf_scan_bytes = 0;
     for (uint4*& p : s->d_gdesc_l) dfree(s, p);
     for (uint32_t*& p : s->d_glist_l) dfree(s, p);
     s->groups.clear();
@@ -2619,6 +2674,7 @@ int setup_overflow(ssw_sweep* s, uint32_t* d_ovf_count) {
     TRY(dalloc(s, &s->d_ovf_start, DN));
     TRY(dalloc(s, (uint8_t**)&s->d_ovf, (size_t)std::max<int64_t>(1, s->ovf_total) * entry_size(s)));
     uint32_t* d_cursor = nullptr;
+    Defer cleanup{[&] { dfree(s, d_cursor); }};
     TRY(dalloc(s, &d_cursor, 1));
     HIP_TRY(hipMemsetAsync(d_cursor, 0, 4, s->stream));
     Geo g = make_geo(s);
@@ -2706,12 +2762,14 @@ int ssw_create(const ssw_sweep_parameters* params, ssw_sweep** out) {
     if (r == SSW_OK) r = dalloc(s, &s->d_need, (size_t)p.num_timestep_levels);
     if (r == SSW_OK) r = dalloc(s, &s->d_fail, 1);
     if (r == SSW_OK) r = dalloc(s, &s->d_active, 1);
+    if (r == SSW_OK) r = dalloc(s, &s->d_limit, 3);
     if (r != SSW_OK) {
         ssw_destroy(s);
         return r;
     }
     if (hipMemcpy(s->d_dirs, dirs.data(), dirs.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(s->d_fail, 0, 8) != hipSuccess || hipMemset(s->d_active, 0, 8) != hipSuccess) {
+        hipMemset(s->d_fail, 0, 8) != hipSuccess || hipMemset(s->d_active, 0, 8) != hipSuccess ||
+        hipMemset(s->d_limit, 0, 24) != hipSuccess) {
         ssw_destroy(s);
         return fail(SSW_ERR_HIP, "initialisation of the handle's device memory failed");
     }
@@ -2729,6 +2787,7 @@ void ssw_destroy(ssw_sweep* s) {
     dfree(s, s->d_need);
     dfree(s, s->d_fail);
     dfree(s, s->d_active);
+    dfree(s, s->d_limit);
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -2915,6 +2974,7 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     HIP_TRY(hipMemcpyAsync(s->d_level, lev.data(), n, hipMemcpyHostToDevice, s->stream));
     // mirror check, downwind statistics and overflow counts (into cnt)
     unsigned long long* d_stats = nullptr;
+    Defer cleanup{[&] { dfree(s, d_stats); }};
     TRY(dalloc(s, &d_stats, 5));
     HIP_TRY(hipMemsetAsync(d_stats, 0, 5 * 8, s->stream));
     Geo g = make_geo(s);
@@ -3004,6 +3064,7 @@ int ssw_upload_state(ssw_sweep* s, const double* density, const double* xhii, co
     if (s->Nper) HIP_TRY(hipMemsetAsync(s->d_P, 0, (size_t)s->D * s->Nper * es, s->stream));
     HIP_TRY(hipMemsetAsync(s->d_fail, 0, 8, s->stream));
     HIP_TRY(hipMemsetAsync(s->d_active, 0, 8, s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_limit, 0, 24, s->stream));
     std::vector<uint8_t> lev(n, (uint8_t)(L - 1));  // src/sweep/mod.rs:206
     HIP_TRY(hipMemcpyAsync(s->d_level, lev.data(), n, hipMemcpyHostToDevice, s->stream));
     if (s->fp32)
@@ -3042,6 +3103,7 @@ int ssw_download(ssw_sweep* s, double* x, double* T, double* sts, double* photon
     if (level) HIP_TRY(hipMemcpyAsync(level, s->d_level, n, hipMemcpyDeviceToHost, s->stream));
     if (photon) {
         double* d_pr = nullptr;
+        Defer cleanup{[&] { dfree(s, d_pr); }};
         TRY(dalloc(s, &d_pr, n));
         Geo g = make_geo(s);
         if (s->fp32)
@@ -3072,6 +3134,7 @@ int ssw_download_chemistry_outputs(ssw_sweep* s, double* photoionization, double
     const int64_t n = s->N;
     double* outs[4] = {photoionization, heating, recombination, collisional_ionization};
     double* d_out = nullptr;
+    Defer cleanup{[&] { dfree(s, d_out); }};
     TRY(dalloc(s, &d_out, n));
     const Geo g = make_geo(s);
     const ChemArgs ca = chem_args(s);
@@ -3094,6 +3157,7 @@ int ssw_download_counts(ssw_sweep* s, int32_t lvl, uint32_t* counts) {
     if (!s || !counts) return fail(SSW_ERR_INVALID, "null argument");
     if (!s->have_grid) return fail(SSW_ERR_INVALID, "ssw_upload_grid must be called first");
     if (lvl < 0 || lvl >= s->p.num_timestep_levels) return fail(SSW_ERR_INVALID, "sweep_level out of range");
+    Defer cleanup{[&] { release_layering_scratch(s); }};
     TRY(ensure_layering_scratch(s, false));
     Geo g = make_geo(s);
     k_count<<<grid_for((int64_t)s->N * s->D), kBlock, 0, s->stream>>>(g, lvl, 1);
@@ -3117,9 +3181,13 @@ int ssw_download_levelsets(ssw_sweep* s, int32_t lvl, uint32_t* depth) {
     int64_t active = 0;
     for (uint8_t v : lev) active += v >= lvl;
     int k = 0;
+    uint32_t* d_depth = nullptr;
+    Defer cleanup{[&] {
+        dfree(s, d_depth);
+        release_layering_scratch(s);
+    }};
     TRY(ensure_layering_scratch(s));
     TRY(kahn_layering(s, lvl, active * s->D, &k, nullptr));
-    uint32_t* d_depth = nullptr;
     const int64_t DN = (int64_t)s->D * s->N;
     TRY(dalloc(s, &d_depth, (size_t)DN));
     k_fill_u32<<<grid_for(DN), kBlock, 0, s->stream>>>(d_depth, kNone, DN);
@@ -3139,6 +3207,9 @@ int ssw_download_fluxes(ssw_sweep* s, double* incoming, double* outgoing, double
     if (!s->have_state) return fail(SSW_ERR_INVALID, "ssw_upload_state must be called first");
     const size_t m = (size_t)s->D * s->N;
     double *dI = nullptr, *dO = nullptr, *dP = nullptr;
+    Defer cleanup{[&] {
+        for (double** p : {&dI, &dO, &dP}) dfree(s, *p);
+    }};
     TRY(dalloc(s, &dI, m));
     TRY(dalloc(s, &dO, m));
     TRY(dalloc(s, &dP, m));
@@ -3168,6 +3239,7 @@ int ssw_download_transport(ssw_sweep* s, void* X, void* I) {
     const int64_t DN = (int64_t)s->D * s->N;
     const size_t m = (size_t)DN * real_size(s);
     uint8_t* dtmp = nullptr;
+    Defer cleanup{[&] { dfree(s, dtmp); }};
     if (X && s->compiled) {  // X is stored in schedule-position order
         TRY(dalloc(s, &dtmp, m));
         if (s->fp32)
@@ -3254,6 +3326,7 @@ int save_t(ssw_sweep* s, uint8_t* buf) {
     R* I = X + DN;
     R* P = I + DN;
     R* tmp = nullptr;
+    Defer cleanup{[&] { dfree(s, tmp); }};
     if (s->compiled) {
         TRY(dalloc(s, &tmp, (size_t)DN));
         k_gather_x<R><<<grid_for(DN), kBlock, 0, s->stream>>>((const R*)s->d_X, s->d_xpos, tmp, DN);
@@ -3324,6 +3397,7 @@ int load_t(ssw_sweep* s, const uint8_t* buf) {
     HIP_TRY(hipMemcpyAsync(s->d_change, cell.data() + 7 * n, n * 8, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_level, lev.data(), n, hipMemcpyHostToDevice, s->stream));
     R* tmp = nullptr;
+    Defer cleanup{[&] { dfree(s, tmp); }};
     if (s->compiled) {
         TRY(dalloc(s, &tmp, (size_t)DN));
         HIP_TRY(hipMemcpyAsync(tmp, X.data(), DN * sizeof(R), hipMemcpyHostToDevice, s->stream));
@@ -3403,12 +3477,16 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info) {
     info->group_block = s->group_block;
     info->locality_order = s->permuted ? 1 : 0;
     if (s->have_state) {
-        unsigned long long f = 0, a = 0;
+        unsigned long long f = 0, a = 0, lim[3] = {0, 0, 0};
         HIP_TRY(hipMemcpyAsync(&f, s->d_fail, 8, hipMemcpyDeviceToHost, s->stream));
         HIP_TRY(hipMemcpyAsync(&a, s->d_active, 8, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipMemcpyAsync(lim, s->d_limit, 24, hipMemcpyDeviceToHost, s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
         info->chemistry_failures = (int64_t)f;
         info->updates_last_step = (int64_t)a * s->D;
+        info->limiting_temperature = (int64_t)lim[0];
+        info->limiting_ionization_fraction = (int64_t)lim[1];
+        info->limiting_photon_rate = (int64_t)lim[2];
     }
     return SSW_OK;
 }

@@ -247,12 +247,16 @@ CHEMISTRY_OUTPUTS = ("photoionization_rate", "heating_rate", "recombination_rate
 @dataclass
 class SweepPlugin:
     """The plugin's step system over a `Sweep`: run_sweep_system
-    (src/sweep/mod.rs:694-739). The first call only marks IsFirstTime.
+    (src/sweep/mod.rs:694-739). The first call only marks IsFirstTime and
+    returns None: the components keep the initial conditions, which `last`
+    then holds (the state the time series of that frame are computed from,
+    src/sweep/mod.rs:150-160).
 
     `optional_outputs` names the optional chemistry components
-    (src/components.rs:50-68) to evaluate after each step, as the
-    sweep_optional_output_system instances that the plugin registers for
-    the desired output fields (src/sweep/mod.rs:162-165, 770-781); their
+    (src/components.rs:50-68) to evaluate after every call, the first one
+    included, as the sweep_optional_output_system instances that the plugin
+    registers for the desired output fields and that run after
+    run_sweep_system each frame (src/sweep/mod.rs:162-165, 770-781); their
     values land in `chemistry`."""
 
     sweep: Sweep
@@ -271,6 +275,8 @@ class SweepPlugin:
     def run_sweep_system(self) -> SweepResults | None:
         if self.is_first_time:
             self.is_first_time = False
+            self.last = self.sweep.results()
+            self._optional_outputs()
             return None
         self.time += self.sweep.run_sweeps()
         res = self.sweep.results()
@@ -279,7 +285,10 @@ class SweepPlugin:
         newly = (res.ionized_hydrogen_fraction > 0.5) & (self.ionization_time == np.inf)
         self.ionization_time[newly] = self.time
         self.last = res
+        self._optional_outputs()
+        return res
+
+    def _optional_outputs(self) -> None:
         if self.optional_outputs:
             out = self.sweep.chemistry_outputs()
             self.chemistry = {k: out[k] for k in self.optional_outputs}
-        return res

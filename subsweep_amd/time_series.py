@@ -90,7 +90,11 @@ class TimeSeriesOutput:
         return vals
 
     def write_plugin_step(self, plugin) -> dict | None:
-        """After plugin.run_sweep_system(): nothing on the skipped first call."""
+        """After each plugin.run_sweep_system(). The reference's series
+        systems run every frame (AfterSweep, src/sweep/mod.rs:150-160), the
+        first one too, whose run_sweep_system skips the sweep: that frame
+        writes the initial conditions at the plugin's time, 0. None before
+        the plugin's first call."""
         if plugin.last is None:
             return None
         return self.write(plugin.time, plugin.last, plugin.sweep.chemistry_outputs()["photoionization_rate"])

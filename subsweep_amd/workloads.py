@@ -58,11 +58,15 @@ class Problem:
 
 def jittered_problem(n: int, sigma_ln: float, sources: str, num_directions: int, levels: int = 1,
                      threshold: float = 0.0, periodic: bool = True, steps: int = 10, seed: int = 1338,
-                     precision: int = 64, name: str | None = None, threads: int = 0) -> Problem:
+                     precision: int = 64, name: str | None = None, threads: int = 0,
+                     rho_mean: float = RHO_MEAN) -> Problem:
+    """`rho_mean` scales the gas: the default makes a 64^3 grid's cells
+    optically thick (tau ~ 200-1100 at x = 1e-10); 1e-3 of it gives tau ~ 1
+    per cell on a 16^3 grid, where photons cross the box."""
     pts = jittered_lattice(n, BOX, 0.3, seed)
     grid = voronoi_grid(pts, BOX, threads=threads)
     N = grid.num_cells
-    rho = lognormal_density(N, RHO_MEAN, sigma_ln, seed + 1)
+    rho = lognormal_density(N, rho_mean, sigma_ln, seed + 1)
     rng = np.random.default_rng(seed - 1)
     if sources == "centre":
         pos = np.array([[BOX / 2, BOX / 2, BOX / 2]])
@@ -101,11 +105,12 @@ def power_law_luminosities(k: int, lo: float, hi: float, seed: int) -> np.ndarra
 def clustered_problem(n: int, num_directions: int = 84, levels: int = 4, sources: int = 1024,
                       sigma_ln: float = 2.0, mesh: int = 128, escape_fraction: float = 0.01,
                       threshold: float = 0.0, periodic: bool = True, steps: int = 10, precision: int = 32,
-                      timestep_safety_factor: float = 0.1, name: str | None = None, threads: int = 0) -> Problem:
+                      timestep_safety_factor: float = 0.1, name: str | None = None, threads: int = 0,
+                      rho_mean: float = RHO_MEAN) -> Problem:
     pts, delta = clustered_points(n, BOX, mesh=mesh, sigma_ln=sigma_ln, seed=1340)
     grid = voronoi_grid(pts, BOX, threads=threads)
     N = grid.num_cells
-    rho = RHO_MEAN * delta
+    rho = rho_mean * delta
     # the densest cells (ties by index) each get one source
     order = np.lexsort((np.arange(N), -rho))
     src = np.zeros(N)

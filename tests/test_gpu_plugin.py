@@ -48,7 +48,13 @@ def test_plugin_optional_chemistry_outputs(hip_available):
     names = ("heating_rate", "photoionization_rate")
     plugin = SweepPlugin(Sweep(pb.params, pb.grid, *pb.state()), optional_outputs=names)
     direct = Sweep(pb.params, pb.grid, *pb.state())
-    plugin.run_sweep_system()
+    # the first call skips the sweep but the output systems still run: the
+    # outputs of the initial state
+    assert plugin.run_sweep_system() is None
+    want = direct.chemistry_outputs()
+    for k in names:
+        assert np.array_equal(plugin.chemistry[k], want[k]), k
+    assert (plugin.chemistry["photoionization_rate"] > 0).any()  # the source cell's own rate
     for _ in range(pb.steps):
         plugin.run_sweep_system()
         direct.run_sweeps()
@@ -75,12 +81,12 @@ def test_plugin_time_series_against_oracle(tmp_path, hip_available):
     plugin = SweepPlugin(Sweep(pb.params, pb.grid, *pb.state()))
     out = TS.TimeSeriesOutput(str(tmp_path), volume, density, pb.params.max_timestep, pb.params.num_timestep_levels)
     o = OracleSweep(pb.params, pb.grid, *pb.state())
-    plugin.run_sweep_system()
-    assert out.write_plugin_step(plugin) is None  # the skipped first call writes nothing
-    for _ in range(pb.steps):
-        plugin.run_sweep_system()
+    assert out.write_plugin_step(plugin) is None  # nothing before the plugin's first call
+    for step in range(pb.steps + 1):
+        plugin.run_sweep_system()  # the first call: the initial conditions at time 0
         got = out.write_plugin_step(plugin)
-        o.run_sweeps()
+        if step > 0:
+            o.run_sweeps()
         ro = o.results()
         want = TS.compute(ro.ionized_hydrogen_fraction, ro.temperature, o.chemistry_outputs()["photoionization_rate"],
                           ro.level, volume, density, pb.params.max_timestep, pb.params.num_timestep_levels)
@@ -88,6 +94,8 @@ def test_plugin_time_series_against_oracle(tmp_path, hip_available):
             assert got[k] == pytest.approx(want[k], rel=1e-10, abs=0), k
         assert got[TS.LEVELS] == want[TS.LEVELS]
     assert got["photoionization_rate_volume_average"] > 0
-    assert len(TS.read(str(tmp_path), TS.LEVELS)) == pb.steps
+    entries = TS.read(str(tmp_path), TS.LEVELS)
+    assert len(entries) == pb.steps + 1 and entries[0]["time"] == 0.0
+    assert entries[0]["val"][-1]["num"] == pb.grid.num_cells  # every cell starts on the deepest level
     plugin.sweep.close()
     o.close()
