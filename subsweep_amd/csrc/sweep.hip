@@ -2170,10 +2170,10 @@ int build_schedule(ssw_sweep* s) {
     dfree(s, s->d_cnt);
     // sort the tasks by depth, stable over t = d*N + c (double-buffered: four
     // DN-word arrays and a small temporary)
-    uint32_t *d_depth = nullptr, *d_keys = nullptr, *d_vals = nullptr, *d_vals2 = nullptr;
+    uint32_t *d_depth = nullptr, *d_keys = nullptr, *d_vals = nullptr, *d_vals2 = nullptr, *d_sorted = nullptr;
     uint8_t* d_tmp = nullptr;
     Defer cleanup{[&] {
-        for (uint32_t** p : {&d_depth, &d_keys, &d_vals, &d_vals2}) dfree(s, *p);
+        for (uint32_t** p : {&d_depth, &d_keys, &d_vals, &d_vals2, &d_sorted}) dfree(s, *p);
         dfree(s, d_tmp);
     }};
     TRY(dalloc(s, &d_depth, DN));
@@ -2196,12 +2196,16 @@ int build_schedule(ssw_sweep* s) {
     TRY(dalloc(s, &d_tmp, tmp_bytes));
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp_bytes, kb, vb, (int)DN, 0, bits, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    uint32_t* d_sorted = vb.Current();
+    // the sorted order moves to d_sorted: the buffer it is stays owned by
+    // exactly one pointer, so that it is freed once
+    d_sorted = vb.Current();
+    if (d_sorted == d_vals) d_vals = nullptr;
+    else d_vals2 = nullptr;
     dfree(s, d_tmp);
     dfree(s, d_keys);
     dfree(s, d_depth);
-    if (d_sorted == d_vals) dfree(s, d_vals2);
-    else dfree(s, d_vals);
+    dfree(s, d_vals);
+    dfree(s, d_vals2);
     TRY(dalloc(s, &s->d_fstart, s->full_start.size()));
     HIP_TRY(hipMemcpyAsync(s->d_fstart, s->full_start.data(), s->full_start.size() * 4, hipMemcpyHostToDevice,
                            s->stream));
