@@ -120,6 +120,19 @@ int ssw_download(ssw_sweep* s, double* ionized_fraction, double* temperature,
                  double* species_timestep, double* photon_rate, uint8_t* level,
                  double* change_timescale);
 
+/* ssw_download without any allocation: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the same outputs,
+ * bitwise equal to ssw_download's, staged through a fixed 64 MiB device
+ * array that each device the library runs on holds until the process ends,
+ * so it handles at most 8388608 cells (SSW_ERR_INVALID above that). It calls
+ * no hipMalloc or hipFree. Every call on one device shares that array, so
+ * calls that overlap on one device (handles driven from several threads)
+ * race: serialise them. */
+int ssw_download_noalloc(ssw_sweep* s, double* ionized_fraction, double* temperature,
+                         double* species_timestep, double* photon_rate, uint8_t* level,
+                         double* change_timescale);
+
 /* The optional chemistry outputs of sweep_optional_output_system
  * (src/sweep/chemistry_output.rs:25-67): per cell, the Solver of
  * Sweep::get_solver (src/sweep/mod.rs:612-632; rate = sum over directions of

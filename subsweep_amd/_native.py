@@ -146,6 +146,7 @@ SWEEP_SIGNATURES = {
     "ssw_upload_state": (C.c_int, [P, PD, PD, PD, PD]),
     "ssw_step": (C.c_int, [P, PD]),
     "ssw_download": (C.c_int, [P, PD, PD, PD, PD, PU8, PD]),
+    "ssw_download_noalloc": (C.c_int, [P, PD, PD, PD, PD, PU8, PD]),
     "ssw_download_chemistry_outputs": (C.c_int, [P, PD, PD, PD, PD]),
     "ssw_download_counts": (C.c_int, [P, C.c_int32, PU32]),
     "ssw_download_levelsets": (C.c_int, [P, C.c_int32, PU32]),

@@ -50,12 +50,14 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -89,6 +91,16 @@ constexpr int kSortCap = 4096;             // grouped replay build: entries of o
 constexpr int kMaxGroup = 8;               // grouped replay: level sets per group
 constexpr uint32_t kNarrowGroup = 2048;    // grouped replay: level sets this narrow go kMaxGroup to a group
 constexpr uint32_t kXcds = 8;              // MI355X: accelerator dies, each with its own L2
+
+// Staging of ssw_download_noalloc, which allocates nothing: a file-scope
+// device array that every device the library runs on holds until the process
+// ends. One output array of at most kNoAllocBytes / 8 cells passes through it
+// at a time. Host code never uses the name `scratch` as an address (in host
+// code it is a stand-in, not the device array): it uses the per-device address
+// from hipGetSymbolAddress (noalloc_scratch below).
+constexpr size_t kNoAllocBytes = size_t(64) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
 
 thread_local std::string g_err;
 // Handles alive in the process. Replay graphs are captured only while a single
@@ -1368,7 +1380,7 @@ __global__ void k_levels(Geo g, const double* change_ts, uint8_t* level, double 
 // (cell, direction) in SI units (parity hooks and the photon-rate output).
 template <typename R>
 __global__ void k_fluxes(Geo g, Flux<R> fl, double lref, double* I_out, double* O_out, double* P_out,
-                         double* photon_rate) {
+                         double* photon_rate, const uint32_t* rate_idx) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     double total = 0.0;
@@ -1395,7 +1407,16 @@ __global__ void k_fluxes(Geo g, Flux<R> fl, double lref, double* I_out, double* 
         if (P_out) P_out[o] = (double)per * lref;
         if (O_out) O_out[o] = (double)fl.X[xidx(fl, g.N, (uint32_t)d, (uint32_t)c)] * (A * fl.inv_aref) * lref;
     }
-    if (photon_rate) photon_rate[c] = total;
+    // rate_idx: the caller's index of each internal cell (NULL: internal order)
+    if (photon_rate) photon_rate[rate_idx ? rate_idx[c] : c] = total;
+}
+
+// dst[idx[i]] = src[i]: a per-cell array from the internal into the caller's order.
+template <typename T>
+__global__ void k_to_caller(const T* __restrict__ src, const uint32_t* __restrict__ idx, T* __restrict__ dst,
+                            int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[idx[i]] = src[i];
 }
 
 // Direction-major copy of X from the schedule-position order.
@@ -1516,6 +1537,7 @@ struct ssw_sweep {
     std::vector<double> pos;
     std::vector<uint32_t> perm, inv;
     bool permuted = false;
+    uint32_t* d_inv = nullptr;                      // inv on the device (permuted grids only)
     uint32_t *d_off = nullptr, *d_nk = nullptr, *d_pslot = nullptr, *d_pcells = nullptr;
     std::vector<uint32_t> h_pcells;                 // internal cell of each periodic slot
     uint32_t *d_pfoff = nullptr, *d_pfidx = nullptr;
@@ -2643,6 +2665,7 @@ void free_grid(ssw_sweep* s) {
     for (uint32_t*& p : s->d_glist_l) dfree(s, p);
     s->groups.clear();
     dfree(s, s->d_off);
+    dfree(s, s->d_inv);
     dfree(s, s->d_geo);
     dfree(s, s->d_nk);
     dfree(s, s->d_vol);
@@ -2686,6 +2709,46 @@ int setup_overflow(ssw_sweep* s, uint32_t* d_ovf_count) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->stream));
     dfree(s, d_cursor);
+    return SSW_OK;
+}
+
+// The device address of `scratch` on device `dev`, looked up once per device
+// with that device current (each device holds its own copy of the array at
+// its own address) and kept for the life of the process.
+std::mutex g_scratch_mu;
+std::array<void*, kMaxDevices> g_scratch_ptr{};
+
+int noalloc_scratch(int dev, uint8_t** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(SSW_ERR_INVALID, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int cur = -1;
+        HIP_TRY(hipGetDevice(&cur));
+        if (cur != dev) HIP_TRY(hipSetDevice(dev));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) HIP_TRY(hipSetDevice(cur));
+        if (e != hipSuccess || !p)
+            return fail(SSW_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = (uint8_t*)g_scratch_ptr[dev];
+    return SSW_OK;
+}
+
+// One per-cell output of ssw_download_noalloc: into the caller's order through
+// the staging buffer when the grid is permuted, else copied directly.
+template <typename T>
+int noalloc_out(ssw_sweep* s, const T* src, T* dst, uint8_t* stage) {
+    const int64_t n = s->N;
+    if (s->permuted) {
+        k_to_caller<T><<<grid_for(n), kBlock, 0, s->stream>>>(src, s->d_inv, (T*)stage, n);
+        HIP_TRY(hipGetLastError());
+        src = (const T*)stage;
+    }
+    // the next output reuses the staging buffer: stream order puts its kernel
+    // after this copy
+    HIP_TRY(hipMemcpyAsync(dst, src, (size_t)n * sizeof(T), hipMemcpyDeviceToHost, s->stream));
     return SSW_OK;
 }
 
@@ -2962,6 +3025,10 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     TRY(dalloc(s, &s->d_level, n));
     TRY(dalloc(s, &s->d_ctr, 3));
     TRY(dalloc(s, &s->d_cnt, DN));
+    if (s->permuted) {
+        TRY(dalloc(s, &s->d_inv, n));
+        HIP_TRY(hipMemcpyAsync(s->d_inv, s->inv.data(), n * 4, hipMemcpyHostToDevice, s->stream));
+    }
     HIP_TRY(hipMemcpyAsync(s->d_off, off32.data(), (n + 1) * 4, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_geo, geo.data(), F * sizeof(double4), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->d_nk, nk.data(), F * 4, hipMemcpyHostToDevice, s->stream));
@@ -3112,10 +3179,10 @@ int ssw_download(ssw_sweep* s, double* x, double* T, double* sts, double* photon
         Geo g = make_geo(s);
         if (s->fp32)
             k_fluxes<float><<<grid_for(n), kBlock, 0, s->stream>>>(g, make_flux<float>(s), s->lref, nullptr, nullptr,
-                                                                  nullptr, d_pr);
+                                                                  nullptr, d_pr, nullptr);
         else
             k_fluxes<double><<<grid_for(n), kBlock, 0, s->stream>>>(g, make_flux<double>(s), s->lref, nullptr,
-                                                                   nullptr, nullptr, d_pr);
+                                                                   nullptr, nullptr, d_pr, nullptr);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(photon, d_pr, n * 8, hipMemcpyDeviceToHost, s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
@@ -3128,6 +3195,39 @@ int ssw_download(ssw_sweep* s, double* x, double* T, double* sts, double* photon
     if (change) to_caller(s, change, 1);
     if (level) to_caller(s, level, 1);
     if (photon) to_caller(s, photon, 1);
+    return SSW_OK;
+}
+
+int ssw_download_noalloc(ssw_sweep* s, double* x, double* T, double* sts, double* photon, uint8_t* level,
+                         double* change) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    if (!s->have_state) return fail(SSW_ERR_INVALID, "ssw_upload_state must be called first");
+    const int64_t n = s->N;
+    if ((size_t)n * 8 > kNoAllocBytes)
+        return fail(SSW_ERR_INVALID, "ssw_download_noalloc: " + std::to_string(n) + " cells exceed its limit of " +
+                                         std::to_string(kNoAllocBytes / 8) + " cells (use ssw_download)");
+    uint8_t* stage = nullptr;
+    TRY(noalloc_scratch(s->device, &stage));
+    if (x) TRY(noalloc_out(s, s->d_x, x, stage));
+    if (T) TRY(noalloc_out(s, s->d_T, T, stage));
+    if (sts) TRY(noalloc_out(s, s->d_sts, sts, stage));
+    if (change) TRY(noalloc_out(s, s->d_change, change, stage));
+    if (level) TRY(noalloc_out(s, s->d_level, level, stage));
+    if (photon) {
+        // the photon rate of every cell lands at its caller index in the staging buffer
+        const Geo g = make_geo(s);
+        double* pr = (double*)stage;
+        const uint32_t* ridx = s->permuted ? s->d_inv : nullptr;
+        if (s->fp32)
+            k_fluxes<float><<<grid_for(n), kBlock, 0, s->stream>>>(g, make_flux<float>(s), s->lref, nullptr, nullptr,
+                                                                  nullptr, pr, ridx);
+        else
+            k_fluxes<double><<<grid_for(n), kBlock, 0, s->stream>>>(g, make_flux<double>(s), s->lref, nullptr,
+                                                                   nullptr, nullptr, pr, ridx);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(photon, pr, (size_t)n * 8, hipMemcpyDeviceToHost, s->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
     return SSW_OK;
 }
 
@@ -3219,10 +3319,11 @@ int ssw_download_fluxes(ssw_sweep* s, double* incoming, double* outgoing, double
     TRY(dalloc(s, &dP, m));
     Geo g = make_geo(s);
     if (s->fp32)
-        k_fluxes<float><<<grid_for(s->N), kBlock, 0, s->stream>>>(g, make_flux<float>(s), s->lref, dI, dO, dP, nullptr);
+        k_fluxes<float><<<grid_for(s->N), kBlock, 0, s->stream>>>(g, make_flux<float>(s), s->lref, dI, dO, dP, nullptr,
+                                                                  nullptr);
     else
         k_fluxes<double><<<grid_for(s->N), kBlock, 0, s->stream>>>(g, make_flux<double>(s), s->lref, dI, dO, dP,
-                                                                  nullptr);
+                                                                  nullptr, nullptr);
     HIP_TRY(hipGetLastError());
     if (incoming) HIP_TRY(hipMemcpyAsync(incoming, dI, m * 8, hipMemcpyDeviceToHost, s->stream));
     if (outgoing) HIP_TRY(hipMemcpyAsync(outgoing, dO, m * 8, hipMemcpyDeviceToHost, s->stream));
@@ -3531,9 +3632,10 @@ int64_t ssw_estimate_device_bytes(const ssw_sweep_parameters* p, int64_t n, int6
     const int64_t DN = D * n;
     const bool comp = mode == SSW_TRANSPORT_MODE_COMPILED;
     const int64_t ovf = comp ? DN * es / 16 : 0;   // overflow entries (bound used for the estimate)
-    // held from the grid upload on: grid arrays, level starts, chunk table
+    // held from the grid upload on: grid arrays (with the caller-order map of
+    // a permuted grid), level starts, chunk table
     const int64_t grid = (n + 1) * 4 + F * 32 + F * 4 + n * 16 + n * 4 + num_periodic_cells * 8 + F * 4 + n +
-                         (n + 4) * 4 + DN / kChunk * 40 + (1 << 20);
+                         (n + 4) * 4 + n * 4 + DN / kChunk * 40 + (1 << 20);
     const int64_t sched = comp ? DN * (16 + rs + kPlanes * es + 4) + ovf : DN * 16;  // compiled: + positions
     // compiled replay with the default grouped replay (4 level sets, 64 roots):
     // group lists instead of active-set lists, and the cone build's scratch

@@ -137,13 +137,16 @@ class Sweep:
         N.check(self._lib.ssw_step(self._h, C.byref(el)))
         return el.value
 
-    def results(self, photon_rate: bool = True) -> SweepResults:
+    def results(self, photon_rate: bool = True, noalloc: bool = False) -> SweepResults:
+        """The per-cell outputs (`ssw_download`; with `noalloc`,
+        `ssw_download_noalloc`, which stages them through the library's fixed
+        per-device buffer instead of allocating)."""
         n = self.num_cells
         x, T, ts, pr, ch = (np.empty(n) for _ in range(5))
         lev = np.empty(n, np.uint8)
-        N.check(self._lib.ssw_download(self._h, N.ptr(x, C.c_double), N.ptr(T, C.c_double), N.ptr(ts, C.c_double),
-                                       N.ptr(pr, C.c_double) if photon_rate else None, N.ptr(lev, C.c_uint8),
-                                       N.ptr(ch, C.c_double)))
+        fn = self._lib.ssw_download_noalloc if noalloc else self._lib.ssw_download
+        N.check(fn(self._h, N.ptr(x, C.c_double), N.ptr(T, C.c_double), N.ptr(ts, C.c_double),
+                   N.ptr(pr, C.c_double) if photon_rate else None, N.ptr(lev, C.c_uint8), N.ptr(ch, C.c_double)))
         return SweepResults(x, T, ts, pr if photon_rate else None, lev, ch)
 
     def chemistry_outputs(self) -> dict:
