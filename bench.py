@@ -1,29 +1,36 @@
 """Sweep benchmark: cell-direction updates per second on one MI355X.
 
-Workload (one "step" = one Sweep::run_sweeps call, src/sweep/mod.rs:258-272):
-262144 cells, the periodic Voronoi tessellation of a 64^3 jittered lattice in
-a 4000 kpc box, lognormal density (sigma_ln 1.5, mean 1e-27 g/cm^3), 64 point
-sources with luminosities 10^U(48, 51) photons/s, 84 directions, 1 timestep
-level, significant_rate_threshold 1e-5 s^-1, max timestep 1 Myr. Inputs are
+Headline workload (one "step" = one Sweep::run_sweeps call,
+src/sweep/mod.rs:258-272): the 2097152-cell clustered grid, the periodic
+Voronoi tessellation of points sampled from a lognormal density field
+(Gaussian random field with P(k) ~ k^-2 on a 128^3 mesh, smoothed on 2 mesh
+cells, sigma_ln 2, so that cell volumes span about 1e3), 1024 sources on the
+densest cells with luminosities from p(L) ~ L^-2 on [1e48, 1e52] photons/s
+times an escape fraction of 0.01, 84 directions, 4 timestep levels,
+timestep_safety_factor 0.1, FP32 transport with FP64 chemistry, replays
+captured as HIP graphs, launch shape chosen by the autotuner. Inputs are
 synthetic and seeded; grid and state are resident on the device before the
 timed region starts.
 
 `value` is the FP32-transport + FP64-chemistry throughput; the FP64-transport
-throughput of the same workload is reported beside it. The launch shape of
-each (replay form, block size, graph replay) is the autotuner's choice. The
-roofline figure is the algorithmic byte count of the transport per
-cell-direction update (8+8 B incoming/periodic reads, 16 B outgoing
-read/write, 16 B per-cell absorbed-rate accumulation, and per downwind face
-36/28/12 B for local/periodic/boundary faces in FP64, 24/16/8 B and 4+4+8 B in
-FP32) times the updates of one launch of the transport kernel, divided by its
-average launch duration: HIP events recorded on the library's stream around
-each sweep's whole replay in extra steps run right after the timed steps
-(the timed steps carry no events), divided by the launched level sets.
+throughput of the same workload is reported beside it. The roofline figure
+is the algorithmic byte count of the transport per cell-direction update
+(8+8 B incoming/periodic reads, 16 B outgoing read/write, 16 B per-cell
+absorbed-rate accumulation, and per downwind face 36/28/12 B for
+local/periodic/boundary faces in FP64, 24/16/8 B and 4+4+8 B in FP32) times
+the updates of one launch of the transport kernel, divided by its average
+launch duration: HIP events recorded on the library's stream around each
+sweep's whole replay in extra steps run right after the timed steps (the
+timed steps carry no events), divided by the launched level-set groups.
 `traffic` is the HBM bytes per launch from the committed rocprofv3
-FETCH_SIZE / WRITE_SIZE summary. cpu_baseline times the oracle (the
-single-threaded CPU restatement of the reference's sweep) on one step of the
-same workload. The 2097152-cell clustered workload (FP32 vs FP64) and the
-serving loop (p50/p99 latency) are reported in the same JSON line.
+FETCH_SIZE / WRITE_SIZE summary of the same kernel on the same workload.
+cpu_baseline times the oracle (the single-threaded CPU restatement of the
+reference's sweep) on one thread pinned to one core, on a bounded sample of
+the same workload: the first step's sweep over 16 of the 84 directions.
+The 262144-cell precision workload (FP32 vs FP64, with the FP32 error
+against the FP64 oracle over its 20 steps from the committed GPU test
+report) and the serving loop (p50/p99 latency) are reported in the same
+JSON line.
 """
 
 from __future__ import annotations
@@ -100,6 +107,7 @@ def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3,
         "ms_per_step": elapsed / steps * 1e3,
         "updates_per_step": updates / steps,
         "avg_launch_ms": avg_launch_ms,
+        "updates_per_launch": upd_per_launch,
         "launches_per_step": launches / max(1, profile_steps),
         "kernel_ms_per_step": kernel_ms / max(1, profile_steps),
         "bytes_per_update": bpu,
@@ -213,123 +221,131 @@ def tuned(pb, precision: int, warmup: int, enabled: bool, mode: int, block: int)
     return r.kwargs(), [list(x) for x in r.table]
 
 
-def run_clustered(steps: int = 4, warmup: int = 2, autotune: bool = True):
-    """2097152-cell clustered grid, 84 directions, 4 timestep levels, 1024
-    sources: FP32 transport vs FP64 transport (FP64 chemistry in both). The
-    warmup steps are the two first steps, which sweep only the top level."""
+def run_precision(warmup: int, steps: int = 20, autotune: bool = True):
+    """The 262144-cell precision workload (64^3 jittered lattice, sigma_ln 1.5,
+    64 sources, 84 directions, 1 level, threshold 1e-5 s^-1): FP32 transport
+    vs FP64 transport (FP64 chemistry in both), `steps` timed steps each, and
+    the FP32 error against the FP64 oracle after its 20 steps from the
+    committed report of the GPU parity test that computes it."""
     from subsweep_amd import workloads as W
     t = time.perf_counter()
-    pb = W.clustered_2m()
-    log(f"clustered grid: {pb.grid.num_cells} cells, {pb.grid.num_faces} faces, built in "
+    pb = W.precision_64()
+    log(f"precision grid: {pb.grid.num_cells} cells, {pb.grid.num_faces} faces, built in "
         f"{time.perf_counter() - t:.1f} s")
     sh32, t32 = tuned(pb, 32, warmup, autotune, 0, 0)
-    r32 = run_gpu(pb, 32, steps, warmup, profile_steps=1, shape=sh32)
-    log(f"clustered fp32: {r32['updates_per_s'] / 1e9:.3f} G updates/s, {r32['ms_per_step']:.1f} ms/step")
+    r32 = run_gpu(pb, 32, steps, warmup, shape=sh32)
+    log(f"precision fp32: {r32['updates_per_s'] / 1e9:.3f} G updates/s, {r32['ms_per_step']:.2f} ms/step, "
+        f"kernel {r32['avg_launch_ms']:.4f} ms/launch, {r32['achieved_gbs']:.0f} GB/s")
     sh64, t64 = tuned(pb, 64, warmup, autotune, 0, 0)
-    r64 = run_gpu(pb, 64, steps, warmup, profile_steps=1, shape=sh64)
-    log(f"clustered fp64: {r64['updates_per_s'] / 1e9:.3f} G updates/s, {r64['ms_per_step']:.1f} ms/step")
+    r64 = run_gpu(pb, 64, steps, warmup, shape=sh64)
+    log(f"precision fp64: {r64['updates_per_s'] / 1e9:.3f} G updates/s, {r64['ms_per_step']:.2f} ms/step")
+    report = None
+    rpath = os.path.join(ROOT, "profiles", "precision_64_errors.json")
+    if os.path.exists(rpath):
+        report = json.load(open(rpath))
     return {
-        "workload": "2097152-cell clustered lognormal-field Voronoi grid (sigma_ln 2), 84 directions, "
-                    "4 timestep levels, 1024 sources, escape fraction 0.01",
+        "workload": "262144-cell jittered-lattice periodic Voronoi grid (sigma_ln 1.5), 84 directions, 1 timestep "
+                    "level, 64 sources, threshold 1e-5 s^-1",
         "steps": steps,
-        "warmup": warmup,
         "dag_depth": int(r32["info"].max_depth),
         "fp32_updates_per_s": r32["updates_per_s"],
         "fp64_updates_per_s": r64["updates_per_s"],
         "fp32_over_fp64": r32["updates_per_s"] / r64["updates_per_s"],
         "fp32_ms_per_step": r32["ms_per_step"],
         "fp64_ms_per_step": r64["ms_per_step"],
-        "fp32_transport_mode": r32["transport_mode"],
-        "fp64_transport_mode": r64["transport_mode"],
-        "fp32_replay_block": r32["replay_block"],
-        "fp64_replay_block": r64["replay_block"],
-        "fp32_group_levels": r32["group_levels"],
-        "fp64_group_levels": r64["group_levels"],
-        "autotune_fp32_ms_per_step": t32,
-        "autotune_fp64_ms_per_step": t64,
-        "fp32_error_vs_fp64": field_errors(r32["results"], r64["results"]),
+        "fp32_roofline_frac": r32["achieved_gbs"] / PEAK_HBM_GBS,
+        "fp32_avg_launch_ms": r32["avg_launch_ms"],
+        "launch_shape_fp32": shape_of(r32, t32),
+        "launch_shape_fp64": shape_of(r64, t64),
+        "fp32_error_vs_fp64_gpu": field_errors(r32["results"], r64["results"]),
+        "error_vs_fp64_oracle_after_20_steps": report,
     }
 
 
-def run_cpu_baseline(pb, budget_s: float = 30.0):
+def shape_of(r, table):
+    return {"transport_mode": r["transport_mode"], "replay_block": r["replay_block"],
+            "graph_replay": r["graph_replay"], "narrow_runs": r["narrow_runs"], "group_levels": r["group_levels"],
+            "group_block": r["group_block"], "autotune_ms_per_step": table}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_cpu_baseline(pb, directions: int = 16):
+    """The oracle on one thread pinned to one core: the first step of the
+    workload (one sweep of every cell, then chemistry) over the first
+    `directions` rows of its direction table, a bounded sample of the same
+    work (updates/s is per cell-direction update either way)."""
     from oracle.oracle import OracleSweep
-    o = OracleSweep(pb.params, pb.grid, *pb.state())
-    t0 = time.perf_counter()
-    o.run_sweeps()
-    dt = time.perf_counter() - t0
+    from subsweep_amd import direction_table
+    D = pb.params.directions if isinstance(pb.params.directions, int) else len(pb.params.directions)
+    sub = pb.with_params(directions=direction_table(D)[:directions].tolist())
+    cores = sorted(os.sched_getaffinity(0))
+    core = cores[0]
+    o = OracleSweep(sub.params, sub.grid, *sub.state(), threads=1)
+    old = os.sched_getaffinity(0)
+    os.sched_setaffinity(0, {core})  # this thread only (pid 0 = the calling thread)
+    try:
+        t0 = time.perf_counter()
+        o.run_sweeps()
+        dt = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, old)
     upd = o.updates_last_step()
     o.close()
     return {"value": upd / dt, "unit": "cell-direction updates/s", "cores": 1, "kind": "port",
-            "sample": f"1 step ({upd} cell-direction updates) of the same workload, FP64, single thread"}
-
-
-def serving_pool(num_grids: int, seed: int):
-    """Grids of the serving workload: sizes log-uniform in [8192, 524288] cells,
-    alternately jittered-lattice Voronoi (n^3 cells, n = round(cbrt(size)))
-    and clustered lognormal-field Voronoi."""
-    import numpy as np
-    from subsweep_amd import grid as G
-    from subsweep_amd import workloads as W
-    rng = np.random.default_rng(seed)
-    pool = []
-    for i in range(num_grids):
-        size = int(np.exp(rng.uniform(np.log(8192), np.log(524288))))
-        if i % 2 == 0:
-            n = max(2, int(round(size ** (1.0 / 3.0))))
-            pts = G.jittered_lattice(n, W.BOX, 0.3, seed + 10 + i)
-            delta = None
-        else:
-            pts, delta = G.clustered_points(size, W.BOX, mesh=64, sigma_ln=2.0, seed=seed + 10 + i)
-        pool.append((G.voronoi_grid(pts, W.BOX), pts, delta))
-    return pool
-
-
-def serving_requests(pool, count: int, seed: int, steps: int):
-    """Request mix: a pool grid, directions from {16, 32, 64, 84}, timestep
-    levels from {1, 2, 4}, 1-64 sources of 1e50 photons/s at uniform positions,
-    lognormal gas (sigma_ln 1) on jittered grids and the field's density on
-    clustered grids, FP32 transport, `steps` steps."""
-    import numpy as np
-    from subsweep_amd import SweepParameters
-    from subsweep_amd import grid as G
-    from subsweep_amd import units as U
-    from subsweep_amd import workloads as W
-    rng = np.random.default_rng(seed)
-    out = []
-    for k in range(count):
-        grid, pts, delta = pool[int(rng.integers(len(pool)))]
-        N = grid.num_cells
-        dirs = int(rng.choice([16, 32, 64, 84]))
-        levels = int(rng.choice([1, 2, 4]))
-        ns = int(rng.integers(1, 65))
-        src, _ = G.assign_sources(pts, rng.uniform(0.0, W.BOX, size=(ns, 3)), np.full(ns, 1e50))
-        rho = W.RHO_MEAN * delta if delta is not None else G.lognormal_density(N, W.RHO_MEAN, 1.0, seed + 1000 + k)
-        params = SweepParameters(directions=dirs, num_timestep_levels=levels, periodic=True,
-                                 max_timestep=1.0 * U.MEGAYEARS, transport_precision=32)
-        out.append(W.Problem(f"request{k}", grid, params, rho, np.full(N, 1e-10), np.full(N, 1e4), src, steps))
-    return out
+            "cpu_model": cpu_model(), "pinned_core": core, "seconds": dt,
+            "sample": f"first step of the same workload (every cell on the top timestep level, one sweep then "
+                      f"chemistry) over {directions} of its {D} directions: {upd} cell-direction updates, FP64, "
+                      f"one thread pinned to core {core}"}
 
 
 def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, steps: int = 10):
+    """The serving loop on one server: first all `count` requests submitted at
+    once (saturated throughput; grids become resident as they first arrive),
+    then the same requests with Poisson arrivals at 0.8 x that completion rate
+    (latency at 80 % load; grids resident from the first pass)."""
     import threading
     import numpy as np
     from subsweep_amd.serve import Server
+    from subsweep_amd import workloads as W
     t = time.perf_counter()
-    pool = serving_pool(8, seed)
-    reqs = serving_requests(pool, count, seed + 1, steps)
+    pool = W.serving_pool(8, seed)
+    reqs = W.serving_requests(pool, count, seed + 1, steps)
     log(f"serving: {len(pool)} grids ({', '.join(str(g.num_cells) for g, _, _ in pool)} cells), "
         f"{count} requests built in {time.perf_counter() - t:.1f} s")
     budget = int(budget_gb * (1 << 30))
-    # capacity: a saturated batch of 8 requests on the same server shape
+
+    def summary(res, span):
+        bad = [r for r in res if r.status != 0]
+        if bad:
+            raise RuntimeError(f"{len(bad)} serving requests failed: {bad[0].error}")
+        lat = np.array([r.latency_s for r in res])
+        return {"p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
+                "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
+                "mean_service_ms": float(np.mean([r.end_s - r.start_s for r in res]) * 1e3),
+                "updates_per_s": sum(r.updates for r in res) / span,
+                "reused_grid_fraction": float(np.mean([r.reused for r in res]))}
+
     with Server(slots=slots, hbm_budget_bytes=budget) as sv:
         t0 = time.perf_counter()
-        cal = [sv.wait(x) for x in [sv.submit(r, steps) for r in reqs[:8]]]
-        capacity = len(cal) / (time.perf_counter() - t0)
-    log(f"serving: calibration batch of {len(cal)} requests at {capacity:.2f} requests/s")
-    rate = 0.8 * capacity  # Poisson arrivals at 80 % of the measured capacity
-    gaps = np.random.default_rng(seed + 2).exponential(1.0 / rate, size=count)
-    arrivals = np.cumsum(gaps) - gaps[0]
-    with Server(slots=slots, hbm_budget_bytes=budget) as sv:
+        tickets = [sv.submit(r, steps, grid_key=r.pool_index + 1) for r in reqs]
+        sat = [sv.wait(x) for x in tickets]
+        span = time.perf_counter() - t0
+        saturated = summary(sat, span)
+        capacity = count / span
+        log(f"serving saturated: {capacity:.1f} requests/s, {saturated['updates_per_s'] / 1e9:.3f} G updates/s, "
+            f"{saturated['reused_grid_fraction']:.2f} of the requests on resident grids")
+        rate = 0.8 * capacity  # Poisson arrivals at 80 % of the measured capacity
+        gaps = np.random.default_rng(seed + 2).exponential(1.0 / rate, size=count)
+        arrivals = np.cumsum(gaps) - gaps[0]
         tickets = [None] * count
         start = time.perf_counter()
 
@@ -338,7 +354,7 @@ def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, step
                 delay = arrivals[i] - (time.perf_counter() - start)
                 if delay > 0:
                     time.sleep(delay)
-                tickets[i] = sv.submit(r, steps)
+                tickets[i] = sv.submit(r, steps, grid_key=r.pool_index + 1)
 
         th = threading.Thread(target=feeder)
         th.start()
@@ -347,28 +363,27 @@ def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, step
             while tickets[i] is None:
                 time.sleep(0.01)
             res.append(sv.wait(tickets[i]))
-            if (i + 1) % 32 == 0:
+            if (i + 1) % 64 == 0:
                 log(f"serving: {i + 1}/{count} done at {time.perf_counter() - start:.1f} s")
         th.join()
         stats = sv.stats()
-    bad = [r for r in res if r.status != 0]
-    if bad:
-        raise RuntimeError(f"{len(bad)} serving requests failed: {bad[0].error}")
-    lat = np.array([r.latency_s for r in res])
     t_first = min(r.submit_s for r in res)
     t_last = max(r.end_s for r in res)
-    updates = sum(r.updates for r in res)
+    poisson = summary(res, t_last - t_first)
     return {
         "requests": count,
         "slots": slots,
         "hbm_budget_gb": budget_gb,
         "steps_per_request": steps,
+        "grids": [int(g.num_cells) for g, _, _ in pool],
         "arrival_rate_per_s": rate,
-        "arrival_rule": "Poisson at 0.8 x the completion rate of a saturated 8-request batch",
-        "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
-        "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
-        "mean_service_ms": float(np.mean([r.end_s - r.start_s for r in res]) * 1e3),
-        "updates_per_s": updates / (t_last - t_first),
+        "arrival_rule": "Poisson at 0.8 x the completion rate of the saturated pass",
+        "p50_latency_ms": poisson["p50_latency_ms"],
+        "p99_latency_ms": poisson["p99_latency_ms"],
+        "mean_service_ms": poisson["mean_service_ms"],
+        "updates_per_s": poisson["updates_per_s"],
+        "reused_grid_fraction": poisson["reused_grid_fraction"],
+        "saturated": saturated,
         "peak_running": stats["peak_running"],
         "peak_reserved_gb": stats["peak_reserved_bytes"] / (1 << 30),
     }
@@ -383,12 +398,14 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--precision", type=int, default=0, choices=(0, 32, 64),
-                    help="run only this transport precision (0: both)")
+                    help="profiling run: only this transport precision of --workload, no JSON line")
+    ap.add_argument("--workload", default="clustered", choices=("clustered", "precision"),
+                    help="workload of a --precision profiling run")
     ap.add_argument("--serving-requests", type=int, default=256,
                     help="requests of the serving-loop measurement (0: skip it)")
     ap.add_argument("--serving-slots", type=int, default=4)
-    ap.add_argument("--no-clustered", action="store_true",
-                    help="skip the 2097152-cell clustered FP32-vs-FP64 measurement")
+    ap.add_argument("--no-precision-workload", action="store_true",
+                    help="skip the 262144-cell FP32-vs-FP64 precision workload")
     ap.add_argument("--largest", action="store_true",
                     help="only run the 16777216-cell largest-domain measurement and print its JSON")
     ap.add_argument("--mode", type=int, default=0, choices=(0, 3, 4),
@@ -426,46 +443,48 @@ def main():
 
     from subsweep_amd import workloads as W
     t = time.perf_counter()
-    pb = W.precision_64()
+    pb = W.precision_64() if (args.precision and args.workload == "precision") else W.clustered_2m()
     log(f"grid: {pb.grid.num_cells} cells, {pb.grid.num_faces} faces, built in {time.perf_counter() - t:.1f} s")
-
-    tune = not args.no_autotune and args.precision == 0
-    tune32 = tune64 = None
-    if args.precision in (0, 32):
-        sh, tune32 = tuned(pb, 32, args.warmup, tune, args.mode, args.block)
-        fp32 = run_gpu(pb, 32, args.steps, args.warmup, shape=sh)
-        log(f"fp32 transport: {fp32['updates_per_s'] / 1e9:.3f} G updates/s, {fp32['ms_per_step']:.2f} ms/step, "
-            f"kernel {fp32['avg_launch_ms']:.3f} ms/launch, {fp32['achieved_gbs']:.0f} GB/s")
-    if args.precision in (0, 64):
-        sh, tune64 = tuned(pb, 64, args.warmup, tune, args.mode, args.block)
-        fp64 = run_gpu(pb, 64, args.steps, args.warmup, shape=sh)
-        log(f"fp64 transport: {fp64['updates_per_s'] / 1e9:.3f} G updates/s, {fp64['ms_per_step']:.2f} ms/step, "
-            f"kernel {fp64['avg_launch_ms']:.3f} ms/launch, {fp64['achieved_gbs']:.0f} GB/s")
-    if args.precision != 0:  # profiling runs: no JSON line
+    if args.precision:  # profiling run: one precision, default launch shape unless tuned, no JSON line
+        sh, _ = tuned(pb, args.precision, args.warmup, not args.no_autotune, args.mode, args.block)
+        r = run_gpu(pb, args.precision, args.steps, args.warmup, shape=sh)
+        log(f"fp{args.precision} transport: {r['updates_per_s'] / 1e9:.3f} G updates/s, {r['ms_per_step']:.2f} "
+            f"ms/step, kernel {r['avg_launch_ms']:.4f} ms/launch, {r['achieved_gbs']:.0f} GB/s")
         return
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = run_cpu_baseline(pb)
+        log(f"cpu baseline (oracle, 1 thread on core {cpu['pinned_core']}, {cpu['cpu_model']}): "
+            f"{cpu['value'] / 1e6:.2f} M updates/s in {cpu['seconds']:.1f} s")
+
+    tune = not args.no_autotune
+    sh32, tune32 = tuned(pb, 32, args.warmup, tune, args.mode, args.block)
+    fp32 = run_gpu(pb, 32, args.steps, args.warmup, shape=sh32)
+    log(f"fp32 transport: {fp32['updates_per_s'] / 1e9:.3f} G updates/s, {fp32['ms_per_step']:.2f} ms/step, "
+        f"kernel {fp32['avg_launch_ms']:.4f} ms/launch, {fp32['achieved_gbs']:.0f} GB/s")
+    sh64, tune64 = tuned(pb, 64, args.warmup, tune, args.mode, args.block)
+    fp64 = run_gpu(pb, 64, args.steps, args.warmup, shape=sh64)
+    log(f"fp64 transport: {fp64['updates_per_s'] / 1e9:.3f} G updates/s, {fp64['ms_per_step']:.2f} ms/step, "
+        f"kernel {fp64['avg_launch_ms']:.4f} ms/launch, {fp64['achieved_gbs']:.0f} GB/s")
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "transport_traffic.json")
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get("fp32_bytes_per_launch")
+            traffic = json.load(open(tpath)).get("clustered_2m_fp32_bytes_per_launch")
         except Exception:
             traffic = None
 
-    clustered = None
-    if not args.no_clustered:
-        clustered = run_clustered(autotune=not args.no_autotune)
+    precision = None
+    if not args.no_precision_workload:
+        precision = run_precision(args.warmup, autotune=tune)
 
     serving = None
     if args.serving_requests > 0:
         serving = run_serving(args.serving_requests, args.serving_slots, 200.0)
         log(f"serving: p50 {serving['p50_latency_ms']:.1f} ms, p99 {serving['p99_latency_ms']:.1f} ms, "
             f"{serving['updates_per_s'] / 1e9:.3f} G updates/s, peak {serving['peak_running']} running")
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = run_cpu_baseline(pb)
-        log(f"cpu baseline (oracle, 1 thread): {cpu['value'] / 1e6:.2f} M updates/s")
 
     info = fp32["info"]
     line = {
@@ -480,21 +499,26 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 transport, f64 chemistry",
-        "data": "synthetic (seeded jittered-lattice Voronoi grid, lognormal gas, random sources)",
+        "data": "synthetic (seeded clustered lognormal-field Voronoi grid, sources on the densest cells)",
         "config": {
-            "workload": "precision comparison: 64^3 = 262144-cell jittered-lattice periodic Voronoi grid, "
-                        "84 directions, 1 timestep level, 64 sources, threshold 1e-5 s^-1",
+            "workload": "clustered batched sweep: 2097152-cell clustered lognormal-field periodic Voronoi grid "
+                        "(sigma_ln 2, cell volumes spanning ~1e3), 1024 sources on the densest cells (p(L) ~ L^-2 "
+                        "on 1e48-1e52 photons/s, escape fraction 0.01), 84 directions, 4 timestep levels, "
+                        "timestep safety factor 0.1, FP32 transport + FP64 chemistry, graph-captured replays, "
+                        "autotuned launch shape",
             "cells": int(info.num_cells),
             "faces": int(info.num_faces),
             "directions": int(info.num_directions),
-            "timestep_levels": 1,
+            "timestep_levels": 4,
             "dag_depth": int(info.max_depth),
             "updates_per_step": fp32["updates_per_step"],
+            "sweeps_per_step": int(info.sweeps_last_step),
             "parallelism": "single GPU, whole domain resident",
         },
         "fp64_transport_value": fp64["updates_per_s"],
         "fp64_ms_per_step": fp64["ms_per_step"],
         "fp32_over_fp64": fp32["updates_per_s"] / fp64["updates_per_s"],
+        "fp32_error_vs_fp64": field_errors(fp32["results"], fp64["results"]),
         "roofline": {
             "bound": "hbm",
             "achieved": fp32["achieved_gbs"],
@@ -506,19 +530,14 @@ def main():
             "avg_launch_ms": fp32["avg_launch_ms"],
             "launches_per_step": fp32["launches_per_step"],
             "kernel_ms_per_step": fp32["kernel_ms_per_step"],
+            "updates_per_launch": fp32["updates_per_launch"],
             "bytes_per_update": fp32["bytes_per_update"],
             "fp64_achieved": fp64["achieved_gbs"],
             "fp64_bytes_per_update": fp64["bytes_per_update"],
         },
-        "launch_shape": {"fp32": {"replay_block": fp32["replay_block"], "graph_replay": fp32["graph_replay"],
-                                  "narrow_runs": fp32["narrow_runs"], "group_levels": fp32["group_levels"],
-                                  "transport_mode": fp32["transport_mode"], "autotune_ms_per_step": tune32},
-                         "fp64": {"replay_block": fp64["replay_block"], "graph_replay": fp64["graph_replay"],
-                                  "narrow_runs": fp64["narrow_runs"], "group_levels": fp64["group_levels"],
-                                  "transport_mode": fp64["transport_mode"], "autotune_ms_per_step": tune64}},
+        "launch_shape": {"fp32": shape_of(fp32, tune32), "fp64": shape_of(fp64, tune64)},
         "cpu_baseline": cpu,
-        "fp32_error_vs_fp64": field_errors(fp32["results"], fp64["results"]),
-        "clustered_2m": clustered,
+        "precision_262k": precision,
         "serving": serving,
     }
     print(json.dumps(line), flush=True)

@@ -145,6 +145,32 @@ int ssw_download_noalloc(ssw_sweep* s, double* ionized_fraction, double* tempera
 int ssw_download_chemistry_outputs(ssw_sweep* s, double* photoionization_rate, double* heating_rate,
                                    double* recombination_rate, double* collisional_ionization_rate);
 
+/* rotate_directions_system (src/sweep/direction/mod.rs:158-175), which the
+ * reference runs after every run_sweep_system call when
+ * SweepParameters::rotate_directions is set (src/sweep/mod.rs:143-148,
+ * src/sweep/parameters.rs:20-22): every direction d becomes matrix . d
+ * (matrix row-major 3 x 3, multiply_by_matrix :150-155), the dependency graph
+ * and schedule are rebuilt for the rotated directions, and the incoming,
+ * outgoing and periodic rates of every cell are remapped as the reference's
+ * remap (:190-205) does: its inner loop overwrites values[i] for every j, so
+ * direction i ends up with the old rate of the last direction if the last
+ * rotated direction is the one most aligned with old direction i (the last
+ * maximum of 1 + old_i . new_j, kernel_f :177-188), and with 0 otherwise.
+ * Later sweeps add their corrections to these rates, as the reference's
+ * accumulated rates do. A handle with rotated directions does not save or
+ * load checkpoints. */
+int ssw_rotate_directions(ssw_sweep* s, const double* matrix);
+
+/* get_rotation_matrix (src/sweep/direction/mod.rs:112-135): the row-major
+ * rotation by `angle` (radians) about the unit vector `axis`, into out[9]. */
+int ssw_rotation_matrix(const double* axis, double angle, double* out);
+
+/* get_random_rotation_matrix (src/sweep/direction/mod.rs:137-148) for its
+ * three random draws, phi in [0, 2 pi), u in [0, 1) and psi in [0, 2 pi)
+ * (the reference takes them from a StdRng seeded with 1337, :207-210): the
+ * rotation by psi about the axis of polar angle acos(2u - 1) and azimuth phi. */
+int ssw_random_rotation_matrix(double phi, double u, double psi, double* out);
+
 /* Waits until all work queued on the handle's stream has finished. */
 int ssw_synchronize(ssw_sweep* s);
 
@@ -258,9 +284,9 @@ int ssw_set_cell_positions(ssw_sweep* s, int64_t num_cells, const double* positi
 
 /* Replays of an unchanged active set are captured as one HIP graph from their
  * second use on (enabled = 1, default) or launched one level set at a time
- * (0). Capture needs the process to hold no other handle at that moment
- * (other threads' allocations invalidate a stream capture), so the request
- * loop turns it off. No counterpart in the reference. */
+ * (0). The library's own allocations, frees and stream set-up wait while a
+ * handle captures (they would invalidate the capture), so handles driven
+ * from several threads capture too. No counterpart in the reference. */
 int ssw_set_graph_replay(ssw_sweep* s, int32_t enabled);
 
 /* Compiled replay: each run of two or more consecutive level sets with at
@@ -331,6 +357,27 @@ double ssw_chemistry_timestep(double* state, double density, double volume, doub
 int64_t ssw_estimate_device_bytes(const ssw_sweep_parameters* params, int64_t num_cells, int64_t num_faces,
                                   int64_t num_periodic_cells, int32_t transport_mode);
 
+/* Source rates of star particles (host only), the reference's Arepo input
+ * path (src/arepo_postprocess/sources.rs:57-128); the rates then go to the
+ * nearest cells like any point source (src/source_systems.rs:48-80).
+ * ssw_bpass_source_rate: bpass_lookup (src/arepo_postprocess/bpass.rs:11-56),
+ *   ionising photons/s of a stellar population of `mass_solar` solar masses,
+ *   age (years) and metallicity (mass fraction) clamped into the table by a
+ *   factor 1.00001, bilinear in age and metallicity.
+ * ssw_time_between_scale_factors: CosmologyParams::time_difference_between_
+ *   scalefactors (src/cosmology.rs:80-107), seconds from scale factor a0 to
+ *   a1 in a flat cosmology (NaN if omega_lambda + omega_0 is off 1 by > 1e-2).
+ * ssw_star_sources: per particle, is_star = formation scale factor > 0 (the
+ *   others are wind particles, sources.rs:117-118) and rate = bpass(age from
+ *   the formation scale factor to `scale_factor`, metallicity, mass) x
+ *   escape_fraction, 0 for wind particles (new_bpass_source, :57-76).
+ *   Returns the number of stars or SSW_ERR_INVALID. */
+double ssw_bpass_source_rate(double age_years, double metallicity, double mass_solar);
+double ssw_time_between_scale_factors(double a0, double a1, double omega_lambda, double omega_0, double h);
+int64_t ssw_star_sources(int64_t n, const double* metallicity, const double* formation_scale_factor,
+                         const double* mass_solar, double escape_fraction, double omega_lambda, double omega_0,
+                         double h, double scale_factor, double* rate, uint8_t* is_star);
+
 /* Request loop (no counterpart in the reference, which runs one simulation
  * per process): a server with `slots` worker threads runs independent solves
  * concurrently, each in its own handle and HIP stream, so their kernels
@@ -338,10 +385,19 @@ int64_t ssw_estimate_device_bytes(const ssw_sweep_parameters* params, int64_t nu
  * worker is free and their device-memory estimate fits in what is left of
  * the HBM budget; a request above the whole budget fails with
  * SSW_ERR_BUDGET. Each request runs create, upload grid and state, `steps`
- * steps and a download into its output buffers (any may be NULL), then frees
- * its device memory. Input and output buffers must stay valid until
- * ssw_server_wait returns for the request. Results do not depend on what else
- * runs: every kernel's arithmetic is independent of scheduling. */
+ * steps and a download into its output buffers (any may be NULL). Input and
+ * output buffers must stay valid until ssw_server_wait returns for the
+ * request. Results do not depend on what else runs: every kernel's
+ * arithmetic is independent of scheduling.
+ *
+ * Resident grids: a request with a non-zero grid_key leaves its handle (grid,
+ * dependency graph, schedule and replay lists on the device) in the server's
+ * cache when it completes; a later request with the same grid_key and the
+ * same parameters, transport mode and replay block reuses it, uploading only
+ * its state (the caller promises that equal keys name equal grids and cell
+ * positions). Cached handles count against the HBM budget and are freed,
+ * least recently used first, when a request needs their memory. A request
+ * with grid_key 0 frees its handle at the end. */
 typedef struct ssw_server ssw_server;
 
 typedef struct ssw_request {
@@ -370,6 +426,7 @@ typedef struct ssw_request {
     double* out_photon_rate;
     uint8_t* out_level;
     double* out_change_timescale;
+    int64_t grid_key;              /* 0: no reuse; else the grid's identity    */
 } ssw_request;
 
 typedef struct ssw_request_result {
@@ -382,6 +439,8 @@ typedef struct ssw_request_result {
     int64_t device_bytes;          /* most device memory its handle held      */
     int64_t estimated_bytes;       /* the estimate it was admitted with       */
     char error[256];
+    int32_t reused;                /* 1 if it ran on a cached handle          */
+    int32_t reserved1;
 } ssw_request_result;
 
 int ssw_server_create(int32_t slots, int64_t hbm_budget_bytes, ssw_server** out);
@@ -389,7 +448,8 @@ int ssw_server_create(int32_t slots, int64_t hbm_budget_bytes, ssw_server** out)
 int ssw_server_submit(ssw_server* server, const ssw_request* request, int64_t* ticket);
 /* Blocks until the request has finished; each ticket can be waited on once. */
 int ssw_server_wait(ssw_server* server, int64_t ticket, ssw_request_result* result);
-/* Highest reserved budget and highest number of concurrently running requests so far. */
+/* Highest reserved budget (running requests' estimates plus cached handles)
+ * and highest number of concurrently running requests so far. */
 int ssw_server_stats(ssw_server* server, int64_t* peak_reserved_bytes, int32_t* peak_running);
 /* Runs every queued request to completion, then stops the workers. */
 void ssw_server_destroy(ssw_server* server);

@@ -64,6 +64,7 @@ def lib():
             "orc_chemistry_failures": (C.c_int64, [P]),
             "orc_num_directions": (C.c_int, [P]),
             "orc_directions": (None, [P, PD]),
+            "orc_rotate_directions": (C.c_int, [P, PD]),
             "orc_last_error": (C.c_char_p, []),
             "orc_solver_rates": (None, [C.c_double, C.c_double, C.c_double, C.c_double, PD]),
             "orc_solver_perform_timestep": (C.c_double, [PD, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -263,6 +264,12 @@ class OracleSweep:
         out = np.zeros(3, np.int64)
         lib().orc_limiting_processes(self._h, _p(out, C.c_int64))
         return {"temperature": int(out[0]), "ionization_fraction": int(out[1]), "photon_rate": int(out[2])}
+
+    def rotate_directions(self, matrix):
+        """rotate_directions_system (src/sweep/direction/mod.rs:158-175) with
+        the given 3 x 3 rotation matrix."""
+        m = np.ascontiguousarray(matrix, dtype=np.float64).reshape(9)
+        _check(lib().orc_rotate_directions(self._h, _p(m, C.c_double)))
 
     def directions(self):
         out = np.empty(3 * self.num_directions)

@@ -840,6 +840,53 @@ int orc_download_chemistry_outputs(void* h, double* photoionization, double* hea
     return SSW_OK;
 }
 
+// rotate_directions_system (src/sweep/direction/mod.rs:158-175) with the
+// rotation matrix given (row-major 3 x 3; the reference draws it from its
+// StdRng, get_random_rotation_matrix :137-148): every direction becomes
+// matrix . direction (multiply_by_matrix :150-155), then incoming, outgoing
+// and periodic rates of every site are remapped (remap :190-205). remap's
+// inner loop assigns values[i] = old[j] * kernel[i][j] for every j, so only
+// j = D-1 survives: values[i] = old[D-1] * kernel[i][D-1], where kernel[i][j]
+// is 1 if new direction j is the last of the new directions most aligned
+// with old direction i (max_by_key keeps the last maximum of 1 + old_i . new_j,
+// kernel_f :177-188) and 0 otherwise.
+int orc_rotate_directions(void* h, const double* m) {
+    Oracle* o = static_cast<Oracle*>(h);
+    if (!o || !m) return fail(SSW_ERR_INVALID, "null argument");
+    const int D = o->D;
+    std::vector<double> old_dirs = o->dirs;
+    for (int d = 0; d < D; d++) {
+        const double x = o->dirs[3 * d], y = o->dirs[3 * d + 1], z = o->dirs[3 * d + 2];
+        o->dirs[3 * d] = x * m[0] + y * m[1] + z * m[2];
+        o->dirs[3 * d + 1] = x * m[3] + y * m[4] + z * m[5];
+        o->dirs[3 * d + 2] = x * m[6] + y * m[7] + z * m[8];
+    }
+    std::vector<double> kept(D);  // kernel[i][D-1]
+    for (int i = 0; i < D; i++) {
+        int best = 0;
+        double best_v = 0.0;
+        for (int j = 0; j < D; j++) {
+            const double v = 1.0 + ((old_dirs[3 * i] * o->dirs[3 * j] + old_dirs[3 * i + 1] * o->dirs[3 * j + 1]) +
+                                    old_dirs[3 * i + 2] * o->dirs[3 * j + 2]);
+            if (j == 0 || v >= best_v) {  // Iterator::max_by_key: the last maximum
+                best = j;
+                best_v = v;
+            }
+        }
+        kept[i] = best == D - 1 ? 1.0 : 0.0;
+    }
+    if (o->have_state) {
+        for (std::vector<double>* arr : {&o->I, &o->O, &o->P}) {
+            std::vector<double>& a = *arr;
+            for (int64_t c = 0; c < o->N; c++) {
+                const double last = a[c * D + D - 1];
+                for (int i = 0; i < D; i++) a[c * D + i] = last * kept[i];
+            }
+        }
+    }
+    return SSW_OK;
+}
+
 int64_t orc_updates_last_step(void* h) { return static_cast<Oracle*>(h)->updates_last_step; }
 int64_t orc_chemistry_failures(void* h) { return static_cast<Oracle*>(h)->chemistry_failures; }
 int orc_num_directions(void* h) { return static_cast<Oracle*>(h)->D; }

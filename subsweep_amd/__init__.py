@@ -6,10 +6,12 @@ the reference's sweep plugin interface."""
 
 from ._native import SweepError, FACE_LOCAL, FACE_PERIODIC, FACE_BOUNDARY, TRANSPORT_FP32, TRANSPORT_FP64
 from .grid import Grid, cartesian_grid, voronoi_grid, jittered_lattice, lognormal_density, assign_sources
-from .sweep import Sweep, SweepParameters, SweepPlugin, SweepResults, direction_table
+from .sweep import (Sweep, SweepParameters, SweepPlugin, SweepResults, direction_table, random_rotation_matrix,
+                    rotation_matrix)
 
 __all__ = [
     "Grid", "Sweep", "SweepError", "SweepParameters", "SweepPlugin", "SweepResults",
     "cartesian_grid", "voronoi_grid", "jittered_lattice", "lognormal_density", "assign_sources",
     "FACE_LOCAL", "FACE_PERIODIC", "FACE_BOUNDARY", "TRANSPORT_FP32", "TRANSPORT_FP64", "direction_table",
+    "rotation_matrix", "random_rotation_matrix",
 ]

@@ -114,6 +114,7 @@ class RequestC(C.Structure):
         ("out_photon_rate", C.c_void_p),
         ("out_level", C.c_void_p),
         ("out_change_timescale", C.c_void_p),
+        ("grid_key", C.c_int64),
     ]
 
 
@@ -128,6 +129,8 @@ class RequestResultC(C.Structure):
         ("device_bytes", C.c_int64),
         ("estimated_bytes", C.c_int64),
         ("error", C.c_char * 256),
+        ("reused", C.c_int32),
+        ("reserved1", C.c_int32),
     ]
 
 
@@ -147,6 +150,9 @@ SWEEP_SIGNATURES = {
     "ssw_step": (C.c_int, [P, PD]),
     "ssw_download": (C.c_int, [P, PD, PD, PD, PD, PU8, PD]),
     "ssw_download_noalloc": (C.c_int, [P, PD, PD, PD, PD, PU8, PD]),
+    "ssw_rotate_directions": (C.c_int, [P, PD]),
+    "ssw_rotation_matrix": (C.c_int, [PD, C.c_double, PD]),
+    "ssw_random_rotation_matrix": (C.c_int, [C.c_double, C.c_double, C.c_double, PD]),
     "ssw_download_chemistry_outputs": (C.c_int, [P, PD, PD, PD, PD]),
     "ssw_download_counts": (C.c_int, [P, C.c_int32, PU32]),
     "ssw_download_levelsets": (C.c_int, [P, C.c_int32, PU32]),
@@ -174,6 +180,9 @@ SWEEP_SIGNATURES = {
     "ssw_last_error": (C.c_char_p, []),
     "ssw_estimate_device_bytes": (C.c_int64, [C.POINTER(SweepParametersC), C.c_int64, C.c_int64, C.c_int64,
                                               C.c_int32]),
+    "ssw_bpass_source_rate": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+    "ssw_time_between_scale_factors": (C.c_double, [C.c_double] * 5),
+    "ssw_star_sources": (C.c_int64, [C.c_int64, PD, PD, PD] + [C.c_double] * 5 + [PD, PU8]),
     "ssw_server_create": (C.c_int, [C.c_int32, C.c_int64, C.POINTER(P)]),
     "ssw_server_submit": (C.c_int, [P, C.POINTER(RequestC), PI64]),
     "ssw_server_wait": (C.c_int, [P, C.c_int64, C.POINTER(RequestResultC)]),

@@ -6,9 +6,16 @@
 // order, each when a worker is free and the device-memory estimate of the
 // request fits in what is left of the server's HBM budget; a request whose
 // estimate exceeds the whole budget fails with SSW_ERR_BUDGET. A request runs
-// create -> upload grid -> upload state -> `steps` x step -> download ->
-// destroy, so its grid and flux state stay resident on the device for the
-// whole solve, and its memory returns to the budget when it completes.
+// create -> upload grid -> upload state -> `steps` x step -> download, so its
+// grid and flux state stay resident on the device for the whole solve.
+//
+// Resident grids: a request with a non-zero grid_key leaves its handle in an
+// idle cache keyed by (grid_key, parameters, transport mode, replay block,
+// positions given); a later request with the same key takes it and only
+// uploads its state, skipping the grid upload, the Kahn layering and the
+// schedule and cone build. Cached handles hold their device memory against
+// the budget; when a request does not fit, idle handles are freed least
+// recently used first. Other requests free their handle when they complete.
 //
 // The reference has no counterpart (it runs one simulation per process); the
 // serving loop is how this library is driven with many independent solves.
@@ -31,6 +38,10 @@ namespace {
 
 struct Req {
     ssw_request r;
+    std::string key;          // cache key; empty for grid_key 0
+    std::vector<double> dirs;  // explicit directions, copied at submission
+    ssw_sweep* cached = nullptr;  // idle handle taken from the cache at admission
+    bool reused = false;
     int64_t ticket = 0;
     int64_t estimate = 0;
     int32_t mode = SSW_TRANSPORT_MODE_GEOMETRIC;
@@ -44,6 +55,46 @@ struct Req {
 
 double now_s(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// An idle handle in the cache and the device memory it holds.
+struct Idle {
+    ssw_sweep* s = nullptr;
+    int64_t bytes = 0;
+    uint64_t last_use = 0;
+};
+
+template <typename T>
+void put(std::string& k, const T& v) {
+    k.append(reinterpret_cast<const char*>(&v), sizeof(v));
+}
+
+// Everything a handle's device state was built from: the grid (by the
+// caller's key), the parameters (explicit directions by value), the
+// transport mode, the replay block and whether cell positions were given.
+std::string cache_key(const ssw_request& r, int32_t mode) {
+    if (r.grid_key == 0) return std::string();
+    std::string k;
+    put(k, r.grid_key);
+    put(k, r.num_cells);
+    const ssw_sweep_parameters& p = r.params;
+    put(k, p.num_directions);
+    if (p.explicit_directions) k.append(reinterpret_cast<const char*>(p.explicit_directions),
+                                        sizeof(double) * 3 * (size_t)std::max(0, p.num_directions));
+    put(k, p.num_timestep_levels);
+    put(k, p.periodic);
+    put(k, p.max_timestep);
+    put(k, p.significant_rate_threshold);
+    put(k, p.timestep_safety_factor);
+    put(k, p.chemistry_timestep_safety_factor);
+    put(k, p.prevent_cooling);
+    put(k, p.transport_precision);
+    put(k, p.scale_factor);
+    put(k, mode);
+    put(k, r.replay_block);
+    const int32_t pos = r.cell_position ? 1 : 0;
+    put(k, pos);
+    return k;
 }
 
 }  // namespace
@@ -63,29 +114,37 @@ struct ssw_server {
     int peak_running = 0;
     bool stop = false;
     std::vector<std::thread> workers;
+    // idle cached handles by key, and the device memory they hold
+    std::unordered_map<std::string, std::vector<Idle>> idle;
+    int64_t cached_bytes = 0;
+    uint64_t use_clock = 0;
 };
 
 namespace {
 
-void run_request(Req* q) {
-    ssw_sweep* s = nullptr;
+// Runs the request; returns its handle if it should go to the cache (else
+// the handle has been freed).
+ssw_sweep* run_request(Req* q) {
+    ssw_sweep* s = q->cached;
     const ssw_request& r = q->r;
     auto fail_with = [&](int code) {
         q->status = code;
         q->error = ssw_last_error();
     };
-    int rc = ssw_create(&r.params, &s);
-    if (rc != SSW_OK) {
-        fail_with(rc);
-        return;
+    int rc = SSW_OK;
+    if (!s) {
+        rc = ssw_create(&r.params, &s);
+        if (rc != SSW_OK) {
+            fail_with(rc);
+            return nullptr;
+        }
+        rc = ssw_set_transport_mode(s, q->mode);
+        if (rc == SSW_OK) rc = ssw_set_replay_block(s, r.replay_block);
+        if (rc == SSW_OK && r.cell_position) rc = ssw_set_cell_positions(s, r.num_cells, r.cell_position);
+        if (rc == SSW_OK)
+            rc = ssw_upload_grid(s, r.num_cells, r.face_offsets, r.face_neighbour, r.face_kind, r.face_area,
+                                 r.face_normal, r.cell_volume, r.cell_size);
     }
-    rc = ssw_set_transport_mode(s, q->mode);
-    if (rc == SSW_OK) rc = ssw_set_replay_block(s, r.replay_block);
-    if (rc == SSW_OK) rc = ssw_set_graph_replay(s, 0);  // handles of other requests allocate concurrently
-    if (rc == SSW_OK && r.cell_position) rc = ssw_set_cell_positions(s, r.num_cells, r.cell_position);
-    if (rc == SSW_OK)
-        rc = ssw_upload_grid(s, r.num_cells, r.face_offsets, r.face_neighbour, r.face_kind, r.face_area, r.face_normal,
-                             r.cell_volume, r.cell_size);
     if (rc == SSW_OK) rc = ssw_upload_state(s, r.density, r.ionized_fraction, r.temperature, r.source);
     ssw_info info;
     for (int32_t k = 0; rc == SSW_OK && k < r.steps; k++) {
@@ -100,10 +159,38 @@ void run_request(Req* q) {
     if (rc == SSW_OK) rc = ssw_get_info(s, &info);
     if (rc == SSW_OK) q->device_bytes = info.peak_device_bytes;
     if (rc != SSW_OK) fail_with(rc);
+    if (rc == SSW_OK && !q->key.empty()) return s;
     ssw_destroy(s);
+    return nullptr;
+}
+
+// Takes idle handles out of the cache, least recently used first, until
+// `need` more bytes fit in the budget or the cache is empty (the caller holds
+// sv->mu and frees the returned handles after releasing it).
+void evict_for(ssw_server* sv, int64_t need, std::vector<ssw_sweep*>& out) {
+    if (sv->reserved + need > sv->budget) return;  // running requests must finish first
+    while (sv->reserved + sv->cached_bytes + need > sv->budget && sv->cached_bytes > 0) {
+        auto best = sv->idle.end();
+        size_t bi = 0;
+        uint64_t oldest = UINT64_MAX;
+        for (auto it = sv->idle.begin(); it != sv->idle.end(); ++it)
+            for (size_t i = 0; i < it->second.size(); i++)
+                if (it->second[i].last_use < oldest) {
+                    oldest = it->second[i].last_use;
+                    best = it;
+                    bi = i;
+                }
+        if (best == sv->idle.end()) break;
+        Idle e = best->second[bi];
+        best->second.erase(best->second.begin() + (std::ptrdiff_t)bi);
+        if (best->second.empty()) sv->idle.erase(best);
+        sv->cached_bytes -= e.bytes;
+        out.push_back(e.s);
+    }
 }
 
 void worker(ssw_server* sv) {
+    std::vector<ssw_sweep*> evicted;
     for (;;) {
         Req* q = nullptr;
         {
@@ -122,27 +209,69 @@ void worker(ssw_server* sv) {
                         sv->cv_done.notify_all();
                         continue;
                     }
-                    if (sv->reserved + head->estimate <= sv->budget) {
+                    // a cached handle of the same key: its memory moves from
+                    // the cache into the request's reservation (and back if
+                    // the request cannot start yet)
+                    Idle hit;
+                    if (!head->key.empty()) {
+                        auto it = sv->idle.find(head->key);
+                        if (it != sv->idle.end() && !it->second.empty()) {
+                            hit = it->second.back();
+                            sv->cached_bytes -= hit.bytes;
+                            it->second.pop_back();
+                            if (it->second.empty()) sv->idle.erase(it);
+                        }
+                    }
+                    evict_for(sv, head->estimate, evicted);
+                    const bool fits = sv->reserved + sv->cached_bytes + head->estimate <= sv->budget;
+                    if (hit.s && !fits) {
+                        sv->idle[head->key].push_back(hit);
+                        sv->cached_bytes += hit.bytes;
+                    } else if (hit.s) {
+                        head->cached = hit.s;
+                        head->reused = true;
+                    }
+                    if (fits) {
                         sv->queue.pop_front();
                         q = head;
                         sv->reserved += q->estimate;
                         sv->running++;
-                        sv->peak_reserved = std::max(sv->peak_reserved, sv->reserved);
+                        sv->peak_reserved = std::max(sv->peak_reserved, sv->reserved + sv->cached_bytes);
                         sv->peak_running = std::max(sv->peak_running, sv->running);
                         q->t_start = now_s(sv->t0);
                         break;
                     }
                 }
+                if (!evicted.empty()) break;  // free them, then look again
                 sv->cv_work.wait(lk);
             }
         }
-        run_request(q);
+        for (ssw_sweep* e : evicted) ssw_destroy(e);
+        if (!q) {
+            evicted.clear();
+            continue;
+        }
+        evicted.clear();
+        ssw_sweep* keep = run_request(q);
+        int64_t held = 0;
+        if (keep) {
+            ssw_info info;
+            if (ssw_get_info(keep, &info) == SSW_OK) held = info.device_bytes;
+            else {
+                ssw_destroy(keep);
+                keep = nullptr;
+            }
+        }
         {
             std::lock_guard<std::mutex> lk(sv->mu);
             q->t_end = now_s(sv->t0);
             q->done = true;
             sv->reserved -= q->estimate;
             sv->running--;
+            if (keep) {
+                sv->idle[q->key].push_back(Idle{keep, held, ++sv->use_clock});
+                sv->cached_bytes += held;
+            }
         }
         sv->cv_done.notify_all();
         sv->cv_work.notify_all();
@@ -184,6 +313,12 @@ int ssw_server_submit(ssw_server* sv, const ssw_request* r, int64_t* ticket) {
         q->mode = comp <= sv->budget / sv->slots ? SSW_TRANSPORT_MODE_COMPILED : SSW_TRANSPORT_MODE_GEOMETRIC;
     }
     q->estimate = ssw_estimate_device_bytes(&r->params, r->num_cells, F, nper, q->mode);
+    q->key = cache_key(*r, q->mode);
+    q->r.params.explicit_directions = nullptr;  // read once, into the key; handles copy them at creation
+    if (r->params.explicit_directions) {
+        q->dirs.assign(r->params.explicit_directions, r->params.explicit_directions + 3 * r->params.num_directions);
+        q->r.params.explicit_directions = q->dirs.data();
+    }
     {
         std::lock_guard<std::mutex> lk(sv->mu);
         if (sv->stop) return SSW_ERR_INVALID;
@@ -213,6 +348,7 @@ int ssw_server_wait(ssw_server* sv, int64_t ticket, ssw_request_result* res) {
     res->updates = q->updates;
     res->device_bytes = q->device_bytes;
     res->estimated_bytes = q->estimate;
+    res->reused = q->reused ? 1 : 0;
     std::strncpy(res->error, q->error.c_str(), sizeof(res->error) - 1);
     sv->reqs.erase(it);
     return SSW_OK;
@@ -234,6 +370,8 @@ void ssw_server_destroy(ssw_server* sv) {
     }
     sv->cv_work.notify_all();
     for (std::thread& t : sv->workers) t.join();
+    for (auto& kv : sv->idle)
+        for (Idle& e : kv.second) ssw_destroy(e.s);
     delete sv;
 }
 

@@ -58,6 +58,7 @@
 #include <cstring>
 #include <functional>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -103,10 +104,14 @@ constexpr int kMaxDevices = 64;
 static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
 
 thread_local std::string g_err;
-// Handles alive in the process. Replay graphs are captured only while a single
-// handle exists: a stream capture is invalidated by the device-synchronising
-// calls (allocation, free, blocking copies) that other handles' threads make.
+// Handles alive in the process.
 std::atomic<int> g_live_handles{0};
+// A stream capture is invalidated by the device-synchronising calls
+// (allocation, free, blocking copies, stream creation and destruction) that
+// other threads make while it runs. Those calls hold this lock shared; a
+// capture holds it exclusively, so that handles driven from several threads
+// (the request loop) capture their replays too.
+std::shared_mutex g_capture_mu;
 
 int fail(int code, const std::string& msg) {
     g_err = msg;
@@ -191,6 +196,13 @@ struct Flux {
     const uint32_t* xpos;    // compiled replay: X of (c, d) at the task's schedule position xpos[d*N + c]
     R* I;                    // D x N
     R* P;                    // D x Nper
+    // After a rotation of the directions (ssw_rotate_directions), the rates
+    // that the remap of the state left in place, as offsets to the pull sums:
+    // the incoming rate of (c, d) is offI[d*N + c] + sum of the upwind X, the
+    // periodic source offP[d*Nper + slot] + its periodic sum. NULL before any
+    // rotation (offsets 0).
+    R* offI;                 // D x N
+    R* offP;                 // D x Nper
     const SA<R>* sa;         // N: {source / D in the storage scale, attenuation} (one load per task)
     double thr;              // threshold in the storage scale
     double inv_lref;         // storage scale of rates
@@ -270,6 +282,7 @@ struct CellIn {
     double src;
     double att;
     R per;
+    R off;  // incoming-rate offset after a rotation (0 before)
 };
 
 template <typename R>
@@ -282,6 +295,7 @@ __device__ __forceinline__ CellIn<R> cell_inputs(const Geo& g, const Flux<R>& fl
     // the periodic load uses slot 0 for cells without one (always a valid index)
     const R p = fl.P ? fl.P[(size_t)d * g.Nper + (ps != kNone ? ps : 0u)] : (R)0;
     ci.per = (ps != kNone) ? p : (R)0;
+    ci.off = fl.offI ? fl.offI[(size_t)d * g.N + c] : (R)0;
     return ci;
 }
 
@@ -291,7 +305,20 @@ __device__ __forceinline__ CellIn<R> cell_inputs(const Geo& g, const Flux<R>& fl
 template <typename R>
 __device__ __forceinline__ void finish_task(const Flux<R>& fl, size_t ox, size_t oi, R inc, const CellIn<R>& ci,
                                             R A) {
-    if (inc < (R)0) inc = 0;  // Photons::make_positive
+    if (fl.offI) {
+        // after a rotation the incoming rate is the remapped rate plus the
+        // corrections since, offset + upwind sum; its persistent clamp
+        // (Photons::make_positive) sets the offset to -sum, so that the rate
+        // is 0 now and grows by the later corrections only
+        R t = ci.off + inc;
+        if (t < (R)0) {
+            fl.offI[oi] = -inc;
+            t = 0;
+        }
+        inc = t;
+    } else if (inc < (R)0) {
+        inc = 0;  // Photons::make_positive
+    }
     R in = (inc + (R)ci.src) + ci.per;
     // the threshold in the storage scale is compared in double: in FP32 it is
     // threshold / lref (e.g. 1e-5 / 2^160), far below the float range
@@ -1231,7 +1258,7 @@ __global__ void k_periodic(Geo g, Flux<R> fl) {
     const uint32_t d = t - s * (uint32_t)g.D;
     if ((int64_t)s >= g.Nper) return;
     const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
-    R acc = 0;
+    R acc = fl.offP ? fl.offP[(size_t)d * g.Nper + s] : (R)0;
     for (uint32_t e = g.pfoff[s]; e < g.pfoff[s + 1]; e++) {
         const uint32_t f = g.pfidx[e];
         const uint32_t nk = g.nk[f];
@@ -1400,6 +1427,8 @@ __global__ void k_fluxes(Geo g, Flux<R> fl, double lref, double* I_out, double* 
                 else per += v;
             }
         }
+        if (fl.offI) inc = fl.offI[(size_t)d * g.N + c] + inc;
+        if (fl.offP && g.pslot[c] != kNone) per = fl.offP[(size_t)d * g.Nper + g.pslot[c]] + per;
         double inc_si = (double)inc * lref;
         total += inc_si;
         size_t o = (size_t)d * g.N + c;
@@ -1409,6 +1438,72 @@ __global__ void k_fluxes(Geo g, Flux<R> fl, double lref, double* I_out, double* 
     }
     // rate_idx: the caller's index of each internal cell (NULL: internal order)
     if (photon_rate) photon_rate[rate_idx ? rate_idx[c] : c] = total;
+}
+
+// rotate_directions_system (src/sweep/direction/mod.rs:158-175), state part,
+// one thread per (cell c, direction d) after the schedule of the rotated
+// directions is built: remap (:190-205) leaves every rate of direction d equal
+// to the old rate of the last direction times kept[d] (1 or 0), for incoming,
+// outgoing and periodic rates alike. Old rates (FP64, SI, [d*N + c]) come
+// from k_fluxes before the rotation. Writes X = O / A with the rotated
+// downwind area, I, the periodic source, and the remapped I and periodic
+// source into the offsets (k_rotate_offsets subtracts the pull sums).
+template <typename R>
+__global__ void k_rotate_state(Geo g, Flux<R> fl, const double* __restrict__ kept, const double* __restrict__ Iold,
+                               const double* __restrict__ Oold, const double* __restrict__ Pold) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)g.D * g.N) return;
+    const uint32_t d = (uint32_t)(t / g.N);
+    const uint32_t c = (uint32_t)(t - (int64_t)d * g.N);
+    const size_t last = (size_t)(g.D - 1) * g.N + c;
+    const double k = kept[d];
+    const double inc = Iold[last] * k, out = Oold[last] * k, per = Pold[last] * k;
+    const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+    double A = 0.0;
+    for (uint32_t f = g.off[c]; f < g.off[c + 1]; f++) {
+        const double4 a = g.geo[f];
+        const double dd = fdot(a, dx, dy, dz);
+        if (dd > 0.0) A += a.w * fl.inv_aref * dd;
+    }
+    const R Ar = (R)A;
+    const R o = (R)(out * fl.inv_lref);
+    fl.X[xidx(fl, g.N, d, c)] = (Ar > (R)0) ? o / Ar : (R)0;
+    const size_t oi = (size_t)d * g.N + c;
+    fl.I[oi] = (R)(inc * fl.inv_lref);
+    fl.offI[oi] = (R)(inc * fl.inv_lref);
+    const uint32_t ps = g.pslot[c];
+    if (ps != kNone && fl.P) {
+        fl.P[(size_t)d * g.Nper + ps] = (R)(per * fl.inv_lref);
+        fl.offP[(size_t)d * g.Nper + ps] = (R)(per * fl.inv_lref);
+    }
+}
+
+// The offsets of the rotated state: remapped rate minus the pull sum of the
+// rotated X over the upwind faces (local for I, periodic for the periodic
+// source), so that offset + sum gives the remapped rate back and grows by the
+// corrections of later solves, as the reference's accumulated rates do.
+template <typename R>
+__global__ void k_rotate_offsets(Geo g, Flux<R> fl) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)g.D * g.N) return;
+    const uint32_t d = (uint32_t)(t / g.N);
+    const uint32_t c = (uint32_t)(t - (int64_t)d * g.N);
+    const double dx = g.dirs[3 * d], dy = g.dirs[3 * d + 1], dz = g.dirs[3 * d + 2];
+    R inc = 0, per = 0;
+    for (uint32_t f = g.off[c]; f < g.off[c + 1]; f++) {
+        const double4 a = g.geo[f];
+        const uint32_t nk = g.nk[f];
+        const double dd = fdot(a, dx, dy, dz);
+        if (dd < 0.0 && (nk >> 30) != kKindBoundary) {
+            const R v = (R)(a.w * fl.inv_aref * -dd) * fl.X[xidx(fl, g.N, d, nk & kNbrMask)];
+            if ((nk >> 30) == kKindLocal) inc += v;
+            else per += v;
+        }
+    }
+    const size_t oi = (size_t)d * g.N + c;
+    fl.offI[oi] = fl.offI[oi] - inc;
+    const uint32_t ps = g.pslot[c];
+    if (ps != kNone && fl.P) fl.offP[(size_t)d * g.Nper + ps] = fl.offP[(size_t)d * g.Nper + ps] - per;
 }
 
 // dst[idx[i]] = src[i]: a per-cell array from the internal into the caller's order.
@@ -1596,6 +1691,8 @@ struct ssw_sweep {
     unsigned long long* d_limit = nullptr;          // [3] limiting processes of the last step
     uint8_t* d_level = nullptr;
     void *d_X = nullptr, *d_I = nullptr, *d_P = nullptr;
+    void *d_offI = nullptr, *d_offP = nullptr;      // rate offsets after a rotation (Flux::offI / offP)
+    bool rotated = false;                           // directions rotated since ssw_create
     unsigned long long* d_fail = nullptr;
     double lref = 1.0;
     TimestepState ts;
@@ -1616,6 +1713,7 @@ namespace {
 template <typename T>
 void dfree(ssw_sweep* s, T*& p) {
     if (p) {
+        std::shared_lock<std::shared_mutex> lock(g_capture_mu);
         auto it = s->alloc.find((void*)p);
         if (it != s->alloc.end()) {
             s->bytes -= (int64_t)it->second;
@@ -1630,7 +1728,11 @@ template <typename T>
 int dalloc(ssw_sweep* s, T** p, size_t count) {
     dfree(s, *p);
     if (count == 0) count = 1;
-    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    hipError_t e;
+    {
+        std::shared_lock<std::shared_mutex> lock(g_capture_mu);
+        e = hipMalloc((void**)p, count * sizeof(T));
+    }
     if (e != hipSuccess) {
         *p = nullptr;
         (void)hipGetLastError();
@@ -1684,6 +1786,8 @@ Flux<R> make_flux(ssw_sweep* s) {
     f.xpos = s->compiled ? s->d_xpos : nullptr;
     f.I = (R*)s->d_I;
     f.P = (R*)s->d_P;
+    f.offI = (R*)s->d_offI;
+    f.offP = (R*)s->d_offP;
     f.sa = (const SA<R>*)s->d_sa;
     f.inv_lref = 1.0 / s->lref;
     f.inv_aref = 1.0 / s->aref;
@@ -2491,6 +2595,7 @@ void launch_replay(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<hi
 // (counted in ssw_info).
 void capture(ssw_sweep* s, const Geo& g, int lvl) {
     hipGraph_t graph = nullptr;
+    std::unique_lock<std::shared_mutex> lock(g_capture_mu);
     if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
         (void)hipGetLastError();
         s->capture_failures++;
@@ -2498,6 +2603,7 @@ void capture(ssw_sweep* s, const Geo& g, int lvl) {
     }
     launch_replay(s, g, lvl, nullptr);
     hipError_t ce = hipStreamEndCapture(s->stream, &graph);
+    lock.unlock();
     if (ce != hipSuccess || !graph) {
         if (graph) (void)hipGraphDestroy(graph);
         (void)hipGetLastError();
@@ -2586,7 +2692,7 @@ int run_step(ssw_sweep* s, double* elapsed) {
         // events would add their own packets between the launches); the
         // average launch duration is the total over the replay's launches
         hipEvent_t t0 = s->timing ? mark() : nullptr;
-        if (s->graphs_on && !s->graphs[lvl] && s->replays[lvl] >= 1 && g_live_handles.load() == 1)
+        if (s->graphs_on && !s->graphs[lvl] && s->replays[lvl] >= 1)
             capture(s, g, lvl);
         if (s->graphs[lvl]) HIP_TRY(hipGraphLaunch(s->graphs[lvl], s->stream));
         else launch_replay(s, g, lvl, nullptr);
@@ -2642,6 +2748,8 @@ void free_state(ssw_sweep* s) {
     dfree(s, s->d_X);
     dfree(s, s->d_I);
     dfree(s, s->d_P);
+    dfree(s, s->d_offI);
+    dfree(s, s->d_offP);
 }
 
 void free_grid(ssw_sweep* s) {
@@ -2752,6 +2860,97 @@ int noalloc_out(ssw_sweep* s, const T* src, T* dst, uint8_t* stage) {
     return SSW_OK;
 }
 
+// The per-direction dependency graph of the uploaded faces under the current
+// directions: mirror check (first build), downwind statistics, overflow
+// counts, the transport mode (choose_mode) and the schedule. d_cnt is held.
+int build_direction_graph(ssw_sweep* s, bool choose_mode) {
+    const int64_t n = s->N;
+    const int L = s->p.num_timestep_levels;
+    const size_t DN = (size_t)s->D * n;
+    unsigned long long* d_stats = nullptr;
+    Defer cleanup{[&] { dfree(s, d_stats); }};
+    TRY(dalloc(s, &d_stats, 5));
+    HIP_TRY(hipMemsetAsync(d_stats, 0, 5 * 8, s->stream));
+    Geo g = make_geo(s);
+    if (choose_mode) k_check_mirror<<<grid_for(n), kBlock, 0, s->stream>>>(g, d_stats);
+    k_face_stats<<<grid_for((int64_t)DN), kBlock, 0, s->stream>>>(g, d_stats + 1, s->d_cnt);
+    HIP_TRY(hipGetLastError());
+    unsigned long long hb[5] = {0, 0, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(hb, d_stats, sizeof(hb), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    dfree(s, d_stats);
+    if (hb[0] != 0)
+        return fail(SSW_ERR_GRID, std::to_string(hb[0]) +
+                                      " local faces have no mirror face with the exactly negated normal");
+    for (int k = 0; k < 3; k++) s->dw[k] = (double)hb[k + 1] / ((double)n * s->D);
+    s->ovf_total = (int64_t)hb[4];
+    if (!choose_mode) {
+        if (s->compiled) TRY(setup_overflow(s, s->d_cnt));
+        return build_schedule(s);
+    }
+    // transport mode: compiled when, at its peak (schedule arrays, overflow
+    // starts and sorted order while the schedule is written; then the
+    // active-set lists and the transport state), it needs at most 85% of the
+    // free device memory (automatic), else geometric
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    const size_t rs = real_size(s);
+    const size_t comp_bytes = DN * (16 + rs + kPlanes * entry_size(s) + 4) + (size_t)s->ovf_total * entry_size(s);
+    const bool grouped = s->group_request >= 2;
+    const size_t lists = grouped ? (size_t)group_bytes((int64_t)DN, n, L, s->group_block) + DN * 5
+                                 : DN * 4 * (size_t)(L - 1);  // grouped: + build scratch
+    const size_t comp_need = comp_bytes + std::max<size_t>(DN * 8, lists + DN * 2 * rs);
+    if (s->mode_request == SSW_TRANSPORT_MODE_COMPILED) s->compiled = true;
+    else if (s->mode_request == SSW_TRANSPORT_MODE_GEOMETRIC) s->compiled = false;
+    else s->compiled = (double)comp_need <= 0.85 * (double)free_b;
+    if (s->compiled) TRY(setup_overflow(s, s->d_cnt));
+    return build_schedule(s);
+}
+
+// The row-major matrix times every direction (multiply_by_matrix,
+// src/sweep/direction/mod.rs:150-155), and kept[i] = kernel[i][D-1] of remap
+// (:177-205): 1 if the last rotated direction is the last of the rotated
+// directions most aligned with old direction i (Iterator::max_by_key keeps
+// the last maximum of 1 + old_i . new_j), else 0.
+void rotate_table(const double* m, std::vector<double>& dirs, std::vector<double>& kept) {
+    const int D = (int)dirs.size() / 3;
+    const std::vector<double> old = dirs;
+    for (int d = 0; d < D; d++) {
+        const double x = old[3 * d], y = old[3 * d + 1], z = old[3 * d + 2];
+        dirs[3 * d] = x * m[0] + y * m[1] + z * m[2];
+        dirs[3 * d + 1] = x * m[3] + y * m[4] + z * m[5];
+        dirs[3 * d + 2] = x * m[6] + y * m[7] + z * m[8];
+    }
+    kept.assign(D, 0.0);
+    for (int i = 0; i < D; i++) {
+        int best = 0;
+        double best_v = 0.0;
+        for (int j = 0; j < D; j++) {
+            const double v =
+                1.0 + ((old[3 * i] * dirs[3 * j] + old[3 * i + 1] * dirs[3 * j + 1]) + old[3 * i + 2] * dirs[3 * j + 2]);
+            if (j == 0 || v >= best_v) {
+                best = j;
+                best_v = v;
+            }
+        }
+        kept[i] = best == D - 1 ? 1.0 : 0.0;
+    }
+}
+
+template <typename R>
+int rotate_state_t(ssw_sweep* s, const double* d_kept, const double* dI, const double* dO, const double* dP) {
+    const int64_t DN = (int64_t)s->D * s->N;
+    const size_t rs = sizeof(R);
+    if (!s->d_offI) TRY(dalloc(s, (uint8_t**)&s->d_offI, (size_t)DN * rs));
+    if (!s->d_offP) TRY(dalloc(s, (uint8_t**)&s->d_offP, (size_t)s->D * std::max<int64_t>(1, s->Nper) * rs));
+    const Geo g = make_geo(s);
+    const Flux<R> fl = make_flux<R>(s);
+    k_rotate_state<R><<<grid_for(DN), kBlock, 0, s->stream>>>(g, fl, d_kept, dI, dO, dP);
+    k_rotate_offsets<R><<<grid_for(DN), kBlock, 0, s->stream>>>(g, fl);
+    HIP_TRY(hipGetLastError());
+    return SSW_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2820,7 +3019,12 @@ int ssw_create(const ssw_sweep_parameters* params, ssw_sweep** out) {
     s->dirs = dirs;
     s->fp32 = p.transport_precision == SSW_TRANSPORT_FP32;
     (void)hipGetDevice(&s->device);
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    hipError_t se;
+    {
+        std::shared_lock<std::shared_mutex> lock(g_capture_mu);
+        se = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    }
+    if (se != hipSuccess) {
         delete s;
         g_live_handles--;
         return fail(SSW_ERR_HIP, "hipStreamCreate failed");
@@ -2834,9 +3038,10 @@ int ssw_create(const ssw_sweep_parameters* params, ssw_sweep** out) {
         ssw_destroy(s);
         return r;
     }
-    if (hipMemcpy(s->d_dirs, dirs.data(), dirs.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(s->d_fail, 0, 8) != hipSuccess || hipMemset(s->d_active, 0, 8) != hipSuccess ||
-        hipMemset(s->d_limit, 0, 24) != hipSuccess) {
+    if (hipMemcpyAsync(s->d_dirs, dirs.data(), dirs.size() * 8, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+        hipMemsetAsync(s->d_fail, 0, 8, s->stream) != hipSuccess ||
+        hipMemsetAsync(s->d_active, 0, 8, s->stream) != hipSuccess ||
+        hipMemsetAsync(s->d_limit, 0, 24, s->stream) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) {
         ssw_destroy(s);
         return fail(SSW_ERR_HIP, "initialisation of the handle's device memory failed");
     }
@@ -2856,7 +3061,10 @@ void ssw_destroy(ssw_sweep* s) {
     dfree(s, s->d_active);
     dfree(s, s->d_limit);
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->stream) {
+        std::shared_lock<std::shared_mutex> lock(g_capture_mu);
+        (void)hipStreamDestroy(s->stream);
+    }
     delete s;
     g_live_handles--;
 }
@@ -3043,43 +3251,7 @@ int ssw_upload_grid(ssw_sweep* s, int64_t n, const int64_t* off, const int32_t* 
     HIP_TRY(hipMemcpyAsync(s->d_pfoff, pfoff.data(), pfoff.size() * 4, hipMemcpyHostToDevice, s->stream));
     std::vector<uint8_t> lev(n, (uint8_t)(L - 1));
     HIP_TRY(hipMemcpyAsync(s->d_level, lev.data(), n, hipMemcpyHostToDevice, s->stream));
-    // mirror check, downwind statistics and overflow counts (into cnt)
-    unsigned long long* d_stats = nullptr;
-    Defer cleanup{[&] { dfree(s, d_stats); }};
-    TRY(dalloc(s, &d_stats, 5));
-    HIP_TRY(hipMemsetAsync(d_stats, 0, 5 * 8, s->stream));
-    Geo g = make_geo(s);
-    k_check_mirror<<<grid_for(n), kBlock, 0, s->stream>>>(g, d_stats);
-    k_face_stats<<<grid_for((int64_t)DN), kBlock, 0, s->stream>>>(g, d_stats + 1, s->d_cnt);
-    HIP_TRY(hipGetLastError());
-    unsigned long long hb[5] = {0, 0, 0, 0, 0};
-    HIP_TRY(hipMemcpyAsync(hb, d_stats, sizeof(hb), hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    dfree(s, d_stats);
-    if (hb[0] != 0)
-        return fail(SSW_ERR_GRID, std::to_string(hb[0]) +
-                                      " local faces have no mirror face with the exactly negated normal");
-    for (int k = 0; k < 3; k++) s->dw[k] = (double)hb[k + 1] / ((double)n * s->D);
-    s->ovf_total = (int64_t)hb[4];
-    // transport mode: compiled when its arrays fit (automatic: within half of
-    // the free device memory), else geometric
-    // transport mode: compiled when, at its peak (schedule arrays, overflow
-    // starts and sorted order while the schedule is written; then the
-    // active-set lists and the transport state), it needs at most 85% of the
-    // free device memory (automatic), else geometric
-    size_t free_b = 0, total_b = 0;
-    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-    const size_t rs = real_size(s);
-    const size_t comp_bytes = DN * (16 + rs + kPlanes * entry_size(s) + 4) + (size_t)s->ovf_total * entry_size(s);
-    const bool grouped = s->group_request >= 2;
-    const size_t lists = grouped ? (size_t)group_bytes((int64_t)DN, n, L, s->group_block) + DN * 5
-                                 : DN * 4 * (size_t)(L - 1);  // grouped: + build scratch
-    const size_t comp_need = comp_bytes + std::max<size_t>(DN * 8, lists + DN * 2 * rs);
-    if (s->mode_request == SSW_TRANSPORT_MODE_COMPILED) s->compiled = true;
-    else if (s->mode_request == SSW_TRANSPORT_MODE_GEOMETRIC) s->compiled = false;
-    else s->compiled = (double)comp_need <= 0.85 * (double)free_b;
-    if (s->compiled) TRY(setup_overflow(s, s->d_cnt));
-    TRY(build_schedule(s));
+    TRY(build_direction_graph(s, true));
     s->have_grid = true;
     return SSW_OK;
 }
@@ -3155,6 +3327,71 @@ int ssw_step(ssw_sweep* s, double* elapsed) {
     if (!s) return fail(SSW_ERR_INVALID, "null handle");
     if (!s->have_state) return fail(SSW_ERR_INVALID, "ssw_upload_state must be called first");
     return s->fp32 ? run_step<float>(s, elapsed) : run_step<double>(s, elapsed);
+}
+
+int ssw_rotate_directions(ssw_sweep* s, const double* matrix) {
+    if (!s || !matrix) return fail(SSW_ERR_INVALID, "null argument");
+    if (!s->have_grid) return fail(SSW_ERR_INVALID, "ssw_upload_grid must be called first");
+    for (int k = 0; k < 9; k++)
+        if (!std::isfinite(matrix[k])) return fail(SSW_ERR_INVALID, "rotation matrix must be finite");
+    const int64_t n = s->N;
+    const size_t DN = (size_t)s->D * n;
+    // the rates before the rotation, as the reference's sites hold them
+    double *dI = nullptr, *dO = nullptr, *dP = nullptr, *d_kept = nullptr;
+    Defer cleanup{[&] {
+        for (double** p : {&dI, &dO, &dP, &d_kept}) dfree(s, *p);
+    }};
+    if (s->have_state) {
+        TRY(dalloc(s, &dI, DN));
+        TRY(dalloc(s, &dO, DN));
+        TRY(dalloc(s, &dP, DN));
+        const Geo g = make_geo(s);
+        if (s->fp32)
+            k_fluxes<float><<<grid_for(n), kBlock, 0, s->stream>>>(g, make_flux<float>(s), s->lref, dI, dO, dP, nullptr,
+                                                                  nullptr);
+        else
+            k_fluxes<double><<<grid_for(n), kBlock, 0, s->stream>>>(g, make_flux<double>(s), s->lref, dI, dO, dP,
+                                                                   nullptr, nullptr);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
+    std::vector<double> kept;
+    rotate_table(matrix, s->dirs, kept);
+    HIP_TRY(hipMemcpyAsync(s->d_dirs, s->dirs.data(), s->dirs.size() * 8, hipMemcpyHostToDevice, s->stream));
+    // the dependency graph and schedule of the rotated directions
+    drop_all_lists(s);
+    TRY(dalloc(s, &s->d_cnt, DN));
+    TRY(build_direction_graph(s, false));
+    s->rotated = true;
+    if (s->have_state) {
+        TRY(dalloc(s, &d_kept, kept.size()));
+        HIP_TRY(hipMemcpyAsync(d_kept, kept.data(), kept.size() * 8, hipMemcpyHostToDevice, s->stream));
+        TRY(s->fp32 ? rotate_state_t<float>(s, d_kept, dI, dO, dP) : rotate_state_t<double>(s, d_kept, dI, dO, dP));
+        TRY(set_need_all(s, 1u));
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SSW_OK;
+}
+
+int ssw_rotation_matrix(const double* axis, double angle, double* out) {
+    if (!axis || !out) return fail(SSW_ERR_INVALID, "null argument");
+    // get_rotation_matrix (src/sweep/direction/mod.rs:112-135)
+    const double x = axis[0], y = axis[1], z = axis[2];
+    const double c = std::cos(angle), sn = std::sin(angle);
+    const double m[9] = {c + x * x * (1.0 - c),     x * y * (1.0 - c) - z * sn, x * z * (1.0 - c) + y * sn,
+                         y * x * (1.0 - c) + z * sn, c + y * y * (1.0 - c),     y * z * (1.0 - c) - x * sn,
+                         z * x * (1.0 - c) - y * sn, z * y * (1.0 - c) + x * sn, c + z * z * (1.0 - c)};
+    std::memcpy(out, m, sizeof(m));
+    return SSW_OK;
+}
+
+int ssw_random_rotation_matrix(double phi, double u, double psi, double* out) {
+    if (!out) return fail(SSW_ERR_INVALID, "null argument");
+    // get_random_rotation_matrix (src/sweep/direction/mod.rs:137-148) for its
+    // three draws: phi in [0, 2 pi), u in [0, 1), psi in [0, 2 pi)
+    const double theta = std::acos(2.0 * u - 1.0);
+    const double axis[3] = {std::cos(phi) * std::sin(theta), std::sin(phi) * std::sin(theta), std::cos(theta)};
+    return ssw_rotation_matrix(axis, psi, out);
 }
 
 int ssw_synchronize(ssw_sweep* s) {
@@ -3537,6 +3774,7 @@ int ssw_save_checkpoint(ssw_sweep* s, void* buf, int64_t bytes) {
     if (!s || !buf) return fail(SSW_ERR_INVALID, "null argument");
     if (!s->have_state) return fail(SSW_ERR_INVALID, "ssw_upload_state must be called first");
     if (bytes < ckpt_bytes(s)) return fail(SSW_ERR_INVALID, "checkpoint buffer too small");
+    if (s->rotated) return fail(SSW_ERR_INVALID, "checkpoints of a handle with rotated directions are not supported");
     return s->fp32 ? save_t<float>(s, (uint8_t*)buf) : save_t<double>(s, (uint8_t*)buf);
 }
 
@@ -3544,6 +3782,7 @@ int ssw_load_checkpoint(ssw_sweep* s, const void* buf, int64_t bytes) {
     if (!s || !buf) return fail(SSW_ERR_INVALID, "null argument");
     if (!s->have_state) return fail(SSW_ERR_INVALID, "ssw_upload_state must be called first");
     if (bytes < (int64_t)sizeof(CkptHeader)) return fail(SSW_ERR_INVALID, "checkpoint too small");
+    if (s->rotated) return fail(SSW_ERR_INVALID, "checkpoints of a handle with rotated directions are not supported");
     CkptHeader h;
     std::memcpy(&h, buf, sizeof(h));
     if (h.magic != kCkptMagic || h.version != 1u) return fail(SSW_ERR_INVALID, "not a checkpoint of this library");

@@ -77,6 +77,28 @@ def direction_table(num_directions: int) -> np.ndarray:
     return out
 
 
+def rotation_matrix(axis, angle: float) -> np.ndarray:
+    """get_rotation_matrix (src/sweep/direction/mod.rs:112-135): 3 x 3
+    rotation by `angle` about the unit vector `axis`."""
+    a = np.ascontiguousarray(axis, dtype=np.float64).reshape(3)
+    out = np.zeros(9)
+    N.check(N.sweep_lib().ssw_rotation_matrix(N.ptr(a, C.c_double), float(angle), N.ptr(out, C.c_double)))
+    return out.reshape(3, 3)
+
+
+def random_rotation_matrix(rng: np.random.Generator) -> np.ndarray:
+    """get_random_rotation_matrix (src/sweep/direction/mod.rs:137-148) with its
+    three draws (phi in [0, 2 pi), u in [0, 1), psi in [0, 2 pi)) taken from
+    `rng`. The reference draws them from a StdRng seeded with 1337, whose
+    stream numpy does not reproduce."""
+    phi = rng.uniform(0.0, 2.0 * np.pi)
+    u = rng.uniform(0.0, 1.0)
+    psi = rng.uniform(0.0, 2.0 * np.pi)
+    out = np.zeros(9)
+    N.check(N.sweep_lib().ssw_random_rotation_matrix(phi, u, psi, N.ptr(out, C.c_double)))
+    return out.reshape(3, 3)
+
+
 @dataclass
 class SweepResults:
     ionized_hydrogen_fraction: np.ndarray
@@ -178,6 +200,13 @@ class Sweep:
         shape = (self.num_directions, self.num_cells)
         return i.reshape(shape), o.reshape(shape), p.reshape(shape)
 
+    def rotate_directions(self, matrix) -> None:
+        """rotate_directions_system (src/sweep/direction/mod.rs:158-175) with
+        the given 3 x 3 rotation matrix: rotated directions, their dependency
+        graph, and the rates remapped as the reference's remap does."""
+        m = np.ascontiguousarray(matrix, dtype=np.float64).reshape(9)
+        N.check(self._lib.ssw_rotate_directions(self._h, N.ptr(m, C.c_double)))
+
     def synchronize(self) -> None:
         N.check(self._lib.ssw_synchronize(self._h))
 
@@ -260,7 +289,13 @@ class SweepPlugin:
     included, as the sweep_optional_output_system instances that the plugin
     registers for the desired output fields and that run after
     run_sweep_system each frame (src/sweep/mod.rs:162-165, 770-781); their
-    values land in `chemistry`."""
+    values land in `chemistry`.
+
+    `rotate_directions` (SweepParameters::rotate_directions,
+    src/sweep/parameters.rs:20-22) rotates the directions by a random rotation
+    after every call, the first one included, as rotate_directions_system runs
+    after run_sweep_system each frame (src/sweep/mod.rs:143-148); the draws
+    come from `rotation_rng` (numpy, seeded with the reference's 1337)."""
 
     sweep: Sweep
     time: float = 0.0
@@ -269,6 +304,8 @@ class SweepPlugin:
     last: SweepResults | None = field(default=None, repr=False)
     optional_outputs: tuple = ()
     chemistry: dict = field(default_factory=dict, repr=False)
+    rotate_directions: bool = False
+    rotation_rng: np.random.Generator = field(default_factory=lambda: np.random.default_rng(1337), repr=False)
 
     def __post_init__(self):
         unknown = [k for k in self.optional_outputs if k not in CHEMISTRY_OUTPUTS]
@@ -276,6 +313,12 @@ class SweepPlugin:
             raise ValueError(f"unknown optional chemistry outputs {unknown}; known: {CHEMISTRY_OUTPUTS}")
 
     def run_sweep_system(self) -> SweepResults | None:
+        res = self._run_sweep_system()
+        if self.rotate_directions:
+            self.sweep.rotate_directions(random_rotation_matrix(self.rotation_rng))
+        return res
+
+    def _run_sweep_system(self) -> SweepResults | None:
         if self.is_first_time:
             self.is_first_time = False
             self.last = self.sweep.results()

@@ -18,6 +18,9 @@
   clustered_2m       2097152 clustered cells, 1024 sources, 84 directions,
                      4 timestep levels, FP32 transport.
   cartesian_problem  the reference's cartesian test grid with uniform gas.
+  serving_pool,      the request mix of the serving loop: grids of 8192-524288
+  serving_requests   cells (log-uniform), jittered or clustered, with random
+                     direction counts, timestep levels and sources.
 
 Seeds: lattice 1338, density 1339, sources 1337, clustered field 1340/1341,
 clustered sources 1342.
@@ -145,3 +148,49 @@ def cartesian_problem(n: int, num_directions: int, levels: int = 1, periodic: bo
     )
     return Problem(f"cartesian{n}", grid, params, np.full(N, density), np.full(N, 1e-10), np.full(N, temperature),
                    src, steps)
+
+
+def serving_pool(num_grids: int, seed: int, lo: int = 8192, hi: int = 524288):
+    """Grids of the serving workload: sizes log-uniform in [lo, hi] cells,
+    alternately jittered-lattice Voronoi (n^3 cells, n = round(cbrt(size)))
+    and clustered lognormal-field Voronoi (64^3 field mesh, sigma_ln 2), as
+    (grid, generator points, field density or None)."""
+    rng = np.random.default_rng(seed)
+    pool = []
+    for i in range(num_grids):
+        size = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        if i % 2 == 0:
+            n = max(2, int(round(size ** (1.0 / 3.0))))
+            pts = jittered_lattice(n, BOX, 0.3, seed + 10 + i)
+            delta = None
+        else:
+            pts, delta = clustered_points(size, BOX, mesh=64, sigma_ln=2.0, seed=seed + 10 + i)
+        pool.append((voronoi_grid(pts, BOX), pts, delta))
+    return pool
+
+
+def serving_requests(pool, count: int, seed: int, steps: int, precision: int = 32, rho_scale: float = 1.0):
+    """Request mix: a pool grid, directions from {16, 32, 64, 84}, timestep
+    levels from {1, 2, 4}, 1-64 sources of 1e50 photons/s at uniform positions,
+    lognormal gas (sigma_ln 1) on jittered grids and the field's density on
+    clustered grids (both times `rho_scale`), `steps` steps. Each Problem
+    carries `pool_index`, the index of its grid in the pool (a key for keeping
+    the grid resident)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        gi = int(rng.integers(len(pool)))
+        grid, pts, delta = pool[gi]
+        N = grid.num_cells
+        dirs = int(rng.choice([16, 32, 64, 84]))
+        levels = int(rng.choice([1, 2, 4]))
+        ns = int(rng.integers(1, 65))
+        src, _ = assign_sources(pts, rng.uniform(0.0, BOX, size=(ns, 3)), np.full(ns, 1e50))
+        rho = RHO_MEAN * delta if delta is not None else lognormal_density(N, RHO_MEAN, 1.0, seed + 1000 + k)
+        rho = rho * rho_scale
+        params = SweepParameters(directions=dirs, num_timestep_levels=levels, periodic=True,
+                                 max_timestep=1.0 * U.MEGAYEARS, transport_precision=precision)
+        pb = Problem(f"request{k}", grid, params, rho, np.full(N, 1e-10), np.full(N, 1e4), src, steps)
+        pb.pool_index = gi
+        out.append(pb)
+    return out
