@@ -70,8 +70,11 @@ def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3,
     from subsweep_amd import Sweep
     pp = pb.with_params(transport_precision=precision)
     s = Sweep(pp.params, pp.grid, *pp.state(), **(shape or {}))
-    for _ in range(warmup):
+    for k in range(warmup):
         s.run_sweeps()
+        if progress:
+            s.synchronize()
+            log(f"  warmup step {k + 1}/{warmup} done")
     s.synchronize()
     # timed region: whole steps, no per-launch events
     updates = 0
@@ -415,6 +418,9 @@ def main():
                     help="measure one GPU's share of a direction-sharded run over G GPUs and print its JSON only")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use --mode/--block instead of the autotuner's choice")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="profiling run: launch the replays one by one instead of as captured graphs")
+    ap.add_argument("--progress", action="store_true", help="profiling run: log every step")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -447,7 +453,9 @@ def main():
     log(f"grid: {pb.grid.num_cells} cells, {pb.grid.num_faces} faces, built in {time.perf_counter() - t:.1f} s")
     if args.precision:  # profiling run: one precision, default launch shape unless tuned, no JSON line
         sh, _ = tuned(pb, args.precision, args.warmup, not args.no_autotune, args.mode, args.block)
-        r = run_gpu(pb, args.precision, args.steps, args.warmup, shape=sh)
+        if args.no_graphs:
+            sh = dict(sh, graph_replay=False)
+        r = run_gpu(pb, args.precision, args.steps, args.warmup, shape=sh, progress=args.progress)
         log(f"fp{args.precision} transport: {r['updates_per_s'] / 1e9:.3f} G updates/s, {r['ms_per_step']:.2f} "
             f"ms/step, kernel {r['avg_launch_ms']:.4f} ms/launch, {r['achieved_gbs']:.0f} GB/s")
         return

@@ -312,7 +312,7 @@ int ssw_set_narrow_runs(ssw_sweep* s, int32_t enabled);
  * counterpart in the reference. */
 int ssw_set_group_levels(ssw_sweep* s, int32_t levels);
 
-/* Root tasks per workgroup of the grouped replay: 64 or 128 (0 = 64, the
+/* Root tasks per workgroup of the grouped replay: 32, 64 or 128 (0 = 64, the
  * default); the workgroup has twice as many lanes. Set before
  * ssw_upload_grid. No counterpart in the reference. */
 int ssw_set_group_block(ssw_sweep* s, int32_t roots);

@@ -12,7 +12,7 @@ replayed:
 * narrow runs - runs of narrow level sets solved by one workgroup per run,
   or one launch per level set (compiled replay);
 * group levels and group block - launches of 2-8 consecutive level sets
-  (overlapping upwind cones of 64 or 128 root tasks, one workgroup each), or
+  (overlapping upwind cones of 32, 64 or 128 root tasks, one workgroup each), or
   one launch per level set (compiled replay).
 
 `autotune` times a few steps of each candidate on the problem itself and
@@ -47,7 +47,7 @@ class TuneResult:
                 "group_levels": self.group_levels, "group_block": self.group_block}
 
 
-def candidates(blocks=(64, 128), groups=(4,), group_blocks=(64, 128)):
+def candidates(blocks=(64, 128), groups=(4,), group_blocks=(32, 64, 128)):
     C = N.TRANSPORT_MODE_COMPILED
     out = [(C, b, True, True, 1, 0) for b in blocks]
     out.append((C, blocks[0], False, True, 1, 0))

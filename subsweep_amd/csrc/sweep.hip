@@ -2181,7 +2181,10 @@ int build_groups_t(ssw_sweep* s) {
         TRY(dalloc(s, &meta, hmeta.size()));
         HIP_TRY(hipMemcpyAsync(meta, hmeta.data(), hmeta.size() * sizeof(uint4), hipMemcpyHostToDevice, s->stream));
         const unsigned nb = (unsigned)hmeta.size();
-        if (T == 64)
+        if (T == 32)
+            k_cone<R, 32><<<nb, 32, 0, s->stream>>>(hdr, planes, ovf, s->d_fstart, meta, rl, todo, s->d_glist,
+                                                    (uint32_t)list_cap, ctl, desc_rm + 2 * (size_t)ndesc, ctl + 1);
+        else if (T == 64)
             k_cone<R, 64><<<nb, 64, 0, s->stream>>>(hdr, planes, ovf, s->d_fstart, meta, rl, todo, s->d_glist,
                                                     (uint32_t)list_cap, ctl, desc_rm + 2 * (size_t)ndesc, ctl + 1);
         else
@@ -2537,7 +2540,11 @@ void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<
             // full sweeps: die-aware order inside each pass; restricted sweeps
             // launch their filtered descriptors in order
             const PassTab pt = all ? gr.passes : s->gpass_l[lvl][it.tab];
-            if (s->group_block == 64)
+            if (s->group_block == 32)
+                k_sweep_group<R, 64><<<nblk, 64, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf,
+                                                                    s->d_fstart, gr.k0, gr.levels, dsc, gl,
+                                                                    s->glist_len, pt);
+            else if (s->group_block == 64)
                 k_sweep_group<R, 128><<<nblk, 128, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf,
                                                                       s->d_fstart, gr.k0, gr.levels, dsc, gl,
                                                                       s->glist_len, pt);
@@ -3132,7 +3139,8 @@ int ssw_direction_table(int32_t num_directions, double* out) {
 
 int ssw_set_group_block(ssw_sweep* s, int32_t roots) {
     if (!s) return fail(SSW_ERR_INVALID, "null handle");
-    if (roots != 0 && roots != 64 && roots != 128) return fail(SSW_ERR_INVALID, "group block must be 0, 64 or 128");
+    if (roots != 0 && roots != 32 && roots != 64 && roots != 128)
+        return fail(SSW_ERR_INVALID, "group block must be 0, 32, 64 or 128");
     if (s->have_grid) return fail(SSW_ERR_INVALID, "ssw_set_group_block must be called before ssw_upload_grid");
     s->group_block = roots ? roots : kGroupT;
     return SSW_OK;

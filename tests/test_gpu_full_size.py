@@ -2,16 +2,21 @@
 timestep levels), where the oracle would take minutes per step, checked
 through properties that do not depend on size:
 
-  * launch-shape invariance: grouped replay (default), one launch per level
-    set and the geometric replay give bitwise the same fields;
+  * launch-shape invariance: grouped replay (default, and with 32 roots per
+    workgroup), one launch per level set and the geometric replay give
+    bitwise the same fields;
   * the Kahn depths form a valid order: for every local face f of cell c
     whose normal points against direction d, depth(c, d) > depth(nb, d)
     (checked for a sample of directions), and the dependency counts equal
-    the number of such upwind faces per (cell, direction)."""
+    the number of such upwind faces per (cell, direction);
+  * at full size, the dependency counts and Kahn depths of all 84 directions
+    are bit-exact against the oracle's (its Kahn layering takes about a
+    minute on eight host threads, far less than a step of its sweep)."""
 
 import numpy as np
 import pytest
 
+from oracle.oracle import OracleSweep
 from subsweep_amd import Sweep, direction_table
 from subsweep_amd import _native as N
 from subsweep_amd import workloads as W
@@ -27,7 +32,7 @@ def clustered():
 def test_clustered_launch_shapes_bitwise(clustered, hip_available):
     pb = clustered
     out = []
-    for kw in ({}, {"group_levels": 1}, {"transport_mode": N.TRANSPORT_MODE_GEOMETRIC}):
+    for kw in ({}, {"group_block": 32}, {"group_levels": 1}, {"transport_mode": N.TRANSPORT_MODE_GEOMETRIC}):
         s = Sweep(pb.params, pb.grid, *pb.state(), **kw)
         for _ in range(3):
             s.run_sweeps()
@@ -56,3 +61,14 @@ def test_clustered_depths_are_a_valid_order(clustered, hip_available):
         assert np.all(depth[d][c[up]] > depth[d][nb[up]]), f"direction {d}: a task not after its upwind task"
         n_up = np.bincount(c[up], minlength=g.num_cells)
         assert np.array_equal(counts[d].astype(np.int64), n_up), f"direction {d}: counts"
+
+
+def test_clustered_structure_against_the_oracle(clustered, hip_available):
+    pb = clustered
+    s = Sweep(pb.params, pb.grid, *pb.state())
+    depth, counts = s.levelsets(0), s.counts(0)
+    s.close()
+    o = OracleSweep(pb.with_params(transport_precision=64).params, pb.grid, *pb.state(), threads=8)
+    assert np.array_equal(counts, o.counts(0)), "dependency counts differ from the oracle's"
+    assert np.array_equal(depth, o.levelsets(0)), "Kahn depths differ from the oracle's"
+    o.close()

@@ -200,7 +200,7 @@ def test_grouped_full_sweeps_bitwise(precision, levels, hip_available):
     pb = W.jittered_problem(12, 1.0, "8", 84, levels=levels, periodic=True, steps=5, precision=precision)
     pb = pb.with_params(max_timestep=1e-3 * 3.15576e13)
     ref = None
-    for m, gb in ((1, 0), (2, 128), (3, 64), (4, 128), (6, 64), (8, 128)):
+    for m, gb in ((1, 0), (2, 128), (3, 64), (4, 128), (6, 64), (8, 128), (4, 32), (8, 32)):
         s = Sweep(pb.params, pb.grid, *pb.state(), transport_mode=N.TRANSPORT_MODE_COMPILED, group_levels=m,
                   group_block=gb)
         info = s.info()
