@@ -63,6 +63,27 @@ def bytes_per_update(info, precision: int) -> float:
     return base + info.downwind_local * bl + info.downwind_periodic * bp + info.downwind_boundary * bb
 
 
+def compute_units() -> int:
+    import torch
+    return int(torch.cuda.get_device_properties(0).multi_processor_count)
+
+
+def occupancy(info) -> dict:
+    """Waves of the grouped replay kernel per CU: launched (mean workgroups
+    per launch of a full sweep x waves per workgroup / CUs) against what one
+    CU can hold of this kernel at once (its register and LDS limits)."""
+    if info.group_launches <= 0:
+        return None
+    waves_per_wg = 2 * int(info.group_block) // 64
+    cus = compute_units()
+    wg = info.group_workgroups / info.group_launches
+    limit = int(info.group_blocks_per_cu) * waves_per_wg
+    launched = wg * waves_per_wg / cus
+    return {"kernel": "k_sweep_group", "compute_units": cus, "workgroups_per_launch": wg,
+            "waves_per_workgroup": waves_per_wg, "launched_waves_per_cu": launched,
+            "limit_waves_per_cu": limit, "frac": launched / limit if limit else None}
+
+
 def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3, shape: dict = None,
             progress: bool = False):
     """`shape`: Sweep launch-shape keywords (transport_mode, replay_block,
@@ -122,6 +143,7 @@ def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3,
         "group_levels": int(info.group_levels),
         "group_block": int(info.group_block),
         "group_tasks_per_full_sweep": int(info.group_tasks),
+        "occupancy": occupancy(info),
         "transport_mode": int(info.transport_mode),
         "info": info,
         "results": res,
@@ -542,6 +564,7 @@ def main():
             "bytes_per_update": fp32["bytes_per_update"],
             "fp64_achieved": fp64["achieved_gbs"],
             "fp64_bytes_per_update": fp64["bytes_per_update"],
+            "occupancy": fp32["occupancy"],
         },
         "launch_shape": {"fp32": shape_of(fp32, tune32), "fp64": shape_of(fp64, tune64)},
         "cpu_baseline": cpu,

@@ -219,7 +219,8 @@ typedef struct ssw_info {
     int64_t group_tasks;           /* tasks a grouped full sweep solves, cones
                                       overlapping (>= the sweep's tasks)       */
     int32_t group_block;           /* grouped replay: root tasks per workgroup */
-    int32_t reserved3;
+    int32_t group_blocks_per_cu;   /* grouped replay: workgroups one CU can hold
+                                      at once (register and LDS limits)      */
     /* TimescaleCounter of the last step (src/chemistry/timescale.rs:70-111,
      * counted in update_chemistry, src/sweep/mod.rs:572, reset per run_sweeps,
      * :267): chemistry updates whose change timescale is below max_timestep,
@@ -228,6 +229,10 @@ typedef struct ssw_info {
     int64_t limiting_temperature;
     int64_t limiting_ionization_fraction;
     int64_t limiting_photon_rate;
+    /* grouped replay of a full sweep: launches and workgroups over them
+     * (mean workgroups per launch = group_workgroups / group_launches) */
+    int64_t group_launches;
+    int64_t group_workgroups;
 } ssw_info;
 
 int ssw_get_info(ssw_sweep* s, ssw_info* info);

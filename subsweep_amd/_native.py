@@ -75,10 +75,12 @@ class InfoC(C.Structure):
         ("group_fallback", C.c_int32),
         ("group_tasks", C.c_int64),
         ("group_block", C.c_int32),
-        ("reserved3", C.c_int32),
+        ("group_blocks_per_cu", C.c_int32),
         ("limiting_temperature", C.c_int64),
         ("limiting_ionization_fraction", C.c_int64),
         ("limiting_photon_rate", C.c_int64),
+        ("group_launches", C.c_int64),
+        ("group_workgroups", C.c_int64),
     ]
 
 

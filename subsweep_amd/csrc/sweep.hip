@@ -3802,6 +3802,21 @@ int ssw_load_checkpoint(ssw_sweep* s, const void* buf, int64_t bytes) {
     return s->fp32 ? load_t<float>(s, (const uint8_t*)buf) : load_t<double>(s, (const uint8_t*)buf);
 }
 
+extern "C++" {
+namespace {
+// Workgroups of the grouped replay kernel one CU holds at once (0 if unknown).
+template <typename R>
+int group_blocks_per_cu(int roots) {
+    int n = 0;
+    hipError_t e;
+    if (roots == 32) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sweep_group<R, 64>, 64, 0);
+    else if (roots == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sweep_group<R, 128>, 128, 0);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sweep_group<R, 256>, 256, 0);
+    return e == hipSuccess ? n : 0;
+}
+}  // namespace
+}
+
 int ssw_get_info(ssw_sweep* s, ssw_info* info) {
     if (!s || !info) return fail(SSW_ERR_INVALID, "null argument");
     std::memset(info, 0, sizeof(*info));
@@ -3827,6 +3842,12 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info) {
     info->group_fallback = s->group_fallback ? 1 : 0;
     info->group_tasks = s->group_tasks;
     info->group_block = s->group_block;
+    if (!s->groups.empty()) {
+        info->group_launches = (int64_t)s->groups.size();
+        for (const ssw_sweep::Group& gr : s->groups) info->group_workgroups += gr.ndesc;
+        info->group_blocks_per_cu =
+            s->fp32 ? group_blocks_per_cu<float>(s->group_block) : group_blocks_per_cu<double>(s->group_block);
+    }
     info->locality_order = s->permuted ? 1 : 0;
     if (s->have_state) {
         unsigned long long f = 0, a = 0, lim[3] = {0, 0, 0};
