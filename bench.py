@@ -63,11 +63,6 @@ def bytes_per_update(info, precision: int) -> float:
     return base + info.downwind_local * bl + info.downwind_periodic * bp + info.downwind_boundary * bb
 
 
-def compute_units() -> int:
-    import torch
-    return int(torch.cuda.get_device_properties(0).multi_processor_count)
-
-
 def occupancy(info) -> dict:
     """Waves of the grouped replay kernel per CU: launched (mean workgroups
     per launch of a full sweep x waves per workgroup / CUs) against what one
@@ -75,13 +70,13 @@ def occupancy(info) -> dict:
     if info.group_launches <= 0:
         return None
     waves_per_wg = 2 * int(info.group_block) // 64
-    cus = compute_units()
+    cus = int(info.compute_units)
     wg = info.group_workgroups / info.group_launches
     limit = int(info.group_blocks_per_cu) * waves_per_wg
-    launched = wg * waves_per_wg / cus
+    launched = wg * waves_per_wg / cus if cus else None
     return {"kernel": "k_sweep_group", "compute_units": cus, "workgroups_per_launch": wg,
             "waves_per_workgroup": waves_per_wg, "launched_waves_per_cu": launched,
-            "limit_waves_per_cu": limit, "frac": launched / limit if limit else None}
+            "limit_waves_per_cu": limit, "frac": launched / limit if (limit and launched) else None}
 
 
 def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3, shape: dict = None,

@@ -233,6 +233,8 @@ typedef struct ssw_info {
      * (mean workgroups per launch = group_workgroups / group_launches) */
     int64_t group_launches;
     int64_t group_workgroups;
+    int32_t compute_units;         /* CUs of the handle's device               */
+    int32_t reserved4;
 } ssw_info;
 
 int ssw_get_info(ssw_sweep* s, ssw_info* info);

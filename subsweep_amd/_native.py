@@ -81,6 +81,8 @@ class InfoC(C.Structure):
         ("limiting_photon_rate", C.c_int64),
         ("group_launches", C.c_int64),
         ("group_workgroups", C.c_int64),
+        ("compute_units", C.c_int32),
+        ("reserved4", C.c_int32),
     ]
 
 

@@ -3848,6 +3848,11 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info) {
         info->group_blocks_per_cu =
             s->fp32 ? group_blocks_per_cu<float>(s->group_block) : group_blocks_per_cu<double>(s->group_block);
     }
+    {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) == hipSuccess)
+            info->compute_units = cus;
+    }
     info->locality_order = s->permuted ? 1 : 0;
     if (s->have_state) {
         unsigned long long f = 0, a = 0, lim[3] = {0, 0, 0};

@@ -107,6 +107,7 @@ def test_thin_jittered_fp64_every_shape(thin16, shape, hip_available):
         assert info.group_levels == 4 and info.group_tasks > DN, "cones of different workgroups should overlap"
         assert info.group_launches > 0 and info.group_workgroups >= info.group_launches
         assert info.group_blocks_per_cu >= 1, "the grouped kernel's occupancy limit should be known"
+        assert info.compute_units >= 1
     for step in range(pb.steps):
         assert g.run_sweeps() == want[step][0]
         io = _compare_step(g, want[step], step, shape)
