@@ -285,7 +285,8 @@ def run_precision(warmup: int, steps: int = 20, autotune: bool = True):
 def shape_of(r, table):
     return {"transport_mode": r["transport_mode"], "replay_block": r["replay_block"],
             "graph_replay": r["graph_replay"], "narrow_runs": r["narrow_runs"], "group_levels": r["group_levels"],
-            "group_block": r["group_block"], "autotune_ms_per_step": table}
+            "group_block": r["group_block"], "group_tasks_per_full_sweep": r["group_tasks_per_full_sweep"],
+            "group_launches_per_full_sweep": int(r["info"].group_launches), "autotune_ms_per_step": table}
 
 
 def cpu_model() -> str:
@@ -327,11 +328,12 @@ def run_cpu_baseline(pb, directions: int = 16):
                       f"one thread pinned to core {core}"}
 
 
-def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, steps: int = 10):
+def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, steps: int = 10, max_batch: int = 0):
     """The serving loop on one server: first all `count` requests submitted at
     once (saturated throughput; grids become resident as they first arrive),
     then the same requests with Poisson arrivals at 0.8 x that completion rate
-    (latency at 80 % load; grids resident from the first pass)."""
+    (latency at 80 % load; grids resident from the first pass). max_batch >= 2:
+    the running requests are stepped together in packed steps."""
     import threading
     import numpy as np
     from subsweep_amd.serve import Server
@@ -354,14 +356,15 @@ def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, step
                 "updates_per_s": sum(r.updates for r in res) / span,
                 "reused_grid_fraction": float(np.mean([r.reused for r in res]))}
 
-    with Server(slots=slots, hbm_budget_bytes=budget) as sv:
+    with Server(slots=slots, hbm_budget_bytes=budget, max_batch=max_batch) as sv:
         t0 = time.perf_counter()
         tickets = [sv.submit(r, steps, grid_key=r.pool_index + 1) for r in reqs]
         sat = [sv.wait(x) for x in tickets]
         span = time.perf_counter() - t0
         saturated = summary(sat, span)
         capacity = count / span
-        log(f"serving saturated: {capacity:.1f} requests/s, {saturated['updates_per_s'] / 1e9:.3f} G updates/s, "
+        log(f"serving{' packed' if max_batch else ''} saturated: {capacity:.1f} requests/s, "
+            f"{saturated['updates_per_s'] / 1e9:.3f} G updates/s, "
             f"{saturated['reused_grid_fraction']:.2f} of the requests on resident grids")
         rate = 0.8 * capacity  # Poisson arrivals at 80 % of the measured capacity
         gaps = np.random.default_rng(seed + 2).exponential(1.0 / rate, size=count)
@@ -404,6 +407,8 @@ def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, step
         "updates_per_s": poisson["updates_per_s"],
         "reused_grid_fraction": poisson["reused_grid_fraction"],
         "saturated": saturated,
+        "max_batch": max_batch,
+        "packed_requests_per_step": stats["packed_requests_per_step"],
         "peak_running": stats["peak_running"],
         "peak_reserved_gb": stats["peak_reserved_bytes"] / (1 << 30),
     }
@@ -424,6 +429,9 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=256,
                     help="requests of the serving-loop measurement (0: skip it)")
     ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--serving-packed-slots", type=int, default=8,
+                    help="slots of the packed serving pass (requests stepped together, up to --serving-max-batch)")
+    ap.add_argument("--serving-max-batch", type=int, default=8, help="0: skip the packed serving pass")
     ap.add_argument("--no-precision-workload", action="store_true",
                     help="skip the 262144-cell FP32-vs-FP64 precision workload")
     ap.add_argument("--largest", action="store_true",
@@ -510,6 +518,13 @@ def main():
         serving = run_serving(args.serving_requests, args.serving_slots, 200.0)
         log(f"serving: p50 {serving['p50_latency_ms']:.1f} ms, p99 {serving['p99_latency_ms']:.1f} ms, "
             f"{serving['updates_per_s'] / 1e9:.3f} G updates/s, peak {serving['peak_running']} running")
+        if args.serving_max_batch >= 2:
+            packed = run_serving(args.serving_requests, args.serving_packed_slots, 200.0,
+                                 max_batch=args.serving_max_batch)
+            log(f"serving packed: p50 {packed['p50_latency_ms']:.1f} ms, p99 {packed['p99_latency_ms']:.1f} ms, "
+                f"{packed['updates_per_s'] / 1e9:.3f} G updates/s, {packed['packed_requests_per_step']:.2f} "
+                f"requests per packed step")
+            serving["packed"] = packed
 
     info = fp32["info"]
     line = {

@@ -113,6 +113,17 @@ int ssw_upload_state(ssw_sweep* s, const double* density, const double* ionized_
  * run_sweep_system call (src/sweep/mod.rs:708-714); that is the caller's job. */
 int ssw_step(ssw_sweep* s, double* time_elapsed);
 
+/* ssw_step of n handles at once (no counterpart in the reference, which
+ * steps one simulation; for callers that hold many independent problems, as
+ * the request loop does): the grouped launches of handles with the same
+ * transport precision and group block are packed into shared launches, up to
+ * 8 problems per launch, so that small problems fill the device together.
+ * Each handle's results are bitwise those of ssw_step on it alone; a handle's
+ * stream continues after the packed step. time_elapsed (n entries, may be
+ * NULL) receives each handle's elapsed time. All handles on one device, none
+ * twice, none used by another thread during the call. */
+int ssw_step_batch(ssw_sweep* const* handles, int32_t n, double* time_elapsed);
+
 /* Copies results back, like the tail of run_sweep_system (src/sweep/mod.rs:715-738).
  * Any pointer may be NULL. photon_rate = sum over directions of the incoming
  * rate (no source, no periodic term, src/sweep/mod.rs:727-730). */
@@ -451,6 +462,16 @@ typedef struct ssw_request_result {
 } ssw_request_result;
 
 int ssw_server_create(int32_t slots, int64_t hbm_budget_bytes, ssw_server** out);
+/* A server whose requests are stepped together: the slot workers set
+ * requests up (handle, grid, state) and download their results, and one
+ * stepper thread steps every set-up request, up to max_batch per iteration,
+ * with ssw_step_batch (their grouped launches packed into shared launches).
+ * max_batch < 2 gives ssw_server_create's server. Results are bitwise those
+ * of the unpacked server. */
+int ssw_server_create_packed(int32_t slots, int64_t hbm_budget_bytes, int32_t max_batch, ssw_server** out);
+/* Packed servers: stepper iterations so far and the steps they took (steps /
+ * batches = mean requests per packed step). */
+int ssw_server_pack_stats(ssw_server* server, int64_t* batches, int64_t* batched_steps);
 /* Queues a copy of *request (not of its arrays); *ticket identifies it. */
 int ssw_server_submit(ssw_server* server, const ssw_request* request, int64_t* ticket);
 /* Blocks until the request has finished; each ticket can be waited on once. */

@@ -152,6 +152,7 @@ SWEEP_SIGNATURES = {
     "ssw_upload_grid": (C.c_int, [P, C.c_int64, PI64, PI32, PU8, PD, PD, PD, PD]),
     "ssw_upload_state": (C.c_int, [P, PD, PD, PD, PD]),
     "ssw_step": (C.c_int, [P, PD]),
+    "ssw_step_batch": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, PD]),
     "ssw_download": (C.c_int, [P, PD, PD, PD, PD, PU8, PD]),
     "ssw_download_noalloc": (C.c_int, [P, PD, PD, PD, PD, PU8, PD]),
     "ssw_rotate_directions": (C.c_int, [P, PD]),
@@ -188,6 +189,8 @@ SWEEP_SIGNATURES = {
     "ssw_time_between_scale_factors": (C.c_double, [C.c_double] * 5),
     "ssw_star_sources": (C.c_int64, [C.c_int64, PD, PD, PD] + [C.c_double] * 5 + [PD, PU8]),
     "ssw_server_create": (C.c_int, [C.c_int32, C.c_int64, C.POINTER(P)]),
+    "ssw_server_create_packed": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, C.POINTER(P)]),
+    "ssw_server_pack_stats": (C.c_int, [P, PI64, PI64]),
     "ssw_server_submit": (C.c_int, [P, C.POINTER(RequestC), PI64]),
     "ssw_server_wait": (C.c_int, [P, C.c_int64, C.POINTER(RequestResultC)]),
     "ssw_server_stats": (C.c_int, [P, PI64, PI32]),

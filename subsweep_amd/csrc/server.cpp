@@ -17,6 +17,13 @@
 // the budget; when a request does not fit, idle handles are freed least
 // recently used first. Other requests free their handle when they complete.
 //
+// Packed stepping (ssw_server_create_packed): the workers only set requests
+// up and finish them; one stepper thread steps every request that is set up,
+// up to max_batch at a time, with ssw_step_batch, one step each per
+// iteration, so the grouped launches of the running requests are packed into
+// shared launches. A request joins at the next iteration after its set-up and
+// leaves after its last step (continuous batching at step granularity).
+//
 // The reference has no counterpart (it runs one simulation per process); the
 // serving loop is how this library is driven with many independent solves.
 
@@ -51,6 +58,11 @@ struct Req {
     int64_t updates = 0;
     int64_t device_bytes = 0;
     bool done = false;
+    // packed stepping: the handle while set up, steps taken, and whether the
+    // stepper has finished with it
+    ssw_sweep* handle = nullptr;
+    int32_t steps_done = 0;
+    bool stepped = false;
 };
 
 double now_s(std::chrono::steady_clock::time_point t0) {
@@ -114,6 +126,13 @@ struct ssw_server {
     int peak_running = 0;
     bool stop = false;
     std::vector<std::thread> workers;
+    // packed stepping (max_batch >= 2): requests set up and waiting for steps
+    int max_batch = 0;
+    std::deque<Req*> ready;
+    std::condition_variable cv_ready, cv_stepped;
+    std::thread stepper;
+    bool stop_stepper = false;
+    int64_t batches = 0, batched_steps = 0;
     // idle cached handles by key, and the device memory they hold
     std::unordered_map<std::string, std::vector<Idle>> idle;
     int64_t cached_bytes = 0;
@@ -122,20 +141,17 @@ struct ssw_server {
 
 namespace {
 
-// Runs the request; returns its handle if it should go to the cache (else
-// the handle has been freed).
-ssw_sweep* run_request(Req* q) {
+// Creates or reuses the request's handle and uploads grid and state; returns
+// the handle, or nullptr with the request's status set.
+ssw_sweep* setup_request(Req* q) {
     ssw_sweep* s = q->cached;
     const ssw_request& r = q->r;
-    auto fail_with = [&](int code) {
-        q->status = code;
-        q->error = ssw_last_error();
-    };
     int rc = SSW_OK;
     if (!s) {
         rc = ssw_create(&r.params, &s);
         if (rc != SSW_OK) {
-            fail_with(rc);
+            q->status = rc;
+            q->error = ssw_last_error();
             return nullptr;
         }
         rc = ssw_set_transport_mode(s, q->mode);
@@ -146,22 +162,126 @@ ssw_sweep* run_request(Req* q) {
                                  r.face_normal, r.cell_volume, r.cell_size);
     }
     if (rc == SSW_OK) rc = ssw_upload_state(s, r.density, r.ionized_fraction, r.temperature, r.source);
-    ssw_info info;
-    for (int32_t k = 0; rc == SSW_OK && k < r.steps; k++) {
-        double el = 0.0;
-        rc = ssw_step(s, &el);
-        if (rc == SSW_OK) rc = ssw_get_info(s, &info);
-        if (rc == SSW_OK) q->updates += info.updates_last_step;
+    if (rc != SSW_OK) {
+        q->status = rc;
+        q->error = ssw_last_error();
+        ssw_destroy(s);
+        return nullptr;
     }
+    return s;
+}
+
+// Downloads the results of a stepped request; returns its handle if it should
+// go to the cache (else the handle has been freed).
+ssw_sweep* finish_request(Req* q, ssw_sweep* s) {
+    const ssw_request& r = q->r;
+    int rc = q->status;
+    ssw_info info;
     if (rc == SSW_OK)
         rc = ssw_download(s, r.out_ionized_fraction, r.out_temperature, r.out_timestep, r.out_photon_rate, r.out_level,
                           r.out_change_timescale);
     if (rc == SSW_OK) rc = ssw_get_info(s, &info);
     if (rc == SSW_OK) q->device_bytes = info.peak_device_bytes;
-    if (rc != SSW_OK) fail_with(rc);
+    if (rc != SSW_OK && q->status == SSW_OK) {
+        q->status = rc;
+        q->error = ssw_last_error();
+    }
     if (rc == SSW_OK && !q->key.empty()) return s;
     ssw_destroy(s);
     return nullptr;
+}
+
+// Runs the request on its worker; returns its handle if it should go to the
+// cache (else the handle has been freed).
+ssw_sweep* run_request(ssw_server* sv, Req* q) {
+    ssw_sweep* s = setup_request(q);
+    if (!s) return nullptr;
+    if (sv->max_batch >= 2) {  // the stepper steps it, packed with the others
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            q->handle = s;
+            sv->ready.push_back(q);
+        }
+        sv->cv_ready.notify_all();
+        std::unique_lock<std::mutex> lk(sv->mu);
+        sv->cv_stepped.wait(lk, [&] { return q->stepped; });
+    } else {
+        ssw_info info;
+        int rc = SSW_OK;
+        for (int32_t k = 0; rc == SSW_OK && k < q->r.steps; k++) {
+            double el = 0.0;
+            rc = ssw_step(s, &el);
+            if (rc == SSW_OK) rc = ssw_get_info(s, &info);
+            if (rc == SSW_OK) q->updates += info.updates_last_step;
+        }
+        if (rc != SSW_OK) {
+            q->status = rc;
+            q->error = ssw_last_error();
+        }
+    }
+    return finish_request(q, s);
+}
+
+// Packed stepping: one step of up to max_batch set-up requests per
+// iteration; requests leave after their last step (or a failed one).
+void stepper(ssw_server* sv) {
+    std::vector<Req*> batch;
+    std::vector<ssw_sweep*> hs;
+    std::vector<double> el;
+    for (;;) {
+        batch.clear();
+        {
+            std::unique_lock<std::mutex> lk(sv->mu);
+            sv->cv_ready.wait(lk, [&] { return sv->stop_stepper || !sv->ready.empty(); });
+            if (sv->ready.empty()) return;
+            for (Req* q : sv->ready) {
+                if ((int)batch.size() == sv->max_batch) break;
+                batch.push_back(q);
+            }
+        }
+        hs.clear();
+        std::vector<Req*> run;
+        for (Req* q : batch) {
+            if (q->steps_done < q->r.steps) {
+                hs.push_back(q->handle);
+                run.push_back(q);
+            }
+        }
+        int rc = SSW_OK;
+        std::string err;
+        if (!hs.empty()) {
+            el.assign(hs.size(), 0.0);
+            rc = ssw_step_batch(hs.data(), (int32_t)hs.size(), el.data());
+            if (rc != SSW_OK) err = ssw_last_error();
+            for (Req* q : run) {
+                ssw_info info;
+                if (rc == SSW_OK && ssw_get_info(q->handle, &info) == SSW_OK) q->updates += info.updates_last_step;
+                q->steps_done++;
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            if (!hs.empty()) {
+                sv->batches++;
+                sv->batched_steps += (int64_t)hs.size();
+            }
+            for (Req* q : batch) {
+                const bool failed = rc != SSW_OK && std::find(run.begin(), run.end(), q) != run.end();
+                if (failed) {
+                    q->status = rc;
+                    q->error = err;
+                }
+                if (failed || q->steps_done >= q->r.steps) {
+                    q->stepped = true;
+                    sv->ready.erase(std::find(sv->ready.begin(), sv->ready.end(), q));
+                } else {  // to the back, so that more than max_batch requests take turns
+                    sv->ready.erase(std::find(sv->ready.begin(), sv->ready.end(), q));
+                    sv->ready.push_back(q);
+                }
+            }
+        }
+        sv->cv_stepped.notify_all();
+    }
 }
 
 // Takes idle handles out of the cache, least recently used first, until
@@ -252,7 +372,7 @@ void worker(ssw_server* sv) {
             continue;
         }
         evicted.clear();
-        ssw_sweep* keep = run_request(q);
+        ssw_sweep* keep = run_request(sv, q);
         int64_t held = 0;
         if (keep) {
             ssw_info info;
@@ -282,14 +402,29 @@ void worker(ssw_server* sv) {
 
 extern "C" {
 
-int ssw_server_create(int32_t slots, int64_t hbm_budget_bytes, ssw_server** out) {
-    if (!out || slots < 1 || slots > 64 || hbm_budget_bytes <= 0) return SSW_ERR_INVALID;
+int ssw_server_create_packed(int32_t slots, int64_t hbm_budget_bytes, int32_t max_batch, ssw_server** out) {
+    if (!out || slots < 1 || slots > 64 || hbm_budget_bytes <= 0 || max_batch < 0 || max_batch > 64)
+        return SSW_ERR_INVALID;
     ssw_server* sv = new ssw_server();
     sv->slots = slots;
     sv->budget = hbm_budget_bytes;
+    sv->max_batch = max_batch >= 2 ? max_batch : 0;
     sv->t0 = std::chrono::steady_clock::now();
+    if (sv->max_batch) sv->stepper = std::thread(stepper, sv);
     for (int i = 0; i < slots; i++) sv->workers.emplace_back(worker, sv);
     *out = sv;
+    return SSW_OK;
+}
+
+int ssw_server_create(int32_t slots, int64_t hbm_budget_bytes, ssw_server** out) {
+    return ssw_server_create_packed(slots, hbm_budget_bytes, 0, out);
+}
+
+int ssw_server_pack_stats(ssw_server* sv, int64_t* batches, int64_t* batched_steps) {
+    if (!sv) return SSW_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (batches) *batches = sv->batches;
+    if (batched_steps) *batched_steps = sv->batched_steps;
     return SSW_OK;
 }
 
@@ -370,6 +505,14 @@ void ssw_server_destroy(ssw_server* sv) {
     }
     sv->cv_work.notify_all();
     for (std::thread& t : sv->workers) t.join();
+    if (sv->stepper.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            sv->stop_stepper = true;
+        }
+        sv->cv_ready.notify_all();
+        sv->stepper.join();
+    }
     for (auto& kv : sv->idle)
         for (Idle& e : kv.second) ssw_destroy(e.s);
     delete sv;

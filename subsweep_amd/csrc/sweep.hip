@@ -1118,13 +1118,13 @@ __global__ void k_permute_desc(const uint4* in, const uint32_t* perm, uint4* out
 // so that the waits stay exact. Tasks beyond the first E of a level set are
 // solved without this overlap.
 template <typename R, int E>
-__global__ void __launch_bounds__(E) k_sweep_group(Geo g, Flux<R> fl, const uint4* __restrict__ hdr,
-                                                   const R* __restrict__ Av, const Entry<R>* __restrict__ planes,
-                                                   const Entry<R>* __restrict__ ovf,
-                                                   const uint32_t* __restrict__ fstart, int k0, int phases,
-                                                   const uint4* __restrict__ desc, const uint32_t* __restrict__ list,
-                                                   uint32_t list_len, PassTab pt) {
-    const uint32_t w = pass_item(pt, blockIdx.x);
+__device__ __forceinline__ void group_body(const Geo& g, const Flux<R>& fl, const uint4* __restrict__ hdr,
+                                           const R* __restrict__ Av, const Entry<R>* __restrict__ planes,
+                                           const Entry<R>* __restrict__ ovf, const uint32_t* __restrict__ fstart,
+                                           int k0, int phases, const uint4* __restrict__ desc,
+                                           const uint32_t* __restrict__ list, uint32_t list_len, const PassTab& pt,
+                                           uint32_t blk) {
+    const uint32_t w = pass_item(pt, blk);
     const uint4 d0 = desc[2 * w], d1 = desc[2 * w + 1];
     const uint32_t tid = threadIdx.x;
     uint32_t n[kMaxGroup], off[kMaxGroup], bb[kMaxGroup], WW[kMaxGroup], P[kMaxGroup];
@@ -1156,6 +1156,74 @@ __global__ void __launch_bounds__(E) k_sweep_group(Geo g, Flux<R> fl, const uint
             cur = nxt;
         }
     }
+}
+
+template <typename R, int E>
+__global__ void __launch_bounds__(E) k_sweep_group(Geo g, Flux<R> fl, const uint4* __restrict__ hdr,
+                                                   const R* __restrict__ Av, const Entry<R>* __restrict__ planes,
+                                                   const Entry<R>* __restrict__ ovf,
+                                                   const uint32_t* __restrict__ fstart, int k0, int phases,
+                                                   const uint4* __restrict__ desc, const uint32_t* __restrict__ list,
+                                                   uint32_t list_len, PassTab pt) {
+    group_body<R, E>(g, fl, hdr, Av, planes, ovf, fstart, k0, phases, desc, list, list_len, pt, blockIdx.x);
+}
+
+// One problem's grouped launch inside a packed launch: everything the grouped
+// kernel reads of its handle, and its workgroups [start, start + blocks) of
+// the packed grid (start a multiple of kXcds, so that the die-aware order of
+// its descriptors holds as in a launch of its own).
+template <typename R>
+struct GroupJob {
+    int64_t N, Nper;
+    R* X;
+    R* I;
+    R* P;
+    R* offI;
+    const SA<R>* sa;
+    double thr;
+    const uint4* hdr;
+    const R* A;
+    const Entry<R>* planes;
+    const Entry<R>* ovf;
+    const uint32_t* fstart;
+    const uint4* desc;
+    const uint32_t* list;
+    uint32_t list_len;
+    int k0, phases;
+    uint32_t start, blocks;
+    PassTab pt;
+};
+
+constexpr int kMaxPack = 8;  // problems per packed launch (kernel arguments of ~1.5 KB)
+
+template <typename R>
+struct PackArgs {
+    GroupJob<R> job[kMaxPack];
+    int n;
+};
+
+// Grouped launches of up to kMaxPack problems in one launch: workgroup b runs
+// the job whose range holds it, as k_sweep_group would run it alone.
+template <typename R, int E>
+__global__ void __launch_bounds__(E) k_sweep_group_pack(const PackArgs<R> a) {
+    int j = 0;
+    for (int k = 1; k < a.n; k++)
+        if (blockIdx.x >= a.job[k].start) j = k;
+    const GroupJob<R>& J = a.job[j];
+    const uint32_t b = blockIdx.x - J.start;
+    if (b >= J.blocks) return;  // padding up to the next job's die-aligned start
+    Geo g{};
+    g.N = J.N;
+    g.Nper = J.Nper;
+    Flux<R> fl{};
+    fl.X = J.X;
+    fl.I = J.I;
+    fl.P = J.P;
+    fl.offI = J.offI;
+    fl.sa = J.sa;
+    fl.thr = J.thr;
+    group_body<R, E>(g, fl, J.hdr, J.A, J.planes, J.ovf, J.fstart, J.k0, J.phases, J.desc, J.list, J.list_len, J.pt,
+                     b);
 }
 
 // The grouped replay of a restricted sweep (active set {level >= lvl}):
@@ -1704,6 +1772,7 @@ struct ssw_sweep {
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
+    hipEvent_t sync_ev = nullptr;                   // orders the handle's stream against a packed step
     double t_transport_total = 0, t_chem_total = 0, t_build_total = 0;
     int64_t n_transport = 0;
 };
@@ -2517,76 +2586,77 @@ int make_list(ssw_sweep* s, int lvl) {
     return SSW_OK;
 }
 
+// Launches one item of sweep level lvl's replay plan on stream st: a group
+// of level sets (its cones one workgroup each), a run of narrow level sets
+// (one workgroup), or one level set.
+template <typename R>
+void launch_item_t(ssw_sweep* s, const Geo& g, const Flux<R>& fl, int lvl, const ssw_sweep::ReplayItem& it,
+                   hipStream_t st) {
+    const std::vector<uint32_t>& fs = s->full_start;
+    const bool all = lvl == 0 || s->list_full[lvl];
+    const uint32_t* idx = all ? nullptr : s->d_idx[lvl];
+    const Entry<R>* ovf = (const Entry<R>*)s->d_ovf;
+    if (it.group) {
+        const ssw_sweep::Group& gr = s->groups[it.tab];
+        const uint4* dsc = (all ? s->d_gdesc : s->d_gdesc_l[lvl]) + 2 * (size_t)gr.desc0;
+        const uint32_t* gl = all ? s->d_glist : s->d_glist_l[lvl];
+        const uint32_t nblk = all ? gr.ndesc : s->gblocks[lvl][it.tab];  // restricted: workgroups with active tasks
+        const Entry<R>* pl = (const Entry<R>*)s->d_planes;
+        // full sweeps: die-aware order inside each pass; restricted sweeps
+        // launch their filtered descriptors in order
+        const PassTab pt = all ? gr.passes : s->gpass_l[lvl][it.tab];
+        if (s->group_block == 32)
+            k_sweep_group<R, 64><<<nblk, 64, 0, st>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf, s->d_fstart, gr.k0,
+                                                      gr.levels, dsc, gl, s->glist_len, pt);
+        else if (s->group_block == 64)
+            k_sweep_group<R, 128><<<nblk, 128, 0, st>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf, s->d_fstart,
+                                                        gr.k0, gr.levels, dsc, gl, s->glist_len, pt);
+        else
+            k_sweep_group<R, 256><<<nblk, 256, 0, st>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf, s->d_fstart,
+                                                        gr.k0, gr.levels, dsc, gl, s->glist_len, pt);
+        return;
+    }
+    if (it.levels > 0) {  // a run of narrow level sets, one workgroup
+        k_sweep_comp_run<R><<<1, run_block<R>(), 0, st>>>(g, fl, s->d_hdr, (const R*)s->d_A,
+                                                          (const Entry<R>*)s->d_planes, ovf, idx,
+                                                          s->d_runtab[lvl] + it.tab, it.levels);
+        return;
+    }
+    const std::vector<uint32_t>& cs = all ? fs : s->cstart[lvl];
+    const int k = it.k;
+    const uint32_t count = cs[k + 1] - cs[k];
+    const uint32_t b = fs[k], W = fs[k + 1] - fs[k];
+    const uint32_t* ik = idx ? idx + cs[k] : nullptr;
+    if (s->compiled) {
+        const uint4* hdr = s->d_hdr + b;
+        const R* A = (const R*)s->d_A + b;
+        const Entry<R>* planes = (const Entry<R>*)s->d_planes + (size_t)kPlanes * b;
+        switch (s->rec_block) {
+            case 128:
+                k_sweep_comp<R, 128><<<(count + 127) / 128, 128, 0, st>>>(g, fl, hdr, A, planes, ovf, b, W, ik, count);
+                break;
+            case 256:
+                k_sweep_comp<R, 256><<<(count + 255) / 256, 256, 0, st>>>(g, fl, hdr, A, planes, ovf, b, W, ik, count);
+                break;
+            default:
+                k_sweep_comp<R, 64><<<(count + 63) / 64, 64, 0, st>>>(g, fl, hdr, A, planes, ovf, b, W, ik, count);
+        }
+    } else {
+        k_sweep_rec<R><<<(count + kTasksPerBlock - 1) / kTasksPerBlock, kBlock, 0, st>>>(g, fl, s->d_recs + b, ik,
+                                                                                        count);
+    }
+}
+
 // Launches the replay of sweep level lvl's level sets on the handle's stream,
-// one exact-size launch per non-empty level set; with `ev` each launch is
-// bracketed by events.
+// one exact-size launch per plan item; with `ev` each launch is bracketed by
+// events.
 template <typename R>
 void launch_replay_t(ssw_sweep* s, const Geo& g, int lvl, std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev,
                      const std::function<hipEvent_t()>* mark) {
-    Flux<R> fl = make_flux<R>(s);
-    const std::vector<uint32_t>& fs = s->full_start;
-    const bool all = lvl == 0 || s->list_full[lvl];
-    const std::vector<uint32_t>& cs = all ? fs : s->cstart[lvl];
-    const uint32_t* idx = all ? nullptr : s->d_idx[lvl];
-    const Entry<R>* ovf = (const Entry<R>*)s->d_ovf;
+    const Flux<R> fl = make_flux<R>(s);
     for (const ssw_sweep::ReplayItem& it : s->plan[lvl]) {
         hipEvent_t a = ev ? (*mark)() : nullptr;
-        if (it.group) {  // a group of level sets, its cones one workgroup each
-            const ssw_sweep::Group& gr = s->groups[it.tab];
-            const uint4* dsc = (all ? s->d_gdesc : s->d_gdesc_l[lvl]) + 2 * (size_t)gr.desc0;
-            const uint32_t* gl = all ? s->d_glist : s->d_glist_l[lvl];
-            const uint32_t nblk = all ? gr.ndesc : s->gblocks[lvl][it.tab];  // restricted: workgroups with active tasks
-            const Entry<R>* pl = (const Entry<R>*)s->d_planes;
-            // full sweeps: die-aware order inside each pass; restricted sweeps
-            // launch their filtered descriptors in order
-            const PassTab pt = all ? gr.passes : s->gpass_l[lvl][it.tab];
-            if (s->group_block == 32)
-                k_sweep_group<R, 64><<<nblk, 64, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf,
-                                                                    s->d_fstart, gr.k0, gr.levels, dsc, gl,
-                                                                    s->glist_len, pt);
-            else if (s->group_block == 64)
-                k_sweep_group<R, 128><<<nblk, 128, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf,
-                                                                      s->d_fstart, gr.k0, gr.levels, dsc, gl,
-                                                                      s->glist_len, pt);
-            else
-                k_sweep_group<R, 256><<<nblk, 256, 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf,
-                                                                      s->d_fstart, gr.k0, gr.levels, dsc, gl,
-                                                                      s->glist_len, pt);
-            if (ev) ev->push_back({a, (*mark)()});
-            continue;
-        }
-        if (it.levels > 0) {  // a run of narrow level sets, one workgroup
-            k_sweep_comp_run<R><<<1, run_block<R>(), 0, s->stream>>>(g, fl, s->d_hdr, (const R*)s->d_A,
-                                                                 (const Entry<R>*)s->d_planes, ovf, idx,
-                                                                 s->d_runtab[lvl] + it.tab, it.levels);
-            if (ev) ev->push_back({a, (*mark)()});
-            continue;
-        }
-        const int k = it.k;
-        const uint32_t count = cs[k + 1] - cs[k];
-        const uint32_t b = fs[k], W = fs[k + 1] - fs[k];
-        const uint32_t* ik = idx ? idx + cs[k] : nullptr;
-        if (s->compiled) {
-            const uint4* hdr = s->d_hdr + b;
-            const R* A = (const R*)s->d_A + b;
-            const Entry<R>* planes = (const Entry<R>*)s->d_planes + (size_t)kPlanes * b;
-            switch (s->rec_block) {
-                case 128:
-                    k_sweep_comp<R, 128><<<(count + 127) / 128, 128, 0, s->stream>>>(g, fl, hdr, A, planes, ovf, b,
-                                                                                   W, ik, count);
-                    break;
-                case 256:
-                    k_sweep_comp<R, 256><<<(count + 255) / 256, 256, 0, s->stream>>>(g, fl, hdr, A, planes, ovf, b,
-                                                                                   W, ik, count);
-                    break;
-                default:
-                    k_sweep_comp<R, 64><<<(count + 63) / 64, 64, 0, s->stream>>>(g, fl, hdr, A, planes, ovf, b, W,
-                                                                               ik, count);
-            }
-        } else {
-            k_sweep_rec<R><<<(count + kTasksPerBlock - 1) / kTasksPerBlock, kBlock, 0, s->stream>>>(
-                g, fl, s->d_recs + b, ik, count);
-        }
+        launch_item_t<R>(s, g, fl, lvl, it, s->stream);
         if (ev) ev->push_back({a, (*mark)()});
     }
 }
@@ -2740,6 +2810,197 @@ int run_step(ssw_sweep* s, double* elapsed) {
         }
     }
     if (elapsed) *elapsed = el;
+    return SSW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Packed steps: one step of each of several handles, their grouped launches
+// packed into shared launches (k_sweep_group_pack).
+//
+// The handles' sweeps are aligned by index: sweep i of every handle whose
+// step has more than i sweeps runs together. Inside sweep i, the t-th items of
+// the handles' replay plans are launched together: grouped items of handles
+// with the same transport precision and group block go into one packed launch
+// (up to kMaxPack per launch), other items are launched alone. Each handle's
+// items still run in its plan's order (everything is on one stream), and a
+// packed workgroup computes exactly what it computes in a launch of its own,
+// so every handle's results are bitwise those of stepping it alone.
+
+struct BatchSweep {
+    ssw_sweep* s;
+    std::vector<int> order;
+    Geo g;
+};
+
+template <typename R>
+GroupJob<R> group_job(ssw_sweep* s, int lvl, const ssw_sweep::ReplayItem& it, uint32_t* nblk) {
+    const bool all = lvl == 0 || s->list_full[lvl];
+    const ssw_sweep::Group& gr = s->groups[it.tab];
+    const Flux<R> fl = make_flux<R>(s);
+    const Geo g = make_geo(s);
+    GroupJob<R> J;
+    J.N = g.N;
+    J.Nper = g.Nper;
+    J.X = fl.X;
+    J.I = fl.I;
+    J.P = fl.P;
+    J.offI = fl.offI;
+    J.sa = fl.sa;
+    J.thr = fl.thr;
+    J.hdr = s->d_hdr;
+    J.A = (const R*)s->d_A;
+    J.planes = (const Entry<R>*)s->d_planes;
+    J.ovf = (const Entry<R>*)s->d_ovf;
+    J.fstart = s->d_fstart;
+    J.desc = (all ? s->d_gdesc : s->d_gdesc_l[lvl]) + 2 * (size_t)gr.desc0;
+    J.list = all ? s->d_glist : s->d_glist_l[lvl];
+    J.list_len = s->glist_len;
+    J.k0 = gr.k0;
+    J.phases = gr.levels;
+    J.start = 0;
+    *nblk = all ? gr.ndesc : s->gblocks[lvl][it.tab];
+    J.blocks = *nblk;
+    J.pt = all ? gr.passes : s->gpass_l[lvl][it.tab];
+    return J;
+}
+
+// A packed launch under construction: jobs of one precision and group block.
+template <typename R>
+struct Pack {
+    PackArgs<R> a;
+    uint32_t grid = 0;
+};
+
+template <typename R>
+void flush_pack(Pack<R>& p, int roots, hipStream_t st) {
+    if (p.a.n == 0) return;
+    if (roots == 32) k_sweep_group_pack<R, 64><<<p.grid, 64, 0, st>>>(p.a);
+    else if (roots == 64) k_sweep_group_pack<R, 128><<<p.grid, 128, 0, st>>>(p.a);
+    else k_sweep_group_pack<R, 256><<<p.grid, 256, 0, st>>>(p.a);
+    p.a.n = 0;
+    p.grid = 0;
+}
+
+template <typename R>
+void add_job(Pack<R>& p, const GroupJob<R>& J, int roots, hipStream_t st) {
+    if (p.a.n == kMaxPack) flush_pack<R>(p, roots, st);
+    GroupJob<R> j = J;
+    j.start = p.grid;
+    p.grid += (J.blocks + kXcds - 1) / kXcds * kXcds;
+    p.a.job[p.a.n++] = j;
+}
+
+// Packs keyed by (precision, group block).
+struct Packs {
+    Pack<float> f[3];
+    Pack<double> d[3];
+    static int slot(int roots) { return roots == 32 ? 0 : roots == 64 ? 1 : 2; }
+    void flush(hipStream_t st) {
+        const int roots[3] = {32, 64, 128};
+        for (int k = 0; k < 3; k++) {
+            flush_pack<float>(f[k], roots[k], st);
+            flush_pack<double>(d[k], roots[k], st);
+        }
+    }
+};
+
+template <typename R>
+void packed_item(ssw_sweep* s, const Geo& g, int lvl, const ssw_sweep::ReplayItem& it, Pack<R>& pk, hipStream_t st) {
+    if (it.group) {
+        uint32_t nblk = 0;
+        const GroupJob<R> J = group_job<R>(s, lvl, it, &nblk);
+        if (nblk) add_job<R>(pk, J, s->group_block, st);
+        return;
+    }
+    launch_item_t<R>(s, g, make_flux<R>(s), lvl, it, st);
+}
+
+template <typename R>
+void batch_chemistry(ssw_sweep* s, const Geo& g, int lvl, hipStream_t st) {
+    const Flux<R> fl = make_flux<R>(s);
+    k_periodic<R><<<grid_for(std::max<int64_t>(s->Nper, 1) * s->D), kBlock, 0, st>>>(g, fl);
+    k_chemistry<R><<<grid_for(s->N), kBlock, 0, st>>>(g, fl, chem_args(s), lvl);
+}
+
+int step_batch(ssw_sweep* const* hs, int n, double* elapsed) {
+    std::vector<BatchSweep> bs((size_t)n);
+    const hipStream_t st = hs[0]->stream;
+    // per handle, on its own stream: the sweep levels whose active set
+    // changed, and their active-set lists (the lists depend only on the
+    // timestep levels, which change at the end of a step)
+    for (int h = 0; h < n; h++) {
+        ssw_sweep* s = hs[h];
+        BatchSweep& b = bs[h];
+        b.s = s;
+        b.order = s->ts.sweep_order();
+        b.g = make_geo(s);
+        const int L = s->p.num_timestep_levels;
+        s->sweeps_last = 0;
+        s->builds_last = 0;
+        s->t_transport_total = s->t_chem_total = s->t_build_total = 0;
+        s->n_transport = 0;
+        std::vector<uint32_t> need(L, 1u);
+        if (L > 1) {
+            HIP_TRY(hipMemcpyAsync(need.data(), s->d_need, L * 4, hipMemcpyDeviceToHost, s->stream));
+            HIP_TRY(hipStreamSynchronize(s->stream));
+        }
+        for (int l = 1; l < L; l++)
+            if (need[l]) drop_list(s, l);
+        for (int lvl : b.order)
+            if (lvl > 0 && !s->list_ok[lvl]) {
+                TRY(make_list(s, lvl));
+                s->builds_last++;
+            }
+        if (h > 0) {
+            if (!s->sync_ev) HIP_TRY(hipEventCreateWithFlags(&s->sync_ev, hipEventDisableTiming));
+            HIP_TRY(hipEventRecord(s->sync_ev, s->stream));
+            HIP_TRY(hipStreamWaitEvent(st, s->sync_ev, 0));
+        }
+    }
+    for (BatchSweep& b : bs) {
+        k_fill_u32<<<1, 2, 0, st>>>((uint32_t*)b.s->d_active, 0u, 2);
+        k_fill_u32<<<1, 6, 0, st>>>((uint32_t*)b.s->d_limit, 0u, 6);
+    }
+    size_t sweeps = 0;
+    for (const BatchSweep& b : bs) sweeps = std::max(sweeps, b.order.size());
+    Packs pk;
+    for (size_t i = 0; i < sweeps; i++) {
+        size_t items = 0;
+        for (const BatchSweep& b : bs)
+            if (i < b.order.size()) items = std::max(items, b.s->plan[b.order[i]].size());
+        for (size_t t = 0; t < items; t++) {
+            for (BatchSweep& b : bs) {
+                if (i >= b.order.size()) continue;
+                const int lvl = b.order[i];
+                if (t >= b.s->plan[lvl].size()) continue;
+                const ssw_sweep::ReplayItem& it = b.s->plan[lvl][t];
+                const int k = Packs::slot(b.s->group_block);
+                if (b.s->fp32) packed_item<float>(b.s, b.g, lvl, it, pk.f[k], st);
+                else packed_item<double>(b.s, b.g, lvl, it, pk.d[k], st);
+            }
+            pk.flush(st);
+        }
+        for (BatchSweep& b : bs) {
+            if (i >= b.order.size()) continue;
+            if (b.s->fp32) batch_chemistry<float>(b.s, b.g, b.order[i], st);
+            else batch_chemistry<double>(b.s, b.g, b.order[i], st);
+            b.s->sweeps_last++;
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    for (int h = 0; h < n; h++) {
+        ssw_sweep* s = hs[h];
+        const double el = s->ts.current_max_timestep();
+        s->ts.advance();
+        k_levels<<<grid_for(s->N), kBlock, 0, st>>>(bs[h].g, s->d_change, s->d_level, s->p.max_timestep,
+                                                   s->p.timestep_safety_factor, s->ts.lowest);
+        if (elapsed) elapsed[h] = el;
+    }
+    HIP_TRY(hipGetLastError());
+    // the handles' own streams continue after the packed step
+    if (!hs[0]->sync_ev) HIP_TRY(hipEventCreateWithFlags(&hs[0]->sync_ev, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(hs[0]->sync_ev, st));
+    for (int h = 1; h < n; h++) HIP_TRY(hipStreamWaitEvent(hs[h]->stream, hs[0]->sync_ev, 0));
     return SSW_OK;
 }
 
@@ -3068,6 +3329,7 @@ void ssw_destroy(ssw_sweep* s) {
     dfree(s, s->d_active);
     dfree(s, s->d_limit);
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
+    if (s->sync_ev) (void)hipEventDestroy(s->sync_ev);
     if (s->stream) {
         std::shared_lock<std::shared_mutex> lock(g_capture_mu);
         (void)hipStreamDestroy(s->stream);
@@ -3335,6 +3597,19 @@ int ssw_step(ssw_sweep* s, double* elapsed) {
     if (!s) return fail(SSW_ERR_INVALID, "null handle");
     if (!s->have_state) return fail(SSW_ERR_INVALID, "ssw_upload_state must be called first");
     return s->fp32 ? run_step<float>(s, elapsed) : run_step<double>(s, elapsed);
+}
+
+int ssw_step_batch(ssw_sweep* const* handles, int32_t n, double* elapsed) {
+    if (!handles || n < 1) return fail(SSW_ERR_INVALID, "no handles");
+    for (int h = 0; h < n; h++) {
+        if (!handles[h]) return fail(SSW_ERR_INVALID, "null handle");
+        if (!handles[h]->have_state) return fail(SSW_ERR_INVALID, "ssw_upload_state must be called first");
+        if (handles[h]->device != handles[0]->device)
+            return fail(SSW_ERR_INVALID, "the handles of a packed step must be on one device");
+        for (int k = 0; k < h; k++)
+            if (handles[k] == handles[h]) return fail(SSW_ERR_INVALID, "a handle appears twice in a packed step");
+    }
+    return step_batch(handles, n, elapsed);
 }
 
 int ssw_rotate_directions(ssw_sweep* s, const double* matrix) {

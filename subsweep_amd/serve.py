@@ -6,6 +6,10 @@ HIP stream, admitted in submission order against an HBM budget.
         t = sv.submit(problem, steps=10, grid_key=7)
         r = sv.wait(t)        # RequestResult: status, times, outputs
 
+With `max_batch` >= 2 the running requests are stepped together: one step
+of each per iteration, their grouped launches packed into shared launches
+(`ssw_step_batch`), with bitwise the same results.
+
 A non-zero `grid_key` keeps the request's grid resident: its handle (grid,
 dependency graph, schedule) stays cached on the device after the request,
 and a later request of the same key and parameters reuses it.
@@ -62,10 +66,12 @@ def estimate_device_bytes(problem: Problem, transport_mode: int) -> int:
 
 
 class Server:
-    def __init__(self, slots: int = 4, hbm_budget_bytes: int = 200 << 30):
+    def __init__(self, slots: int = 4, hbm_budget_bytes: int = 200 << 30, max_batch: int = 0):
+        """`max_batch` >= 2: requests are stepped together, up to max_batch per
+        packed step (`ssw_server_create_packed`); else each on its own worker."""
         self._lib = N.sweep_lib()
         h = C.c_void_p()
-        N.check(self._lib.ssw_server_create(int(slots), int(hbm_budget_bytes), C.byref(h)))
+        N.check(self._lib.ssw_server_create_packed(int(slots), int(hbm_budget_bytes), int(max_batch), C.byref(h)))
         self._h = h
         self._pending = {}
 
@@ -115,7 +121,10 @@ class Server:
     def stats(self):
         b, n = C.c_int64(), C.c_int32()
         N.check(self._lib.ssw_server_stats(self._h, C.byref(b), C.byref(n)))
-        return {"peak_reserved_bytes": b.value, "peak_running": n.value}
+        nb, ns = C.c_int64(), C.c_int64()
+        N.check(self._lib.ssw_server_pack_stats(self._h, C.byref(nb), C.byref(ns)))
+        return {"peak_reserved_bytes": b.value, "peak_running": n.value, "packed_steps": nb.value,
+                "packed_requests_per_step": ns.value / nb.value if nb.value else 0.0}
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -13,6 +13,9 @@
     on a one-slot server and all at once on a four-slot server, and every
     output field is bitwise equal (the kernels' arithmetic does not depend on
     scheduling, the replay form or what else runs);
+  * packed stepping (max_batch 8: the running requests stepped together,
+    their grouped launches packed into shared launches): the same results,
+    bitwise, as the one-slot server;
   * a one-slot solve equals a direct `Sweep` life cycle of the same problem;
   * admission: requests whose estimates do not fit the budget together never
     overlap, a request above the whole budget fails with SSW_ERR_BUDGET, and
@@ -59,15 +62,18 @@ def _same(a, b, what):
         assert np.array_equal(u.view(np.uint8), v.view(np.uint8)), f"{what}: {f} differs"
 
 
-def test_alone_equals_mix(hip_available):
+@pytest.mark.parametrize("max_batch", [0, 8])
+def test_alone_equals_mix(max_batch, hip_available):
     probs = _problems()
     with Server(slots=1) as sv:
         alone = [sv.wait(sv.submit(pb, STEPS)) for pb in probs]
-    with Server(slots=4) as sv:
+    with Server(slots=4, max_batch=max_batch) as sv:
         tickets = [sv.submit(pb, STEPS) for pb in probs]
         mix = [sv.wait(t) for t in tickets]
         stats = sv.stats()
     assert stats["peak_running"] > 1, "the four-slot server never ran two requests at once"
+    if max_batch:
+        assert stats["packed_requests_per_step"] > 1, "no step was packed"
     for i, (a, m) in enumerate(zip(alone, mix)):
         assert a.status == 0 and m.status == 0, (a.error, m.error)
         assert a.updates == m.updates > 0
@@ -134,25 +140,35 @@ def serving_mix():
     return pool, W.serving_requests(pool, 32, 2025, 4)
 
 
-def test_serving_mix_real_sizes_alone_equals_mix(serving_mix, hip_available):
+@pytest.fixture(scope="module")
+def serving_alone(serving_mix):
+    _, reqs = serving_mix
+    with Server(slots=1) as sv:
+        return [sv.wait(sv.submit(r, r.steps)) for r in reqs]
+
+
+@pytest.mark.parametrize("max_batch", [0, 8])
+def test_serving_mix_real_sizes_alone_equals_mix(max_batch, serving_mix, serving_alone, hip_available):
     pool, reqs = serving_mix
     sizes = [g.num_cells for g, _, _ in pool]
     assert min(sizes) >= 8192 and max(sizes) <= 524288 and max(sizes) > 100000
     assert {r.params.directions for r in reqs} >= {16, 84} and {r.params.num_timestep_levels for r in reqs} >= {1, 4}
-    with Server(slots=1) as sv:
-        alone = [sv.wait(sv.submit(r, r.steps)) for r in reqs]
-    with Server(slots=4) as sv:
+    alone = serving_alone
+    with Server(slots=8 if max_batch else 4, max_batch=max_batch) as sv:
         tickets = [sv.submit(r, r.steps, grid_key=r.pool_index + 1) for r in reqs]
         mix = [sv.wait(t) for t in tickets]
-        assert sv.stats()["peak_running"] > 1
+        st = sv.stats()
+        assert st["peak_running"] > 1
+        if max_batch:
+            assert st["packed_requests_per_step"] > 1
     assert any(m.reused for m in mix), "some requests should run on resident grids"
     for i, (a, m) in enumerate(zip(alone, mix)):
         assert a.status == 0 and m.status == 0, (a.error, m.error)
         _same(a, m, f"request {i} ({reqs[i].grid.num_cells} cells) alone vs in the mix")
 
 
-@pytest.mark.parametrize("precision", [64, 32])
-def test_served_small_requests_match_the_oracle(precision, hip_available):
+@pytest.mark.parametrize("precision,max_batch", [(64, 0), (32, 0), (64, 8)])
+def test_served_small_requests_match_the_oracle(precision, max_batch, hip_available):
     # optically thin gas (1e-3 of the mix's density): the serving mix's own
     # gas leaves most rates a few cells from a source near the double (FP64)
     # or float (FP32) underflow, where the relative rate change that picks a
@@ -160,7 +176,7 @@ def test_served_small_requests_match_the_oracle(precision, hip_available):
     pool = W.serving_pool(3, 77, lo=8192, hi=20000)
     reqs = W.serving_requests(pool, 3, 78, 3, precision=precision, rho_scale=1e-3)
     big = W.serving_requests(W.serving_pool(1, 79, lo=200000, hi=300000), 2, 80, 3)
-    with Server(slots=4) as sv:
+    with Server(slots=4, max_batch=max_batch) as sv:
         tickets = [sv.submit(r, r.steps, grid_key=10 + i) for i, r in enumerate(big)]
         tickets += [sv.submit(r, r.steps, grid_key=r.pool_index + 1) for r in reqs]
         out = [sv.wait(t) for t in tickets][len(big):]
