@@ -2867,13 +2867,13 @@ GroupJob<R> group_job(ssw_sweep* s, int lvl, const ssw_sweep::ReplayItem& it, ui
 // A packed launch under construction: jobs of one precision and group block.
 template <typename R>
 struct Pack {
-    PackArgs<R> a;
+    PackArgs<R> a{};  // value-initialised: no jobs yet
     uint32_t grid = 0;
 };
 
 template <typename R>
 void flush_pack(Pack<R>& p, int roots, hipStream_t st) {
-    if (p.a.n == 0) return;
+    if (p.a.n <= 0) return;
     if (roots == 32) k_sweep_group_pack<R, 64><<<p.grid, 64, 0, st>>>(p.a);
     else if (roots == 64) k_sweep_group_pack<R, 128><<<p.grid, 128, 0, st>>>(p.a);
     else k_sweep_group_pack<R, 256><<<p.grid, 256, 0, st>>>(p.a);
@@ -2883,7 +2883,7 @@ void flush_pack(Pack<R>& p, int roots, hipStream_t st) {
 
 template <typename R>
 void add_job(Pack<R>& p, const GroupJob<R>& J, int roots, hipStream_t st) {
-    if (p.a.n == kMaxPack) flush_pack<R>(p, roots, st);
+    if (p.a.n >= kMaxPack) flush_pack<R>(p, roots, st);
     GroupJob<R> j = J;
     j.start = p.grid;
     p.grid += (J.blocks + kXcds - 1) / kXcds * kXcds;
