@@ -86,7 +86,7 @@ def test_packed_more_problems_than_one_launch_holds(hip_available):
     # plan item (8 + 3), and problems whose plans end at different items
     probs = []
     for k in range(11):
-        pb = W.jittered_problem(6 + k % 4, 1.0, "2", 16 + 16 * (k % 3), levels=1 + k % 3, periodic=True, steps=3,
+        pb = W.jittered_problem(6 + k % 4, 1.0, "2", (16, 32, 64)[k % 3], levels=1 + k % 3, periodic=True, steps=3,
                                 precision=64, rho_mean=THIN, seed=2000 + k)
         probs.append((pb.with_params(max_timestep=1e-3 * MYR), {}))
     alone = _alone(probs, 3)
