@@ -123,6 +123,11 @@ int ssw_step(ssw_sweep* s, double* time_elapsed);
  * NULL) receives each handle's elapsed time. All handles on one device, none
  * twice, none used by another thread during the call. */
 int ssw_step_batch(ssw_sweep* const* handles, int32_t n, double* time_elapsed);
+/* Packed sweeps whose handles, sweep levels and replay plans repeat are
+ * captured as one graph the second time they come and replayed from then on
+ * (handles with graph replay off are always launched directly): graphs
+ * captured and graph launches so far, process-wide. */
+int ssw_packed_graph_stats(int64_t* captured, int64_t* replayed);
 
 /* Copies results back, like the tail of run_sweep_system (src/sweep/mod.rs:715-738).
  * Any pointer may be NULL. photon_rate = sum over directions of the incoming

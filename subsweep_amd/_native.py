@@ -153,6 +153,7 @@ SWEEP_SIGNATURES = {
     "ssw_upload_state": (C.c_int, [P, PD, PD, PD, PD]),
     "ssw_step": (C.c_int, [P, PD]),
     "ssw_step_batch": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, PD]),
+    "ssw_packed_graph_stats": (C.c_int, [PI64, PI64]),
     "ssw_download": (C.c_int, [P, PD, PD, PD, PD, PU8, PD]),
     "ssw_download_noalloc": (C.c_int, [P, PD, PD, PD, PD, PU8, PD]),
     "ssw_rotate_directions": (C.c_int, [P, PD]),

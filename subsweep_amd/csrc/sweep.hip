@@ -112,6 +112,7 @@ std::atomic<int> g_live_handles{0};
 // capture holds it exclusively, so that handles driven from several threads
 // (the request loop) capture their replays too.
 std::shared_mutex g_capture_mu;
+std::atomic<uint64_t> g_next_uid{0};
 
 int fail(int code, const std::string& msg) {
     g_err = msg;
@@ -1773,6 +1774,8 @@ struct ssw_sweep {
     bool timing = false;
     std::vector<hipEvent_t> ev;
     hipEvent_t sync_ev = nullptr;                   // orders the handle's stream against a packed step
+    uint64_t uid = 0;                               // process-unique handle id (packed-step graphs)
+    uint64_t epoch = 0;                             // bumped whenever a replay plan or its graph changes
     double t_transport_total = 0, t_chem_total = 0, t_build_total = 0;
     int64_t n_transport = 0;
 };
@@ -1963,6 +1966,7 @@ int download_sa(ssw_sweep* s, std::vector<double2>& sa) {
 }
 
 void drop_list(ssw_sweep* s, int lvl) {
+    s->epoch++;
     if (lvl >= (int)s->list_ok.size()) return;
     s->list_ok[lvl] = 0;
     s->list_full[lvl] = 0;
@@ -2068,6 +2072,7 @@ int64_t group_bytes(int64_t DN, int64_t n, int64_t L, int64_t T) {
 // becomes one single-workgroup launch over a run table entry per level set;
 // every other non-empty level set is one launch.
 int plan_replay(ssw_sweep* s, int lvl) {
+    s->epoch++;  // graphs of packed sweeps captured from the old plan no longer match
     const std::vector<uint32_t>& fs = s->full_start;
     const std::vector<uint32_t>& cs = lvl == 0 ? fs : s->cstart[lvl];
     std::vector<ssw_sweep::ReplayItem>& plan = s->plan[lvl];
@@ -2543,6 +2548,7 @@ int make_group_list(ssw_sweep* s, int lvl) {
     s->list_full[lvl] = (kept == s->group_tasks) ? 1 : 0;
     s->list_ok[lvl] = 1;
     s->replays[lvl] = 0;
+    s->epoch++;
     if (s->graphs[lvl]) {
         (void)hipGraphExecDestroy(s->graphs[lvl]);
         s->graphs[lvl] = nullptr;
@@ -2579,6 +2585,7 @@ int make_list(ssw_sweep* s, int lvl) {
     TRY(plan_replay(s, lvl));
     s->list_ok[lvl] = 1;
     s->replays[lvl] = 0;
+    s->epoch++;
     if (s->graphs[lvl]) {
         (void)hipGraphExecDestroy(s->graphs[lvl]);
         s->graphs[lvl] = nullptr;
@@ -2922,6 +2929,74 @@ void batch_chemistry(ssw_sweep* s, const Geo& g, int lvl, hipStream_t st) {
     k_chemistry<R><<<grid_for(s->N), kBlock, 0, st>>>(g, fl, chem_args(s), lvl);
 }
 
+// Graphs of packed sweeps, keyed by (uid, epoch, sweep level) of every
+// handle in the sweep: a key seen once is captured the second time it comes
+// and replayed from then on, as a handle's own replays are. The epoch of a
+// handle changes whenever one of its replay plans is rebuilt, so a graph is
+// only replayed with the plans (and buffers) it was captured from; graphs of
+// destroyed handles are never matched again (uids are not reused) and leave
+// the cache by age.
+struct PackedGraph {
+    std::vector<uint64_t> key;
+    int device;
+    hipGraphExec_t ge;  // null: seen once, not captured yet
+    uint64_t last_use;
+};
+std::mutex g_pack_mu;
+std::vector<PackedGraph> g_pack_graphs;
+uint64_t g_pack_clock = 0;
+int64_t g_pack_captured = 0, g_pack_replayed = 0;
+constexpr size_t kPackGraphs = 256;
+
+// Under g_pack_mu: the entry of key on device dev, or nullptr.
+PackedGraph* find_packed(const std::vector<uint64_t>& key, int dev) {
+    for (PackedGraph& e : g_pack_graphs)
+        if (e.device == dev && e.key == key) {
+            e.last_use = ++g_pack_clock;
+            return &e;
+        }
+    return nullptr;
+}
+
+// Under g_pack_mu: adds an entry, evicting the least recently used one when full.
+void add_packed(const std::vector<uint64_t>& key, int dev, hipGraphExec_t ge) {
+    if (g_pack_graphs.size() >= kPackGraphs) {
+        size_t old = 0;
+        for (size_t k = 1; k < g_pack_graphs.size(); k++)
+            if (g_pack_graphs[k].last_use < g_pack_graphs[old].last_use) old = k;
+        if (g_pack_graphs[old].ge) (void)hipGraphExecDestroy(g_pack_graphs[old].ge);
+        g_pack_graphs.erase(g_pack_graphs.begin() + (std::ptrdiff_t)old);
+    }
+    g_pack_graphs.push_back(PackedGraph{key, dev, ge, ++g_pack_clock});
+}
+
+// Captures what `enqueue` launches on st as one graph (nothing runs). Returns
+// null if the capture failed; the caller then launches the work directly.
+hipGraphExec_t capture_packed(hipStream_t st, const std::function<void()>& enqueue) {
+    hipGraph_t graph = nullptr;
+    std::unique_lock<std::shared_mutex> lock(g_capture_mu);
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    enqueue();
+    hipError_t ce = hipStreamEndCapture(st, &graph);
+    lock.unlock();
+    if (ce != hipSuccess || !graph) {
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    hipGraphExec_t ge = nullptr;
+    hipError_t ie = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return ge;
+}
+
 int step_batch(ssw_sweep* const* hs, int n, double* elapsed) {
     std::vector<BatchSweep> bs((size_t)n);
     const hipStream_t st = hs[0]->stream;
@@ -2963,29 +3038,57 @@ int step_batch(ssw_sweep* const* hs, int n, double* elapsed) {
     }
     size_t sweeps = 0;
     for (const BatchSweep& b : bs) sweeps = std::max(sweeps, b.order.size());
-    Packs pk;
+    bool graphs = true;
+    for (const BatchSweep& b : bs) graphs = graphs && b.s->graphs_on;
+    const int dev = hs[0]->device;
     for (size_t i = 0; i < sweeps; i++) {
-        size_t items = 0;
-        for (const BatchSweep& b : bs)
-            if (i < b.order.size()) items = std::max(items, b.s->plan[b.order[i]].size());
-        for (size_t t = 0; t < items; t++) {
+        // sweep i of every handle: the t-th plan items together, then each
+        // handle's periodic sources and chemistry
+        const std::function<void()> enqueue = [&]() {
+            Packs pk;
+            size_t items = 0;
+            for (const BatchSweep& b : bs)
+                if (i < b.order.size()) items = std::max(items, b.s->plan[b.order[i]].size());
+            for (size_t t = 0; t < items; t++) {
+                for (BatchSweep& b : bs) {
+                    if (i >= b.order.size()) continue;
+                    const int lvl = b.order[i];
+                    if (t >= b.s->plan[lvl].size()) continue;
+                    const ssw_sweep::ReplayItem& it = b.s->plan[lvl][t];
+                    const int k = Packs::slot(b.s->group_block);
+                    if (b.s->fp32) packed_item<float>(b.s, b.g, lvl, it, pk.f[k], st);
+                    else packed_item<double>(b.s, b.g, lvl, it, pk.d[k], st);
+                }
+                pk.flush(st);
+            }
             for (BatchSweep& b : bs) {
                 if (i >= b.order.size()) continue;
-                const int lvl = b.order[i];
-                if (t >= b.s->plan[lvl].size()) continue;
-                const ssw_sweep::ReplayItem& it = b.s->plan[lvl][t];
-                const int k = Packs::slot(b.s->group_block);
-                if (b.s->fp32) packed_item<float>(b.s, b.g, lvl, it, pk.f[k], st);
-                else packed_item<double>(b.s, b.g, lvl, it, pk.d[k], st);
+                if (b.s->fp32) batch_chemistry<float>(b.s, b.g, b.order[i], st);
+                else batch_chemistry<double>(b.s, b.g, b.order[i], st);
             }
-            pk.flush(st);
+        };
+        bool done = false;
+        if (graphs) {
+            std::vector<uint64_t> key;
+            for (const BatchSweep& b : bs)
+                if (i < b.order.size()) key.insert(key.end(), {b.s->uid, b.s->epoch, (uint64_t)b.order[i]});
+            std::lock_guard<std::mutex> lk(g_pack_mu);
+            PackedGraph* e = find_packed(key, dev);
+            if (e && !e->ge) {  // the second time: capture
+                e->ge = capture_packed(st, enqueue);
+                if (e->ge) g_pack_captured++;
+            }
+            if (e && e->ge) {
+                HIP_TRY(hipGraphLaunch(e->ge, st));
+                g_pack_replayed++;
+                done = true;
+            } else if (!e) {
+                add_packed(key, dev, nullptr);
+            }
         }
-        for (BatchSweep& b : bs) {
-            if (i >= b.order.size()) continue;
-            if (b.s->fp32) batch_chemistry<float>(b.s, b.g, b.order[i], st);
-            else batch_chemistry<double>(b.s, b.g, b.order[i], st);
-            b.s->sweeps_last++;
-        }
+        if (!done) enqueue();
+        for (BatchSweep& b : bs)
+            if (i < b.order.size()) b.s->sweeps_last++;
         HIP_TRY(hipGetLastError());
     }
     for (int h = 0; h < n; h++) {
@@ -3280,6 +3383,7 @@ int ssw_create(const ssw_sweep_parameters* params, ssw_sweep** out) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(SSW_ERR_NO_DEVICE, "no HIP device");
     ssw_sweep* s = new ssw_sweep();
+    s->uid = g_next_uid.fetch_add(1) + 1;
     g_live_handles++;
     s->p = p;
     s->p.explicit_directions = nullptr;
@@ -3353,6 +3457,7 @@ int ssw_set_replay_block(ssw_sweep* s, int32_t block) {
     if (block != 0 && block != 64 && block != 128 && block != 256)
         return fail(SSW_ERR_INVALID, "replay block size must be 0, 64, 128 or 256");
     s->rec_block = block ? block : 64;
+    s->epoch++;
     for (hipGraphExec_t& ge : s->graphs)  // graphs hold the old launch shape
         if (ge) {
             (void)hipGraphExecDestroy(ge);
@@ -3418,6 +3523,7 @@ int ssw_set_narrow_runs(ssw_sweep* s, int32_t enabled) {
 int ssw_set_graph_replay(ssw_sweep* s, int32_t enabled) {
     if (!s) return fail(SSW_ERR_INVALID, "null handle");
     s->graphs_on = enabled != 0;
+    s->epoch++;
     if (!s->graphs_on)
         for (hipGraphExec_t& ge : s->graphs)
             if (ge) {
@@ -3597,6 +3703,13 @@ int ssw_step(ssw_sweep* s, double* elapsed) {
     if (!s) return fail(SSW_ERR_INVALID, "null handle");
     if (!s->have_state) return fail(SSW_ERR_INVALID, "ssw_upload_state must be called first");
     return s->fp32 ? run_step<float>(s, elapsed) : run_step<double>(s, elapsed);
+}
+
+int ssw_packed_graph_stats(int64_t* captured, int64_t* replayed) {
+    std::lock_guard<std::mutex> lk(g_pack_mu);
+    if (captured) *captured = g_pack_captured;
+    if (replayed) *replayed = g_pack_replayed;
+    return SSW_OK;
 }
 
 int ssw_step_batch(ssw_sweep* const* handles, int32_t n, double* elapsed) {

@@ -172,6 +172,13 @@ class Sweep:
         N.check(sweeps[0]._lib.ssw_step_batch(hs, len(sweeps), N.ptr(el, C.c_double)))
         return [float(e) for e in el]
 
+    @staticmethod
+    def packed_graph_stats() -> dict:
+        """Graphs of packed sweeps captured and graph launches so far."""
+        c, r = C.c_int64(), C.c_int64()
+        N.check(N.sweep_lib().ssw_packed_graph_stats(C.byref(c), C.byref(r)))
+        return {"captured": c.value, "replayed": r.value}
+
     def results(self, photon_rate: bool = True, noalloc: bool = False) -> SweepResults:
         """The per-cell outputs (`ssw_download`; with `noalloc`,
         `ssw_download_noalloc`, which stages them through the library's fixed

@@ -103,6 +103,28 @@ def test_packed_more_problems_than_one_launch_holds(hip_available):
         h.close()
 
 
+def test_packed_graph_replays_bitwise(hip_available):
+    # one timestep level: the plans stay, so the packed sweeps are captured on
+    # the second step and replayed from the third; every replay is checked
+    # against the lone steps, with downloads (eager work) between replays
+    probs = []
+    for k, (n, dirs, prec) in enumerate(((8, 84, 32), (10, 32, 32), (7, 16, 64), (9, 64, 64), (8, 84, 32))):
+        pb = W.jittered_problem(n, 1.0, "3", dirs, levels=1, periodic=True, steps=5, precision=prec, rho_mean=THIN,
+                                seed=4000 + k)
+        probs.append((pb.with_params(max_timestep=1e-3 * MYR), {"group_block": 32} if k == 4 else {}))
+    alone = _alone(probs, 5)
+    hs = [Sweep(pb.params, pb.grid, *pb.state(), **kw) for pb, kw in probs]
+    before = Sweep.packed_graph_stats()
+    for step in range(5):
+        Sweep.run_sweeps_packed(hs)
+        for k, h in enumerate(hs):
+            _assert_same(_state(h), alone[k][step][1], f"problem {k} step {step} (graphs)")
+    after = Sweep.packed_graph_stats()
+    assert after["captured"] > before["captured"] and after["replayed"] - before["replayed"] >= 3
+    for h in hs:
+        h.close()
+
+
 def test_packed_step_rejects_bad_batches(problems, hip_available):
     pb, kw = problems[0]
     a = Sweep(pb.params, pb.grid, *pb.state())
