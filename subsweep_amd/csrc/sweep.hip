@@ -20,8 +20,10 @@
 //                         A, and its upwind local faces as (neighbour, weight)
 //                         pairs in kPlanes planes: entry q of the task at
 //                         position p of a level set starting at b with width W
-//                         is planes[kPlanes * b + q * W + (p - b)] (coalesced
-//                         across tasks, addressable without reading the header);
+//                         is planes[kPlanes * b + q * W + (p - b)] in FP64 and
+//                         planes[kPlanes * b + ((q / 2) * W + (p - b)) * 2 + q % 2]
+//                         in FP32, plane pairs of 16 bytes (coalesced across
+//                         tasks, addressable without reading the header);
 //                         entries q >= kPlanes (rare) sit in an overflow list,
 //                         indexed per (direction, cell).
 //              A sweep at timestep level l solves the cells with level >= l.
@@ -83,6 +85,7 @@ constexpr uint32_t kCellMask = (1u << kCellBits) - 1u;
 constexpr int kMaxFaces = 127;             // faces per cell
 constexpr int kQueueCap = kTasksPerBlock * kMaxFaces;
 constexpr int kPlanes = 12;                // upwind entries per task held in planes
+static_assert(kPlanes % 2 == 0, "FP32 plane entries are stored in pairs");
 constexpr int kChunk = 4096;               // compaction chunk (tasks, within one level set)
 constexpr int kRunBlock = 1024;            // workgroup of a run of narrow level sets (tasks per level set)
 constexpr int kGroupT = 64;                // grouped replay: root tasks per workgroup (default; 64 or 128)
@@ -155,6 +158,18 @@ struct __align__(16) Entry<double> {
     uint32_t pad;
     double w;
 };
+
+// Offset of plane entry q of the task at offset p of a level set of width W
+// from the level set's first entry. FP32 entries (8 bytes) are stored in
+// plane pairs, entries 2h and 2h+1 of a task side by side, so that a task
+// reads its kPlanes entries with kPlanes / 2 loads of 16 bytes (half the
+// load instructions, and half the cache-line lookups where a workgroup's
+// tasks sit at scattered positions); FP64 entries (16 bytes) plane by plane.
+template <typename R>
+__host__ __device__ __forceinline__ size_t plane_off(uint32_t q, uint32_t W, uint32_t p) {
+    if constexpr (sizeof(R) == 4) return ((size_t)(q >> 1) * W + p) * 2 + (q & 1u);
+    else return (size_t)q * W + p;
+}
 
 struct Geo {
     int64_t N;
@@ -623,7 +638,7 @@ __global__ void k_schedule(Geo g, double inv_aref, const uint32_t* sorted, const
             e.nb = xpos[(size_t)d * g.N + (nk & kNbrMask)];
             if constexpr (sizeof(R) == 8) e.pad = 0;
             e.w = (R)(fa.w * inv_aref * -dd);
-            if (q < (uint32_t)kPlanes) planes[(size_t)kPlanes * b + (size_t)q * W + (p - b)] = e;
+            if (q < (uint32_t)kPlanes) planes[(size_t)kPlanes * b + plane_off<R>(q, W, p - b)] = e;
             else ovf[os + q - kPlanes] = e;
             q++;
         }
@@ -755,8 +770,20 @@ struct TaskG {
 template <typename R>
 __device__ __forceinline__ void load_task(TaskS<R>& t, const uint4* hdr, const R* Av, const Entry<R>* planes,
                                           uint32_t W, uint32_t p) {
+    if constexpr (sizeof(R) == 4) {  // plane pairs: one 16-byte load per two entries
+        const uint4* pp = reinterpret_cast<const uint4*>(planes);
 #pragma unroll
-    for (int q = 0; q < kPlanes; q++) t.e[q] = planes[(size_t)q * W + p];
+        for (int h = 0; h < kPlanes / 2; h++) {
+            const uint4 v = pp[(size_t)h * W + p];
+            t.e[2 * h].nb = v.x;
+            t.e[2 * h].w = __uint_as_float(v.y);
+            t.e[2 * h + 1].nb = v.z;
+            t.e[2 * h + 1].w = __uint_as_float(v.w);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < kPlanes; q++) t.e[q] = planes[plane_off<R>((uint32_t)q, W, p)];
+    }
     t.h = hdr[p];
     t.A = Av[p];
 }
@@ -915,7 +942,7 @@ __device__ void push_upwind(const uint4* hdr, const Entry<R>* planes, const Entr
                             uint32_t W, uint32_t lo, uint32_t hi, uint32_t* cand, uint32_t* ncand, uint32_t* over) {
     const uint4 h = hdr[P];
     for (uint32_t q = 0; q < h.w; q++) {
-        const uint32_t v = q < (uint32_t)kPlanes ? planes[(size_t)kPlanes * b + (size_t)q * W + (P - b)].nb
+        const uint32_t v = q < (uint32_t)kPlanes ? planes[(size_t)kPlanes * b + plane_off<R>(q, W, P - b)].nb
                                                  : ovf[h.z + q - kPlanes].nb;
         if (v < lo || v >= hi) continue;
         const uint32_t k = atomicAdd(ncand, 1u);
