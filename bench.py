@@ -190,6 +190,71 @@ def run_largest(steps: int = 10):
     }
 
 
+def run_packed(count: int = 16, steps: int = 6, packs=(1, 4)):
+    """Packed steps against separate streams: `count` small problems (one
+    timestep level, 8000-27000 cells, 16-84 directions, FP32 transport)
+    stepped (a) each on its own handle and stream from one thread, (b) as
+    `packs` groups of count/packs problems, each group one ssw_step_batch per
+    step from a thread of its own (the library's graphs of packed sweeps are
+    captured on the second step). Whole steps of all problems, timed after 2
+    warmup steps; results are bitwise the same in every arrangement."""
+    import threading
+    from subsweep_amd import Sweep
+    from subsweep_amd import workloads as W
+    shapes = [(24, 84), (20, 64), (28, 32), (22, 84), (26, 16), (24, 84), (30, 32), (21, 64)]
+    probs = []
+    for k in range(count):
+        n, dirs = shapes[k % len(shapes)]
+        pb = W.jittered_problem(n, 1.0, "8", dirs, levels=1, periodic=True, steps=steps, precision=32, seed=3000 + k)
+        probs.append(pb.with_params(max_timestep=1e-3 * 3.15576e13))
+    out = {"problems": count, "cells": [int(p.grid.num_cells) for p in probs], "steps": steps}
+
+    def timed(step_all, hs):
+        for _ in range(2):
+            step_all()
+        for h in hs:
+            h.synchronize()
+        t = time.perf_counter()
+        upd = 0
+        for _ in range(steps):
+            step_all()
+            for h in hs:
+                h.synchronize()
+            upd += sum(h.info().updates_last_step for h in hs)
+        dt = time.perf_counter() - t
+        return {"ms_per_step": dt / steps * 1e3, "updates_per_s": upd / dt}
+
+    hs = [Sweep(p.params, p.grid, *p.state()) for p in probs]
+
+    def alone():
+        for h in hs:
+            h.run_sweeps()
+    out["separate_streams"] = timed(alone, hs)
+    for h in hs:
+        h.close()
+    for npk in packs:
+        hs = [Sweep(p.params, p.grid, *p.state()) for p in probs]
+        groups = [hs[i::npk] for i in range(npk)]
+
+        def packed():
+            if npk == 1:
+                Sweep.run_sweeps_packed(groups[0])
+                return
+            th = [threading.Thread(target=Sweep.run_sweeps_packed, args=(g,)) for g in groups]
+            for t in th:
+                t.start()
+            for t in th:
+                t.join()
+        out[f"packed_{npk}_groups"] = timed(packed, hs)
+        for h in hs:
+            h.close()
+    for k, v in out.items():
+        if isinstance(v, dict):
+            log(f"packed demo {k}: {v['ms_per_step']:.2f} ms per step of all {count}, "
+                f"{v['updates_per_s'] / 1e9:.3f} G updates/s")
+    return out
+
+
 def run_shard(gpus: int, steps: int = 4, warmup: int = 2):
     """One GPU's share of a direction-sharded run over `gpus` GPUs: the
     2097152-cell clustered grid (4 timestep levels, 1024 sources) with the
@@ -429,11 +494,14 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=256,
                     help="requests of the serving-loop measurement (0: skip it)")
     ap.add_argument("--serving-slots", type=int, default=4)
-    ap.add_argument("--serving-packed-slots", type=int, default=8,
+    ap.add_argument("--serving-packed-slots", type=int, default=16,
                     help="slots of the packed serving pass (requests stepped together, up to --serving-max-batch)")
-    ap.add_argument("--serving-max-batch", type=int, default=8, help="0: skip the packed serving pass")
+    ap.add_argument("--serving-max-batch", type=int, default=4,
+                    help="requests per packed step (slots / max-batch steppers, at most 4); 0: skip the packed pass")
     ap.add_argument("--no-precision-workload", action="store_true",
                     help="skip the 262144-cell FP32-vs-FP64 precision workload")
+    ap.add_argument("--packed-demo", action="store_true",
+                    help="only run the packed-steps comparison (16 small problems) and print its JSON")
     ap.add_argument("--largest", action="store_true",
                     help="only run the 16777216-cell largest-domain measurement and print its JSON")
     ap.add_argument("--mode", type=int, default=0, choices=(0, 3, 4),
@@ -462,6 +530,9 @@ def main():
               "--gpus above 1 is not supported", file=sys.stderr)
         sys.exit(2)
 
+    if args.packed_demo:
+        print(json.dumps(run_packed()), flush=True)
+        return
     if args.largest:
         print(json.dumps(run_largest()), flush=True)
         return

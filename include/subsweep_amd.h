@@ -468,11 +468,12 @@ typedef struct ssw_request_result {
 
 int ssw_server_create(int32_t slots, int64_t hbm_budget_bytes, ssw_server** out);
 /* A server whose requests are stepped together: the slot workers set
- * requests up (handle, grid, state) and download their results, and one
- * stepper thread steps every set-up request, up to max_batch per iteration,
- * with ssw_step_batch (their grouped launches packed into shared launches).
- * max_batch < 2 gives ssw_server_create's server. Results are bitwise those
- * of the unpacked server. */
+ * requests up (handle, grid, state) and download their results, and
+ * slots / max_batch stepper threads (1 to 4) step the set-up requests, each
+ * taking up to max_batch per iteration, with ssw_step_batch (their grouped
+ * launches packed into shared launches). max_batch < 2 gives
+ * ssw_server_create's server. Results are bitwise those of the unpacked
+ * server. */
 int ssw_server_create_packed(int32_t slots, int64_t hbm_budget_bytes, int32_t max_batch, ssw_server** out);
 /* Packed servers: stepper iterations so far and the steps they took (steps /
  * batches = mean requests per packed step). */

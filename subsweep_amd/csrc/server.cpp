@@ -18,11 +18,14 @@
 // recently used first. Other requests free their handle when they complete.
 //
 // Packed stepping (ssw_server_create_packed): the workers only set requests
-// up and finish them; one stepper thread steps every request that is set up,
-// up to max_batch at a time, with ssw_step_batch, one step each per
-// iteration, so the grouped launches of the running requests are packed into
-// shared launches. A request joins at the next iteration after its set-up and
-// leaves after its last step (continuous batching at step granularity).
+// up and finish them; stepper threads step the requests that are set up, each
+// taking up to max_batch of them per iteration and stepping them once with
+// ssw_step_batch, so their grouped launches are packed into shared launches.
+// A request joins at the next iteration after its set-up and leaves after its
+// last step (continuous batching at step granularity). There are
+// slots / max_batch stepper threads (1 to 4): the device runs launches of up
+// to four streams at once, and a packed launch lasts as long as its slowest
+// problem, so a few small packs side by side beat one large one.
 //
 // The reference has no counterpart (it runs one simulation per process); the
 // serving loop is how this library is driven with many independent solves.
@@ -130,7 +133,7 @@ struct ssw_server {
     int max_batch = 0;
     std::deque<Req*> ready;
     std::condition_variable cv_ready, cv_stepped;
-    std::thread stepper;
+    std::vector<std::thread> steppers;
     bool stop_stepper = false;
     int64_t batches = 0, batched_steps = 0;
     // idle cached handles by key, and the device memory they hold
@@ -196,7 +199,7 @@ ssw_sweep* finish_request(Req* q, ssw_sweep* s) {
 ssw_sweep* run_request(ssw_server* sv, Req* q) {
     ssw_sweep* s = setup_request(q);
     if (!s) return nullptr;
-    if (sv->max_batch >= 2) {  // the stepper steps it, packed with the others
+    if (sv->max_batch >= 2 && q->r.steps > 0) {  // the steppers step it, packed with others
         {
             std::lock_guard<std::mutex> lk(sv->mu);
             q->handle = s;
@@ -223,7 +226,8 @@ ssw_sweep* run_request(ssw_server* sv, Req* q) {
 }
 
 // Packed stepping: one step of up to max_batch set-up requests per
-// iteration; requests leave after their last step (or a failed one).
+// iteration; requests leave after their last step (or a failed one). A
+// request is out of the ready queue while a stepper steps it.
 void stepper(ssw_server* sv) {
     std::vector<Req*> batch;
     std::vector<ssw_sweep*> hs;
@@ -234,53 +238,36 @@ void stepper(ssw_server* sv) {
             std::unique_lock<std::mutex> lk(sv->mu);
             sv->cv_ready.wait(lk, [&] { return sv->stop_stepper || !sv->ready.empty(); });
             if (sv->ready.empty()) return;
-            for (Req* q : sv->ready) {
-                if ((int)batch.size() == sv->max_batch) break;
-                batch.push_back(q);
+            while (!sv->ready.empty() && (int)batch.size() < sv->max_batch) {
+                batch.push_back(sv->ready.front());
+                sv->ready.pop_front();
             }
         }
         hs.clear();
-        std::vector<Req*> run;
+        for (Req* q : batch) hs.push_back(q->handle);
+        el.assign(hs.size(), 0.0);
+        const int rc = ssw_step_batch(hs.data(), (int32_t)hs.size(), el.data());
+        const std::string err = rc != SSW_OK ? ssw_last_error() : std::string();
         for (Req* q : batch) {
-            if (q->steps_done < q->r.steps) {
-                hs.push_back(q->handle);
-                run.push_back(q);
-            }
-        }
-        int rc = SSW_OK;
-        std::string err;
-        if (!hs.empty()) {
-            el.assign(hs.size(), 0.0);
-            rc = ssw_step_batch(hs.data(), (int32_t)hs.size(), el.data());
-            if (rc != SSW_OK) err = ssw_last_error();
-            for (Req* q : run) {
-                ssw_info info;
-                if (rc == SSW_OK && ssw_get_info(q->handle, &info) == SSW_OK) q->updates += info.updates_last_step;
-                q->steps_done++;
-            }
+            ssw_info info;
+            if (rc == SSW_OK && ssw_get_info(q->handle, &info) == SSW_OK) q->updates += info.updates_last_step;
+            q->steps_done++;
         }
         {
             std::lock_guard<std::mutex> lk(sv->mu);
-            if (!hs.empty()) {
-                sv->batches++;
-                sv->batched_steps += (int64_t)hs.size();
-            }
+            sv->batches++;
+            sv->batched_steps += (int64_t)hs.size();
             for (Req* q : batch) {
-                const bool failed = rc != SSW_OK && std::find(run.begin(), run.end(), q) != run.end();
-                if (failed) {
+                if (rc != SSW_OK) {
                     q->status = rc;
                     q->error = err;
                 }
-                if (failed || q->steps_done >= q->r.steps) {
-                    q->stepped = true;
-                    sv->ready.erase(std::find(sv->ready.begin(), sv->ready.end(), q));
-                } else {  // to the back, so that more than max_batch requests take turns
-                    sv->ready.erase(std::find(sv->ready.begin(), sv->ready.end(), q));
-                    sv->ready.push_back(q);
-                }
+                if (rc != SSW_OK || q->steps_done >= q->r.steps) q->stepped = true;
+                else sv->ready.push_back(q);  // to the back: more than max_batch requests take turns
             }
         }
         sv->cv_stepped.notify_all();
+        sv->cv_ready.notify_all();
     }
 }
 
@@ -410,7 +397,10 @@ int ssw_server_create_packed(int32_t slots, int64_t hbm_budget_bytes, int32_t ma
     sv->budget = hbm_budget_bytes;
     sv->max_batch = max_batch >= 2 ? max_batch : 0;
     sv->t0 = std::chrono::steady_clock::now();
-    if (sv->max_batch) sv->stepper = std::thread(stepper, sv);
+    if (sv->max_batch) {
+        const int n = std::max(1, std::min(4, slots / sv->max_batch));
+        for (int i = 0; i < n; i++) sv->steppers.emplace_back(stepper, sv);
+    }
     for (int i = 0; i < slots; i++) sv->workers.emplace_back(worker, sv);
     *out = sv;
     return SSW_OK;
@@ -505,13 +495,13 @@ void ssw_server_destroy(ssw_server* sv) {
     }
     sv->cv_work.notify_all();
     for (std::thread& t : sv->workers) t.join();
-    if (sv->stepper.joinable()) {
+    if (!sv->steppers.empty()) {
         {
             std::lock_guard<std::mutex> lk(sv->mu);
             sv->stop_stepper = true;
         }
         sv->cv_ready.notify_all();
-        sv->stepper.join();
+        for (std::thread& t : sv->steppers) t.join();
     }
     for (auto& kv : sv->idle)
         for (Idle& e : kv.second) ssw_destroy(e.s);

@@ -13,9 +13,9 @@
     on a one-slot server and all at once on a four-slot server, and every
     output field is bitwise equal (the kernels' arithmetic does not depend on
     scheduling, the replay form or what else runs);
-  * packed stepping (max_batch 8: the running requests stepped together,
-    their grouped launches packed into shared launches): the same results,
-    bitwise, as the one-slot server;
+  * packed stepping (max_batch 8: one stepper packs up to 8 running requests
+    per step; max_batch 2: several steppers, each packing 2, side by side):
+    the same results, bitwise, as the one-slot server;
   * a one-slot solve equals a direct `Sweep` life cycle of the same problem;
   * admission: requests whose estimates do not fit the budget together never
     overlap, a request above the whole budget fails with SSW_ERR_BUDGET, and
@@ -62,7 +62,7 @@ def _same(a, b, what):
         assert np.array_equal(u.view(np.uint8), v.view(np.uint8)), f"{what}: {f} differs"
 
 
-@pytest.mark.parametrize("max_batch", [0, 8])
+@pytest.mark.parametrize("max_batch", [0, 8, 2])
 def test_alone_equals_mix(max_batch, hip_available):
     probs = _problems()
     with Server(slots=1) as sv:
@@ -72,7 +72,7 @@ def test_alone_equals_mix(max_batch, hip_available):
         mix = [sv.wait(t) for t in tickets]
         stats = sv.stats()
     assert stats["peak_running"] > 1, "the four-slot server never ran two requests at once"
-    if max_batch:
+    if max_batch == 8:  # one stepper: the requests set up while it steps join its next packed step
         assert stats["packed_requests_per_step"] > 1, "no step was packed"
     for i, (a, m) in enumerate(zip(alone, mix)):
         assert a.status == 0 and m.status == 0, (a.error, m.error)
@@ -147,7 +147,7 @@ def serving_alone(serving_mix):
         return [sv.wait(sv.submit(r, r.steps)) for r in reqs]
 
 
-@pytest.mark.parametrize("max_batch", [0, 8])
+@pytest.mark.parametrize("max_batch", [0, 8, 2])
 def test_serving_mix_real_sizes_alone_equals_mix(max_batch, serving_mix, serving_alone, hip_available):
     pool, reqs = serving_mix
     sizes = [g.num_cells for g, _, _ in pool]
@@ -159,7 +159,7 @@ def test_serving_mix_real_sizes_alone_equals_mix(max_batch, serving_mix, serving
         mix = [sv.wait(t) for t in tickets]
         st = sv.stats()
         assert st["peak_running"] > 1
-        if max_batch:
+        if max_batch == 8:
             assert st["packed_requests_per_step"] > 1
     assert any(m.reused for m in mix), "some requests should run on resident grids"
     for i, (a, m) in enumerate(zip(alone, mix)):
