@@ -193,11 +193,12 @@ def run_largest(steps: int = 10):
 def run_packed(count: int = 16, steps: int = 6, packs=(1, 4)):
     """Packed steps against separate streams: `count` small problems (one
     timestep level, 8000-27000 cells, 16-84 directions, FP32 transport)
-    stepped (a) each on its own handle and stream from one thread, (b) as
-    `packs` groups of count/packs problems, each group one ssw_step_batch per
-    step from a thread of its own (the library's graphs of packed sweeps are
-    captured on the second step). Whole steps of all problems, timed after 2
-    warmup steps; results are bitwise the same in every arrangement."""
+    stepped (a) each on its own handle and stream from one thread, (b) with
+    one ssw_step_batch call per step (the library cuts them into packs of
+    four on four streams), (c) as 4 groups of count/4, each group one
+    ssw_step_batch per step from a thread of its own. Graphs of packed sweeps
+    are captured on the second step. Whole steps of all problems, timed
+    after 2 warmup steps; results are bitwise the same in every arrangement."""
     import threading
     from subsweep_amd import Sweep
     from subsweep_amd import workloads as W
@@ -245,7 +246,7 @@ def run_packed(count: int = 16, steps: int = 6, packs=(1, 4)):
                 t.start()
             for t in th:
                 t.join()
-        out[f"packed_{npk}_groups"] = timed(packed, hs)
+        out["packed_one_call" if npk == 1 else f"packed_{npk}_threads"] = timed(packed, hs)
         for h in hs:
             h.close()
     for k, v in out.items():

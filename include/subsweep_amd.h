@@ -115,13 +115,15 @@ int ssw_step(ssw_sweep* s, double* time_elapsed);
 
 /* ssw_step of n handles at once (no counterpart in the reference, which
  * steps one simulation; for callers that hold many independent problems, as
- * the request loop does): the grouped launches of handles with the same
- * transport precision and group block are packed into shared launches, up to
- * 8 problems per launch, so that small problems fill the device together.
- * Each handle's results are bitwise those of ssw_step on it alone; a handle's
- * stream continues after the packed step. time_elapsed (n entries, may be
- * NULL) receives each handle's elapsed time. All handles on one device, none
- * twice, none used by another thread during the call. */
+ * the request loop does). The handles are cut into packs of about four with
+ * similar plan lengths, one pack per stream on up to four streams; within a
+ * pack the grouped launches of handles with the same transport precision and
+ * group block are packed into shared launches (up to 8 problems per launch),
+ * so that small problems fill the device together. Each handle's results are
+ * bitwise those of ssw_step on it alone; a handle's stream continues after
+ * the packed step. time_elapsed (n entries, may be NULL) receives each
+ * handle's elapsed time. All handles on one device, none twice, none used by
+ * another thread during the call. */
 int ssw_step_batch(ssw_sweep* const* handles, int32_t n, double* time_elapsed);
 /* Packed sweeps whose handles, sweep levels and replay plans repeat are
  * captured as one graph the second time they come and replayed from then on

@@ -3024,9 +3024,104 @@ hipGraphExec_t capture_packed(hipStream_t st, const std::function<void()>& enque
     return ge;
 }
 
+// One packed step of the prepared handles bs[0..n) on the stream of the
+// first (the others' streams wait for it before and it waits for theirs).
+int step_pack(BatchSweep* bs, int n, double* elapsed) {
+    const hipStream_t st = bs[0].s->stream;
+    for (int h = 1; h < n; h++) {
+        ssw_sweep* s = bs[h].s;
+        if (!s->sync_ev) HIP_TRY(hipEventCreateWithFlags(&s->sync_ev, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(s->sync_ev, s->stream));
+        HIP_TRY(hipStreamWaitEvent(st, s->sync_ev, 0));
+    }
+    for (int h = 0; h < n; h++) {
+        k_fill_u32<<<1, 2, 0, st>>>((uint32_t*)bs[h].s->d_active, 0u, 2);
+        k_fill_u32<<<1, 6, 0, st>>>((uint32_t*)bs[h].s->d_limit, 0u, 6);
+    }
+    size_t sweeps = 0;
+    for (int h = 0; h < n; h++) sweeps = std::max(sweeps, bs[h].order.size());
+    bool graphs = true;
+    for (int h = 0; h < n; h++) graphs = graphs && bs[h].s->graphs_on;
+    const int dev = bs[0].s->device;
+    for (size_t i = 0; i < sweeps; i++) {
+        // sweep i of every handle: the t-th plan items together, then each
+        // handle's periodic sources and chemistry
+        const std::function<void()> enqueue = [&]() {
+            Packs pk;
+            size_t items = 0;
+            for (int h = 0; h < n; h++)
+                if (i < bs[h].order.size()) items = std::max(items, bs[h].s->plan[bs[h].order[i]].size());
+            for (size_t t = 0; t < items; t++) {
+                for (int h = 0; h < n; h++) {
+                    BatchSweep& b = bs[h];
+                    if (i >= b.order.size()) continue;
+                    const int lvl = b.order[i];
+                    if (t >= b.s->plan[lvl].size()) continue;
+                    const ssw_sweep::ReplayItem& it = b.s->plan[lvl][t];
+                    const int k = Packs::slot(b.s->group_block);
+                    if (b.s->fp32) packed_item<float>(b.s, b.g, lvl, it, pk.f[k], st);
+                    else packed_item<double>(b.s, b.g, lvl, it, pk.d[k], st);
+                }
+                pk.flush(st);
+            }
+            for (int h = 0; h < n; h++) {
+                BatchSweep& b = bs[h];
+                if (i >= b.order.size()) continue;
+                if (b.s->fp32) batch_chemistry<float>(b.s, b.g, b.order[i], st);
+                else batch_chemistry<double>(b.s, b.g, b.order[i], st);
+            }
+        };
+        bool done = false;
+        if (graphs) {
+            std::vector<uint64_t> key;
+            for (int h = 0; h < n; h++)
+                if (i < bs[h].order.size())
+                    key.insert(key.end(), {bs[h].s->uid, bs[h].s->epoch, (uint64_t)bs[h].order[i]});
+            std::lock_guard<std::mutex> lk(g_pack_mu);
+            PackedGraph* e = find_packed(key, dev);
+            if (e && !e->ge) {  // the second time: capture
+                e->ge = capture_packed(st, enqueue);
+                if (e->ge) g_pack_captured++;
+            }
+            if (e && e->ge) {
+                HIP_TRY(hipGraphLaunch(e->ge, st));
+                g_pack_replayed++;
+                done = true;
+            } else if (!e) {
+                add_packed(key, dev, nullptr);
+            }
+        }
+        if (!done) enqueue();
+        for (int h = 0; h < n; h++)
+            if (i < bs[h].order.size()) bs[h].s->sweeps_last++;
+        HIP_TRY(hipGetLastError());
+    }
+    for (int h = 0; h < n; h++) {
+        ssw_sweep* s = bs[h].s;
+        const double el = s->ts.current_max_timestep();
+        s->ts.advance();
+        k_levels<<<grid_for(s->N), kBlock, 0, st>>>(bs[h].g, s->d_change, s->d_level, s->p.max_timestep,
+                                                   s->p.timestep_safety_factor, s->ts.lowest);
+        if (elapsed) elapsed[h] = el;
+    }
+    HIP_TRY(hipGetLastError());
+    // the handles' own streams continue after the packed step
+    ssw_sweep* s0 = bs[0].s;
+    if (!s0->sync_ev) HIP_TRY(hipEventCreateWithFlags(&s0->sync_ev, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(s0->sync_ev, st));
+    for (int h = 1; h < n; h++) HIP_TRY(hipStreamWaitEvent(bs[h].s->stream, s0->sync_ev, 0));
+    return SSW_OK;
+}
+
+// Packs of about kPackSize handles, on up to kPackStreams streams: the device
+// runs the launches of a few streams at once, and a packed launch lasts as
+// long as its slowest problem, so a few packs of handles with similar plan
+// lengths side by side beat one pack of all of them.
+constexpr int kPackSize = 4;
+constexpr int kPackStreams = 4;
+
 int step_batch(ssw_sweep* const* hs, int n, double* elapsed) {
     std::vector<BatchSweep> bs((size_t)n);
-    const hipStream_t st = hs[0]->stream;
     // per handle, on its own stream: the sweep levels whose active set
     // changed, and their active-set lists (the lists depend only on the
     // timestep levels, which change at the end of a step)
@@ -3053,84 +3148,25 @@ int step_batch(ssw_sweep* const* hs, int n, double* elapsed) {
                 TRY(make_list(s, lvl));
                 s->builds_last++;
             }
-        if (h > 0) {
-            if (!s->sync_ev) HIP_TRY(hipEventCreateWithFlags(&s->sync_ev, hipEventDisableTiming));
-            HIP_TRY(hipEventRecord(s->sync_ev, s->stream));
-            HIP_TRY(hipStreamWaitEvent(st, s->sync_ev, 0));
-        }
     }
-    for (BatchSweep& b : bs) {
-        k_fill_u32<<<1, 2, 0, st>>>((uint32_t*)b.s->d_active, 0u, 2);
-        k_fill_u32<<<1, 6, 0, st>>>((uint32_t*)b.s->d_limit, 0u, 6);
+    // handles in order of their full sweep's plan length (a proxy of their
+    // per-step chain), cut into consecutive packs
+    std::vector<int> idx((size_t)n);
+    for (int h = 0; h < n; h++) idx[h] = h;
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int x, int y) { return bs[x].s->plan[0].size() > bs[y].s->plan[0].size(); });
+    const int npack = std::min(kPackStreams, (n + kPackSize - 1) / kPackSize);
+    std::vector<BatchSweep> sorted((size_t)n);
+    std::vector<double> el((size_t)n, 0.0);
+    for (int h = 0; h < n; h++) sorted[h] = bs[idx[h]];
+    int start = 0;
+    for (int k = 0; k < npack; k++) {
+        const int cnt = n / npack + (k < n % npack ? 1 : 0);
+        TRY(step_pack(sorted.data() + start, cnt, el.data() + start));
+        start += cnt;
     }
-    size_t sweeps = 0;
-    for (const BatchSweep& b : bs) sweeps = std::max(sweeps, b.order.size());
-    bool graphs = true;
-    for (const BatchSweep& b : bs) graphs = graphs && b.s->graphs_on;
-    const int dev = hs[0]->device;
-    for (size_t i = 0; i < sweeps; i++) {
-        // sweep i of every handle: the t-th plan items together, then each
-        // handle's periodic sources and chemistry
-        const std::function<void()> enqueue = [&]() {
-            Packs pk;
-            size_t items = 0;
-            for (const BatchSweep& b : bs)
-                if (i < b.order.size()) items = std::max(items, b.s->plan[b.order[i]].size());
-            for (size_t t = 0; t < items; t++) {
-                for (BatchSweep& b : bs) {
-                    if (i >= b.order.size()) continue;
-                    const int lvl = b.order[i];
-                    if (t >= b.s->plan[lvl].size()) continue;
-                    const ssw_sweep::ReplayItem& it = b.s->plan[lvl][t];
-                    const int k = Packs::slot(b.s->group_block);
-                    if (b.s->fp32) packed_item<float>(b.s, b.g, lvl, it, pk.f[k], st);
-                    else packed_item<double>(b.s, b.g, lvl, it, pk.d[k], st);
-                }
-                pk.flush(st);
-            }
-            for (BatchSweep& b : bs) {
-                if (i >= b.order.size()) continue;
-                if (b.s->fp32) batch_chemistry<float>(b.s, b.g, b.order[i], st);
-                else batch_chemistry<double>(b.s, b.g, b.order[i], st);
-            }
-        };
-        bool done = false;
-        if (graphs) {
-            std::vector<uint64_t> key;
-            for (const BatchSweep& b : bs)
-                if (i < b.order.size()) key.insert(key.end(), {b.s->uid, b.s->epoch, (uint64_t)b.order[i]});
-            std::lock_guard<std::mutex> lk(g_pack_mu);
-            PackedGraph* e = find_packed(key, dev);
-            if (e && !e->ge) {  // the second time: capture
-                e->ge = capture_packed(st, enqueue);
-                if (e->ge) g_pack_captured++;
-            }
-            if (e && e->ge) {
-                HIP_TRY(hipGraphLaunch(e->ge, st));
-                g_pack_replayed++;
-                done = true;
-            } else if (!e) {
-                add_packed(key, dev, nullptr);
-            }
-        }
-        if (!done) enqueue();
-        for (BatchSweep& b : bs)
-            if (i < b.order.size()) b.s->sweeps_last++;
-        HIP_TRY(hipGetLastError());
-    }
-    for (int h = 0; h < n; h++) {
-        ssw_sweep* s = hs[h];
-        const double el = s->ts.current_max_timestep();
-        s->ts.advance();
-        k_levels<<<grid_for(s->N), kBlock, 0, st>>>(bs[h].g, s->d_change, s->d_level, s->p.max_timestep,
-                                                   s->p.timestep_safety_factor, s->ts.lowest);
-        if (elapsed) elapsed[h] = el;
-    }
-    HIP_TRY(hipGetLastError());
-    // the handles' own streams continue after the packed step
-    if (!hs[0]->sync_ev) HIP_TRY(hipEventCreateWithFlags(&hs[0]->sync_ev, hipEventDisableTiming));
-    HIP_TRY(hipEventRecord(hs[0]->sync_ev, st));
-    for (int h = 1; h < n; h++) HIP_TRY(hipStreamWaitEvent(hs[h]->stream, hs[0]->sync_ev, 0));
+    if (elapsed)
+        for (int h = 0; h < n; h++) elapsed[idx[h]] = el[h];
     return SSW_OK;
 }
 
