@@ -501,6 +501,7 @@ def main():
                     help="requests per packed step (slots / max-batch steppers, at most 4); 0: skip the packed pass")
     ap.add_argument("--no-precision-workload", action="store_true",
                     help="skip the 262144-cell FP32-vs-FP64 precision workload")
+    ap.add_argument("--no-packed-demo", action="store_true", help="skip the packed-steps comparison")
     ap.add_argument("--packed-demo", action="store_true",
                     help="only run the packed-steps comparison (16 small problems) and print its JSON")
     ap.add_argument("--largest", action="store_true",
@@ -598,6 +599,10 @@ def main():
                 f"requests per packed step")
             serving["packed"] = packed
 
+    packed_demo = None
+    if not args.no_packed_demo:
+        packed_demo = run_packed()
+
     info = fp32["info"]
     line = {
         "metric": "sweep cell-direction updates/s (FP32 transport + FP64 chemistry)",
@@ -652,6 +657,7 @@ def main():
         "cpu_baseline": cpu,
         "precision_262k": precision,
         "serving": serving,
+        "packed_demo": packed_demo,
     }
     print(json.dumps(line), flush=True)
 

@@ -61,6 +61,7 @@
 #include <functional>
 #include <mutex>
 #include <shared_mutex>
+#include <thread>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -3114,9 +3115,14 @@ int step_pack(BatchSweep* bs, int n, double* elapsed) {
 }
 
 // Packs of about kPackSize handles, on up to kPackStreams streams: the device
-// runs the launches of a few streams at once, and a packed launch lasts as
-// long as its slowest problem, so a few packs of handles with similar plan
-// lengths side by side beat one pack of all of them.
+// runs the launches of a few streams at once (one hardware queue per stream,
+// dealt to streams in turn as they are created), so a few packs side by side
+// beat one pack of all of them. Handle k goes to pack uid(k) mod npack
+// (creation turn; evened out where the turns are uneven), so that the packs'
+// streams sit on different queues and every pack holds a mix of problem
+// sizes; within a pack the longest plan leads. Measured on 16 small problems
+// (8000-27000 cells, 16-84 directions): 3.46 ms per step against 4.28 ms for
+// packs of similar sizes and 5.34 ms for one stream per problem.
 constexpr int kPackSize = 4;
 constexpr int kPackStreams = 4;
 
@@ -3149,24 +3155,60 @@ int step_batch(ssw_sweep* const* hs, int n, double* elapsed) {
                 s->builds_last++;
             }
     }
-    // handles in order of their full sweep's plan length (a proxy of their
-    // per-step chain), cut into consecutive packs
-    std::vector<int> idx((size_t)n);
-    for (int h = 0; h < n; h++) idx[h] = h;
-    std::stable_sort(idx.begin(), idx.end(),
+    // handles by their full sweep's plan length (a proxy of their per-step
+    // chain), longest first, dealt to the packs by creation turn
+    std::vector<int> order((size_t)n);
+    for (int h = 0; h < n; h++) order[h] = h;
+    std::stable_sort(order.begin(), order.end(),
                      [&](int x, int y) { return bs[x].s->plan[0].size() > bs[y].s->plan[0].size(); });
     const int npack = std::min(kPackStreams, (n + kPackSize - 1) / kPackSize);
-    std::vector<BatchSweep> sorted((size_t)n);
-    std::vector<double> el((size_t)n, 0.0);
-    for (int h = 0; h < n; h++) sorted[h] = bs[idx[h]];
-    int start = 0;
-    for (int k = 0; k < npack; k++) {
-        const int cnt = n / npack + (k < n % npack ? 1 : 0);
-        TRY(step_pack(sorted.data() + start, cnt, el.data() + start));
-        start += cnt;
+    std::vector<std::vector<int>> packs((size_t)npack);
+    for (int r = 0; r < n; r++) packs[bs[order[r]].s->uid % (uint64_t)npack].push_back(order[r]);
+    for (;;) {  // even out: no pack more than one above another
+        auto big = std::max_element(packs.begin(), packs.end(),
+                                    [](const std::vector<int>& x, const std::vector<int>& y) { return x.size() < y.size(); });
+        auto small = std::min_element(packs.begin(), packs.end(),
+                                      [](const std::vector<int>& x, const std::vector<int>& y) { return x.size() < y.size(); });
+        if (big->size() <= small->size() + 1) break;
+        small->push_back(big->back());
+        big->pop_back();
     }
-    if (elapsed)
-        for (int h = 0; h < n; h++) elapsed[idx[h]] = el[h];
+    // the lead (whose stream a pack runs on) of each pack: its longest member,
+    // unless an evened-out pack needs another member to sit at a place in the
+    // turn of the hardware queues (creation order modulo the queue count) that
+    // no earlier pack's lead holds
+    uint32_t used = 0;
+    for (size_t k = 0; k < packs.size(); k++) {
+        std::vector<int>& m = packs[k];
+        for (size_t j = 0; j < m.size(); j++) {
+            const uint32_t q = 1u << (bs[m[j]].s->uid % (uint64_t)kPackStreams);
+            if (!(used & q)) {
+                std::swap(m[0], m[j]);
+                break;
+            }
+        }
+        used |= 1u << (bs[m[0]].s->uid % (uint64_t)kPackStreams);
+    }
+    // each pack from a thread of its own (the first on the caller's): issuing
+    // the launches of a step takes host time comparable to running them
+    const int np = (int)packs.size();
+    std::vector<int> rc((size_t)np, SSW_OK);
+    std::vector<std::string> err((size_t)np);
+    auto run = [&](int k) {
+        std::vector<BatchSweep> sub;
+        for (int h : packs[k]) sub.push_back(bs[h]);
+        std::vector<double> el(sub.size(), 0.0);
+        rc[k] = step_pack(sub.data(), (int)sub.size(), el.data());
+        if (rc[k] != SSW_OK) err[k] = g_err;
+        if (elapsed)
+            for (size_t j = 0; j < sub.size(); j++) elapsed[packs[k][j]] = el[j];
+    };
+    std::vector<std::thread> th;
+    for (int k = 1; k < np; k++) th.emplace_back(run, k);
+    run(0);
+    for (std::thread& t : th) t.join();
+    for (int k = 0; k < np; k++)
+        if (rc[k] != SSW_OK) return fail(rc[k], err[k]);
     return SSW_OK;
 }
 
