@@ -58,6 +58,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <functional>
 #include <mutex>
 #include <shared_mutex>
@@ -3204,8 +3205,13 @@ int step_batch(ssw_sweep* const* hs, int n, double* elapsed) {
             for (size_t j = 0; j < sub.size(); j++) elapsed[packs[k][j]] = el[j];
     };
     std::vector<std::thread> th;
-    for (int k = 1; k < np; k++) th.emplace_back(run, k);
+    int spawned = 1;
+    try {
+        for (; spawned < np; spawned++) th.emplace_back(run, spawned);
+    } catch (const std::exception&) {  // no thread to spare: the rest on this one
+    }
     run(0);
+    for (int k = spawned; k < np; k++) run(k);
     for (std::thread& t : th) t.join();
     for (int k = 0; k < np; k++)
         if (rc[k] != SSW_OK) return fail(rc[k], err[k]);
