@@ -494,10 +494,10 @@ def main():
                     help="workload of a --precision profiling run")
     ap.add_argument("--serving-requests", type=int, default=256,
                     help="requests of the serving-loop measurement (0: skip it)")
-    ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--serving-slots", type=int, default=8)
     ap.add_argument("--serving-packed-slots", type=int, default=16,
                     help="slots of the packed serving pass (requests stepped together, up to --serving-max-batch)")
-    ap.add_argument("--serving-max-batch", type=int, default=4,
+    ap.add_argument("--serving-max-batch", type=int, default=16,
                     help="requests per packed step (slots / max-batch steppers, at most 4); 0: skip the packed pass")
     ap.add_argument("--no-precision-workload", action="store_true",
                     help="skip the 262144-cell FP32-vs-FP64 precision workload")
