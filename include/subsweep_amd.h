@@ -78,6 +78,17 @@ typedef struct ssw_sweep ssw_sweep;
  * directions. */
 int ssw_direction_table(int32_t num_directions, double* out);
 
+/* set_source_terms_system (src/source_systems.rs:48-80) on the device: each
+ * of the nsrc sources (positions src_pos, nsrc x 3, rates src_rate) goes to
+ * its nearest cell (squared Euclidean distance to the n cell positions pts,
+ * n x 3, non-periodic, ties to the lower cell index), and source_out (n)
+ * receives each cell's summed rate, added in source order; cell_out (nsrc,
+ * may be NULL) the chosen cells. Host arrays in and out; bitwise the result
+ * of the host placement (libssw_grid's ssw_assign_sources). The kiddo
+ * k-d tree's order among exactly equidistant cells is not reproduced. */
+int ssw_assign_sources_device(int64_t n, const double* pts, int64_t nsrc, const double* src_pos,
+                              const double* src_rate, double* source_out, int64_t* cell_out);
+
 /* Fills *p with the reference defaults (src/sweep/parameters.rs:64-78):
  * threshold 0, safety factors 0.1, prevent_cooling true, FP64 transport,
  * scale factor 1. num_directions/levels/max_timestep are left for the caller. */

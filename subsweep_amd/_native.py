@@ -152,6 +152,7 @@ SWEEP_SIGNATURES = {
     "ssw_upload_grid": (C.c_int, [P, C.c_int64, PI64, PI32, PU8, PD, PD, PD, PD]),
     "ssw_upload_state": (C.c_int, [P, PD, PD, PD, PD]),
     "ssw_step": (C.c_int, [P, PD]),
+    "ssw_assign_sources_device": (C.c_int, [C.c_int64, PD, C.c_int64, PD, PD, PD, PI64]),
     "ssw_step_batch": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, PD]),
     "ssw_packed_graph_stats": (C.c_int, [PI64, PI64]),
     "ssw_download": (C.c_int, [P, PD, PD, PD, PD, PU8, PD]),

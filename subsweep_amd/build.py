@@ -42,6 +42,8 @@ def _run(cmd: list[str]) -> None:
 
 def build_sweep() -> str:
     src = os.path.join(CSRC, "sweep.hip")
+    placement = os.path.join(CSRC, "placement.hip")
+    placement_obj = os.path.join(CSRC, "placement.o")
     server = os.path.join(CSRC, "server.cpp")
     sources = os.path.join(CSRC, "sources.cpp")
     sweep_obj = os.path.join(CSRC, "sweep.o")
@@ -50,13 +52,14 @@ def build_sweep() -> str:
     tmp = SWEEP_LIB + ".tmp"
     compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
     _run([HIPCC, *compile_flags, "-c", src, "-o", sweep_obj])
+    _run([HIPCC, *compile_flags, "-c", placement, "-o", placement_obj])
     # the request loop and the star-particle sources are host-only C++
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-pthread", "-I", os.path.join(ROOT, "include"), "-c", server,
           "-o", server_obj])
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"), "-c",
           sources, "-o", sources_obj])
-    _run([HIPCC, "-shared", "-fPIC", sweep_obj, server_obj, sources_obj, "-lpthread", "-o", tmp])
-    for o in (sweep_obj, server_obj, sources_obj):
+    _run([HIPCC, "-shared", "-fPIC", sweep_obj, placement_obj, server_obj, sources_obj, "-lpthread", "-o", tmp])
+    for o in (sweep_obj, placement_obj, server_obj, sources_obj):
         os.remove(o)
     os.replace(tmp, SWEEP_LIB)
     return SWEEP_LIB

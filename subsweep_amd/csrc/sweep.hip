@@ -1672,6 +1672,14 @@ unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kBlock
 
 }  // namespace
 
+// Shared with the library's other translation units (placement.hip): the
+// thread's error message and the lock that device allocations hold shared
+// while a stream capture holds it exclusively.
+namespace ssw_detail {
+void set_error(const std::string& msg) { g_err = msg; }
+std::shared_mutex& capture_mutex() { return g_capture_mu; }
+}  // namespace ssw_detail
+
 struct ssw_sweep {
     ssw_sweep_parameters p{};
     int D = 0;

@@ -179,15 +179,22 @@ def clustered_points(n: int, box: float, mesh: int = 128, sigma_ln: float = 2.0,
     return pts, delta[cell]
 
 
-def assign_sources(points: np.ndarray, positions: np.ndarray, rates: np.ndarray):
+def assign_sources(points: np.ndarray, positions: np.ndarray, rates: np.ndarray, device: bool = False):
     """Adds each source's rate to its nearest cell (Euclidean, non-periodic),
     as set_source_terms_system does (src/source_systems.rs:48-80). Returns
-    (per-cell source rate, cell index of each source)."""
+    (per-cell source rate, cell index of each source). `device`: the nearest
+    cells are found on the GPU (`ssw_assign_sources_device`), with bitwise the
+    same results as the host search."""
     pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
     pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
     rt = np.ascontiguousarray(rates, dtype=np.float64).reshape(-1)
     src = np.empty(pts.shape[0], np.float64)
     cells = np.empty(pos.shape[0], np.int64)
+    if device:
+        N.check(N.sweep_lib().ssw_assign_sources_device(pts.shape[0], N.ptr(pts, C.c_double), pos.shape[0],
+                                                        N.ptr(pos, C.c_double), N.ptr(rt, C.c_double),
+                                                        N.ptr(src, C.c_double), N.ptr(cells, C.c_int64)))
+        return src, cells
     rc = N.grid_lib().ssw_assign_sources(pts.shape[0], N.ptr(pts, C.c_double), pos.shape[0],
                                          N.ptr(pos, C.c_double), N.ptr(rt, C.c_double),
                                          N.ptr(src, C.c_double), N.ptr(cells, C.c_int64))

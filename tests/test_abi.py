@@ -78,3 +78,16 @@ def test_direction_tables_match_the_reference():
     assert np.array_equal(direction_table(1), [[1.0, 0.0, 0.0]])
     with pytest.raises(N.SweepError):
         direction_table(5)
+
+
+def test_device_source_placement_rejects_bad_arguments_before_the_device():
+    lib = N.sweep_lib()
+    one = (C.c_double * 3)(0.0, 0.0, 0.0)
+    out = (C.c_double * 1)()
+    assert lib.ssw_assign_sources_device(0, one, 1, one, one, out, None) == N.SSW_ERR_INVALID
+    assert lib.ssw_assign_sources_device(1, one, -1, one, one, out, None) == N.SSW_ERR_INVALID
+    assert lib.ssw_assign_sources_device(1, None, 1, one, one, out, None) == N.SSW_ERR_INVALID
+    # no sources: every cell's rate is zero, nothing touches the device
+    out[0] = 5.0
+    assert lib.ssw_assign_sources_device(1, one, 0, None, None, out, None) == N.SSW_OK
+    assert out[0] == 0.0
