@@ -1147,19 +1147,24 @@ __global__ void k_permute_desc(const uint4* in, const uint32_t* perm, uint4* out
 // unconditional (lanes without a task read the level set's first position)
 // so that the waits stay exact. Tasks beyond the first E of a level set are
 // solved without this overlap.
-template <typename R, int E>
+// PH > 0: the group has exactly PH level sets (known at compile time, so no
+// list entry is read and no schedule data is prefetched for a level set past
+// the group's last); PH = 0: phases level sets, at most kMaxGroup.
+template <typename R, int E, int PH>
 __device__ __forceinline__ void group_body(const Geo& g, const Flux<R>& fl, const uint4* __restrict__ hdr,
                                            const R* __restrict__ Av, const Entry<R>* __restrict__ planes,
                                            const Entry<R>* __restrict__ ovf, const uint32_t* __restrict__ fstart,
                                            int k0, int phases, const uint4* __restrict__ desc,
                                            const uint32_t* __restrict__ list, uint32_t list_len, const PassTab& pt,
                                            uint32_t blk) {
+    constexpr int MaxPh = PH > 0 ? PH : kMaxGroup;
+    if constexpr (PH > 0) phases = PH;
     const uint32_t w = pass_item(pt, blk);
     const uint4 d0 = desc[2 * w], d1 = desc[2 * w + 1];
     const uint32_t tid = threadIdx.x;
-    uint32_t n[kMaxGroup], off[kMaxGroup], bb[kMaxGroup], WW[kMaxGroup], P[kMaxGroup];
+    uint32_t n[MaxPh], off[MaxPh], bb[MaxPh], WW[MaxPh], P[MaxPh];
 #pragma unroll
-    for (int ph = 0; ph < kMaxGroup; ph++) {
+    for (int ph = 0; ph < MaxPh; ph++) {
         n[ph] = desc_count(d0, d1, ph);
         off[ph] = ph == 0 ? d0.x : off[ph - 1] + n[ph - 1];
         const int k = k0 + min(ph, phases - 1);
@@ -1171,31 +1176,32 @@ __device__ __forceinline__ void group_body(const Geo& g, const Flux<R>& fl, cons
     TaskS<R> cur;
     load_task<R>(cur, hdr + bb[0], Av + bb[0], planes + (size_t)kPlanes * bb[0], WW[0], P[0] - bb[0]);
 #pragma unroll
-    for (int ph = 0; ph < kMaxGroup; ph++) {
+    for (int ph = 0; ph < MaxPh; ph++) {
         if (ph < phases) {
             TaskG<R> G;
             issue_task<R>(G, g, fl, cur, ovf, P[ph]);
             TaskS<R> nxt;
-            const int q = ph + 1 < kMaxGroup ? ph + 1 : ph;
-            load_task<R>(nxt, hdr + bb[q], Av + bb[q], planes + (size_t)kPlanes * bb[q], WW[q], P[q] - bb[q]);
+            const int q = ph + 1 < MaxPh ? ph + 1 : ph;
+            if (PH == 0 || ph + 1 < MaxPh)
+                load_task<R>(nxt, hdr + bb[q], Av + bb[q], planes + (size_t)kPlanes * bb[q], WW[q], P[q] - bb[q]);
             if (tid < n[ph]) finish_task_c<R>(g, fl, cur, G, ovf, P[ph]);
             for (uint32_t i = tid + E; i < n[ph]; i += E)
                 comp_task<R>(g, fl, hdr + bb[ph], Av + bb[ph], planes + (size_t)kPlanes * bb[ph], ovf, bb[ph],
                              WW[ph], list[off[ph] + i] - bb[ph]);
             __syncthreads();
-            cur = nxt;
+            if (PH == 0 || ph + 1 < MaxPh) cur = nxt;
         }
     }
 }
 
-template <typename R, int E>
+template <typename R, int E, int PH>
 __global__ void __launch_bounds__(E) k_sweep_group(Geo g, Flux<R> fl, const uint4* __restrict__ hdr,
                                                    const R* __restrict__ Av, const Entry<R>* __restrict__ planes,
                                                    const Entry<R>* __restrict__ ovf,
                                                    const uint32_t* __restrict__ fstart, int k0, int phases,
                                                    const uint4* __restrict__ desc, const uint32_t* __restrict__ list,
                                                    uint32_t list_len, PassTab pt) {
-    group_body<R, E>(g, fl, hdr, Av, planes, ovf, fstart, k0, phases, desc, list, list_len, pt, blockIdx.x);
+    group_body<R, E, PH>(g, fl, hdr, Av, planes, ovf, fstart, k0, phases, desc, list, list_len, pt, blockIdx.x);
 }
 
 // One problem's grouped launch inside a packed launch: everything the grouped
@@ -1234,7 +1240,7 @@ struct PackArgs {
 
 // Grouped launches of up to kMaxPack problems in one launch: workgroup b runs
 // the job whose range holds it, as k_sweep_group would run it alone.
-template <typename R, int E>
+template <typename R, int E, int PH>
 __global__ void __launch_bounds__(E) k_sweep_group_pack(const PackArgs<R> a) {
     int j = 0;
     for (int k = 1; k < a.n; k++)
@@ -1252,8 +1258,8 @@ __global__ void __launch_bounds__(E) k_sweep_group_pack(const PackArgs<R> a) {
     fl.offI = J.offI;
     fl.sa = J.sa;
     fl.thr = J.thr;
-    group_body<R, E>(g, fl, J.hdr, J.A, J.planes, J.ovf, J.fstart, J.k0, J.phases, J.desc, J.list, J.list_len, J.pt,
-                     b);
+    group_body<R, E, PH>(g, fl, J.hdr, J.A, J.planes, J.ovf, J.fstart, J.k0, J.phases, J.desc, J.list, J.list_len,
+                         J.pt, b);
 }
 
 // The grouped replay of a restricted sweep (active set {level >= lvl}):
@@ -2649,15 +2655,25 @@ void launch_item_t(ssw_sweep* s, const Geo& g, const Flux<R>& fl, int lvl, const
         // full sweeps: die-aware order inside each pass; restricted sweeps
         // launch their filtered descriptors in order
         const PassTab pt = all ? gr.passes : s->gpass_l[lvl][it.tab];
-        if (s->group_block == 32)
-            k_sweep_group<R, 64><<<nblk, 64, 0, st>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf, s->d_fstart, gr.k0,
-                                                      gr.levels, dsc, gl, s->glist_len, pt);
-        else if (s->group_block == 64)
-            k_sweep_group<R, 128><<<nblk, 128, 0, st>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf, s->d_fstart,
-                                                        gr.k0, gr.levels, dsc, gl, s->glist_len, pt);
-        else
-            k_sweep_group<R, 256><<<nblk, 256, 0, st>>>(g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf, s->d_fstart,
-                                                        gr.k0, gr.levels, dsc, gl, s->glist_len, pt);
+        // groups of exactly 4 level sets (all but the narrow and last ones)
+        // run the kernel that knows its level-set count at compile time
+        using GroupKernel = void (*)(Geo, Flux<R>, const uint4*, const R*, const Entry<R>*, const Entry<R>*,
+                                     const uint32_t*, int, int, const uint4*, const uint32_t*, uint32_t, PassTab);
+        const bool four = gr.levels == 4;
+        GroupKernel kern;
+        int lanes;
+        if (s->group_block == 32) {
+            lanes = 64;
+            kern = four ? k_sweep_group<R, 64, 4> : k_sweep_group<R, 64, 0>;
+        } else if (s->group_block == 64) {
+            lanes = 128;
+            kern = four ? k_sweep_group<R, 128, 4> : k_sweep_group<R, 128, 0>;
+        } else {
+            lanes = 256;
+            kern = four ? k_sweep_group<R, 256, 4> : k_sweep_group<R, 256, 0>;
+        }
+        hipLaunchKernelGGL(kern, dim3(nblk), dim3(lanes), 0, st, g, fl, s->d_hdr, (const R*)s->d_A, pl, ovf,
+                           s->d_fstart, gr.k0, gr.levels, dsc, gl, s->glist_len, pt);
         return;
     }
     if (it.levels > 0) {  // a run of narrow level sets, one workgroup
@@ -2918,9 +2934,18 @@ struct Pack {
 template <typename R>
 void flush_pack(Pack<R>& p, int roots, hipStream_t st) {
     if (p.a.n <= 0) return;
-    if (roots == 32) k_sweep_group_pack<R, 64><<<p.grid, 64, 0, st>>>(p.a);
-    else if (roots == 64) k_sweep_group_pack<R, 128><<<p.grid, 128, 0, st>>>(p.a);
-    else k_sweep_group_pack<R, 256><<<p.grid, 256, 0, st>>>(p.a);
+    bool four = true;
+    for (int k = 0; k < p.a.n; k++) four = four && p.a.job[k].phases == 4;
+    if (roots == 32) {
+        if (four) k_sweep_group_pack<R, 64, 4><<<p.grid, 64, 0, st>>>(p.a);
+        else k_sweep_group_pack<R, 64, 0><<<p.grid, 64, 0, st>>>(p.a);
+    } else if (roots == 64) {
+        if (four) k_sweep_group_pack<R, 128, 4><<<p.grid, 128, 0, st>>>(p.a);
+        else k_sweep_group_pack<R, 128, 0><<<p.grid, 128, 0, st>>>(p.a);
+    } else {
+        if (four) k_sweep_group_pack<R, 256, 4><<<p.grid, 256, 0, st>>>(p.a);
+        else k_sweep_group_pack<R, 256, 0><<<p.grid, 256, 0, st>>>(p.a);
+    }
     p.a.n = 0;
     p.grid = 0;
 }
@@ -4316,9 +4341,9 @@ template <typename R>
 int group_blocks_per_cu(int roots) {
     int n = 0;
     hipError_t e;
-    if (roots == 32) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sweep_group<R, 64>, 64, 0);
-    else if (roots == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sweep_group<R, 128>, 128, 0);
-    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sweep_group<R, 256>, 256, 0);
+    if (roots == 32) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sweep_group<R, 64, 4>, 64, 0);
+    else if (roots == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sweep_group<R, 128, 4>, 128, 0);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sweep_group<R, 256, 4>, 256, 0);
     return e == hipSuccess ? n : 0;
 }
 }  // namespace
