@@ -161,17 +161,28 @@ struct __align__(16) Entry<double> {
     double w;
 };
 
-// Offset of plane entry q of the task at offset p of a level set of width W
-// from the level set's first entry. FP32 entries (8 bytes) are stored in
-// plane pairs, entries 2h and 2h+1 of a task side by side, so that a task
-// reads its kPlanes entries with kPlanes / 2 loads of 16 bytes (half the
-// load instructions, and half the cache-line lookups where a workgroup's
-// tasks sit at scattered positions); FP64 entries (16 bytes) plane by plane.
+// The plane entries of a level set of width W, from the level set's first
+// entry (kPlanes * W entries of storage; p is a task's offset in the level
+// set). FP32 entries (8 bytes) are stored in plane pairs, entries 2h and
+// 2h+1 of a task side by side, so that a task reads its kPlanes entries with
+// kPlanes / 2 loads of 16 bytes (half the load instructions, and half the
+// cache-line lookups where a workgroup's tasks sit at scattered positions).
+// FP64 entries are split: the upwind positions four to 16 bytes (entry q of
+// task p at uint32 index ((q / 4) * W + p) * 4 + q % 4), then the weights two
+// to 16 bytes (at double index 6 * W + ((q / 2) * W + p) * 2 + q % 2), so that
+// a task reads them with kPlanes / 4 + kPlanes / 2 loads of 16 bytes and no
+// padding; the last quarter of the level set's storage is unused.
 template <typename R>
 __host__ __device__ __forceinline__ size_t plane_off(uint32_t q, uint32_t W, uint32_t p) {
-    if constexpr (sizeof(R) == 4) return ((size_t)(q >> 1) * W + p) * 2 + (q & 1u);
-    else return (size_t)q * W + p;
+    return ((size_t)(q >> 1) * W + p) * 2 + (q & 1u);  // FP32
 }
+__host__ __device__ __forceinline__ size_t plane_nb_off(uint32_t q, uint32_t W, uint32_t p) {
+    return ((size_t)(q >> 2) * W + p) * 4 + (q & 3u);  // FP64 positions, uint32 index
+}
+__host__ __device__ __forceinline__ size_t plane_w_off(uint32_t q, uint32_t W, uint32_t p) {
+    return (size_t)6 * W + ((size_t)(q >> 1) * W + p) * 2 + (q & 1u);  // FP64 weights, double index
+}
+static_assert(kPlanes % 4 == 0, "FP64 upwind positions are stored four to 16 bytes");
 
 struct Geo {
     int64_t N;
@@ -640,8 +651,17 @@ __global__ void k_schedule(Geo g, double inv_aref, const uint32_t* sorted, const
             e.nb = xpos[(size_t)d * g.N + (nk & kNbrMask)];
             if constexpr (sizeof(R) == 8) e.pad = 0;
             e.w = (R)(fa.w * inv_aref * -dd);
-            if (q < (uint32_t)kPlanes) planes[(size_t)kPlanes * b + plane_off<R>(q, W, p - b)] = e;
-            else ovf[os + q - kPlanes] = e;
+            if (q < (uint32_t)kPlanes) {
+                Entry<R>* base = planes + (size_t)kPlanes * b;
+                if constexpr (sizeof(R) == 4) {
+                    base[plane_off<R>(q, W, p - b)] = e;
+                } else {
+                    reinterpret_cast<uint32_t*>(base)[plane_nb_off(q, W, p - b)] = e.nb;
+                    reinterpret_cast<double*>(base)[plane_w_off(q, W, p - b)] = e.w;
+                }
+            } else {
+                ovf[os + q - kPlanes] = e;
+            }
             q++;
         }
     }
@@ -782,9 +802,23 @@ __device__ __forceinline__ void load_task(TaskS<R>& t, const uint4* hdr, const R
             t.e[2 * h + 1].nb = v.z;
             t.e[2 * h + 1].w = __uint_as_float(v.w);
         }
-    } else {
+    } else {  // positions four to a 16-byte load, weights two
+        const uint4* pn = reinterpret_cast<const uint4*>(planes);
+        const double2* pw = reinterpret_cast<const double2*>(reinterpret_cast<const double*>(planes) + 6 * (size_t)W);
 #pragma unroll
-        for (int q = 0; q < kPlanes; q++) t.e[q] = planes[plane_off<R>((uint32_t)q, W, p)];
+        for (int h = 0; h < kPlanes / 4; h++) {
+            const uint4 v = pn[(size_t)h * W + p];
+            t.e[4 * h].nb = v.x;
+            t.e[4 * h + 1].nb = v.y;
+            t.e[4 * h + 2].nb = v.z;
+            t.e[4 * h + 3].nb = v.w;
+        }
+#pragma unroll
+        for (int h = 0; h < kPlanes / 2; h++) {
+            const double2 v = pw[(size_t)h * W + p];
+            t.e[2 * h].w = v.x;
+            t.e[2 * h + 1].w = v.y;
+        }
     }
     t.h = hdr[p];
     t.A = Av[p];
@@ -944,8 +978,11 @@ __device__ void push_upwind(const uint4* hdr, const Entry<R>* planes, const Entr
                             uint32_t W, uint32_t lo, uint32_t hi, uint32_t* cand, uint32_t* ncand, uint32_t* over) {
     const uint4 h = hdr[P];
     for (uint32_t q = 0; q < h.w; q++) {
-        const uint32_t v = q < (uint32_t)kPlanes ? planes[(size_t)kPlanes * b + plane_off<R>(q, W, P - b)].nb
-                                                 : ovf[h.z + q - kPlanes].nb;
+        const Entry<R>* base = planes + (size_t)kPlanes * b;
+        uint32_t v;
+        if (q >= (uint32_t)kPlanes) v = ovf[h.z + q - kPlanes].nb;
+        else if constexpr (sizeof(R) == 4) v = base[plane_off<R>(q, W, P - b)].nb;
+        else v = reinterpret_cast<const uint32_t*>(base)[plane_nb_off(q, W, P - b)];
         if (v < lo || v >= hi) continue;
         const uint32_t k = atomicAdd(ncand, 1u);
         if (k < (uint32_t)kConeCap) cand[k] = v;
