@@ -82,7 +82,7 @@ class InfoC(C.Structure):
         ("group_launches", C.c_int64),
         ("group_workgroups", C.c_int64),
         ("compute_units", C.c_int32),
-        ("reserved4", C.c_int32),
+        ("direction_sweeps", C.c_int32),
     ]
 
 
@@ -173,6 +173,7 @@ SWEEP_SIGNATURES = {
     "ssw_set_narrow_runs": (C.c_int, [P, C.c_int32]),
     "ssw_set_group_levels": (C.c_int, [P, C.c_int32]),
     "ssw_set_group_block": (C.c_int, [P, C.c_int32]),
+    "ssw_set_direction_sweeps": (C.c_int, [P, C.c_int32]),
     "ssw_direction_table": (C.c_int, [C.c_int32, C.POINTER(C.c_double)]),
     "ssw_checkpoint_bytes": (C.c_int64, [P]),
     "ssw_save_checkpoint": (C.c_int, [P, C.c_void_p, C.c_int64]),

@@ -942,6 +942,67 @@ __global__ void __launch_bounds__(run_block<R>()) k_sweep_comp_run(Geo g, Flux<R
     }
 }
 
+// Direction sweeps (full sweeps, compiled replay): the tasks of one
+// direction depend only on tasks of the same direction, so one workgroup per
+// direction solves the whole sweep of its direction, level set after level
+// set with a workgroup barrier in between, and no dependency crosses
+// workgroups or launches. The schedule is sorted by (depth, direction, cell),
+// so direction d's tasks of level set k are one range of it:
+// run[d * kmax + k] = {first position b of level set k, its width W, offset
+// of d's tasks in it, their count}; d has levels[d] non-empty level sets, the
+// first ones. Level sets of more tasks than lanes take further passes before
+// the barrier. Pipelined like the run kernel above: the next level set's
+// first-pass schedule data are loaded while the current gathers are in flight.
+template <typename R>
+__global__ void __launch_bounds__(run_block<R>()) k_sweep_dir(Geo g, Flux<R> fl, const uint4* __restrict__ hdr,
+                                                              const R* __restrict__ Av,
+                                                              const Entry<R>* __restrict__ planes,
+                                                              const Entry<R>* __restrict__ ovf,
+                                                              const uint4* __restrict__ runs, int kmax,
+                                                              const uint32_t* __restrict__ levels) {
+    const uint32_t i = threadIdx.x;
+    const uint4* run = runs + (size_t)blockIdx.x * kmax;
+    const int nl = (int)levels[blockIdx.x];
+    if (nl == 0) return;
+    const int last = nl - 1;
+    // position in its level set of the task of lane i in pass q0 (a valid position for every lane)
+    auto pos = [&](const uint4& r, uint32_t q0) -> uint32_t { return r.z + min(q0 + i, r.w - 1u); };
+    uint4 r = run[0];
+    TaskS<R> cur;
+    uint32_t p = pos(r, 0);
+    load_task<R>(cur, hdr + r.x, Av + r.x, planes + (size_t)kPlanes * r.x, r.y, p);
+    uint4 rn = run[min(1, last)];
+    for (int j = 0; j < nl; j++) {
+        TaskG<R> G;
+        issue_task<R>(G, g, fl, cur, ovf, r.x + p);
+        TaskS<R> nxt;
+        const uint32_t pn = pos(rn, 0);
+        load_task<R>(nxt, hdr + rn.x, Av + rn.x, planes + (size_t)kPlanes * rn.x, rn.y, pn);
+        const uint4 rnn = run[min(j + 2, last)];
+        if (i < r.w) finish_task_c<R>(g, fl, cur, G, ovf, r.x + p);
+        for (uint32_t q0 = blockDim.x; q0 < r.w; q0 += blockDim.x) {  // wide level sets: further passes
+            const uint32_t pq = pos(r, q0);
+            TaskS<R> t;
+            TaskG<R> Gq;
+            load_task<R>(t, hdr + r.x, Av + r.x, planes + (size_t)kPlanes * r.x, r.y, pq);
+            issue_task<R>(Gq, g, fl, t, ovf, r.x + pq);
+            if (q0 + i < r.w) finish_task_c<R>(g, fl, t, Gq, ovf, r.x + pq);
+        }
+        __syncthreads();
+        r = rn;
+        p = pn;
+        cur = nxt;
+        rn = rnn;
+    }
+}
+
+// Tasks per (level set, direction): cnt[depth[t] * D + t / N] for every task t = d*N + c.
+__global__ void k_dir_counts(const uint32_t* __restrict__ depth, uint32_t N, uint32_t D, uint32_t n,
+                             uint32_t* __restrict__ cnt) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) atomicAdd(&cnt[(size_t)depth[t] * D + t / N], 1u);
+}
+
 // ---------------------------------------------------------------------------
 // grouped replay (full sweeps): m consecutive level sets per launch. Each
 // workgroup owns a block of root tasks (kGroupT consecutive tasks of one level
@@ -1734,6 +1795,9 @@ struct ssw_sweep {
     int rec_block = 64;                             // compiled replay block size
     bool graphs_on = true;                          // capture replays as graphs
     bool narrow_runs = true;                        // runs of narrow level sets in one workgroup
+    bool dir_sweeps = false;                        // full sweeps: one workgroup per direction
+    uint4* d_dirrun = nullptr;                      // direction sweeps' run table (D x kmax)
+    uint32_t* d_dirlev = nullptr;                   // direction sweeps: level sets per direction
     int group_request = kGroupLevels;               // level sets per grouped launch (1: off)
     int group_block = kGroupT;                      // grouped replay: root tasks per workgroup
     // grouped replay of full sweeps (compiled): per group of level sets its
@@ -2159,6 +2223,11 @@ int plan_replay(ssw_sweep* s, int lvl) {
     std::vector<uint4>& tab = s->runtab[lvl];
     plan.clear();
     tab.clear();
+    if (s->d_dirrun && (lvl == 0 || s->list_full[lvl])) {  // direction sweeps: one launch
+        plan.push_back({0, s->kmax, 0u, 2});
+        dfree(s, s->d_runtab[lvl]);
+        return SSW_OK;
+    }
     if (!s->groups.empty()) {  // the grouped replay (restricted sweeps: groups with active tasks)
         for (size_t i = 0; i < s->groups.size(); i++)
             if (s->groups[i].ndesc && (lvl == 0 || s->gcount[lvl][i] > 0))
@@ -2438,6 +2507,48 @@ int build_groups(ssw_sweep* s) {
     return s->fp32 ? build_groups_t<float>(s) : build_groups_t<double>(s);
 }
 
+// The run table of the direction sweeps from every task's depth: per
+// direction d and level set k, {first position of k, width of k, offset of
+// d's tasks in k, their count}, and d's number of level sets (its tasks of
+// depth k exist for every k below its deepest, since a task of depth k has an
+// upwind task of depth k - 1 in its own direction).
+int build_dir_runs(ssw_sweep* s, const uint32_t* d_depth) {
+    const size_t DN = (size_t)s->D * s->N;
+    const size_t KD = (size_t)s->kmax * s->D;
+    uint32_t* d_cnt = nullptr;
+    Defer cleanup{[&] { dfree(s, d_cnt); }};
+    TRY(dalloc(s, &d_cnt, KD));
+    k_fill_u32<<<grid_for((int64_t)KD), kBlock, 0, s->stream>>>(d_cnt, 0u, (int64_t)KD);
+    k_dir_counts<<<grid_for((int64_t)DN), kBlock, 0, s->stream>>>(d_depth, (uint32_t)s->N, (uint32_t)s->D,
+                                                                  (uint32_t)DN, d_cnt);
+    HIP_TRY(hipGetLastError());
+    std::vector<uint32_t> cnt(KD);
+    HIP_TRY(hipMemcpyAsync(cnt.data(), d_cnt, KD * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const std::vector<uint32_t>& fs = s->full_start;
+    std::vector<uint4> run(KD, make_uint4(0, 0, 0, 0));
+    std::vector<uint32_t> lev(s->D, 0);
+    for (int k = 0; k < s->kmax; k++) {
+        uint32_t off = 0;
+        for (int d = 0; d < s->D; d++) {
+            const uint32_t c = cnt[(size_t)k * s->D + d];
+            run[(size_t)d * s->kmax + k] = make_uint4(fs[k], fs[k + 1] - fs[k], off, c);
+            if (c) {
+                if (lev[d] != (uint32_t)k) return fail(SSW_ERR_HIP, "direction sweeps: a direction skips a level set");
+                lev[d] = (uint32_t)k + 1;
+            }
+            off += c;
+        }
+        if (off != fs[k + 1] - fs[k]) return fail(SSW_ERR_HIP, "direction sweeps: level set counts differ");
+    }
+    TRY(dalloc(s, &s->d_dirrun, KD));
+    TRY(dalloc(s, &s->d_dirlev, (size_t)s->D));
+    HIP_TRY(hipMemcpyAsync(s->d_dirrun, run.data(), KD * sizeof(uint4), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_dirlev, lev.data(), (size_t)s->D * 4, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SSW_OK;
+}
+
 // The full schedule, once per grid: Kahn layering of the whole graph, each
 // task's depth, a stable sort by depth of the tasks in (direction, cell)
 // order, then the compiled or geometric per-task data in that order, and the
@@ -2465,6 +2576,7 @@ int build_schedule(ssw_sweep* s) {
                                                                                                   d_depth);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->compiled && s->dir_sweeps) TRY(build_dir_runs(s, d_depth));
     dfree(s, s->d_flat);
     TRY(dalloc(s, &d_keys, DN));
     TRY(dalloc(s, &d_vals, DN));
@@ -2683,6 +2795,12 @@ void launch_item_t(ssw_sweep* s, const Geo& g, const Flux<R>& fl, int lvl, const
     const bool all = lvl == 0 || s->list_full[lvl];
     const uint32_t* idx = all ? nullptr : s->d_idx[lvl];
     const Entry<R>* ovf = (const Entry<R>*)s->d_ovf;
+    if (it.group == 2) {  // direction sweeps: one workgroup per direction
+        k_sweep_dir<R><<<s->D, run_block<R>(), 0, st>>>(g, fl, s->d_hdr, (const R*)s->d_A,
+                                                         (const Entry<R>*)s->d_planes, ovf, s->d_dirrun, s->kmax,
+                                                         s->d_dirlev);
+        return;
+    }
     if (it.group) {
         const ssw_sweep::Group& gr = s->groups[it.tab];
         const uint4* dsc = (all ? s->d_gdesc : s->d_gdesc_l[lvl]) + 2 * (size_t)gr.desc0;
@@ -3012,7 +3130,7 @@ struct Packs {
 
 template <typename R>
 void packed_item(ssw_sweep* s, const Geo& g, int lvl, const ssw_sweep::ReplayItem& it, Pack<R>& pk, hipStream_t st) {
-    if (it.group) {
+    if (it.group == 1) {
         uint32_t nblk = 0;
         const GroupJob<R> J = group_job<R>(s, lvl, it, &nblk);
         if (nblk) add_job<R>(pk, J, s->group_block, st);
@@ -3340,6 +3458,8 @@ void free_grid(ssw_sweep* s) {
     dfree(s, s->d_cnt);
     dfree(s, s->d_level);
     dfree(s, s->d_fstart);
+    dfree(s, s->d_dirrun);
+    dfree(s, s->d_dirlev);
     dfree(s, s->d_recs);
     dfree(s, s->d_hdr);
     dfree(s, s->d_A);
@@ -3664,6 +3784,13 @@ int ssw_set_group_levels(ssw_sweep* s, int32_t levels) {
     if (levels < 1 || levels > kMaxGroup) return fail(SSW_ERR_INVALID, "group levels must be 1 to 8");
     if (s->have_grid) return fail(SSW_ERR_INVALID, "ssw_set_group_levels must be called before ssw_upload_grid");
     s->group_request = levels;
+    return SSW_OK;
+}
+
+int ssw_set_direction_sweeps(ssw_sweep* s, int32_t enabled) {
+    if (!s) return fail(SSW_ERR_INVALID, "null handle");
+    if (s->have_grid) return fail(SSW_ERR_INVALID, "ssw_set_direction_sweeps must be called before ssw_upload_grid");
+    s->dir_sweeps = enabled != 0;
     return SSW_OK;
 }
 
@@ -4407,6 +4534,7 @@ int ssw_get_info(ssw_sweep* s, ssw_info* info) {
     info->replay_block = s->rec_block;
     info->graph_replay = s->graphs_on ? 1 : 0;
     info->narrow_runs = (s->compiled && s->narrow_runs) ? 1 : 0;
+    info->direction_sweeps = s->d_dirrun ? 1 : 0;
     info->group_levels = s->groups.empty() ? 1 : s->group_request;
     info->group_fallback = s->group_fallback ? 1 : 0;
     info->group_tasks = s->group_tasks;

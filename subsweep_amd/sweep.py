@@ -115,7 +115,7 @@ class Sweep:
     def __init__(self, params: SweepParameters, grid: Grid, density, ionized_fraction, temperature, source,
                  transport_mode: int = N.TRANSPORT_MODE_AUTO, replay_block: int = 0, graph_replay: bool = True,
                  locality_order: bool = True, narrow_runs: bool = True, group_levels: int = 0,
-                 group_block: int = 0):
+                 group_block: int = 0, direction_sweeps: bool = False):
         lib = N.sweep_lib()
         self._lib = lib
         self.params = params
@@ -133,6 +133,7 @@ class Sweep:
             if group_levels:  # 0: the library's default
                 N.check(lib.ssw_set_group_levels(h, int(group_levels)))
             N.check(lib.ssw_set_group_block(h, int(group_block)))
+            N.check(lib.ssw_set_direction_sweeps(h, 1 if direction_sweeps else 0))
             if locality_order and grid.points is not None:
                 pts = np.ascontiguousarray(grid.points, dtype=np.float64).reshape(-1, 3)
                 if pts.shape[0] == grid.num_cells:

@@ -3,8 +3,8 @@ timestep levels), where the oracle would take minutes per step, checked
 through properties that do not depend on size:
 
   * launch-shape invariance: grouped replay (default, and with 32 roots per
-    workgroup), one launch per level set and the geometric replay give
-    bitwise the same fields;
+    workgroup), one launch per level set, the geometric replay and the
+    direction sweeps give bitwise the same fields;
   * the Kahn depths form a valid order: for every local face f of cell c
     whose normal points against direction d, depth(c, d) > depth(nb, d)
     (checked for a sample of directions), and the dependency counts equal
@@ -32,7 +32,8 @@ def clustered():
 def test_clustered_launch_shapes_bitwise(clustered, hip_available):
     pb = clustered
     out = []
-    for kw in ({}, {"group_block": 32}, {"group_levels": 1}, {"transport_mode": N.TRANSPORT_MODE_GEOMETRIC}):
+    for kw in ({}, {"group_block": 32}, {"group_levels": 1}, {"transport_mode": N.TRANSPORT_MODE_GEOMETRIC},
+               {"direction_sweeps": True}):
         s = Sweep(pb.params, pb.grid, *pb.state(), **kw)
         for _ in range(3):
             s.run_sweeps()

@@ -45,6 +45,7 @@ SHAPES = {
     "grouped": {},
     "per level set": {"group_levels": 1},
     "geometric": {"transport_mode": N.TRANSPORT_MODE_GEOMETRIC},
+    "direction sweeps": {"direction_sweeps": True},
 }
 FLUX_FLOOR = 1e-20
 ORACLE_THREADS = 8
@@ -108,6 +109,7 @@ def test_thin_jittered_fp64_every_shape(thin16, shape, hip_available):
         assert info.group_launches > 0 and info.group_workgroups >= info.group_launches
         assert info.group_blocks_per_cu >= 1, "the grouped kernel's occupancy limit should be known"
         assert info.compute_units >= 1
+    assert info.direction_sweeps == (1 if shape == "direction sweeps" else 0)
     for step in range(pb.steps):
         assert g.run_sweeps() == want[step][0]
         io = _compare_step(g, want[step], step, shape)
@@ -121,7 +123,7 @@ def test_thin_clustered_fp64_levels(thin_clustered, hip_available):
     pb = thin_clustered
     want = _oracle_run(pb)
     assert len(np.unique(want[-1][1].level)) == 4, "the levels should spread over all four"
-    for shape in ("grouped", "geometric"):
+    for shape in ("grouped", "geometric", "direction sweeps"):
         g = Sweep(pb.params, pb.grid, *pb.state(), **SHAPES[shape])
         for step in range(pb.steps):
             assert g.run_sweeps() == want[step][0]
