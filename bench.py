@@ -55,6 +55,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+DIR_KERNEL = ("k_sweep_dir (direction sweeps: one launch per sweep, one 512-lane workgroup per direction solving its "
+              "level sets in order with workgroup barriers only, recent upwind values from an LDS ring)")
+
+
+def kernel_name(info) -> str:
+    return DIR_KERNEL if info.direction_sweeps else KERNELS.get(info.transport_mode, "?")
+
+
 def bytes_per_update(info, precision: int) -> float:
     if precision == 64:
         base, bl, bp, bb = 8 + 8 + 16 + 16, 36.0, 28.0, 12.0
@@ -64,13 +72,21 @@ def bytes_per_update(info, precision: int) -> float:
 
 
 def occupancy(info) -> dict:
-    """Waves of the grouped replay kernel per CU: launched (mean workgroups
-    per launch of a full sweep x waves per workgroup / CUs) against what one
-    CU can hold of this kernel at once (its register and LDS limits)."""
+    """Waves of the dominant transport kernel per CU: launched (mean
+    workgroups per launch x waves per workgroup / CUs) against what one CU
+    can hold of this kernel at once (its register and LDS limits). The
+    direction sweeps launch one 8-wave workgroup per direction, and their
+    128 KiB LDS ring admits one workgroup per CU."""
+    cus = int(info.compute_units)
+    if info.direction_sweeps:
+        wg = int(info.num_directions)
+        launched = wg * 8 / cus if cus else None
+        return {"kernel": "k_sweep_dir", "compute_units": cus, "workgroups_per_launch": wg,
+                "waves_per_workgroup": 8, "launched_waves_per_cu": launched, "limit_waves_per_cu": 8,
+                "frac": launched / 8 if launched else None}
     if info.group_launches <= 0:
         return None
     waves_per_wg = 2 * int(info.group_block) // 64
-    cus = int(info.compute_units)
     wg = info.group_workgroups / info.group_launches
     limit = int(info.group_blocks_per_cu) * waves_per_wg
     launched = wg * waves_per_wg / cus if cus else None
@@ -131,7 +147,8 @@ def run_gpu(pb, precision: int, steps: int, warmup: int, profile_steps: int = 3,
         "kernel_ms_per_step": kernel_ms / max(1, profile_steps),
         "bytes_per_update": bpu,
         "achieved_gbs": achieved,
-        "kernel": KERNELS.get(info.transport_mode, "?"),
+        "kernel": kernel_name(info),
+        "direction_sweeps": bool(info.direction_sweeps),
         "replay_block": int(info.replay_block),
         "graph_replay": bool(info.graph_replay),
         "narrow_runs": bool(info.narrow_runs),
@@ -301,9 +318,9 @@ def tuned(pb, precision: int, warmup: int, enabled: bool, mode: int, block: int)
     r = autotune(pb.with_params(transport_precision=precision), warmup=warmup)
     log(f"autotune fp{precision}: mode {r.transport_mode}, block {r.replay_block}, graphs {r.graph_replay}, "
         f"narrow runs {r.narrow_runs}, group levels {r.group_levels}, group block {r.group_block}, "
-        f"{r.ms_per_step:.2f} ms/step ({time.perf_counter() - t:.1f} s; " +
-        ", ".join(f"m{m}/b{b}/g{int(gr)}/n{int(nr)}/l{gl}x{gb} {ms:.2f}" for m, b, gr, nr, gl, gb, ms in r.table) +
-        ")")
+        f"direction sweeps {r.direction_sweeps}, {r.ms_per_step:.2f} ms/step ({time.perf_counter() - t:.1f} s; " +
+        ", ".join(f"m{m}/b{b}/g{int(gr)}/n{int(nr)}/l{gl}x{gb}/d{int(ds)} {ms:.2f}"
+                  for m, b, gr, nr, gl, gb, ds, ms in r.table) + ")")
     return r.kwargs(), [list(x) for x in r.table]
 
 
@@ -348,11 +365,16 @@ def run_precision(warmup: int, steps: int = 20, autotune: bool = True):
     }
 
 
+def info_is_dir(r) -> bool:
+    return bool(r["info"].direction_sweeps)
+
+
 def shape_of(r, table):
     return {"transport_mode": r["transport_mode"], "replay_block": r["replay_block"],
             "graph_replay": r["graph_replay"], "narrow_runs": r["narrow_runs"], "group_levels": r["group_levels"],
             "group_block": r["group_block"], "group_tasks_per_full_sweep": r["group_tasks_per_full_sweep"],
-            "group_launches_per_full_sweep": int(r["info"].group_launches), "autotune_ms_per_step": table}
+            "group_launches_per_full_sweep": int(r["info"].group_launches),
+            "direction_sweeps": r["direction_sweeps"], "autotune_ms_per_step": table}
 
 
 def cpu_model() -> str:
@@ -512,7 +534,9 @@ def main():
     ap.add_argument("--shard", type=int, default=0, metavar="G",
                     help="measure one GPU's share of a direction-sharded run over G GPUs and print its JSON only")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="use --mode/--block instead of the autotuner's choice")
+                    help="use --mode/--block (and --direction-sweeps) instead of the autotuner's choice")
+    ap.add_argument("--direction-sweeps", action="store_true",
+                    help="with --no-autotune: one launch per sweep, one workgroup per direction")
     ap.add_argument("--no-graphs", action="store_true",
                     help="profiling run: launch the replays one by one instead of as captured graphs")
     ap.add_argument("--progress", action="store_true", help="profiling run: log every step")
@@ -553,6 +577,8 @@ def main():
         sh, _ = tuned(pb, args.precision, args.warmup, not args.no_autotune, args.mode, args.block)
         if args.no_graphs:
             sh = dict(sh, graph_replay=False)
+        if args.no_autotune and args.direction_sweeps:
+            sh = dict(sh, direction_sweeps=True)
         r = run_gpu(pb, args.precision, args.steps, args.warmup, shape=sh, progress=args.progress)
         log(f"fp{args.precision} transport: {r['updates_per_s'] / 1e9:.3f} G updates/s, {r['ms_per_step']:.2f} "
             f"ms/step, kernel {r['avg_launch_ms']:.4f} ms/launch, {r['achieved_gbs']:.0f} GB/s")
@@ -578,7 +604,8 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "transport_traffic.json")
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get("clustered_2m_fp32_bytes_per_launch")
+            key = ("k_sweep_dir_" if info_is_dir(fp32) else "") + "clustered_2m_fp32_bytes_per_launch"
+            traffic = json.load(open(tpath)).get(key)
         except Exception:
             traffic = None
 

@@ -263,7 +263,7 @@ typedef struct ssw_info {
     int64_t group_launches;
     int64_t group_workgroups;
     int32_t compute_units;         /* CUs of the handle's device               */
-    int32_t direction_sweeps;      /* 1 if full sweeps run one workgroup per
+    int32_t direction_sweeps;      /* 1 if sweeps run one workgroup per
                                       direction (ssw_set_direction_sweeps)   */
 } ssw_info;
 
@@ -354,13 +354,15 @@ int ssw_set_group_levels(ssw_sweep* s, int32_t levels);
  * ssw_upload_grid. No counterpart in the reference. */
 int ssw_set_group_block(ssw_sweep* s, int32_t roots);
 
-/* Compiled replay: full sweeps (and restricted sweeps whose active set is
- * every task) as one launch of one workgroup per direction, each solving its
- * direction's level sets in order with a workgroup barrier in between
- * (enabled = 1; 0 = the grouped or per-level-set replay, default). A
- * direction's tasks depend only on tasks of the same direction
+/* Compiled replay: every sweep as one launch of one workgroup per direction,
+ * each solving its direction's level sets in order with a workgroup barrier
+ * in between (enabled = 1; 0 = the grouped or per-level-set replay,
+ * default). A direction's tasks depend only on tasks of the same direction
  * (src/sweep/mod.rs:453-503 hands corrections only to neighbours in the same
- * direction), so no dependency crosses workgroups; same arithmetic per task,
+ * direction), so no dependency crosses workgroups. X is kept in each
+ * direction's Kahn order, and each workgroup holds the values it solved last
+ * in LDS, from which most upwind values are read. A restricted sweep passes
+ * every task and solves those of active cells. Same arithmetic per task,
  * bitwise the same results. Set before ssw_upload_grid. No counterpart in
  * the reference. */
 int ssw_set_direction_sweeps(ssw_sweep* s, int32_t enabled);

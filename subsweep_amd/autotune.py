@@ -13,7 +13,9 @@ replayed:
   or one launch per level set (compiled replay);
 * group levels and group block - launches of 2-8 consecutive level sets
   (overlapping upwind cones of 32, 64 or 128 root tasks, one workgroup each), or
-  one launch per level set (compiled replay).
+  one launch per level set (compiled replay);
+* direction sweeps - every sweep as one launch of one workgroup per direction
+  (compiled replay).
 
 `autotune` times a few steps of each candidate on the problem itself and
 returns the fastest. Every candidate computes bitwise the same results, so
@@ -38,34 +40,40 @@ class TuneResult:
     narrow_runs: bool
     group_levels: int
     group_block: int
+    direction_sweeps: bool
     ms_per_step: float
-    table: list  # (mode, block, graphs, narrow runs, group levels, group block, ms per step) per candidate
+    # (mode, block, graphs, narrow runs, group levels, group block, direction sweeps, ms per step) per candidate
+    table: list
 
     def kwargs(self) -> dict:
         return {"transport_mode": self.transport_mode, "replay_block": self.replay_block,
                 "graph_replay": self.graph_replay, "narrow_runs": self.narrow_runs,
-                "group_levels": self.group_levels, "group_block": self.group_block}
+                "group_levels": self.group_levels, "group_block": self.group_block,
+                "direction_sweeps": self.direction_sweeps}
 
 
-def candidates(blocks=(64, 128), groups=(4,), group_blocks=(32, 64, 128)):
+def candidates(blocks=(64, 128), groups=(4,), group_blocks=(32, 64, 128), direction_sweeps=True):
     C = N.TRANSPORT_MODE_COMPILED
-    out = [(C, b, True, True, 1, 0) for b in blocks]
-    out.append((C, blocks[0], False, True, 1, 0))
-    out.append((C, blocks[0], True, False, 1, 0))
-    out += [(C, blocks[0], True, True, m, gb) for m in groups for gb in group_blocks]
-    out.append((N.TRANSPORT_MODE_GEOMETRIC, 0, True, False, 1, 0))  # one launch shape (16 lanes per task)
+    out = [(C, b, True, True, 1, 0, False) for b in blocks]
+    out.append((C, blocks[0], False, True, 1, 0, False))
+    out.append((C, blocks[0], True, False, 1, 0, False))
+    out += [(C, blocks[0], True, True, m, gb, False) for m in groups for gb in group_blocks]
+    if direction_sweeps:
+        out.append((C, blocks[0], True, True, 1, 0, True))
+    out.append((N.TRANSPORT_MODE_GEOMETRIC, 0, True, False, 1, 0, False))  # one launch shape (16 lanes per task)
     return out
 
 
 def autotune(problem: Problem, cands=None, steps: int = 2, warmup: int = 2) -> TuneResult:
     """Times `steps` steps after `warmup` steps of every (mode, block, graphs,
-    narrow runs, group levels, group block) candidate on a fresh handle each;
-    candidates that do not fit in device memory are skipped."""
+    narrow runs, group levels, group block, direction sweeps) candidate on a
+    fresh handle each; candidates that do not fit in device memory are
+    skipped."""
     table = []
-    for m, b, gr, nr, gl, gb in (cands or candidates()):
+    for m, b, gr, nr, gl, gb, ds in (cands or candidates()):
         try:
             s = Sweep(problem.params, problem.grid, *problem.state(), transport_mode=m, replay_block=b,
-                      graph_replay=gr, narrow_runs=nr, group_levels=gl, group_block=gb)
+                      graph_replay=gr, narrow_runs=nr, group_levels=gl, group_block=gb, direction_sweeps=ds)
         except N.SweepError as e:
             if e.code == N.SSW_ERR_OOM:
                 continue
@@ -78,11 +86,12 @@ def autotune(problem: Problem, cands=None, steps: int = 2, warmup: int = 2) -> T
             for _ in range(steps):
                 s.run_sweeps()
             s.synchronize()
-            gb = s.info().group_block
-            table.append((m, b, gr, nr, gl, gb, (time.perf_counter() - t0) / steps * 1e3))
+            info = s.info()
+            table.append((m, b, gr, nr, gl, int(info.group_block), bool(info.direction_sweeps),
+                          (time.perf_counter() - t0) / steps * 1e3))
         finally:
             s.close()
     if not table:
         raise RuntimeError("no autotuning candidate fits on the device")
-    m, b, gr, nr, gl, gb, ms = min(table, key=lambda r: r[6])
-    return TuneResult(m, b, gr, nr, gl, gb, ms, table)
+    m, b, gr, nr, gl, gb, ds, ms = min(table, key=lambda r: r[7])
+    return TuneResult(m, b, gr, nr, gl, gb, ds, ms, table)

@@ -330,10 +330,10 @@ __device__ __forceinline__ CellIn<R> cell_inputs(const Geo& g, const Flux<R>& fl
 
 // Outgoing rate of a solved task (Site::get_rate + HydrogenOnly::get_outgoing_rate,
 // src/sweep/site.rs:49-56, src/chemistry/hydrogen_only/mod.rs:72-88), stored as X = O / A.
-// X goes to ox, I to oi.
+// X goes to ox, I to oi; returns the stored X.
 template <typename R>
-__device__ __forceinline__ void finish_task(const Flux<R>& fl, size_t ox, size_t oi, R inc, const CellIn<R>& ci,
-                                            R A) {
+__device__ __forceinline__ R finish_task(const Flux<R>& fl, size_t ox, size_t oi, R inc, const CellIn<R>& ci,
+                                         R A) {
     if (fl.offI) {
         // after a rotation the incoming rate is the remapped rate plus the
         // corrections since, offset + upwind sum; its persistent clamp
@@ -352,8 +352,10 @@ __device__ __forceinline__ void finish_task(const Flux<R>& fl, size_t ox, size_t
     // the threshold in the storage scale is compared in double: in FP32 it is
     // threshold / lref (e.g. 1e-5 / 2^160), far below the float range
     R out = ((double)in < fl.thr) ? (R)0 : in * (R)ci.att;
-    fl.X[ox] = (A > (R)0) ? out / A : (R)0;
+    const R xv = (A > (R)0) ? out / A : (R)0;
+    fl.X[ox] = xv;
     fl.I[oi] = inc;
+    return xv;
 }
 
 // The face loop of one task, kLanes lanes per task (one face each), shared by
@@ -942,58 +944,227 @@ __global__ void __launch_bounds__(run_block<R>()) k_sweep_comp_run(Geo g, Flux<R
     }
 }
 
-// Direction sweeps (full sweeps, compiled replay): the tasks of one
-// direction depend only on tasks of the same direction, so one workgroup per
-// direction solves the whole sweep of its direction, level set after level
-// set with a workgroup barrier in between, and no dependency crosses
-// workgroups or launches. The schedule is sorted by (depth, direction, cell),
-// so direction d's tasks of level set k are one range of it:
-// run[d * kmax + k] = {first position b of level set k, its width W, offset
-// of d's tasks in it, their count}; d has levels[d] non-empty level sets, the
-// first ones. Level sets of more tasks than lanes take further passes before
-// the barrier. Pipelined like the run kernel above: the next level set's
-// first-pass schedule data are loaded while the current gathers are in flight.
+// Direction sweeps (compiled replay): the tasks of one direction depend only
+// on tasks of the same direction, so one workgroup per direction solves its
+// direction's whole sweep, level set after level set with a workgroup barrier
+// in between, and no dependency crosses workgroups or launches. The schedule
+// is sorted by (depth, direction, cell), so direction d's tasks of level set k
+// are one range of it: run[d * kmax + k] = {first position b of level set k,
+// its width W (bit 31: wait for all memory operations at the end of this
+// level set), offset of d's tasks in it, their count}; d has levels[d]
+// non-empty level sets, the first ones.
+//
+// In this mode X is stored in each direction's Kahn order: the task at offset
+// j of d's tasks in level set k has X index d*N + q, q = (d's tasks in level
+// sets before k) + j, and the upwind entries of the schedule name these
+// indices. The workgroup keeps the last kR values it solved in an LDS ring
+// (slot q mod kR), written for the last kR tasks of each level set, so that
+// an upwind value with q >= (end of the current level set) - kR is read from
+// LDS and the barrier between level sets waits for LDS operations only.
+// Older values are read from memory: the host flags a level set for a full
+// wait whenever the level sets since the last flagged one would hold more
+// than kR tasks, so every value older than the ring was stored before a
+// completed wait and barrier.
+//
+// In a restricted sweep (lvl > 0; the host passes 0 when every cell is
+// active) every task still passes through: a task whose cell is below level
+// lvl is not solved and puts its stored X into the ring. Software pipelined
+// two level sets deep: while level set j is solved, level set j+1's cell
+// inputs, first overflow entries (and level and stored X) and level set
+// j+2's schedule data are loaded. Same arithmetic per task as the other
+// replays.
+constexpr int kDirBlock = 512;  // lanes of a direction sweep's workgroup
+constexpr uint32_t kDirWait = 0x80000000u;      // run entry: a full wait at the end of this level set
+constexpr uint32_t kDirWidth = 0x7FFFFFFFu;     // run entry: the level set's width
+
+// overflow entries loaded ahead per task (FP32: tasks of up to 18 upwind
+// faces in LDS only; FP64 keeps two, within its registers)
 template <typename R>
-__global__ void __launch_bounds__(run_block<R>()) k_sweep_dir(Geo g, Flux<R> fl, const uint4* __restrict__ hdr,
-                                                              const R* __restrict__ Av,
-                                                              const Entry<R>* __restrict__ planes,
-                                                              const Entry<R>* __restrict__ ovf,
-                                                              const uint4* __restrict__ runs, int kmax,
-                                                              const uint32_t* __restrict__ levels) {
+constexpr int dir_ovf() {
+    return sizeof(R) == 4 ? 6 : 2;
+}
+
+template <typename R>
+constexpr uint32_t dir_ring() {
+    return sizeof(R) == 4 ? 32768u : 16384u;  // 128 KiB of LDS
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kDirBlock) k_sweep_dir(Geo g, Flux<R> fl, const uint4* __restrict__ hdr,
+                                                         const R* __restrict__ Av, const Entry<R>* __restrict__ planes,
+                                                         const Entry<R>* __restrict__ ovf,
+                                                         const uint4* __restrict__ runs, int kmax,
+                                                         const uint32_t* __restrict__ levels, int lvl) {
+    constexpr uint32_t kR = dir_ring<R>();
+    constexpr int kDirOvf = dir_ovf<R>();
+    __shared__ R ring[kR];
     const uint32_t i = threadIdx.x;
-    const uint4* run = runs + (size_t)blockIdx.x * kmax;
-    const int nl = (int)levels[blockIdx.x];
+    const uint32_t d = blockIdx.x;
+    const uint4* run = runs + (size_t)d * kmax;
+    const int nl = (int)levels[d];
     if (nl == 0) return;
     const int last = nl - 1;
-    // position in its level set of the task of lane i in pass q0 (a valid position for every lane)
+    const uint32_t dN = d * (uint32_t)g.N;
+    // position in its level set of lane i's task in pass q0 (a valid one for every lane)
     auto pos = [&](const uint4& r, uint32_t q0) -> uint32_t { return r.z + min(q0 + i, r.w - 1u); };
-    uint4 r = run[0];
-    TaskS<R> cur;
-    uint32_t p = pos(r, 0);
-    load_task<R>(cur, hdr + r.x, Av + r.x, planes + (size_t)kPlanes * r.x, r.y, p);
-    uint4 rn = run[min(1, last)];
-    for (int j = 0; j < nl; j++) {
-        TaskG<R> G;
-        issue_task<R>(G, g, fl, cur, ovf, r.x + p);
-        TaskS<R> nxt;
-        const uint32_t pn = pos(rn, 0);
-        load_task<R>(nxt, hdr + rn.x, Av + rn.x, planes + (size_t)kPlanes * rn.x, rn.y, pn);
-        const uint4 rnn = run[min(j + 2, last)];
-        if (i < r.w) finish_task_c<R>(g, fl, cur, G, ovf, r.x + p);
-        for (uint32_t q0 = blockDim.x; q0 < r.w; q0 += blockDim.x) {  // wide level sets: further passes
-            const uint32_t pq = pos(r, q0);
-            TaskS<R> t;
-            TaskG<R> Gq;
-            load_task<R>(t, hdr + r.x, Av + r.x, planes + (size_t)kPlanes * r.x, r.y, pq);
-            issue_task<R>(Gq, g, fl, t, ovf, r.x + pq);
-            if (q0 + i < r.w) finish_task_c<R>(g, fl, t, Gq, ovf, r.x + pq);
+    auto ld = [&](TaskS<R>& t, const uint4& r, uint32_t p) {
+        load_task<R>(t, hdr + r.x, Av + r.x, planes + (size_t)kPlanes * r.x, r.y & kDirWidth, p);
+    };
+    // cell inputs, and in a restricted sweep the cell's level and the stored
+    // X (compared and used only when the task is solved, a level set later)
+    struct Pre {
+        CellIn<R> ci;
+        R xold;
+        uint32_t lev;
+        Entry<R> o[kDirOvf];  // the first overflow entries (index clamped to the task's own)
+    };
+    auto pre = [&](const TaskS<R>& t, uint32_t self) -> Pre {
+        Pre q;
+        const uint32_t c = t.h.x & kCellMask;
+        q.ci = cell_inputs<R>(g, fl, c, d, t.h.y);
+        if (t.h.w > (uint32_t)kPlanes) {
+            const uint32_t lastq = t.h.w - kPlanes - 1u;
+#pragma unroll
+            for (int k = 0; k < kDirOvf; k++) q.o[k] = ovf[t.h.z + min((uint32_t)k, lastq)];
+        } else {
+#pragma unroll
+            for (int k = 0; k < kDirOvf; k++) {
+                q.o[k].nb = self;
+                q.o[k].w = (R)0;
+            }
         }
-        __syncthreads();
-        r = rn;
-        p = pn;
-        cur = nxt;
-        rn = rnn;
+        if (lvl > 0) {
+            q.lev = g.level[c];
+            q.xold = fl.X[self];
+        } else {
+            q.lev = 0;
+            q.xold = (R)0;
+        }
+        return q;
+    };
+    // an upwind value: from the ring when it holds it, else from memory
+    auto up = [&](uint32_t nb, uint32_t hi) -> R {
+        const uint32_t qu = nb - dN;
+        return (qu + kR >= hi) ? ring[qu & (kR - 1u)] : fl.X[nb];
+    };
+    // the task's upwind sum in entry order, its stores, and its ring slot.
+    // Waves whose tasks find every upwind value in the ring (and have at most
+    // kDirOvf overflow entries, loaded a level set ahead) read LDS only, so
+    // that no wait on memory loads (which would also wait for the loads of
+    // the next level sets) enters the chain; other waves take the slow path.
+    auto solve = [&](const TaskS<R>& t, const Pre& pr, uint32_t self, uint32_t hi, bool lane) {
+        const uint32_t nup = t.h.w;
+        bool near = nup <= (uint32_t)(kPlanes + kDirOvf);
+#pragma unroll
+        for (int q = 0; q < kPlanes; q++)
+            near = near && ((uint32_t)q >= nup || (t.e[q].nb - dN) + kR >= hi);
+#pragma unroll
+        for (int k = 0; k < kDirOvf; k++) near = near && (pr.o[k].nb - dN) + kR >= hi;
+        R x[kPlanes];
+        R inc = 0;
+        if (__ballot(!near) == 0ull) {
+#pragma unroll
+            for (int q = 0; q < kPlanes; q++) x[q] = ring[(((uint32_t)q < nup ? t.e[q].nb : self) - dN) & (kR - 1u)];
+            R xo[kDirOvf];
+#pragma unroll
+            for (int k = 0; k < kDirOvf; k++) xo[k] = ring[(pr.o[k].nb - dN) & (kR - 1u)];
+#pragma unroll
+            for (int q = 0; q < kPlanes; q++) inc += ((uint32_t)q < nup) ? t.e[q].w * x[q] : (R)0;
+#pragma unroll
+            for (int k = 0; k < kDirOvf; k++)  // overflow entries, in entry order
+                if ((uint32_t)(kPlanes + k) < nup) inc += pr.o[k].w * xo[k];
+        } else {
+#pragma unroll
+            for (int q = 0; q < kPlanes; q++) x[q] = up((uint32_t)q < nup ? t.e[q].nb : self, hi);
+#pragma unroll
+            for (int q = 0; q < kPlanes; q++) inc += ((uint32_t)q < nup) ? t.e[q].w * x[q] : (R)0;
+            for (uint32_t q = kPlanes; q < nup; q++) {  // overflow entries, in entry order
+                const Entry<R> o = ovf[t.h.z + q - kPlanes];
+                inc += o.w * up(o.nb, hi);
+            }
+        }
+        if (lane) {
+            const bool act = pr.lev >= (uint32_t)lvl;
+            const R xv = act ? finish_task<R>(fl, self, (size_t)d * g.N + (t.h.x & kCellMask), inc, pr.ci, t.A)
+                             : pr.xold;
+            const uint32_t qs = self - dN;
+            if (qs + kR >= hi) ring[qs & (kR - 1u)] = xv;
+        }
+    };
+    // level sets j (r0), j+1 (r1), j+2 (r2) and their first q (past the last
+    // level set the entries repeat the last one)
+    uint4 r0 = run[0], r1 = run[min(1, last)], r2 = run[min(2, last)];
+    uint32_t qs0 = 0;
+    uint32_t qs1 = last >= 1 ? r0.w : 0u;
+    uint32_t qs2 = last >= 2 ? qs1 + r1.w : qs1;
+    uint32_t p0 = pos(r0, 0), p1 = pos(r1, 0);
+    TaskS<R> t0, t1;
+    ld(t0, r0, p0);
+    ld(t1, r1, p1);
+    Pre c0 = pre(t0, dN + qs0 + (p0 - r0.z));
+    for (int j = 0; j < nl; j++) {
+        const Pre c1 = pre(t1, dN + qs1 + (p1 - r1.z));
+        const uint32_t p2 = pos(r2, 0);
+        TaskS<R> t2;
+        ld(t2, r2, p2);
+        const uint4 r3 = run[min(j + 3, last)];
+        const uint32_t hi = qs0 + r0.w;
+        solve(t0, c0, dN + qs0 + (p0 - r0.z), hi, i < r0.w);
+        for (uint32_t q0 = kDirBlock; q0 < r0.w; q0 += kDirBlock) {  // wide level sets: further passes
+            const uint32_t pq = pos(r0, q0);
+            const uint32_t self = dN + qs0 + (pq - r0.z);
+            TaskS<R> t;
+            ld(t, r0, pq);
+            solve(t, pre(t, self), self, hi, q0 + i < r0.w);
+        }
+        if (r0.y & kDirWait) {  // every store so far complete before the barrier
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+        } else {  // LDS operations only
+            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        }
+        r0 = r1;
+        p0 = p1;
+        t0 = t1;
+        c0 = c1;
+        qs0 = qs1;
+        r1 = r2;
+        p1 = p2;
+        t1 = t2;
+        qs1 = qs2;
+        qs2 = (j + 3 <= last) ? qs2 + r2.w : qs2;
+        r2 = r3;
     }
+}
+
+// The lowest timestep level of the cells (direction sweeps: a restricted
+// sweep whose active set is every cell runs as a full one).
+__global__ void k_min_level(const uint8_t* __restrict__ level, int64_t n, uint32_t* out) {
+    uint32_t m = 255u;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x)
+        m = min(m, (uint32_t)level[c]);
+    for (int o = 32; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_down((int)m, o));
+    if (__lane_id() == 0) atomicMin(out, m);
+}
+
+// X index of every task in the direction sweeps' order: the task at position
+// p of level set k (sorted[p] = d*N + c) goes to d*N + qstart + (p - b_k - off)
+// with oq[k*D + d] = {off, qstart} of d's tasks in k.
+__global__ void k_invert_dir(const uint32_t* __restrict__ sorted, const uint32_t* __restrict__ fstart, int kmax,
+                             const uint2* __restrict__ oq, uint32_t N, uint32_t D, uint32_t* __restrict__ xpos,
+                             uint32_t n) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int a = 0, bb = kmax - 1;  // level set of position p
+    while (a < bb) {
+        const int m = (a + bb + 1) >> 1;
+        if (fstart[m] <= p) a = m;
+        else bb = m - 1;
+    }
+    const uint32_t t = sorted[p];
+    const uint32_t d = t / N;
+    const uint2 v = oq[(size_t)a * D + d];
+    xpos[t] = d * N + v.y + (p - fstart[a] - v.x);
 }
 
 // Tasks per (level set, direction): cnt[depth[t] * D + t / N] for every task t = d*N + c.
@@ -1798,6 +1969,7 @@ struct ssw_sweep {
     bool dir_sweeps = false;                        // full sweeps: one workgroup per direction
     uint4* d_dirrun = nullptr;                      // direction sweeps' run table (D x kmax)
     uint32_t* d_dirlev = nullptr;                   // direction sweeps: level sets per direction
+    std::vector<uint2> dir_oq;                      // direction sweeps: {offset, first q} per (level set, direction)
     int group_request = kGroupLevels;               // level sets per grouped launch (1: off)
     int group_block = kGroupT;                      // grouped replay: root tasks per workgroup
     // grouped replay of full sweeps (compiled): per group of level sets its
@@ -2223,7 +2395,7 @@ int plan_replay(ssw_sweep* s, int lvl) {
     std::vector<uint4>& tab = s->runtab[lvl];
     plan.clear();
     tab.clear();
-    if (s->d_dirrun && (lvl == 0 || s->list_full[lvl])) {  // direction sweeps: one launch
+    if (s->d_dirrun) {  // direction sweeps: one launch, restricted sweeps included
         plan.push_back({0, s->kmax, 0u, 2});
         dfree(s, s->d_runtab[lvl]);
         return SSW_OK;
@@ -2503,7 +2675,7 @@ int build_groups(ssw_sweep* s) {
     s->groups.clear();
     s->group_tasks = 0;
     s->group_fallback = false;
-    if (!s->compiled || s->group_request < 2) return SSW_OK;
+    if (!s->compiled || s->group_request < 2 || s->d_dirrun) return SSW_OK;  // direction sweeps need no cones
     return s->fp32 ? build_groups_t<float>(s) : build_groups_t<double>(s);
 }
 
@@ -2527,19 +2699,39 @@ int build_dir_runs(ssw_sweep* s, const uint32_t* d_depth) {
     HIP_TRY(hipStreamSynchronize(s->stream));
     const std::vector<uint32_t>& fs = s->full_start;
     std::vector<uint4> run(KD, make_uint4(0, 0, 0, 0));
-    std::vector<uint32_t> lev(s->D, 0);
+    std::vector<uint32_t> lev(s->D, 0), qs(s->D, 0);
+    s->dir_oq.assign(KD, make_uint2(0, 0));
     for (int k = 0; k < s->kmax; k++) {
         uint32_t off = 0;
         for (int d = 0; d < s->D; d++) {
             const uint32_t c = cnt[(size_t)k * s->D + d];
             run[(size_t)d * s->kmax + k] = make_uint4(fs[k], fs[k + 1] - fs[k], off, c);
+            s->dir_oq[(size_t)k * s->D + d] = make_uint2(off, qs[d]);
             if (c) {
                 if (lev[d] != (uint32_t)k) return fail(SSW_ERR_HIP, "direction sweeps: a direction skips a level set");
                 lev[d] = (uint32_t)k + 1;
             }
             off += c;
+            qs[d] += c;
         }
         if (off != fs[k + 1] - fs[k]) return fail(SSW_ERR_HIP, "direction sweeps: level set counts differ");
+        if (off > kDirWidth) return fail(SSW_ERR_INVALID, "direction sweeps: a level set is too wide");
+    }
+    // full waits (kDirWait): at the end of level set k - 1 whenever the
+    // level sets since the last wait, k included, would hold more tasks than
+    // the LDS ring, so that every upwind value older than the ring was stored
+    // before a completed wait and barrier
+    const uint64_t ring = s->fp32 ? dir_ring<float>() : dir_ring<double>();
+    for (int d = 0; d < s->D; d++) {
+        uint64_t acc = 0;
+        for (uint32_t k = 0; k < lev[d]; k++) {
+            const uint64_t c = run[(size_t)d * s->kmax + k].w;
+            if (k > 0 && acc + c > ring) {
+                run[(size_t)d * s->kmax + k - 1].y |= kDirWait;
+                acc = 0;
+            }
+            acc += c;
+        }
     }
     TRY(dalloc(s, &s->d_dirrun, KD));
     TRY(dalloc(s, &s->d_dirlev, (size_t)s->D));
@@ -2607,7 +2799,22 @@ int build_schedule(ssw_sweep* s) {
     const double inv_aref = 1.0 / s->aref;
     if (s->compiled) {
         TRY(dalloc(s, &s->d_xpos, DN));
-        k_invert<<<grid_for((int64_t)DN), kBlock, 0, s->stream>>>(d_sorted, s->d_xpos, (uint32_t)DN);
+        if (s->d_dirrun) {  // direction sweeps: X in each direction's Kahn order
+            uint2* d_oq = nullptr;
+            Defer free_oq{[&] { dfree(s, d_oq); }};
+            TRY(dalloc(s, &d_oq, s->dir_oq.size()));
+            HIP_TRY(hipMemcpyAsync(d_oq, s->dir_oq.data(), s->dir_oq.size() * sizeof(uint2), hipMemcpyHostToDevice,
+                                   s->stream));
+            k_invert_dir<<<grid_for((int64_t)DN), kBlock, 0, s->stream>>>(d_sorted, s->d_fstart, s->kmax, d_oq,
+                                                                         (uint32_t)s->N, (uint32_t)s->D, s->d_xpos,
+                                                                         (uint32_t)DN);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipStreamSynchronize(s->stream));
+            s->dir_oq.clear();
+            s->dir_oq.shrink_to_fit();
+        } else {
+            k_invert<<<grid_for((int64_t)DN), kBlock, 0, s->stream>>>(d_sorted, s->d_xpos, (uint32_t)DN);
+        }
         TRY(dalloc(s, &s->d_hdr, DN));
         TRY(dalloc(s, (uint8_t**)&s->d_A, DN * real_size(s)));
         TRY(dalloc(s, (uint8_t**)&s->d_planes, (size_t)kPlanes * DN * entry_size(s)));
@@ -2750,6 +2957,26 @@ int make_group_list(ssw_sweep* s, int lvl) {
 }
 
 int make_list(ssw_sweep* s, int lvl) {
+    if (s->d_dirrun) {  // direction sweeps pass every task and test its level: no list
+        uint32_t* d_min = nullptr;
+        Defer free_min{[&] { dfree(s, d_min); }};
+        TRY(dalloc(s, &d_min, 1));
+        k_fill_u32<<<1, 1, 0, s->stream>>>(d_min, 255u, 1);
+        k_min_level<<<std::min<unsigned>(grid_for(s->N), 1024u), kBlock, 0, s->stream>>>(s->d_level, s->N, d_min);
+        HIP_TRY(hipGetLastError());
+        uint32_t mn = 0;
+        HIP_TRY(hipMemcpyAsync(&mn, d_min, 4, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        s->list_full[lvl] = mn >= (uint32_t)lvl ? 1 : 0;  // every cell active: a full sweep
+        s->list_ok[lvl] = 1;
+        s->replays[lvl] = 0;
+        s->epoch++;
+        if (s->graphs[lvl]) {
+            (void)hipGraphExecDestroy(s->graphs[lvl]);
+            s->graphs[lvl] = nullptr;
+        }
+        return plan_replay(s, lvl);
+    }
     if (!s->groups.empty()) return make_group_list(s, lvl);
     const size_t DN = (size_t)s->D * s->N;
     if (!s->d_idx[lvl]) TRY(dalloc(s, &s->d_idx[lvl], DN));
@@ -2796,9 +3023,8 @@ void launch_item_t(ssw_sweep* s, const Geo& g, const Flux<R>& fl, int lvl, const
     const uint32_t* idx = all ? nullptr : s->d_idx[lvl];
     const Entry<R>* ovf = (const Entry<R>*)s->d_ovf;
     if (it.group == 2) {  // direction sweeps: one workgroup per direction
-        k_sweep_dir<R><<<s->D, run_block<R>(), 0, st>>>(g, fl, s->d_hdr, (const R*)s->d_A,
-                                                         (const Entry<R>*)s->d_planes, ovf, s->d_dirrun, s->kmax,
-                                                         s->d_dirlev);
+        k_sweep_dir<R><<<s->D, kDirBlock, 0, st>>>(g, fl, s->d_hdr, (const R*)s->d_A, (const Entry<R>*)s->d_planes,
+                                                    ovf, s->d_dirrun, s->kmax, s->d_dirlev, all ? 0 : lvl);
         return;
     }
     if (it.group) {

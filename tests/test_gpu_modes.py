@@ -119,10 +119,11 @@ def test_autotune_picks_a_measured_candidate(hip_available):
     from subsweep_amd.autotune import autotune, candidates
     pb = W.jittered_problem(8, 1.0, "4", 16, levels=1, periodic=True, steps=2, precision=32)
     r = autotune(pb, cands=candidates((64, 256), groups=(2,), group_blocks=(64,)), steps=1, warmup=2)
-    assert len(r.table) == 6
+    assert len(r.table) == 7
+    assert any(x[6] for x in r.table), "the direction sweeps are a candidate"
     assert (r.transport_mode, r.replay_block, r.graph_replay, r.narrow_runs, r.group_levels, r.group_block,
-            r.ms_per_step) in [tuple(x) for x in r.table]
-    assert r.ms_per_step == min(x[6] for x in r.table)
+            r.direction_sweeps, r.ms_per_step) in [tuple(x) for x in r.table]
+    assert r.ms_per_step == min(x[7] for x in r.table)
 
 
 @pytest.mark.parametrize("mode", [N.TRANSPORT_MODE_GEOMETRIC, N.TRANSPORT_MODE_COMPILED])
