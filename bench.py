@@ -416,7 +416,8 @@ def run_cpu_baseline(pb, directions: int = 16):
                       f"one thread pinned to core {core}"}
 
 
-def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, steps: int = 10, max_batch: int = 0):
+def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, steps: int = 10, max_batch: int = 0,
+                direction_sweeps: bool = False):
     """The serving loop on one server: first all `count` requests submitted at
     once (saturated throughput; grids become resident as they first arrive),
     then the same requests with Poisson arrivals at 0.8 x that completion rate
@@ -446,7 +447,7 @@ def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, step
 
     with Server(slots=slots, hbm_budget_bytes=budget, max_batch=max_batch) as sv:
         t0 = time.perf_counter()
-        tickets = [sv.submit(r, steps, grid_key=r.pool_index + 1) for r in reqs]
+        tickets = [sv.submit(r, steps, grid_key=r.pool_index + 1, direction_sweeps=direction_sweeps) for r in reqs]
         sat = [sv.wait(x) for x in tickets]
         span = time.perf_counter() - t0
         saturated = summary(sat, span)
@@ -465,7 +466,7 @@ def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, step
                 delay = arrivals[i] - (time.perf_counter() - start)
                 if delay > 0:
                     time.sleep(delay)
-                tickets[i] = sv.submit(r, steps, grid_key=r.pool_index + 1)
+                tickets[i] = sv.submit(r, steps, grid_key=r.pool_index + 1, direction_sweeps=direction_sweeps)
 
         th = threading.Thread(target=feeder)
         th.start()
@@ -496,10 +497,27 @@ def run_serving(count: int, slots: int, budget_gb: float, seed: int = 2024, step
         "reused_grid_fraction": poisson["reused_grid_fraction"],
         "saturated": saturated,
         "max_batch": max_batch,
+        "direction_sweeps": direction_sweeps,
         "packed_requests_per_step": stats["packed_requests_per_step"],
         "peak_running": stats["peak_running"],
         "peak_reserved_gb": stats["peak_reserved_bytes"] / (1 << 30),
     }
+
+
+def serving_passes(args):
+    """The unpacked and (with --serving-max-batch >= 2) packed serving passes."""
+    ds = args.serving_launch == "direction"
+    serving = run_serving(args.serving_requests, args.serving_slots, 200.0, direction_sweeps=ds)
+    log(f"serving: p50 {serving['p50_latency_ms']:.1f} ms, p99 {serving['p99_latency_ms']:.1f} ms, "
+        f"{serving['updates_per_s'] / 1e9:.3f} G updates/s, peak {serving['peak_running']} running")
+    if args.serving_max_batch >= 2:
+        packed = run_serving(args.serving_requests, args.serving_packed_slots, 200.0,
+                             max_batch=args.serving_max_batch, direction_sweeps=ds)
+        log(f"serving packed: p50 {packed['p50_latency_ms']:.1f} ms, p99 {packed['p99_latency_ms']:.1f} ms, "
+            f"{packed['updates_per_s'] / 1e9:.3f} G updates/s, {packed['packed_requests_per_step']:.2f} "
+            f"requests per packed step")
+        serving["packed"] = packed
+    return serving
 
 
 def main():
@@ -521,6 +539,10 @@ def main():
                     help="slots of the packed serving pass (requests stepped together, up to --serving-max-batch)")
     ap.add_argument("--serving-max-batch", type=int, default=16,
                     help="requests per packed step (slots / max-batch steppers, at most 4); 0: skip the packed pass")
+    ap.add_argument("--serving-launch", default="direction", choices=("direction", "grouped"),
+                    help="launch shape of the served requests: direction sweeps or the library default")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="only run the serving measurement and print its JSON")
     ap.add_argument("--no-precision-workload", action="store_true",
                     help="skip the 262144-cell FP32-vs-FP64 precision workload")
     ap.add_argument("--no-packed-demo", action="store_true", help="skip the packed-steps comparison")
@@ -558,6 +580,9 @@ def main():
 
     if args.packed_demo:
         print(json.dumps(run_packed()), flush=True)
+        return
+    if args.serving_only:
+        print(json.dumps(serving_passes(args)), flush=True)
         return
     if args.largest:
         print(json.dumps(run_largest()), flush=True)
@@ -615,16 +640,7 @@ def main():
 
     serving = None
     if args.serving_requests > 0:
-        serving = run_serving(args.serving_requests, args.serving_slots, 200.0)
-        log(f"serving: p50 {serving['p50_latency_ms']:.1f} ms, p99 {serving['p99_latency_ms']:.1f} ms, "
-            f"{serving['updates_per_s'] / 1e9:.3f} G updates/s, peak {serving['peak_running']} running")
-        if args.serving_max_batch >= 2:
-            packed = run_serving(args.serving_requests, args.serving_packed_slots, 200.0,
-                                 max_batch=args.serving_max_batch)
-            log(f"serving packed: p50 {packed['p50_latency_ms']:.1f} ms, p99 {packed['p99_latency_ms']:.1f} ms, "
-                f"{packed['updates_per_s'] / 1e9:.3f} G updates/s, {packed['packed_requests_per_step']:.2f} "
-                f"requests per packed step")
-            serving["packed"] = packed
+        serving = serving_passes(args)
 
     packed_demo = None
     if not args.no_packed_demo:

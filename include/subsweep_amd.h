@@ -456,7 +456,7 @@ typedef struct ssw_request {
                                       fits budget / slots, else geometric      */
     int32_t steps;                 /* ssw_step calls                          */
     int32_t replay_block;          /* as ssw_set_replay_block (0: default)     */
-    int32_t reserved0;
+    int32_t direction_sweeps;      /* 1: as ssw_set_direction_sweeps (0: off)  */
     int64_t num_cells;             /* grid, as for ssw_upload_grid            */
     const int64_t* face_offsets;
     const int32_t* face_neighbour;

@@ -98,7 +98,7 @@ class RequestC(C.Structure):
         ("transport_mode", C.c_int32),
         ("steps", C.c_int32),
         ("replay_block", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("direction_sweeps", C.c_int32),
         ("num_cells", C.c_int64),
         ("face_offsets", C.c_void_p),
         ("face_neighbour", C.c_void_p),

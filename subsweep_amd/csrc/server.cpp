@@ -159,6 +159,7 @@ ssw_sweep* setup_request(Req* q) {
         }
         rc = ssw_set_transport_mode(s, q->mode);
         if (rc == SSW_OK) rc = ssw_set_replay_block(s, r.replay_block);
+        if (rc == SSW_OK) rc = ssw_set_direction_sweeps(s, r.direction_sweeps ? 1 : 0);
         if (rc == SSW_OK && r.cell_position) rc = ssw_set_cell_positions(s, r.num_cells, r.cell_position);
         if (rc == SSW_OK)
             rc = ssw_upload_grid(s, r.num_cells, r.face_offsets, r.face_neighbour, r.face_kind, r.face_area,

@@ -76,7 +76,7 @@ class Server:
         self._pending = {}
 
     def submit(self, problem: Problem, steps: int, transport_mode: int = N.TRANSPORT_MODE_AUTO,
-               replay_block: int = 0, grid_key: int = 0) -> int:
+               replay_block: int = 0, grid_key: int = 0, direction_sweeps: bool = False) -> int:
         g = problem.grid
         n = g.num_cells
         arrays = list(g.arrays()) + [np.ascontiguousarray(a, dtype=np.float64) for a in problem.state()]
@@ -88,6 +88,7 @@ class Server:
         r.transport_mode = int(transport_mode)
         r.steps = int(steps)
         r.replay_block = int(replay_block)
+        r.direction_sweeps = 1 if direction_sweeps else 0
         r.num_cells = n
         r.grid_key = int(grid_key)
         names = ["face_offsets", "face_neighbour", "face_kind", "face_area", "face_normal", "cell_volume",

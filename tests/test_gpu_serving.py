@@ -95,6 +95,26 @@ def test_server_equals_direct_sweep(hip_available):
         assert np.array_equal(getattr(r, f), getattr(d, f)), f
 
 
+def test_served_direction_sweeps_equal_direct_sweep(hip_available):
+    # requests asking for direction sweeps (one of them twice, so that the
+    # second is served on the resident grid) give bitwise the grouped replay's
+    # results of a direct Sweep
+    probs = _problems()[:3]
+    with Server(slots=2, max_batch=0) as sv:
+        tickets = [sv.submit(pb, STEPS, grid_key=k + 1, direction_sweeps=True) for k, pb in enumerate(probs)]
+        tickets.append(sv.submit(probs[0], STEPS, grid_key=1, direction_sweeps=True))
+        rs = [sv.wait(t) for t in tickets]
+    for r, pb in zip(rs, probs + [probs[0]]):
+        assert r.status == 0, r.error
+        s = Sweep(pb.params, pb.grid, *pb.state())
+        for _ in range(STEPS):
+            s.run_sweeps()
+        d = s.results()
+        s.close()
+        for f in FIELDS:
+            assert np.array_equal(getattr(r, f), getattr(d, f)), f
+
+
 def test_budget_admission(hip_available):
     pb = _problems()[0]
     est = estimate_device_bytes(pb, N.TRANSPORT_MODE_GEOMETRIC)
