@@ -32,7 +32,7 @@ def test_checkpoint_continue_bitwise(precision, hip_available):
     ref = _state(a)
     a.close()
     for kw in ({}, {"locality_order": False}, {"transport_mode": N.TRANSPORT_MODE_GEOMETRIC},
-               {"group_levels": 1}):
+               {"group_levels": 1}, {"direction_sweeps": True}):
         b = Sweep(pb.params, pb.grid, *pb.state(), **kw)
         b.load_checkpoint(ckpt)
         for _ in range(3):

@@ -26,6 +26,7 @@ SHAPES = {
     "grouped": {},
     "per level set": {"group_levels": 1},
     "geometric": {"transport_mode": N.TRANSPORT_MODE_GEOMETRIC},
+    "direction sweeps": {"direction_sweeps": True},
 }
 
 
