@@ -180,7 +180,7 @@ def field_errors(a, b):
     return out
 
 
-def run_largest(steps: int = 10):
+def run_largest(steps: int = 10, direction_sweeps: bool = False):
     """16777216-cell clustered grid (sigma_ln 2.5, 256^3 field mesh), 8192
     sources, 84 directions, 4 timestep levels, periodic, threshold 1e-5 s^-1,
     FP32 transport with FP64 chemistry, `steps` steps, timed whole."""
@@ -190,7 +190,8 @@ def run_largest(steps: int = 10):
                              steps=steps, precision=32, name="clustered_16m")
     log(f"largest grid: {pb.grid.num_cells} cells, {pb.grid.num_faces} faces, built in "
         f"{time.perf_counter() - t:.1f} s")
-    r = run_gpu(pb, 32, steps, 0, profile_steps=1, progress=True)
+    r = run_gpu(pb, 32, steps, 0, profile_steps=1, progress=True,
+                shape={"direction_sweeps": True} if direction_sweeps else None)
     log(f"largest fp32: {r['updates_per_s'] / 1e9:.3f} G updates/s, {r['ms_per_step']:.1f} ms/step, "
         f"mode {r['transport_mode']}, {r['info'].device_bytes / 2**30:.1f} GiB on the device")
     return {
@@ -201,6 +202,7 @@ def run_largest(steps: int = 10):
         "ms_per_step": r["ms_per_step"],
         "updates_per_step": r["updates_per_step"],
         "transport_mode": r["transport_mode"],
+        "direction_sweeps": r["direction_sweeps"],
         "device_gib": r["info"].device_bytes / 2**30,
         "peak_device_gib": r["info"].peak_device_bytes / 2**30,
         "dag_depth": int(r["info"].max_depth),
@@ -558,7 +560,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="use --mode/--block (and --direction-sweeps) instead of the autotuner's choice")
     ap.add_argument("--direction-sweeps", action="store_true",
-                    help="with --no-autotune: one launch per sweep, one workgroup per direction")
+                    help="with --no-autotune or --largest: one launch per sweep, one workgroup per direction")
     ap.add_argument("--no-graphs", action="store_true",
                     help="profiling run: launch the replays one by one instead of as captured graphs")
     ap.add_argument("--progress", action="store_true", help="profiling run: log every step")
@@ -585,7 +587,7 @@ def main():
         print(json.dumps(serving_passes(args)), flush=True)
         return
     if args.largest:
-        print(json.dumps(run_largest()), flush=True)
+        print(json.dumps(run_largest(direction_sweeps=args.direction_sweeps)), flush=True)
         return
     if args.shard:
         if not 1 < args.shard <= 84:
